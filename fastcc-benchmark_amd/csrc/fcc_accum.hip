@@ -1,0 +1,662 @@
+// Planning, exact count pass and numeric accumulation.
+//
+// A unit of work is (left tile, column window). Its accumulator is one of
+//   DENSE  an f64 LDS tile TL x W: the reference's TileAccumulator
+//          (index_tensors.hpp:994-1065) with the bitmask replaced by the sign
+//          of zero (slots start at -0.0; a slot is touched iff its bits are no
+//          longer those of -0.0) or, when inputs hold zeros or tiny values, by
+//          an explicit LDS bitmap;
+//   LHASH  an open-addressing LDS hash (u32 local key, f64 value), the
+//          TileAccumulatorMap kind (index_tensors.hpp:843-883); when it fills,
+//          its slots are sealed and new keys go to the global hash;
+//   GHASH  the shared global-memory hash only.
+// The per-product loop restates the reference's join (contract.hpp:886-896):
+// for every key of the left tile, every (left, right) pair of the two fibers
+// adds lv*rv at (left row, right column).
+// Work distribution: a persistent grid pulls units from an atomic counter;
+// inside a unit, heavy groups (fiber >= HEAVY_MIN) are taken one per
+// wavefront (lanes across the fiber, the group's rows looped), light entries
+// one per lane.
+#include "fcc_accum.h"
+
+namespace fcc {
+
+constexpr uint32_t LEMPTY = 0xFFFFFFFFu, LSEALED = 0xFFFFFFFEu;
+constexpr unsigned long long GEMPTY = ~0ull;
+constexpr int NWAVES = ACC_THREADS / 64;
+constexpr uint64_t NEG_ZERO_BITS = 0x8000000000000000ull;
+
+__device__ __forceinline__ uint64_t mix64(uint64_t z) {
+    z += 0x9E3779B97F4A7C15ull;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+
+__device__ __forceinline__ uint32_t lds_load(const uint32_t *p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
+__device__ __forceinline__ void lds_add_f64(double *p, double v) {
+    (void)__hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
+__device__ __forceinline__ uint32_t uniform_u32(uint32_t x) { return __builtin_amdgcn_readfirstlane(x); }
+
+// First index in [lo, hi) with arr[i] >= x; all 64 lanes call it together.
+__device__ uint32_t wave_lower_bound(const uint32_t *arr, uint32_t lo, uint32_t hi, uint32_t x) {
+    const uint32_t lane = threadIdx.x & 63;
+    while (hi - lo > 64) {
+        uint32_t step = (hi - lo + 63) >> 6;
+        uint32_t p = lo + lane * step;
+        bool lt = p < hi && arr[p] < x;
+        uint32_t c = (uint32_t)__popcll(__ballot(lt));
+        if (c == 0) return lo;
+        uint32_t nlo = lo + (c - 1) * step + 1;
+        uint32_t nhi = lo + c * step;
+        lo = nlo;
+        hi = nhi < hi ? nhi : hi;
+    }
+    uint32_t p = lo + lane;
+    bool lt = p < hi && arr[p] < x;
+    return lo + (uint32_t)__popcll(__ballot(lt));
+}
+
+// ---------------------------------------------------------- global hash
+
+__device__ bool ghash_add(const AccArgs &A, uint64_t key, double v, uint32_t unit) {
+    uint64_t h = mix64(key) & A.gmask;
+    for (int probe = 0; probe < 8192; probe++) {
+        unsigned long long old = atomicCAS(&A.gkeys[h], GEMPTY, (unsigned long long)key);
+        if (old == GEMPTY) {
+            A.gunit[h] = unit;
+            atomicAdd(A.gcount, 1ull);
+            unsafeAtomicAdd(&A.gvals[h], v);
+            return true;
+        }
+        if (old == key) {
+            unsafeAtomicAdd(&A.gvals[h], v);
+            return true;
+        }
+        h = (h + 1) & A.gmask;
+    }
+    atomicOr(A.gflag, 1u);
+    return false;
+}
+
+// count pass: set semantics, a new key adds one to its unit's count
+__device__ void gset_insert(const AccArgs &A, uint64_t key, uint32_t unit) {
+    uint64_t h = mix64(key) & A.gmask;
+    for (int probe = 0; probe < 8192; probe++) {
+        unsigned long long old = atomicCAS(&A.gkeys[h], GEMPTY, (unsigned long long)key);
+        if (old == GEMPTY) {
+            atomicAdd(&A.unit_count[unit], 1ull);
+            atomicAdd(A.gcount, 1ull);
+            return;
+        }
+        if (old == key) return;
+        h = (h + 1) & A.gmask;
+    }
+    atomicOr(A.gflag, 1u);
+}
+
+// ------------------------------------------------------------ LDS hash
+// Linear probing with sealing: once `limit` keys are held, an empty slot met
+// by a new key is sealed instead of claimed, and any key whose probe meets a
+// sealed slot lives in the global hash. A key claimed earlier always sits
+// before any slot sealed on its probe path, so it is never duplicated.
+template <int BITS>
+__device__ __forceinline__ int lds_probe(uint32_t *keys, uint32_t *cnt, uint32_t limit, uint32_t key) {
+    // returns slot index, or -1 when the key must go to the global hash
+    uint32_t h = (key * 2654435761u) >> (32 - BITS);
+    for (uint32_t probe = 0; probe < (1u << BITS); probe++) {
+        uint32_t cur = lds_load(&keys[h]);
+        if (cur == key) return (int)h;
+        if (cur == LSEALED) return -1;
+        if (cur == LEMPTY) {
+            uint32_t want = lds_load(cnt) >= limit ? LSEALED : key;
+            uint32_t old = atomicCAS(&keys[h], LEMPTY, want);
+            if (old == LEMPTY) {
+                if (want == LSEALED) return -1;
+                atomicAdd(cnt, 1u);
+                return (int)h;
+            }
+            if (old == key) return (int)h;
+            if (old == LSEALED) return -1;
+        }
+        h = (h + 1) & ((1u << BITS) - 1);
+    }
+    return -1;
+}
+
+// ------------------------------------------------------------- per unit
+
+struct UnitCtx {
+    uint32_t u, a, tile, c0, W, kind, multi;
+    uint64_t row0;
+};
+
+template <bool NUM, bool BITMAP>
+struct Acc {
+    // NUM: DENSE f64 slots / LHASH vals+keys; SYM: DENSE bitmap / HASH key set
+    double *fv;       // numeric values
+    uint32_t *keys;   // LHASH keys (numeric) or key set (count)
+    uint32_t *bits;   // DENSE bitmap (count pass, or numeric BITMAP mode)
+    uint32_t *cnt;
+};
+
+template <bool NUM, bool BITMAP>
+__device__ __forceinline__ void accum(const AccArgs &A, const UnitCtx &U, const Acc<NUM, BITMAP> &S, uint32_t li,
+                                      uint32_t col, double v) {
+    if (U.kind == KIND_DENSE) {
+        uint32_t slot = li * U.W + col;
+        if (!NUM || BITMAP) atomicOr(&S.bits[slot >> 5], 1u << (slot & 31));
+        if (NUM) lds_add_f64(&S.fv[slot], v);
+    } else if (U.kind == KIND_LHASH) {
+        uint32_t key = li * U.W + col;
+        int slot = NUM ? lds_probe<12>(S.keys, S.cnt, LH_MAX, key) : lds_probe<14>(S.keys, S.cnt, SYM_MAX, key);
+        if (slot >= 0) {
+            if (NUM) lds_add_f64(&S.fv[slot], v);
+        } else {
+            uint64_t gk = (U.row0 + li) * A.spanB + (U.c0 + col);
+            if (NUM) ghash_add(A, gk, v, U.u);
+            else gset_insert(A, gk, U.u);
+        }
+    } else {
+        uint64_t gk = (U.row0 + li) * A.spanB + (U.c0 + col);
+        if (NUM) ghash_add(A, gk, v, U.u);
+        else gset_insert(A, gk, U.u);
+    }
+}
+
+template <bool NUM, bool BITMAP>
+__device__ void process_unit(const AccArgs &A, const UnitCtx &U, const Acc<NUM, BITMAP> &S, uint32_t *s_gnext) {
+    const uint32_t lane = threadIdx.x & 63;
+    const uint32_t g0 = A.atile_g0[U.a], gl = A.atile_lg[U.a], g1 = A.atile_g0[U.a + 1];
+    const uint32_t cend = U.c0 + U.W;
+    // heavy groups: one wavefront per group
+    for (;;) {
+        uint32_t g = 0;
+        if (lane == 0) g = atomicAdd(s_gnext, 1u);
+        g = uniform_u32(__shfl(g, 0, 64));
+        g += g0;
+        if (g >= gl) break;
+        uint32_t f = uniform_u32(A.group_fiber[g]);
+        uint32_t ep0 = uniform_u32(A.group_ptr[g]), ep1 = uniform_u32(A.group_ptr[g + 1]);
+        uint32_t s = uniform_u32(A.fiber_ptr[f]), e = uniform_u32(A.fiber_ptr[f + 1]);
+        if (U.multi) {
+            s = uniform_u32(wave_lower_bound(A.Bcol, s, e, U.c0));
+            e = uniform_u32(wave_lower_bound(A.Bcol, s, e, cend));
+        }
+        for (uint32_t cb = s; cb < e; cb += 64) {
+            uint32_t p = cb + lane;
+            bool ok = p < e;
+            uint32_t col = ok ? A.Bcol[p] - U.c0 : 0;
+            double b = ok ? A.Bval[p] : 0.0;
+            for (uint32_t k = ep0; k < ep1; k++) {
+                uint32_t li = A.Ali[k];
+                double av = A.Aval[k];
+                if (ok) accum<NUM, BITMAP>(A, U, S, li, col, av * b);
+            }
+        }
+    }
+    // light entries: one lane per entry, its fiber scanned serially
+    const uint32_t le0 = A.group_ptr[gl], le1 = A.group_ptr[g1];
+    for (uint32_t e = le0 + threadIdx.x; e < le1; e += ACC_THREADS) {
+        uint32_t li = A.Ali[e];
+        double av = A.Aval[e];
+        uint32_t f = A.Afib[e];
+        uint32_t p0 = A.fiber_ptr[f], p1 = A.fiber_ptr[f + 1];
+        for (uint32_t p = p0; p < p1; p++) {
+            uint32_t col = A.Bcol[p];
+            if (U.multi) {
+                if (col < U.c0) continue;
+                if (col >= cend) break;
+            }
+            accum<NUM, BITMAP>(A, U, S, li, col - U.c0, av * A.Bval[p]);
+        }
+    }
+}
+
+// Block-wide ordered compaction over slots [0, n): emit(slot, rank) for every
+// slot with pred(slot); returns the number emitted.
+template <class Pred, class Emit>
+__device__ uint32_t block_compact(uint32_t n, uint32_t *s_w, Pred pred, Emit emit) {
+    const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const uint64_t lt = lane ? (~0ull >> (64 - lane)) : 0ull;
+    uint32_t running = 0;
+    for (uint32_t base = 0; base < n; base += ACC_THREADS) {
+        uint32_t s = base + threadIdx.x;
+        bool t = s < n && pred(s);
+        uint64_t bal = __ballot(t);
+        if (lane == 0) s_w[wave] = (uint32_t)__popcll(bal);
+        __syncthreads();
+        uint32_t woff = 0, tot = 0;
+#pragma unroll
+        for (int w = 0; w < NWAVES; w++) {
+            uint32_t c = s_w[w];
+            woff += (w < (int)wave) ? c : 0;
+            tot += c;
+        }
+        if (t) emit(s, running + woff + (uint32_t)__popcll(bal & lt));
+        running += tot;
+        __syncthreads();
+    }
+    return running;
+}
+
+template <class Pred>
+__device__ uint32_t block_count(uint32_t n, uint32_t *s_w, Pred pred) {
+    const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    uint32_t c = 0;
+    for (uint32_t s = threadIdx.x; s < n; s += ACC_THREADS) c += pred(s) ? 1 : 0;
+    for (int d = 32; d > 0; d >>= 1) c += __shfl_xor(c, d, 64);
+    if (lane == 0) s_w[wave] = c;
+    __syncthreads();
+    uint32_t tot = 0;
+#pragma unroll
+    for (int w = 0; w < NWAVES; w++) tot += s_w[w];
+    __syncthreads();
+    return tot;
+}
+
+struct SinkAcc {
+    uint64_t count = 0, hash = 0;
+    double vsum = 0.0, wsum = 0.0;
+};
+
+__device__ __forceinline__ void emit_output(const AccArgs &A, uint64_t pos, uint64_t row, uint64_t col, double v,
+                                            bool sink, SinkAcc &sk) {
+    if (sink) {
+        uint64_t z = mix64(row * A.spanB + col);
+        sk.count++;
+        sk.hash += z;
+        sk.vsum += v;
+        sk.wsum += v * ((double)(z >> 11) * 0x1.0p-53);
+        return;
+    }
+    if (pos >= A.capacity) {  // a count/fill disagreement: flag it, never write past the output
+        atomicOr(A.fail_flag, 2u);
+        return;
+    }
+    uint32_t r = (uint32_t)row, c = (uint32_t)col;
+    for (int m = A.nxa - 1; m >= 0; m--) {
+        uint32_t E = A.xaE[m];
+        uint32_t q = r / E;
+        A.ocoords[(uint64_t)m * A.ostride + pos] = r - q * E;
+        r = q;
+    }
+    for (int m = A.nxb - 1; m >= 0; m--) {
+        uint32_t E = A.xbE[m];
+        uint32_t q = c / E;
+        A.ocoords[(uint64_t)(A.nxa + m) * A.ostride + pos] = c - q * E;
+        c = q;
+    }
+    A.ovals[pos] = v;
+}
+
+__device__ void sink_flush(const AccArgs &A, SinkAcc &sk, unsigned long long *s_u, double *s_d) {
+    const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    for (int d = 32; d > 0; d >>= 1) {
+        sk.count += __shfl_xor(sk.count, d, 64);
+        sk.hash += __shfl_xor(sk.hash, d, 64);
+        sk.vsum += __shfl_xor(sk.vsum, d, 64);
+        sk.wsum += __shfl_xor(sk.wsum, d, 64);
+    }
+    if (lane == 0) {
+        s_u[2 * wave] = sk.count;
+        s_u[2 * wave + 1] = sk.hash;
+        s_d[2 * wave] = sk.vsum;
+        s_d[2 * wave + 1] = sk.wsum;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned long long c = 0, h = 0;
+        double vs = 0, ws = 0;
+        for (int w = 0; w < NWAVES; w++) {
+            c += s_u[2 * w];
+            h += s_u[2 * w + 1];
+            vs += s_d[2 * w];
+            ws += s_d[2 * w + 1];
+        }
+        if (c) {
+            atomicAdd(&A.sink[0], c);
+            atomicAdd(&A.sink[1], h);
+            unsafeAtomicAdd(&A.sinkf[0], vs);
+            unsafeAtomicAdd(&A.sinkf[1], ws);
+        }
+    }
+}
+
+// MODE: 0 exact placement, 1 learned placement (cursor)
+template <bool BITMAP, bool SINK, int MODE>
+__global__ __launch_bounds__(ACC_THREADS) void k_numeric(AccArgs A) {
+    __shared__ __align__(16) double s_f64[DENSE_SLOTS];
+    __shared__ uint32_t s_bits[BITMAP ? DENSE_SLOTS / 32 : 1];
+    __shared__ uint32_t s_cnt, s_gnext, s_unit, s_w[NWAVES];
+    __shared__ unsigned long long s_base;
+    __shared__ unsigned long long s_su[2 * NWAVES];
+    __shared__ double s_sd[2 * NWAVES];
+    Acc<true, BITMAP> S;
+    S.fv = s_f64;
+    S.keys = reinterpret_cast<uint32_t *>(s_f64 + LH_SLOTS);
+    S.bits = s_bits;
+    S.cnt = &s_cnt;
+    SinkAcc sk;
+    for (;;) {
+        if (threadIdx.x == 0) s_unit = atomicAdd(A.work, 1u);
+        __syncthreads();
+        const uint32_t ui = s_unit;
+        if (ui >= A.nunits) break;
+        UnitCtx U;
+        U.u = A.unit_order ? A.unit_order[ui] : ui;
+        U.a = A.unit_tile[U.u];
+        U.tile = A.atile_id[U.a];
+        U.c0 = A.unit_c0[U.u];
+        U.W = A.unit_W[U.u];
+        U.kind = A.unit_kind[U.u];
+        U.multi = A.unit_multi[U.u];
+        U.row0 = (uint64_t)U.tile << A.logTL;
+        const uint32_t TL = 1u << A.logTL;
+        const uint32_t nslots = U.kind == KIND_DENSE ? TL * U.W : (U.kind == KIND_LHASH ? LH_SLOTS : 0);
+        if (U.kind == KIND_DENSE) {
+            const double init = BITMAP ? 0.0 : -0.0;
+            for (uint32_t s = threadIdx.x; s < nslots; s += ACC_THREADS) s_f64[s] = init;
+            if (BITMAP)
+                for (uint32_t s = threadIdx.x; s < (nslots + 31) / 32; s += ACC_THREADS) s_bits[s] = 0;
+        } else if (U.kind == KIND_LHASH) {
+            for (uint32_t s = threadIdx.x; s < LH_SLOTS; s += ACC_THREADS) {
+                s_f64[s] = 0.0;
+                S.keys[s] = LEMPTY;
+            }
+        }
+        if (threadIdx.x == 0) {
+            s_cnt = 0;
+            s_gnext = 0;
+        }
+        __syncthreads();
+        process_unit<true, BITMAP>(A, U, S, &s_gnext);
+        __syncthreads();
+        if (U.kind != KIND_GHASH) {
+            auto pred = [&](uint32_t s) -> bool {
+                if (U.kind == KIND_DENSE) {
+                    if (BITMAP) return (s_bits[s >> 5] >> (s & 31)) & 1u;
+                    return __double_as_longlong(s_f64[s]) != (long long)NEG_ZERO_BITS;
+                }
+                return S.keys[s] < LSEALED;
+            };
+            bool skip = false;
+            if (MODE == 1 && !SINK) {
+                uint32_t tot = block_count(nslots, s_w, pred);
+                if (threadIdx.x == 0) {
+                    unsigned long long b = atomicAdd(A.cursor, (unsigned long long)tot);
+                    if (b + tot > A.capacity) {
+                        atomicOr(A.fail_flag, 1u);
+                        b = ~0ull;
+                    }
+                    s_base = b;
+                }
+                __syncthreads();
+                skip = s_base == ~0ull;
+            } else if (MODE == 0 && !SINK) {
+                if (threadIdx.x == 0) s_base = A.unit_off[U.u];
+                __syncthreads();
+            }
+            if (!skip) {
+                const uint64_t base = s_base;
+                uint32_t emitted = block_compact(nslots, s_w, pred, [&](uint32_t s, uint32_t rank) {
+                    uint32_t key = U.kind == KIND_DENSE ? s : S.keys[s];
+                    uint32_t li = key / U.W, cl = key - li * U.W;
+                    emit_output(A, base + rank, U.row0 + li, (uint64_t)U.c0 + cl, s_f64[s], SINK, sk);
+                });
+                if (MODE == 0 && !SINK && threadIdx.x == 0) A.unit_cursor[U.u] = emitted;
+            }
+        } else if (MODE == 0 && !SINK && threadIdx.x == 0) {
+            A.unit_cursor[U.u] = 0;
+        }
+        __syncthreads();
+    }
+    if (SINK) sink_flush(A, sk, s_su, s_sd);
+}
+
+// Count pass: structural support only.
+__global__ __launch_bounds__(ACC_THREADS) void k_count(AccArgs A) {
+    __shared__ __align__(16) uint32_t s_set[SYM_SLOTS];
+    __shared__ uint32_t s_cnt, s_gnext, s_unit, s_w[NWAVES];
+    Acc<false, false> S;
+    S.fv = nullptr;
+    S.keys = s_set;
+    S.bits = s_set;
+    S.cnt = &s_cnt;
+    for (;;) {
+        if (threadIdx.x == 0) s_unit = atomicAdd(A.work, 1u);
+        __syncthreads();
+        const uint32_t ui = s_unit;
+        if (ui >= A.nunits) break;
+        UnitCtx U;
+        U.u = A.unit_order ? A.unit_order[ui] : ui;
+        U.a = A.unit_tile[U.u];
+        U.tile = A.atile_id[U.a];
+        U.c0 = A.unit_c0[U.u];
+        U.W = A.unit_W[U.u];
+        U.kind = A.unit_kind[U.u];
+        U.multi = A.unit_multi[U.u];
+        U.row0 = (uint64_t)U.tile << A.logTL;
+        const uint32_t TL = 1u << A.logTL;
+        const uint32_t nwords = U.kind == KIND_DENSE ? (TL * U.W + 31) / 32 : 0;
+        if (U.kind == KIND_DENSE) {
+            for (uint32_t s = threadIdx.x; s < nwords; s += ACC_THREADS) s_set[s] = 0;
+        } else if (U.kind == KIND_LHASH) {
+            for (uint32_t s = threadIdx.x; s < SYM_SLOTS; s += ACC_THREADS) s_set[s] = LEMPTY;
+        }
+        if (threadIdx.x == 0) {
+            s_cnt = 0;
+            s_gnext = 0;
+        }
+        __syncthreads();
+        process_unit<false, false>(A, U, S, &s_gnext);
+        __syncthreads();
+        uint32_t c;
+        if (U.kind == KIND_DENSE) {
+            uint32_t part = 0;
+            for (uint32_t s = threadIdx.x; s < nwords; s += ACC_THREADS) part += __popc(s_set[s]);
+            for (int d = 32; d > 0; d >>= 1) part += __shfl_xor(part, d, 64);
+            if ((threadIdx.x & 63) == 0) s_w[threadIdx.x >> 6] = part;
+            __syncthreads();
+            c = 0;
+            for (int w = 0; w < NWAVES; w++) c += s_w[w];
+        } else {
+            c = s_cnt;
+        }
+        if (threadIdx.x == 0 && c) atomicAdd(&A.unit_count[U.u], (unsigned long long)c);
+        __syncthreads();
+    }
+}
+
+// Drains the global hash: GHASH units and sealed-LHASH overflow.
+template <bool SINK, int MODE>
+__global__ __launch_bounds__(256) void k_gdrain(AccArgs A) {
+    __shared__ unsigned long long s_su[2 * 4];
+    __shared__ double s_sd[2 * 4];
+    __shared__ unsigned long long s_base;
+    __shared__ uint32_t s_w[4];
+    SinkAcc sk;
+    const uint64_t cap = A.gmask + 1;
+    const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const uint64_t lt = lane ? (~0ull >> (64 - lane)) : 0ull;
+    for (uint64_t base = (uint64_t)blockIdx.x * 256; base < cap; base += (uint64_t)gridDim.x * 256) {
+        uint64_t h = base + threadIdx.x;
+        unsigned long long key = h < cap ? A.gkeys[h] : GEMPTY;
+        bool t = key != GEMPTY;
+        uint64_t row = t ? key / A.spanB : 0, col = t ? key - row * A.spanB : 0;
+        double v = t ? A.gvals[h] : 0.0;
+        if (SINK) {
+            if (t) emit_output(A, 0, row, col, v, true, sk);
+            continue;
+        }
+        if (MODE == 0) {
+            if (t) {
+                uint32_t u = A.gunit[h];
+                unsigned long long r = atomicAdd(&A.unit_cursor[u], 1ull);
+                emit_output(A, A.unit_off[u] + r, row, col, v, false, sk);
+            }
+        } else {
+            uint64_t bal = __ballot(t);
+            if (lane == 0) s_w[wave] = (uint32_t)__popcll(bal);
+            __syncthreads();
+            if (threadIdx.x == 0) {
+                uint32_t tot = s_w[0] + s_w[1] + s_w[2] + s_w[3];
+                unsigned long long b = tot ? atomicAdd(A.cursor, (unsigned long long)tot) : 0;
+                if (tot && b + tot > A.capacity) {
+                    atomicOr(A.fail_flag, 1u);
+                    b = ~0ull;
+                }
+                s_base = b;
+            }
+            __syncthreads();
+            uint32_t woff = 0;
+            for (uint32_t w = 0; w < wave; w++) woff += s_w[w];
+            if (t && s_base != ~0ull) emit_output(A, s_base + woff + (uint32_t)__popcll(bal & lt), row, col, v, false, sk);
+            __syncthreads();
+        }
+    }
+    if (SINK) {
+        for (int d = 32; d > 0; d >>= 1) {
+            sk.count += __shfl_xor(sk.count, d, 64);
+            sk.hash += __shfl_xor(sk.hash, d, 64);
+            sk.vsum += __shfl_xor(sk.vsum, d, 64);
+            sk.wsum += __shfl_xor(sk.wsum, d, 64);
+        }
+        if (lane == 0 && sk.count) {
+            atomicAdd(&A.sink[0], (unsigned long long)sk.count);
+            atomicAdd(&A.sink[1], (unsigned long long)sk.hash);
+            unsafeAtomicAdd(&A.sinkf[0], sk.vsum);
+            unsafeAtomicAdd(&A.sinkf[1], sk.wsum);
+        }
+    }
+}
+
+// --------------------------------------------------------------- planner
+
+// Per active tile: accumulator kind and number of column windows.
+// est = exact planner: the products bound; learned planner: predicted outputs.
+__global__ void k_plan_tiles(const uint64_t *st_products, const float *pred, uint32_t natiles, PlanParams P,
+                             uint32_t *t_nwin, uint32_t *t_W, uint8_t *t_kind) {
+    uint32_t a = blockIdx.x * blockDim.x + threadIdx.x;
+    if (a >= natiles) return;
+    const double prod = (double)st_products[a];
+    const double S = (double)P.spanB;
+    const double area = (double)P.TL * S;
+    double est = pred ? (double)pred[a] * (1.0 + P.margin) + 8.0 : prod;
+    if (est > prod) est = prod;
+    if (est > area) est = area;
+    uint32_t kind, nwin = 1;
+    uint64_t W = P.spanB;
+    const int force = P.force;
+    bool dense;
+    if (force) dense = force == KIND_DENSE;
+    else if (est <= (double)HASH_TARGET) dense = false;
+    else dense = area <= (double)DENSE_SLOTS || est >= P.dense_ratio * area;
+    if (dense) {
+        kind = KIND_DENSE;
+        if (area > (double)DENSE_SLOTS) {
+            W = P.Wd;
+            nwin = (uint32_t)((P.spanB + W - 1) / W);
+        }
+    } else {
+        kind = force == KIND_GHASH ? KIND_GHASH : KIND_LHASH;
+        double nw = kind == KIND_GHASH ? 1.0 : ceil(est / (double)HASH_TARGET);
+        if (nw < 1) nw = 1;
+        if (nw > S) nw = S;
+        nwin = (uint32_t)nw;
+        W = (P.spanB + nwin - 1) / nwin;
+        // local keys li*W + col must stay below the LDS sentinels
+        while ((uint64_t)P.TL * W >= 0xFFFFFFF0ull) {
+            nwin *= 2;
+            W = (P.spanB + nwin - 1) / nwin;
+        }
+        nwin = (uint32_t)((P.spanB + W - 1) / W);
+    }
+    t_nwin[a] = nwin;
+    t_W[a] = (uint32_t)W;
+    t_kind[a] = (uint8_t)kind;
+}
+
+__global__ void k_emit_units(const uint32_t *t_nwin, const uint32_t *t_W, const uint8_t *t_kind,
+                             const uint32_t *t_ubase, uint32_t natiles, uint64_t spanB, const float *pred,
+                             uint32_t *u_tile, uint32_t *u_c0, uint32_t *u_W, uint8_t *u_kind, uint8_t *u_multi,
+                             float *u_pred) {
+    uint32_t a = blockIdx.x * blockDim.x + threadIdx.x;
+    if (a >= natiles) return;
+    uint32_t nw = t_nwin[a], W = t_W[a], base = t_ubase[a];
+    for (uint32_t w = 0; w < nw; w++) {
+        uint64_t c0 = (uint64_t)w * W;
+        uint64_t wid = spanB - c0 < W ? spanB - c0 : W;
+        u_tile[base + w] = a;
+        u_c0[base + w] = (uint32_t)c0;
+        u_W[base + w] = (uint32_t)wid;
+        u_kind[base + w] = t_kind[a];
+        u_multi[base + w] = nw > 1;
+        if (u_pred) u_pred[base + w] = pred ? pred[a] / (float)nw : 0.0f;
+    }
+}
+
+// Exact planner, after the count pass: hash units whose exact count exceeds
+// the LDS hash go to the global hash; returns their total through *gtotal.
+__global__ void k_refine_kinds(const unsigned long long *unit_count, uint32_t n, int force, uint8_t *u_kind,
+                               unsigned long long *gtotal) {
+    uint32_t u = blockIdx.x * blockDim.x + threadIdx.x;
+    if (u >= n) return;
+    if (u_kind[u] == KIND_LHASH && unit_count[u] > LH_MAX && force != KIND_LHASH) u_kind[u] = KIND_GHASH;
+    if (u_kind[u] == KIND_GHASH) atomicAdd(gtotal, unit_count[u]);
+}
+
+// ------------------------------------------------------------ launchers
+
+void launch_plan_tiles(const Index &ix, const float *pred, const PlanParams &P, uint32_t *t_nwin, uint32_t *t_W,
+                       uint8_t *t_kind, hipStream_t s) {
+    if (!ix.natiles) return;
+    k_plan_tiles<<<(unsigned)div_up(ix.natiles, 256), 256, 0, s>>>((const uint64_t *)ix.st_products, pred,
+                                                                     ix.natiles, P, t_nwin, t_W, t_kind);
+}
+
+void launch_emit_units(const uint32_t *t_nwin, const uint32_t *t_W, const uint8_t *t_kind, const uint32_t *t_ubase,
+                       uint32_t natiles, uint64_t spanB, const float *pred, Units &U, hipStream_t s) {
+    if (!natiles) return;
+    k_emit_units<<<(unsigned)div_up(natiles, 256), 256, 0, s>>>(t_nwin, t_W, t_kind, t_ubase, natiles, spanB, pred,
+                                                                  U.tile, U.c0, U.W, U.kind, U.multi, U.pred);
+}
+
+void launch_refine_kinds(const unsigned long long *unit_count, uint32_t n, int force, uint8_t *u_kind,
+                         unsigned long long *gtotal, hipStream_t s) {
+    if (!n) return;
+    k_refine_kinds<<<(unsigned)div_up(n, 256), 256, 0, s>>>(unit_count, n, force, u_kind, gtotal);
+}
+
+void launch_count(const AccArgs &A, unsigned grid, hipStream_t s) {
+    if (!A.nunits) return;
+    k_count<<<grid, ACC_THREADS, 0, s>>>(A);
+}
+
+void launch_numeric(const AccArgs &A, unsigned grid, bool bitmap, bool sink, int mode, hipStream_t s) {
+    if (!A.nunits) return;
+#define FCC_NUM(BM, SK, MD) k_numeric<BM, SK, MD><<<grid, ACC_THREADS, 0, s>>>(A)
+    if (bitmap) {
+        if (sink) { if (mode) FCC_NUM(true, true, 1); else FCC_NUM(true, true, 0); }
+        else { if (mode) FCC_NUM(true, false, 1); else FCC_NUM(true, false, 0); }
+    } else {
+        if (sink) { if (mode) FCC_NUM(false, true, 1); else FCC_NUM(false, true, 0); }
+        else { if (mode) FCC_NUM(false, false, 1); else FCC_NUM(false, false, 0); }
+    }
+#undef FCC_NUM
+}
+
+void launch_gdrain(const AccArgs &A, bool sink, int mode, hipStream_t s) {
+    uint64_t cap = A.gmask + 1;
+    unsigned grid = (unsigned)std::min<uint64_t>(div_up(cap, 256), 4096);
+    if (sink) k_gdrain<true, 0><<<grid, 256, 0, s>>>(A);
+    else if (mode) k_gdrain<false, 1><<<grid, 256, 0, s>>>(A);
+    else k_gdrain<false, 0><<<grid, 256, 0, s>>>(A);
+}
+
+}  // namespace fcc

@@ -1,0 +1,1029 @@
+// C ABI (include/fcc.h): context, orchestration of one contraction, results.
+//
+// fcc_contract stands in for Tensor<double>::fastcc_multiply
+// (contract.hpp:824-912). Phases:
+//   index   extents (contract.hpp:124-144), right CSR by contraction key and
+//           left join (replacing both TileIndexedTensor builds,
+//           contract.hpp:852-862), left tiles
+//   plan    per-tile kind/window choice from the exact bounds or the MLP
+//   count   exact planner only: structural count per unit, scan -> offsets
+//   numeric accumulate + drain straight into the SoA output (replacing the
+//           per-thread TileAccumulator + ListTensor + heaps,
+//           contract.hpp:864-911)
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <random>
+#include <string>
+#include <vector>
+
+#include "../../include/fcc.h"
+#include "fcc_accum.h"
+#include "fcc_mlp.h"
+
+using namespace fcc;
+
+struct fcc_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    DevPool pool;
+    std::string err;
+    int num_cus = 256;
+    // learned planner
+    std::vector<float> mlp_host;
+    float *mlp_dev = nullptr;
+    bool mlp_ready = false;
+    // autotuner
+    std::vector<fcc_tune_entry> tune;
+    // statistics of the last collect_stats contraction
+    std::vector<float> feat, labels;
+    hipEvent_t ev[12] = {};
+};
+
+namespace {
+
+struct ResultPriv {
+    void *coords = nullptr;
+    void *vals = nullptr;
+};
+
+int fail(fcc_ctx *ctx, int code, const std::string &msg) {
+    if (ctx) ctx->err = msg;
+    return code;
+}
+
+float ev_ms(hipEvent_t a, hipEvent_t b) {
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, a, b) != hipSuccess) {
+        (void)hipGetLastError();
+        return 0.f;
+    }
+    return ms;
+}
+
+Geom make_geom(const fcc_tensor *a, const int32_t *am, const fcc_tensor *b, const int32_t *bm, int r,
+               const uint32_t *EA, const uint32_t *EB) {
+    Geom g;
+    memset(&g, 0, sizeof(g));
+    g.dA = a->order;
+    g.dB = b->order;
+    g.r = r;
+    g.dC = a->order + b->order - 2 * r;
+    for (int i = 0; i < r; i++) {
+        g.am[i] = am[i];
+        g.bm[i] = bm[i];
+        g.R[i] = std::max<uint64_t>(EA[am[i]], EB[bm[i]]);
+    }
+    auto in = [](const int32_t *l, int n, int m) {
+        for (int i = 0; i < n; i++)
+            if (l[i] == m) return true;
+        return false;
+    };
+    g.spanA = 1;
+    g.spanB = 1;
+    for (int m = 0; m < a->order; m++)
+        if (!in(am, r, m)) {
+            g.xa[g.nxa] = m;
+            g.xaE[g.nxa] = EA[m];
+            g.spanA *= EA[m];
+            g.nxa++;
+        }
+    for (int m = 0; m < b->order; m++)
+        if (!in(bm, r, m)) {
+            g.xb[g.nxb] = m;
+            g.xbE[g.nxb] = EB[m];
+            g.spanB *= EB[m];
+            g.nxb++;
+        }
+    g.K = 1;
+    long double K = 1.0L;
+    for (int i = 0; i < r; i++) {
+        K *= (long double)g.R[i];
+        g.K *= g.R[i];
+    }
+    if (K >= 9.0e18L) g.K = ~0ull;
+    g.rbits = std::max(1, ceil_log2(g.spanA));
+    g.cbits = std::max(1, ceil_log2(g.spanB));
+    g.kbits = std::max(1, ceil_log2(g.K));
+    return g;
+}
+
+int validate(const fcc_tensor *t, const char *name, std::string &msg) {
+    if (!t) { msg = std::string(name) + " is NULL"; return FCC_ERR_INVALID; }
+    if (t->order < 1 || t->order > FCC_MAX_ORDER) { msg = std::string(name) + ": order out of range"; return FCC_ERR_INVALID; }
+    if (t->nnz < 0 || t->nnz >= (int64_t)1 << 32) { msg = std::string(name) + ": nnz out of range"; return FCC_ERR_INVALID; }
+    if (t->nnz > 0 && (!t->coords || !t->vals)) { msg = std::string(name) + ": NULL coords or vals"; return FCC_ERR_INVALID; }
+    return FCC_OK;
+}
+
+// Rows per left tile (and dense window) from global statistics, the tuner
+// table, or the caller's hint (the reference's tile_size, rounded up to a
+// power of two like contract.hpp:844-846).
+void choose_tiles(fcc_ctx *ctx, const fcc_options &o, const Geom &g, uint64_t products, uint32_t nvalid, int *logTL,
+                  uint32_t *Wd) {
+    double cells = (double)g.spanA * (double)g.spanB;
+    double dens = cells > 0 ? (double)products / cells : 0.0;
+    double rows = (double)g.spanA * (1.0 - std::exp(-(double)nvalid / std::max(1.0, (double)g.spanA)));
+    rows = std::max(1.0, rows);
+    double prow = (double)products / rows;
+    uint64_t TL = 1, W = 0;
+    int32_t bucket = fcc_density_bucket(dens);
+    const fcc_tune_entry *te = nullptr;
+    for (auto &e : ctx->tune)
+        if (e.bucket == bucket) te = &e;
+    if (o.tile_hint > 0) {
+        TL = pow2_ceil((uint64_t)o.tile_hint);
+    } else if (te && te->row_tile > 0) {
+        TL = (uint64_t)te->row_tile;
+    } else if (dens >= 0.25) {
+        uint64_t tf = 1;
+        while (tf * 2 * g.spanB <= DENSE_SLOTS) tf *= 2;
+        if (tf * g.spanB <= DENSE_SLOTS && tf >= 4) TL = std::min<uint64_t>(tf, 64);
+        else TL = 32;
+    } else {
+        double t = (double)HASH_TARGET / std::max(1.0, prow);
+        TL = 1;
+        while ((double)(TL * 2) <= t && TL < 1024) TL *= 2;
+    }
+    if (o.col_window > 0) W = (uint64_t)o.col_window;
+    else if (te && te->col_window > 0) W = (uint64_t)te->col_window;
+    TL = std::min<uint64_t>(TL, 4096);  // the dense LDS tile must hold at least one column per row
+    int lt = 0;
+    while ((1ull << lt) < TL) lt++;
+    lt = std::min(lt, g.rbits);
+    TL = 1ull << lt;
+    if (W == 0) {
+        W = 1;
+        while (W * 2 * TL <= DENSE_SLOTS) W *= 2;
+    }
+    if (W * TL > DENSE_SLOTS) W = std::max<uint64_t>(1, DENSE_SLOTS / TL);
+    *logTL = lt;
+    *Wd = (uint32_t)W;
+}
+
+template <typename T> T d2h(const T *dev, hipStream_t s) {
+    T h;
+    FCC_HIP(hipMemcpyAsync(&h, dev, sizeof(T), hipMemcpyDeviceToHost, s));
+    FCC_HIP(hipStreamSynchronize(s));
+    return h;
+}
+
+__global__ void k_sample_rows(const uint32_t *rowA, uint32_t *fidA, size_t n, int logT, uint32_t stride) {
+    size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+    if (i < n && ((rowA[i] >> logT) % stride) != 0) fidA[i] = 0xFFFFFFFFu;
+}
+
+__global__ void k_count_valid(const uint32_t *fidA, const uint32_t *fiber_ptr, size_t n, unsigned long long *out) {
+    size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+    unsigned long long c = 0, p = 0;
+    if (i < n && fidA[i] != 0xFFFFFFFFu) {
+        c = 1;
+        p = fiber_ptr[fidA[i] + 1] - fiber_ptr[fidA[i]];
+    }
+    for (int d = 32; d > 0; d >>= 1) {
+        c += __shfl_xor(c, d, 64);
+        p += __shfl_xor(p, d, 64);
+    }
+    if ((threadIdx.x & 63) == 0 && c) {
+        atomicAdd(&out[0], c);
+        atomicAdd(&out[1], p);
+    }
+}
+
+__global__ void k_unit_to_tile(const unsigned long long *unit_count, const uint32_t *unit_tile, uint32_t n,
+                               unsigned long long *tile_count) {
+    uint32_t u = blockIdx.x * blockDim.x + threadIdx.x;
+    if (u < n && unit_count[u]) atomicAdd(&tile_count[unit_tile[u]], unit_count[u]);
+}
+
+__global__ void k_kind_hist(const uint8_t *kind, uint32_t n, unsigned long long *h) {
+    uint32_t u = blockIdx.x * blockDim.x + threadIdx.x;
+    if (u < n) atomicAdd(&h[kind[u] & 3], 1ull);
+}
+
+// Distinct output columns per active tile = |union of its groups' fiber
+// columns| (training label). LDS bitmap when the column span allows, else an
+// LDS hash set that saturates at its capacity.
+constexpr uint32_t COLSET = 16384;
+__global__ __launch_bounds__(256) void k_tile_cols(const uint32_t *atile_g0, const uint32_t *group_fiber,
+                                                   const uint32_t *fiber_ptr, const uint32_t *Bcol, uint32_t natiles,
+                                                   uint64_t spanB, unsigned long long *tile_cols) {
+    __shared__ uint32_t set[COLSET];
+    __shared__ uint32_t cnt;
+    const bool bitmap = spanB <= (uint64_t)COLSET * 32;
+    for (uint32_t a = blockIdx.x; a < natiles; a += gridDim.x) {
+        for (uint32_t i = threadIdx.x; i < COLSET; i += blockDim.x) set[i] = bitmap ? 0u : 0xFFFFFFFFu;
+        if (threadIdx.x == 0) cnt = 0;
+        __syncthreads();
+        for (uint32_t g = atile_g0[a]; g < atile_g0[a + 1]; g++) {
+            uint32_t f = group_fiber[g];
+            for (uint32_t p = fiber_ptr[f] + threadIdx.x; p < fiber_ptr[f + 1]; p += blockDim.x) {
+                uint32_t c = Bcol[p];
+                if (bitmap) {
+                    atomicOr(&set[c >> 5], 1u << (c & 31));
+                } else {
+                    uint32_t h = (c * 2654435761u) >> 18;
+                    for (uint32_t pr = 0; pr < COLSET; pr++) {
+                        if (cnt >= COLSET * 3 / 4) break;
+                        uint32_t old = atomicCAS(&set[h], 0xFFFFFFFFu, c);
+                        if (old == 0xFFFFFFFFu) { atomicAdd(&cnt, 1u); break; }
+                        if (old == c) break;
+                        h = (h + 1) & (COLSET - 1);
+                    }
+                }
+            }
+        }
+        __syncthreads();
+        if (bitmap) {
+            uint32_t part = 0;
+            for (uint32_t i = threadIdx.x; i < COLSET; i += blockDim.x) part += __popc(set[i]);
+            atomicAdd(&cnt, part);
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) tile_cols[a] = cnt;
+        __syncthreads();
+    }
+}
+
+__global__ void k_labels(const unsigned long long *tile_nnz, const unsigned long long *tile_cols, uint32_t n,
+                         float *labels) {
+    uint32_t a = blockIdx.x * blockDim.x + threadIdx.x;
+    if (a >= n) return;
+    labels[2 * a] = log1pf((float)tile_nnz[a]);
+    labels[2 * a + 1] = log1pf((float)tile_cols[a]);
+}
+
+// MAPE of predicted vs actual tile nnz: sum |p - a| / max(a, 1)
+__global__ void k_mape(const float *pred, const unsigned long long *actual, uint32_t n, double *acc) {
+    uint32_t a = blockIdx.x * blockDim.x + threadIdx.x;
+    double e = 0.0;
+    if (a < n) {
+        double act = (double)actual[a];
+        e = fabs((double)pred[a] - act) / (act > 1.0 ? act : 1.0);
+    }
+    for (int d = 32; d > 0; d >>= 1) e += __shfl_xor(e, d, 64);
+    if ((threadIdx.x & 63) == 0 && e != 0.0) unsafeAtomicAdd(acc, e);
+}
+
+__global__ void k_pred_sum(const float *pred, uint32_t n, unsigned long long *sum) {
+    uint32_t a = blockIdx.x * blockDim.x + threadIdx.x;
+    unsigned long long v = a < n ? (unsigned long long)ceilf(pred[a]) : 0ull;
+    for (int d = 32; d > 0; d >>= 1) v += __shfl_xor(v, d, 64);
+    if ((threadIdx.x & 63) == 0 && v) atomicAdd(sum, v);
+}
+
+__global__ void k_summarize(const uint32_t *coords, const double *vals, uint64_t n, uint64_t stride, int d,
+                            const uint32_t *E, unsigned long long *hsum, double *vs) {
+    uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+    unsigned long long h = 0;
+    double s = 0.0, w = 0.0;
+    if (i < n) {
+        uint64_t lin = 0;
+        for (int m = 0; m < d; m++) lin = lin * E[m] + coords[(uint64_t)m * stride + i];
+        uint64_t z = lin + 0x9E3779B97F4A7C15ull;
+        z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+        z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+        z = z ^ (z >> 31);
+        h = z;
+        s = vals[i];
+        w = vals[i] * ((double)(z >> 11) * 0x1.0p-53);
+    }
+    for (int dd = 32; dd > 0; dd >>= 1) {
+        h += __shfl_xor(h, dd, 64);
+        s += __shfl_xor(s, dd, 64);
+        w += __shfl_xor(w, dd, 64);
+    }
+    if ((threadIdx.x & 63) == 0) {
+        atomicAdd(hsum, h);
+        unsafeAtomicAdd(&vs[0], s);
+        unsafeAtomicAdd(&vs[1], w);
+    }
+}
+
+unsigned grid256(size_t n) { return (unsigned)div_up(n ? n : 1, 256); }
+
+struct Contraction {
+    fcc_ctx *ctx;
+    hipStream_t s;
+    fcc_options o;
+    fcc_result *out;
+
+    int run(const fcc_tensor *a, const int32_t *am, const fcc_tensor *b, const int32_t *bm, int r);
+};
+
+int Contraction::run(const fcc_tensor *ta, const int32_t *am, const fcc_tensor *tb, const int32_t *bm, int r) {
+    DevPool &pool = ctx->pool;
+    hipEvent_t *ev = ctx->ev;
+    FCC_HIP(hipEventRecord(ev[0], s));
+
+    // ---- operands on the device
+    Tmp<uint32_t> ca, cb;
+    Tmp<double> va, vb;
+    Operand A{ta->order, ta->nnz, ta->coords, ta->vals}, B{tb->order, tb->nnz, tb->coords, tb->vals};
+    if (!ta->on_device && ta->nnz) {
+        ca = Tmp<uint32_t>(pool, (size_t)ta->order * ta->nnz);
+        va = Tmp<double>(pool, (size_t)ta->nnz);
+        FCC_HIP(hipMemcpyAsync(ca.get(), ta->coords, 4ull * ta->order * ta->nnz, hipMemcpyHostToDevice, s));
+        FCC_HIP(hipMemcpyAsync(va.get(), ta->vals, 8ull * ta->nnz, hipMemcpyHostToDevice, s));
+        A.coords = ca;
+        A.vals = va;
+    }
+    if (!tb->on_device && tb->nnz) {
+        if (tb == ta || (tb->coords == ta->coords && tb->vals == ta->vals && tb->order == ta->order &&
+                         tb->nnz == ta->nnz && !ta->on_device)) {
+            B.coords = A.coords;
+            B.vals = A.vals;
+        } else {
+            cb = Tmp<uint32_t>(pool, (size_t)tb->order * tb->nnz);
+            vb = Tmp<double>(pool, (size_t)tb->nnz);
+            FCC_HIP(hipMemcpyAsync(cb.get(), tb->coords, 4ull * tb->order * tb->nnz, hipMemcpyHostToDevice, s));
+            FCC_HIP(hipMemcpyAsync(vb.get(), tb->vals, 8ull * tb->nnz, hipMemcpyHostToDevice, s));
+            B.coords = cb;
+            B.vals = vb;
+        }
+    }
+
+    // ---- extents (contract.hpp:124-144) and the zero/tiny value flag
+    Tmp<uint32_t> dext(pool, 2 * (kMaxOrder + 1));
+    fill_u32(dext, 2 * (kMaxOrder + 1), 0, s);
+    launch_extents(A, dext.get(), s);
+    launch_extents(B, dext.get() + kMaxOrder + 1, s);
+    uint32_t hext[2 * (kMaxOrder + 1)];
+    FCC_HIP(hipMemcpyAsync(hext, dext.get(), sizeof(hext), hipMemcpyDeviceToHost, s));
+    FCC_HIP(hipStreamSynchronize(s));
+    uint32_t EA[kMaxOrder], EB[kMaxOrder];
+    for (int m = 0; m < ta->order; m++) EA[m] = ta->extents ? ta->extents[m] : (ta->nnz ? hext[m] + 1 : 0);
+    for (int m = 0; m < tb->order; m++)
+        EB[m] = tb->extents ? tb->extents[m] : (tb->nnz ? hext[kMaxOrder + 1 + m] + 1 : 0);
+    const bool bitmap = (hext[ta->order] | hext[kMaxOrder + 1 + tb->order]) != 0;
+    if (ta->extents)
+        for (int m = 0; m < ta->order; m++)
+            if (ta->nnz && hext[m] >= EA[m]) throw LibError{FCC_ERR_INVALID, "a: coordinate exceeds given extent"};
+    if (tb->extents)
+        for (int m = 0; m < tb->order; m++)
+            if (tb->nnz && hext[kMaxOrder + 1 + m] >= EB[m])
+                throw LibError{FCC_ERR_INVALID, "b: coordinate exceeds given extent"};
+
+    Geom g = make_geom(ta, am, tb, bm, r, EA, EB);
+    out->order = g.dC;
+    for (int m = 0; m < g.nxa; m++) out->extents[m] = g.xaE[m];
+    for (int m = 0; m < g.nxb; m++) out->extents[g.nxa + m] = g.xbE[m];
+    fcc_stats &st = out->stats;
+    st.mlp_mape = -1.0;
+    if (ta->nnz == 0 || tb->nnz == 0 || g.spanA == 0 || g.spanB == 0) {
+        FCC_HIP(hipEventRecord(ev[1], s));
+        FCC_HIP(hipEventSynchronize(ev[1]));
+        st.ms_total = ev_ms(ev[0], ev[1]);
+        return FCC_OK;
+    }
+    if (g.spanA >= (1ull << 32) || g.spanB >= (1ull << 32))
+        throw LibError{FCC_ERR_INVALID, "external index span must stay below 2^32"};
+    if (g.K == ~0ull || g.kbits + g.cbits > 64)
+        throw LibError{FCC_ERR_INVALID, "contraction key and column exceed 64 bits"};
+
+    // ---- index
+    std::vector<std::shared_ptr<void>> hold;
+    Index ix;
+    build_right(pool, g, B, ix, hold, s);
+    Tmp<uint32_t> rowA(pool, (size_t)A.nnz), fidA(pool, (size_t)A.nnz);
+    uint64_t products = 0;
+    uint32_t nvalid = 0;
+    join_left(pool, g, A, ix, rowA, fidA, &products, &nvalid, s);
+    if (o.left_tile_stride > 1) {
+        uint64_t T = pow2_ceil(o.tile_hint > 0 ? (uint64_t)o.tile_hint : 128);
+        int lT = ceil_log2(T);
+        k_sample_rows<<<grid256(A.nnz), 256, 0, s>>>(rowA, fidA, (size_t)A.nnz, lT, (uint32_t)o.left_tile_stride);
+        Tmp<unsigned long long> cv(pool, 2);
+        fill_u64(cv, 2, 0, s);
+        k_count_valid<<<grid256(A.nnz), 256, 0, s>>>(fidA, ix.fiber_ptr, (size_t)A.nnz, cv);
+        unsigned long long h[2];
+        FCC_HIP(hipMemcpyAsync(h, cv.get(), 16, hipMemcpyDeviceToHost, s));
+        FCC_HIP(hipStreamSynchronize(s));
+        nvalid = (uint32_t)h[0];
+        products = h[1];
+    }
+    st.products = products;
+    int logTL;
+    uint32_t Wd;
+    choose_tiles(ctx, o, g, products, nvalid, &logTL, &Wd);
+    build_left(pool, g, A, rowA, fidA, nvalid, logTL, ix, hold, s);
+    rowA.reset();
+    fidA.reset();
+    st.row_tile = (int32_t)ix.TL;
+    st.col_window = (int32_t)Wd;
+    st.left_tiles = ix.natiles;
+    FCC_HIP(hipEventRecord(ev[1], s));
+
+    const bool learned = o.planner == FCC_PLAN_LEARNED;
+    const bool sink = o.sink == FCC_SINK_COUNT_CHECKSUM;
+    if (learned && !ctx->mlp_ready) throw LibError{FCC_ERR_INVALID, "learned planner requested but no MLP parameters set"};
+
+    uint64_t nnz_out = 0;
+    Tmp<unsigned long long> sinkbuf(pool, 4);
+    fill_u64(sinkbuf, 4, 0, s);
+    if (ix.natiles == 0) {
+        FCC_HIP(hipEventRecord(ev[2], s));
+        FCC_HIP(hipEventSynchronize(ev[2]));
+        st.ms_index = ev_ms(ev[0], ev[1]);
+        st.ms_total = ev_ms(ev[0], ev[2]);
+        st.bytes_alg = (uint64_t)ta->nnz * (4 * ta->order + 8) + (uint64_t)tb->nnz * (4 * tb->order + 8);
+        return FCC_OK;
+    }
+
+    // ---- plan
+    const uint32_t T = ix.natiles;
+    Tmp<float> feat, pred_t, predc_t;
+    if (learned || o.collect_stats) {
+        feat = Tmp<float>(pool, (size_t)T * MLP_IN);
+        mlp_tile_features(ix, g.spanB, feat, s);
+    }
+    if (learned) {
+        pred_t = Tmp<float>(pool, T);
+        predc_t = Tmp<float>(pool, T);
+        mlp_infer(ctx->mlp_dev, feat, T, pred_t, predc_t, s);
+    }
+    PlanParams P;
+    P.force = o.force_acc;
+    P.TL = ix.TL;
+    P.spanB = g.spanB;
+    P.Wd = Wd;
+    P.margin = 0.15f;
+    P.dense_ratio = learned ? 0.15f : 0.5f;
+    Tmp<uint32_t> t_nwin(pool, T), t_W(pool, T), t_ubase(pool, T), d_nunits(pool, 1);
+    Tmp<uint8_t> t_kind(pool, T);
+    launch_plan_tiles(ix, learned ? pred_t.get() : nullptr, P, t_nwin, t_W, t_kind, s);
+    scan_u32(pool, t_nwin, t_ubase, T, d_nunits, s);
+    Units U;
+    U.n = d2h(d_nunits.get(), s);
+    Tmp<uint32_t> u_tile(pool, U.n), u_c0(pool, U.n), u_W(pool, U.n);
+    Tmp<uint8_t> u_kind(pool, U.n), u_multi(pool, U.n);
+    Tmp<float> u_pred;
+    if (learned) u_pred = Tmp<float>(pool, U.n);
+    U.tile = u_tile;
+    U.c0 = u_c0;
+    U.W = u_W;
+    U.kind = u_kind;
+    U.multi = u_multi;
+    U.pred = u_pred.get();
+    launch_emit_units(t_nwin, t_W, t_kind, t_ubase, T, g.spanB, learned ? pred_t.get() : nullptr, U, s);
+    st.units = U.n;
+    FCC_HIP(hipEventRecord(ev[2], s));
+
+    // ---- accumulate arguments
+    AccArgs X;
+    memset(&X, 0, sizeof(X));
+    X.Bcol = ix.Bcol;
+    X.Bval = ix.Bval;
+    X.fiber_ptr = ix.fiber_ptr;
+    X.Ali = ix.Ali;
+    X.Aval = ix.Aval;
+    X.Afib = ix.Afib;
+    X.group_ptr = ix.group_ptr;
+    X.group_fiber = ix.group_fiber;
+    X.atile_id = ix.atile_id;
+    X.atile_g0 = ix.atile_g0;
+    X.atile_lg = ix.atile_lg;
+    X.nunits = U.n;
+    X.unit_tile = U.tile;
+    X.unit_c0 = U.c0;
+    X.unit_W = U.W;
+    X.unit_kind = U.kind;
+    X.unit_multi = U.multi;
+    X.unit_order = nullptr;
+    X.logTL = ix.logTL;
+    X.spanB = g.spanB;
+    X.nxa = g.nxa;
+    X.nxb = g.nxb;
+    for (int m = 0; m < kMaxOrder; m++) {
+        X.xaE[m] = g.xaE[m];
+        X.xbE[m] = g.xbE[m];
+    }
+    Tmp<unsigned int> work(pool, 4);
+    Tmp<unsigned long long> gmeta(pool, 4);  // [0]=gcount [1]=cursor [2]=gtotal
+    Tmp<uint32_t> flags(pool, 4);            // [0]=gflag [1]=fail
+    X.work = work;
+    X.gcount = gmeta.get();
+    X.cursor = gmeta.get() + 1;
+    X.gflag = flags.get();
+    X.fail_flag = flags.get() + 1;
+    X.sink = sinkbuf.get();
+    X.sinkf = (double *)(sinkbuf.get() + 2);
+    unsigned grid = (unsigned)(o.workgroups > 0 ? o.workgroups : ctx->num_cus * 2);
+    grid = std::max(1u, std::min<unsigned>(grid, U.n));
+
+    Tmp<unsigned long long> gk;
+    Tmp<double> gv;
+    Tmp<uint32_t> gu;
+    auto alloc_ghash = [&](uint64_t want) {
+        uint64_t cap = pow2_ceil(std::max<uint64_t>(want, 1u << 16));
+        gk = Tmp<unsigned long long>(pool, cap);
+        gv = Tmp<double>(pool, cap);
+        gu = Tmp<uint32_t>(pool, cap);
+        X.gkeys = gk;
+        X.gvals = gv;
+        X.gunit = gu;
+        X.gmask = cap - 1;
+    };
+    auto reset_ghash = [&]() {
+        fill_u64(X.gkeys, X.gmask + 1, ~0ull, s);
+        fill_f64(X.gvals, X.gmask + 1, 0.0, s);
+        fill_u64(gmeta, 4, 0, s);
+        fill_u32(flags, 4, 0, s);
+        fill_u32(work, 4, 0, s);
+    };
+
+    Tmp<unsigned long long> unit_count(pool, U.n), unit_off(pool, U.n), unit_cur(pool, U.n), d_total(pool, 1);
+    X.unit_count = unit_count;
+    X.unit_off = unit_off;
+    X.unit_cursor = unit_cur;
+    Tmp<uint32_t> out_c;
+    Tmp<double> out_v;
+    uint64_t stride = 0;
+
+    if (!learned) {
+        // ---- exact symbolic planning: count, scan, then fill
+        alloc_ghash(1u << 16);
+        for (int attempt = 0;; attempt++) {
+            reset_ghash();
+            fill_u64(unit_count, U.n, 0, s);
+            launch_count(X, grid, s);
+            uint32_t gf = d2h(flags.get(), s);
+            if (!gf) break;
+            if (attempt >= 3) throw LibError{FCC_ERR_OVERFLOW, "count-pass global set overflow"};
+            alloc_ghash((X.gmask + 1) * 16);
+        }
+        launch_refine_kinds(unit_count, U.n, o.force_acc, U.kind, gmeta.get() + 2, s);
+        scan_u64(pool, (const uint64_t *)unit_count.get(), (uint64_t *)unit_off.get(), U.n, (uint64_t *)d_total.get(),
+                 s);
+        unsigned long long hm[4];
+        FCC_HIP(hipMemcpyAsync(hm, gmeta.get(), 32, hipMemcpyDeviceToHost, s));
+        nnz_out = d2h(d_total.get(), s);
+        uint64_t gtotal = hm[2];
+        FCC_HIP(hipEventRecord(ev[3], s));
+        if (o.collect_stats) {
+            // labels: exact tile nnz and distinct output columns
+            Tmp<unsigned long long> tn(pool, T), tc(pool, T);
+            fill_u64(tn, T, 0, s);
+            k_unit_to_tile<<<grid256(U.n), 256, 0, s>>>(unit_count, U.tile, U.n, tn);
+            k_tile_cols<<<std::min<unsigned>(T, 4096), 256, 0, s>>>(ix.atile_g0, ix.group_fiber, ix.fiber_ptr,
+                                                                     ix.Bcol, T, g.spanB, tc);
+            Tmp<float> lab(pool, (size_t)T * 2);
+            k_labels<<<grid256(T), 256, 0, s>>>(tn, tc, T, lab);
+            ctx->feat.resize((size_t)T * MLP_IN);
+            ctx->labels.resize((size_t)T * 2);
+            FCC_HIP(hipMemcpyAsync(ctx->feat.data(), feat.get(), ctx->feat.size() * 4, hipMemcpyDeviceToHost, s));
+            FCC_HIP(hipMemcpyAsync(ctx->labels.data(), lab.get(), ctx->labels.size() * 4, hipMemcpyDeviceToHost, s));
+            FCC_HIP(hipStreamSynchronize(s));
+        }
+        if (!sink) {
+            out_c = Tmp<uint32_t>(pool, (size_t)g.dC * nnz_out);
+            out_v = Tmp<double>(pool, (size_t)nnz_out);
+        }
+        stride = nnz_out;
+        X.ocoords = out_c.get();
+        X.ovals = out_v.get();
+        X.ostride = stride;
+        X.capacity = nnz_out;
+        if (gtotal > (X.gmask + 1) / 2) alloc_ghash(2 * gtotal);
+        reset_ghash();
+        FCC_HIP(hipEventRecord(ev[4], s));
+        launch_numeric(X, grid, bitmap, sink, 0, s);
+        FCC_HIP(hipEventRecord(ev[5], s));
+        launch_gdrain(X, sink, 0, s);
+        FCC_HIP(hipEventRecord(ev[6], s));
+        uint32_t hf[2];
+        FCC_HIP(hipMemcpyAsync(hf, flags.get(), 8, hipMemcpyDeviceToHost, s));
+        FCC_HIP(hipStreamSynchronize(s));
+        if (hf[0]) throw LibError{FCC_ERR_OVERFLOW, "global hash overflow in the numeric pass"};
+        if (hf[1]) throw LibError{FCC_ERR_OVERFLOW, "numeric pass disagreed with the count pass"};
+        st.ghash_entries = d2h(gmeta.get(), s);
+    } else {
+        // ---- learned planning: predicted capacity, single numeric pass
+        Tmp<unsigned long long> psum(pool, 1);
+        fill_u64(psum, 1, 0, s);
+        k_pred_sum<<<grid256(T), 256, 0, s>>>(pred_t, T, psum);
+        uint64_t predicted = d2h(psum.get(), s);
+        uint64_t capacity = (uint64_t)((double)predicted * 1.10) + 4096;
+        alloc_ghash(1u << 20);
+        FCC_HIP(hipEventRecord(ev[3], s));
+        for (int attempt = 0;; attempt++) {
+            if (!sink) {
+                out_c = Tmp<uint32_t>(pool, (size_t)g.dC * capacity);
+                out_v = Tmp<double>(pool, (size_t)capacity);
+            }
+            stride = capacity;
+            X.ocoords = out_c.get();
+            X.ovals = out_v.get();
+            X.ostride = stride;
+            X.capacity = sink ? ~0ull : capacity;
+            reset_ghash();
+            if (sink) fill_u64(sinkbuf, 4, 0, s);
+            if (attempt == 0) FCC_HIP(hipEventRecord(ev[4], s));
+            launch_numeric(X, grid, bitmap, sink, 1, s);
+            if (attempt == 0) FCC_HIP(hipEventRecord(ev[5], s));
+            launch_gdrain(X, sink, 1, s);
+            uint32_t hf[2];
+            FCC_HIP(hipMemcpyAsync(hf, flags.get(), 8, hipMemcpyDeviceToHost, s));
+            unsigned long long hm[4];
+            FCC_HIP(hipMemcpyAsync(hm, gmeta.get(), 32, hipMemcpyDeviceToHost, s));
+            FCC_HIP(hipStreamSynchronize(s));
+            st.ghash_entries = hm[0];
+            if (hf[0]) {
+                // global hash exhausted: grow it and run again
+                if (attempt >= 4) throw LibError{FCC_ERR_OVERFLOW, "global hash overflow in the learned pass"};
+                alloc_ghash((X.gmask + 1) * 16);
+                st.reruns++;
+                continue;
+            }
+            if (hf[1]) {
+                capacity = hm[1];
+                st.reruns++;
+                if (attempt >= 4) throw LibError{FCC_ERR_OVERFLOW, "learned capacity re-run did not converge"};
+                continue;
+            }
+            nnz_out = sink ? 0 : hm[1];
+            break;
+        }
+        FCC_HIP(hipEventRecord(ev[6], s));
+        // tile-level MAPE against the actual per-unit outputs (LDS part)
+        Tmp<unsigned long long> tn(pool, T);
+        fill_u64(tn, T, 0, s);
+        k_unit_to_tile<<<grid256(U.n), 256, 0, s>>>(unit_cur, U.tile, U.n, tn);
+        Tmp<double> mp(pool, 1);
+        fill_f64(mp, 1, 0.0, s);
+        k_mape<<<grid256(T), 256, 0, s>>>(pred_t, tn, T, mp);
+        (void)mp;
+    }
+
+    // ---- results
+    unsigned long long sk[4];
+    FCC_HIP(hipMemcpyAsync(sk, sinkbuf.get(), 32, hipMemcpyDeviceToHost, s));
+    FCC_HIP(hipStreamSynchronize(s));
+    if (sink) {
+        nnz_out = sk[0];
+        out->coord_hash = sk[0] ? sk[1] : 0;
+        double d[2];
+        memcpy(d, &sk[2], 16);
+        out->val_sum = d[0];
+        out->val_wsum = d[1];
+    }
+    out->nnz = (int64_t)nnz_out;
+    out->coord_stride = (int64_t)stride;
+    if (!sink) {
+        ResultPriv *rp = new ResultPriv;
+        rp->coords = out_c.p;
+        rp->vals = out_v.p;
+        out->coords = out_c.p;
+        out->vals = out_v.p;
+        out_c.p = nullptr;
+        out_v.p = nullptr;
+        out->priv = rp;
+        if (o.summarize && nnz_out) {
+            Tmp<unsigned long long> hs(pool, 1);
+            Tmp<double> vs(pool, 2);
+            Tmp<uint32_t> E(pool, kMaxOrder);
+            fill_u64(hs, 1, 0, s);
+            fill_f64(vs, 2, 0.0, s);
+            FCC_HIP(hipMemcpyAsync(E.get(), out->extents, 4 * kMaxOrder, hipMemcpyHostToDevice, s));
+            k_summarize<<<grid256(nnz_out), 256, 0, s>>>(out->coords, out->vals, nnz_out, stride, g.dC, E, hs, vs);
+            unsigned long long hh;
+            double dv[2];
+            FCC_HIP(hipMemcpyAsync(&hh, hs.get(), 8, hipMemcpyDeviceToHost, s));
+            FCC_HIP(hipMemcpyAsync(dv, vs.get(), 16, hipMemcpyDeviceToHost, s));
+            FCC_HIP(hipStreamSynchronize(s));
+            out->coord_hash = hh;
+            out->val_sum = dv[0];
+            out->val_wsum = dv[1];
+        }
+    }
+    // stats
+    {
+        Tmp<unsigned long long> kh(pool, 4);
+        fill_u64(kh, 4, 0, s);
+        k_kind_hist<<<grid256(U.n), 256, 0, s>>>(U.kind, U.n, kh);
+        unsigned long long h[4];
+        FCC_HIP(hipMemcpyAsync(h, kh.get(), 32, hipMemcpyDeviceToHost, s));
+        FCC_HIP(hipStreamSynchronize(s));
+        st.units_dense = h[KIND_DENSE];
+        st.units_lhash = h[KIND_LHASH];
+        st.units_ghash = h[KIND_GHASH];
+    }
+    st.ms_index = ev_ms(ev[0], ev[1]);
+    st.ms_plan = ev_ms(ev[1], ev[2]);
+    st.ms_symbolic = learned ? 0.0 : ev_ms(ev[2], ev[3]);
+    st.ms_numeric = ev_ms(ev[4], ev[5]);
+    st.ms_numeric_kernel = st.ms_numeric;
+    st.ms_overflow = ev_ms(ev[5], ev[6]);
+    st.ms_total = ev_ms(ev[0], ev[6]);
+    st.bytes_alg = (uint64_t)ta->nnz * (4 * ta->order + 8) + (uint64_t)tb->nnz * (4 * tb->order + 8) +
+                   nnz_out * (4 * (uint64_t)g.dC + 8);
+    return FCC_OK;
+}
+
+}  // namespace
+
+// ====================================================================== ABI
+
+extern "C" {
+
+void fcc_options_default(fcc_options *o) {
+    if (!o) return;
+    memset(o, 0, sizeof(*o));
+    o->planner = FCC_PLAN_EXACT;
+    o->sink = FCC_SINK_MATERIALIZE;
+    o->force_acc = FCC_ACC_AUTO;
+    o->left_tile_stride = 1;
+}
+
+int fcc_create(fcc_ctx **out, int device) {
+    if (!out) return FCC_ERR_INVALID;
+    *out = nullptr;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) {
+        (void)hipGetLastError();
+        return FCC_ERR_HIP;
+    }
+    if (device < 0 || device >= n) return FCC_ERR_INVALID;
+    fcc_ctx *c = new (std::nothrow) fcc_ctx;
+    if (!c) return FCC_ERR_OOM;
+    c->device = device;
+    try {
+        FCC_HIP(hipSetDevice(device));
+        FCC_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+        for (auto &e : c->ev) FCC_HIP(hipEventCreate(&e));
+        hipDeviceProp_t prop;
+        FCC_HIP(hipGetDeviceProperties(&prop, device));
+        c->num_cus = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 256;
+        FCC_HIP(hipMalloc(&c->mlp_dev, sizeof(float) * MLP_PARAMS));
+    } catch (HipError &e) {
+        for (auto &ev : c->ev)
+            if (ev) (void)hipEventDestroy(ev);
+        if (c->stream) (void)hipStreamDestroy(c->stream);
+        delete c;
+        return FCC_ERR_HIP;
+    }
+    *out = c;
+    return FCC_OK;
+}
+
+void fcc_destroy(fcc_ctx *c) {
+    if (!c) return;
+    (void)hipSetDevice(c->device);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    c->pool.release_all();
+    if (c->mlp_dev) (void)hipFree(c->mlp_dev);
+    for (auto &e : c->ev)
+        if (e) (void)hipEventDestroy(e);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+}
+
+const char *fcc_last_error(const fcc_ctx *c) { return c ? c->err.c_str() : "no context"; }
+
+void *fcc_stream(fcc_ctx *c) { return c ? (void *)c->stream : nullptr; }
+
+int fcc_contract(fcc_ctx *ctx, const fcc_tensor *a, const int32_t *a_modes, const fcc_tensor *b,
+                 const int32_t *b_modes, int32_t n_contr, const fcc_options *options, fcc_result *out) {
+    if (!ctx) return FCC_ERR_INVALID;
+    if (!out) return fail(ctx, FCC_ERR_INVALID, "out is NULL");
+    memset(out, 0, sizeof(*out));
+    std::string msg;
+    int rc = validate(a, "a", msg);
+    if (!rc) rc = validate(b, "b", msg);
+    if (rc) return fail(ctx, rc, msg);
+    if (n_contr < 0 || n_contr > a->order || n_contr > b->order || (n_contr > 0 && (!a_modes || !b_modes)))
+        return fail(ctx, FCC_ERR_INVALID, "bad contraction mode count");
+    for (int i = 0; i < n_contr; i++) {
+        if (a_modes[i] < 0 || a_modes[i] >= a->order || b_modes[i] < 0 || b_modes[i] >= b->order)
+            return fail(ctx, FCC_ERR_INVALID, "contraction mode out of range");
+        for (int k = 0; k < i; k++)
+            if (a_modes[k] == a_modes[i] || b_modes[k] == b_modes[i])
+                return fail(ctx, FCC_ERR_INVALID, "repeated contraction mode");
+    }
+    if (a->order + b->order - 2 * n_contr > FCC_MAX_ORDER) return fail(ctx, FCC_ERR_INVALID, "output order too large");
+    Contraction c;
+    c.ctx = ctx;
+    c.s = ctx->stream;
+    fcc_options_default(&c.o);
+    if (options) c.o = *options;
+    if (c.o.left_tile_stride < 1) c.o.left_tile_stride = 1;
+    c.out = out;
+    try {
+        FCC_HIP(hipSetDevice(ctx->device));
+        rc = c.run(a, a_modes, b, b_modes, n_contr);
+        ctx->err.clear();
+        return rc;
+    } catch (HipError &e) {
+        (void)hipGetLastError();
+        fcc_result_free(ctx, out);
+        return fail(ctx, FCC_ERR_HIP, e.where + ": " + hipGetErrorString(e.code));
+    } catch (LibError &e) {
+        fcc_result_free(ctx, out);
+        return fail(ctx, e.code, e.msg);
+    } catch (std::bad_alloc &) {
+        fcc_result_free(ctx, out);
+        return fail(ctx, FCC_ERR_OOM, "host allocation failed");
+    }
+}
+
+int fcc_result_to_host(fcc_ctx *ctx, const fcc_result *r, uint32_t *coords, double *vals) {
+    if (!ctx || !r) return FCC_ERR_INVALID;
+    if (r->nnz == 0) return FCC_OK;
+    if (!r->coords || !r->vals) return fail(ctx, FCC_ERR_INVALID, "result holds no materialised output");
+    try {
+        for (int m = 0; m < r->order && coords; m++)
+            FCC_HIP(hipMemcpyAsync(coords + (size_t)m * r->nnz, r->coords + (size_t)m * r->coord_stride,
+                                   4ull * r->nnz, hipMemcpyDeviceToHost, ctx->stream));
+        if (vals)
+            FCC_HIP(hipMemcpyAsync(vals, r->vals, 8ull * r->nnz, hipMemcpyDeviceToHost, ctx->stream));
+        FCC_HIP(hipStreamSynchronize(ctx->stream));
+    } catch (HipError &e) {
+        (void)hipGetLastError();
+        return fail(ctx, FCC_ERR_HIP, e.where);
+    }
+    return FCC_OK;
+}
+
+void fcc_result_free(fcc_ctx *ctx, fcc_result *r) {
+    if (!r) return;
+    ResultPriv *rp = (ResultPriv *)r->priv;
+    if (rp && ctx) {
+        if (rp->coords) ctx->pool.put(rp->coords);
+        if (rp->vals) ctx->pool.put(rp->vals);
+    }
+    delete rp;
+    r->priv = nullptr;
+    r->coords = nullptr;
+    r->vals = nullptr;
+}
+
+int32_t fcc_density_bucket(double d) {
+    if (!(d > 0)) return 0;
+    int b = (int)std::floor(std::log10(d)) + 6;  // 1e-6 -> 0 ... 1e3 -> 9
+    return std::max(0, std::min(11, b));
+}
+
+int fcc_tune_set(fcc_ctx *ctx, const fcc_tune_entry *e, int32_t n) {
+    if (!ctx || (n > 0 && !e) || n < 0) return FCC_ERR_INVALID;
+    ctx->tune.assign(e, e + n);
+    return FCC_OK;
+}
+
+int fcc_tune_get(fcc_ctx *ctx, fcc_tune_entry *e, int32_t cap, int32_t *n) {
+    if (!ctx || !n) return FCC_ERR_INVALID;
+    *n = (int32_t)ctx->tune.size();
+    for (int32_t i = 0; i < std::min<int32_t>(cap, *n); i++) e[i] = ctx->tune[i];
+    return FCC_OK;
+}
+
+int64_t fcc_mlp_param_count(void) { return MLP_PARAMS; }
+
+int fcc_mlp_set_params(fcc_ctx *ctx, const float *p) {
+    if (!ctx || !p) return FCC_ERR_INVALID;
+    ctx->mlp_host.assign(p, p + MLP_PARAMS);
+    if (hipMemcpy(ctx->mlp_dev, p, sizeof(float) * MLP_PARAMS, hipMemcpyHostToDevice) != hipSuccess) {
+        (void)hipGetLastError();
+        return fail(ctx, FCC_ERR_HIP, "mlp upload");
+    }
+    ctx->mlp_ready = true;
+    return FCC_OK;
+}
+
+int fcc_mlp_get_params(fcc_ctx *ctx, float *p) {
+    if (!ctx || !p) return FCC_ERR_INVALID;
+    if (!ctx->mlp_ready) return fail(ctx, FCC_ERR_INVALID, "no MLP parameters");
+    if (hipMemcpy(p, ctx->mlp_dev, sizeof(float) * MLP_PARAMS, hipMemcpyDeviceToHost) != hipSuccess) {
+        (void)hipGetLastError();
+        return fail(ctx, FCC_ERR_HIP, "mlp download");
+    }
+    return FCC_OK;
+}
+
+int64_t fcc_tile_stats_count(fcc_ctx *ctx) { return ctx ? (int64_t)ctx->labels.size() / 2 : 0; }
+
+int fcc_tile_stats_copy(fcc_ctx *ctx, float *features, float *labels, int64_t n) {
+    if (!ctx || n < 0 || n > fcc_tile_stats_count(ctx)) return FCC_ERR_INVALID;
+    if (features) memcpy(features, ctx->feat.data(), sizeof(float) * n * MLP_IN);
+    if (labels) memcpy(labels, ctx->labels.data(), sizeof(float) * n * 2);
+    return FCC_OK;
+}
+
+int fcc_mlp_forward_backward(fcc_ctx *ctx, const float *features, const float *labels, int64_t n, float *pred,
+                             float *loss, float *grad) {
+    if (!ctx || !features || !labels || n <= 0 || !ctx->mlp_ready) return FCC_ERR_INVALID;
+    try {
+        FCC_HIP(hipSetDevice(ctx->device));
+        hipStream_t s = ctx->stream;
+        Tmp<float> f(ctx->pool, (size_t)n * MLP_IN), l(ctx->pool, (size_t)n * MLP_OUT), g(ctx->pool, MLP_PARAMS),
+            ls(ctx->pool, 1), pr(ctx->pool, (size_t)n * MLP_OUT);
+        FCC_HIP(hipMemcpyAsync(f.get(), features, 4ull * n * MLP_IN, hipMemcpyHostToDevice, s));
+        FCC_HIP(hipMemcpyAsync(l.get(), labels, 4ull * n * MLP_OUT, hipMemcpyHostToDevice, s));
+        mlp_fwd_bwd(ctx->mlp_dev, f, l, nullptr, (uint32_t)n, g, ls, pr, s);
+        std::vector<float> gh(MLP_PARAMS, 0.f);
+        FCC_HIP(hipMemcpyAsync(gh.data(), g.get(), 4ull * mlp_trainable_count(), hipMemcpyDeviceToHost, s));
+        if (pred) FCC_HIP(hipMemcpyAsync(pred, pr.get(), 4ull * n * MLP_OUT, hipMemcpyDeviceToHost, s));
+        if (loss) FCC_HIP(hipMemcpyAsync(loss, ls.get(), 4, hipMemcpyDeviceToHost, s));
+        FCC_HIP(hipStreamSynchronize(s));
+        if (grad) memcpy(grad, gh.data(), sizeof(float) * MLP_PARAMS);
+    } catch (HipError &e) {
+        (void)hipGetLastError();
+        return fail(ctx, FCC_ERR_HIP, e.where);
+    } catch (LibError &e) {
+        return fail(ctx, e.code, e.msg);
+    }
+    return FCC_OK;
+}
+
+int fcc_mlp_train(fcc_ctx *ctx, const float *features, const float *labels, int64_t n, int32_t epochs, int32_t batch,
+                  float lr, uint64_t seed, float *loss_out) {
+    if (!ctx || !features || !labels || n <= 0 || epochs <= 0 || batch <= 0) return FCC_ERR_INVALID;
+    try {
+        FCC_HIP(hipSetDevice(ctx->device));
+        hipStream_t s = ctx->stream;
+        // normalisation from the training set (host), then Xavier-uniform init
+        std::vector<float> P(MLP_PARAMS, 0.f);
+        const int OFF_IM = MLP_PARAMS - 2 * (MLP_IN + MLP_OUT);
+        std::mt19937_64 rng(seed);
+        for (int i = 0; i < MLP_IN; i++) {
+            double m = 0, v = 0;
+            for (int64_t k = 0; k < n; k++) m += features[k * MLP_IN + i];
+            m /= n;
+            for (int64_t k = 0; k < n; k++) v += (features[k * MLP_IN + i] - m) * (features[k * MLP_IN + i] - m);
+            v = std::sqrt(v / n);
+            P[OFF_IM + i] = (float)m;
+            P[OFF_IM + MLP_IN + i] = (float)(v > 1e-3 ? v : 1.0);
+        }
+        for (int o = 0; o < MLP_OUT; o++) {
+            double m = 0, v = 0;
+            for (int64_t k = 0; k < n; k++) m += labels[k * MLP_OUT + o];
+            m /= n;
+            for (int64_t k = 0; k < n; k++) v += (labels[k * MLP_OUT + o] - m) * (labels[k * MLP_OUT + o] - m);
+            v = std::sqrt(v / n);
+            P[OFF_IM + 2 * MLP_IN + o] = (float)m;
+            P[OFF_IM + 2 * MLP_IN + MLP_OUT + o] = (float)(v > 1e-3 ? v : 1.0);
+        }
+        auto init = [&](int off, int fan_in, int fan_out) {
+            float lim = std::sqrt(6.f / (fan_in + fan_out));
+            std::uniform_real_distribution<float> U(-lim, lim);
+            for (int i = 0; i < fan_in * fan_out; i++) P[off + i] = U(rng);
+        };
+        const int W1 = 0, B1 = W1 + MLP_IN * MLP_HID, W2 = B1 + MLP_HID, B2 = W2 + MLP_HID * MLP_HID,
+                  W3 = B2 + MLP_HID;
+        init(W1, MLP_IN, MLP_HID);
+        init(W2, MLP_HID, MLP_HID);
+        init(W3, MLP_HID, MLP_OUT);
+        FCC_HIP(hipMemcpyAsync(ctx->mlp_dev, P.data(), 4ull * MLP_PARAMS, hipMemcpyHostToDevice, s));
+        Tmp<float> f(ctx->pool, (size_t)n * MLP_IN), l(ctx->pool, (size_t)n * MLP_OUT), g(ctx->pool, MLP_PARAMS),
+            ls(ctx->pool, 1), m1(ctx->pool, MLP_PARAMS), m2(ctx->pool, MLP_PARAMS);
+        Tmp<uint32_t> perm(ctx->pool, (size_t)n);
+        FCC_HIP(hipMemcpyAsync(f.get(), features, 4ull * n * MLP_IN, hipMemcpyHostToDevice, s));
+        FCC_HIP(hipMemcpyAsync(l.get(), labels, 4ull * n * MLP_OUT, hipMemcpyHostToDevice, s));
+        std::vector<float> zeros(MLP_PARAMS, 0.f);
+        FCC_HIP(hipMemcpyAsync(m1.get(), zeros.data(), 4ull * MLP_PARAMS, hipMemcpyHostToDevice, s));
+        FCC_HIP(hipMemcpyAsync(m2.get(), zeros.data(), 4ull * MLP_PARAMS, hipMemcpyHostToDevice, s));
+        std::vector<uint32_t> ph((size_t)n);
+        for (int64_t i = 0; i < n; i++) ph[i] = (uint32_t)i;
+        int step = 0;
+        float last = 0.f;
+        for (int e = 0; e < epochs; e++) {
+            std::shuffle(ph.begin(), ph.end(), rng);
+            FCC_HIP(hipMemcpyAsync(perm.get(), ph.data(), 4ull * n, hipMemcpyHostToDevice, s));
+            for (int64_t b0 = 0; b0 < n; b0 += batch) {
+                uint32_t bn = (uint32_t)std::min<int64_t>(batch, n - b0);
+                mlp_fwd_bwd(ctx->mlp_dev, f, l, perm.get() + b0, bn, g, ls, nullptr, s);
+                mlp_adam(ctx->mlp_dev, g, m1, m2, lr, ++step, s);
+            }
+        }
+        // final full-set loss
+        mlp_fwd_bwd(ctx->mlp_dev, f, l, nullptr, (uint32_t)n, g, ls, nullptr, s);
+        FCC_HIP(hipMemcpyAsync(&last, ls.get(), 4, hipMemcpyDeviceToHost, s));
+        FCC_HIP(hipStreamSynchronize(s));
+        if (loss_out) *loss_out = last;
+        ctx->mlp_host.resize(MLP_PARAMS);
+        FCC_HIP(hipMemcpy(ctx->mlp_host.data(), ctx->mlp_dev, 4ull * MLP_PARAMS, hipMemcpyDeviceToHost));
+        ctx->mlp_ready = true;
+    } catch (HipError &e) {
+        (void)hipGetLastError();
+        return fail(ctx, FCC_ERR_HIP, e.where);
+    } catch (LibError &e) {
+        return fail(ctx, e.code, e.msg);
+    }
+    return FCC_OK;
+}
+
+int fcc_contract_batch(fcc_ctx *ctx, int32_t n, const fcc_tensor *a, const int32_t *a_modes, const fcc_tensor *b,
+                       const int32_t *b_modes, int32_t n_contr, const fcc_options *options, fcc_result *outs) {
+    if (!ctx || n < 0 || (n > 0 && (!a || !b || !outs))) return FCC_ERR_INVALID;
+    for (int32_t i = 0; i < n; i++) {
+        int rc = fcc_contract(ctx, &a[i], a_modes, &b[i], b_modes, n_contr, options, &outs[i]);
+        if (rc) {
+            for (int32_t k = 0; k < i; k++) fcc_result_free(ctx, &outs[k]);
+            return rc;
+        }
+    }
+    return FCC_OK;
+}
+
+}  // extern "C"

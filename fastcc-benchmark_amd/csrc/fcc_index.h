@@ -1,0 +1,28 @@
+// Index-build entry points (fcc_index.hip).
+#pragma once
+#include "fcc_internal.h"
+
+namespace fcc {
+
+void fill_u32(uint32_t *p, size_t n, uint32_t v, hipStream_t s);
+void fill_u64(unsigned long long *p, size_t n, unsigned long long v, hipStream_t s);
+void fill_f64(double *p, size_t n, double v, hipStream_t s);
+
+// out[m] = max coordinate of mode m (m < order); out[order] |= 1 when a value
+// is zero or tiny. out must be zeroed by the caller.
+void launch_extents(const Operand &t, uint32_t *dev_out, hipStream_t s);
+
+// Right operand: linearise, sort by (key, col), fibers. Synchronises to read nfib.
+void build_right(DevPool &pool, const Geom &g, const Operand &B, Index &ix,
+                 std::vector<std::shared_ptr<void>> &hold, hipStream_t s);
+
+// Left operand join: row and matching fiber per entry; total products and
+// matched entry count (synchronises).
+void join_left(DevPool &pool, const Geom &g, const Operand &A, const Index &ix, uint32_t *rowA, uint32_t *fidA,
+               uint64_t *products, uint32_t *nvalid, hipStream_t s);
+
+// Left operand tiles of 2^logTL rows: sort, groups, active tiles, statistics.
+void build_left(DevPool &pool, const Geom &g, const Operand &A, const uint32_t *rowA, const uint32_t *fidA,
+                uint32_t nvalid, int logTL, Index &ix, std::vector<std::shared_ptr<void>> &hold, hipStream_t s);
+
+}  // namespace fcc

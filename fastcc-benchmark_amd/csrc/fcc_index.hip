@@ -1,0 +1,420 @@
+// Index build: extents, linearisation, right-operand CSR keyed by contraction
+// key, left-operand tiles over the external index, per-tile statistics.
+//
+// Replaces TileIndexedTensor (index_tensors.hpp:491-547) and the reference's
+// linearisers CoOrdinate::gather_linearize / remove_linearize
+// (coordinate.hpp:723-733, 767-781). Instead of one hash map per tile, both
+// operands are sorted on composite keys:
+//   B: (key, col)                     -> fibers, CSR by contraction key
+//   A: (tile, class, fiber, row%TL)   -> per tile: heavy groups, then light entries
+#include "fcc_index.h"
+
+namespace fcc {
+
+__global__ void k_fill_u32(uint32_t *p, size_t n, uint32_t v) {
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) p[i] = v;
+}
+__global__ void k_fill_u64(unsigned long long *p, size_t n, unsigned long long v) {
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) p[i] = v;
+}
+__global__ void k_fill_f64(double *p, size_t n, double v) {
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) p[i] = v;
+}
+
+static unsigned fill_grid(size_t n) {
+    size_t b = div_up(n ? n : 1, 256);
+    return (unsigned)(b > 8192 ? 8192 : b);
+}
+void fill_u32(uint32_t *p, size_t n, uint32_t v, hipStream_t s) {
+    if (n) k_fill_u32<<<fill_grid(n), 256, 0, s>>>(p, n, v);
+}
+void fill_u64(unsigned long long *p, size_t n, unsigned long long v, hipStream_t s) {
+    if (n) k_fill_u64<<<fill_grid(n), 256, 0, s>>>(p, n, v);
+}
+void fill_f64(double *p, size_t n, double v, hipStream_t s) {
+    if (n) k_fill_f64<<<fill_grid(n), 256, 0, s>>>(p, n, v);
+}
+
+// ---------------------------------------------------------------- extents
+// out[m] = max coordinate of mode m (the reference's shape, contract.hpp:124-144);
+// out[order] |= 1 if some value is zero, subnormal, or smaller than 2^-511 in
+// magnitude (then a product can round to zero and the accumulators must track
+// touched slots explicitly).
+__global__ void k_extents(const uint32_t *coords, const double *vals, int order, size_t nnz, uint32_t *out) {
+    const int m = blockIdx.y;
+    uint32_t mx = 0, flag = 0;
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < nnz; i += (size_t)gridDim.x * blockDim.x) {
+        if (m < order) {
+            uint32_t c = coords[(size_t)m * nnz + i];
+            mx = c > mx ? c : mx;
+        } else {
+            double v = vals[i];
+            double a = fabs(v);
+            if (!(a >= 0x1.0p-511) && !isnan(v)) flag = 1;
+        }
+    }
+    for (int d = 32; d > 0; d >>= 1) {
+        uint32_t o = __shfl_xor(mx, d, 64);
+        mx = o > mx ? o : mx;
+        flag |= __shfl_xor(flag, d, 64);
+    }
+    if ((threadIdx.x & 63) == 0) {
+        if (m < order) atomicMax(&out[m], mx);
+        else if (flag) atomicOr(&out[order], 1u);
+    }
+}
+
+void launch_extents(const Operand &t, uint32_t *dev_out, hipStream_t s) {
+    if (t.nnz == 0) return;
+    unsigned gx = (unsigned)std::min<size_t>(div_up((size_t)t.nnz, 256), 1024);
+    dim3 grid(gx, t.order + 1);
+    k_extents<<<grid, 256, 0, s>>>(t.coords, t.vals, t.order, (size_t)t.nnz, dev_out);
+}
+
+// -------------------------------------------------------------- linearise
+__device__ inline uint64_t lin_key(const uint32_t *coords, size_t nnz, size_t n, const int *modes, const uint64_t *R,
+                                   int r) {
+    uint64_t k = 0;
+    for (int i = 0; i < r; i++) k = k * R[i] + coords[(size_t)modes[i] * nnz + n];
+    return k;
+}
+__device__ inline uint32_t lin_ext(const uint32_t *coords, size_t nnz, size_t n, const int *xm, const uint32_t *xE,
+                                   int nx) {
+    uint32_t e = 0;
+    for (int i = 0; i < nx; i++) e = e * xE[i] + coords[(size_t)xm[i] * nnz + n];
+    return e;
+}
+
+// B: comp = key << cbits | col  (coordinate.hpp:723-733 and 767-781)
+__global__ void k_lin_B(const uint32_t *coords, size_t nnz, Geom g, uint64_t *comp, uint32_t *idx) {
+    size_t n = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+    if (n >= nnz) return;
+    uint64_t key = lin_key(coords, nnz, n, g.bm, g.R, g.r);
+    uint32_t col = lin_ext(coords, nnz, n, g.xb, g.xbE, g.nxb);
+    comp[n] = (key << g.cbits) | col;
+    idx[n] = (uint32_t)n;
+}
+
+__global__ void k_gather_B(const uint64_t *comp, const uint32_t *idx, const double *vals, size_t n, int cbits,
+                           uint32_t *Bcol, double *Bval, uint32_t *flag) {
+    size_t p = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    uint64_t c = comp[p];
+    Bcol[p] = (uint32_t)(c & ((1ull << cbits) - 1));
+    Bval[p] = vals[idx[p]];
+    flag[p] = (p == 0 || (c >> cbits) != (comp[p - 1] >> cbits)) ? 1u : 0u;
+}
+
+__global__ void k_fibers(const uint64_t *comp, const uint32_t *flag, const uint32_t *fid, size_t n, int cbits,
+                         uint32_t *fiber_ptr, uint64_t *fiber_key) {
+    size_t p = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    if (flag[p]) {
+        fiber_ptr[fid[p]] = (uint32_t)p;
+        fiber_key[fid[p]] = comp[p] >> cbits;
+    }
+    if (p == n - 1) fiber_ptr[fid[p] + flag[p]] = (uint32_t)n;
+}
+
+__global__ void k_keymap(const uint64_t *fiber_key, uint32_t nfib, uint32_t *keymap) {
+    uint32_t f = blockIdx.x * blockDim.x + threadIdx.x;
+    if (f < nfib) keymap[fiber_key[f]] = f;
+}
+
+__device__ inline uint32_t find_fiber(const uint64_t *fk, uint32_t nfib, uint64_t key) {
+    uint32_t lo = 0, hi = nfib;
+    while (lo < hi) {
+        uint32_t mid = (lo + hi) >> 1;
+        if (fk[mid] < key) lo = mid + 1; else hi = mid;
+    }
+    return (lo < nfib && fk[lo] == key) ? lo : 0xFFFFFFFFu;
+}
+
+// A join: row, matching fiber (or none), global product count.
+__global__ void k_join_A(const uint32_t *coords, size_t nnz, Geom g, const uint32_t *keymap,
+                         const uint64_t *fiber_key, uint32_t nfib, const uint32_t *fiber_ptr, uint32_t *rowA,
+                         uint32_t *fidA, unsigned long long *products, unsigned int *nvalid) {
+    size_t n = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+    uint64_t prod = 0;
+    uint32_t valid = 0;
+    if (n < nnz) {
+        uint64_t key = lin_key(coords, nnz, n, g.am, g.R, g.r);
+        uint32_t f = keymap ? keymap[key] : find_fiber(fiber_key, nfib, key);
+        rowA[n] = lin_ext(coords, nnz, n, g.xa, g.xaE, g.nxa);
+        fidA[n] = f;
+        if (f != 0xFFFFFFFFu) {
+            prod = fiber_ptr[f + 1] - fiber_ptr[f];
+            valid = 1;
+        }
+    }
+    for (int d = 32; d > 0; d >>= 1) {
+        prod += __shfl_xor(prod, d, 64);
+        valid += __shfl_xor(valid, d, 64);
+    }
+    if ((threadIdx.x & 63) == 0) {
+        if (prod) atomicAdd(products, (unsigned long long)prod);
+        if (valid) atomicAdd(nvalid, valid);
+    }
+}
+
+// A composite: tile | class | fiber | row-in-tile; unmatched entries sort last.
+__global__ void k_comp_A(const uint32_t *rowA, const uint32_t *fidA, const uint32_t *fiber_ptr, size_t nnz,
+                         int logTL, int fbits, uint64_t *comp, uint32_t *idx) {
+    size_t n = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+    if (n >= nnz) return;
+    uint32_t f = fidA[n];
+    uint64_t c = ~0ull;
+    if (f != 0xFFFFFFFFu) {
+        uint32_t row = rowA[n];
+        uint64_t tile = row >> logTL;
+        uint64_t li = row & ((1u << logTL) - 1);
+        uint64_t cls = (fiber_ptr[f + 1] - fiber_ptr[f]) >= HEAVY_MIN ? 0 : 1;
+        c = (tile << (1 + fbits + logTL)) | (cls << (fbits + logTL)) | ((uint64_t)f << logTL) | li;
+    }
+    comp[n] = c;
+    idx[n] = (uint32_t)n;
+}
+
+__global__ void k_gather_A(const uint64_t *comp, const uint32_t *idx, const double *vals, size_t n, int logTL,
+                           int fbits, uint32_t *Ali, double *Aval, uint32_t *Afib, uint32_t *gflag, uint32_t *tflag) {
+    size_t p = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    uint64_t c = comp[p];
+    Ali[p] = (uint32_t)(c & ((1ull << logTL) - 1));
+    Afib[p] = (uint32_t)((c >> logTL) & ((1ull << fbits) - 1));
+    Aval[p] = vals[idx[p]];
+    uint64_t prev = p ? comp[p - 1] : ~0ull;
+    gflag[p] = (p == 0 || (c >> logTL) != (prev >> logTL)) ? 1u : 0u;
+    tflag[p] = (p == 0 || (c >> (1 + fbits + logTL)) != (prev >> (1 + fbits + logTL))) ? 1u : 0u;
+}
+
+__global__ void k_groups(const uint64_t *comp, const uint32_t *gflag, const uint32_t *gid, const uint32_t *tflag,
+                         const uint32_t *aid, size_t n, int logTL, int fbits, uint32_t *group_ptr,
+                         uint32_t *group_fiber, uint32_t *group_atile, uint8_t *group_light, uint32_t *atile_id,
+                         uint32_t *atile_g0) {
+    size_t p = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    uint64_t c = comp[p];
+    uint32_t a = aid[p] + tflag[p] - 1;
+    if (gflag[p]) {
+        uint32_t g = gid[p];
+        group_ptr[g] = (uint32_t)p;
+        group_fiber[g] = (uint32_t)((c >> logTL) & ((1ull << fbits) - 1));
+        group_atile[g] = a;
+        group_light[g] = (uint8_t)((c >> (fbits + logTL)) & 1);
+    }
+    if (tflag[p]) {
+        atile_id[a] = (uint32_t)(c >> (1 + fbits + logTL));
+        atile_g0[a] = gid[p];
+    }
+    if (p == n - 1) {
+        group_ptr[gid[p] + gflag[p]] = (uint32_t)n;
+        atile_g0[a + 1] = gid[p] + gflag[p];
+    }
+}
+
+__global__ void k_tile_light_init(const uint32_t *atile_g0, uint32_t natiles, uint32_t *atile_lg) {
+    uint32_t a = blockIdx.x * blockDim.x + threadIdx.x;
+    if (a < natiles) atile_lg[a] = atile_g0[a + 1];
+}
+
+__global__ void k_tile_light(const uint8_t *group_light, const uint32_t *group_atile, const uint32_t *atile_g0,
+                             uint32_t ngroups, uint32_t *atile_lg) {
+    uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= ngroups || !group_light[g]) return;
+    uint32_t a = group_atile[g];
+    if (g == atile_g0[a] || !group_light[g - 1]) atile_lg[a] = g;
+}
+
+__device__ inline uint32_t hash32(uint32_t x) {
+    x ^= x >> 16;
+    x *= 0x7feb352dU;
+    x ^= x >> 15;
+    x *= 0x846ca68bU;
+    x ^= x >> 16;
+    return x;
+}
+
+// Per active tile statistics, one thread per group, wave-aggregated atomics.
+//  nnz      A entries of the tile          (num_nnz_in_tile, index_tensors.hpp:562)
+//  keys     distinct contraction keys      (num_active_columns, index_tensors.hpp:604)
+//  sumfib   summed right-fiber lengths
+//  products sum of m * |fiber| = an upper bound on the tile's output count
+//  sketch   two min-hashes over sampled fiber columns (distinct-column estimate)
+__global__ void k_tile_stats(const uint32_t *group_ptr, const uint32_t *group_fiber, const uint32_t *group_atile,
+                             const uint32_t *fiber_ptr, const uint32_t *Bcol, uint32_t ngroups, uint32_t *st_nnz,
+                             uint32_t *st_keys, unsigned long long *st_sumfib, unsigned long long *st_products,
+                             uint32_t *st_sketch) {
+    uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+    bool ok = g < ngroups;
+    uint32_t a = ok ? group_atile[g] : 0xFFFFFFFFu;
+    uint64_t m = 0, nf = 0;
+    uint32_t h1 = 0xFFFFFFFFu, h2 = 0xFFFFFFFFu;
+    if (ok) {
+        m = group_ptr[g + 1] - group_ptr[g];
+        uint32_t f = group_fiber[g];
+        uint32_t f0 = fiber_ptr[f], f1 = fiber_ptr[f + 1];
+        nf = f1 - f0;
+        // sample up to 4 evenly spaced columns of the fiber
+        uint32_t step = (uint32_t)((nf + 3) / 4);
+        for (uint32_t p = f0; p < f1; p += step) {
+            uint32_t c = Bcol[p];
+            uint32_t x1 = hash32(c * 2u + 1u), x2 = hash32(c ^ 0x5bd1e995u);
+            h1 = x1 < h1 ? x1 : h1;
+            h2 = x2 < h2 ? x2 : h2;
+        }
+    }
+    uint32_t a0 = __shfl(a, 0, 64);
+    bool uniform = __all(a == a0);
+    if (uniform && a0 != 0xFFFFFFFFu) {
+        uint64_t sm = m, snf = nf, sp = m * nf;
+        uint32_t sk = ok ? 1 : 0;
+        for (int d = 32; d > 0; d >>= 1) {
+            sm += __shfl_xor(sm, d, 64);
+            snf += __shfl_xor(snf, d, 64);
+            sp += __shfl_xor(sp, d, 64);
+            sk += __shfl_xor(sk, d, 64);
+            uint32_t o1 = __shfl_xor(h1, d, 64), o2 = __shfl_xor(h2, d, 64);
+            h1 = o1 < h1 ? o1 : h1;
+            h2 = o2 < h2 ? o2 : h2;
+        }
+        if ((threadIdx.x & 63) == 0) {
+            atomicAdd(&st_nnz[a0], (uint32_t)sm);
+            atomicAdd(&st_keys[a0], sk);
+            atomicAdd(&st_sumfib[a0], (unsigned long long)snf);
+            atomicAdd(&st_products[a0], (unsigned long long)sp);
+            atomicMin(&st_sketch[2 * a0], h1);
+            atomicMin(&st_sketch[2 * a0 + 1], h2);
+        }
+    } else if (ok) {
+        atomicAdd(&st_nnz[a], (uint32_t)m);
+        atomicAdd(&st_keys[a], 1u);
+        atomicAdd(&st_sumfib[a], (unsigned long long)nf);
+        atomicAdd(&st_products[a], (unsigned long long)(m * nf));
+        atomicMin(&st_sketch[2 * a], h1);
+        atomicMin(&st_sketch[2 * a + 1], h2);
+    }
+}
+
+// ---------------------------------------------------------------- host side
+
+static unsigned grid_for(size_t n, unsigned bs = 256) { return (unsigned)div_up(n ? n : 1, bs); }
+
+void build_right(DevPool &pool, const Geom &g, const Operand &B, Index &ix, std::vector<std::shared_ptr<void>> &hold,
+                 hipStream_t s) {
+    size_t n = (size_t)B.nnz;
+    ix.nB = (uint32_t)n;
+    auto keep = [&](size_t bytes) -> void * {
+        void *p = pool.get(bytes + 16);
+        DevPool *pl = &pool;
+        hold.emplace_back(p, [pl](void *q) { pl->put(q); });
+        return p;
+    };
+    Tmp<uint64_t> k1(pool, n), k2(pool, n);
+    Tmp<uint32_t> v1(pool, n), v2(pool, n);
+    k_lin_B<<<grid_for(n), 256, 0, s>>>(B.coords, n, g, k1, v1);
+    uint64_t *ks;
+    uint32_t *vs;
+    radix_sort_pairs(pool, k1, v1, k2, v2, n, g.kbits + g.cbits, s, &ks, &vs);
+    ix.Bcol = (uint32_t *)keep(n * 4);
+    ix.Bval = (double *)keep(n * 8);
+    Tmp<uint32_t> flag(pool, n), fid(pool, n);
+    k_gather_B<<<grid_for(n), 256, 0, s>>>(ks, vs, B.vals, n, g.cbits, ix.Bcol, ix.Bval, flag);
+    Tmp<uint32_t> total(pool, 1);
+    scan_u32(pool, flag, fid, n, total, s);
+    ix.fiber_ptr = (uint32_t *)keep((n + 1) * 4);
+    ix.fiber_key = (uint64_t *)keep(n * 8);
+    k_fibers<<<grid_for(n), 256, 0, s>>>(ks, flag, fid, n, g.cbits, ix.fiber_ptr, ix.fiber_key);
+    FCC_HIP(hipMemcpyAsync(&ix.nfib, total.get(), 4, hipMemcpyDeviceToHost, s));
+    FCC_HIP(hipStreamSynchronize(s));
+}
+
+void join_left(DevPool &pool, const Geom &g, const Operand &A, const Index &ix, uint32_t *rowA, uint32_t *fidA,
+               uint64_t *products, uint32_t *nvalid, hipStream_t s) {
+    size_t n = (size_t)A.nnz;
+    Tmp<uint32_t> keymap;
+    if (g.K <= (1ull << 26)) {
+        keymap = Tmp<uint32_t>(pool, g.K);
+        fill_u32(keymap, g.K, 0xFFFFFFFFu, s);
+        if (ix.nfib) k_keymap<<<grid_for(ix.nfib), 256, 0, s>>>(ix.fiber_key, ix.nfib, keymap);
+    }
+    Tmp<unsigned long long> cnt(pool, 2);
+    fill_u64(cnt, 2, 0, s);
+    k_join_A<<<grid_for(n), 256, 0, s>>>(A.coords, n, g, keymap.get(), ix.fiber_key, ix.nfib, ix.fiber_ptr, rowA,
+                                          fidA, cnt.get(), (unsigned int *)(cnt.get() + 1));
+    unsigned long long h[2];
+    FCC_HIP(hipMemcpyAsync(h, cnt.get(), 16, hipMemcpyDeviceToHost, s));
+    FCC_HIP(hipStreamSynchronize(s));
+    *products = h[0];
+    *nvalid = (uint32_t)(h[1] & 0xFFFFFFFFu);
+}
+
+void build_left(DevPool &pool, const Geom &g, const Operand &A, const uint32_t *rowA, const uint32_t *fidA,
+                uint32_t nvalid, int logTL, Index &ix, std::vector<std::shared_ptr<void>> &hold, hipStream_t s) {
+    size_t n = (size_t)A.nnz;
+    auto keep = [&](size_t bytes) -> void * {
+        void *p = pool.get(bytes + 16);
+        DevPool *pl = &pool;
+        hold.emplace_back(p, [pl](void *q) { pl->put(q); });
+        return p;
+    };
+    ix.logTL = logTL;
+    ix.TL = 1u << logTL;
+    ix.nAv = nvalid;
+    int fbits = std::max(1, ceil_log2((uint64_t)ix.nfib + 1));
+    int tbits = std::max(0, g.rbits - logTL);
+    int nbits = tbits + 1 + fbits + logTL + 1;
+    if (nbits > 64) throw LibError{ERR_INVALID, "left composite key exceeds 64 bits"};
+    Tmp<uint64_t> k1(pool, n), k2(pool, n);
+    Tmp<uint32_t> v1(pool, n), v2(pool, n);
+    k_comp_A<<<grid_for(n), 256, 0, s>>>(rowA, fidA, ix.fiber_ptr, n, logTL, fbits, k1, v1);
+    uint64_t *ks;
+    uint32_t *vs;
+    radix_sort_pairs(pool, k1, v1, k2, v2, n, std::min(nbits, 64), s, &ks, &vs);
+    size_t m = nvalid;
+    ix.Ali = (uint32_t *)keep(m * 4);
+    ix.Aval = (double *)keep(m * 8);
+    ix.Afib = (uint32_t *)keep(m * 4);
+    Tmp<uint32_t> gflag(pool, m), tflag(pool, m), gid(pool, m), aid(pool, m), tot(pool, 2);
+    if (m) k_gather_A<<<grid_for(m), 256, 0, s>>>(ks, vs, A.vals, m, logTL, fbits, ix.Ali, ix.Aval, ix.Afib, gflag, tflag);
+    scan_u32(pool, gflag, gid, m, tot.get(), s);
+    scan_u32(pool, tflag, aid, m, tot.get() + 1, s);
+    uint32_t h[2];
+    FCC_HIP(hipMemcpyAsync(h, tot.get(), 8, hipMemcpyDeviceToHost, s));
+    FCC_HIP(hipStreamSynchronize(s));
+    ix.ngroups = h[0];
+    ix.natiles = h[1];
+    ix.group_ptr = (uint32_t *)keep((size_t)(ix.ngroups + 1) * 4);
+    ix.group_fiber = (uint32_t *)keep((size_t)ix.ngroups * 4);
+    ix.group_atile = (uint32_t *)keep((size_t)ix.ngroups * 4);
+    Tmp<uint8_t> glight(pool, ix.ngroups + 1);
+    ix.atile_id = (uint32_t *)keep((size_t)ix.natiles * 4);
+    ix.atile_g0 = (uint32_t *)keep((size_t)(ix.natiles + 1) * 4);
+    ix.atile_lg = (uint32_t *)keep((size_t)ix.natiles * 4);
+    if (m) {
+        k_groups<<<grid_for(m), 256, 0, s>>>(ks, gflag, gid, tflag, aid, m, logTL, fbits, ix.group_ptr, ix.group_fiber,
+                                             ix.group_atile, glight, ix.atile_id, ix.atile_g0);
+        k_tile_light_init<<<grid_for(ix.natiles), 256, 0, s>>>(ix.atile_g0, ix.natiles, ix.atile_lg);
+        k_tile_light<<<grid_for(ix.ngroups), 256, 0, s>>>(glight, ix.group_atile, ix.atile_g0, ix.ngroups, ix.atile_lg);
+    }
+    // statistics
+    size_t T = ix.natiles;
+    ix.st_nnz = (uint32_t *)keep(T * 4);
+    ix.st_keys = (uint32_t *)keep(T * 4);
+    ix.st_sumfib = (uint64_t *)keep(T * 8);
+    ix.st_products = (uint64_t *)keep(T * 8);
+    ix.st_sketch = (uint32_t *)keep(T * 8);
+    if (T) {
+        fill_u32(ix.st_nnz, T, 0, s);
+        fill_u32(ix.st_keys, T, 0, s);
+        fill_u64((unsigned long long *)ix.st_sumfib, T, 0, s);
+        fill_u64((unsigned long long *)ix.st_products, T, 0, s);
+        fill_u32(ix.st_sketch, 2 * T, 0xFFFFFFFFu, s);
+        k_tile_stats<<<grid_for(ix.ngroups), 256, 0, s>>>(ix.group_ptr, ix.group_fiber, ix.group_atile, ix.fiber_ptr,
+                                                          ix.Bcol, ix.ngroups, ix.st_nnz, ix.st_keys,
+                                                          (unsigned long long *)ix.st_sumfib,
+                                                          (unsigned long long *)ix.st_products, ix.st_sketch);
+    }
+}
+
+}  // namespace fcc

@@ -1,0 +1,212 @@
+// Internal declarations shared by the HIP translation units of libfastcc_hip.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+#include <map>
+#include <memory>
+#include <string>
+#include <vector>
+
+namespace fcc {
+
+constexpr int kMaxOrder = 8;
+
+// Contraction geometry, computed on the host from the inferred extents and
+// passed to kernels by value.
+struct Geom {
+    int dA, dB, r, dC;
+    int am[kMaxOrder], bm[kMaxOrder];      // paired contraction modes
+    int nxa, nxb;
+    int xa[kMaxOrder], xb[kMaxOrder];      // external modes, ascending
+    uint32_t xaE[kMaxOrder], xbE[kMaxOrder];
+    uint64_t R[kMaxOrder];                 // contraction-key radix per pair
+    uint64_t spanA, spanB, K;              // external spans (< 2^32), key span
+    int rbits, cbits, kbits;               // bit widths of row, col, key
+};
+
+// Error codes mirrored from fcc.h
+enum : int { OK = 0, ERR_INVALID = -1, ERR_OOM = -2, ERR_OVERFLOW = -3, ERR_HIP = -4 };
+
+// Caching device allocator: blocks are kept across calls so a steady-state
+// contraction performs no hipMalloc.
+class DevPool {
+  public:
+    ~DevPool();
+    void *get(size_t bytes);
+    void put(void *p);
+    void release_all();
+    size_t bytes_cached() const { return cached_; }
+
+  private:
+    std::multimap<size_t, void *> free_;
+    std::map<void *, size_t> used_;
+    size_t cached_ = 0;
+};
+
+// RAII temporary from the pool.
+template <typename T> struct Tmp {
+    DevPool *pool = nullptr;
+    T *p = nullptr;
+    Tmp() = default;
+    Tmp(DevPool &pl, size_t n) : pool(&pl) { p = (T *)pl.get(n * sizeof(T) + 16); }
+    Tmp(const Tmp &) = delete;
+    Tmp &operator=(const Tmp &) = delete;
+    Tmp(Tmp &&o) noexcept : pool(o.pool), p(o.p) { o.p = nullptr; }
+    Tmp &operator=(Tmp &&o) noexcept {
+        reset();
+        pool = o.pool; p = o.p; o.p = nullptr;
+        return *this;
+    }
+    ~Tmp() { reset(); }
+    void reset() { if (p && pool) pool->put(p); p = nullptr; }
+    T *get() const { return p; }
+    operator T *() const { return p; }
+};
+
+struct HipError {
+    hipError_t code;
+    std::string where;
+};
+struct LibError {
+    int code;
+    std::string msg;
+};
+
+#define FCC_HIP(x)                                                       \
+    do {                                                                 \
+        hipError_t e_ = (x);                                             \
+        if (e_ != hipSuccess) throw ::fcc::HipError{e_, #x};             \
+    } while (0)
+
+inline int ceil_log2(uint64_t x) {
+    int b = 0;
+    while (b < 64 && (1ULL << b) < x) b++;
+    return b;
+}
+inline uint64_t pow2_ceil(uint64_t x) { return x <= 1 ? 1 : 1ULL << ceil_log2(x); }
+inline uint64_t div_up(uint64_t a, uint64_t b) { return (a + b - 1) / b; }
+
+// ---------------------------------------------------------------- scan / sort
+// Exclusive scan of n elements (in may equal out). *total_dev (optional)
+// receives the sum on the device.
+void scan_u32(DevPool &pool, const uint32_t *in, uint32_t *out, size_t n, uint32_t *total_dev,
+              hipStream_t s);
+void scan_u64(DevPool &pool, const uint64_t *in, uint64_t *out, size_t n, uint64_t *total_dev,
+              hipStream_t s);
+
+// Stable LSD radix sort of (key, value) pairs on key bits [0, nbits).
+// Returns pointers to the sorted arrays (either the inputs or the alternates).
+void radix_sort_pairs(DevPool &pool, uint64_t *keys, uint32_t *vals, uint64_t *keys_alt,
+                      uint32_t *vals_alt, size_t n, int nbits, hipStream_t s,
+                      uint64_t **keys_out, uint32_t **vals_out);
+
+// ------------------------------------------------------------------- index
+struct Operand {
+    int order;
+    int64_t nnz;
+    const uint32_t *coords;  // device SoA
+    const double *vals;      // device
+};
+
+// Accumulator kinds (match FCC_ACC_*).
+enum : uint8_t { KIND_DENSE = 1, KIND_LHASH = 2, KIND_GHASH = 3 };
+
+constexpr uint32_t HEAVY_MIN = 32;      // fibers at least this long are processed wave-per-group
+constexpr uint32_t DENSE_SLOTS = 9984;  // f64 LDS accumulator slots (78 KB: two workgroups per CU)
+constexpr uint32_t LH_SLOTS = 4096;     // LDS hash slots (f64 value + u32 key)
+constexpr uint32_t LH_MAX = 3072;       // LDS hash distinct-key limit before sealing
+constexpr uint32_t SYM_SLOTS = 16384;   // LDS hash-set slots of the count pass (u32 keys)
+constexpr uint32_t SYM_MAX = 12288;
+constexpr uint32_t HASH_TARGET = 2048;  // planned distinct outputs per hash unit
+constexpr int ACC_THREADS = 512;
+
+// Device-side index of one contraction: B's CSR keyed by contraction key
+// ("fibers") and A's entries grouped by (left tile, fiber).
+struct Index {
+    // B: nB entries sorted by (key, col); fiber f = [fiber_ptr[f], fiber_ptr[f+1])
+    uint32_t nB = 0, nfib = 0;
+    uint32_t *Bcol = nullptr;
+    double *Bval = nullptr;
+    uint32_t *fiber_ptr = nullptr;
+    uint64_t *fiber_key = nullptr;
+    // A: nAv entries with a matching fiber, sorted by (tile, class, fiber, row-in-tile)
+    uint32_t nAv = 0, ngroups = 0, natiles = 0;
+    uint32_t *Ali = nullptr;   // row within tile
+    double *Aval = nullptr;
+    uint32_t *Afib = nullptr;  // fiber id per entry
+    uint32_t *group_ptr = nullptr;    // ngroups+1
+    uint32_t *group_fiber = nullptr;
+    uint32_t *group_atile = nullptr;
+    uint32_t *atile_id = nullptr;     // tile index of active tile a
+    uint32_t *atile_g0 = nullptr;     // natiles+1: first group of active tile a
+    uint32_t *atile_lg = nullptr;     // first light group of active tile a
+    // per active tile statistics (planner features)
+    uint64_t *st_products = nullptr;  // sum over groups of m * |fiber|
+    uint64_t *st_sumfib = nullptr;    // sum over groups of |fiber|
+    uint32_t *st_nnz = nullptr;       // A entries in tile
+    uint32_t *st_keys = nullptr;      // groups (distinct contraction keys) in tile
+    uint32_t *st_rows = nullptr;      // distinct rows in tile
+    uint32_t *st_sketch = nullptr;    // distinct-column sketch (KMV estimate)
+    int logTL = 0;
+    uint32_t TL = 1;
+    uint64_t products = 0;
+};
+
+struct Units {
+    uint32_t n = 0;
+    uint32_t *tile = nullptr;  // active tile index
+    uint32_t *c0 = nullptr;    // first column of the window
+    uint32_t *W = nullptr;     // window width
+    uint8_t *kind = nullptr;
+    uint8_t *multi = nullptr;  // 1 when the tile has more than one window
+    float *pred = nullptr;     // predicted outputs (learned planner)
+};
+
+// Everything an accumulate kernel needs, passed by value.
+struct AccArgs {
+    const uint32_t *Bcol;
+    const double *Bval;
+    const uint32_t *fiber_ptr;
+    const uint32_t *Ali;
+    const double *Aval;
+    const uint32_t *Afib;
+    const uint32_t *group_ptr;
+    const uint32_t *group_fiber;
+    const uint32_t *atile_id;
+    const uint32_t *atile_g0;
+    const uint32_t *atile_lg;
+    uint32_t nunits;
+    const uint32_t *unit_tile, *unit_c0, *unit_W;
+    const uint8_t *unit_kind, *unit_multi;
+    const uint32_t *unit_order;  // processing order (heaviest first) or null
+    int logTL;
+    uint64_t spanB;
+    int nxa, nxb;
+    uint32_t xaE[kMaxOrder], xbE[kMaxOrder];
+    // shared global hash (overflow / GHASH units)
+    unsigned long long *gkeys;
+    double *gvals;
+    uint32_t *gunit;
+    uint64_t gmask;
+    unsigned long long *gcount;
+    uint32_t *gflag;
+    // count pass
+    unsigned long long *unit_count;
+    // placement
+    const unsigned long long *unit_off;  // exact planner
+    unsigned long long *unit_cursor;     // exact planner, GHASH entries
+    unsigned long long *cursor;          // learned planner
+    uint64_t capacity;
+    uint32_t *fail_flag;
+    uint32_t *ocoords;
+    double *ovals;
+    uint64_t ostride;
+    // sink
+    unsigned long long *sink;  // [0]=count [1]=hash
+    double *sinkf;             // [0]=val_sum [1]=val_wsum
+    unsigned int *work;        // persistent work counter
+};
+
+}  // namespace fcc

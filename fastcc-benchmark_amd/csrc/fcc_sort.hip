@@ -1,0 +1,251 @@
+// Device-wide exclusive scan and stable LSD radix sort of (u64 key, u32 value)
+// pairs, written for CDNA (64-lane wavefronts) -- plus the caching allocator.
+//
+// These replace the reference's hash-map tile index build
+// (index_tensors.hpp:506-547, ankerl::unordered_dense maps): grouping by
+// sorting the linearised composite keys instead of hashing them.
+#include "fcc_internal.h"
+
+namespace fcc {
+
+// ------------------------------------------------------------ caching pool
+
+DevPool::~DevPool() { release_all(); }
+
+void *DevPool::get(size_t bytes) {
+    if (bytes == 0) bytes = 16;
+    bytes = (bytes + 255) & ~size_t(255);
+    auto it = free_.lower_bound(bytes);
+    if (it != free_.end() && it->first <= bytes * 2 + (1u << 20)) {
+        void *p = it->second;
+        size_t sz = it->first;
+        free_.erase(it);
+        used_[p] = sz;
+        return p;
+    }
+    void *p = nullptr;
+    hipError_t e = hipMalloc(&p, bytes);
+    if (e != hipSuccess) {
+        // give cached blocks back and retry once
+        (void)hipGetLastError();
+        for (auto &kv : free_) {
+            (void)hipFree(kv.second);
+            cached_ -= kv.first;
+        }
+        free_.clear();
+        e = hipMalloc(&p, bytes);
+        if (e != hipSuccess) {
+            (void)hipGetLastError();
+            throw LibError{ERR_OOM, "device allocation of " + std::to_string(bytes) + " bytes failed"};
+        }
+    }
+    cached_ += bytes;
+    used_[p] = bytes;
+    return p;
+}
+
+void DevPool::put(void *p) {
+    auto it = used_.find(p);
+    if (it == used_.end()) return;
+    free_.emplace(it->second, p);
+    used_.erase(it);
+}
+
+void DevPool::release_all() {
+    for (auto &kv : free_) (void)hipFree(kv.second);
+    for (auto &kv : used_) (void)hipFree(kv.first);
+    free_.clear();
+    used_.clear();
+    cached_ = 0;
+}
+
+// ------------------------------------------------------------------- scan
+
+constexpr int SCAN_THREADS = 256;
+constexpr int SCAN_ITEMS = 8;
+constexpr int SCAN_TILE = SCAN_THREADS * SCAN_ITEMS;
+
+template <typename T> __device__ inline T wave_incl_scan(T v) {
+    const int lane = threadIdx.x & 63;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        T o = __shfl_up(v, d, 64);
+        if (lane >= d) v += o;
+    }
+    return v;
+}
+
+// Exclusive block scan of one value per thread; returns the prefix and writes
+// the block total to *total.
+template <typename T> __device__ inline T block_excl_scan(T v, T *total) {
+    __shared__ T wsum[SCAN_THREADS / 64];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    T inc = wave_incl_scan(v);
+    if (lane == 63) wsum[w] = inc;
+    __syncthreads();
+    T off = 0, tot = 0;
+#pragma unroll
+    for (int i = 0; i < SCAN_THREADS / 64; i++) {
+        if (i < w) off += wsum[i];
+        tot += wsum[i];
+    }
+    __syncthreads();
+    *total = tot;
+    return off + inc - v;
+}
+
+template <typename T> __global__ __launch_bounds__(SCAN_THREADS) void k_block_reduce(const T *in, size_t n, T *sums) {
+    size_t base = (size_t)blockIdx.x * SCAN_TILE;
+    T acc = 0;
+#pragma unroll
+    for (int i = 0; i < SCAN_ITEMS; i++) {
+        size_t k = base + threadIdx.x + (size_t)i * SCAN_THREADS;
+        if (k < n) acc += in[k];
+    }
+    T tot;
+    (void)block_excl_scan(acc, &tot);
+    if (threadIdx.x == 0) sums[blockIdx.x] = tot;
+}
+
+template <typename T>
+__global__ __launch_bounds__(SCAN_THREADS) void k_block_scan(const T *in, T *out, size_t n, const T *offsets,
+                                                             T *total) {
+    size_t base = (size_t)blockIdx.x * SCAN_TILE + (size_t)threadIdx.x * SCAN_ITEMS;
+    T v[SCAN_ITEMS];
+    T s = 0;
+#pragma unroll
+    for (int i = 0; i < SCAN_ITEMS; i++) {
+        T x = (base + i < n) ? in[base + i] : T(0);
+        v[i] = s;
+        s += x;
+    }
+    T tot;
+    T pre = block_excl_scan(s, &tot);
+    T off = (offsets ? offsets[blockIdx.x] : T(0)) + pre;
+#pragma unroll
+    for (int i = 0; i < SCAN_ITEMS; i++)
+        if (base + i < n) out[base + i] = v[i] + off;
+    if (total && blockIdx.x == gridDim.x - 1 && threadIdx.x == 0)
+        *total = (offsets ? offsets[blockIdx.x] : T(0)) + tot;
+}
+
+template <typename T> __global__ void k_set_scalar(T *p, T v) { *p = v; }
+
+template <typename T>
+static void scan_impl(DevPool &pool, const T *in, T *out, size_t n, T *total_dev, hipStream_t s) {
+    if (n == 0) {
+        if (total_dev) k_set_scalar<T><<<1, 1, 0, s>>>(total_dev, T(0));
+        return;
+    }
+    size_t nb = div_up(n, SCAN_TILE);
+    if (nb == 1) {
+        k_block_scan<T><<<1, SCAN_THREADS, 0, s>>>(in, out, n, nullptr, total_dev);
+        return;
+    }
+    Tmp<T> sums(pool, nb);
+    k_block_reduce<T><<<(unsigned)nb, SCAN_THREADS, 0, s>>>(in, n, sums);
+    scan_impl<T>(pool, sums, sums, nb, nullptr, s);
+    k_block_scan<T><<<(unsigned)nb, SCAN_THREADS, 0, s>>>(in, out, n, sums, total_dev);
+}
+
+void scan_u32(DevPool &pool, const uint32_t *in, uint32_t *out, size_t n, uint32_t *total_dev, hipStream_t s) {
+    scan_impl<uint32_t>(pool, in, out, n, total_dev, s);
+}
+void scan_u64(DevPool &pool, const uint64_t *in, uint64_t *out, size_t n, uint64_t *total_dev, hipStream_t s) {
+    scan_impl<uint64_t>(pool, in, out, n, total_dev, s);
+}
+
+// -------------------------------------------------------------- radix sort
+// One pass = histogram per 4096-element tile, exclusive scan of the
+// digit-major histogram table, stable scatter. Stability inside a tile: the
+// tile is walked in 16 rounds of 256 consecutive elements; within a round a
+// wavefront ranks equal digits with 8 ballots (lane order), and the four
+// wavefronts are ordered through per-wave digit counts in LDS.
+
+constexpr int RS_THREADS = 256;
+constexpr int RS_ITEMS = 16;
+constexpr int RS_TILE = RS_THREADS * RS_ITEMS;
+
+__global__ __launch_bounds__(RS_THREADS) void k_rs_hist(const uint64_t *keys, size_t n, int shift, uint32_t *hist,
+                                                        unsigned nblocks) {
+    __shared__ uint32_t h[256];
+    h[threadIdx.x] = 0;
+    __syncthreads();
+    size_t t0 = (size_t)blockIdx.x * RS_TILE;
+#pragma unroll 4
+    for (int j = 0; j < RS_ITEMS; j++) {
+        size_t idx = t0 + (size_t)j * RS_THREADS + threadIdx.x;
+        if (idx < n) atomicAdd(&h[(uint32_t)(keys[idx] >> shift) & 255u], 1u);
+    }
+    __syncthreads();
+    hist[(size_t)threadIdx.x * nblocks + blockIdx.x] = h[threadIdx.x];
+}
+
+__global__ __launch_bounds__(RS_THREADS) void k_rs_scatter(const uint64_t *keys, const uint32_t *vals, uint64_t *okeys,
+                                                           uint32_t *ovals, size_t n, int shift,
+                                                           const uint32_t *hist_scan, unsigned nblocks) {
+    __shared__ uint32_t base[256];
+    __shared__ uint32_t wcnt[RS_THREADS / 64][256];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    base[tid] = hist_scan[(size_t)tid * nblocks + blockIdx.x];
+#pragma unroll
+    for (int w = 0; w < RS_THREADS / 64; w++) wcnt[w][tid] = 0;
+    __syncthreads();
+    const uint64_t lt_mask = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+    size_t t0 = (size_t)blockIdx.x * RS_TILE;
+    for (int j = 0; j < RS_ITEMS; j++) {
+        size_t idx = t0 + (size_t)j * RS_THREADS + tid;
+        bool valid = idx < n;
+        uint64_t k = valid ? keys[idx] : 0;
+        uint32_t v = valid ? vals[idx] : 0;
+        uint32_t d = (uint32_t)(k >> shift) & 255u;
+        uint64_t peers = __ballot(valid);
+#pragma unroll
+        for (int b = 0; b < 8; b++) {
+            bool bit = (d >> b) & 1u;
+            uint64_t bb = __ballot(bit);
+            peers &= bit ? bb : ~bb;
+        }
+        uint32_t rank = (uint32_t)__popcll(peers & lt_mask);
+        if (valid && rank == 0) wcnt[wave][d] = (uint32_t)__popcll(peers);
+        __syncthreads();
+        if (valid) {
+            uint32_t off = base[d];
+            for (int w = 0; w < wave; w++) off += wcnt[w][d];
+            okeys[off + rank] = k;
+            ovals[off + rank] = v;
+        }
+        __syncthreads();
+        uint32_t t = 0;
+#pragma unroll
+        for (int w = 0; w < RS_THREADS / 64; w++) {
+            t += wcnt[w][tid];
+            wcnt[w][tid] = 0;
+        }
+        base[tid] += t;
+        __syncthreads();
+    }
+}
+
+void radix_sort_pairs(DevPool &pool, uint64_t *keys, uint32_t *vals, uint64_t *keys_alt, uint32_t *vals_alt,
+                      size_t n, int nbits, hipStream_t s, uint64_t **keys_out, uint32_t **vals_out) {
+    if (n >= (1ull << 32)) throw LibError{ERR_INVALID, "radix sort limited to < 2^32 elements"};
+    uint64_t *ka = keys, *kb = keys_alt;
+    uint32_t *va = vals, *vb = vals_alt;
+    if (n > 1 && nbits > 0) {
+        unsigned nblocks = (unsigned)div_up(n, RS_TILE);
+        size_t hn = (size_t)256 * nblocks;
+        Tmp<uint32_t> hist(pool, hn);
+        for (int shift = 0; shift < nbits; shift += 8) {
+            k_rs_hist<<<nblocks, RS_THREADS, 0, s>>>(ka, n, shift, hist, nblocks);
+            scan_u32(pool, hist, hist, hn, nullptr, s);
+            k_rs_scatter<<<nblocks, RS_THREADS, 0, s>>>(ka, va, kb, vb, n, shift, hist, nblocks);
+            std::swap(ka, kb);
+            std::swap(va, vb);
+        }
+    }
+    *keys_out = ka;
+    *vals_out = va;
+}
+
+}  // namespace fcc

@@ -1,0 +1,343 @@
+"""fastcc_amd -- FaSTCC sparse-tensor contraction on MI355X (gfx950).
+
+Python host layer over the C ABI of ``libfastcc_hip.so`` (include/fcc.h).
+``Tensor`` / ``ListTensor`` / ``Tensor.fastcc_multiply`` mirror the
+reference's C++ interface (contract.hpp:25-35, 824-827; index_tensors.hpp:80-126)
+so tests read like uses of the reference; ``Context.contract`` exposes every
+option of the C ABI. All computation happens in the HIP library: when it is
+missing, loading raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import _abi
+from ._abi import (FCC_ACC_AUTO, FCC_ACC_DENSE, FCC_ACC_GHASH, FCC_ACC_LHASH, FCC_PLAN_EXACT, FCC_PLAN_LEARNED,
+                   FCC_SINK_COUNT_CHECKSUM, FCC_SINK_MATERIALIZE)
+
+__all__ = ["Context", "Result", "DeviceTensor", "Tensor", "ListTensor", "FastccError", "TileAccumulator",
+           "FCC_ACC_AUTO", "FCC_ACC_DENSE", "FCC_ACC_LHASH", "FCC_ACC_GHASH", "FCC_PLAN_EXACT", "FCC_PLAN_LEARNED"]
+
+
+class FastccError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"fcc error {code}: {msg}")
+        self.code = code
+
+
+@dataclass
+class DeviceTensor:
+    """Coordinates/values already resident on the GPU (e.g. torch tensors)."""
+    order: int
+    nnz: int
+    coords_ptr: int   # device pointer, uint32 SoA [order][nnz]
+    vals_ptr: int     # device pointer, float64 [nnz]
+    keepalive: object = None
+
+    @staticmethod
+    def from_torch(coords, vals):
+        """coords: int32/uint32 CUDA tensor [order, nnz] (contiguous), vals: float64 CUDA tensor [nnz]."""
+        assert coords.is_cuda and vals.is_cuda and coords.is_contiguous() and vals.is_contiguous()
+        assert coords.element_size() == 4 and vals.element_size() == 8
+        return DeviceTensor(int(coords.shape[0]), int(vals.shape[0]), coords.data_ptr(), vals.data_ptr(),
+                            (coords, vals))
+
+
+def _as_fcc_tensor(t, keep):
+    ft = _abi.FccTensor()
+    if isinstance(t, DeviceTensor):
+        ft.order, ft.nnz = t.order, t.nnz
+        ft.coords, ft.vals = t.coords_ptr, t.vals_ptr
+        ft.on_device = 1
+        ft.extents = None
+        return ft
+    coords = np.ascontiguousarray(t.coords, dtype=np.uint32)
+    vals = np.ascontiguousarray(t.vals, dtype=np.float64)
+    keep += [coords, vals]
+    ft.order = int(coords.shape[0])
+    ft.nnz = int(vals.shape[0])
+    ft.coords = coords.ctypes.data if ft.nnz else None
+    ft.vals = vals.ctypes.data if ft.nnz else None
+    ft.extents = None
+    ft.on_device = 0
+    ext = getattr(t, "given_extents", None)
+    if ext is not None:
+        e = (C.c_uint32 * ft.order)(*[int(x) for x in ext])
+        keep.append(e)
+        ft.extents = C.cast(e, C.POINTER(C.c_uint32))
+    return ft
+
+
+def _stats_dict(s: _abi.FccStats):
+    return {name: getattr(s, name) for name, _ in _abi.FccStats._fields_}
+
+
+class Result:
+    """A contraction result; device memory stays library-owned until free()."""
+
+    def __init__(self, ctx: "Context", raw: _abi.FccResult):
+        self._ctx = ctx
+        self._raw = raw
+        self.order = raw.order
+        self.nnz = raw.nnz
+        self.extents = tuple(raw.extents[i] for i in range(raw.order))
+        self.coord_hash = raw.coord_hash
+        self.val_sum = raw.val_sum
+        self.val_wsum = raw.val_wsum
+        self.stats = _stats_dict(raw.stats)
+        self.coord_stride = raw.coord_stride
+        self.coords_ptr = raw.coords or 0
+        self.vals_ptr = raw.vals or 0
+
+    def to_host(self):
+        """(coords uint32 [order, nnz], vals float64 [nnz])"""
+        coords = np.empty((self.order, self.nnz), dtype=np.uint32)
+        vals = np.empty(self.nnz, dtype=np.float64)
+        if self.nnz:
+            rc = self._ctx.lib.fcc_result_to_host(self._ctx.h, C.byref(self._raw), coords.ctypes.data,
+                                                  vals.ctypes.data)
+            self._ctx._check(rc)
+        return coords, vals
+
+    def free(self):
+        if self._raw is not None and self._ctx.h:
+            self._ctx.lib.fcc_result_free(self._ctx.h, C.byref(self._raw))
+        self._raw = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class Context:
+    """One GPU, one HIP stream, a caching device pool and the MLP planner."""
+
+    def __init__(self, device: int = 0):
+        self.lib = _abi.load()
+        h = C.c_void_p()
+        rc = self.lib.fcc_create(C.byref(h), int(device))
+        if rc != 0:
+            raise FastccError(rc, "fcc_create failed (no usable HIP device?)")
+        self.h = h
+
+    def _check(self, rc):
+        if rc != 0:
+            msg = self.lib.fcc_last_error(self.h)
+            raise FastccError(rc, msg.decode() if msg else "")
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.fcc_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def stream(self) -> int:
+        return self.lib.fcc_stream(self.h) or 0
+
+    @staticmethod
+    def options(planner="exact", tile_hint=0, sink=False, force_acc=FCC_ACC_AUTO, col_window=0, workgroups=0,
+                collect_stats=False, left_tile_stride=1, summarize=False):
+        o = _abi.FccOptions()
+        _abi.load().fcc_options_default(C.byref(o))
+        o.planner = FCC_PLAN_LEARNED if planner == "learned" else FCC_PLAN_EXACT
+        o.tile_hint = int(tile_hint)
+        o.sink = FCC_SINK_COUNT_CHECKSUM if sink else FCC_SINK_MATERIALIZE
+        o.force_acc = int(force_acc)
+        o.col_window = int(col_window)
+        o.workgroups = int(workgroups)
+        o.collect_stats = 1 if collect_stats else 0
+        o.left_tile_stride = int(left_tile_stride)
+        o.summarize = 1 if summarize else 0
+        return o
+
+    def contract(self, a, a_modes, b, b_modes, **opts) -> Result:
+        keep = []
+        ta = _as_fcc_tensor(a, keep)
+        tb = ta if b is a else _as_fcc_tensor(b, keep)
+        r = len(a_modes)
+        if len(b_modes) != r:
+            raise FastccError(_abi.FCC_ERR_INVALID, "mode lists differ in length")
+        am = (C.c_int32 * max(r, 1))(*a_modes)
+        bm = (C.c_int32 * max(r, 1))(*b_modes)
+        o = self.options(**opts)
+        raw = _abi.FccResult()
+        rc = self.lib.fcc_contract(self.h, C.byref(ta), am, C.byref(tb), bm, r, C.byref(o), C.byref(raw))
+        self._check(rc)
+        return Result(self, raw)
+
+    def contract_batch(self, pairs, a_modes, b_modes, **opts):
+        keep = []
+        n = len(pairs)
+        ta = (_abi.FccTensor * max(n, 1))()
+        tb = (_abi.FccTensor * max(n, 1))()
+        for i, (a, b) in enumerate(pairs):
+            ta[i] = _as_fcc_tensor(a, keep)
+            tb[i] = _as_fcc_tensor(b, keep)
+        r = len(a_modes)
+        am = (C.c_int32 * max(r, 1))(*a_modes)
+        bm = (C.c_int32 * max(r, 1))(*b_modes)
+        o = self.options(**opts)
+        raws = (_abi.FccResult * max(n, 1))()
+        rc = self.lib.fcc_contract_batch(self.h, n, ta, am, tb, bm, r, C.byref(o), raws)
+        self._check(rc)
+        return [Result(self, raws[i]) for i in range(n)]
+
+    # ---- learned planner
+    @property
+    def mlp_param_count(self) -> int:
+        return int(self.lib.fcc_mlp_param_count())
+
+    def set_mlp_params(self, params):
+        p = np.ascontiguousarray(params, dtype=np.float32)
+        assert p.size == self.mlp_param_count
+        self._check(self.lib.fcc_mlp_set_params(self.h, p.ctypes.data))
+
+    def mlp_params(self):
+        p = np.empty(self.mlp_param_count, dtype=np.float32)
+        self._check(self.lib.fcc_mlp_get_params(self.h, p.ctypes.data))
+        return p
+
+    def tile_stats(self):
+        n = int(self.lib.fcc_tile_stats_count(self.h))
+        f = np.empty((n, _abi.FCC_MLP_IN), dtype=np.float32)
+        l = np.empty((n, _abi.FCC_MLP_OUT), dtype=np.float32)
+        if n:
+            self._check(self.lib.fcc_tile_stats_copy(self.h, f.ctypes.data, l.ctypes.data, n))
+        return f, l
+
+    def train_mlp(self, features, labels, epochs=30, batch=256, lr=3e-3, seed=1):
+        f = np.ascontiguousarray(features, dtype=np.float32)
+        l = np.ascontiguousarray(labels, dtype=np.float32)
+        loss = C.c_float()
+        self._check(self.lib.fcc_mlp_train(self.h, f.ctypes.data, l.ctypes.data, f.shape[0], int(epochs),
+                                           int(batch), float(lr), int(seed), C.byref(loss)))
+        return float(loss.value)
+
+    def mlp_forward_backward(self, features, labels):
+        f = np.ascontiguousarray(features, dtype=np.float32)
+        l = np.ascontiguousarray(labels, dtype=np.float32)
+        n = f.shape[0]
+        pred = np.empty((n, _abi.FCC_MLP_OUT), dtype=np.float32)
+        grad = np.empty(self.mlp_param_count, dtype=np.float32)
+        loss = C.c_float()
+        self._check(self.lib.fcc_mlp_forward_backward(self.h, f.ctypes.data, l.ctypes.data, n, pred.ctypes.data,
+                                                      C.byref(loss), grad.ctypes.data))
+        return pred, float(loss.value), grad
+
+    # ---- autotuner table
+    def set_tune(self, entries):
+        arr = (_abi.FccTuneEntry * max(len(entries), 1))()
+        for i, e in enumerate(entries):
+            arr[i].bucket, arr[i].row_tile, arr[i].col_window, arr[i].workgroups = (
+                int(e["bucket"]), int(e["row_tile"]), int(e["col_window"]), int(e.get("workgroups", 0)))
+            arr[i].ms = float(e.get("ms", 0.0))
+        self._check(self.lib.fcc_tune_set(self.h, arr, len(entries)))
+
+    def get_tune(self):
+        arr = (_abi.FccTuneEntry * 64)()
+        n = C.c_int32()
+        self._check(self.lib.fcc_tune_get(self.h, arr, 64, C.byref(n)))
+        return [dict(bucket=arr[i].bucket, row_tile=arr[i].row_tile, col_window=arr[i].col_window,
+                     workgroups=arr[i].workgroups, ms=arr[i].ms) for i in range(n.value)]
+
+    def density_bucket(self, d: float) -> int:
+        return int(self.lib.fcc_density_bucket(float(d)))
+
+
+# ---------------------------------------------------------------------------
+# Mirror of the reference interface (contract.hpp / index_tensors.hpp).
+
+class TileAccumulator:
+    """Tag standing for TileAccumulator<double> (index_tensors.hpp:994-1065):
+    the planner's dense LDS tile; hash kinds cover TileAccumulatorMap."""
+
+
+_default_ctx = None
+
+
+def default_context() -> Context:
+    global _default_ctx
+    if _default_ctx is None:
+        _default_ctx = Context(0)
+    return _default_ctx
+
+
+class ListTensor:
+    """Result of fastcc_multiply (index_tensors.hpp:80-126). The reference
+    exposes only compute_nnz_count(); coords()/values() are additions."""
+
+    def __init__(self, result: Result):
+        self._r = result
+        self._host = None
+
+    def compute_nnz_count(self) -> int:
+        return int(self._r.nnz)
+
+    def _fetch(self):
+        if self._host is None:
+            self._host = self._r.to_host()
+        return self._host
+
+    def coords(self) -> np.ndarray:
+        return self._fetch()[0]
+
+    def values(self) -> np.ndarray:
+        return self._fetch()[1]
+
+    @property
+    def result(self) -> Result:
+        return self._r
+
+
+class Tensor:
+    """Sparse tensor in COO form (contract.hpp:25-35). ``coords`` is
+    [order, nnz] uint32, ``vals`` [nnz] float64."""
+
+    def __init__(self, coords, vals):
+        self.coords = np.ascontiguousarray(coords, dtype=np.uint32)
+        self.vals = np.ascontiguousarray(vals, dtype=np.float64)
+        self.dimensionality = 42
+        self.shape = None
+
+    # contract.hpp:101-105
+    def _infer_dimensionality(self):
+        if self.vals.shape[0] > 0:
+            self.dimensionality = int(self.coords.shape[0])
+
+    # contract.hpp:124-144 (shape = max coordinate per mode)
+    def _infer_shape(self):
+        if self.vals.shape[0] > 0:
+            self.shape = self.coords.max(axis=1).astype(np.int64)
+
+    def get_dimensionality(self):
+        if self.dimensionality == 42:
+            self._infer_dimensionality()
+        return self.dimensionality
+
+    def get_shape_ref(self):
+        if self.shape is None:
+            self._infer_dimensionality()
+            self._infer_shape()
+        return self.shape
+
+    def get_size(self):
+        return int(self.vals.shape[0])
+
+    def fastcc_multiply(self, other: "Tensor", left_contr, right_contr, tile_size=100, *, ctx: Context | None = None,
+                        **opts) -> ListTensor:
+        """contract.hpp:824-827: tile_size rounds up to a power of two and sets
+        the rows per left tile."""
+        ctx = ctx or default_context()
+        r = ctx.contract(self, list(left_contr), other, list(right_contr), tile_hint=tile_size, **opts)
+        return ListTensor(r)

@@ -1,0 +1,172 @@
+/* fcc.h -- C ABI of the MI355X (gfx950) FaSTCC sparse-tensor contraction.
+ *
+ * Plain pointers and sizes only. Every entry point returns 0 on success or a
+ * negative FCC_ERR_* code and never exits the process; fcc_last_error()
+ * describes the most recent failure on a context.
+ *
+ * The contraction computed is the reference's fastcc_multiply
+ * (contract.hpp:824-912):
+ *     C(x, y) = sum_k A(x, k) * B(k, y)
+ * where k runs over the paired contraction coordinates a_modes[i] <-> b_modes[i],
+ * x over A's non-contracted modes in ascending order and y over B's. The
+ * output keeps every coordinate reached by at least one product (structural
+ * support, index_tensors.hpp:1028-1037), exact-zero sums included, and holds
+ * A's external coordinates first, then B's. Output order is unspecified (the
+ * reference's order depends on its OpenMP schedule).
+ */
+#ifndef FCC_H
+#define FCC_H
+#include <stdint.h>
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define FCC_MAX_ORDER 8
+
+enum {
+    FCC_OK = 0,
+    FCC_ERR_INVALID = -1,  /* bad argument, shape or mode list */
+    FCC_ERR_OOM = -2,      /* device or host allocation failed */
+    FCC_ERR_OVERFLOW = -3, /* a bounded pool was exhausted and could not be grown */
+    FCC_ERR_HIP = -4       /* a HIP runtime call failed */
+};
+
+enum { FCC_PLAN_EXACT = 0, FCC_PLAN_LEARNED = 1 };
+enum { FCC_SINK_MATERIALIZE = 0, FCC_SINK_COUNT_CHECKSUM = 1 };
+/* accumulator kinds (per unit of work); FCC_ACC_AUTO lets the planner choose */
+enum { FCC_ACC_AUTO = 0, FCC_ACC_DENSE = 1, FCC_ACC_LHASH = 2, FCC_ACC_GHASH = 3 };
+
+/* A sparse tensor in coordinate form. Stands in for the reference's
+ * Tensor<double> storage (contract.hpp:25-35, vector<NNZ<double>>). */
+typedef struct {
+    int32_t order;
+    int64_t nnz;
+    const uint32_t *coords;   /* SoA: coords[m * nnz + n], m < order */
+    const double *vals;       /* nnz values */
+    const uint32_t *extents;  /* host array of order extents, or NULL: max+1 per mode,
+                                 as Tensor::_infer_shape (contract.hpp:124-144) */
+    int32_t on_device;        /* 1: coords/vals are device pointers, 0: host */
+} fcc_tensor;
+
+typedef struct {
+    int32_t planner;     /* FCC_PLAN_EXACT (count-then-fill) or FCC_PLAN_LEARNED (MLP) */
+    int32_t tile_hint;   /* reference tile_size (rows per left tile); 0 = planner/autotuner */
+    int32_t sink;        /* FCC_SINK_MATERIALIZE or FCC_SINK_COUNT_CHECKSUM */
+    int32_t force_acc;   /* FCC_ACC_AUTO, or force one kind for every unit (testing) */
+    int32_t col_window;  /* columns per dense unit; 0 = planner */
+    int32_t workgroups;  /* persistent workgroups for accumulate kernels; 0 = planner */
+    int32_t collect_stats; /* 1: record per-tile features and exact labels (MLP training) */
+    int32_t left_tile_stride; /* 1 (or 0) = all left tiles; s > 1 = only left rows x with
+                                 (x / T) % s == 0, T = tile_hint rounded up to a power of
+                                 two (128 when 0): the sampled baseline of the largest shapes */
+    int32_t summarize;   /* 1: also fill coord_hash / val_sum / val_wsum for a materialised result */
+} fcc_options;
+
+typedef struct {
+    double ms_total;      /* device time of the whole contraction (hipEvents) */
+    double ms_index;      /* extents, linearise, sort, CSR, tiles */
+    double ms_plan;       /* tile features, MLP or bounds, unit formation */
+    double ms_symbolic;   /* exact count pass + scan (0 for the learned planner) */
+    double ms_numeric;    /* accumulate + drain */
+    double ms_overflow;   /* global-hash drain and any re-run */
+    uint64_t products;    /* multiply-adds: sum_k nnzA(k) * nnzB(k) */
+    uint64_t bytes_alg;   /* nnzA(4dA+8) + nnzB(4dB+8) + nnzC(4dC+8) */
+    uint64_t left_tiles;  /* active left tiles */
+    uint64_t units;       /* (tile, column window) units */
+    uint64_t units_dense, units_lhash, units_ghash;
+    uint64_t ghash_entries; /* outputs accumulated in the global hash (planned + spilled) */
+    uint64_t reruns;      /* learned-planner capacity re-runs */
+    int32_t row_tile;     /* rows per left tile used */
+    int32_t col_window;   /* columns per dense unit used */
+    double mlp_mape;      /* learned planner: tile-level MAPE of predicted nnz vs actual (-1 if n/a) */
+    double ms_numeric_kernel; /* average duration of the numeric accumulate kernel */
+} fcc_stats;
+
+typedef struct {
+    int32_t order;      /* dA + dB - 2r */
+    int64_t nnz;        /* output nonzeros */
+    uint32_t *coords;   /* device SoA: coords[m * coord_stride + n], library-owned (NULL in sink mode) */
+    int64_t coord_stride; /* >= nnz; equals nnz for the exact planner */
+    double *vals;       /* device [nnz], library-owned (NULL in sink mode) */
+    uint32_t extents[FCC_MAX_ORDER]; /* output extents */
+    uint64_t coord_hash; /* sum of mix64(linear output index), mod 2^64 */
+    double val_sum;      /* sum of output values */
+    double val_wsum;     /* sum of value * u(mix64(linear index)), u in [0,1) */
+    fcc_stats stats;
+    void *priv;
+} fcc_result;
+
+typedef struct fcc_ctx fcc_ctx;
+
+/* Context: one HIP device, one stream, a caching device pool, MLP weights.
+ * Stands in for the reference's per-call heaps/thread setup
+ * (contract.hpp:847-848, timer.hpp:66-125). One context per host thread. */
+int fcc_create(fcc_ctx **ctx, int device);
+void fcc_destroy(fcc_ctx *ctx);
+const char *fcc_last_error(const fcc_ctx *ctx);
+/* the HIP stream (hipStream_t) all work of this context is issued on */
+void *fcc_stream(fcc_ctx *ctx);
+
+void fcc_options_default(fcc_options *opt);
+
+/* Replaces Tensor<double>::fastcc_multiply<TileAccumulator<double>,double>(
+ *     Tensor<double>& other, CoOrdinate left_contr, CoOrdinate right_contr,
+ *     int tile_size) -- contract.hpp:824-827. Here a = *this, b = other,
+ * a_modes = left_contr, b_modes = right_contr, options->tile_hint = tile_size. */
+int fcc_contract(fcc_ctx *ctx, const fcc_tensor *a, const int32_t *a_modes, const fcc_tensor *b,
+                 const int32_t *b_modes, int32_t n_contr, const fcc_options *options, fcc_result *out);
+
+/* Many independent contractions (same mode lists) in one call; their tiles
+ * share one persistent work-stealing launch. No reference counterpart: the
+ * reference contracts one pair per call (benchmark.cpp:68-73). */
+int fcc_contract_batch(fcc_ctx *ctx, int32_t n, const fcc_tensor *a, const int32_t *a_modes,
+                       const fcc_tensor *b, const int32_t *b_modes, int32_t n_contr,
+                       const fcc_options *options, fcc_result *outs);
+
+/* Copies a materialised result to host SoA arrays (coords: order*nnz, vals: nnz).
+ * Stands in for walking the reference's ListTensor nodes (index_tensors.hpp:80-126). */
+int fcc_result_to_host(fcc_ctx *ctx, const fcc_result *r, uint32_t *coords, double *vals);
+void fcc_result_free(fcc_ctx *ctx, fcc_result *r);
+
+/* ---- learned planner (MLP 6 -> 64 -> 64 -> 2, FP32) -------------------- */
+#define FCC_MLP_IN 6
+#define FCC_MLP_HID 64
+#define FCC_MLP_OUT 2
+/* number of floats in the flat parameter vector: W1,b1,W2,b2,W3,b3 plus the
+ * feature/label normalisation (mean, std per input, mean, std per output) */
+int64_t fcc_mlp_param_count(void);
+/* set / get the context's MLP parameters (host arrays of fcc_mlp_param_count floats) */
+int fcc_mlp_set_params(fcc_ctx *ctx, const float *params);
+int fcc_mlp_get_params(fcc_ctx *ctx, float *params);
+/* Tile statistics recorded by the last fcc_contract with collect_stats=1:
+ * n rows of FCC_MLP_IN features and FCC_MLP_OUT labels (host arrays). */
+int64_t fcc_tile_stats_count(fcc_ctx *ctx);
+int fcc_tile_stats_copy(fcc_ctx *ctx, float *features, float *labels, int64_t n);
+/* Train the context's MLP on the GPU (hand-written forward/backward kernels,
+ * Adam) on host feature/label rows; returns the final loss in *loss. */
+int fcc_mlp_train(fcc_ctx *ctx, const float *features, const float *labels, int64_t n, int32_t epochs,
+                  int32_t batch, float lr, uint64_t seed, float *loss);
+/* One forward + backward on the GPU for a batch: outputs predictions
+ * (n*FCC_MLP_OUT), loss, and the gradient of the mean squared loss with
+ * respect to the flat parameter vector (fcc_mlp_param_count floats). */
+int fcc_mlp_forward_backward(fcc_ctx *ctx, const float *features, const float *labels, int64_t n,
+                             float *pred, float *loss, float *grad);
+
+/* ---- autotuner ---------------------------------------------------------- */
+/* Entries choose (row_tile, col_window, workgroups) per density bucket;
+ * fcc_contract uses them when the options leave those at 0. */
+typedef struct {
+    int32_t bucket;      /* density bucket index, see fcc_density_bucket */
+    int32_t row_tile;
+    int32_t col_window;
+    int32_t workgroups;
+    double ms;           /* measured time of the winning candidate */
+} fcc_tune_entry;
+int32_t fcc_density_bucket(double products_per_output_cell);
+int fcc_tune_set(fcc_ctx *ctx, const fcc_tune_entry *entries, int32_t n);
+int fcc_tune_get(fcc_ctx *ctx, fcc_tune_entry *entries, int32_t cap, int32_t *n);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
