@@ -42,6 +42,11 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-stride", type=int, default=0, help="left-tile sample stride for the CPU baseline")
     ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--tile-hint", type=int, default=0, help="rows per left tile (0 = planner)")
+    ap.add_argument("--col-window", type=int, default=0, help="columns per dense unit (0 = planner)")
+    ap.add_argument("--workgroups", type=int, default=0, help="persistent workgroups (0 = planner)")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run a device-to-device copy of MIB MiB (HBM counter calibration) and exit")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "
@@ -50,6 +55,15 @@ def main():
 
     import numpy as np
     import torch
+    if args.copy_calibration:
+        n = args.copy_calibration << 20
+        src = torch.ones(n // 8, dtype=torch.float64, device="cuda:0")
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_calibration_bytes": n}))
+        return
     from fastcc_amd import Context, DeviceTensor
     from fastcc_amd import workloads as W
 
@@ -68,7 +82,8 @@ def main():
     torch.cuda.synchronize()
 
     ctx = Context(0)
-    opts = dict(planner=args.planner)
+    opts = dict(planner=args.planner, tile_hint=args.tile_hint, col_window=args.col_window,
+                workgroups=args.workgroups)
     if args.planner == "learned":
         params = os.path.join(ROOT, "fastcc-benchmark_amd", "fastcc_amd", "mlp_params.npy")
         ctx.set_mlp_params(np.load(params))

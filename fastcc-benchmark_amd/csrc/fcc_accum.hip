@@ -133,6 +133,7 @@ __device__ __forceinline__ int lds_probe(uint32_t *keys, uint32_t *cnt, uint32_t
 
 struct UnitCtx {
     uint32_t u, a, tile, c0, W, kind, multi;
+    uint32_t Wp;  // dense row stride: W rounded up to a multiple of 32 (hash kinds: W)
     uint64_t row0;
 };
 
@@ -149,7 +150,7 @@ template <bool NUM, bool BITMAP>
 __device__ __forceinline__ void accum(const AccArgs &A, const UnitCtx &U, const Acc<NUM, BITMAP> &S, uint32_t li,
                                       uint32_t col, double v) {
     if (U.kind == KIND_DENSE) {
-        uint32_t slot = li * U.W + col;
+        uint32_t slot = li * U.Wp + col;
         if (!NUM || BITMAP) atomicOr(&S.bits[slot >> 5], 1u << (slot & 31));
         if (NUM) lds_add_f64(&S.fv[slot], v);
     } else if (U.kind == KIND_LHASH) {
@@ -169,51 +170,153 @@ __device__ __forceinline__ void accum(const AccArgs &A, const UnitCtx &U, const 
     }
 }
 
+__device__ __forceinline__ uint32_t rlane(uint32_t v, uint32_t l) {
+    return (uint32_t)__builtin_amdgcn_readlane((int)v, (int)l);
+}
+__device__ __forceinline__ double rlane_f64(double v, uint32_t l) {
+    unsigned long long b = (unsigned long long)__double_as_longlong(v);
+    uint32_t lo = rlane((uint32_t)b, l), hi = rlane((uint32_t)(b >> 32), l);
+    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+
+// per-lane lower bound: first index in [lo, hi) with arr[i] >= x
+__device__ __forceinline__ uint32_t lane_lower_bound(const uint32_t *arr, uint32_t lo, uint32_t hi, uint32_t x) {
+    while (lo < hi) {
+        uint32_t mid = (lo + hi) >> 1;
+        if (arr[mid] < x) lo = mid + 1;
+        else hi = mid;
+    }
+    return lo;
+}
+
+// One unit: heavy groups, then light entries.
+// Heavy groups are claimed 64 at a time per wavefront; lane l holds group l's
+// descriptor (fiber range, entry range), read back as scalars with readlane.
+// For group j the wavefront's lanes cover 64 columns of the fiber segment
+// and loop over the group's rows (row index and value broadcast from lanes);
+// the first row/column chunks of group j+1 are loaded before group j's
+// accumulation so their latency overlaps it.
 template <bool NUM, bool BITMAP>
 __device__ void process_unit(const AccArgs &A, const UnitCtx &U, const Acc<NUM, BITMAP> &S, uint32_t *s_gnext) {
     const uint32_t lane = threadIdx.x & 63;
     const uint32_t g0 = A.atile_g0[U.a], gl = A.atile_lg[U.a], g1 = A.atile_g0[U.a + 1];
     const uint32_t cend = U.c0 + U.W;
-    // heavy groups: one wavefront per group
     for (;;) {
-        uint32_t g = 0;
-        if (lane == 0) g = atomicAdd(s_gnext, 1u);
-        g = uniform_u32(__shfl(g, 0, 64));
-        g += g0;
-        if (g >= gl) break;
-        uint32_t f = uniform_u32(A.group_fiber[g]);
-        uint32_t ep0 = uniform_u32(A.group_ptr[g]), ep1 = uniform_u32(A.group_ptr[g + 1]);
-        uint32_t s = uniform_u32(A.fiber_ptr[f]), e = uniform_u32(A.fiber_ptr[f + 1]);
-        if (U.multi) {
-            s = uniform_u32(wave_lower_bound(A.Bcol, s, e, U.c0));
-            e = uniform_u32(wave_lower_bound(A.Bcol, s, e, cend));
+        uint32_t bs = 0;
+        if (lane == 0) bs = atomicAdd(s_gnext, 64u);
+        bs = uniform_u32(__shfl(bs, 0, 64));
+        if (bs >= gl - g0) break;
+        const uint32_t bg = g0 + bs;
+        const uint32_t nb = min(64u, gl - bg);
+        uint4 d = make_uint4(0, 0, 0, 0);
+        if (lane < nb) d = A.gdesc[bg + lane];
+        uint32_t s_l = d.x, e_l = d.x + d.y;
+        if (U.multi && lane < nb) {
+            s_l = lane_lower_bound(A.Bcol, s_l, e_l, U.c0);
+            e_l = lane_lower_bound(A.Bcol, s_l, e_l, cend);
         }
-        for (uint32_t cb = s; cb < e; cb += 64) {
-            uint32_t p = cb + lane;
-            bool ok = p < e;
-            uint32_t col = ok ? A.Bcol[p] - U.c0 : 0;
-            double b = ok ? A.Bval[p] : 0.0;
-            for (uint32_t k = ep0; k < ep1; k++) {
-                uint32_t li = A.Ali[k];
-                double av = A.Aval[k];
-                if (ok) accum<NUM, BITMAP>(A, U, S, li, col, av * b);
+        uint32_t s = rlane(s_l, 0), e = rlane(e_l, 0), ep0 = rlane(d.z, 0), m = rlane(d.w, 0);
+        uint32_t nli = 0, ncol = 0;
+        double nav = 0.0, nbv = 0.0;
+        if (lane < m) {
+            nli = A.Ali[ep0 + lane];
+            if (NUM) nav = A.Aval[ep0 + lane];
+        }
+        if (s + lane < e) {
+            ncol = A.Bcol[s + lane];
+            if (NUM) nbv = A.Bval[s + lane];
+        }
+        for (uint32_t j = 0; j < nb; j++) {
+            const uint32_t cs = s, ce = e, cep0 = ep0, cm = m;
+            const uint32_t cli = nli, ccol = ncol;
+            const double cav = nav, cbv = nbv;
+            if (j + 1 < nb) {
+                s = rlane(s_l, j + 1);
+                e = rlane(e_l, j + 1);
+                ep0 = rlane(d.z, j + 1);
+                m = rlane(d.w, j + 1);
+                nli = 0;
+                ncol = 0;
+                nav = 0.0;
+                nbv = 0.0;
+                if (lane < m) {
+                    nli = A.Ali[ep0 + lane];
+                    if (NUM) nav = A.Aval[ep0 + lane];
+                }
+                if (s + lane < e) {
+                    ncol = A.Bcol[s + lane];
+                    if (NUM) nbv = A.Bval[s + lane];
+                }
+            }
+            for (uint32_t cb = cs; cb < ce; cb += 64) {
+                uint32_t col = ccol;
+                double b = cbv;
+                const bool ok = cb + lane < ce;
+                if (cb != cs) {
+                    col = 0;
+                    b = 0.0;
+                    if (ok) {
+                        col = A.Bcol[cb + lane];
+                        if (NUM) b = A.Bval[cb + lane];
+                    }
+                }
+                col -= U.c0;
+                for (uint32_t kb = 0; kb < cm; kb += 64) {
+                    uint32_t li_l = cli;
+                    double av_l = cav;
+                    if (kb != 0) {
+                        li_l = 0;
+                        av_l = 0.0;
+                        if (kb + lane < cm) {
+                            li_l = A.Ali[cep0 + kb + lane];
+                            if (NUM) av_l = A.Aval[cep0 + kb + lane];
+                        }
+                    }
+                    const uint32_t mk = min(64u, cm - kb);
+                    if (U.kind == KIND_DENSE) {
+                        // Dense tile: the chunk's columns are ascending, so lanes
+                        // sharing a 32-column bitmap word are contiguous; OR their
+                        // bits together and let one lane per word touch the bitmap.
+                        uint32_t wc = ok ? (col >> 5) : 0xFFFFFFFFu;
+                        uint32_t bm = ok ? (1u << (col & 31)) : 0u;
+                        bool leader = false;
+                        if (!NUM || BITMAP) {
+#pragma unroll
+                            for (int dd = 1; dd < 64; dd <<= 1) {
+                                uint32_t ob = __shfl_down(bm, dd, 64), ow = __shfl_down(wc, dd, 64);
+                                if (lane + dd < 64 && ow == wc) bm |= ob;
+                            }
+                            uint32_t pw = __shfl_up(wc, 1, 64);
+                            leader = ok && (lane == 0 || pw != wc);
+                        }
+                        const uint32_t wpr = U.Wp >> 5;
+                        for (uint32_t kk = 0; kk < mk; kk++) {
+                            const uint32_t li = rlane(li_l, kk);
+                            if ((!NUM || BITMAP) && leader) atomicOr(&S.bits[li * wpr + wc], bm);
+                            if (NUM && ok) lds_add_f64(&S.fv[li * U.Wp + col], rlane_f64(av_l, kk) * b);
+                        }
+                    } else {
+                        for (uint32_t kk = 0; kk < mk; kk++) {
+                            const uint32_t li = rlane(li_l, kk);
+                            const double av = NUM ? rlane_f64(av_l, kk) : 0.0;
+                            if (ok) accum<NUM, BITMAP>(A, U, S, li, col, av * b);
+                        }
+                    }
+                }
             }
         }
     }
-    // light entries: one lane per entry, its fiber scanned serially
+    // light entries: one lane per entry, fiber bounds precomputed per entry
     const uint32_t le0 = A.group_ptr[gl], le1 = A.group_ptr[g1];
     for (uint32_t e = le0 + threadIdx.x; e < le1; e += ACC_THREADS) {
-        uint32_t li = A.Ali[e];
-        double av = A.Aval[e];
-        uint32_t f = A.Afib[e];
-        uint32_t p0 = A.fiber_ptr[f], p1 = A.fiber_ptr[f + 1];
-        for (uint32_t p = p0; p < p1; p++) {
-            uint32_t col = A.Bcol[p];
-            if (U.multi) {
-                if (col < U.c0) continue;
-                if (col >= cend) break;
-            }
-            accum<NUM, BITMAP>(A, U, S, li, col - U.c0, av * A.Bval[p]);
+        const uint32_t li = A.Ali[e];
+        const double av = NUM ? A.Aval[e] : 0.0;
+        const uint2 fp = A.Afp[e];
+#pragma unroll 4
+        for (uint32_t p = fp.x; p < fp.y; p++) {
+            const uint32_t col = A.Bcol[p];
+            if (U.multi && (col < U.c0 || col >= cend)) continue;
+            accum<NUM, BITMAP>(A, U, S, li, col - U.c0, NUM ? av * A.Bval[p] : 0.0);
         }
     }
 }
@@ -356,9 +459,10 @@ __global__ __launch_bounds__(ACC_THREADS) void k_numeric(AccArgs A) {
         U.W = A.unit_W[U.u];
         U.kind = A.unit_kind[U.u];
         U.multi = A.unit_multi[U.u];
+        U.Wp = U.kind == KIND_DENSE ? ((U.W + 31) & ~31u) : U.W;
         U.row0 = (uint64_t)U.tile << A.logTL;
         const uint32_t TL = 1u << A.logTL;
-        const uint32_t nslots = U.kind == KIND_DENSE ? TL * U.W : (U.kind == KIND_LHASH ? LH_SLOTS : 0);
+        const uint32_t nslots = U.kind == KIND_DENSE ? TL * U.Wp : (U.kind == KIND_LHASH ? LH_SLOTS : 0);
         if (U.kind == KIND_DENSE) {
             const double init = BITMAP ? 0.0 : -0.0;
             for (uint32_t s = threadIdx.x; s < nslots; s += ACC_THREADS) s_f64[s] = init;
@@ -406,7 +510,7 @@ __global__ __launch_bounds__(ACC_THREADS) void k_numeric(AccArgs A) {
                 const uint64_t base = s_base;
                 uint32_t emitted = block_compact(nslots, s_w, pred, [&](uint32_t s, uint32_t rank) {
                     uint32_t key = U.kind == KIND_DENSE ? s : S.keys[s];
-                    uint32_t li = key / U.W, cl = key - li * U.W;
+                    uint32_t li = key / U.Wp, cl = key - li * U.Wp;
                     emit_output(A, base + rank, U.row0 + li, (uint64_t)U.c0 + cl, s_f64[s], SINK, sk);
                 });
                 if (MODE == 0 && !SINK && threadIdx.x == 0) A.unit_cursor[U.u] = emitted;
@@ -441,9 +545,10 @@ __global__ __launch_bounds__(ACC_THREADS) void k_count(AccArgs A) {
         U.W = A.unit_W[U.u];
         U.kind = A.unit_kind[U.u];
         U.multi = A.unit_multi[U.u];
+        U.Wp = U.kind == KIND_DENSE ? ((U.W + 31) & ~31u) : U.W;
         U.row0 = (uint64_t)U.tile << A.logTL;
         const uint32_t TL = 1u << A.logTL;
-        const uint32_t nwords = U.kind == KIND_DENSE ? (TL * U.W + 31) / 32 : 0;
+        const uint32_t nwords = U.kind == KIND_DENSE ? TL * U.Wp / 32 : 0;
         if (U.kind == KIND_DENSE) {
             for (uint32_t s = threadIdx.x; s < nwords; s += ACC_THREADS) s_set[s] = 0;
         } else if (U.kind == KIND_LHASH) {
@@ -547,6 +652,9 @@ __global__ void k_plan_tiles(const uint64_t *st_products, const float *pred, uin
     const double prod = (double)st_products[a];
     const double S = (double)P.spanB;
     const double area = (double)P.TL * S;
+    // dense rows are padded to a multiple of 32 columns
+    const double parea = (double)P.TL * (double)((P.spanB + 31) & ~31ull);
+    const bool dense_ok = (uint64_t)P.TL * 32 <= DENSE_SLOTS;
     double est = pred ? (double)pred[a] * (1.0 + P.margin) + 8.0 : prod;
     if (est > prod) est = prod;
     if (est > area) est = area;
@@ -554,12 +662,13 @@ __global__ void k_plan_tiles(const uint64_t *st_products, const float *pred, uin
     uint64_t W = P.spanB;
     const int force = P.force;
     bool dense;
-    if (force) dense = force == KIND_DENSE;
+    if (!dense_ok) dense = false;
+    else if (force) dense = force == KIND_DENSE;
     else if (est <= (double)HASH_TARGET) dense = false;
-    else dense = area <= (double)DENSE_SLOTS || est >= P.dense_ratio * area;
+    else dense = parea <= (double)DENSE_SLOTS || est >= P.dense_ratio * area;
     if (dense) {
         kind = KIND_DENSE;
-        if (area > (double)DENSE_SLOTS) {
+        if (parea > (double)DENSE_SLOTS) {
             W = P.Wd;
             nwin = (uint32_t)((P.spanB + W - 1) / W);
         }

@@ -138,9 +138,10 @@ void choose_tiles(fcc_ctx *ctx, const fcc_options &o, const Geom &g, uint64_t pr
     } else if (te && te->row_tile > 0) {
         TL = (uint64_t)te->row_tile;
     } else if (dens >= 0.25) {
+        const uint64_t sp = (g.spanB + 31) & ~31ull;  // dense rows are padded to 32 columns
         uint64_t tf = 1;
-        while (tf * 2 * g.spanB <= DENSE_SLOTS) tf *= 2;
-        if (tf * g.spanB <= DENSE_SLOTS && tf >= 4) TL = std::min<uint64_t>(tf, 64);
+        while (tf * 2 * sp <= DENSE_SLOTS) tf *= 2;
+        if (tf * sp <= DENSE_SLOTS && tf >= 4) TL = std::min<uint64_t>(tf, 64);
         else TL = 32;
     } else {
         double t = (double)HASH_TARGET / std::max(1.0, prow);
@@ -155,10 +156,11 @@ void choose_tiles(fcc_ctx *ctx, const fcc_options &o, const Geom &g, uint64_t pr
     lt = std::min(lt, g.rbits);
     TL = 1ull << lt;
     if (W == 0) {
-        W = 1;
+        W = 32;
         while (W * 2 * TL <= DENSE_SLOTS) W *= 2;
     }
-    if (W * TL > DENSE_SLOTS) W = std::max<uint64_t>(1, DENSE_SLOTS / TL);
+    W = (W + 31) & ~31ull;  // dense windows cover whole 32-column bitmap words
+    while (W > 32 && W * TL > DENSE_SLOTS) W -= 32;
     *logTL = lt;
     *Wd = (uint32_t)W;
 }
@@ -479,8 +481,9 @@ int Contraction::run(const fcc_tensor *ta, const int32_t *am, const fcc_tensor *
     X.fiber_ptr = ix.fiber_ptr;
     X.Ali = ix.Ali;
     X.Aval = ix.Aval;
-    X.Afib = ix.Afib;
+    X.Afp = ix.Afp;
     X.group_ptr = ix.group_ptr;
+    X.gdesc = ix.gdesc;
     X.group_fiber = ix.group_fiber;
     X.atile_id = ix.atile_id;
     X.atile_g0 = ix.atile_g0;

@@ -40,7 +40,9 @@ void fill_f64(double *p, size_t n, double v, hipStream_t s) {
 // out[order] |= 1 if some value is zero, subnormal, or smaller than 2^-511 in
 // magnitude (then a product can round to zero and the accumulators must track
 // touched slots explicitly).
-__global__ void k_extents(const uint32_t *coords, const double *vals, int order, size_t nnz, uint32_t *out) {
+__global__ __launch_bounds__(256) void k_extents(const uint32_t *coords, const double *vals, int order, size_t nnz,
+                                                uint32_t *out) {
+    __shared__ uint32_t s_mx[4], s_fl[4];
     const int m = blockIdx.y;
     uint32_t mx = 0, flag = 0;
     for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < nnz; i += (size_t)gridDim.x * blockDim.x) {
@@ -59,6 +61,15 @@ __global__ void k_extents(const uint32_t *coords, const double *vals, int order,
         flag |= __shfl_xor(flag, d, 64);
     }
     if ((threadIdx.x & 63) == 0) {
+        s_mx[threadIdx.x >> 6] = mx;
+        s_fl[threadIdx.x >> 6] = flag;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int w = 1; w < 4; w++) {
+            mx = s_mx[w] > mx ? s_mx[w] : mx;
+            flag |= s_fl[w];
+        }
         if (m < order) atomicMax(&out[m], mx);
         else if (flag) atomicOr(&out[order], 1u);
     }
@@ -66,7 +77,7 @@ __global__ void k_extents(const uint32_t *coords, const double *vals, int order,
 
 void launch_extents(const Operand &t, uint32_t *dev_out, hipStream_t s) {
     if (t.nnz == 0) return;
-    unsigned gx = (unsigned)std::min<size_t>(div_up((size_t)t.nnz, 256), 1024);
+    unsigned gx = (unsigned)std::min<size_t>(div_up((size_t)t.nnz, 256), 512);
     dim3 grid(gx, t.order + 1);
     k_extents<<<grid, 256, 0, s>>>(t.coords, t.vals, t.order, (size_t)t.nnz, dev_out);
 }
@@ -131,20 +142,22 @@ __device__ inline uint32_t find_fiber(const uint64_t *fk, uint32_t nfib, uint64_
 }
 
 // A join: row, matching fiber (or none), global product count.
-__global__ void k_join_A(const uint32_t *coords, size_t nnz, Geom g, const uint32_t *keymap,
-                         const uint64_t *fiber_key, uint32_t nfib, const uint32_t *fiber_ptr, uint32_t *rowA,
-                         uint32_t *fidA, unsigned long long *products, unsigned int *nvalid) {
-    size_t n = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+__global__ __launch_bounds__(256) void k_join_A(const uint32_t *coords, size_t nnz, Geom g, const uint32_t *keymap,
+                                               const uint64_t *fiber_key, uint32_t nfib, const uint32_t *fiber_ptr,
+                                               uint32_t *rowA, uint32_t *fidA, unsigned long long *products,
+                                               unsigned int *nvalid) {
+    __shared__ unsigned long long s_p[4];
+    __shared__ unsigned int s_v[4];
     uint64_t prod = 0;
     uint32_t valid = 0;
-    if (n < nnz) {
+    for (size_t n = blockIdx.x * (size_t)blockDim.x + threadIdx.x; n < nnz; n += (size_t)gridDim.x * blockDim.x) {
         uint64_t key = lin_key(coords, nnz, n, g.am, g.R, g.r);
         uint32_t f = keymap ? keymap[key] : find_fiber(fiber_key, nfib, key);
         rowA[n] = lin_ext(coords, nnz, n, g.xa, g.xaE, g.nxa);
         fidA[n] = f;
         if (f != 0xFFFFFFFFu) {
-            prod = fiber_ptr[f + 1] - fiber_ptr[f];
-            valid = 1;
+            prod += fiber_ptr[f + 1] - fiber_ptr[f];
+            valid++;
         }
     }
     for (int d = 32; d > 0; d >>= 1) {
@@ -152,6 +165,13 @@ __global__ void k_join_A(const uint32_t *coords, size_t nnz, Geom g, const uint3
         valid += __shfl_xor(valid, d, 64);
     }
     if ((threadIdx.x & 63) == 0) {
+        s_p[threadIdx.x >> 6] = prod;
+        s_v[threadIdx.x >> 6] = valid;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        prod = s_p[0] + s_p[1] + s_p[2] + s_p[3];
+        valid = s_v[0] + s_v[1] + s_v[2] + s_v[3];
         if (prod) atomicAdd(products, (unsigned long long)prod);
         if (valid) atomicAdd(nvalid, valid);
     }
@@ -175,13 +195,16 @@ __global__ void k_comp_A(const uint32_t *rowA, const uint32_t *fidA, const uint3
     idx[n] = (uint32_t)n;
 }
 
-__global__ void k_gather_A(const uint64_t *comp, const uint32_t *idx, const double *vals, size_t n, int logTL,
-                           int fbits, uint32_t *Ali, double *Aval, uint32_t *Afib, uint32_t *gflag, uint32_t *tflag) {
+__global__ void k_gather_A(const uint64_t *comp, const uint32_t *idx, const double *vals, const uint32_t *fiber_ptr,
+                           size_t n, int logTL, int fbits, uint32_t *Ali, double *Aval, uint32_t *Afib, uint2 *Afp,
+                           uint32_t *gflag, uint32_t *tflag) {
     size_t p = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
     if (p >= n) return;
     uint64_t c = comp[p];
     Ali[p] = (uint32_t)(c & ((1ull << logTL) - 1));
-    Afib[p] = (uint32_t)((c >> logTL) & ((1ull << fbits) - 1));
+    uint32_t f = (uint32_t)((c >> logTL) & ((1ull << fbits) - 1));
+    Afib[p] = f;
+    Afp[p] = make_uint2(fiber_ptr[f], fiber_ptr[f + 1]);
     Aval[p] = vals[idx[p]];
     uint64_t prev = p ? comp[p - 1] : ~0ull;
     gflag[p] = (p == 0 || (c >> logTL) != (prev >> logTL)) ? 1u : 0u;
@@ -213,6 +236,15 @@ __global__ void k_groups(const uint64_t *comp, const uint32_t *gflag, const uint
     }
 }
 
+__global__ void k_gdesc(const uint32_t *group_ptr, const uint32_t *group_fiber, const uint32_t *fiber_ptr,
+                        uint32_t ngroups, uint4 *gdesc) {
+    uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= ngroups) return;
+    uint32_t f = group_fiber[g];
+    uint32_t fp0 = fiber_ptr[f], e0 = group_ptr[g];
+    gdesc[g] = make_uint4(fp0, fiber_ptr[f + 1] - fp0, e0, group_ptr[g + 1] - e0);
+}
+
 __global__ void k_tile_light_init(const uint32_t *atile_g0, uint32_t natiles, uint32_t *atile_lg) {
     uint32_t a = blockIdx.x * blockDim.x + threadIdx.x;
     if (a < natiles) atile_lg[a] = atile_g0[a + 1];
@@ -241,10 +273,14 @@ __device__ inline uint32_t hash32(uint32_t x) {
 //  sumfib   summed right-fiber lengths
 //  products sum of m * |fiber| = an upper bound on the tile's output count
 //  sketch   two min-hashes over sampled fiber columns (distinct-column estimate)
-__global__ void k_tile_stats(const uint32_t *group_ptr, const uint32_t *group_fiber, const uint32_t *group_atile,
-                             const uint32_t *fiber_ptr, const uint32_t *Bcol, uint32_t ngroups, uint32_t *st_nnz,
-                             uint32_t *st_keys, unsigned long long *st_sumfib, unsigned long long *st_products,
-                             uint32_t *st_sketch) {
+__global__ __launch_bounds__(256) void k_tile_stats(const uint32_t *group_ptr, const uint32_t *group_fiber,
+                                                   const uint32_t *group_atile, const uint32_t *fiber_ptr,
+                                                   const uint32_t *Bcol, uint32_t ngroups, uint32_t *st_nnz,
+                                                   uint32_t *st_keys, unsigned long long *st_sumfib,
+                                                   unsigned long long *st_products, uint32_t *st_sketch) {
+    __shared__ uint32_t s_a0;
+    __shared__ unsigned long long s_red[4][4];
+    __shared__ uint32_t s_h[4][2];
     uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
     bool ok = g < ngroups;
     uint32_t a = ok ? group_atile[g] : 0xFFFFFFFFu;
@@ -255,7 +291,7 @@ __global__ void k_tile_stats(const uint32_t *group_ptr, const uint32_t *group_fi
         uint32_t f = group_fiber[g];
         uint32_t f0 = fiber_ptr[f], f1 = fiber_ptr[f + 1];
         nf = f1 - f0;
-        // sample up to 4 evenly spaced columns of the fiber
+        // sample up to 4 evenly spaced columns of the fiber (fibers are never empty)
         uint32_t step = (uint32_t)((nf + 3) / 4);
         for (uint32_t p = f0; p < f1; p += step) {
             uint32_t c = Bcol[p];
@@ -264,11 +300,12 @@ __global__ void k_tile_stats(const uint32_t *group_ptr, const uint32_t *group_fi
             h2 = x2 < h2 ? x2 : h2;
         }
     }
-    uint32_t a0 = __shfl(a, 0, 64);
-    bool uniform = __all(a == a0);
-    if (uniform && a0 != 0xFFFFFFFFu) {
-        uint64_t sm = m, snf = nf, sp = m * nf;
-        uint32_t sk = ok ? 1 : 0;
+    if (threadIdx.x == 0) s_a0 = a;
+    __syncthreads();
+    const uint32_t a0 = s_a0;
+    // whole block in one tile (large tiles): one set of atomics per block
+    if (__syncthreads_and(!ok || a == a0) && a0 != 0xFFFFFFFFu) {
+        uint64_t sm = m, snf = nf, sp = m * nf, sk = ok ? 1 : 0;
         for (int d = 32; d > 0; d >>= 1) {
             sm += __shfl_xor(sm, d, 64);
             snf += __shfl_xor(snf, d, 64);
@@ -278,15 +315,35 @@ __global__ void k_tile_stats(const uint32_t *group_ptr, const uint32_t *group_fi
             h1 = o1 < h1 ? o1 : h1;
             h2 = o2 < h2 ? o2 : h2;
         }
+        const int w = threadIdx.x >> 6;
         if ((threadIdx.x & 63) == 0) {
+            s_red[w][0] = sm;
+            s_red[w][1] = snf;
+            s_red[w][2] = sp;
+            s_red[w][3] = sk;
+            s_h[w][0] = h1;
+            s_h[w][1] = h2;
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            for (int v = 1; v < 4; v++) {
+                sm += s_red[v][0];
+                snf += s_red[v][1];
+                sp += s_red[v][2];
+                sk += s_red[v][3];
+                h1 = s_h[v][0] < h1 ? s_h[v][0] : h1;
+                h2 = s_h[v][1] < h2 ? s_h[v][1] : h2;
+            }
             atomicAdd(&st_nnz[a0], (uint32_t)sm);
-            atomicAdd(&st_keys[a0], sk);
+            atomicAdd(&st_keys[a0], (uint32_t)sk);
             atomicAdd(&st_sumfib[a0], (unsigned long long)snf);
             atomicAdd(&st_products[a0], (unsigned long long)sp);
             atomicMin(&st_sketch[2 * a0], h1);
             atomicMin(&st_sketch[2 * a0 + 1], h2);
         }
-    } else if (ok) {
+        return;
+    }
+    if (ok) {
         atomicAdd(&st_nnz[a], (uint32_t)m);
         atomicAdd(&st_keys[a], 1u);
         atomicAdd(&st_sumfib[a], (unsigned long long)nf);
@@ -340,7 +397,7 @@ void join_left(DevPool &pool, const Geom &g, const Operand &A, const Index &ix, 
     }
     Tmp<unsigned long long> cnt(pool, 2);
     fill_u64(cnt, 2, 0, s);
-    k_join_A<<<grid_for(n), 256, 0, s>>>(A.coords, n, g, keymap.get(), ix.fiber_key, ix.nfib, ix.fiber_ptr, rowA,
+    k_join_A<<<std::min(grid_for(n), 2048u), 256, 0, s>>>(A.coords, n, g, keymap.get(), ix.fiber_key, ix.nfib, ix.fiber_ptr, rowA,
                                           fidA, cnt.get(), (unsigned int *)(cnt.get() + 1));
     unsigned long long h[2];
     FCC_HIP(hipMemcpyAsync(h, cnt.get(), 16, hipMemcpyDeviceToHost, s));
@@ -376,7 +433,10 @@ void build_left(DevPool &pool, const Geom &g, const Operand &A, const uint32_t *
     ix.Aval = (double *)keep(m * 8);
     ix.Afib = (uint32_t *)keep(m * 4);
     Tmp<uint32_t> gflag(pool, m), tflag(pool, m), gid(pool, m), aid(pool, m), tot(pool, 2);
-    if (m) k_gather_A<<<grid_for(m), 256, 0, s>>>(ks, vs, A.vals, m, logTL, fbits, ix.Ali, ix.Aval, ix.Afib, gflag, tflag);
+    ix.Afp = (uint2 *)keep(m * 8);
+    if (m)
+        k_gather_A<<<grid_for(m), 256, 0, s>>>(ks, vs, A.vals, ix.fiber_ptr, m, logTL, fbits, ix.Ali, ix.Aval, ix.Afib,
+                                               ix.Afp, gflag, tflag);
     scan_u32(pool, gflag, gid, m, tot.get(), s);
     scan_u32(pool, tflag, aid, m, tot.get() + 1, s);
     uint32_t h[2];
@@ -387,6 +447,7 @@ void build_left(DevPool &pool, const Geom &g, const Operand &A, const uint32_t *
     ix.group_ptr = (uint32_t *)keep((size_t)(ix.ngroups + 1) * 4);
     ix.group_fiber = (uint32_t *)keep((size_t)ix.ngroups * 4);
     ix.group_atile = (uint32_t *)keep((size_t)ix.ngroups * 4);
+    ix.gdesc = (uint4 *)keep((size_t)ix.ngroups * 16);
     Tmp<uint8_t> glight(pool, ix.ngroups + 1);
     ix.atile_id = (uint32_t *)keep((size_t)ix.natiles * 4);
     ix.atile_g0 = (uint32_t *)keep((size_t)(ix.natiles + 1) * 4);
@@ -394,6 +455,7 @@ void build_left(DevPool &pool, const Geom &g, const Operand &A, const uint32_t *
     if (m) {
         k_groups<<<grid_for(m), 256, 0, s>>>(ks, gflag, gid, tflag, aid, m, logTL, fbits, ix.group_ptr, ix.group_fiber,
                                              ix.group_atile, glight, ix.atile_id, ix.atile_g0);
+        k_gdesc<<<grid_for(ix.ngroups), 256, 0, s>>>(ix.group_ptr, ix.group_fiber, ix.fiber_ptr, ix.ngroups, ix.gdesc);
         k_tile_light_init<<<grid_for(ix.natiles), 256, 0, s>>>(ix.atile_g0, ix.natiles, ix.atile_lg);
         k_tile_light<<<grid_for(ix.ngroups), 256, 0, s>>>(glight, ix.group_atile, ix.atile_g0, ix.ngroups, ix.atile_lg);
     }

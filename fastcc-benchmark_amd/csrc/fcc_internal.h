@@ -136,7 +136,9 @@ struct Index {
     uint32_t *Ali = nullptr;   // row within tile
     double *Aval = nullptr;
     uint32_t *Afib = nullptr;  // fiber id per entry
+    uint2 *Afp = nullptr;      // fiber [start, end) per entry
     uint32_t *group_ptr = nullptr;    // ngroups+1
+    uint4 *gdesc = nullptr;           // per group: fiber start, fiber length, first entry, entries
     uint32_t *group_fiber = nullptr;
     uint32_t *group_atile = nullptr;
     uint32_t *atile_id = nullptr;     // tile index of active tile a
@@ -171,8 +173,9 @@ struct AccArgs {
     const uint32_t *fiber_ptr;
     const uint32_t *Ali;
     const double *Aval;
-    const uint32_t *Afib;
+    const uint2 *Afp;
     const uint32_t *group_ptr;
+    const uint4 *gdesc;
     const uint32_t *group_fiber;
     const uint32_t *atile_id;
     const uint32_t *atile_g0;
