@@ -513,9 +513,9 @@ __global__ __launch_bounds__(ACC_THREADS) void k_numeric(AccArgs A) {
                     uint32_t li = key / U.Wp, cl = key - li * U.Wp;
                     emit_output(A, base + rank, U.row0 + li, (uint64_t)U.c0 + cl, s_f64[s], SINK, sk);
                 });
-                if (MODE == 0 && !SINK && threadIdx.x == 0) A.unit_cursor[U.u] = emitted;
+                if (!SINK && threadIdx.x == 0) A.unit_cursor[U.u] = emitted;
             }
-        } else if (MODE == 0 && !SINK && threadIdx.x == 0) {
+        } else if (!SINK && threadIdx.x == 0) {
             A.unit_cursor[U.u] = 0;
         }
         __syncthreads();
@@ -622,6 +622,7 @@ __global__ __launch_bounds__(256) void k_gdrain(AccArgs A) {
             uint32_t woff = 0;
             for (uint32_t w = 0; w < wave; w++) woff += s_w[w];
             if (t && s_base != ~0ull) emit_output(A, s_base + woff + (uint32_t)__popcll(bal & lt), row, col, v, false, sk);
+            if (t) atomicAdd(&A.unit_cursor[A.gunit[h]], 1ull);  // per-unit actual count (planner error)
             __syncthreads();
         }
     }

@@ -269,9 +269,18 @@ __global__ void k_mape(const float *pred, const unsigned long long *actual, uint
     if ((threadIdx.x & 63) == 0 && e != 0.0) unsafeAtomicAdd(acc, e);
 }
 
-__global__ void k_pred_sum(const float *pred, uint32_t n, unsigned long long *sum) {
+// Sum of predicted tile outputs, each capped by its exact bounds (products, area).
+__global__ void k_pred_sum(const float *pred, const uint64_t *st_products, double area, uint32_t n,
+                           unsigned long long *sum) {
     uint32_t a = blockIdx.x * blockDim.x + threadIdx.x;
-    unsigned long long v = a < n ? (unsigned long long)ceilf(pred[a]) : 0ull;
+    unsigned long long v = 0ull;
+    if (a < n) {
+        double p = ceil((double)pred[a]);
+        p = p < 0 ? 0 : p;
+        p = p > (double)st_products[a] ? (double)st_products[a] : p;
+        p = p > area ? area : p;
+        v = (unsigned long long)p;
+    }
     for (int d = 32; d > 0; d >>= 1) v += __shfl_xor(v, d, 64);
     if ((threadIdx.x & 63) == 0 && v) atomicAdd(sum, v);
 }
@@ -606,9 +615,11 @@ int Contraction::run(const fcc_tensor *ta, const int32_t *am, const fcc_tensor *
         // ---- learned planning: predicted capacity, single numeric pass
         Tmp<unsigned long long> psum(pool, 1);
         fill_u64(psum, 1, 0, s);
-        k_pred_sum<<<grid256(T), 256, 0, s>>>(pred_t, T, psum);
+        k_pred_sum<<<grid256(T), 256, 0, s>>>(pred_t, (const uint64_t *)ix.st_products,
+                                               (double)ix.TL * (double)g.spanB, T, psum);
         uint64_t predicted = d2h(psum.get(), s);
-        uint64_t capacity = (uint64_t)((double)predicted * 1.10) + 4096;
+        // never more than the products bound: no tile can emit more outputs than products
+        uint64_t capacity = std::min<uint64_t>((uint64_t)((double)predicted * 1.10) + 4096, products + 16);
         alloc_ghash(1u << 20);
         FCC_HIP(hipEventRecord(ev[3], s));
         for (int attempt = 0;; attempt++) {
@@ -657,7 +668,7 @@ int Contraction::run(const fcc_tensor *ta, const int32_t *am, const fcc_tensor *
         Tmp<double> mp(pool, 1);
         fill_f64(mp, 1, 0.0, s);
         k_mape<<<grid256(T), 256, 0, s>>>(pred_t, tn, T, mp);
-        (void)mp;
+        if (!sink) st.mlp_mape = d2h(mp.get(), s) / (double)T;
     }
 
     // ---- results

@@ -77,8 +77,8 @@ __global__ __launch_bounds__(256) void k_mlp_infer(const float *P, const float *
     }
     float l0 = y[0] * w[OFF_OS] + w[OFF_OM];
     float l1 = y[1] * w[OFF_OS + 1] + w[OFF_OM + 1];
-    pred_nnz[t] = expm1f(fminf(l0, 80.f));
-    if (pred_cols) pred_cols[t] = expm1f(fminf(l1, 80.f));
+    pred_nnz[t] = fmaxf(expm1f(fminf(l0, 80.f)), 0.f);
+    if (pred_cols) pred_cols[t] = fmaxf(expm1f(fminf(l1, 80.f)), 0.f);
 }
 
 // ------------------------------------------------------ forward + backward

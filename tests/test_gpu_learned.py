@@ -1,0 +1,93 @@
+"""Learned planner on the GPU: exactness under any prediction, agreement with
+the exact planner, tile-level error, and the gain from skipping the count pass."""
+import numpy as np
+import pytest
+
+from conftest import assert_same_result
+
+from oracle import oracle
+from fastcc_amd import Context, planner
+from fastcc_amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+REL, ABS = 1e-12, 1e-14
+
+
+@pytest.fixture(scope="module")
+def trained_ctx():
+    ctx = Context(0)
+    fs, ls = [], []
+    for wl, hints in [(W.c1(0.25), (0, 16)), (W.c2(0.25), (0, 8)), (W.c2(0.1), (0,))]:
+        f, l = planner.collect(ctx, wl, hints)
+        fs.append(f)
+        ls.append(l)
+    F, L = np.concatenate(fs), np.concatenate(ls)
+    planner.train(ctx, F, L, epochs=40, batch=128, lr=3e-3, seed=3)
+    yield ctx, F, L
+    ctx.close()
+
+
+def test_training_fits_tile_labels(trained_ctx):
+    ctx, F, L = trained_ctx
+    e_nnz, e_cols = planner.tile_mape(ctx, F, L)
+    print(f"train MAPE nnz {e_nnz:.3f} cols {e_cols:.3f} over {F.shape[0]} tiles")
+    assert e_nnz < 0.5
+
+
+@pytest.mark.parametrize("mk", [lambda: W.c1(0.5), lambda: W.c2(0.25), lambda: W.c3(0.02), lambda: W.c4(0.05)])
+def test_learned_equals_oracle(trained_ctx, mk):
+    ctx, _, _ = trained_ctx
+    wl = mk()
+    print(f"workload={wl.name}", flush=True)
+    a, b = wl.tensors()
+    b = a if wl.self_contraction else b
+    ref = oracle.contract_tiled(a, wl.a_modes, b, wl.b_modes, tile_size=128)
+    r = ctx.contract(a, list(wl.a_modes), b, list(wl.b_modes), planner="learned")
+    c, v = r.to_host()
+    assert_same_result(c, v, ref.coords, ref.vals, rel=REL, abs_floor=ABS, what=wl.name)
+    assert r.stats["ms_symbolic"] == 0.0
+    assert r.stats["mlp_mape"] >= 0.0
+    r.free()
+
+
+@pytest.mark.parametrize("bias", [-30.0, 30.0])
+def test_any_prediction_stays_exact(bias):
+    """Predictions forced far too small (every unit overflows its plan: capacity
+    reruns and LDS-hash spills) or far too large: the result is unchanged."""
+    ctx = Context(0)
+    n = ctx.mlp_param_count
+    p = np.zeros(n, dtype=np.float32)
+    p[-4:-2] = bias           # output mean (log1p space)
+    p[-2:] = 1.0              # output std
+    p[-16 + 6:-4] = 1.0       # input std (inputs are standardised with mean 0, std 1)
+    ctx.set_mlp_params(p)
+    wl = W.c1(0.25)
+    a, b = wl.tensors()
+    ref = oracle.contract_tiled(a, wl.a_modes, b, wl.b_modes, tile_size=64)
+    r = ctx.contract(a, list(wl.a_modes), b, list(wl.b_modes), planner="learned")
+    c, v = r.to_host()
+    assert_same_result(c, v, ref.coords, ref.vals, rel=REL, abs_floor=ABS, what=f"bias {bias}")
+    if bias < 0:
+        assert r.stats["reruns"] >= 1 or r.stats["ghash_entries"] > 0
+    r.free()
+    ctx.close()
+
+
+def test_learned_skips_count_pass_and_is_faster(trained_ctx):
+    """What the trained network gains: the count pass disappears, end-to-end
+    device time drops on a mid-size order-4 self-contraction."""
+    ctx, _, _ = trained_ctx
+    wl = W.c2(0.5)
+    a, _ = wl.tensors()
+    times = {}
+    for pl in ("exact", "learned"):
+        ts = []
+        for _ in range(4):
+            r = ctx.contract(a, list(wl.a_modes), a, list(wl.b_modes), planner=pl)
+            ts.append(r.stats["ms_total"])
+            n = r.nnz
+            r.free()
+        times[pl] = min(ts[1:])
+        print(f"{pl}: {times[pl]:.3f} ms, nnz {n}")
+    assert times["learned"] < times["exact"]
