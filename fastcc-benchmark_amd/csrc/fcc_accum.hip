@@ -64,9 +64,19 @@ __device__ uint32_t wave_lower_bound(const uint32_t *arr, uint32_t lo, uint32_t 
 
 // ---------------------------------------------------------- global hash
 
+// A full table fails fast: after a long probe the fill level is checked and
+// the insert gives up (flag set) once the table is 7/8 full; the host then
+// grows the table or re-plans.
+__device__ __forceinline__ bool gtable_full(const AccArgs &A, int probe) {
+    if (probe < 64 || (probe & 63)) return false;
+    unsigned long long c = __hip_atomic_load(A.gcount, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    return c > ((A.gmask + 1) >> 3) * 7;
+}
+
 __device__ bool ghash_add(const AccArgs &A, uint64_t key, double v, uint32_t unit) {
     uint64_t h = mix64(key) & A.gmask;
     for (int probe = 0; probe < 8192; probe++) {
+        if (gtable_full(A, probe)) break;
         unsigned long long old = atomicCAS(&A.gkeys[h], GEMPTY, (unsigned long long)key);
         if (old == GEMPTY) {
             A.gunit[h] = unit;
@@ -88,6 +98,7 @@ __device__ bool ghash_add(const AccArgs &A, uint64_t key, double v, uint32_t uni
 __device__ void gset_insert(const AccArgs &A, uint64_t key, uint32_t unit) {
     uint64_t h = mix64(key) & A.gmask;
     for (int probe = 0; probe < 8192; probe++) {
+        if (gtable_full(A, probe)) break;
         unsigned long long old = atomicCAS(&A.gkeys[h], GEMPTY, (unsigned long long)key);
         if (old == GEMPTY) {
             atomicAdd(&A.unit_count[unit], 1ull);
@@ -646,17 +657,19 @@ __global__ __launch_bounds__(256) void k_gdrain(AccArgs A) {
 
 // Per active tile: accumulator kind and number of column windows.
 // est = exact planner: the products bound; learned planner: predicted outputs.
-__global__ void k_plan_tiles(const uint64_t *st_products, const float *pred, uint32_t natiles, PlanParams P,
-                             uint32_t *t_nwin, uint32_t *t_W, uint8_t *t_kind) {
+__global__ void k_plan_tiles(const uint64_t *st_products, const uint64_t *st_maxgp, const float *pred,
+                             uint32_t natiles, PlanParams P, uint32_t *t_nwin, uint32_t *t_W, uint8_t *t_kind) {
     uint32_t a = blockIdx.x * blockDim.x + threadIdx.x;
     if (a >= natiles) return;
     const double prod = (double)st_products[a];
+    const double lb = (double)st_maxgp[a];  // one group's products are distinct outputs
     const double S = (double)P.spanB;
     const double area = (double)P.TL * S;
     // dense rows are padded to a multiple of 32 columns
     const double parea = (double)P.TL * (double)((P.spanB + 31) & ~31ull);
     const bool dense_ok = (uint64_t)P.TL * 32 <= DENSE_SLOTS;
     double est = pred ? (double)pred[a] * (1.0 + P.margin) + 8.0 : prod;
+    if (est < lb) est = lb;
     if (est > prod) est = prod;
     if (est > area) est = area;
     uint32_t kind, nwin = 1;
@@ -726,8 +739,9 @@ __global__ void k_refine_kinds(const unsigned long long *unit_count, uint32_t n,
 void launch_plan_tiles(const Index &ix, const float *pred, const PlanParams &P, uint32_t *t_nwin, uint32_t *t_W,
                        uint8_t *t_kind, hipStream_t s) {
     if (!ix.natiles) return;
-    k_plan_tiles<<<(unsigned)div_up(ix.natiles, 256), 256, 0, s>>>((const uint64_t *)ix.st_products, pred,
-                                                                     ix.natiles, P, t_nwin, t_W, t_kind);
+    k_plan_tiles<<<(unsigned)div_up(ix.natiles, 256), 256, 0, s>>>((const uint64_t *)ix.st_products,
+                                                                     (const uint64_t *)ix.st_maxgp, pred, ix.natiles,
+                                                                     P, t_nwin, t_W, t_kind);
 }
 
 void launch_emit_units(const uint32_t *t_nwin, const uint32_t *t_W, const uint8_t *t_kind, const uint32_t *t_ubase,

@@ -249,12 +249,15 @@ __global__ __launch_bounds__(256) void k_tile_cols(const uint32_t *atile_g0, con
     }
 }
 
-__global__ void k_labels(const unsigned long long *tile_nnz, const unsigned long long *tile_cols, uint32_t n,
-                         float *labels) {
+// Training labels: gaps (<= 0) between the log1p of the exact tile output
+// nnz / distinct columns and their upper bounds taken from the features.
+__global__ void k_labels(const unsigned long long *tile_nnz, const unsigned long long *tile_cols, const float *feat,
+                         uint32_t n, float *labels) {
     uint32_t a = blockIdx.x * blockDim.x + threadIdx.x;
     if (a >= n) return;
-    labels[2 * a] = log1pf((float)tile_nnz[a]);
-    labels[2 * a + 1] = log1pf((float)tile_cols[a]);
+    const float *f = feat + (size_t)a * MLP_IN;
+    labels[2 * a] = fminf(log1pf((float)tile_nnz[a]) - fminf(f[3], f[5]), 0.f);
+    labels[2 * a + 1] = fminf(log1pf((float)tile_cols[a]) - f[2], 0.f);
 }
 
 // MAPE of predicted vs actual tile nnz: sum |p - a| / max(a, 1)
@@ -270,13 +273,13 @@ __global__ void k_mape(const float *pred, const unsigned long long *actual, uint
 }
 
 // Sum of predicted tile outputs, each capped by its exact bounds (products, area).
-__global__ void k_pred_sum(const float *pred, const uint64_t *st_products, double area, uint32_t n,
-                           unsigned long long *sum) {
+__global__ void k_pred_sum(const float *pred, const uint64_t *st_products, const uint64_t *st_maxgp, double area,
+                           uint32_t n, unsigned long long *sum) {
     uint32_t a = blockIdx.x * blockDim.x + threadIdx.x;
     unsigned long long v = 0ull;
     if (a < n) {
         double p = ceil((double)pred[a]);
-        p = p < 0 ? 0 : p;
+        p = p < (double)st_maxgp[a] ? (double)st_maxgp[a] : p;
         p = p > (double)st_products[a] ? (double)st_products[a] : p;
         p = p > area ? area : p;
         v = (unsigned long long)p;
@@ -397,11 +400,21 @@ int Contraction::run(const fcc_tensor *ta, const int32_t *am, const fcc_tensor *
     // ---- index
     std::vector<std::shared_ptr<void>> hold;
     Index ix;
-    build_right(pool, g, B, ix, hold, s);
     Tmp<uint32_t> rowA(pool, (size_t)A.nnz), fidA(pool, (size_t)A.nnz);
     uint64_t products = 0;
     uint32_t nvalid = 0;
-    join_left(pool, g, A, ix, rowA, fidA, &products, &nvalid, s);
+    bool self = A.coords == B.coords && A.vals == B.vals && ta->order == tb->order && ta->nnz == tb->nnz;
+    for (int i = 0; i < r && self; i++) self = am[i] == bm[i];
+    for (int m = 0; m < ta->order && self; m++) self = EA[m] == EB[m];
+    if (self) {
+        SelfJoin sj{rowA, fidA, 0};
+        build_right(pool, g, B, ix, hold, s, &sj);
+        products = sj.products;
+        nvalid = (uint32_t)A.nnz;
+    } else {
+        build_right(pool, g, B, ix, hold, s);
+        join_left(pool, g, A, ix, rowA, fidA, &products, &nvalid, s);
+    }
     if (o.left_tile_stride > 1) {
         uint64_t T = pow2_ceil(o.tile_hint > 0 ? (uint64_t)o.tile_hint : 128);
         int lT = ceil_log2(T);
@@ -461,26 +474,12 @@ int Contraction::run(const fcc_tensor *ta, const int32_t *am, const fcc_tensor *
     P.spanB = g.spanB;
     P.Wd = Wd;
     P.margin = 0.15f;
-    P.dense_ratio = learned ? 0.15f : 0.5f;
     Tmp<uint32_t> t_nwin(pool, T), t_W(pool, T), t_ubase(pool, T), d_nunits(pool, 1);
     Tmp<uint8_t> t_kind(pool, T);
-    launch_plan_tiles(ix, learned ? pred_t.get() : nullptr, P, t_nwin, t_W, t_kind, s);
-    scan_u32(pool, t_nwin, t_ubase, T, d_nunits, s);
     Units U;
-    U.n = d2h(d_nunits.get(), s);
-    Tmp<uint32_t> u_tile(pool, U.n), u_c0(pool, U.n), u_W(pool, U.n);
-    Tmp<uint8_t> u_kind(pool, U.n), u_multi(pool, U.n);
-    Tmp<float> u_pred;
-    if (learned) u_pred = Tmp<float>(pool, U.n);
-    U.tile = u_tile;
-    U.c0 = u_c0;
-    U.W = u_W;
-    U.kind = u_kind;
-    U.multi = u_multi;
-    U.pred = u_pred.get();
-    launch_emit_units(t_nwin, t_W, t_kind, t_ubase, T, g.spanB, learned ? pred_t.get() : nullptr, U, s);
-    st.units = U.n;
-    FCC_HIP(hipEventRecord(ev[2], s));
+    Tmp<uint32_t> u_tile, u_c0, u_W;
+    Tmp<uint8_t> u_kind, u_multi;
+    Tmp<unsigned long long> unit_count, unit_off, unit_cur, d_total(pool, 1);
 
     // ---- accumulate arguments
     AccArgs X;
@@ -497,12 +496,6 @@ int Contraction::run(const fcc_tensor *ta, const int32_t *am, const fcc_tensor *
     X.atile_id = ix.atile_id;
     X.atile_g0 = ix.atile_g0;
     X.atile_lg = ix.atile_lg;
-    X.nunits = U.n;
-    X.unit_tile = U.tile;
-    X.unit_c0 = U.c0;
-    X.unit_W = U.W;
-    X.unit_kind = U.kind;
-    X.unit_multi = U.multi;
     X.unit_order = nullptr;
     X.logTL = ix.logTL;
     X.spanB = g.spanB;
@@ -522,8 +515,44 @@ int Contraction::run(const fcc_tensor *ta, const int32_t *am, const fcc_tensor *
     X.fail_flag = flags.get() + 1;
     X.sink = sinkbuf.get();
     X.sinkf = (double *)(sinkbuf.get() + 2);
-    unsigned grid = (unsigned)(o.workgroups > 0 ? o.workgroups : ctx->num_cus * 2);
-    grid = std::max(1u, std::min<unsigned>(grid, U.n));
+    unsigned grid = 1;
+
+    // units from the exact bounds (pred == null) or from predictions
+    auto make_units = [&](const float *pred) {
+        P.dense_ratio = pred ? 0.15f : 0.5f;
+        launch_plan_tiles(ix, pred, P, t_nwin, t_W, t_kind, s);
+        scan_u32(pool, t_nwin, t_ubase, T, d_nunits, s);
+        U.n = d2h(d_nunits.get(), s);
+        u_tile = Tmp<uint32_t>(pool, U.n);
+        u_c0 = Tmp<uint32_t>(pool, U.n);
+        u_W = Tmp<uint32_t>(pool, U.n);
+        u_kind = Tmp<uint8_t>(pool, U.n);
+        u_multi = Tmp<uint8_t>(pool, U.n);
+        U.tile = u_tile;
+        U.c0 = u_c0;
+        U.W = u_W;
+        U.kind = u_kind;
+        U.multi = u_multi;
+        U.pred = nullptr;
+        launch_emit_units(t_nwin, t_W, t_kind, t_ubase, T, g.spanB, pred, U, s);
+        unit_count = Tmp<unsigned long long>(pool, U.n);
+        unit_off = Tmp<unsigned long long>(pool, U.n);
+        unit_cur = Tmp<unsigned long long>(pool, U.n);
+        X.nunits = U.n;
+        X.unit_tile = U.tile;
+        X.unit_c0 = U.c0;
+        X.unit_W = U.W;
+        X.unit_kind = U.kind;
+        X.unit_multi = U.multi;
+        X.unit_count = unit_count;
+        X.unit_off = unit_off;
+        X.unit_cursor = unit_cur;
+        grid = (unsigned)(o.workgroups > 0 ? o.workgroups : ctx->num_cus * 2);
+        grid = std::max(1u, std::min<unsigned>(grid, U.n));
+        st.units = U.n;
+    };
+    make_units(learned ? pred_t.get() : nullptr);
+    FCC_HIP(hipEventRecord(ev[2], s));
 
     Tmp<unsigned long long> gk;
     Tmp<double> gv;
@@ -546,77 +575,18 @@ int Contraction::run(const fcc_tensor *ta, const int32_t *am, const fcc_tensor *
         fill_u32(work, 4, 0, s);
     };
 
-    Tmp<unsigned long long> unit_count(pool, U.n), unit_off(pool, U.n), unit_cur(pool, U.n), d_total(pool, 1);
-    X.unit_count = unit_count;
-    X.unit_off = unit_off;
-    X.unit_cursor = unit_cur;
     Tmp<uint32_t> out_c;
     Tmp<double> out_v;
     uint64_t stride = 0;
+    bool use_exact = !learned;
 
-    if (!learned) {
-        // ---- exact symbolic planning: count, scan, then fill
-        alloc_ghash(1u << 16);
-        for (int attempt = 0;; attempt++) {
-            reset_ghash();
-            fill_u64(unit_count, U.n, 0, s);
-            launch_count(X, grid, s);
-            uint32_t gf = d2h(flags.get(), s);
-            if (!gf) break;
-            if (attempt >= 3) throw LibError{FCC_ERR_OVERFLOW, "count-pass global set overflow"};
-            alloc_ghash((X.gmask + 1) * 16);
-        }
-        launch_refine_kinds(unit_count, U.n, o.force_acc, U.kind, gmeta.get() + 2, s);
-        scan_u64(pool, (const uint64_t *)unit_count.get(), (uint64_t *)unit_off.get(), U.n, (uint64_t *)d_total.get(),
-                 s);
-        unsigned long long hm[4];
-        FCC_HIP(hipMemcpyAsync(hm, gmeta.get(), 32, hipMemcpyDeviceToHost, s));
-        nnz_out = d2h(d_total.get(), s);
-        uint64_t gtotal = hm[2];
-        FCC_HIP(hipEventRecord(ev[3], s));
-        if (o.collect_stats) {
-            // labels: exact tile nnz and distinct output columns
-            Tmp<unsigned long long> tn(pool, T), tc(pool, T);
-            fill_u64(tn, T, 0, s);
-            k_unit_to_tile<<<grid256(U.n), 256, 0, s>>>(unit_count, U.tile, U.n, tn);
-            k_tile_cols<<<std::min<unsigned>(T, 4096), 256, 0, s>>>(ix.atile_g0, ix.group_fiber, ix.fiber_ptr,
-                                                                     ix.Bcol, T, g.spanB, tc);
-            Tmp<float> lab(pool, (size_t)T * 2);
-            k_labels<<<grid256(T), 256, 0, s>>>(tn, tc, T, lab);
-            ctx->feat.resize((size_t)T * MLP_IN);
-            ctx->labels.resize((size_t)T * 2);
-            FCC_HIP(hipMemcpyAsync(ctx->feat.data(), feat.get(), ctx->feat.size() * 4, hipMemcpyDeviceToHost, s));
-            FCC_HIP(hipMemcpyAsync(ctx->labels.data(), lab.get(), ctx->labels.size() * 4, hipMemcpyDeviceToHost, s));
-            FCC_HIP(hipStreamSynchronize(s));
-        }
-        if (!sink) {
-            out_c = Tmp<uint32_t>(pool, (size_t)g.dC * nnz_out);
-            out_v = Tmp<double>(pool, (size_t)nnz_out);
-        }
-        stride = nnz_out;
-        X.ocoords = out_c.get();
-        X.ovals = out_v.get();
-        X.ostride = stride;
-        X.capacity = nnz_out;
-        if (gtotal > (X.gmask + 1) / 2) alloc_ghash(2 * gtotal);
-        reset_ghash();
-        FCC_HIP(hipEventRecord(ev[4], s));
-        launch_numeric(X, grid, bitmap, sink, 0, s);
-        FCC_HIP(hipEventRecord(ev[5], s));
-        launch_gdrain(X, sink, 0, s);
-        FCC_HIP(hipEventRecord(ev[6], s));
-        uint32_t hf[2];
-        FCC_HIP(hipMemcpyAsync(hf, flags.get(), 8, hipMemcpyDeviceToHost, s));
-        FCC_HIP(hipStreamSynchronize(s));
-        if (hf[0]) throw LibError{FCC_ERR_OVERFLOW, "global hash overflow in the numeric pass"};
-        if (hf[1]) throw LibError{FCC_ERR_OVERFLOW, "numeric pass disagreed with the count pass"};
-        st.ghash_entries = d2h(gmeta.get(), s);
-    } else {
+    if (learned) {
         // ---- learned planning: predicted capacity, single numeric pass
         Tmp<unsigned long long> psum(pool, 1);
         fill_u64(psum, 1, 0, s);
         k_pred_sum<<<grid256(T), 256, 0, s>>>(pred_t, (const uint64_t *)ix.st_products,
-                                               (double)ix.TL * (double)g.spanB, T, psum);
+                                               (const uint64_t *)ix.st_maxgp, (double)ix.TL * (double)g.spanB, T,
+                                               psum);
         uint64_t predicted = d2h(psum.get(), s);
         // never more than the products bound: no tile can emit more outputs than products
         uint64_t capacity = std::min<uint64_t>((uint64_t)((double)predicted * 1.10) + 4096, products + 16);
@@ -645,11 +615,13 @@ int Contraction::run(const fcc_tensor *ta, const int32_t *am, const fcc_tensor *
             FCC_HIP(hipStreamSynchronize(s));
             st.ghash_entries = hm[0];
             if (hf[0]) {
-                // global hash exhausted: grow it and run again
-                if (attempt >= 4) throw LibError{FCC_ERR_OVERFLOW, "global hash overflow in the learned pass"};
-                alloc_ghash((X.gmask + 1) * 16);
+                // the predictions left more spill than the global hash holds:
+                // fall back to the exact planner (re-planned from the bounds)
                 st.reruns++;
-                continue;
+                use_exact = true;
+                out_c.reset();
+                out_v.reset();
+                break;
             }
             if (hf[1]) {
                 capacity = hm[1];
@@ -660,15 +632,79 @@ int Contraction::run(const fcc_tensor *ta, const int32_t *am, const fcc_tensor *
             nnz_out = sink ? 0 : hm[1];
             break;
         }
+        if (!use_exact) {
+            FCC_HIP(hipEventRecord(ev[6], s));
+            // tile-level MAPE against the actual per-unit outputs
+            Tmp<unsigned long long> tn(pool, T);
+            fill_u64(tn, T, 0, s);
+            k_unit_to_tile<<<grid256(U.n), 256, 0, s>>>(unit_cur, U.tile, U.n, tn);
+            Tmp<double> mp(pool, 1);
+            fill_f64(mp, 1, 0.0, s);
+            k_mape<<<grid256(T), 256, 0, s>>>(pred_t, tn, T, mp);
+            if (!sink) st.mlp_mape = d2h(mp.get(), s) / (double)T;
+        }
+    }
+    if (use_exact) {
+        if (learned) {
+            make_units(nullptr);
+            fill_u64(sinkbuf, 4, 0, s);
+        }
+        // ---- exact symbolic planning: count, scan, then fill
+        alloc_ghash(1u << 16);
+        for (int attempt = 0;; attempt++) {
+            reset_ghash();
+            fill_u64(unit_count, U.n, 0, s);
+            launch_count(X, grid, s);
+            uint32_t gf = d2h(flags.get(), s);
+            if (!gf) break;
+            if (attempt >= 3) throw LibError{FCC_ERR_OVERFLOW, "count-pass global set overflow"};
+            alloc_ghash((X.gmask + 1) * 16);
+        }
+        launch_refine_kinds(unit_count, U.n, o.force_acc, U.kind, gmeta.get() + 2, s);
+        scan_u64(pool, (const uint64_t *)unit_count.get(), (uint64_t *)unit_off.get(), U.n, (uint64_t *)d_total.get(),
+                 s);
+        unsigned long long hm[4];
+        FCC_HIP(hipMemcpyAsync(hm, gmeta.get(), 32, hipMemcpyDeviceToHost, s));
+        nnz_out = d2h(d_total.get(), s);
+        uint64_t gtotal = hm[2];
+        FCC_HIP(hipEventRecord(ev[3], s));
+        if (o.collect_stats) {
+            // labels: exact tile nnz and distinct output columns
+            Tmp<unsigned long long> tn(pool, T), tc(pool, T);
+            fill_u64(tn, T, 0, s);
+            k_unit_to_tile<<<grid256(U.n), 256, 0, s>>>(unit_count, U.tile, U.n, tn);
+            k_tile_cols<<<std::min<unsigned>(T, 4096), 256, 0, s>>>(ix.atile_g0, ix.group_fiber, ix.fiber_ptr,
+                                                                     ix.Bcol, T, g.spanB, tc);
+            Tmp<float> lab(pool, (size_t)T * 2);
+            k_labels<<<grid256(T), 256, 0, s>>>(tn, tc, feat, T, lab);
+            ctx->feat.resize((size_t)T * MLP_IN);
+            ctx->labels.resize((size_t)T * 2);
+            FCC_HIP(hipMemcpyAsync(ctx->feat.data(), feat.get(), ctx->feat.size() * 4, hipMemcpyDeviceToHost, s));
+            FCC_HIP(hipMemcpyAsync(ctx->labels.data(), lab.get(), ctx->labels.size() * 4, hipMemcpyDeviceToHost, s));
+            FCC_HIP(hipStreamSynchronize(s));
+        }
+        if (!sink) {
+            out_c = Tmp<uint32_t>(pool, (size_t)g.dC * nnz_out);
+            out_v = Tmp<double>(pool, (size_t)nnz_out);
+        }
+        stride = nnz_out;
+        X.ocoords = out_c.get();
+        X.ovals = out_v.get();
+        X.ostride = stride;
+        X.capacity = nnz_out;
+        if (gtotal > (X.gmask + 1) / 2) alloc_ghash(2 * gtotal);
+        reset_ghash();
+        FCC_HIP(hipEventRecord(ev[4], s));
+        launch_numeric(X, grid, bitmap, sink, 0, s);
+        FCC_HIP(hipEventRecord(ev[5], s));
+        launch_gdrain(X, sink, 0, s);
         FCC_HIP(hipEventRecord(ev[6], s));
-        // tile-level MAPE against the actual per-unit outputs (LDS part)
-        Tmp<unsigned long long> tn(pool, T);
-        fill_u64(tn, T, 0, s);
-        k_unit_to_tile<<<grid256(U.n), 256, 0, s>>>(unit_cur, U.tile, U.n, tn);
-        Tmp<double> mp(pool, 1);
-        fill_f64(mp, 1, 0.0, s);
-        k_mape<<<grid256(T), 256, 0, s>>>(pred_t, tn, T, mp);
-        if (!sink) st.mlp_mape = d2h(mp.get(), s) / (double)T;
+        uint32_t hf[2];
+        FCC_HIP(hipMemcpyAsync(hf, flags.get(), 8, hipMemcpyDeviceToHost, s));
+        FCC_HIP(hipStreamSynchronize(s));
+        if (hf[0]) throw LibError{FCC_ERR_OVERFLOW, "global hash overflow in the numeric pass"};
+        if (hf[1]) throw LibError{FCC_ERR_OVERFLOW, "numeric pass disagreed with the count pass"};
+        st.ghash_entries = d2h(gmeta.get(), s);
     }
 
     // ---- results
@@ -726,7 +762,7 @@ int Contraction::run(const fcc_tensor *ta, const int32_t *am, const fcc_tensor *
     }
     st.ms_index = ev_ms(ev[0], ev[1]);
     st.ms_plan = ev_ms(ev[1], ev[2]);
-    st.ms_symbolic = learned ? 0.0 : ev_ms(ev[2], ev[3]);
+    st.ms_symbolic = (learned && !use_exact) ? 0.0 : ev_ms(ev[2], ev[3]);
     st.ms_numeric = ev_ms(ev[4], ev[5]);
     st.ms_numeric_kernel = st.ms_numeric;
     st.ms_overflow = ev_ms(ev[5], ev[6]);

@@ -12,9 +12,17 @@ void fill_f64(double *p, size_t n, double v, hipStream_t s);
 // is zero or tiny. out must be zeroed by the caller.
 void launch_extents(const Operand &t, uint32_t *dev_out, hipStream_t s);
 
-// Right operand: linearise, sort by (key, col), fibers. Synchronises to read nfib.
+// Left join read off the right operand's sort (self-contraction).
+struct SelfJoin {
+    uint32_t *rowA;
+    uint32_t *fidA;
+    uint64_t products;
+};
+
+// Right operand: linearise, sort by (key, col), fibers. Synchronises to read
+// nfib. With self != null also fills the left join of a self-contraction.
 void build_right(DevPool &pool, const Geom &g, const Operand &B, Index &ix,
-                 std::vector<std::shared_ptr<void>> &hold, hipStream_t s);
+                 std::vector<std::shared_ptr<void>> &hold, hipStream_t s, SelfJoin *self = nullptr);
 
 // Left operand join: row and matching fiber per entry; total products and
 // matched entry count (synchronises).

@@ -277,10 +277,12 @@ __global__ __launch_bounds__(256) void k_tile_stats(const uint32_t *group_ptr, c
                                                    const uint32_t *group_atile, const uint32_t *fiber_ptr,
                                                    const uint32_t *Bcol, uint32_t ngroups, uint32_t *st_nnz,
                                                    uint32_t *st_keys, unsigned long long *st_sumfib,
-                                                   unsigned long long *st_products, uint32_t *st_sketch) {
+                                                   unsigned long long *st_products, uint32_t *st_sketch,
+                                                   unsigned long long *st_maxgp) {
     __shared__ uint32_t s_a0;
     __shared__ unsigned long long s_red[4][4];
     __shared__ uint32_t s_h[4][2];
+    __shared__ unsigned long long s_mx[4];
     uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
     bool ok = g < ngroups;
     uint32_t a = ok ? group_atile[g] : 0xFFFFFFFFu;
@@ -305,8 +307,10 @@ __global__ __launch_bounds__(256) void k_tile_stats(const uint32_t *group_ptr, c
     const uint32_t a0 = s_a0;
     // whole block in one tile (large tiles): one set of atomics per block
     if (__syncthreads_and(!ok || a == a0) && a0 != 0xFFFFFFFFu) {
-        uint64_t sm = m, snf = nf, sp = m * nf, sk = ok ? 1 : 0;
+        uint64_t sm = m, snf = nf, sp = m * nf, sk = ok ? 1 : 0, mx = m * nf;
         for (int d = 32; d > 0; d >>= 1) {
+            uint64_t om = __shfl_xor(mx, d, 64);
+            mx = om > mx ? om : mx;
             sm += __shfl_xor(sm, d, 64);
             snf += __shfl_xor(snf, d, 64);
             sp += __shfl_xor(sp, d, 64);
@@ -323,6 +327,7 @@ __global__ __launch_bounds__(256) void k_tile_stats(const uint32_t *group_ptr, c
             s_red[w][3] = sk;
             s_h[w][0] = h1;
             s_h[w][1] = h2;
+            s_mx[w] = mx;
         }
         __syncthreads();
         if (threadIdx.x == 0) {
@@ -333,7 +338,9 @@ __global__ __launch_bounds__(256) void k_tile_stats(const uint32_t *group_ptr, c
                 sk += s_red[v][3];
                 h1 = s_h[v][0] < h1 ? s_h[v][0] : h1;
                 h2 = s_h[v][1] < h2 ? s_h[v][1] : h2;
+                mx = s_mx[v] > mx ? s_mx[v] : mx;
             }
+            atomicMax(&st_maxgp[a0], (unsigned long long)mx);
             atomicAdd(&st_nnz[a0], (uint32_t)sm);
             atomicAdd(&st_keys[a0], (uint32_t)sk);
             atomicAdd(&st_sumfib[a0], (unsigned long long)snf);
@@ -350,6 +357,7 @@ __global__ __launch_bounds__(256) void k_tile_stats(const uint32_t *group_ptr, c
         atomicAdd(&st_products[a], (unsigned long long)(m * nf));
         atomicMin(&st_sketch[2 * a], h1);
         atomicMin(&st_sketch[2 * a + 1], h2);
+        atomicMax(&st_maxgp[a], (unsigned long long)(m * nf));
     }
 }
 
@@ -357,8 +365,32 @@ __global__ __launch_bounds__(256) void k_tile_stats(const uint32_t *group_ptr, c
 
 static unsigned grid_for(size_t n, unsigned bs = 256) { return (unsigned)div_up(n ? n : 1, bs); }
 
+// Self-contraction (same tensor, same mode lists): the left join is read off
+// the right operand's sort. Sorted position p holds original entry vs[p], its
+// column (= the left row) and its fiber.
+__global__ __launch_bounds__(256) void k_self_join(const uint32_t *vs, const uint32_t *flag, const uint32_t *fid,
+                                                  const uint32_t *Bcol, const uint32_t *fiber_ptr, size_t n,
+                                                  uint32_t *rowA, uint32_t *fidA, unsigned long long *products) {
+    __shared__ unsigned long long s_p[4];
+    uint64_t prod = 0;
+    for (size_t p = blockIdx.x * (size_t)blockDim.x + threadIdx.x; p < n; p += (size_t)gridDim.x * blockDim.x) {
+        uint32_t f = fid[p] + flag[p] - 1;
+        uint32_t o = vs[p];
+        rowA[o] = Bcol[p];
+        fidA[o] = f;
+        prod += fiber_ptr[f + 1] - fiber_ptr[f];
+    }
+    for (int d = 32; d > 0; d >>= 1) prod += __shfl_xor(prod, d, 64);
+    if ((threadIdx.x & 63) == 0) s_p[threadIdx.x >> 6] = prod;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        prod = s_p[0] + s_p[1] + s_p[2] + s_p[3];
+        if (prod) atomicAdd(products, (unsigned long long)prod);
+    }
+}
+
 void build_right(DevPool &pool, const Geom &g, const Operand &B, Index &ix, std::vector<std::shared_ptr<void>> &hold,
-                 hipStream_t s) {
+                 hipStream_t s, SelfJoin *self) {
     size_t n = (size_t)B.nnz;
     ix.nB = (uint32_t)n;
     auto keep = [&](size_t bytes) -> void * {
@@ -382,23 +414,113 @@ void build_right(DevPool &pool, const Geom &g, const Operand &B, Index &ix, std:
     ix.fiber_ptr = (uint32_t *)keep((n + 1) * 4);
     ix.fiber_key = (uint64_t *)keep(n * 8);
     k_fibers<<<grid_for(n), 256, 0, s>>>(ks, flag, fid, n, g.cbits, ix.fiber_ptr, ix.fiber_key);
+    if (self) {
+        Tmp<unsigned long long> prod(pool, 1);
+        fill_u64(prod, 1, 0, s);
+        k_self_join<<<std::min(grid_for(n), 2048u), 256, 0, s>>>(vs, flag, fid, ix.Bcol, ix.fiber_ptr, n, self->rowA,
+                                                                  self->fidA, prod);
+        FCC_HIP(hipMemcpyAsync(&self->products, prod.get(), 8, hipMemcpyDeviceToHost, s));
+    }
     FCC_HIP(hipMemcpyAsync(&ix.nfib, total.get(), 4, hipMemcpyDeviceToHost, s));
     FCC_HIP(hipStreamSynchronize(s));
+}
+
+// key -> fiber hash map for key spaces too large for a direct table
+__device__ __forceinline__ uint64_t hmix(uint64_t z) {
+    z ^= z >> 33;
+    z *= 0xff51afd7ed558ccdull;
+    z ^= z >> 33;
+    z *= 0xc4ceb9fe1a85ec53ull;
+    return z ^ (z >> 33);
+}
+
+__global__ void k_hmap_build(const uint64_t *fiber_key, uint32_t nfib, unsigned long long *hk, uint32_t *hv,
+                             uint64_t mask) {
+    uint32_t f = blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= nfib) return;
+    unsigned long long key = fiber_key[f];
+    uint64_t h = hmix(key) & mask;
+    for (;;) {  // keys are unique and the table is at most half full: terminates
+        unsigned long long old = atomicCAS(&hk[h], ~0ull, key);
+        if (old == ~0ull) {
+            hv[h] = f;
+            return;
+        }
+        h = (h + 1) & mask;
+    }
+}
+
+__device__ __forceinline__ uint32_t hmap_find(const unsigned long long *hk, const uint32_t *hv, uint64_t mask,
+                                              uint64_t key) {
+    uint64_t h = hmix(key) & mask;
+    for (;;) {
+        unsigned long long k = hk[h];
+        if (k == key) return hv[h];
+        if (k == ~0ull) return 0xFFFFFFFFu;
+        h = (h + 1) & mask;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_join_A_hash(const uint32_t *coords, size_t nnz, Geom g,
+                                                    const unsigned long long *hk, const uint32_t *hv, uint64_t mask,
+                                                    const uint32_t *fiber_ptr, uint32_t *rowA, uint32_t *fidA,
+                                                    unsigned long long *products, unsigned int *nvalid) {
+    __shared__ unsigned long long s_p[4];
+    __shared__ unsigned int s_v[4];
+    uint64_t prod = 0;
+    uint32_t valid = 0;
+    for (size_t n = blockIdx.x * (size_t)blockDim.x + threadIdx.x; n < nnz; n += (size_t)gridDim.x * blockDim.x) {
+        uint64_t key = lin_key(coords, nnz, n, g.am, g.R, g.r);
+        uint32_t f = hmap_find(hk, hv, mask, key);
+        rowA[n] = lin_ext(coords, nnz, n, g.xa, g.xaE, g.nxa);
+        fidA[n] = f;
+        if (f != 0xFFFFFFFFu) {
+            prod += fiber_ptr[f + 1] - fiber_ptr[f];
+            valid++;
+        }
+    }
+    for (int d = 32; d > 0; d >>= 1) {
+        prod += __shfl_xor(prod, d, 64);
+        valid += __shfl_xor(valid, d, 64);
+    }
+    if ((threadIdx.x & 63) == 0) {
+        s_p[threadIdx.x >> 6] = prod;
+        s_v[threadIdx.x >> 6] = valid;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        prod = s_p[0] + s_p[1] + s_p[2] + s_p[3];
+        valid = s_v[0] + s_v[1] + s_v[2] + s_v[3];
+        if (prod) atomicAdd(products, (unsigned long long)prod);
+        if (valid) atomicAdd(nvalid, valid);
+    }
 }
 
 void join_left(DevPool &pool, const Geom &g, const Operand &A, const Index &ix, uint32_t *rowA, uint32_t *fidA,
                uint64_t *products, uint32_t *nvalid, hipStream_t s) {
     size_t n = (size_t)A.nnz;
     Tmp<uint32_t> keymap;
-    if (g.K <= (1ull << 26)) {
+    if (g.K <= (1ull << 26) && ix.nfib) {
         keymap = Tmp<uint32_t>(pool, g.K);
         fill_u32(keymap, g.K, 0xFFFFFFFFu, s);
         if (ix.nfib) k_keymap<<<grid_for(ix.nfib), 256, 0, s>>>(ix.fiber_key, ix.nfib, keymap);
     }
     Tmp<unsigned long long> cnt(pool, 2);
     fill_u64(cnt, 2, 0, s);
-    k_join_A<<<std::min(grid_for(n), 2048u), 256, 0, s>>>(A.coords, n, g, keymap.get(), ix.fiber_key, ix.nfib, ix.fiber_ptr, rowA,
-                                          fidA, cnt.get(), (unsigned int *)(cnt.get() + 1));
+    if (g.K <= (1ull << 26) || ix.nfib == 0) {
+        k_join_A<<<std::min(grid_for(n), 2048u), 256, 0, s>>>(A.coords, n, g, keymap.get(), ix.fiber_key, ix.nfib,
+                                                               ix.fiber_ptr, rowA, fidA, cnt.get(),
+                                                               (unsigned int *)(cnt.get() + 1));
+    } else {
+        uint64_t cap = pow2_ceil((uint64_t)ix.nfib * 2);
+        Tmp<unsigned long long> hk(pool, cap);
+        Tmp<uint32_t> hv(pool, cap);
+        fill_u64(hk, cap, ~0ull, s);
+        k_hmap_build<<<grid_for(ix.nfib), 256, 0, s>>>(ix.fiber_key, ix.nfib, hk, hv, cap - 1);
+        k_join_A_hash<<<std::min(grid_for(n), 2048u), 256, 0, s>>>(A.coords, n, g, hk, hv, cap - 1, ix.fiber_ptr,
+                                                                    rowA, fidA, cnt.get(),
+                                                                    (unsigned int *)(cnt.get() + 1));
+    }
     unsigned long long h[2];
     FCC_HIP(hipMemcpyAsync(h, cnt.get(), 16, hipMemcpyDeviceToHost, s));
     FCC_HIP(hipStreamSynchronize(s));
@@ -466,16 +588,19 @@ void build_left(DevPool &pool, const Geom &g, const Operand &A, const uint32_t *
     ix.st_sumfib = (uint64_t *)keep(T * 8);
     ix.st_products = (uint64_t *)keep(T * 8);
     ix.st_sketch = (uint32_t *)keep(T * 8);
+    ix.st_maxgp = (uint64_t *)keep(T * 8);
     if (T) {
         fill_u32(ix.st_nnz, T, 0, s);
         fill_u32(ix.st_keys, T, 0, s);
         fill_u64((unsigned long long *)ix.st_sumfib, T, 0, s);
         fill_u64((unsigned long long *)ix.st_products, T, 0, s);
         fill_u32(ix.st_sketch, 2 * T, 0xFFFFFFFFu, s);
+        fill_u64((unsigned long long *)ix.st_maxgp, T, 0, s);
         k_tile_stats<<<grid_for(ix.ngroups), 256, 0, s>>>(ix.group_ptr, ix.group_fiber, ix.group_atile, ix.fiber_ptr,
                                                           ix.Bcol, ix.ngroups, ix.st_nnz, ix.st_keys,
                                                           (unsigned long long *)ix.st_sumfib,
-                                                          (unsigned long long *)ix.st_products, ix.st_sketch);
+                                                          (unsigned long long *)ix.st_products, ix.st_sketch,
+                                                          (unsigned long long *)ix.st_maxgp);
     }
 }
 

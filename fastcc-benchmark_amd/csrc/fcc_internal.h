@@ -151,6 +151,7 @@ struct Index {
     uint32_t *st_keys = nullptr;      // groups (distinct contraction keys) in tile
     uint32_t *st_rows = nullptr;      // distinct rows in tile
     uint32_t *st_sketch = nullptr;    // distinct-column sketch (KMV estimate)
+    uint64_t *st_maxgp = nullptr;     // largest single-group product m * |fiber| (lower bound on outputs)
     int logTL = 0;
     uint32_t TL = 1;
     uint64_t products = 0;

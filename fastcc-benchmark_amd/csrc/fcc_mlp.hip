@@ -4,8 +4,13 @@
 // Inputs (per left tile, all log1p-scaled, then standardised):
 //   left-tile nnz, distinct contraction keys, summed right-fiber lengths,
 //   products bound, distinct-column sketch estimate, tile area TL*spanB.
-// Outputs (log1p space, standardised): tile output nnz, tile distinct output
-// columns. The exact count pass produces the labels.
+// Outputs (log1p space, standardised), each relative to a tight upper bound
+// known from the features:
+//   r0 = log1p(tile output nnz)      - min(log1p(products), log1p(area))
+//   r1 = log1p(distinct out columns) - log1p(summed fiber lengths)
+// Both are <= 0; predicting the gap to the bound keeps predictions for tiles
+// unlike the training set near the bound (safe) instead of near zero.
+// The exact count pass produces the labels.
 //
 // The reference has no planner: its tile size is the caller's argument
 // (contract.hpp:844-846); the features restate its per-tile statistics
@@ -75,10 +80,12 @@ __global__ __launch_bounds__(256) void k_mlp_infer(const float *P, const float *
 #pragma unroll
         for (int o = 0; o < NO; o++) y[o] += s * w[OFF_W3 + k * NO + o];
     }
-    float l0 = y[0] * w[OFF_OS] + w[OFF_OM];
-    float l1 = y[1] * w[OFF_OS + 1] + w[OFF_OM + 1];
-    pred_nnz[t] = fmaxf(expm1f(fminf(l0, 80.f)), 0.f);
-    if (pred_cols) pred_cols[t] = fmaxf(expm1f(fminf(l1, 80.f)), 0.f);
+    const float *f = feat + (size_t)t * NI;
+    float r0 = fminf(y[0] * w[OFF_OS] + w[OFF_OM], 0.f);
+    float r1 = fminf(y[1] * w[OFF_OS + 1] + w[OFF_OM + 1], 0.f);
+    float ub0 = fminf(f[3], f[5]), ub1 = f[2];
+    pred_nnz[t] = fmaxf(expm1f(fminf(r0 + ub0, 80.f)), 0.f);
+    if (pred_cols) pred_cols[t] = fmaxf(expm1f(fminf(r1 + ub1, 80.f)), 0.f);
 }
 
 // ------------------------------------------------------ forward + backward

@@ -1,6 +1,11 @@
 """Learned planner: training data from the exact count pass, GPU training,
 tile-level error.
 
+Labels are log-gaps to upper bounds computed from the features (output nnz
+vs min(products, area); distinct columns vs summed fiber lengths), so the
+network's job is the collision/packing factor and a tile unlike the training
+set is predicted near its bound rather than near zero.
+
 The exact planner (count-then-fill) records, for every left tile, six cheap
 features and two labels (exact output nnz, distinct output columns) when
 called with ``collect_stats=True``. ``train`` fits the MLP on the GPU with the
@@ -55,14 +60,16 @@ def predict(ctx, features):
     nout = 2
     om = p[-2 * nout:-nout]
     os_ = p[-nout:]
-    logv = pred * os_ + om
-    return np.expm1(np.minimum(logv, 80.0))
+    gap = np.minimum(pred * os_ + om, 0.0)
+    ub = np.stack([np.minimum(features[:, 3], features[:, 5]), features[:, 2]], 1)
+    return np.maximum(np.expm1(np.minimum(gap + ub, 80.0)), 0.0)
 
 
 def tile_mape(ctx, features, labels):
     """Mean |pred - actual| / max(actual, 1) of tile output nnz (and of distinct columns)."""
     pr = predict(ctx, features)
-    act = np.expm1(labels)
+    ub = np.stack([np.minimum(features[:, 3], features[:, 5]), features[:, 2]], 1)
+    act = np.expm1(labels + ub)
     e_nnz = np.mean(np.abs(pr[:, 0] - act[:, 0]) / np.maximum(act[:, 0], 1.0))
     e_col = np.mean(np.abs(pr[:, 1] - act[:, 1]) / np.maximum(act[:, 1], 1.0))
     return float(e_nnz), float(e_col)
