@@ -200,24 +200,19 @@ __device__ __forceinline__ uint32_t lane_lower_bound(const uint32_t *arr, uint32
     return lo;
 }
 
-// One unit: heavy groups, then light entries.
-// Heavy groups are claimed 64 at a time per wavefront; lane l holds group l's
-// descriptor (fiber range, entry range), read back as scalars with readlane.
-// For group j the wavefront's lanes cover 64 columns of the fiber segment
-// and loop over the group's rows (row index and value broadcast from lanes);
-// the first row/column chunks of group j+1 are loaded before group j's
-// accumulation so their latency overlaps it.
+// Heavy groups of a dense unit with row ownership: every wavefront walks
+// every heavy group but only applies rows li with li % NWAVES == its index, so
+// no two wavefronts ever touch the same slot and the accumulator (and bitmap)
+// is updated with plain LDS read-modify-write instead of atomics. Within a
+// wavefront the 64 lanes hold distinct columns of one row at a time.
 template <bool NUM, bool BITMAP>
-__device__ void process_unit(const AccArgs &A, const UnitCtx &U, const Acc<NUM, BITMAP> &S, uint32_t *s_gnext) {
-    const uint32_t lane = threadIdx.x & 63;
-    const uint32_t g0 = A.atile_g0[U.a], gl = A.atile_lg[U.a], g1 = A.atile_g0[U.a + 1];
+__device__ void dense_heavy_owned(const AccArgs &A, const UnitCtx &U, const Acc<NUM, BITMAP> &S) {
+    const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const uint32_t g0 = A.atile_g0[U.a], gl = A.atile_lg[U.a];
     const uint32_t cend = U.c0 + U.W;
-    for (;;) {
-        uint32_t bs = 0;
-        if (lane == 0) bs = atomicAdd(s_gnext, 64u);
-        bs = uniform_u32(__shfl(bs, 0, 64));
-        if (bs >= gl - g0) break;
-        const uint32_t bg = g0 + bs;
+    const uint32_t wpr = U.Wp >> 5;
+    constexpr bool BITS = !NUM || BITMAP;
+    for (uint32_t bg = g0; bg < gl; bg += 64) {
         const uint32_t nb = min(64u, gl - bg);
         uint4 d = make_uint4(0, 0, 0, 0);
         if (lane < nb) d = A.gdesc[bg + lane];
@@ -259,6 +254,9 @@ __device__ void process_unit(const AccArgs &A, const UnitCtx &U, const Acc<NUM, 
                     if (NUM) nbv = A.Bval[s + lane];
                 }
             }
+            // rows of this group owned by this wavefront (first 64 rows)
+            uint64_t own0 = __ballot(lane < cm && (cli & (NWAVES - 1)) == wave);
+            if (own0 == 0 && cm <= 64) continue;
             for (uint32_t cb = cs; cb < ce; cb += 64) {
                 uint32_t col = ccol;
                 double b = cbv;
@@ -272,9 +270,21 @@ __device__ void process_unit(const AccArgs &A, const UnitCtx &U, const Acc<NUM, 
                     }
                 }
                 col -= U.c0;
+                uint32_t wc = ok ? (col >> 5) : 0xFFFFFFFFu, bm = ok ? (1u << (col & 31)) : 0u;
+                bool leader = false;
+                if (BITS) {
+#pragma unroll
+                    for (int dd = 1; dd < 64; dd <<= 1) {
+                        uint32_t ob = __shfl_down(bm, dd, 64), ow = __shfl_down(wc, dd, 64);
+                        if (lane + dd < 64 && ow == wc) bm |= ob;
+                    }
+                    uint32_t pw = __shfl_up(wc, 1, 64);
+                    leader = ok && (lane == 0 || pw != wc);
+                }
                 for (uint32_t kb = 0; kb < cm; kb += 64) {
                     uint32_t li_l = cli;
                     double av_l = cav;
+                    uint64_t own = own0;
                     if (kb != 0) {
                         li_l = 0;
                         av_l = 0.0;
@@ -282,42 +292,183 @@ __device__ void process_unit(const AccArgs &A, const UnitCtx &U, const Acc<NUM, 
                             li_l = A.Ali[cep0 + kb + lane];
                             if (NUM) av_l = A.Aval[cep0 + kb + lane];
                         }
+                        own = __ballot(kb + lane < cm && (li_l & (NWAVES - 1)) == wave);
                     }
-                    const uint32_t mk = min(64u, cm - kb);
-                    if (U.kind == KIND_DENSE) {
-                        // Dense tile: the chunk's columns are ascending, so lanes
-                        // sharing a 32-column bitmap word are contiguous; OR their
-                        // bits together and let one lane per word touch the bitmap.
-                        uint32_t wc = ok ? (col >> 5) : 0xFFFFFFFFu;
-                        uint32_t bm = ok ? (1u << (col & 31)) : 0u;
-                        bool leader = false;
-                        if (!NUM || BITMAP) {
-#pragma unroll
-                            for (int dd = 1; dd < 64; dd <<= 1) {
-                                uint32_t ob = __shfl_down(bm, dd, 64), ow = __shfl_down(wc, dd, 64);
-                                if (lane + dd < 64 && ow == wc) bm |= ob;
-                            }
-                            uint32_t pw = __shfl_up(wc, 1, 64);
-                            leader = ok && (lane == 0 || pw != wc);
-                        }
-                        const uint32_t wpr = U.Wp >> 5;
-                        for (uint32_t kk = 0; kk < mk; kk++) {
-                            const uint32_t li = rlane(li_l, kk);
-                            if ((!NUM || BITMAP) && leader) atomicOr(&S.bits[li * wpr + wc], bm);
-                            if (NUM && ok) lds_add_f64(&S.fv[li * U.Wp + col], rlane_f64(av_l, kk) * b);
-                        }
-                    } else {
-                        for (uint32_t kk = 0; kk < mk; kk++) {
-                            const uint32_t li = rlane(li_l, kk);
-                            const double av = NUM ? rlane_f64(av_l, kk) : 0.0;
-                            if (ok) accum<NUM, BITMAP>(A, U, S, li, col, av * b);
+                    while (own) {
+                        const uint32_t kk = (uint32_t)__builtin_ctzll(own);
+                        own &= own - 1;
+                        const uint32_t li = rlane(li_l, kk);
+                        if (BITS && leader) S.bits[li * wpr + wc] |= bm;
+                        if (NUM && ok) {
+                            double *slot = &S.fv[li * U.Wp + col];
+                            *slot = *slot + rlane_f64(av_l, kk) * b;
                         }
                     }
                 }
             }
         }
     }
-    // light entries: one lane per entry, fiber bounds precomputed per entry
+}
+
+// Heavy groups claimed 64 at a time per wavefront; lane l holds group l's
+// descriptor (fiber range, entry range), read back as scalars with readlane.
+// For group j the wavefront's lanes cover 64 columns of the fiber segment and
+// loop over the group's rows. Groups are pipelined in blocks of PIPE: the
+// first row and column chunks of the next PIPE groups are loaded while the
+// current PIPE groups accumulate, so 2*PIPE groups' loads are in flight per
+// wavefront. Updates are atomic.
+constexpr int PIPE = 4;
+
+template <bool NUM>
+__device__ __forceinline__ void load_first_chunks(const AccArgs &A, uint32_t lane, uint32_t s, uint32_t e, uint32_t ep0,
+                                                  uint32_t m, uint32_t &li, double &av, uint32_t &col, double &b) {
+    li = 0;
+    col = 0;
+    av = 0.0;
+    b = 0.0;
+    if (lane < m) {
+        li = A.Ali[ep0 + lane];
+        if (NUM) av = A.Aval[ep0 + lane];
+    }
+    if (s + lane < e) {
+        col = A.Bcol[s + lane];
+        if (NUM) b = A.Bval[s + lane];
+    }
+}
+
+template <bool NUM, bool BITMAP>
+__device__ __forceinline__ void heavy_group(const AccArgs &A, const UnitCtx &U, const Acc<NUM, BITMAP> &S,
+                                            uint32_t lane, uint32_t cs, uint32_t ce, uint32_t cep0, uint32_t cm,
+                                            uint32_t cli, double cav, uint32_t ccol, double cbv) {
+    uint32_t ncol = ccol;
+    double nbv = cbv;
+    for (uint32_t cb = cs; cb < ce; cb += 64) {
+        uint32_t col = ncol;
+        double b = nbv;
+        const bool ok = cb + lane < ce;
+        if (cb + 64 < ce) {  // next chunk in flight while this one accumulates
+            ncol = 0;
+            nbv = 0.0;
+            if (cb + 64 + lane < ce) {
+                ncol = A.Bcol[cb + 64 + lane];
+                if (NUM) nbv = A.Bval[cb + 64 + lane];
+            }
+        }
+        col -= U.c0;
+        uint32_t wc = 0, bm = 0;
+        bool leader = false;
+        if (U.kind == KIND_DENSE && (!NUM || BITMAP)) {
+            // The chunk's columns are ascending, so lanes sharing a 32-column
+            // bitmap word are contiguous; OR their bits together and let one
+            // lane per word touch the bitmap.
+            wc = ok ? (col >> 5) : 0xFFFFFFFFu;
+            bm = ok ? (1u << (col & 31)) : 0u;
+#pragma unroll
+            for (int dd = 1; dd < 64; dd <<= 1) {
+                uint32_t ob = __shfl_down(bm, dd, 64), ow = __shfl_down(wc, dd, 64);
+                if (lane + dd < 64 && ow == wc) bm |= ob;
+            }
+            uint32_t pw = __shfl_up(wc, 1, 64);
+            leader = ok && (lane == 0 || pw != wc);
+        }
+        for (uint32_t kb = 0; kb < cm; kb += 64) {
+            uint32_t li_l = cli;
+            double av_l = cav;
+            if (kb != 0) {
+                li_l = 0;
+                av_l = 0.0;
+                if (kb + lane < cm) {
+                    li_l = A.Ali[cep0 + kb + lane];
+                    if (NUM) av_l = A.Aval[cep0 + kb + lane];
+                }
+            }
+            const uint32_t mk = min(64u, cm - kb);
+            if (U.kind == KIND_DENSE) {
+                const uint32_t wpr = U.Wp >> 5;
+                for (uint32_t kk = 0; kk < mk; kk++) {
+                    const uint32_t li = rlane(li_l, kk);
+                    if ((!NUM || BITMAP) && leader) atomicOr(&S.bits[li * wpr + wc], bm);
+                    if (NUM && ok) lds_add_f64(&S.fv[li * U.Wp + col], rlane_f64(av_l, kk) * b);
+                }
+            } else {
+                for (uint32_t kk = 0; kk < mk; kk++) {
+                    const uint32_t li = rlane(li_l, kk);
+                    const double av = NUM ? rlane_f64(av_l, kk) : 0.0;
+                    if (ok) accum<NUM, BITMAP>(A, U, S, li, col, av * b);
+                }
+            }
+        }
+    }
+}
+
+template <bool NUM, bool BITMAP>
+__device__ void heavy_claimed(const AccArgs &A, const UnitCtx &U, const Acc<NUM, BITMAP> &S, uint32_t *s_gnext) {
+    const uint32_t lane = threadIdx.x & 63;
+    const uint32_t g0 = A.atile_g0[U.a], gl = A.atile_lg[U.a];
+    const uint32_t cend = U.c0 + U.W;
+    for (;;) {
+        uint32_t bs = 0;
+        if (lane == 0) bs = atomicAdd(s_gnext, 64u);
+        bs = uniform_u32(__shfl(bs, 0, 64));
+        if (bs >= gl - g0) break;
+        const uint32_t bg = g0 + bs;
+        const uint32_t nb = min(64u, gl - bg);
+        uint4 d = make_uint4(0, 0, 0, 0);
+        if (lane < nb) d = A.gdesc[bg + lane];
+        uint32_t s_l = d.x, e_l = d.x + d.y;
+        if (U.multi && lane < nb) {
+            s_l = lane_lower_bound(A.Bcol, s_l, e_l, U.c0);
+            e_l = lane_lower_bound(A.Bcol, s_l, e_l, cend);
+        }
+        uint32_t li[PIPE], col[PIPE];
+        double av[PIPE], bv[PIPE];
+#pragma unroll
+        for (int q = 0; q < PIPE; q++) {
+            const uint32_t j = q;  // lanes >= nb hold empty descriptors
+            load_first_chunks<NUM>(A, lane, rlane(s_l, j), rlane(e_l, j), rlane(d.z, j), rlane(d.w, j), li[q], av[q],
+                                   col[q], bv[q]);
+        }
+        for (uint32_t jb = 0; jb < nb; jb += PIPE) {
+            uint32_t cli[PIPE], ccol[PIPE];
+            double cav[PIPE], cbv[PIPE];
+#pragma unroll
+            for (int q = 0; q < PIPE; q++) {
+                cli[q] = li[q];
+                ccol[q] = col[q];
+                cav[q] = av[q];
+                cbv[q] = bv[q];
+            }
+            if (jb + PIPE < nb) {
+#pragma unroll
+                for (int q = 0; q < PIPE; q++) {
+                    const uint32_t j = jb + PIPE + q;
+                    load_first_chunks<NUM>(A, lane, rlane(s_l, j & 63), rlane(e_l, j & 63), rlane(d.z, j & 63),
+                                           j < 64 ? rlane(d.w, j & 63) : 0u, li[q], av[q], col[q], bv[q]);
+                }
+            }
+#pragma unroll
+            for (int q = 0; q < PIPE; q++) {
+                const uint32_t j = jb + q;
+                if (j < nb)
+                    heavy_group<NUM, BITMAP>(A, U, S, lane, rlane(s_l, j), rlane(e_l, j), rlane(d.z, j), rlane(d.w, j),
+                                             cli[q], cav[q], ccol[q], cbv[q]);
+            }
+        }
+    }
+}
+
+// One unit: heavy groups, then light entries (one lane per entry, fiber
+// bounds precomputed per entry, atomic updates).
+template <bool NUM, bool BITMAP>
+__device__ void process_unit(const AccArgs &A, const UnitCtx &U, const Acc<NUM, BITMAP> &S, uint32_t *s_gnext) {
+    const uint32_t gl = A.atile_lg[U.a], g1 = A.atile_g0[U.a + 1];
+    const uint32_t cend = U.c0 + U.W;
+    if (A.row_own && U.kind == KIND_DENSE && (1u << A.logTL) >= (uint32_t)NWAVES) {
+        dense_heavy_owned<NUM, BITMAP>(A, U, S);
+        __syncthreads();  // plain updates above, atomic updates below
+    } else {
+        heavy_claimed<NUM, BITMAP>(A, U, S, s_gnext);
+    }
     const uint32_t le0 = A.group_ptr[gl], le1 = A.group_ptr[g1];
     for (uint32_t e = le0 + threadIdx.x; e < le1; e += ACC_THREADS) {
         const uint32_t li = A.Ali[e];

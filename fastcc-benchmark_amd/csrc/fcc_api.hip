@@ -14,6 +14,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <random>
 #include <string>
@@ -141,8 +142,9 @@ void choose_tiles(fcc_ctx *ctx, const fcc_options &o, const Geom &g, uint64_t pr
         const uint64_t sp = (g.spanB + 31) & ~31ull;  // dense rows are padded to 32 columns
         uint64_t tf = 1;
         while (tf * 2 * sp <= DENSE_SLOTS) tf *= 2;
-        if (tf * sp <= DENSE_SLOTS && tf >= 4) TL = std::min<uint64_t>(tf, 64);
-        else TL = 32;
+        // dense units use row ownership (one row class per wavefront): at least 16 rows
+        if (tf * sp <= DENSE_SLOTS && tf >= 16) TL = std::min<uint64_t>(tf, 64);
+        else TL = 16;
     } else {
         double t = (double)HASH_TARGET / std::max(1.0, prow);
         TL = 1;
@@ -406,8 +408,13 @@ int Contraction::run(const fcc_tensor *ta, const int32_t *am, const fcc_tensor *
     bool self = A.coords == B.coords && A.vals == B.vals && ta->order == tb->order && ta->nnz == tb->nnz;
     for (int i = 0; i < r && self; i++) self = am[i] == bm[i];
     for (int m = 0; m < ta->order && self; m++) self = EA[m] == EB[m];
+    Tmp<uint32_t> bfib, vsB;
+    SelfJoin sj{rowA, fidA, nullptr, nullptr, 0};
     if (self) {
-        SelfJoin sj{rowA, fidA, 0};
+        bfib = Tmp<uint32_t>(pool, (size_t)A.nnz);
+        vsB = Tmp<uint32_t>(pool, (size_t)A.nnz);
+        sj.bfib = bfib;
+        sj.vsB = vsB;
         build_right(pool, g, B, ix, hold, s, &sj);
         products = sj.products;
         nvalid = (uint32_t)A.nnz;
@@ -432,7 +439,11 @@ int Contraction::run(const fcc_tensor *ta, const int32_t *am, const fcc_tensor *
     int logTL;
     uint32_t Wd;
     choose_tiles(ctx, o, g, products, nvalid, &logTL, &Wd);
-    build_left(pool, g, A, rowA, fidA, nvalid, logTL, ix, hold, s);
+    if (const char *hm = getenv("FCC_HEAVY_MIN")) ix.heavy_min = (uint32_t)std::max(1, atoi(hm));
+    if (self && o.left_tile_stride <= 1) build_left_self(pool, g, A, sj, logTL, ix, hold, s);
+    else build_left(pool, g, A, rowA, fidA, nvalid, logTL, ix, hold, s);
+    bfib.reset();
+    vsB.reset();
     rowA.reset();
     fidA.reset();
     st.row_tile = (int32_t)ix.TL;
@@ -497,6 +508,7 @@ int Contraction::run(const fcc_tensor *ta, const int32_t *am, const fcc_tensor *
     X.atile_g0 = ix.atile_g0;
     X.atile_lg = ix.atile_lg;
     X.unit_order = nullptr;
+    X.row_own = getenv("FCC_ROW_OWN") ? 1 : 0;
     X.logTL = ix.logTL;
     X.spanB = g.spanB;
     X.nxa = g.nxa;

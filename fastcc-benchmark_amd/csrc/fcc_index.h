@@ -16,6 +16,8 @@ void launch_extents(const Operand &t, uint32_t *dev_out, hipStream_t s);
 struct SelfJoin {
     uint32_t *rowA;
     uint32_t *fidA;
+    uint32_t *bfib;  // fiber of each right-sorted position
+    uint32_t *vsB;   // original entry of each right-sorted position
     uint64_t products;
 };
 
@@ -28,6 +30,10 @@ void build_right(DevPool &pool, const Geom &g, const Operand &B, Index &ix,
 // matched entry count (synchronises).
 void join_left(DevPool &pool, const Geom &g, const Operand &A, const Index &ix, uint32_t *rowA, uint32_t *fidA,
                uint64_t *products, uint32_t *nvalid, hipStream_t s);
+
+// Self-contraction: left tiles from the right operand's order (2 radix passes).
+void build_left_self(DevPool &pool, const Geom &g, const Operand &A, const SelfJoin &sj, int logTL, Index &ix,
+                     std::vector<std::shared_ptr<void>> &hold, hipStream_t s);
 
 // Left operand tiles of 2^logTL rows: sort, groups, active tiles, statistics.
 void build_left(DevPool &pool, const Geom &g, const Operand &A, const uint32_t *rowA, const uint32_t *fidA,

@@ -179,7 +179,7 @@ __global__ __launch_bounds__(256) void k_join_A(const uint32_t *coords, size_t n
 
 // A composite: tile | class | fiber | row-in-tile; unmatched entries sort last.
 __global__ void k_comp_A(const uint32_t *rowA, const uint32_t *fidA, const uint32_t *fiber_ptr, size_t nnz,
-                         int logTL, int fbits, uint64_t *comp, uint32_t *idx) {
+                         int logTL, int fbits, uint32_t heavy_min, uint64_t *comp, uint32_t *idx) {
     size_t n = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
     if (n >= nnz) return;
     uint32_t f = fidA[n];
@@ -188,7 +188,7 @@ __global__ void k_comp_A(const uint32_t *rowA, const uint32_t *fidA, const uint3
         uint32_t row = rowA[n];
         uint64_t tile = row >> logTL;
         uint64_t li = row & ((1u << logTL) - 1);
-        uint64_t cls = (fiber_ptr[f + 1] - fiber_ptr[f]) >= HEAVY_MIN ? 0 : 1;
+        uint64_t cls = (fiber_ptr[f + 1] - fiber_ptr[f]) >= heavy_min ? 0 : 1;
         c = (tile << (1 + fbits + logTL)) | (cls << (fbits + logTL)) | ((uint64_t)f << logTL) | li;
     }
     comp[n] = c;
@@ -370,7 +370,8 @@ static unsigned grid_for(size_t n, unsigned bs = 256) { return (unsigned)div_up(
 // column (= the left row) and its fiber.
 __global__ __launch_bounds__(256) void k_self_join(const uint32_t *vs, const uint32_t *flag, const uint32_t *fid,
                                                   const uint32_t *Bcol, const uint32_t *fiber_ptr, size_t n,
-                                                  uint32_t *rowA, uint32_t *fidA, unsigned long long *products) {
+                                                  uint32_t *rowA, uint32_t *fidA, uint32_t *bfib, uint32_t *vsB,
+                                                  unsigned long long *products) {
     __shared__ unsigned long long s_p[4];
     uint64_t prod = 0;
     for (size_t p = blockIdx.x * (size_t)blockDim.x + threadIdx.x; p < n; p += (size_t)gridDim.x * blockDim.x) {
@@ -378,6 +379,8 @@ __global__ __launch_bounds__(256) void k_self_join(const uint32_t *vs, const uin
         uint32_t o = vs[p];
         rowA[o] = Bcol[p];
         fidA[o] = f;
+        bfib[p] = f;
+        vsB[p] = o;
         prod += fiber_ptr[f + 1] - fiber_ptr[f];
     }
     for (int d = 32; d > 0; d >>= 1) prod += __shfl_xor(prod, d, 64);
@@ -418,7 +421,7 @@ void build_right(DevPool &pool, const Geom &g, const Operand &B, Index &ix, std:
         Tmp<unsigned long long> prod(pool, 1);
         fill_u64(prod, 1, 0, s);
         k_self_join<<<std::min(grid_for(n), 2048u), 256, 0, s>>>(vs, flag, fid, ix.Bcol, ix.fiber_ptr, n, self->rowA,
-                                                                  self->fidA, prod);
+                                                                  self->fidA, self->bfib, self->vsB, prod);
         FCC_HIP(hipMemcpyAsync(&self->products, prod.get(), 8, hipMemcpyDeviceToHost, s));
     }
     FCC_HIP(hipMemcpyAsync(&ix.nfib, total.get(), 4, hipMemcpyDeviceToHost, s));
@@ -528,28 +531,17 @@ void join_left(DevPool &pool, const Geom &g, const Operand &A, const Index &ix, 
     *nvalid = (uint32_t)(h[1] & 0xFFFFFFFFu);
 }
 
-void build_left(DevPool &pool, const Geom &g, const Operand &A, const uint32_t *rowA, const uint32_t *fidA,
-                uint32_t nvalid, int logTL, Index &ix, std::vector<std::shared_ptr<void>> &hold, hipStream_t s) {
-    size_t n = (size_t)A.nnz;
+// Left tiles from the sorted composite keys (tile | class | fiber | row-in-tile)
+// and the original entry index of each sorted position.
+static void left_from_sorted(DevPool &pool, const Operand &A, const uint64_t *ks, const uint32_t *vs, uint32_t nvalid,
+                             int logTL, int fbits, Index &ix, std::vector<std::shared_ptr<void>> &hold,
+                             hipStream_t s) {
     auto keep = [&](size_t bytes) -> void * {
         void *p = pool.get(bytes + 16);
         DevPool *pl = &pool;
         hold.emplace_back(p, [pl](void *q) { pl->put(q); });
         return p;
     };
-    ix.logTL = logTL;
-    ix.TL = 1u << logTL;
-    ix.nAv = nvalid;
-    int fbits = std::max(1, ceil_log2((uint64_t)ix.nfib + 1));
-    int tbits = std::max(0, g.rbits - logTL);
-    int nbits = tbits + 1 + fbits + logTL + 1;
-    if (nbits > 64) throw LibError{ERR_INVALID, "left composite key exceeds 64 bits"};
-    Tmp<uint64_t> k1(pool, n), k2(pool, n);
-    Tmp<uint32_t> v1(pool, n), v2(pool, n);
-    k_comp_A<<<grid_for(n), 256, 0, s>>>(rowA, fidA, ix.fiber_ptr, n, logTL, fbits, k1, v1);
-    uint64_t *ks;
-    uint32_t *vs;
-    radix_sort_pairs(pool, k1, v1, k2, v2, n, std::min(nbits, 64), s, &ks, &vs);
     size_t m = nvalid;
     ix.Ali = (uint32_t *)keep(m * 4);
     ix.Aval = (double *)keep(m * 8);
@@ -602,6 +594,71 @@ void build_left(DevPool &pool, const Geom &g, const Operand &A, const uint32_t *
                                                           (unsigned long long *)ix.st_products, ix.st_sketch,
                                                           (unsigned long long *)ix.st_maxgp);
     }
+}
+
+void build_left(DevPool &pool, const Geom &g, const Operand &A, const uint32_t *rowA, const uint32_t *fidA,
+                uint32_t nvalid, int logTL, Index &ix, std::vector<std::shared_ptr<void>> &hold, hipStream_t s) {
+    size_t n = (size_t)A.nnz;
+    ix.logTL = logTL;
+    ix.TL = 1u << logTL;
+    ix.nAv = nvalid;
+    int fbits = std::max(1, ceil_log2((uint64_t)ix.nfib + 1));
+    int tbits = std::max(0, g.rbits - logTL);
+    int nbits = tbits + 1 + fbits + logTL + 1;
+    if (nbits > 64) throw LibError{ERR_INVALID, "left composite key exceeds 64 bits"};
+    Tmp<uint64_t> k1(pool, n), k2(pool, n);
+    Tmp<uint32_t> v1(pool, n), v2(pool, n);
+    k_comp_A<<<grid_for(n), 256, 0, s>>>(rowA, fidA, ix.fiber_ptr, n, logTL, fbits, ix.heavy_min, k1, v1);
+    uint64_t *ks;
+    uint32_t *vs;
+    radix_sort_pairs(pool, k1, v1, k2, v2, n, std::min(nbits, 64), s, &ks, &vs);
+    left_from_sorted(pool, A, ks, vs, nvalid, logTL, fbits, ix, hold, s);
+}
+
+// Self-contraction: the right operand is already sorted by (fiber, column)
+// and column == left row, so a stable sort on (tile, class) alone yields the
+// left order (tile, class, fiber, row-in-tile).
+__global__ void k_self_tile_key(const uint32_t *Bcol, const uint32_t *bfib, const uint32_t *fiber_ptr, size_t n,
+                                int logTL, uint32_t heavy_min, uint64_t *key, uint32_t *pos) {
+    size_t p = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    uint32_t f = bfib[p];
+    uint64_t cls = (fiber_ptr[f + 1] - fiber_ptr[f]) >= heavy_min ? 0 : 1;
+    key[p] = ((uint64_t)(Bcol[p] >> logTL) << 1) | cls;
+    pos[p] = (uint32_t)p;
+}
+
+__global__ void k_self_composite(const uint64_t *tkey, const uint32_t *pos, const uint32_t *Bcol,
+                                 const uint32_t *bfib, const uint32_t *vsB, size_t n, int logTL, int fbits,
+                                 uint64_t *ks, uint32_t *vs) {
+    size_t q = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+    if (q >= n) return;
+    uint32_t p = pos[q];
+    uint64_t tc = tkey[q];
+    uint64_t li = Bcol[p] & ((1u << logTL) - 1);
+    ks[q] = ((tc >> 1) << (1 + fbits + logTL)) | ((tc & 1) << (fbits + logTL)) | ((uint64_t)bfib[p] << logTL) | li;
+    vs[q] = vsB[p];
+}
+
+void build_left_self(DevPool &pool, const Geom &g, const Operand &A, const SelfJoin &sj, int logTL, Index &ix,
+                     std::vector<std::shared_ptr<void>> &hold, hipStream_t s) {
+    size_t n = (size_t)A.nnz;
+    ix.logTL = logTL;
+    ix.TL = 1u << logTL;
+    ix.nAv = (uint32_t)n;
+    int fbits = std::max(1, ceil_log2((uint64_t)ix.nfib + 1));
+    int tbits = std::max(0, g.rbits - logTL);
+    if (tbits + 1 + fbits + logTL + 1 > 64) throw LibError{ERR_INVALID, "left composite key exceeds 64 bits"};
+    Tmp<uint64_t> k1(pool, n), k2(pool, n);
+    Tmp<uint32_t> v1(pool, n), v2(pool, n);
+    k_self_tile_key<<<grid_for(n), 256, 0, s>>>(ix.Bcol, sj.bfib, ix.fiber_ptr, n, logTL, ix.heavy_min, k1, v1);
+    uint64_t *tk;
+    uint32_t *tp;
+    radix_sort_pairs(pool, k1, v1, k2, v2, n, tbits + 1, s, &tk, &tp);
+    uint64_t *ks = tk == k1.get() ? k2.get() : k1.get();
+    uint32_t *vs = tp == v1.get() ? v2.get() : v1.get();
+    k_self_composite<<<grid_for(n), 256, 0, s>>>(tk, tp, ix.Bcol, sj.bfib, sj.vsB, n, logTL, fbits, ks, vs);
+    left_from_sorted(pool, A, ks, vs, (uint32_t)n, logTL, fbits, ix, hold, s);
 }
 
 }  // namespace fcc

@@ -113,7 +113,7 @@ struct Operand {
 // Accumulator kinds (match FCC_ACC_*).
 enum : uint8_t { KIND_DENSE = 1, KIND_LHASH = 2, KIND_GHASH = 3 };
 
-constexpr uint32_t HEAVY_MIN = 32;      // fibers at least this long are processed wave-per-group
+constexpr uint32_t HEAVY_MIN = 32;      // default: fibers at least this long are processed wave-per-group
 constexpr uint32_t DENSE_SLOTS = 9984;  // f64 LDS accumulator slots (78 KB: two workgroups per CU)
 constexpr uint32_t LH_SLOTS = 4096;     // LDS hash slots (f64 value + u32 key)
 constexpr uint32_t LH_MAX = 3072;       // LDS hash distinct-key limit before sealing
@@ -155,6 +155,7 @@ struct Index {
     int logTL = 0;
     uint32_t TL = 1;
     uint64_t products = 0;
+    uint32_t heavy_min = HEAVY_MIN;  // fiber length from which groups are processed wave-per-group
 };
 
 struct Units {
@@ -211,6 +212,7 @@ struct AccArgs {
     unsigned long long *sink;  // [0]=count [1]=hash
     double *sinkf;             // [0]=val_sum [1]=val_wsum
     unsigned int *work;        // persistent work counter
+    int row_own;               // dense units: row-owned plain updates (experimental)
 };
 
 }  // namespace fcc

@@ -184,20 +184,49 @@ __global__ __launch_bounds__(RS_THREADS) void k_rs_hist(const uint64_t *keys, si
 __global__ __launch_bounds__(RS_THREADS) void k_rs_scatter(const uint64_t *keys, const uint32_t *vals, uint64_t *okeys,
                                                            uint32_t *ovals, size_t n, int shift,
                                                            const uint32_t *hist_scan, unsigned nblocks) {
-    __shared__ uint32_t base[256];
+    // The tile is first placed in LDS in stable digit order, then each digit's
+    // run is written to its global position by consecutive threads
+    // (coalesced), instead of scattering element by element to HBM.
+    __shared__ uint64_t s_key[RS_TILE];
+    __shared__ uint32_t s_val[RS_TILE];
+    __shared__ uint32_t s_gbase[256];   // global start of this tile's run of digit d
+    __shared__ uint32_t s_lbase[256];   // start of digit d's run inside the tile
+    __shared__ uint32_t s_run[256];     // elements of digit d placed so far
     __shared__ uint32_t wcnt[RS_THREADS / 64][256];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    base[tid] = hist_scan[(size_t)tid * nblocks + blockIdx.x];
+    const size_t t0 = (size_t)blockIdx.x * RS_TILE;
+    const uint32_t tn = (uint32_t)(n - t0 < (size_t)RS_TILE ? n - t0 : (size_t)RS_TILE);
+    // local histogram -> exclusive scan (one digit per thread)
+    s_run[tid] = 0;
+    __syncthreads();
+    for (uint32_t i = tid; i < tn; i += RS_THREADS) atomicAdd(&s_run[(uint32_t)(keys[t0 + i] >> shift) & 255u], 1u);
+    __syncthreads();
+    {
+        uint32_t c = s_run[tid];
+        uint32_t incl = c;
+        // block scan over 256 digits: wave scan, then combine 4 wave totals
+        for (int d = 1; d < 64; d <<= 1) {
+            uint32_t o = __shfl_up(incl, d, 64);
+            if (lane >= d) incl += o;
+        }
+        if (lane == 63) wcnt[0][wave] = incl;
+        __syncthreads();
+        uint32_t off = 0;
+        for (int w = 0; w < wave; w++) off += wcnt[0][w];
+        s_lbase[tid] = off + incl - c;
+        s_gbase[tid] = hist_scan[(size_t)tid * nblocks + blockIdx.x];
+        __syncthreads();
+        s_run[tid] = 0;
 #pragma unroll
-    for (int w = 0; w < RS_THREADS / 64; w++) wcnt[w][tid] = 0;
+        for (int w = 0; w < RS_THREADS / 64; w++) wcnt[w][tid] = 0;
+    }
     __syncthreads();
     const uint64_t lt_mask = lane == 0 ? 0ull : (~0ull >> (64 - lane));
-    size_t t0 = (size_t)blockIdx.x * RS_TILE;
     for (int j = 0; j < RS_ITEMS; j++) {
-        size_t idx = t0 + (size_t)j * RS_THREADS + tid;
-        bool valid = idx < n;
-        uint64_t k = valid ? keys[idx] : 0;
-        uint32_t v = valid ? vals[idx] : 0;
+        uint32_t li = (uint32_t)j * RS_THREADS + tid;
+        bool valid = li < tn;
+        uint64_t k = valid ? keys[t0 + li] : 0;
+        uint32_t v = valid ? vals[t0 + li] : 0;
         uint32_t d = (uint32_t)(k >> shift) & 255u;
         uint64_t peers = __ballot(valid);
 #pragma unroll
@@ -210,10 +239,10 @@ __global__ __launch_bounds__(RS_THREADS) void k_rs_scatter(const uint64_t *keys,
         if (valid && rank == 0) wcnt[wave][d] = (uint32_t)__popcll(peers);
         __syncthreads();
         if (valid) {
-            uint32_t off = base[d];
+            uint32_t off = s_lbase[d] + s_run[d];
             for (int w = 0; w < wave; w++) off += wcnt[w][d];
-            okeys[off + rank] = k;
-            ovals[off + rank] = v;
+            s_key[off + rank] = k;
+            s_val[off + rank] = v;
         }
         __syncthreads();
         uint32_t t = 0;
@@ -222,8 +251,15 @@ __global__ __launch_bounds__(RS_THREADS) void k_rs_scatter(const uint64_t *keys,
             t += wcnt[w][tid];
             wcnt[w][tid] = 0;
         }
-        base[tid] += t;
+        s_run[tid] += t;
         __syncthreads();
+    }
+    for (uint32_t i = tid; i < tn; i += RS_THREADS) {
+        uint64_t k = s_key[i];
+        uint32_t d = (uint32_t)(k >> shift) & 255u;
+        uint32_t pos = s_gbase[d] + (i - s_lbase[d]);
+        okeys[pos] = k;
+        ovals[pos] = s_val[i];
     }
 }
 

@@ -261,6 +261,8 @@ static void tindex_free(oc_tindex *ix) {
 
 typedef struct oc_node {
     double val;
+    double absum;
+    uint32_t nterms;
     uint32_t *coords;
     struct oc_node *next;
 } oc_node;
@@ -315,6 +317,8 @@ static void tls_free(oc_tls *t) {
 typedef struct {
     double *d;        /* double accumulator, or NULL */
     long double *q;   /* long double accumulator, or NULL */
+    double *ab;       /* sum of |terms| (checker build only) */
+    uint32_t *nt;     /* number of terms (checker build only) */
     uint8_t *bits;
     uint64_t *active;
     uint64_t nact;
@@ -356,8 +360,14 @@ int oc_contract_tiled(const oc_tensor *a, const int32_t *a_modes, const oc_tenso
     oc_acc *accs = (oc_acc *)calloc(hw, sizeof(oc_acc));
     int err = (!tls || !accs) ? -2 : 0;
     for (int i = 0; i < hw && !err; i++) {
-        if (high_precision) accs[i].q = (long double *)calloc(T * T, sizeof(long double));
-        else accs[i].d = (double *)calloc(T * T, sizeof(double));
+        if (high_precision) {
+            accs[i].q = (long double *)calloc(T * T, sizeof(long double));
+            accs[i].ab = (double *)calloc(T * T, sizeof(double));
+            accs[i].nt = (uint32_t *)calloc(T * T, sizeof(uint32_t));
+            if (!accs[i].ab || !accs[i].nt) err = -2;
+        } else {
+            accs[i].d = (double *)calloc(T * T, sizeof(double));
+        }
         accs[i].bits = (uint8_t *)calloc(T * T / 8 + 1, 1);
         accs[i].active = (uint64_t *)calloc(T * T, sizeof(uint64_t));
         if ((!accs[i].q && !accs[i].d) || !accs[i].bits || !accs[i].active) err = -2;
@@ -388,8 +398,13 @@ int oc_contract_tiled(const oc_tensor *a, const int32_t *a_modes, const oc_tenso
                                 acc->bits[pos / 8] |= bit;
                             }
                             double p = lg->ev[x].val * rg->ev[y].val;
-                            if (acc->q) acc->q[pos] += (long double)p;
-                            else acc->d[pos] += p;
+                            if (acc->q) {
+                                acc->q[pos] += (long double)lg->ev[x].val * (long double)rg->ev[y].val;
+                                acc->ab[pos] += fabs(p);
+                                acc->nt[pos]++;
+                            } else {
+                                acc->d[pos] += p;
+                            }
                         }
                     }
                     tl->products += (uint64_t)lg->n * rg->n;
@@ -411,12 +426,15 @@ int oc_contract_tiled(const oc_tensor *a, const int32_t *a_modes, const oc_tenso
                             decode(left, g.xaE, g.nxa, cc);
                             decode(right, g.xbE, g.nxb, cc + g.nxa);
                             nd->val = v; nd->coords = cc; nd->next = NULL;
+                            nd->absum = acc->ab ? acc->ab[pos] : 0.0;
+                            nd->nterms = acc->nt ? acc->nt[pos] : 0;
                             if (tl->tail) tl->tail->next = nd; else tl->head = nd;
                             tl->tail = nd;
                             tl->count++;
                         }
                     }
-                    if (acc->q) acc->q[pos] = 0.0L; else acc->d[pos] = 0.0;
+                    if (acc->q) { acc->q[pos] = 0.0L; acc->ab[pos] = 0.0; acc->nt[pos] = 0; }
+                    else acc->d[pos] = 0.0;
                     acc->bits[pos / 8] = 0;
                     acc->active[k] = 0;
                 }
@@ -449,7 +467,9 @@ int oc_contract_tiled(const oc_tensor *a, const int32_t *a_modes, const oc_tenso
         if (!sink) {
             out->coords = (uint32_t *)malloc(sizeof(uint32_t) * (size_t)(dC > 0 ? dC : 1) * (total ? total : 1));
             out->vals = (double *)malloc(sizeof(double) * (total ? total : 1));
-            if (!out->coords || !out->vals) err = -2;
+            out->absum = (double *)malloc(sizeof(double) * (total ? total : 1));
+            out->nterms = (uint32_t *)malloc(sizeof(uint32_t) * (total ? total : 1));
+            if (!out->coords || !out->vals || !out->absum || !out->nterms) err = -2;
             uint64_t n = 0;
             for (oc_node *nd = head; nd && !err; nd = nd->next, n++) {
                 uint64_t l = 0;
@@ -459,11 +479,16 @@ int oc_contract_tiled(const oc_tensor *a, const int32_t *a_modes, const oc_tenso
                 for (int m = 0; m < g.nxa; m++) l = l * g.xaE[m] + nd->coords[m];
                 for (int m = 0; m < g.nxb; m++) l = l * g.xbE[m] + nd->coords[g.nxa + m];
                 out->vals[n] = nd->val;
+                out->absum[n] = nd->absum;
+                out->nterms[n] = nd->nterms;
                 summarise(l, nd->val, &out->coord_hash, &out->val_sum, &out->val_wsum);
             }
         }
     }
-    for (int i = 0; i < hw && accs; i++) { free(accs[i].d); free(accs[i].q); free(accs[i].bits); free(accs[i].active); }
+    for (int i = 0; i < hw && accs; i++) {
+        free(accs[i].d); free(accs[i].q); free(accs[i].ab); free(accs[i].nt);
+        free(accs[i].bits); free(accs[i].active);
+    }
     free(accs);
     for (int i = 0; i < hw && tls; i++) tls_free(&tls[i]);
     free(tls);
@@ -523,7 +548,9 @@ int oc_contract_bruteforce(const oc_tensor *a, const int32_t *a_modes, const oc_
     int dC = g.dC;
     out->coords = (uint32_t *)malloc(sizeof(uint32_t) * (size_t)(dC > 0 ? dC : 1) * (uniq ? uniq : 1));
     out->vals = (double *)malloc(sizeof(double) * (uniq ? uniq : 1));
-    if (!out->coords || !out->vals) { free(terms); oc_result_free(out); return -2; }
+    out->absum = (double *)malloc(sizeof(double) * (uniq ? uniq : 1));
+    out->nterms = (uint32_t *)malloc(sizeof(uint32_t) * (uniq ? uniq : 1));
+    if (!out->coords || !out->vals || !out->absum || !out->nterms) { free(terms); oc_result_free(out); return -2; }
     uint64_t E[2 * OC_MAXD];
     for (int i = 0; i < g.nxa; i++) E[i] = g.xaE[i];
     for (int i = 0; i < g.nxb; i++) E[g.nxa + i] = g.xbE[i];
@@ -531,7 +558,10 @@ int oc_contract_bruteforce(const oc_tensor *a, const int32_t *a_modes, const oc_
     for (size_t k = 0; k < n;) {
         size_t e = k;
         long double s = 0.0L;
-        while (e < n && terms[e].lin == terms[k].lin) { s += (long double)terms[e].p; e++; }
+        double ab = 0.0;
+        while (e < n && terms[e].lin == terms[k].lin) { s += (long double)terms[e].p; ab += fabs(terms[e].p); e++; }
+        out->absum[o] = ab;
+        out->nterms[o] = (uint32_t)(e - k);
         uint32_t c[2 * OC_MAXD];
         decode(terms[k].lin, E, dC, c);
         for (int m = 0; m < dC; m++) out->coords[(size_t)m * uniq + o] = c[m];
@@ -566,17 +596,23 @@ int oc_result_sort(oc_result *r) {
     uint64_t *idx = (uint64_t *)malloc(sizeof(uint64_t) * n);
     uint32_t *nc = (uint32_t *)malloc(sizeof(uint32_t) * n * (r->order > 0 ? r->order : 1));
     double *nv = (double *)malloc(sizeof(double) * n);
-    if (!idx || !nc || !nv) { free(idx); free(nc); free(nv); return -2; }
+    double *na = (double *)malloc(sizeof(double) * n);
+    uint32_t *nn = (uint32_t *)malloc(sizeof(uint32_t) * n);
+    if (!idx || !nc || !nv || !na || !nn) { free(idx); free(nc); free(nv); free(na); free(nn); return -2; }
     for (uint64_t i = 0; i < n; i++) idx[i] = i;
     g_sort_r = r;
     qsort(idx, n, sizeof(uint64_t), idx_cmp);
     for (uint64_t k = 0; k < n; k++) {
         for (int m = 0; m < r->order; m++) nc[(uint64_t)m * n + k] = r->coords[(uint64_t)m * n + idx[k]];
         nv[k] = r->vals[idx[k]];
+        na[k] = r->absum ? r->absum[idx[k]] : 0.0;
+        nn[k] = r->nterms ? r->nterms[idx[k]] : 0;
     }
-    free(r->coords); free(r->vals); free(idx);
+    free(r->coords); free(r->vals); free(r->absum); free(r->nterms); free(idx);
     r->coords = nc;
     r->vals = nv;
+    r->absum = na;
+    r->nterms = nn;
     return 0;
 }
 
@@ -584,6 +620,10 @@ void oc_result_free(oc_result *r) {
     if (!r) return;
     free(r->coords);
     free(r->vals);
+    free(r->absum);
+    free(r->nterms);
     r->coords = NULL;
     r->vals = NULL;
+    r->absum = NULL;
+    r->nterms = NULL;
 }

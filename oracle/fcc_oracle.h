@@ -23,6 +23,8 @@ typedef struct {
     int64_t nnz;       /* output nonzeros (structural support) */
     uint32_t *coords;  /* SoA [order][nnz], malloc'd; NULL in sink mode */
     double *vals;      /* [nnz], malloc'd; NULL in sink mode */
+    double *absum;     /* [nnz] sum of |a*b| over the output's terms (materialised mode) */
+    uint32_t *nterms;  /* [nnz] number of terms (products) of each output */
     /* order-independent summaries, filled in both modes */
     uint64_t coord_hash;  /* sum over outputs of mix64(linear output index) mod 2^64 */
     double val_sum;       /* sum of output values */

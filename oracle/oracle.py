@@ -22,6 +22,8 @@ class _OcResult(C.Structure):
         ("nnz", C.c_int64),
         ("coords", C.POINTER(C.c_uint32)),
         ("vals", C.POINTER(C.c_double)),
+        ("absum", C.POINTER(C.c_double)),
+        ("nterms", C.POINTER(C.c_uint32)),
         ("coord_hash", C.c_uint64),
         ("val_sum", C.c_double),
         ("val_wsum", C.c_double),
@@ -57,8 +59,10 @@ def _load():
 
 
 class OracleResult:
-    def __init__(self, order, coords, vals, coord_hash, val_sum, val_wsum, products, seconds, threads, nnz):
+    def __init__(self, order, coords, vals, coord_hash, val_sum, val_wsum, products, seconds, threads, nnz,
+                 absum=None, nterms=None):
         self.order, self.coords, self.vals = order, coords, vals
+        self.absum, self.nterms = absum, nterms
         self.coord_hash, self.val_sum, self.val_wsum = coord_hash, val_sum, val_wsum
         self.products, self.seconds, self.threads, self.nnz = products, seconds, threads, nnz
 
@@ -75,7 +79,7 @@ def _mk(t, keep):
 
 
 def _take(lib, r: _OcResult, sort: bool) -> OracleResult:
-    coords = vals = None
+    coords = vals = absum = nterms = None
     if r.coords:
         if sort:
             rc = lib.oc_result_sort(C.byref(r))
@@ -85,8 +89,11 @@ def _take(lib, r: _OcResult, sort: bool) -> OracleResult:
         coords = np.ctypeslib.as_array(r.coords, shape=(max(r.order, 1) * max(n, 1),))[: r.order * n].copy()
         coords = coords.reshape(r.order, n)
         vals = np.ctypeslib.as_array(r.vals, shape=(max(n, 1),))[:n].copy()
+        if r.absum:
+            absum = np.ctypeslib.as_array(r.absum, shape=(max(n, 1),))[:n].copy()
+            nterms = np.ctypeslib.as_array(r.nterms, shape=(max(n, 1),))[:n].copy()
     out = OracleResult(r.order, coords, vals, r.coord_hash, r.val_sum, r.val_wsum, r.products, r.seconds,
-                       r.threads, r.nnz)
+                       r.threads, r.nnz, absum, nterms)
     lib.oc_result_free(C.byref(r))
     return out
 

@@ -42,21 +42,42 @@ def densify(t, dims=None):
     return d
 
 
-def sort_result(coords, vals):
+def sort_order(coords):
     if coords.shape[0] == 0 or coords.shape[1] <= 1:
-        return coords, vals
-    order = np.lexsort(coords[::-1])
-    return coords[:, order], vals[order]
+        return np.arange(coords.shape[1])
+    return np.lexsort(coords[::-1])
 
 
-def assert_same_result(gc, gv, rc, rv, rel=1e-12, abs_floor=1e-14, what=""):
-    """Sorted coordinate sets bit-exact; values |g - r| <= max(rel*|r|, abs_floor)."""
+def sort_result(coords, vals):
+    o = sort_order(coords)
+    return coords[:, o], vals[o]
+
+
+def assert_same_result(gc, gv, rc, rv, rel=1e-12, abs_floor=1e-14, what="", absum=None, nterms=None):
+    """Sorted coordinate sets bit-exact; values |g - r| <= max(rel*|r|, abs_floor).
+
+    When the oracle supplies, per output, the sum of |terms| and the number of
+    terms, an output whose sum cancels may instead meet the worst-case error of
+    an fp64 sum in an arbitrary order, (n+1) * 2^-53 * sum|terms| (one rounding
+    per product and per addition); the number of outputs that needed this is
+    printed, and must stay a small fraction."""
     gc, gv = sort_result(np.asarray(gc), np.asarray(gv))
-    rc, rv = sort_result(np.asarray(rc), np.asarray(rv))
+    o = sort_order(np.asarray(rc))
+    rc, rv = np.asarray(rc)[:, o], np.asarray(rv)[o]
     assert gc.shape == rc.shape, f"{what}: nnz/order differ {gc.shape} vs {rc.shape}"
     assert np.array_equal(gc, rc), f"{what}: coordinate sets differ"
+    err = np.abs(gv - rv)
     tol = np.maximum(rel * np.abs(rv), abs_floor)
-    bad = np.abs(gv - rv) > tol
+    strict_bad = err > tol
+    if absum is not None and nterms is not None and strict_bad.any():
+        ab, nt = np.asarray(absum)[o], np.asarray(nterms)[o].astype(np.float64)
+        tol_order = (nt + 1.0) * 2.0 ** -53 * ab
+        bad = err > np.maximum(tol, tol_order)
+        n_relaxed = int(strict_bad.sum() - bad.sum())
+        print(f"{what}: {n_relaxed} of {rv.size} outputs cancel and meet the summation-order bound")
+        assert n_relaxed <= max(10, rv.size // 10000), f"{what}: too many cancelling outputs ({n_relaxed})"
+    else:
+        bad = strict_bad
     assert not bad.any(), (f"{what}: {bad.sum()} values out of tolerance; worst "
                            f"{np.max(np.abs(gv - rv) / np.maximum(np.abs(rv), 1e-300))}")
 

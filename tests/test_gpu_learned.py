@@ -45,7 +45,8 @@ def test_learned_equals_oracle(trained_ctx, mk):
     ref = oracle.contract_tiled(a, wl.a_modes, b, wl.b_modes, tile_size=128)
     r = ctx.contract(a, list(wl.a_modes), b, list(wl.b_modes), planner="learned")
     c, v = r.to_host()
-    assert_same_result(c, v, ref.coords, ref.vals, rel=REL, abs_floor=ABS, what=wl.name)
+    assert_same_result(c, v, ref.coords, ref.vals, rel=REL, abs_floor=ABS, what=wl.name, absum=ref.absum,
+                           nterms=ref.nterms)
     assert r.stats["ms_symbolic"] == 0.0
     assert r.stats["mlp_mape"] >= 0.0
     r.free()
@@ -67,7 +68,8 @@ def test_any_prediction_stays_exact(bias):
     ref = oracle.contract_tiled(a, wl.a_modes, b, wl.b_modes, tile_size=64)
     r = ctx.contract(a, list(wl.a_modes), b, list(wl.b_modes), planner="learned")
     c, v = r.to_host()
-    assert_same_result(c, v, ref.coords, ref.vals, rel=REL, abs_floor=ABS, what=f"bias {bias}")
+    assert_same_result(c, v, ref.coords, ref.vals, rel=REL, abs_floor=ABS, what=f"bias {bias}", absum=ref.absum,
+                       nterms=ref.nterms)
     if bias < 0:
         assert r.stats["reruns"] >= 1 or r.stats["ghash_entries"] > 0
     r.free()

@@ -51,7 +51,8 @@ def test_small_cases(gpu_ctx, case, values, force):
     c, v, st = gpu_contract(gpu_ctx, a, am, b, bm, force_acc=force)
     exact = values != "uniform"
     assert_same_result(c, v, ref.coords, ref.vals, rel=0 if exact else REL, abs_floor=0 if exact else ABS,
-                       what=f"case {case} {values} force {force}")
+                       what=f"case {case} {values} force {force}", absum=None if exact else ref.absum,
+                       nterms=None if exact else ref.nterms)
     assert st["products"] == ref.products
 
 
@@ -90,7 +91,8 @@ def test_scaled_workloads(gpu_ctx, name, mk, planner_sink):
     assert r.val_wsum == pytest.approx(ref.val_wsum, rel=1e-10, abs=1e-10)
     if not sink:
         c, v = r.to_host()
-        assert_same_result(c, v, ref.coords, ref.vals, rel=REL, abs_floor=ABS, what=wl.name)
+        assert_same_result(c, v, ref.coords, ref.vals, rel=REL, abs_floor=ABS, what=wl.name, absum=ref.absum,
+                           nterms=ref.nterms)
     assert r.stats["products"] == ref.products
     r.free()
 
@@ -101,7 +103,8 @@ def test_sampled_left_tiles(gpu_ctx):
     ref = oracle.contract_tiled(a, wl.a_modes, a, wl.b_modes, tile_size=64, sample_stride=16)
     r = gpu_ctx.contract(a, list(wl.a_modes), a, list(wl.b_modes), tile_hint=64, left_tile_stride=16)
     c, v = r.to_host()
-    assert_same_result(c, v, ref.coords, ref.vals, rel=REL, abs_floor=ABS, what="sampled")
+    assert_same_result(c, v, ref.coords, ref.vals, rel=REL, abs_floor=ABS, what="sampled", absum=ref.absum,
+                       nterms=ref.nterms)
     r.free()
 
 

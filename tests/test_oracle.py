@@ -209,3 +209,23 @@ def test_golden_vectors():
         val = einsum_contract(a, c["a_modes"], b, c["b_modes"], a.extents(), b.extents())
         rc, rv = dense_to_coo(cnt, val)
         assert_same_result(ec, np.array(c["c_vals"]), rc, rv, rel=0, abs_floor=0, what=c["name"] + " einsum")
+
+
+def test_absum_and_nterms():
+    """Per-output sum of |terms| and term count (the summation-order tolerance)."""
+    rng = np.random.default_rng(21)
+    a = rand_tensor(rng, (8, 5, 7), 150)
+    b = rand_tensor(rng, (5, 7, 9), 140)
+    for res in (oracle.contract_tiled(a, (1, 2), b, (0, 1), tile_size=4, threads=2),
+                oracle.contract_bruteforce(a, (1, 2), b, (0, 1))):
+        Ea, Eb = a.extents(), b.extents()
+        cnt = einsum_contract(a, (1, 2), b, (0, 1), Ea, Eb, boolean=True)
+        aa = type(a)(a.coords, np.abs(a.vals))
+        bb = type(b)(b.coords, np.abs(b.vals))
+        ab = einsum_contract(aa, (1, 2), bb, (0, 1), Ea, Eb)
+        rc, rab = dense_to_coo(cnt, ab)
+        _, rn = dense_to_coo(cnt, cnt.astype(np.float64))
+        o = np.lexsort(res.coords[::-1])
+        assert np.array_equal(res.coords[:, o], rc)
+        np.testing.assert_allclose(res.absum[o], rab, rtol=1e-12)
+        assert np.array_equal(res.nterms[o], rn.astype(np.uint32))
