@@ -5,7 +5,11 @@ exact planner, accumulate, compaction into the SoA output) of the synthetic
 workload with its inputs already resident in HBM.
 
 Default workload: the order-4 self-contraction T(6186x24x77x32, 5.3M nnz,
-mode-0 Zipf(1.1), lognormal values) on modes {0,1}, exact-symbolic planning.
+mode-0 Zipf(1.1), lognormal values) on modes {0,1}, exact-symbolic planning,
+with the learned planner timed beside it on the same inputs.
+--workload c5: 256 uneven independent pairs A_b(1024x128x1024x64) x
+B_b(2048x128x64) in one fused launch (learned planner, count + checksum
+outputs), reporting aggregate output nnz/s and per-pair latency percentiles.
 Prints one JSON line.
 """
 import argparse
@@ -18,6 +22,7 @@ import time
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path[:0] = [os.path.join(ROOT, "fastcc-benchmark_amd"), ROOT]
 
+MLP_PARAMS = os.path.join(ROOT, "fastcc-benchmark_amd", "fastcc_amd", "mlp_params.npy")
 HBM_PEAK_GBS = 8000.0
 FP64_PEAK_TFLOPS = 78.6
 
@@ -37,8 +42,11 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", default="c2", choices=["c1", "c2", "c3", "c4"])
-    ap.add_argument("--planner", default="exact", choices=["exact", "learned"])
+    ap.add_argument("--workload", default="c2", choices=["c1", "c2", "c3", "c4", "c5"])
+    ap.add_argument("--planner", default=None, choices=["exact", "learned"],
+                    help="default: exact (c1-c4), learned (c5)")
+    ap.add_argument("--no-compare", action="store_true", help="skip timing the other planner")
+    ap.add_argument("--pairs", type=int, default=256, help="c5: number of pairs")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-stride", type=int, default=0, help="left-tile sample stride for the CPU baseline")
     ap.add_argument("--cpu-threads", type=int, default=0)
@@ -67,6 +75,10 @@ def main():
     from fastcc_amd import Context, DeviceTensor
     from fastcc_amd import workloads as W
 
+    if args.planner is None:
+        args.planner = "learned" if args.workload == "c5" else "exact"
+    if args.workload == "c5":
+        return bench_batch(args)
     wl = W.FULL[args.workload]()
     ta, tb = wl.tensors()
     dev = torch.device("cuda:0")
@@ -82,46 +94,44 @@ def main():
     torch.cuda.synchronize()
 
     ctx = Context(0)
-    opts = dict(planner=args.planner, tile_hint=args.tile_hint, col_window=args.col_window,
-                workgroups=args.workgroups)
-    if args.planner == "learned":
-        params = os.path.join(ROOT, "fastcc-benchmark_amd", "fastcc_amd", "mlp_params.npy")
-        ctx.set_mlp_params(np.load(params))
+    ctx.set_mlp_params(np.load(MLP_PARAMS))
+    sink = args.workload == "c3"  # ~3.3e10 outputs do not fit in HBM: count + checksum
+    opts = dict(tile_hint=args.tile_hint, col_window=args.col_window, workgroups=args.workgroups, sink=sink)
 
-    def step():
-        r = ctx.contract(da, list(wl.a_modes), db, list(wl.b_modes), **opts)
-        n, st = r.nnz, r.stats
-        r.free()
-        return n, st
+    def run(planner):
+        def step():
+            r = ctx.contract(da, list(wl.a_modes), db, list(wl.b_modes), planner=planner, **opts)
+            n, st = r.nnz, r.stats
+            r.free()
+            return n, st
 
-    for _ in range(args.warmup):
-        step()
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    stats = []
-    nnz = 0
-    for _ in range(args.steps):
-        nnz, st = step()
-        stats.append(st)
-    torch.cuda.synchronize()
-    dt = (time.perf_counter() - t0) / max(args.steps, 1)
+        for _ in range(args.warmup):
+            step()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        stats = []
+        nnz = 0
+        for _ in range(args.steps):
+            nnz, st = step()
+            stats.append(st)
+        torch.cuda.synchronize()
+        return nnz, stats, (time.perf_counter() - t0) / max(args.steps, 1)
+
+    nnz, stats, dt = run(args.planner)
+    compare = None
+    if not args.no_compare:
+        other = "exact" if args.planner == "learned" else "learned"
+        _, ostats, odt = run(other)
+        ms = {args.planner: dt * 1e3, other: odt * 1e3}
+        compare = {"exact_ms_per_step": ms["exact"], "learned_ms_per_step": ms["learned"],
+                   "learned_speedup": ms["exact"] / ms["learned"],
+                   "learned_mlp_mape": (stats if args.planner == "learned" else ostats)[-1]["mlp_mape"]}
 
     products = stats[-1]["products"]
     bytes_alg = stats[-1]["bytes_alg"]
     k_ms = float(np.mean([s["ms_numeric_kernel"] for s in stats]))
     dev_ms = float(np.mean([s["ms_total"] for s in stats]))
-    # dominant kernel: numeric accumulate; FP64-bound when 2*products/peak exceeds bytes/HBM peak
-    t_fp64 = 2.0 * products / (FP64_PEAK_TFLOPS * 1e12)
-    t_hbm = bytes_alg / (HBM_PEAK_GBS * 1e9)
-    if t_fp64 >= t_hbm:
-        achieved = 2.0 * products / (k_ms * 1e-3) / 1e12
-        roof = {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
-                "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None, "kernel": "k_numeric",
-                "note": "fp64 multiply-adds of the numeric accumulate kernel; fp64 peak (no MFMA used)"}
-    else:
-        achieved = bytes_alg / (k_ms * 1e-3) / 1e9
-        roof = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": "k_numeric"}
+    roof = roofline(products, bytes_alg, k_ms)
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -131,9 +141,13 @@ def main():
         r = oracle.contract_tiled(ta, wl.a_modes, tb, wl.b_modes, tile_size=128, threads=threads,
                                   high_precision=False, sample_stride=stride, sink=args.workload in ("c3", "c4"),
                                   sort=False)
+        if stride > 1:
+            cpu_note = "sampled left tiles, rate over the sample"
+        else:
+            cpu_note = "full contraction"
         cpu = {"value": r.nnz / r.seconds, "unit": "output nnz/s", "cores": r.threads, "kind": "port",
                "sample": f"left tiles t % {stride} == 0 at T=128 ({r.nnz} outputs, {r.products} products, "
-                         f"{r.seconds:.2f} s); CPU {cpu_model()}",
+                         f"{r.seconds:.2f} s, {cpu_note}); CPU {cpu_model()}",
                "seconds": r.seconds}
 
     line = {
@@ -151,9 +165,124 @@ def main():
         "dtype": "f64",
         "data": "synthetic (seeded generator, csrc/fcc_gen.c)",
         "config": {"workload": wl.name, "description": wl.description, "planner": args.planner,
-                   "output_nnz": int(nnz), "products": int(products)},
+                   "output": "count + checksum" if sink else "materialised SoA", "output_nnz": int(nnz),
+                   "products": int(products)},
         "hbm_gbs_alg": bytes_alg / dt / 1e9,
         "device_ms": dev_ms,
+        "phases_ms": {k: float(np.mean([s[k] for s in stats])) for k in
+                      ("ms_index", "ms_plan", "ms_symbolic", "ms_numeric", "ms_overflow")},
+        "planner_comparison": compare,
+        "roofline": roof,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+    ctx.close()
+
+
+def roofline(products, bytes_alg, k_ms, kernel="k_numeric"):
+    """Dominant kernel's bound: FP64 when 2*products/peak exceeds bytes/HBM peak."""
+    t_fp64 = 2.0 * products / (FP64_PEAK_TFLOPS * 1e12)
+    t_hbm = bytes_alg / (HBM_PEAK_GBS * 1e9)
+    if t_fp64 >= t_hbm:
+        achieved = 2.0 * products / (k_ms * 1e-3) / 1e12
+        return {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None, "kernel": kernel,
+                "note": "fp64 multiply-adds of the numeric accumulate kernel; fp64 peak (no MFMA used)"}
+    achieved = bytes_alg / (k_ms * 1e-3) / 1e9
+    return {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": kernel}
+
+
+def bench_batch(args):
+    """256 uneven independent pairs in one fused launch per step."""
+    import numpy as np
+    import torch
+    from concurrent.futures import ThreadPoolExecutor
+    from fastcc_amd import Context, DeviceTensor
+    from fastcc_amd import workloads as W
+
+    wls = W.c5_pairs(n_pairs=args.pairs)
+    with ThreadPoolExecutor(max_workers=min(16, os.cpu_count() or 1)) as ex:
+        host = list(ex.map(lambda w: w.tensors(), wls))
+    dev = torch.device("cuda:0")
+    pairs = []
+    for ta, tb in host:
+        ca = torch.from_numpy(ta.coords.view(np.int32)).to(dev).contiguous()
+        va = torch.from_numpy(ta.vals).to(dev).contiguous()
+        cb = torch.from_numpy(tb.coords.view(np.int32)).to(dev).contiguous()
+        vb = torch.from_numpy(tb.vals).to(dev).contiguous()
+        pairs.append((DeviceTensor.from_torch(ca, va), DeviceTensor.from_torch(cb, vb)))
+    torch.cuda.synchronize()
+    ctx = Context(0)
+    ctx.set_mlp_params(np.load(MLP_PARAMS))
+    a_modes, b_modes = list(wls[0].a_modes), list(wls[0].b_modes)
+
+    def step():
+        rs = ctx.contract_batch(pairs, a_modes, b_modes, planner=args.planner, sink=True)
+        n = sum(r.nnz for r in rs)
+        lat = [r.stats["ms_done"] for r in rs]
+        st = rs[0].stats
+        for r in rs:
+            r.free()
+        return n, lat, st
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    lats, stats = [], []
+    nnz = 0
+    for _ in range(args.steps):
+        nnz, lat, st = step()
+        lats += lat
+        stats.append(st)
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / max(args.steps, 1)
+    k_ms = float(np.mean([s["ms_numeric_kernel"] for s in stats]))
+    roof = roofline(stats[-1]["products"], stats[-1]["bytes_alg"], k_ms)
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        from oracle import oracle
+        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+        stride = args.cpu_stride or 8
+        n_s, sec, prod = 0, 0.0, 0
+        for i in range(0, len(host), stride):
+            ta, tb = host[i]
+            r = oracle.contract_tiled(ta, wls[i].a_modes, tb, wls[i].b_modes, tile_size=128, threads=threads,
+                                      high_precision=False, sink=True, sort=False)
+            n_s += r.nnz
+            sec += r.seconds
+            prod += r.products
+        cpu = {"value": n_s / sec, "unit": "output nnz/s", "cores": threads, "kind": "port",
+               "sample": f"every {stride}th pair, each contracted whole at T=128 with count + checksum outputs "
+                         f"({n_s} outputs, {prod} products, {sec:.2f} s); CPU {cpu_model()}",
+               "seconds": sec}
+
+    line = {
+        "metric": f"aggregate output nnz/s ({args.planner} planning, {len(pairs)} pairs in one fused launch)",
+        "value": nnz / dt,
+        "unit": "output nnz/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": dt * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded generator, csrc/fcc_gen.c)",
+        "config": {"workload": "c5_batch_uneven_pairs", "pairs": len(pairs),
+                   "description": "A_b 1024x128x1024x64 x B_b 2048x128x64, modes {1,3}x{1,2}, nnz log-uniform "
+                                  "A 10k-2M / B 5k-1M, mode-1 Zipf(1.1), normal values",
+                   "planner": args.planner, "output": "count + checksum per pair", "output_nnz": int(nnz),
+                   "products": int(stats[-1]["products"]),
+                   "nnz_a": int(sum(t[0].nnz for t in host)), "nnz_b": int(sum(t[1].nnz for t in host))},
+        "latency_ms": {"p50": float(np.percentile(lats, 50)), "p99": float(np.percentile(lats, 99)),
+                       "max": float(np.max(lats)),
+                       "note": "per pair: call start to the pair's output complete, inside the fused launch"},
+        "hbm_gbs_alg": stats[-1]["bytes_alg"] / dt / 1e9,
+        "device_ms": float(np.mean([s["ms_total"] for s in stats])),
         "phases_ms": {k: float(np.mean([s[k] for s in stats])) for k in
                       ("ms_index", "ms_plan", "ms_symbolic", "ms_numeric", "ms_overflow")},
         "roofline": roof,

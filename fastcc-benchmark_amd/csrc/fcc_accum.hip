@@ -528,16 +528,36 @@ __device__ uint32_t block_count(uint32_t n, uint32_t *s_w, Pred pred) {
 struct SinkAcc {
     uint64_t count = 0, hash = 0;
     double vsum = 0.0, wsum = 0.0;
+    uint64_t group = 0;  // row group this accumulator belongs to
 };
+
+__device__ __forceinline__ double hash_weight(uint64_t z) { return (double)(z >> 11) * 0x1.0p-53; }
+
+__device__ __forceinline__ void sink_atomic(const AccArgs &A, uint64_t g, unsigned long long c, unsigned long long h,
+                                            double vs, double ws) {
+    atomicAdd(&A.sink[4 * g], c);
+    atomicAdd(&A.sink[4 * g + 1], h);
+    unsafeAtomicAdd(&A.sinkf[4 * g], vs);
+    unsafeAtomicAdd(&A.sinkf[4 * g + 1], ws);
+}
 
 __device__ __forceinline__ void emit_output(const AccArgs &A, uint64_t pos, uint64_t row, uint64_t col, double v,
                                             bool sink, SinkAcc &sk) {
     if (sink) {
+        if (A.group_rows) {
+            const uint64_t g = row / A.group_rows;
+            row -= g * A.group_rows;
+            if (g != sk.group) {  // a unit straddling two groups: add this output directly
+                uint64_t z = mix64(row * A.spanB + col);
+                sink_atomic(A, g, 1ull, z, v, v * hash_weight(z));
+                return;
+            }
+        }
         uint64_t z = mix64(row * A.spanB + col);
         sk.count++;
         sk.hash += z;
         sk.vsum += v;
-        sk.wsum += v * ((double)(z >> 11) * 0x1.0p-53);
+        sk.wsum += v * hash_weight(z);
         return;
     }
     if (pos >= A.capacity) {  // a count/fill disagreement: flag it, never write past the output
@@ -584,12 +604,7 @@ __device__ void sink_flush(const AccArgs &A, SinkAcc &sk, unsigned long long *s_
             vs += s_d[2 * w];
             ws += s_d[2 * w + 1];
         }
-        if (c) {
-            atomicAdd(&A.sink[0], c);
-            atomicAdd(&A.sink[1], h);
-            unsafeAtomicAdd(&A.sinkf[0], vs);
-            unsafeAtomicAdd(&A.sinkf[1], ws);
-        }
+        if (c) sink_atomic(A, sk.group, c, h, vs, ws);
     }
 }
 
@@ -625,6 +640,8 @@ __global__ __launch_bounds__(ACC_THREADS) void k_numeric(AccArgs A) {
         U.row0 = (uint64_t)U.tile << A.logTL;
         const uint32_t TL = 1u << A.logTL;
         const uint32_t nslots = U.kind == KIND_DENSE ? TL * U.Wp : (U.kind == KIND_LHASH ? LH_SLOTS : 0);
+        if (A.group_rows) sk.group = U.row0 / A.group_rows;
+        if (A.clock_start && ui == 0 && threadIdx.x == 0) *A.clock_start = wall_clock64();
         if (U.kind == KIND_DENSE) {
             const double init = BITMAP ? 0.0 : -0.0;
             for (uint32_t s = threadIdx.x; s < nslots; s += ACC_THREADS) s_f64[s] = init;
@@ -680,9 +697,18 @@ __global__ __launch_bounds__(ACC_THREADS) void k_numeric(AccArgs A) {
         } else if (!SINK && threadIdx.x == 0) {
             A.unit_cursor[U.u] = 0;
         }
+        if (SINK && A.group_rows) {  // summaries per group: flush at every unit's end
+            sink_flush(A, sk, s_su, s_sd);
+            sk = SinkAcc();
+        }
+        if (A.group_left && threadIdx.x == 0) {
+            const uint64_t g = U.row0 / A.group_rows;
+            __threadfence();
+            if (atomicSub(&A.group_left[g], 1u) == 1u) A.group_done[g] = wall_clock64();
+        }
         __syncthreads();
     }
-    if (SINK) sink_flush(A, sk, s_su, s_sd);
+    if (SINK && !A.group_rows) sink_flush(A, sk, s_su, s_sd);
 }
 
 // Count pass: structural support only.
@@ -758,6 +784,30 @@ __global__ __launch_bounds__(256) void k_gdrain(AccArgs A) {
         uint64_t row = t ? key / A.spanB : 0, col = t ? key - row * A.spanB : 0;
         double v = t ? A.gvals[h] : 0.0;
         if (SINK) {
+            if (A.group_rows) {
+                // per-group summaries: one set of atomics per wave when the
+                // wave's entries share a group, else per entry
+                const uint64_t g = t ? row / A.group_rows : 0;
+                const uint64_t lrow = row - g * A.group_rows;
+                const uint64_t z = t ? mix64(lrow * A.spanB + col) : 0;
+                const uint64_t act = __ballot(t);
+                if (!act) continue;
+                const uint64_t g0 = __shfl(g, __ffsll((unsigned long long)act) - 1, 64);
+                if (__ballot(t && g != g0) == 0) {
+                    unsigned long long c = t ? 1ull : 0ull, hz = z;
+                    double vs = v, ws = t ? v * hash_weight(z) : 0.0;
+                    for (int d = 32; d > 0; d >>= 1) {
+                        c += __shfl_xor(c, d, 64);
+                        hz += __shfl_xor(hz, d, 64);
+                        vs += __shfl_xor(vs, d, 64);
+                        ws += __shfl_xor(ws, d, 64);
+                    }
+                    if (lane == 0) sink_atomic(A, g0, c, hz, vs, ws);
+                } else if (t) {
+                    sink_atomic(A, g, 1ull, z, v, v * hash_weight(z));
+                }
+                continue;
+            }
             if (t) emit_output(A, 0, row, col, v, true, sk);
             continue;
         }
@@ -795,12 +845,7 @@ __global__ __launch_bounds__(256) void k_gdrain(AccArgs A) {
             sk.vsum += __shfl_xor(sk.vsum, d, 64);
             sk.wsum += __shfl_xor(sk.wsum, d, 64);
         }
-        if (lane == 0 && sk.count) {
-            atomicAdd(&A.sink[0], (unsigned long long)sk.count);
-            atomicAdd(&A.sink[1], (unsigned long long)sk.hash);
-            unsafeAtomicAdd(&A.sinkf[0], sk.vsum);
-            unsafeAtomicAdd(&A.sinkf[1], sk.wsum);
-        }
+        if (lane == 0 && sk.count) sink_atomic(A, 0, sk.count, sk.hash, sk.vsum, sk.wsum);
     }
 }
 

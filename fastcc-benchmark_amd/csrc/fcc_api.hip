@@ -22,6 +22,7 @@
 
 #include "../../include/fcc.h"
 #include "fcc_accum.h"
+#include "fcc_batch.h"
 #include "fcc_mlp.h"
 
 using namespace fcc;
@@ -32,6 +33,7 @@ struct fcc_ctx {
     DevPool pool;
     std::string err;
     int num_cus = 256;
+    int clock_khz = 100000;  // wall_clock64 rate
     // learned planner
     std::vector<float> mlp_host;
     float *mlp_dev = nullptr;
@@ -45,9 +47,22 @@ struct fcc_ctx {
 
 namespace {
 
+// Device storage shared by the per-pair results of one fused batch; returned
+// to the pool when the last of them is freed.
+struct SharedBlock {
+    DevPool *pool = nullptr;
+    void *coords = nullptr;
+    void *vals = nullptr;
+    ~SharedBlock() {
+        if (coords) pool->put(coords);
+        if (vals) pool->put(vals);
+    }
+};
+
 struct ResultPriv {
     void *coords = nullptr;
     void *vals = nullptr;
+    std::shared_ptr<SharedBlock> block;  // set instead of coords/vals for batch results
 };
 
 int fail(fcc_ctx *ctx, int code, const std::string &msg) {
@@ -64,18 +79,23 @@ float ev_ms(hipEvent_t a, hipEvent_t b) {
     return ms;
 }
 
-Geom make_geom(const fcc_tensor *a, const int32_t *am, const fcc_tensor *b, const int32_t *bm, int r,
+// Key modes: nb batch pairs first, then the contraction pairs (ka/kb, r
+// entries in all). A's external index leads with the batch coordinates (radix
+// = the key radix of their pair), then A's other modes ascending; B's external
+// index is B's modes outside kb, ascending. Outputs therefore carry (batch,
+// A externals, B externals), the batch-mode order of index_tensors.hpp:322-325.
+Geom make_geom(const fcc_tensor *a, const int32_t *ka, const fcc_tensor *b, const int32_t *kb, int r, int nb,
                const uint32_t *EA, const uint32_t *EB) {
     Geom g;
     memset(&g, 0, sizeof(g));
     g.dA = a->order;
     g.dB = b->order;
     g.r = r;
-    g.dC = a->order + b->order - 2 * r;
+    g.dC = a->order + b->order - 2 * r + nb;
     for (int i = 0; i < r; i++) {
-        g.am[i] = am[i];
-        g.bm[i] = bm[i];
-        g.R[i] = std::max<uint64_t>(EA[am[i]], EB[bm[i]]);
+        g.am[i] = ka[i];
+        g.bm[i] = kb[i];
+        g.R[i] = std::max<uint64_t>(EA[ka[i]], EB[kb[i]]);
     }
     auto in = [](const int32_t *l, int n, int m) {
         for (int i = 0; i < n; i++)
@@ -84,15 +104,21 @@ Geom make_geom(const fcc_tensor *a, const int32_t *am, const fcc_tensor *b, cons
     };
     g.spanA = 1;
     g.spanB = 1;
+    for (int i = 0; i < nb; i++) {
+        g.xa[g.nxa] = ka[i];
+        g.xaE[g.nxa] = (uint32_t)g.R[i];
+        g.spanA *= g.R[i];
+        g.nxa++;
+    }
     for (int m = 0; m < a->order; m++)
-        if (!in(am, r, m)) {
+        if (!in(ka, r, m)) {
             g.xa[g.nxa] = m;
             g.xaE[g.nxa] = EA[m];
             g.spanA *= EA[m];
             g.nxa++;
         }
     for (int m = 0; m < b->order; m++)
-        if (!in(bm, r, m)) {
+        if (!in(kb, r, m)) {
             g.xb[g.nxb] = m;
             g.xbE[g.nxb] = EB[m];
             g.spanB *= EB[m];
@@ -325,11 +351,18 @@ struct Contraction {
     hipStream_t s;
     fcc_options o;
     fcc_result *out;
+    // fused batch: the leading batch coordinate is a group index in [0, groups)
+    // with extent groups; summaries and completion times are kept per group
+    uint32_t groups = 0;
+    std::vector<unsigned long long> group_sink;  // 4 per group: count, hash, val_sum, val_wsum bits
+    std::vector<double> group_ms;                // per group: call start -> output complete
 
-    int run(const fcc_tensor *a, const int32_t *am, const fcc_tensor *b, const int32_t *bm, int r);
+    // ka/kb: r key pairs, the first nb of them batch pairs
+    int run(const fcc_tensor *a, const int32_t *ka, const fcc_tensor *b, const int32_t *kb, int r, int nb);
 };
 
-int Contraction::run(const fcc_tensor *ta, const int32_t *am, const fcc_tensor *tb, const int32_t *bm, int r) {
+int Contraction::run(const fcc_tensor *ta, const int32_t *am, const fcc_tensor *tb, const int32_t *bm, int r,
+                     int nb) {
     DevPool &pool = ctx->pool;
     hipEvent_t *ev = ctx->ev;
     FCC_HIP(hipEventRecord(ev[0], s));
@@ -382,7 +415,7 @@ int Contraction::run(const fcc_tensor *ta, const int32_t *am, const fcc_tensor *
             if (tb->nnz && hext[kMaxOrder + 1 + m] >= EB[m])
                 throw LibError{FCC_ERR_INVALID, "b: coordinate exceeds given extent"};
 
-    Geom g = make_geom(ta, am, tb, bm, r, EA, EB);
+    Geom g = make_geom(ta, am, tb, bm, r, nb, EA, EB);
     out->order = g.dC;
     for (int m = 0; m < g.nxa; m++) out->extents[m] = g.xaE[m];
     for (int m = 0; m < g.nxb; m++) out->extents[g.nxa + m] = g.xbE[m];
@@ -405,7 +438,8 @@ int Contraction::run(const fcc_tensor *ta, const int32_t *am, const fcc_tensor *
     Tmp<uint32_t> rowA(pool, (size_t)A.nnz), fidA(pool, (size_t)A.nnz);
     uint64_t products = 0;
     uint32_t nvalid = 0;
-    bool self = A.coords == B.coords && A.vals == B.vals && ta->order == tb->order && ta->nnz == tb->nnz;
+    // self-contraction shortcut: A's external index is B's column index
+    bool self = nb == 0 && A.coords == B.coords && A.vals == B.vals && ta->order == tb->order && ta->nnz == tb->nnz;
     for (int i = 0; i < r && self; i++) self = am[i] == bm[i];
     for (int m = 0; m < ta->order && self; m++) self = EA[m] == EB[m];
     Tmp<uint32_t> bfib, vsB;
@@ -456,8 +490,9 @@ int Contraction::run(const fcc_tensor *ta, const int32_t *am, const fcc_tensor *
     if (learned && !ctx->mlp_ready) throw LibError{FCC_ERR_INVALID, "learned planner requested but no MLP parameters set"};
 
     uint64_t nnz_out = 0;
-    Tmp<unsigned long long> sinkbuf(pool, 4);
-    fill_u64(sinkbuf, 4, 0, s);
+    const size_t nsink = 4 * (size_t)std::max<uint32_t>(groups, 1);
+    Tmp<unsigned long long> sinkbuf(pool, nsink);
+    fill_u64(sinkbuf, nsink, 0, s);
     if (ix.natiles == 0) {
         FCC_HIP(hipEventRecord(ev[2], s));
         FCC_HIP(hipEventSynchronize(ev[2]));
@@ -527,6 +562,23 @@ int Contraction::run(const fcc_tensor *ta, const int32_t *am, const fcc_tensor *
     X.fail_flag = flags.get() + 1;
     X.sink = sinkbuf.get();
     X.sinkf = (double *)(sinkbuf.get() + 2);
+    Tmp<unsigned int> g_left;
+    Tmp<unsigned long long> g_done;
+    if (groups) {
+        X.group_rows = g.spanA / groups;
+        g_left = Tmp<unsigned int>(pool, groups);
+        g_done = Tmp<unsigned long long>(pool, groups + 1);
+        X.group_left = g_left;
+        X.group_done = g_done;
+        X.clock_start = g_done.get() + groups;
+    }
+    // per-group unit counts, consumed by each numeric launch
+    auto arm_groups = [&]() {
+        if (!groups) return;
+        fill_u32(g_left, groups, 0, s);
+        fill_u64(g_done, groups + 1, 0, s);
+        group_units(X.unit_tile, ix.atile_id, X.nunits, ix.logTL, X.group_rows, g_left, s);
+    };
     unsigned grid = 1;
 
     // units from the exact bounds (pred == null) or from predictions
@@ -615,7 +667,8 @@ int Contraction::run(const fcc_tensor *ta, const int32_t *am, const fcc_tensor *
             X.ostride = stride;
             X.capacity = sink ? ~0ull : capacity;
             reset_ghash();
-            if (sink) fill_u64(sinkbuf, 4, 0, s);
+            if (sink) fill_u64(sinkbuf, nsink, 0, s);
+            arm_groups();
             if (attempt == 0) FCC_HIP(hipEventRecord(ev[4], s));
             launch_numeric(X, grid, bitmap, sink, 1, s);
             if (attempt == 0) FCC_HIP(hipEventRecord(ev[5], s));
@@ -659,7 +712,7 @@ int Contraction::run(const fcc_tensor *ta, const int32_t *am, const fcc_tensor *
     if (use_exact) {
         if (learned) {
             make_units(nullptr);
-            fill_u64(sinkbuf, 4, 0, s);
+            fill_u64(sinkbuf, nsink, 0, s);
         }
         // ---- exact symbolic planning: count, scan, then fill
         alloc_ghash(1u << 16);
@@ -706,6 +759,7 @@ int Contraction::run(const fcc_tensor *ta, const int32_t *am, const fcc_tensor *
         X.capacity = nnz_out;
         if (gtotal > (X.gmask + 1) / 2) alloc_ghash(2 * gtotal);
         reset_ghash();
+        arm_groups();
         FCC_HIP(hipEventRecord(ev[4], s));
         launch_numeric(X, grid, bitmap, sink, 0, s);
         FCC_HIP(hipEventRecord(ev[5], s));
@@ -722,6 +776,13 @@ int Contraction::run(const fcc_tensor *ta, const int32_t *am, const fcc_tensor *
     // ---- results
     unsigned long long sk[4];
     FCC_HIP(hipMemcpyAsync(sk, sinkbuf.get(), 32, hipMemcpyDeviceToHost, s));
+    std::vector<unsigned long long> gdone;
+    if (groups) {
+        group_sink.resize(nsink);
+        gdone.resize(groups + 1);
+        FCC_HIP(hipMemcpyAsync(group_sink.data(), sinkbuf.get(), 8 * nsink, hipMemcpyDeviceToHost, s));
+        FCC_HIP(hipMemcpyAsync(gdone.data(), g_done.get(), 8 * (groups + 1), hipMemcpyDeviceToHost, s));
+    }
     FCC_HIP(hipStreamSynchronize(s));
     if (sink) {
         nnz_out = sk[0];
@@ -779,8 +840,19 @@ int Contraction::run(const fcc_tensor *ta, const int32_t *am, const fcc_tensor *
     st.ms_numeric_kernel = st.ms_numeric;
     st.ms_overflow = ev_ms(ev[5], ev[6]);
     st.ms_total = ev_ms(ev[0], ev[6]);
+    st.ms_done = st.ms_total;
     st.bytes_alg = (uint64_t)ta->nnz * (4 * ta->order + 8) + (uint64_t)tb->nnz * (4 * tb->order + 8) +
                    nnz_out * (4 * (uint64_t)g.dC + 8);
+    if (groups) {
+        // a group's output is complete when its last unit ends; entries that
+        // went through the global hash only after the drain
+        group_ms.assign(groups, st.ms_total);
+        const double t0 = ev_ms(ev[0], ev[4]);
+        const unsigned long long c0 = gdone[groups];
+        if (!st.reruns && !st.ghash_entries && c0)
+            for (uint32_t q = 0; q < groups; q++)
+                if (gdone[q] >= c0) group_ms[q] = t0 + (double)(gdone[q] - c0) / (double)ctx->clock_khz;
+    }
     return FCC_OK;
 }
 
@@ -818,6 +890,9 @@ int fcc_create(fcc_ctx **out, int device) {
         hipDeviceProp_t prop;
         FCC_HIP(hipGetDeviceProperties(&prop, device));
         c->num_cus = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 256;
+        int khz = 0;
+        if (hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, device) == hipSuccess && khz > 0)
+            c->clock_khz = khz;
         FCC_HIP(hipMalloc(&c->mlp_dev, sizeof(float) * MLP_PARAMS));
     } catch (HipError &e) {
         for (auto &ev : c->ev)
@@ -846,25 +921,49 @@ const char *fcc_last_error(const fcc_ctx *c) { return c ? c->err.c_str() : "no c
 
 void *fcc_stream(fcc_ctx *c) { return c ? (void *)c->stream : nullptr; }
 
-int fcc_contract(fcc_ctx *ctx, const fcc_tensor *a, const int32_t *a_modes, const fcc_tensor *b,
-                 const int32_t *b_modes, int32_t n_contr, const fcc_options *options, fcc_result *out) {
+namespace {
+int check_modes(const fcc_tensor *a, const fcc_tensor *b, const int32_t *am, const int32_t *bm, int32_t r,
+                const int32_t *ab, const int32_t *bb, int32_t nb, std::vector<int32_t> &ka, std::vector<int32_t> &kb,
+                std::string &msg) {
+    if (r < 0 || nb < 0 || r + nb > a->order || r + nb > b->order || (r > 0 && (!am || !bm)) ||
+        (nb > 0 && (!ab || !bb))) {
+        msg = "bad contraction/batch mode count";
+        return FCC_ERR_INVALID;
+    }
+    ka.assign(ab, ab + nb);
+    kb.assign(bb, bb + nb);
+    ka.insert(ka.end(), am, am + r);
+    kb.insert(kb.end(), bm, bm + r);
+    for (size_t i = 0; i < ka.size(); i++) {
+        if (ka[i] < 0 || ka[i] >= a->order || kb[i] < 0 || kb[i] >= b->order) {
+            msg = "contraction or batch mode out of range";
+            return FCC_ERR_INVALID;
+        }
+        for (size_t k = 0; k < i; k++)
+            if (ka[k] == ka[i] || kb[k] == kb[i]) {
+                msg = "repeated contraction or batch mode";
+                return FCC_ERR_INVALID;
+            }
+    }
+    if (a->order + b->order - 2 * (r + nb) + nb > FCC_MAX_ORDER) {
+        msg = "output order too large";
+        return FCC_ERR_INVALID;
+    }
+    return FCC_OK;
+}
+
+int contract_impl(fcc_ctx *ctx, const fcc_tensor *a, const int32_t *a_modes, const int32_t *a_batch,
+                  const fcc_tensor *b, const int32_t *b_modes, const int32_t *b_batch, int32_t n_contr,
+                  int32_t n_batch, const fcc_options *options, fcc_result *out) {
     if (!ctx) return FCC_ERR_INVALID;
     if (!out) return fail(ctx, FCC_ERR_INVALID, "out is NULL");
     memset(out, 0, sizeof(*out));
     std::string msg;
     int rc = validate(a, "a", msg);
     if (!rc) rc = validate(b, "b", msg);
+    std::vector<int32_t> ka, kb;
+    if (!rc) rc = check_modes(a, b, a_modes, b_modes, n_contr, a_batch, b_batch, n_batch, ka, kb, msg);
     if (rc) return fail(ctx, rc, msg);
-    if (n_contr < 0 || n_contr > a->order || n_contr > b->order || (n_contr > 0 && (!a_modes || !b_modes)))
-        return fail(ctx, FCC_ERR_INVALID, "bad contraction mode count");
-    for (int i = 0; i < n_contr; i++) {
-        if (a_modes[i] < 0 || a_modes[i] >= a->order || b_modes[i] < 0 || b_modes[i] >= b->order)
-            return fail(ctx, FCC_ERR_INVALID, "contraction mode out of range");
-        for (int k = 0; k < i; k++)
-            if (a_modes[k] == a_modes[i] || b_modes[k] == b_modes[i])
-                return fail(ctx, FCC_ERR_INVALID, "repeated contraction mode");
-    }
-    if (a->order + b->order - 2 * n_contr > FCC_MAX_ORDER) return fail(ctx, FCC_ERR_INVALID, "output order too large");
     Contraction c;
     c.ctx = ctx;
     c.s = ctx->stream;
@@ -874,7 +973,7 @@ int fcc_contract(fcc_ctx *ctx, const fcc_tensor *a, const int32_t *a_modes, cons
     c.out = out;
     try {
         FCC_HIP(hipSetDevice(ctx->device));
-        rc = c.run(a, a_modes, b, b_modes, n_contr);
+        rc = c.run(a, ka.data(), b, kb.data(), n_contr + n_batch, n_batch);
         ctx->err.clear();
         return rc;
     } catch (HipError &e) {
@@ -888,6 +987,18 @@ int fcc_contract(fcc_ctx *ctx, const fcc_tensor *a, const int32_t *a_modes, cons
         fcc_result_free(ctx, out);
         return fail(ctx, FCC_ERR_OOM, "host allocation failed");
     }
+}
+}  // namespace
+
+int fcc_contract(fcc_ctx *ctx, const fcc_tensor *a, const int32_t *a_modes, const fcc_tensor *b,
+                 const int32_t *b_modes, int32_t n_contr, const fcc_options *options, fcc_result *out) {
+    return contract_impl(ctx, a, a_modes, nullptr, b, b_modes, nullptr, n_contr, 0, options, out);
+}
+
+int fcc_contract_ex(fcc_ctx *ctx, const fcc_tensor *a, const int32_t *a_modes, const int32_t *a_batch,
+                    const fcc_tensor *b, const int32_t *b_modes, const int32_t *b_batch, int32_t n_contr,
+                    int32_t n_batch, const fcc_options *options, fcc_result *out) {
+    return contract_impl(ctx, a, a_modes, a_batch, b, b_modes, b_batch, n_contr, n_batch, options, out);
 }
 
 int fcc_result_to_host(fcc_ctx *ctx, const fcc_result *r, uint32_t *coords, double *vals) {
@@ -911,7 +1022,7 @@ int fcc_result_to_host(fcc_ctx *ctx, const fcc_result *r, uint32_t *coords, doub
 void fcc_result_free(fcc_ctx *ctx, fcc_result *r) {
     if (!r) return;
     ResultPriv *rp = (ResultPriv *)r->priv;
-    if (rp && ctx) {
+    if (rp && ctx && !rp->block) {
         if (rp->coords) ctx->pool.put(rp->coords);
         if (rp->vals) ctx->pool.put(rp->vals);
     }
@@ -1077,15 +1188,209 @@ int fcc_mlp_train(fcc_ctx *ctx, const float *features, const float *labels, int6
 
 int fcc_contract_batch(fcc_ctx *ctx, int32_t n, const fcc_tensor *a, const int32_t *a_modes, const fcc_tensor *b,
                        const int32_t *b_modes, int32_t n_contr, const fcc_options *options, fcc_result *outs) {
-    if (!ctx || n < 0 || (n > 0 && (!a || !b || !outs))) return FCC_ERR_INVALID;
-    for (int32_t i = 0; i < n; i++) {
-        int rc = fcc_contract(ctx, &a[i], a_modes, &b[i], b_modes, n_contr, options, &outs[i]);
-        if (rc) {
-            for (int32_t k = 0; k < i; k++) fcc_result_free(ctx, &outs[k]);
-            return rc;
+    if (!ctx) return FCC_ERR_INVALID;
+    if (n < 0 || (n > 0 && (!a || !b || !outs))) return fail(ctx, FCC_ERR_INVALID, "bad batch arguments");
+    for (int32_t i = 0; i < n; i++) memset(&outs[i], 0, sizeof(fcc_result));
+    if (n == 0) return FCC_OK;
+    std::string msg;
+    std::vector<int32_t> ka, kb;
+    int rc = FCC_OK;
+    for (int32_t i = 0; i < n && !rc; i++) {
+        rc = validate(&a[i], "a[i]", msg);
+        if (!rc) rc = validate(&b[i], "b[i]", msg);
+        if (!rc && (a[i].order != a[0].order || b[i].order != b[0].order)) {
+            msg = "batch pairs differ in order";
+            rc = FCC_ERR_INVALID;
         }
     }
-    return FCC_OK;
+    if (!rc) rc = check_modes(&a[0], &b[0], a_modes, b_modes, n_contr, nullptr, nullptr, 0, ka, kb, msg);
+    if (rc) return fail(ctx, rc, msg);
+    const int dA = a[0].order, dB = b[0].order, dC = dA + dB - 2 * n_contr;
+    if (dA + 1 > FCC_MAX_ORDER || dB + 1 > FCC_MAX_ORDER || dC + 1 > FCC_MAX_ORDER) {
+        // no room for the pair-index mode: one call per pair
+        for (int32_t i = 0; i < n; i++) {
+            rc = fcc_contract(ctx, &a[i], a_modes, &b[i], b_modes, n_contr, options, &outs[i]);
+            if (rc) {
+                for (int32_t k = 0; k < i; k++) fcc_result_free(ctx, &outs[k]);
+                return rc;
+            }
+        }
+        return FCC_OK;
+    }
+    fcc_options o;
+    fcc_options_default(&o);
+    if (options) o = *options;
+    if (o.left_tile_stride < 1) o.left_tile_stride = 1;
+    const bool sink = o.sink == FCC_SINK_COUNT_CHECKSUM;
+    DevPool &pool = ctx->pool;
+    hipStream_t s = ctx->stream;
+    try {
+        FCC_HIP(hipSetDevice(ctx->device));
+        for (int32_t c0 = 0; c0 < n; c0 += (int32_t)kMaxBatchPairs) {
+            const uint32_t cnt = (uint32_t)std::min<int32_t>(n - c0, (int32_t)kMaxBatchPairs);
+            const fcc_tensor *pa = a + c0, *pb = b + c0;
+            fcc_result *po = outs + c0;
+            // ---- concatenate each operand with a trailing pair-index mode
+            struct Side {
+                int d;
+                std::vector<unsigned long long> off;
+                Tmp<uint32_t> c;
+                Tmp<double> v;
+                uint32_t E[kMaxOrder + 1];
+            } side[2];
+            const fcc_tensor *src[2] = {pa, pb};
+            for (int k = 0; k < 2; k++) {
+                Side &S = side[k];
+                S.d = k ? dB : dA;
+                S.off.assign(cnt + 1, 0);
+                uint64_t host_n = 0;
+                for (uint32_t i = 0; i < cnt; i++) {
+                    S.off[i + 1] = S.off[i] + (uint64_t)src[k][i].nnz;
+                    if (!src[k][i].on_device) host_n += (uint64_t)src[k][i].nnz;
+                }
+                const uint64_t total = S.off[cnt];
+                if (total >= (1ull << 32)) throw LibError{FCC_ERR_INVALID, "batch operand exceeds 2^32 nonzeros"};
+                Tmp<uint32_t> stc(pool, (size_t)S.d * host_n);
+                Tmp<double> stv(pool, (size_t)host_n);
+                std::vector<PairSrc> ps(cnt);
+                uint64_t ho = 0;
+                for (uint32_t i = 0; i < cnt; i++) {
+                    const fcc_tensor &t = src[k][i];
+                    ps[i].nnz = (uint64_t)t.nnz;
+                    if (t.on_device || !t.nnz) {
+                        ps[i].coords = t.coords;
+                        ps[i].vals = t.vals;
+                    } else {
+                        FCC_HIP(hipMemcpyAsync(stc.get() + (size_t)S.d * ho, t.coords, 4ull * S.d * t.nnz,
+                                               hipMemcpyHostToDevice, s));
+                        FCC_HIP(hipMemcpyAsync(stv.get() + ho, t.vals, 8ull * t.nnz, hipMemcpyHostToDevice, s));
+                        ps[i].coords = stc.get() + (size_t)S.d * ho;
+                        ps[i].vals = stv.get() + ho;
+                        ho += (uint64_t)t.nnz;
+                    }
+                }
+                Tmp<unsigned char> dps(pool, sizeof(PairSrc) * cnt);
+                Tmp<unsigned long long> doff(pool, cnt + 1);
+                Tmp<uint32_t> dmax(pool, kMaxOrder);
+                FCC_HIP(hipMemcpyAsync(dps.get(), ps.data(), sizeof(PairSrc) * cnt, hipMemcpyHostToDevice, s));
+                FCC_HIP(hipMemcpyAsync(doff.get(), S.off.data(), 8ull * (cnt + 1), hipMemcpyHostToDevice, s));
+                fill_u32(dmax, kMaxOrder, 0, s);
+                S.c = Tmp<uint32_t>(pool, (size_t)(S.d + 1) * total);
+                S.v = Tmp<double>(pool, (size_t)total);
+                gather_pairs((const PairSrc *)dps.get(), doff, cnt, S.d, total, S.c, S.v, dmax, s);
+                uint32_t hmax[kMaxOrder];
+                FCC_HIP(hipMemcpyAsync(hmax, dmax.get(), sizeof(hmax), hipMemcpyDeviceToHost, s));
+                FCC_HIP(hipStreamSynchronize(s));
+                // extents: bounding box of every pair (explicit extents included), pair mode = cnt
+                for (int m = 0; m < S.d; m++) {
+                    uint32_t e = total ? hmax[m] + 1 : 0;
+                    for (uint32_t i = 0; i < cnt; i++)
+                        if (src[k][i].extents) e = std::max(e, src[k][i].extents[m]);
+                    S.E[m] = e;
+                }
+                S.E[S.d] = cnt;
+            }
+            // ---- one batch-mode contraction over all pairs
+            fcc_tensor ta{dA + 1, (int64_t)side[0].off[cnt], side[0].c.get(), side[0].v.get(), side[0].E, 1};
+            fcc_tensor tb{dB + 1, (int64_t)side[1].off[cnt], side[1].c.get(), side[1].v.get(), side[1].E, 1};
+            std::vector<int32_t> fa{dA}, fb{dB};
+            fa.insert(fa.end(), a_modes, a_modes + n_contr);
+            fb.insert(fb.end(), b_modes, b_modes + n_contr);
+            fcc_result fused;
+            memset(&fused, 0, sizeof(fused));
+            struct FusedGuard {
+                fcc_ctx *ctx;
+                fcc_result *r;
+                ~FusedGuard() { fcc_result_free(ctx, r); }
+            } guard{ctx, &fused};
+            Contraction c;
+            c.ctx = ctx;
+            c.s = s;
+            c.o = o;
+            c.out = &fused;
+            c.groups = cnt;
+            c.run(&ta, fa.data(), &tb, fb.data(), n_contr + 1, 1);
+            side[0].c.reset();
+            side[0].v.reset();
+            side[1].c.reset();
+            side[1].v.reset();
+            // ---- per-pair results
+            for (uint32_t i = 0; i < cnt; i++) {
+                fcc_result &r = po[i];
+                r.order = dC;
+                for (int m = 0; m < dC; m++) r.extents[m] = fused.extents[m + 1];
+                r.stats = fused.stats;
+                r.stats.ms_done = c.group_ms.empty() ? fused.stats.ms_total : c.group_ms[i];
+            }
+            if (sink) {
+                for (uint32_t i = 0; i < cnt && !c.group_sink.empty(); i++) {
+                    const unsigned long long *g4 = &c.group_sink[4 * (size_t)i];
+                    po[i].nnz = (int64_t)g4[0];
+                    po[i].coord_hash = g4[0] ? g4[1] : 0;
+                    memcpy(&po[i].val_sum, &g4[2], 8);
+                    memcpy(&po[i].val_wsum, &g4[3], 8);
+                }
+            } else if (fused.nnz > 0) {
+                const uint64_t fn = (uint64_t)fused.nnz;
+                Tmp<unsigned long long> counts(pool, cnt), offs(pool, cnt), cursor(pool, cnt);
+                fill_u64(counts, cnt, 0, s);
+                fill_u64(cursor, cnt, 0, s);
+                split_count(fused.coords, fn, cnt, counts, s);
+                std::vector<unsigned long long> hc(cnt), ho(cnt);
+                FCC_HIP(hipMemcpyAsync(hc.data(), counts.get(), 8ull * cnt, hipMemcpyDeviceToHost, s));
+                FCC_HIP(hipStreamSynchronize(s));
+                unsigned long long acc = 0;
+                for (uint32_t i = 0; i < cnt; i++) {
+                    ho[i] = acc;
+                    acc += hc[i];
+                }
+                FCC_HIP(hipMemcpyAsync(offs.get(), ho.data(), 8ull * cnt, hipMemcpyHostToDevice, s));
+                auto blk = std::make_shared<SharedBlock>();
+                blk->pool = &pool;
+                blk->coords = dC ? pool.get(4ull * dC * fn) : nullptr;
+                blk->vals = pool.get(8ull * fn);
+                split_scatter(fused.coords, (uint64_t)fused.coord_stride, fused.vals, fn, dC + 1, cnt, offs, counts,
+                              cursor, (uint32_t *)blk->coords, (double *)blk->vals, s);
+                for (uint32_t i = 0; i < cnt; i++) {
+                    fcc_result &r = po[i];
+                    r.nnz = (int64_t)hc[i];
+                    r.coord_stride = (int64_t)hc[i];
+                    r.coords = dC ? (uint32_t *)blk->coords + (size_t)dC * ho[i] : nullptr;
+                    r.vals = (double *)blk->vals + ho[i];
+                    ResultPriv *rp = new ResultPriv;
+                    rp->block = blk;
+                    r.priv = rp;
+                    if (o.summarize && hc[i]) {
+                        Tmp<unsigned long long> hs(pool, 1);
+                        Tmp<double> vs(pool, 2);
+                        Tmp<uint32_t> E(pool, kMaxOrder);
+                        fill_u64(hs, 1, 0, s);
+                        fill_f64(vs, 2, 0.0, s);
+                        FCC_HIP(hipMemcpyAsync(E.get(), r.extents, 4 * kMaxOrder, hipMemcpyHostToDevice, s));
+                        k_summarize<<<grid256(hc[i]), 256, 0, s>>>(r.coords, r.vals, hc[i], hc[i], dC, E, hs, vs);
+                        double dv[2];
+                        FCC_HIP(hipMemcpyAsync(&r.coord_hash, hs.get(), 8, hipMemcpyDeviceToHost, s));
+                        FCC_HIP(hipMemcpyAsync(dv, vs.get(), 16, hipMemcpyDeviceToHost, s));
+                        FCC_HIP(hipStreamSynchronize(s));
+                        r.val_sum = dv[0];
+                        r.val_wsum = dv[1];
+                    }
+                }
+                FCC_HIP(hipStreamSynchronize(s));
+            }
+        }
+        ctx->err.clear();
+        return FCC_OK;
+    } catch (HipError &e) {
+        (void)hipGetLastError();
+        rc = fail(ctx, FCC_ERR_HIP, e.where + ": " + hipGetErrorString(e.code));
+    } catch (LibError &e) {
+        rc = fail(ctx, e.code, e.msg);
+    } catch (std::bad_alloc &) {
+        rc = fail(ctx, FCC_ERR_OOM, "host allocation failed");
+    }
+    for (int32_t k = 0; k < n; k++) fcc_result_free(ctx, &outs[k]);
+    return rc;
 }
 
 }  // extern "C"

@@ -208,9 +208,17 @@ struct AccArgs {
     uint32_t *ocoords;
     double *ovals;
     uint64_t ostride;
-    // sink
-    unsigned long long *sink;  // [0]=count [1]=hash
-    double *sinkf;             // [0]=val_sum [1]=val_wsum
+    // sink: per group g, sink[4g]=count sink[4g+1]=hash, sinkf[4g]=val_sum
+    // sinkf[4g+1]=val_wsum (sinkf aliases sink + 2)
+    unsigned long long *sink;
+    double *sinkf;
+    // groups of rows (independent contractions fused into one launch): group
+    // g holds left rows [g*group_rows, (g+1)*group_rows); 0 = one group.
+    // Sink summaries are kept per group, hashed on the row within the group.
+    uint64_t group_rows;
+    unsigned int *group_left;            // units still to finish per group, or null
+    unsigned long long *group_done;      // wall clock when a group's last unit finished
+    unsigned long long *clock_start;     // wall clock when the first unit started
     unsigned int *work;        // persistent work counter
     int row_own;               // dense units: row-owned plain updates (experimental)
 };

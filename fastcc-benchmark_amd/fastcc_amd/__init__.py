@@ -161,18 +161,27 @@ class Context:
         o.summarize = 1 if summarize else 0
         return o
 
-    def contract(self, a, a_modes, b, b_modes, **opts) -> Result:
+    def contract(self, a, a_modes, b, b_modes, a_batch=(), b_batch=(), **opts) -> Result:
+        """fastcc_multiply (contract.hpp:824-827); with batch lists, the
+        batch-mode multiply (contract.hpp:285-367): outputs carry (batch
+        coords, A's other externals, B's other externals)."""
         keep = []
         ta = _as_fcc_tensor(a, keep)
         tb = ta if b is a else _as_fcc_tensor(b, keep)
-        r = len(a_modes)
-        if len(b_modes) != r:
+        r, nb = len(a_modes), len(a_batch)
+        if len(b_modes) != r or len(b_batch) != nb:
             raise FastccError(_abi.FCC_ERR_INVALID, "mode lists differ in length")
         am = (C.c_int32 * max(r, 1))(*a_modes)
         bm = (C.c_int32 * max(r, 1))(*b_modes)
         o = self.options(**opts)
         raw = _abi.FccResult()
-        rc = self.lib.fcc_contract(self.h, C.byref(ta), am, C.byref(tb), bm, r, C.byref(o), C.byref(raw))
+        if nb:
+            ab = (C.c_int32 * nb)(*a_batch)
+            bb = (C.c_int32 * nb)(*b_batch)
+            rc = self.lib.fcc_contract_ex(self.h, C.byref(ta), am, ab, C.byref(tb), bm, bb, r, nb, C.byref(o),
+                                          C.byref(raw))
+        else:
+            rc = self.lib.fcc_contract(self.h, C.byref(ta), am, C.byref(tb), bm, r, C.byref(o), C.byref(raw))
         self._check(rc)
         return Result(self, raw)
 

@@ -18,6 +18,7 @@ FCC_MLP_IN, FCC_MLP_HID, FCC_MLP_OUT = 6, 64, 2
 # every symbol include/fcc.h declares
 EXPORTS = (
     "fcc_create", "fcc_destroy", "fcc_last_error", "fcc_stream", "fcc_options_default", "fcc_contract",
+    "fcc_contract_ex",
     "fcc_contract_batch", "fcc_result_to_host", "fcc_result_free", "fcc_mlp_param_count", "fcc_mlp_set_params",
     "fcc_mlp_get_params", "fcc_tile_stats_count", "fcc_tile_stats_copy", "fcc_mlp_train",
     "fcc_mlp_forward_backward", "fcc_density_bucket", "fcc_tune_set", "fcc_tune_get",
@@ -70,6 +71,7 @@ class FccStats(C.Structure):
         ("col_window", C.c_int32),
         ("mlp_mape", C.c_double),
         ("ms_numeric_kernel", C.c_double),
+        ("ms_done", C.c_double),
     ]
 
 
@@ -125,6 +127,10 @@ def load(path: str | None = None):
     lib.fcc_contract.argtypes = [P, C.POINTER(FccTensor), C.POINTER(C.c_int32), C.POINTER(FccTensor),
                                  C.POINTER(C.c_int32), C.c_int32, C.POINTER(FccOptions), C.POINTER(FccResult)]
     lib.fcc_contract.restype = C.c_int
+    lib.fcc_contract_ex.argtypes = [P, C.POINTER(FccTensor), C.POINTER(C.c_int32), C.POINTER(C.c_int32),
+                                    C.POINTER(FccTensor), C.POINTER(C.c_int32), C.POINTER(C.c_int32), C.c_int32,
+                                    C.c_int32, C.POINTER(FccOptions), C.POINTER(FccResult)]
+    lib.fcc_contract_ex.restype = C.c_int
     lib.fcc_contract_batch.argtypes = [P, C.c_int32, C.POINTER(FccTensor), C.POINTER(C.c_int32),
                                        C.POINTER(FccTensor), C.POINTER(C.c_int32), C.c_int32,
                                        C.POINTER(FccOptions), C.POINTER(FccResult)]

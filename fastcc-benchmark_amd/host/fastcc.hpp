@@ -210,4 +210,37 @@ template <class DT> class Tensor {
         if (rc != FCC_OK) throw fastcc_error(rc, fcc_last_error(ctx));
         return ListTensor<RES>(ctx, r);
     }
+
+    // contract.hpp:285-367: contraction with batch pairs. The result holds
+    // (batch coords, this tensor's other externals, other's other externals)
+    // per nonzero (index_tensors.hpp:318-353), computed on the tiled GPU path.
+    template <class RES, class RIGHT>
+    Tensor<RES> multiply(Tensor<RIGHT> &other, CoOrdinate left_contr, CoOrdinate left_batch,
+                         CoOrdinate right_contr, CoOrdinate right_batch, const fcc_options *opts = nullptr) {
+        static_assert(std::is_same<RES, double>::value && std::is_same<RIGHT, double>::value,
+                      "FP64 operands and result");
+        if (left_contr.get_dimensionality() != right_contr.get_dimensionality() ||
+            left_batch.get_dimensionality() != right_batch.get_dimensionality())
+            throw fastcc_error(FCC_ERR_INVALID, "mode lists differ in length");
+        fcc_ctx *ctx = fastcc_detail::thread_context();
+        fcc_options o;
+        if (opts) o = *opts;
+        else fcc_options_default(&o);
+        fcc_tensor a = view(), b = other.view();
+        std::vector<int32_t> am(left_contr.values().begin(), left_contr.values().end());
+        std::vector<int32_t> bm(right_contr.values().begin(), right_contr.values().end());
+        std::vector<int32_t> ab(left_batch.values().begin(), left_batch.values().end());
+        std::vector<int32_t> bb(right_batch.values().begin(), right_batch.values().end());
+        fcc_result r{};
+        int rc = fcc_contract_ex(ctx, &a, am.data(), ab.data(), &b, bm.data(), bb.data(), (int32_t)am.size(),
+                                 (int32_t)ab.size(), &o, &r);
+        if (rc != FCC_OK) throw fastcc_error(rc, fcc_last_error(ctx));
+        std::vector<uint32_t> c((size_t)r.order * (size_t)r.nnz);
+        std::vector<RES> v((size_t)r.nnz);
+        rc = fcc_result_to_host(ctx, &r, c.data(), v.data());
+        int order = r.order;
+        fcc_result_free(ctx, &r);
+        if (rc != FCC_OK) throw fastcc_error(rc, fcc_last_error(ctx));
+        return Tensor<RES>(order, std::move(c), std::move(v));
+    }
 };

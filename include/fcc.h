@@ -80,10 +80,13 @@ typedef struct {
     int32_t col_window;   /* columns per dense unit used */
     double mlp_mape;      /* learned planner: tile-level MAPE of predicted nnz vs actual (-1 if n/a) */
     double ms_numeric_kernel; /* average duration of the numeric accumulate kernel */
+    double ms_done;       /* time from the call's start until this contraction's output was
+                             complete: ms_total for fcc_contract; per pair for fcc_contract_batch
+                             (device clock at the pair's last accumulate unit, plus the drain) */
 } fcc_stats;
 
 typedef struct {
-    int32_t order;      /* dA + dB - 2r */
+    int32_t order;      /* dA + dB - 2r (dA + dB - 2r - nb with nb batch pairs) */
     int64_t nnz;        /* output nonzeros */
     uint32_t *coords;   /* device SoA: coords[m * coord_stride + n], library-owned (NULL in sink mode) */
     int64_t coord_stride; /* >= nnz; equals nnz for the exact planner */
@@ -116,9 +119,32 @@ void fcc_options_default(fcc_options *opt);
 int fcc_contract(fcc_ctx *ctx, const fcc_tensor *a, const int32_t *a_modes, const fcc_tensor *b,
                  const int32_t *b_modes, int32_t n_contr, const fcc_options *options, fcc_result *out);
 
-/* Many independent contractions (same mode lists) in one call; their tiles
- * share one persistent work-stealing launch. No reference counterpart: the
- * reference contracts one pair per call (benchmark.cpp:68-73). */
+/* Replaces Tensor<double>::multiply<double,double>(Tensor<double>& other,
+ *     CoOrdinate left_contr, CoOrdinate left_batch, CoOrdinate right_contr,
+ *     CoOrdinate right_batch) -- contract.hpp:285-367 -- on the FaSTCC tiled
+ * path. a_batch[i] <-> b_batch[i] are batch pairs: their coordinates must
+ * agree like a contraction pair's, and they lead the output coordinate, which
+ * is (batch coords in list order, A's other externals ascending, B's other
+ * externals ascending), index_tensors.hpp:318-353. Output order is
+ * n_batch + (a->order - n_contr - n_batch) + (b->order - n_contr - n_batch).
+ * With n_batch = 0 this is fcc_contract. */
+int fcc_contract_ex(fcc_ctx *ctx, const fcc_tensor *a, const int32_t *a_modes, const int32_t *a_batch,
+                    const fcc_tensor *b, const int32_t *b_modes, const int32_t *b_batch, int32_t n_contr,
+                    int32_t n_batch, const fcc_options *options, fcc_result *out);
+
+/* Many independent contractions (same orders and mode lists) in one call.
+ * No reference counterpart: the reference contracts one pair per call
+ * (benchmark.cpp:68-73). The pairs run as ONE batch-mode contraction whose
+ * batch pair is a pair index appended to both operands, so every pair's tiles
+ * share one persistent work-stealing accumulate launch (up to 4096 pairs per
+ * launch; more are taken 4096 at a time). outs[i] is pair i's result, with the
+ * same coordinates and values as fcc_contract on that pair, except that:
+ *   - extents are the per-mode maximum over all pairs (the summaries
+ *     coord_hash / val_wsum linearise with these extents);
+ *   - stats describe the fused launch, and stats.ms_done is the time from the
+ *     call's start until pair i's output was complete;
+ *   - materialised outputs of one call share one device block, released when
+ *     the last of them is freed (fcc_result_free on each, as usual). */
 int fcc_contract_batch(fcc_ctx *ctx, int32_t n, const fcc_tensor *a, const int32_t *a_modes,
                        const fcc_tensor *b, const int32_t *b_modes, int32_t n_contr,
                        const fcc_options *options, fcc_result *outs);

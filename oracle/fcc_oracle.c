@@ -27,6 +27,12 @@
  *                                                      (index_tensors.hpp:80-126,
  *                                                       coordinate.hpp:402-427,
  *                                                       timer.hpp:66-125)
+ * oc_contract_tiled_ex adds the batch pairs of Tensor::multiply
+ * (contract.hpp:285-367): each batch pair joins the contraction key ahead of
+ * the contraction pairs (the reference indexes both operands by
+ * (left_batch, left_contr), contract.hpp:291-295) and leads the output
+ * coordinate (index_tensors.hpp:318-353, output_coords built from
+ * batch_coords, left_external, right_external at contract.hpp:330-331).
  * Three reference defects are corrected, not reproduced: the drain shift uses
  * log2(T) (the reference uses natural log, index_tensors.hpp:1042); decode
  * radices are the external modes' extents (the reference takes the leading
@@ -60,56 +66,68 @@ static double now_s(void) {
 /* ------------------------------------------------------------------ geometry */
 
 typedef struct {
-    int dA, dB, r, dC;
+    int dA, dB, r, nb, dC;
     uint32_t EA[OC_MAXD], EB[OC_MAXD];
-    uint64_t R[OC_MAXD]; /* key radix per contraction pair */
+    int ka[OC_MAXD], kb[OC_MAXD]; /* key modes: batch pairs first, then contraction pairs */
+    uint64_t R[OC_MAXD];          /* key radix per pair */
     int nxa, nxb;
-    int xa[OC_MAXD], xb[OC_MAXD];  /* external modes, ascending */
+    int xa[OC_MAXD], xb[OC_MAXD];  /* A: batch modes, then other externals ascending; B: externals ascending */
     uint64_t xaE[OC_MAXD], xbE[OC_MAXD];
     uint64_t spanA, spanB;
 } oc_geom;
 
-static int in_list(const int32_t *l, int n, int m) {
+static int in_list(const int *l, int n, int m) {
     for (int i = 0; i < n; i++)
         if (l[i] == m) return 1;
     return 0;
 }
 
-static int geom_init(oc_geom *g, const oc_tensor *a, const int32_t *am, const oc_tensor *b,
-                     const int32_t *bm, int r) {
-    memset(g, 0, sizeof(*g));
-    if (!a || !b || a->order < 1 || b->order < 1 || a->order > OC_MAXD || b->order > OC_MAXD) return -1;
-    if (r < 0 || r > a->order || r > b->order) return -1;
-    g->dA = a->order; g->dB = b->order; g->r = r; g->dC = a->order + b->order - 2 * r;
-    if (g->dC > OC_MAXD) return -1;
-    for (int i = 0; i < r; i++) {
-        if (am[i] < 0 || am[i] >= a->order || bm[i] < 0 || bm[i] >= b->order) return -1;
-        for (int k = 0; k < i; k++)
-            if (am[k] == am[i] || bm[k] == bm[i]) return -1;
-    }
-    for (int m = 0; m < a->order; m++) {
+static int extents_of(const oc_tensor *t, uint32_t *E) {
+    for (int m = 0; m < t->order; m++) {
         uint32_t mx = 0;
-        for (int64_t n = 0; n < a->nnz; n++) { uint32_t c = a->coords[(int64_t)m * a->nnz + n]; if (c > mx) mx = c; }
-        g->EA[m] = mx + 1;
+        for (int64_t n = 0; n < t->nnz; n++) { uint32_t c = t->coords[(int64_t)m * t->nnz + n]; if (c > mx) mx = c; }
+        if (t->extents) {
+            if (t->nnz && mx >= t->extents[m]) return -1;
+            E[m] = t->extents[m];
+        } else {
+            E[m] = t->nnz ? mx + 1 : 0;
+        }
     }
-    for (int m = 0; m < b->order; m++) {
-        uint32_t mx = 0;
-        for (int64_t n = 0; n < b->nnz; n++) { uint32_t c = b->coords[(int64_t)m * b->nnz + n]; if (c > mx) mx = c; }
-        g->EB[m] = mx + 1;
-    }
-    for (int i = 0; i < r; i++) {
-        uint32_t ea = g->EA[am[i]], eb = g->EB[bm[i]];
-        g->R[i] = ea > eb ? ea : eb;
-    }
-    g->spanA = 1; g->spanB = 1;
-    for (int m = 0; m < a->order; m++)
-        if (!in_list(am, r, m)) { g->xa[g->nxa] = m; g->xaE[g->nxa] = g->EA[m]; g->spanA *= g->EA[m]; g->nxa++; }
-    for (int m = 0; m < b->order; m++)
-        if (!in_list(bm, r, m)) { g->xb[g->nxb] = m; g->xbE[g->nxb] = g->EB[m]; g->spanB *= g->EB[m]; g->nxb++; }
     return 0;
 }
 
-static inline uint64_t key_of(const oc_tensor *t, const int32_t *modes, const oc_geom *g, int64_t n) {
+/* Key = (batch pairs, contraction pairs) row-major, radix per pair the larger
+ * paired extent; A's external index = (batch coords, other A modes ascending). */
+static int geom_init(oc_geom *g, const oc_tensor *a, const int32_t *am, const int32_t *ab, const oc_tensor *b,
+                     const int32_t *bm, const int32_t *bb, int r, int nb) {
+    memset(g, 0, sizeof(*g));
+    if (!a || !b || a->order < 1 || b->order < 1 || a->order > OC_MAXD || b->order > OC_MAXD) return -1;
+    if (r < 0 || nb < 0 || r + nb > a->order || r + nb > b->order) return -1;
+    g->dA = a->order; g->dB = b->order; g->r = r + nb; g->nb = nb;
+    g->dC = a->order + b->order - 2 * (r + nb) + nb;
+    if (g->dC > OC_MAXD) return -1;
+    for (int i = 0; i < nb; i++) { g->ka[i] = ab[i]; g->kb[i] = bb[i]; }
+    for (int i = 0; i < r; i++) { g->ka[nb + i] = am[i]; g->kb[nb + i] = bm[i]; }
+    for (int i = 0; i < g->r; i++) {
+        if (g->ka[i] < 0 || g->ka[i] >= a->order || g->kb[i] < 0 || g->kb[i] >= b->order) return -1;
+        for (int k = 0; k < i; k++)
+            if (g->ka[k] == g->ka[i] || g->kb[k] == g->kb[i]) return -1;
+    }
+    if (extents_of(a, g->EA) || extents_of(b, g->EB)) return -1;
+    for (int i = 0; i < g->r; i++) {
+        uint32_t ea = g->EA[g->ka[i]], eb = g->EB[g->kb[i]];
+        g->R[i] = ea > eb ? ea : eb;
+    }
+    g->spanA = 1; g->spanB = 1;
+    for (int i = 0; i < nb; i++) { g->xa[g->nxa] = g->ka[i]; g->xaE[g->nxa] = g->R[i]; g->spanA *= g->R[i]; g->nxa++; }
+    for (int m = 0; m < a->order; m++)
+        if (!in_list(g->ka, g->r, m)) { g->xa[g->nxa] = m; g->xaE[g->nxa] = g->EA[m]; g->spanA *= g->EA[m]; g->nxa++; }
+    for (int m = 0; m < b->order; m++)
+        if (!in_list(g->kb, g->r, m)) { g->xb[g->nxb] = m; g->xbE[g->nxb] = g->EB[m]; g->spanB *= g->EB[m]; g->nxb++; }
+    return 0;
+}
+
+static inline uint64_t key_of(const oc_tensor *t, const int *modes, const oc_geom *g, int64_t n) {
     uint64_t k = 0;
     for (int i = 0; i < g->r; i++) k = k * g->R[i] + t->coords[(int64_t)modes[i] * t->nnz + n];
     return k;
@@ -217,8 +235,8 @@ typedef struct {
 } oc_tindex;
 
 /* index_tensors.hpp:506-547 */
-static void tindex_build(oc_tindex *ix, const oc_tensor *t, const int32_t *modes, const oc_geom *g,
-                         int left, uint64_t T, int hw) {
+static void tindex_build(oc_tindex *ix, const oc_tensor *t, const oc_geom *g, int left, uint64_t T, int hw) {
+    const int *modes = left ? g->ka : g->kb;
     const int *xm = left ? g->xa : g->xb;
     const uint64_t *xE = left ? g->xaE : g->xbE;
     int nx = left ? g->nxa : g->nxb;
@@ -330,10 +348,18 @@ int oc_contract_tiled(const oc_tensor *a, const int32_t *a_modes, const oc_tenso
                       const int32_t *b_modes, int32_t n_contr, int32_t tile_size,
                       int32_t nthreads, int32_t high_precision, int32_t sample_stride,
                       int32_t sink, oc_result *out) {
+    return oc_contract_tiled_ex(a, a_modes, NULL, b, b_modes, NULL, n_contr, 0, tile_size, nthreads,
+                                high_precision, sample_stride, sink, out);
+}
+
+int oc_contract_tiled_ex(const oc_tensor *a, const int32_t *a_modes, const int32_t *a_batch,
+                         const oc_tensor *b, const int32_t *b_modes, const int32_t *b_batch,
+                         int32_t n_contr, int32_t n_batch, int32_t tile_size, int32_t nthreads,
+                         int32_t high_precision, int32_t sample_stride, int32_t sink, oc_result *out) {
     if (!out) return -1;
     memset(out, 0, sizeof(*out));
     oc_geom g;
-    if (geom_init(&g, a, a_modes, b, b_modes, n_contr) != 0) return -1;
+    if (geom_init(&g, a, a_modes, a_batch, b, b_modes, b_batch, n_contr, n_batch) != 0) return -1;
     out->order = g.dC;
     if (sample_stride < 1) sample_stride = 1;
     int hw = nthreads > 0 ? nthreads : omp_get_num_procs();
@@ -350,8 +376,8 @@ int oc_contract_tiled(const oc_tensor *a, const int32_t *a_modes, const oc_tenso
     omp_set_max_active_levels(2);
 #pragma omp parallel num_threads(2)
     {
-        if (omp_get_thread_num() == 0) tindex_build(&L, a, a_modes, &g, 1, T, hw);
-        else tindex_build(&Rx, b, b_modes, &g, 0, T, hw);
+        if (omp_get_thread_num() == 0) tindex_build(&L, a, &g, 1, T, hw);
+        else tindex_build(&Rx, b, &g, 0, T, hw);
     }
     omp_set_max_active_levels(nested_prev);
     if (L.err || Rx.err || !L.tiles || !Rx.tiles) { tindex_free(&L); tindex_free(&Rx); return -2; }
@@ -510,10 +536,16 @@ static int term_cmp(const void *x, const void *y) {
 
 int oc_contract_bruteforce(const oc_tensor *a, const int32_t *a_modes, const oc_tensor *b,
                            const int32_t *b_modes, int32_t n_contr, oc_result *out) {
+    return oc_contract_bruteforce_ex(a, a_modes, NULL, b, b_modes, NULL, n_contr, 0, out);
+}
+
+int oc_contract_bruteforce_ex(const oc_tensor *a, const int32_t *a_modes, const int32_t *a_batch,
+                              const oc_tensor *b, const int32_t *b_modes, const int32_t *b_batch,
+                              int32_t n_contr, int32_t n_batch, oc_result *out) {
     if (!out) return -1;
     memset(out, 0, sizeof(*out));
     oc_geom g;
-    if (geom_init(&g, a, a_modes, b, b_modes, n_contr) != 0) return -1;
+    if (geom_init(&g, a, a_modes, a_batch, b, b_modes, b_batch, n_contr, n_batch) != 0) return -1;
     out->order = g.dC;
     double t0 = now_s();
     size_t cap = 1024, n = 0;
@@ -522,11 +554,12 @@ int oc_contract_bruteforce(const oc_tensor *a, const int32_t *a_modes, const oc_
     for (int64_t p = 0; p < a->nnz; p++) {
         for (int64_t q = 0; q < b->nnz; q++) {
             int match = 1;
-            for (int i = 0; i < n_contr && match; i++)
-                match = a->coords[(int64_t)a_modes[i] * a->nnz + p] == b->coords[(int64_t)b_modes[i] * b->nnz + q];
+            for (int i = 0; i < g.r && match; i++)
+                match = a->coords[(int64_t)g.ka[i] * a->nnz + p] == b->coords[(int64_t)g.kb[i] * b->nnz + q];
             if (!match) continue;
-            /* output coordinate: A externals (ascending), then B externals; its
-             * lexicographic order equals the order of this mixed-radix index */
+            /* output coordinate: batch coords, A's other externals (ascending),
+             * then B's externals; its lexicographic order equals the order of
+             * this mixed-radix index */
             uint64_t lin = 0;
             for (int i = 0; i < g.nxa; i++) lin = lin * g.xaE[i] + a->coords[(int64_t)g.xa[i] * a->nnz + p];
             for (int i = 0; i < g.nxb; i++) lin = lin * g.xbE[i] + b->coords[(int64_t)g.xb[i] * b->nnz + q];

@@ -16,10 +16,11 @@ typedef struct {
     int64_t nnz;
     const uint32_t *coords; /* SoA: coords[m * nnz + n] */
     const double *vals;
+    const uint32_t *extents; /* order extents, or NULL: max+1 per mode */
 } oc_tensor;
 
 typedef struct {
-    int32_t order;     /* output order dA + dB - 2r */
+    int32_t order;     /* output order dA + dB - 2r - nb (nb batch pairs) */
     int64_t nnz;       /* output nonzeros (structural support) */
     uint32_t *coords;  /* SoA [order][nnz], malloc'd; NULL in sink mode */
     double *vals;      /* [nnz], malloc'd; NULL in sink mode */
@@ -48,6 +49,18 @@ int oc_contract_tiled(const oc_tensor *a, const int32_t *a_modes, const oc_tenso
                       int32_t nthreads, int32_t high_precision, int32_t sample_stride,
                       int32_t sink, oc_result *out);
 
+/* Batch-mode form of oc_contract_tiled: Tensor::multiply(other, left_contr,
+ * left_batch, right_contr, right_batch) (contract.hpp:285-367). A batch pair
+ * a_batch[i] <-> b_batch[i] must agree like a contraction pair but is kept in
+ * the output: coordinates are (batch coordinates in list order, A's other
+ * externals ascending, B's other externals ascending)
+ * (index_tensors.hpp:318-353). Restated as the tiled algorithm with the batch
+ * coordinates leading both the contraction key and the left external index. */
+int oc_contract_tiled_ex(const oc_tensor *a, const int32_t *a_modes, const int32_t *a_batch,
+                         const oc_tensor *b, const int32_t *b_modes, const int32_t *b_batch,
+                         int32_t n_contr, int32_t n_batch, int32_t tile_size, int32_t nthreads,
+                         int32_t high_precision, int32_t sample_stride, int32_t sink, oc_result *out);
+
 /* Direct statement of the contraction semantics: every (p in A, q in B) pair
  * whose contraction coordinates are equal contributes a_p * b_q at output
  * coordinate (A's external coords, B's external coords). Output sorted
@@ -55,6 +68,11 @@ int oc_contract_tiled(const oc_tensor *a, const int32_t *a_modes, const oc_tenso
  * tiny inputs only. */
 int oc_contract_bruteforce(const oc_tensor *a, const int32_t *a_modes, const oc_tensor *b,
                            const int32_t *b_modes, int32_t n_contr, oc_result *out);
+/* ... with batch pairs: a pair (p, q) contributes only when its batch
+ * coordinates are equal too, at (batch coords, A's other externals, B's). */
+int oc_contract_bruteforce_ex(const oc_tensor *a, const int32_t *a_modes, const int32_t *a_batch,
+                              const oc_tensor *b, const int32_t *b_modes, const int32_t *b_batch,
+                              int32_t n_contr, int32_t n_batch, oc_result *out);
 
 /* Sort a materialised result lexicographically by coordinate (in place). */
 int oc_result_sort(oc_result *r);

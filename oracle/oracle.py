@@ -13,7 +13,8 @@ LIB = os.path.join(_HERE, "libfcc_oracle.so")
 
 
 class _OcTensor(C.Structure):
-    _fields_ = [("order", C.c_int32), ("nnz", C.c_int64), ("coords", C.c_void_p), ("vals", C.c_void_p)]
+    _fields_ = [("order", C.c_int32), ("nnz", C.c_int64), ("coords", C.c_void_p), ("vals", C.c_void_p),
+                ("extents", C.c_void_p)]
 
 
 class _OcResult(C.Structure):
@@ -46,6 +47,16 @@ def _load():
                                            C.POINTER(C.c_int32), C.c_int32, C.c_int32, C.c_int32, C.c_int32,
                                            C.c_int32, C.c_int32, C.POINTER(_OcResult)]
         _lib.oc_contract_tiled.restype = C.c_int
+        _lib.oc_contract_tiled_ex.argtypes = [C.POINTER(_OcTensor), C.POINTER(C.c_int32), C.POINTER(C.c_int32),
+                                              C.POINTER(_OcTensor), C.POINTER(C.c_int32), C.POINTER(C.c_int32),
+                                              C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_int32,
+                                              C.c_int32, C.POINTER(_OcResult)]
+        _lib.oc_contract_tiled_ex.restype = C.c_int
+        _lib.oc_contract_bruteforce_ex.argtypes = [C.POINTER(_OcTensor), C.POINTER(C.c_int32),
+                                                   C.POINTER(C.c_int32), C.POINTER(_OcTensor),
+                                                   C.POINTER(C.c_int32), C.POINTER(C.c_int32), C.c_int32,
+                                                   C.c_int32, C.POINTER(_OcResult)]
+        _lib.oc_contract_bruteforce_ex.restype = C.c_int
         _lib.oc_contract_bruteforce.argtypes = [C.POINTER(_OcTensor), C.POINTER(C.c_int32), C.POINTER(_OcTensor),
                                                 C.POINTER(C.c_int32), C.c_int32, C.POINTER(_OcResult)]
         _lib.oc_contract_bruteforce.restype = C.c_int
@@ -67,7 +78,7 @@ class OracleResult:
         self.products, self.seconds, self.threads, self.nnz = products, seconds, threads, nnz
 
 
-def _mk(t, keep):
+def _mk(t, keep, ext=None):
     coords = np.ascontiguousarray(t.coords, dtype=np.uint32)
     vals = np.ascontiguousarray(t.vals, dtype=np.float64)
     keep += [coords, vals]
@@ -75,6 +86,10 @@ def _mk(t, keep):
     o.order, o.nnz = coords.shape[0], vals.shape[0]
     o.coords = coords.ctypes.data if o.nnz else None
     o.vals = vals.ctypes.data if o.nnz else None
+    if ext is not None:
+        e = np.ascontiguousarray(ext, dtype=np.uint32)
+        keep.append(e)
+        o.extents = e.ctypes.data
     return o
 
 
@@ -98,32 +113,42 @@ def _take(lib, r: _OcResult, sort: bool) -> OracleResult:
     return out
 
 
+def _ilist(l):
+    l = list(l)
+    return (C.c_int32 * max(len(l), 1))(*l), len(l)
+
+
 def contract_tiled(a, a_modes, b, b_modes, tile_size=128, threads=0, high_precision=True, sample_stride=1,
-                   sink=False, sort=True) -> OracleResult:
-    """Restatement of the reference fastcc_multiply (see fcc_oracle.c)."""
+                   sink=False, sort=True, a_batch=(), b_batch=(), a_extents=None,
+                   b_extents=None) -> OracleResult:
+    """Restatement of the reference fastcc_multiply (see fcc_oracle.c); with
+    batch lists, of the batch-mode multiply."""
     lib = _load()
     keep = []
-    ta, tb = _mk(a, keep), _mk(b, keep)
-    r = len(a_modes)
-    am = (C.c_int32 * max(r, 1))(*a_modes)
-    bm = (C.c_int32 * max(r, 1))(*b_modes)
+    ta, tb = _mk(a, keep, a_extents), _mk(b, keep, b_extents)
+    am, r = _ilist(a_modes)
+    bm, _ = _ilist(b_modes)
+    ab, nb = _ilist(a_batch)
+    bb, _ = _ilist(b_batch)
     res = _OcResult()
-    rc = lib.oc_contract_tiled(C.byref(ta), am, C.byref(tb), bm, r, int(tile_size), int(threads),
-                               1 if high_precision else 0, int(sample_stride), 1 if sink else 0, C.byref(res))
+    rc = lib.oc_contract_tiled_ex(C.byref(ta), am, ab, C.byref(tb), bm, bb, r, nb, int(tile_size), int(threads),
+                                  1 if high_precision else 0, int(sample_stride), 1 if sink else 0, C.byref(res))
     if rc:
         raise RuntimeError(f"oc_contract_tiled failed {rc}")
     return _take(lib, res, sort)
 
 
-def contract_bruteforce(a, a_modes, b, b_modes) -> OracleResult:
+def contract_bruteforce(a, a_modes, b, b_modes, a_batch=(), b_batch=(), a_extents=None,
+                        b_extents=None) -> OracleResult:
     lib = _load()
     keep = []
-    ta, tb = _mk(a, keep), _mk(b, keep)
-    r = len(a_modes)
-    am = (C.c_int32 * max(r, 1))(*a_modes)
-    bm = (C.c_int32 * max(r, 1))(*b_modes)
+    ta, tb = _mk(a, keep, a_extents), _mk(b, keep, b_extents)
+    am, r = _ilist(a_modes)
+    bm, _ = _ilist(b_modes)
+    ab, nb = _ilist(a_batch)
+    bb, _ = _ilist(b_batch)
     res = _OcResult()
-    rc = lib.oc_contract_bruteforce(C.byref(ta), am, C.byref(tb), bm, r, C.byref(res))
+    rc = lib.oc_contract_bruteforce_ex(C.byref(ta), am, ab, C.byref(tb), bm, bb, r, nb, C.byref(res))
     if rc:
         raise RuntimeError(f"oc_contract_bruteforce failed {rc}")
     return _take(lib, res, sort=False)

@@ -65,7 +65,7 @@ def test_struct_layout_matches_header():
 #include "fcc.h"
 int main(void){
  printf("%zu %zu %zu %zu %zu\n", sizeof(fcc_tensor), sizeof(fcc_options), sizeof(fcc_stats), sizeof(fcc_result), sizeof(fcc_tune_entry));
- printf("%zu %zu %zu\n", offsetof(fcc_result, stats), offsetof(fcc_result, coord_stride), offsetof(fcc_stats, ms_numeric_kernel));
+ printf("%zu %zu %zu\n", offsetof(fcc_result, stats), offsetof(fcc_result, coord_stride), offsetof(fcc_stats, ms_done));
  return 0;}
 '''
     import tempfile
@@ -79,7 +79,7 @@ int main(void){
                      ctypes.sizeof(_abi.FccResult), ctypes.sizeof(_abi.FccTuneEntry)]
     offs = [int(x) for x in line2.split()]
     assert offs == [_abi.FccResult.stats.offset, _abi.FccResult.coord_stride.offset,
-                    _abi.FccStats.ms_numeric_kernel.offset]
+                    _abi.FccStats.ms_done.offset]
 
 
 def test_cpp_dropin_header_compiles():

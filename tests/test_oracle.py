@@ -25,13 +25,22 @@ from conftest import ROOT, assert_same_result, densify, rand_tensor, sort_result
 
 from oracle import oracle
 from fastcc_amd import workloads as W
+from fastcc_amd.workloads import SparseTensor
 
 
-def einsum_contract(a, am, b, bm, Ea, Eb, boolean=False):
-    """Dense reference with numpy: output modes A-externals then B-externals."""
+def einsum_contract(a, am, b, bm, Ea, Eb, boolean=False, ab=(), bb=()):
+    """Dense reference with numpy: output modes batch pairs (list order), then
+    A's other externals, then B's (einsum keeps a letter shared by both
+    operands and named in the output as a batch index)."""
     letters = iter(string.ascii_letters)
     la = [None] * a.order
     lb = [None] * b.order
+    batch = []
+    for x, y in zip(ab, bb):
+        c = next(letters)
+        la[x] = c
+        lb[y] = c
+        batch.append(c)
     for x, y in zip(am, bm):
         c = next(letters)
         la[x] = c
@@ -42,10 +51,11 @@ def einsum_contract(a, am, b, bm, Ea, Eb, boolean=False):
     for i in range(b.order):
         if lb[i] is None:
             lb[i] = next(letters)
-    out = [la[i] for i in range(a.order) if i not in am] + [lb[i] for i in range(b.order) if i not in bm]
+    out = batch + [la[i] for i in range(a.order) if i not in am and i not in ab] + \
+        [lb[i] for i in range(b.order) if i not in bm and i not in bb]
     # paired extents share the larger radix
     Ea, Eb = list(Ea), list(Eb)
-    for x, y in zip(am, bm):
+    for x, y in list(zip(am, bm)) + list(zip(ab, bb)):
         m = max(Ea[x], Eb[y])
         Ea[x] = Eb[y] = m
     da, db = densify(a, tuple(Ea)), densify(b, tuple(Eb))
@@ -112,6 +122,80 @@ def test_tiled_matches_bruteforce(case, tile, threads):
         # small-integer values: every sum is exact, so equality is exact
         assert_same_result(got.coords, got.vals, ref.coords, ref.vals, rel=0, abs_floor=0, what="tiled")
         assert got.products == ref.products
+
+
+BATCH_CASES = [
+    # (dims A, nnz A, a_modes, a_batch, dims B, nnz B, b_modes, b_batch)
+    ((4, 5, 6), 60, (2,), (0,), (4, 6, 3), 40, (1,), (0,)),          # bij,bjk->bik
+    ((3, 5, 4, 6), 90, (1, 3), (2,), (6, 4, 5, 2), 70, (3, 0), (1,)),  # batch mode in the middle
+    ((3, 4, 5), 40, (), (0, 2), (5, 3, 6), 30, (), (0, 1)),           # batch only, two batch pairs
+    ((2, 3, 4), 20, (1,), (2, 0), (3, 2, 4), 22, (0,), (2, 1)),       # every A mode keyed
+]
+
+
+@pytest.mark.parametrize("case", range(len(BATCH_CASES)))
+@pytest.mark.parametrize("values", ["uniform", "zeros"])
+def test_batch_bruteforce_matches_einsum(case, values):
+    """Batch pairs (contract.hpp:285-367): coordinates must agree like a
+    contraction pair and lead the output (index_tensors.hpp:318-353)."""
+    rng = np.random.default_rng(700 + case)
+    da, na, am, ab, db, nb, bm, bb = BATCH_CASES[case]
+    a = rand_tensor(rng, da, na, values)
+    b = rand_tensor(rng, db, nb, values)
+    res = oracle.contract_bruteforce(a, am, b, bm, a_batch=ab, b_batch=bb)
+    Ea, Eb = a.extents(), b.extents()
+    cnt = einsum_contract(a, am, b, bm, Ea, Eb, boolean=True, ab=ab, bb=bb)
+    val = einsum_contract(a, am, b, bm, Ea, Eb, ab=ab, bb=bb)
+    rc, rv = dense_to_coo(cnt, val)
+    assert res.order == len(ab) + a.order + b.order - 2 * (len(am) + len(ab))
+    assert res.nnz == rc.shape[1]
+    assert_same_result(res.coords, res.vals, rc, rv, rel=1e-13, abs_floor=1e-13, what="batch bruteforce vs einsum")
+
+
+@pytest.mark.parametrize("case", range(len(BATCH_CASES)))
+@pytest.mark.parametrize("tile,threads", [(1, 1), (4, 3), (128, 8)])
+def test_batch_tiled_matches_bruteforce(case, tile, threads):
+    rng = np.random.default_rng(800 + case)
+    da, na, am, ab, db, nb, bm, bb = BATCH_CASES[case]
+    a = rand_tensor(rng, da, na, "smallint")
+    b = rand_tensor(rng, db, nb, "smallint")
+    ref = oracle.contract_bruteforce(a, am, b, bm, a_batch=ab, b_batch=bb)
+    got = oracle.contract_tiled(a, am, b, bm, tile_size=tile, threads=threads, a_batch=ab, b_batch=bb)
+    assert_same_result(got.coords, got.vals, ref.coords, ref.vals, rel=0, abs_floor=0, what="batch tiled")
+    assert got.products == ref.products
+
+
+def test_batch_equals_per_slice_contractions():
+    """A batch pair is the same as contracting each batch slice on its own and
+    prefixing the slice index: checked against the plain (batch-free) oracle."""
+    rng = np.random.default_rng(901)
+    a = rand_tensor(rng, (5, 7, 6), 120, "uniform")
+    b = rand_tensor(rng, (6, 5, 4), 80, "uniform")
+    got = oracle.contract_bruteforce(a, (2,), b, (0,), a_batch=(0,), b_batch=(1,))
+    cs, vs = [], []
+    for z in range(5):
+        ka, kb = a.coords[0] == z, b.coords[1] == z
+        sa = SparseTensor(np.ascontiguousarray(a.coords[1:, ka]), a.vals[ka])
+        sb = SparseTensor(np.ascontiguousarray(b.coords[[0, 2]][:, kb]), b.vals[kb])
+        if sa.nnz == 0 or sb.nnz == 0:
+            continue
+        # pin the slice extents to the full ones so coordinates line up
+        r = oracle.contract_bruteforce(sa, (1,), sb, (0,), a_extents=(7, 6), b_extents=(6, 4))
+        cs.append(np.vstack([np.full((1, r.nnz), z, dtype=np.uint32), r.coords]))
+        vs.append(r.vals)
+    rc, rv = np.hstack(cs), np.concatenate(vs)
+    assert_same_result(got.coords, got.vals, rc, rv, rel=1e-15, abs_floor=0, what="batch vs slices")
+
+
+def test_explicit_extents_checked():
+    rng = np.random.default_rng(902)
+    a = rand_tensor(rng, (5, 6), 20, "uniform")
+    b = rand_tensor(rng, (6, 3), 10, "uniform")
+    r1 = oracle.contract_tiled(a, (1,), b, (0,), sink=True)
+    r2 = oracle.contract_tiled(a, (1,), b, (0,), sink=True, a_extents=(9, 6), b_extents=(6, 3))
+    assert r1.nnz == r2.nnz and r1.val_sum == r2.val_sum
+    with pytest.raises(RuntimeError):
+        oracle.contract_tiled(a, (1,), b, (0,), a_extents=(2, 6))
 
 
 def test_all_ones_counts_matches():
