@@ -404,21 +404,28 @@ __device__ __forceinline__ void heavy_group(const AccArgs &A, const UnitCtx &U, 
 template <bool NUM, bool BITMAP>
 __device__ void heavy_claimed(const AccArgs &A, const UnitCtx &U, const Acc<NUM, BITMAP> &S, uint32_t *s_gnext) {
     const uint32_t lane = threadIdx.x & 63;
-    const uint32_t g0 = A.atile_g0[U.a], gl = A.atile_lg[U.a];
+    const uint32_t g0 = A.atile_h0[U.a], gl = A.atile_h0[U.a + 1];
     const uint32_t cend = U.c0 + U.W;
+    // claim size: enough claims for every wavefront of the block, at most 64
+    const uint32_t claim = max(1u, min(64u, (gl - g0 + 2 * NWAVES - 1) / (2 * NWAVES)));
     for (;;) {
         uint32_t bs = 0;
-        if (lane == 0) bs = atomicAdd(s_gnext, 64u);
+        if (lane == 0) bs = atomicAdd(s_gnext, claim);
         bs = uniform_u32(__shfl(bs, 0, 64));
         if (bs >= gl - g0) break;
         const uint32_t bg = g0 + bs;
-        const uint32_t nb = min(64u, gl - bg);
+        const uint32_t nb = min(claim, gl - bg);
         uint4 d = make_uint4(0, 0, 0, 0);
-        if (lane < nb) d = A.gdesc[bg + lane];
+        if (lane < nb) d = A.hdesc[bg + lane];
         uint32_t s_l = d.x, e_l = d.x + d.y;
-        if (U.multi && lane < nb) {
-            s_l = lane_lower_bound(A.Bcol, s_l, e_l, U.c0);
-            e_l = lane_lower_bound(A.Bcol, s_l, e_l, cend);
+        if (U.multi && lane < nb && s_l < e_l) {
+            // segments wholly outside the window are dropped on their end columns
+            if (A.Bcol[s_l] >= cend || A.Bcol[e_l - 1] < U.c0) {
+                e_l = s_l;
+            } else {
+                s_l = lane_lower_bound(A.Bcol, s_l, e_l, U.c0);
+                e_l = lane_lower_bound(A.Bcol, s_l, e_l, cend);
+            }
         }
         uint32_t li[PIPE], col[PIPE];
         double av[PIPE], bv[PIPE];
@@ -457,30 +464,118 @@ __device__ void heavy_claimed(const AccArgs &A, const UnitCtx &U, const Acc<NUM,
     }
 }
 
-// One unit: heavy groups, then light entries (one lane per entry, fiber
-// bounds precomputed per entry, atomic updates).
+// Light entries [le0, le1) of a unit: the products of every entry (its right
+// fiber, clipped to the unit's column window) are flattened and spread over
+// all lanes, so a tile of a few entries with long fibers keeps the block busy.
+// With `stage` (2 * ACC_THREADS words of LDS) the flattening is block-wide:
+// an inclusive scan of fiber lengths over 512 entries at a time, and each
+// product finds its entry by binary search in LDS. Without it each wavefront
+// flattens its own 64 entries in registers (owner found by shuffles).
 template <bool NUM, bool BITMAP>
-__device__ void process_unit(const AccArgs &A, const UnitCtx &U, const Acc<NUM, BITMAP> &S, uint32_t *s_gnext) {
-    const uint32_t gl = A.atile_lg[U.a], g1 = A.atile_g0[U.a + 1];
+__device__ void light_entries(const AccArgs &A, const UnitCtx &U, const Acc<NUM, BITMAP> &S, uint32_t le0,
+                              uint32_t le1, uint32_t *stage, uint32_t *s_w) {
+    const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const uint32_t cend = U.c0 + U.W;
+    auto span_of = [&](uint32_t e, uint32_t &s0, uint32_t &len) {
+        const uint2 fp = A.Afp[e];
+        uint32_t a = fp.x, b = fp.y;
+        if (U.multi) {
+            a = lane_lower_bound(A.Bcol, a, b, U.c0);
+            b = lane_lower_bound(A.Bcol, a, b, cend);
+        }
+        s0 = a;
+        len = b - a;
+    };
+    if (stage) {
+        uint32_t *s_inc = stage, *s_s0 = stage + ACC_THREADS;
+        for (uint32_t eb = le0; eb < le1; eb += ACC_THREADS) {
+            const uint32_t e = eb + threadIdx.x;
+            uint32_t s0 = 0, len = 0;
+            if (e < le1) span_of(e, s0, len);
+            uint32_t inc = len;
+            for (int d = 1; d < 64; d <<= 1) {
+                uint32_t t = __shfl_up(inc, d, 64);
+                if (lane >= (uint32_t)d) inc += t;
+            }
+            if (lane == 63) s_w[wave] = inc;
+            __syncthreads();
+            uint32_t pre = 0, P = 0;
+#pragma unroll
+            for (int w = 0; w < NWAVES; w++) {
+                const uint32_t c = s_w[w];
+                pre += (w < (int)wave) ? c : 0;
+                P += c;
+            }
+            s_inc[threadIdx.x] = inc + pre;
+            s_s0[threadIdx.x] = s0;
+            __syncthreads();
+            const uint32_t nent = min(le1 - eb, (uint32_t)ACC_THREADS);
+            for (uint32_t q = threadIdx.x; q < P; q += ACC_THREADS) {
+                uint32_t lo = 0, hi = nent - 1;  // s_inc[nent - 1] == P > q
+                while (lo < hi) {
+                    const uint32_t mid = (lo + hi) >> 1;
+                    if (s_inc[mid] > q) hi = mid;
+                    else lo = mid + 1;
+                }
+                const uint32_t exc = lo ? s_inc[lo - 1] : 0u;
+                const uint32_t p = s_s0[lo] + (q - exc);
+                const uint32_t ee = eb + lo;
+                const uint32_t col = A.Bcol[p];
+                accum<NUM, BITMAP>(A, U, S, A.Ali[ee], col - U.c0, NUM ? A.Aval[ee] * A.Bval[p] : 0.0);
+            }
+            __syncthreads();  // s_inc / s_s0 / s_w are rewritten by the next chunk
+        }
+        return;
+    }
+    for (uint32_t eb = le0 + wave * 64; eb < le1; eb += ACC_THREADS) {
+        const uint32_t e = eb + lane;
+        uint32_t s0 = 0, len = 0, li = 0;
+        double av = 0.0;
+        if (e < le1) {
+            span_of(e, s0, len);
+            li = A.Ali[e];
+            if (NUM) av = A.Aval[e];
+        }
+        uint32_t inc = len;
+        for (int d = 1; d < 64; d <<= 1) {
+            uint32_t t = __shfl_up(inc, d, 64);
+            if (lane >= (uint32_t)d) inc += t;
+        }
+        const uint32_t P = __shfl(inc, 63, 64);
+        for (uint32_t q0 = 0; q0 < P; q0 += 64) {
+            const uint32_t q = q0 + lane;
+            uint32_t j = 0;  // owner: number of lanes whose inclusive end is <= q
+#pragma unroll
+            for (int b = 32; b >= 1; b >>= 1) {
+                const uint32_t v = __shfl(inc, (int)(j + b - 1), 64);
+                if (v <= q) j += b;
+            }
+            const uint32_t oinc = __shfl(inc, (int)(j & 63), 64), olen = __shfl(len, (int)(j & 63), 64);
+            const uint32_t os0 = __shfl(s0, (int)(j & 63), 64), oli = __shfl(li, (int)(j & 63), 64);
+            const double oav = NUM ? __shfl(av, (int)(j & 63), 64) : 0.0;
+            if (q < P) {
+                const uint32_t p = os0 + (q - (oinc - olen));
+                const uint32_t col = A.Bcol[p];
+                accum<NUM, BITMAP>(A, U, S, oli, col - U.c0, NUM ? oav * A.Bval[p] : 0.0);
+            }
+        }
+    }
+}
+
+static_assert(LH_SLOTS + LH_SLOTS / 2 + ACC_THREADS <= DENSE_SLOTS, "LHASH flattening stage must fit in the LDS tile");
+
+// One unit: heavy groups (a wavefront per group), then the light entries.
+template <bool NUM, bool BITMAP>
+__device__ void process_unit(const AccArgs &A, const UnitCtx &U, const Acc<NUM, BITMAP> &S, uint32_t *s_gnext,
+                             uint32_t *stage, uint32_t *s_w) {
+    const uint32_t gl = A.atile_lg[U.a], g1 = A.atile_g0[U.a + 1];
     if (A.row_own && U.kind == KIND_DENSE && (1u << A.logTL) >= (uint32_t)NWAVES) {
         dense_heavy_owned<NUM, BITMAP>(A, U, S);
         __syncthreads();  // plain updates above, atomic updates below
     } else {
         heavy_claimed<NUM, BITMAP>(A, U, S, s_gnext);
     }
-    const uint32_t le0 = A.group_ptr[gl], le1 = A.group_ptr[g1];
-    for (uint32_t e = le0 + threadIdx.x; e < le1; e += ACC_THREADS) {
-        const uint32_t li = A.Ali[e];
-        const double av = NUM ? A.Aval[e] : 0.0;
-        const uint2 fp = A.Afp[e];
-#pragma unroll 4
-        for (uint32_t p = fp.x; p < fp.y; p++) {
-            const uint32_t col = A.Bcol[p];
-            if (U.multi && (col < U.c0 || col >= cend)) continue;
-            accum<NUM, BITMAP>(A, U, S, li, col - U.c0, NUM ? av * A.Bval[p] : 0.0);
-        }
-    }
+    light_entries<NUM, BITMAP>(A, U, S, A.group_ptr[gl], A.group_ptr[g1], stage, s_w);
 }
 
 // Block-wide ordered compaction over slots [0, n): emit(slot, rank) for every
@@ -640,7 +735,12 @@ __global__ __launch_bounds__(ACC_THREADS) void k_numeric(AccArgs A) {
         U.row0 = (uint64_t)U.tile << A.logTL;
         const uint32_t TL = 1u << A.logTL;
         const uint32_t nslots = U.kind == KIND_DENSE ? TL * U.Wp : (U.kind == KIND_LHASH ? LH_SLOTS : 0);
-        if (A.group_rows) sk.group = U.row0 / A.group_rows;
+        if (SINK && A.group_rows && U.row0 / A.group_rows != sk.group) {
+            // summaries are kept per group: flush when this block moves to another group
+            sink_flush(A, sk, s_su, s_sd);
+            sk = SinkAcc();
+            sk.group = U.row0 / A.group_rows;
+        }
         if (A.clock_start && ui == 0 && threadIdx.x == 0) *A.clock_start = wall_clock64();
         if (U.kind == KIND_DENSE) {
             const double init = BITMAP ? 0.0 : -0.0;
@@ -658,7 +758,13 @@ __global__ __launch_bounds__(ACC_THREADS) void k_numeric(AccArgs A) {
             s_gnext = 0;
         }
         __syncthreads();
-        process_unit<true, BITMAP>(A, U, S, &s_gnext);
+        // flattening stage: LDS the unit's accumulator leaves free (LHASH uses
+        // 4096 values + 4096 keys = 6144 doubles' worth, GHASH none)
+        uint32_t *stage = nullptr;
+        if (U.kind == KIND_LHASH) stage = reinterpret_cast<uint32_t *>(s_f64 + 6144);
+        else if (U.kind == KIND_GHASH) stage = reinterpret_cast<uint32_t *>(s_f64);
+        else if (nslots + ACC_THREADS <= DENSE_SLOTS) stage = reinterpret_cast<uint32_t *>(s_f64 + DENSE_SLOTS - ACC_THREADS);
+        process_unit<true, BITMAP>(A, U, S, &s_gnext, stage, s_w);
         __syncthreads();
         if (U.kind != KIND_GHASH) {
             auto pred = [&](uint32_t s) -> bool {
@@ -697,10 +803,6 @@ __global__ __launch_bounds__(ACC_THREADS) void k_numeric(AccArgs A) {
         } else if (!SINK && threadIdx.x == 0) {
             A.unit_cursor[U.u] = 0;
         }
-        if (SINK && A.group_rows) {  // summaries per group: flush at every unit's end
-            sink_flush(A, sk, s_su, s_sd);
-            sk = SinkAcc();
-        }
         if (A.group_left && threadIdx.x == 0) {
             const uint64_t g = U.row0 / A.group_rows;
             __threadfence();
@@ -708,12 +810,13 @@ __global__ __launch_bounds__(ACC_THREADS) void k_numeric(AccArgs A) {
         }
         __syncthreads();
     }
-    if (SINK && !A.group_rows) sink_flush(A, sk, s_su, s_sd);
+    if (SINK) sink_flush(A, sk, s_su, s_sd);
 }
 
 // Count pass: structural support only.
 __global__ __launch_bounds__(ACC_THREADS) void k_count(AccArgs A) {
     __shared__ __align__(16) uint32_t s_set[SYM_SLOTS];
+    __shared__ uint32_t s_stage[2 * ACC_THREADS];
     __shared__ uint32_t s_cnt, s_gnext, s_unit, s_w[NWAVES];
     Acc<false, false> S;
     S.fv = nullptr;
@@ -747,7 +850,7 @@ __global__ __launch_bounds__(ACC_THREADS) void k_count(AccArgs A) {
             s_gnext = 0;
         }
         __syncthreads();
-        process_unit<false, false>(A, U, S, &s_gnext);
+        process_unit<false, false>(A, U, S, &s_gnext, s_stage, s_w);
         __syncthreads();
         uint32_t c;
         if (U.kind == KIND_DENSE) {

@@ -229,8 +229,13 @@ __global__ void k_unit_to_tile(const unsigned long long *unit_count, const uint3
 }
 
 __global__ void k_kind_hist(const uint8_t *kind, uint32_t n, unsigned long long *h) {
-    uint32_t u = blockIdx.x * blockDim.x + threadIdx.x;
-    if (u < n) atomicAdd(&h[kind[u] & 3], 1ull);
+    uint32_t c[4] = {0, 0, 0, 0};
+    for (uint32_t u = blockIdx.x * blockDim.x + threadIdx.x; u < n; u += gridDim.x * blockDim.x) c[kind[u] & 3]++;
+    for (int k = 0; k < 4; k++) {
+        uint32_t v = c[k];
+        for (int d = 32; d > 0; d >>= 1) v += __shfl_xor(v, d, 64);
+        if ((threadIdx.x & 63) == 0 && v) atomicAdd(&h[k], (unsigned long long)v);
+    }
 }
 
 // Distinct output columns per active tile = |union of its groups' fiber
@@ -356,6 +361,36 @@ struct Contraction {
     uint32_t groups = 0;
     std::vector<unsigned long long> group_sink;  // 4 per group: count, hash, val_sum, val_wsum bits
     std::vector<double> group_ms;                // per group: call start -> output complete
+    // plans: keep what the numeric pass needs after run(), keep the sort
+    // permutations (values can be gathered again) and use explicit-bitmap
+    // dense accumulators (later values may hold zeros)
+    bool keep = false;
+    struct Kept {
+        std::vector<std::shared_ptr<void>> hold;
+        Index ix;
+        Geom g;
+        Units U;
+        Tmp<uint32_t> u_tile, u_c0, u_W;
+        Tmp<uint8_t> u_kind, u_multi;
+        Tmp<unsigned long long> unit_count, unit_off, unit_cur;
+        Tmp<unsigned int> work;
+        Tmp<unsigned long long> gmeta, sinkbuf;
+        Tmp<uint32_t> flags;
+        Tmp<unsigned int> g_left;
+        Tmp<unsigned long long> g_done;
+        Tmp<unsigned long long> gk;
+        Tmp<double> gv;
+        Tmp<uint32_t> gu;
+        Tmp<uint32_t> out_c;
+        Tmp<double> out_v;
+        Tmp<float> pred;  // per-tile MLP predictions (learned planner)
+        AccArgs X;
+        unsigned grid = 1;
+        bool bitmap = false, sink = false, empty = true;
+        size_t nsink = 4;
+        uint64_t stride = 0, nnz_out = 0;
+        int mode = 0;  // placement of the last numeric pass: 0 exact offsets, 1 atomic cursor
+    } k;
 
     // ka/kb: r key pairs, the first nb of them batch pairs
     int run(const fcc_tensor *a, const int32_t *ka, const fcc_tensor *b, const int32_t *kb, int r, int nb);
@@ -406,7 +441,8 @@ int Contraction::run(const fcc_tensor *ta, const int32_t *am, const fcc_tensor *
     for (int m = 0; m < ta->order; m++) EA[m] = ta->extents ? ta->extents[m] : (ta->nnz ? hext[m] + 1 : 0);
     for (int m = 0; m < tb->order; m++)
         EB[m] = tb->extents ? tb->extents[m] : (tb->nnz ? hext[kMaxOrder + 1 + m] + 1 : 0);
-    const bool bitmap = (hext[ta->order] | hext[kMaxOrder + 1 + tb->order]) != 0;
+    const bool bitmap = keep || (hext[ta->order] | hext[kMaxOrder + 1 + tb->order]) != 0;
+    k.bitmap = bitmap;
     if (ta->extents)
         for (int m = 0; m < ta->order; m++)
             if (ta->nnz && hext[m] >= EA[m]) throw LibError{FCC_ERR_INVALID, "a: coordinate exceeds given extent"};
@@ -416,6 +452,7 @@ int Contraction::run(const fcc_tensor *ta, const int32_t *am, const fcc_tensor *
                 throw LibError{FCC_ERR_INVALID, "b: coordinate exceeds given extent"};
 
     Geom g = make_geom(ta, am, tb, bm, r, nb, EA, EB);
+    k.g = g;
     out->order = g.dC;
     for (int m = 0; m < g.nxa; m++) out->extents[m] = g.xaE[m];
     for (int m = 0; m < g.nxb; m++) out->extents[g.nxa + m] = g.xbE[m];
@@ -433,8 +470,9 @@ int Contraction::run(const fcc_tensor *ta, const int32_t *am, const fcc_tensor *
         throw LibError{FCC_ERR_INVALID, "contraction key and column exceed 64 bits"};
 
     // ---- index
-    std::vector<std::shared_ptr<void>> hold;
-    Index ix;
+    auto &hold = k.hold;
+    Index &ix = k.ix;
+    ix.keep_perm = keep;
     Tmp<uint32_t> rowA(pool, (size_t)A.nnz), fidA(pool, (size_t)A.nnz);
     uint64_t products = 0;
     uint32_t nvalid = 0;
@@ -491,7 +529,10 @@ int Contraction::run(const fcc_tensor *ta, const int32_t *am, const fcc_tensor *
 
     uint64_t nnz_out = 0;
     const size_t nsink = 4 * (size_t)std::max<uint32_t>(groups, 1);
-    Tmp<unsigned long long> sinkbuf(pool, nsink);
+    k.nsink = nsink;
+    k.sink = sink;
+    auto &sinkbuf = k.sinkbuf;
+    sinkbuf = Tmp<unsigned long long>(pool, nsink);
     fill_u64(sinkbuf, nsink, 0, s);
     if (ix.natiles == 0) {
         FCC_HIP(hipEventRecord(ev[2], s));
@@ -504,7 +545,8 @@ int Contraction::run(const fcc_tensor *ta, const int32_t *am, const fcc_tensor *
 
     // ---- plan
     const uint32_t T = ix.natiles;
-    Tmp<float> feat, pred_t, predc_t;
+    Tmp<float> feat, predc_t;
+    auto &pred_t = k.pred;
     if (learned || o.collect_stats) {
         feat = Tmp<float>(pool, (size_t)T * MLP_IN);
         mlp_tile_features(ix, g.spanB, feat, s);
@@ -522,13 +564,14 @@ int Contraction::run(const fcc_tensor *ta, const int32_t *am, const fcc_tensor *
     P.margin = 0.15f;
     Tmp<uint32_t> t_nwin(pool, T), t_W(pool, T), t_ubase(pool, T), d_nunits(pool, 1);
     Tmp<uint8_t> t_kind(pool, T);
-    Units U;
-    Tmp<uint32_t> u_tile, u_c0, u_W;
-    Tmp<uint8_t> u_kind, u_multi;
-    Tmp<unsigned long long> unit_count, unit_off, unit_cur, d_total(pool, 1);
+    Units &U = k.U;
+    auto &u_tile = k.u_tile, &u_c0 = k.u_c0, &u_W = k.u_W;
+    auto &u_kind = k.u_kind, &u_multi = k.u_multi;
+    auto &unit_count = k.unit_count, &unit_off = k.unit_off, &unit_cur = k.unit_cur;
+    Tmp<unsigned long long> d_total(pool, 1);
 
     // ---- accumulate arguments
-    AccArgs X;
+    AccArgs &X = k.X;
     memset(&X, 0, sizeof(X));
     X.Bcol = ix.Bcol;
     X.Bval = ix.Bval;
@@ -538,6 +581,8 @@ int Contraction::run(const fcc_tensor *ta, const int32_t *am, const fcc_tensor *
     X.Afp = ix.Afp;
     X.group_ptr = ix.group_ptr;
     X.gdesc = ix.gdesc;
+    X.hdesc = ix.hdesc;
+    X.atile_h0 = ix.atile_h0;
     X.group_fiber = ix.group_fiber;
     X.atile_id = ix.atile_id;
     X.atile_g0 = ix.atile_g0;
@@ -552,9 +597,12 @@ int Contraction::run(const fcc_tensor *ta, const int32_t *am, const fcc_tensor *
         X.xaE[m] = g.xaE[m];
         X.xbE[m] = g.xbE[m];
     }
-    Tmp<unsigned int> work(pool, 4);
-    Tmp<unsigned long long> gmeta(pool, 4);  // [0]=gcount [1]=cursor [2]=gtotal
-    Tmp<uint32_t> flags(pool, 4);            // [0]=gflag [1]=fail
+    auto &work = k.work;
+    auto &gmeta = k.gmeta;  // [0]=gcount [1]=cursor [2]=gtotal
+    auto &flags = k.flags;  // [0]=gflag [1]=fail
+    work = Tmp<unsigned int>(pool, 4);
+    gmeta = Tmp<unsigned long long>(pool, 4);
+    flags = Tmp<uint32_t>(pool, 4);
     X.work = work;
     X.gcount = gmeta.get();
     X.cursor = gmeta.get() + 1;
@@ -562,8 +610,8 @@ int Contraction::run(const fcc_tensor *ta, const int32_t *am, const fcc_tensor *
     X.fail_flag = flags.get() + 1;
     X.sink = sinkbuf.get();
     X.sinkf = (double *)(sinkbuf.get() + 2);
-    Tmp<unsigned int> g_left;
-    Tmp<unsigned long long> g_done;
+    auto &g_left = k.g_left;
+    auto &g_done = k.g_done;
     if (groups) {
         X.group_rows = g.spanA / groups;
         g_left = Tmp<unsigned int>(pool, groups);
@@ -579,7 +627,8 @@ int Contraction::run(const fcc_tensor *ta, const int32_t *am, const fcc_tensor *
         fill_u64(g_done, groups + 1, 0, s);
         group_units(X.unit_tile, ix.atile_id, X.nunits, ix.logTL, X.group_rows, g_left, s);
     };
-    unsigned grid = 1;
+    unsigned &grid = k.grid;
+    grid = 1;
 
     // units from the exact bounds (pred == null) or from predictions
     auto make_units = [&](const float *pred) {
@@ -618,9 +667,9 @@ int Contraction::run(const fcc_tensor *ta, const int32_t *am, const fcc_tensor *
     make_units(learned ? pred_t.get() : nullptr);
     FCC_HIP(hipEventRecord(ev[2], s));
 
-    Tmp<unsigned long long> gk;
-    Tmp<double> gv;
-    Tmp<uint32_t> gu;
+    auto &gk = k.gk;
+    auto &gv = k.gv;
+    auto &gu = k.gu;
     auto alloc_ghash = [&](uint64_t want) {
         uint64_t cap = pow2_ceil(std::max<uint64_t>(want, 1u << 16));
         gk = Tmp<unsigned long long>(pool, cap);
@@ -639,8 +688,8 @@ int Contraction::run(const fcc_tensor *ta, const int32_t *am, const fcc_tensor *
         fill_u32(work, 4, 0, s);
     };
 
-    Tmp<uint32_t> out_c;
-    Tmp<double> out_v;
+    auto &out_c = k.out_c;
+    auto &out_v = k.out_v;
     uint64_t stride = 0;
     bool use_exact = !learned;
 
@@ -794,15 +843,21 @@ int Contraction::run(const fcc_tensor *ta, const int32_t *am, const fcc_tensor *
     }
     out->nnz = (int64_t)nnz_out;
     out->coord_stride = (int64_t)stride;
+    k.stride = stride;
+    k.nnz_out = nnz_out;
+    k.mode = (learned && !use_exact) ? 1 : 0;
+    k.empty = false;
     if (!sink) {
-        ResultPriv *rp = new ResultPriv;
-        rp->coords = out_c.p;
-        rp->vals = out_v.p;
         out->coords = out_c.p;
         out->vals = out_v.p;
-        out_c.p = nullptr;
-        out_v.p = nullptr;
-        out->priv = rp;
+        if (!keep) {  // the result owns the output; a plan keeps it
+            ResultPriv *rp = new ResultPriv;
+            rp->coords = out_c.p;
+            rp->vals = out_v.p;
+            out_c.p = nullptr;
+            out_v.p = nullptr;
+            out->priv = rp;
+        }
         if (o.summarize && nnz_out) {
             Tmp<unsigned long long> hs(pool, 1);
             Tmp<double> vs(pool, 2);
@@ -825,7 +880,7 @@ int Contraction::run(const fcc_tensor *ta, const int32_t *am, const fcc_tensor *
     {
         Tmp<unsigned long long> kh(pool, 4);
         fill_u64(kh, 4, 0, s);
-        k_kind_hist<<<grid256(U.n), 256, 0, s>>>(U.kind, U.n, kh);
+        k_kind_hist<<<std::min(grid256(U.n), 1024u), 256, 0, s>>>(U.kind, U.n, kh);
         unsigned long long h[4];
         FCC_HIP(hipMemcpyAsync(h, kh.get(), 32, hipMemcpyDeviceToHost, s));
         FCC_HIP(hipStreamSynchronize(s));
@@ -1391,6 +1446,291 @@ int fcc_contract_batch(fcc_ctx *ctx, int32_t n, const fcc_tensor *a, const int32
     }
     for (int32_t k = 0; k < n; k++) fcc_result_free(ctx, &outs[k]);
     return rc;
+}
+
+// ---------------------------------------------------------------- plans
+
+struct fcc_plan {
+    fcc_ctx *ctx = nullptr;
+    std::unique_ptr<Contraction> c;
+    bool self_values = false;  // b's values are a's (the same tensor on both sides)
+    int64_t nA = 0, nB = 0;
+    Tmp<double> a_stage, b_stage;  // value staging: the graph reads fixed addresses
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    fcc_stats stats{};
+    fcc_plan_info info{};
+    int32_t order = 0;
+    uint32_t extents[FCC_MAX_ORDER] = {};
+    ~fcc_plan() {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+    }
+};
+
+namespace {
+
+// The numeric pass of a plan on the values staged at asrc / bsrc: gather the
+// values into index order, clear the counters and the global hash with fill
+// kernels (no memset nodes: the sequence may be captured into a graph), run
+// the persistent accumulate kernel and the global-hash drain.
+void plan_sequence(fcc_plan *P, const double *asrc, const double *bsrc, hipStream_t s) {
+    Contraction::Kept &k = P->c->k;
+    AccArgs &X = k.X;
+    perm_gather(bsrc, k.ix.permB, k.ix.nB, k.ix.Bval, s);
+    perm_gather(asrc, k.ix.permA, k.ix.nAv, k.ix.Aval, s);
+    fill_u64(X.gkeys, X.gmask + 1, ~0ull, s);
+    fill_f64(X.gvals, X.gmask + 1, 0.0, s);
+    fill_u64(k.gmeta, 4, 0, s);
+    fill_u32(k.flags, 4, 0, s);
+    fill_u32(k.work, 4, 0, s);
+    fill_u64(k.sinkbuf, k.nsink, 0, s);
+    launch_numeric(X, k.grid, k.bitmap, k.sink, 0, s);
+    launch_gdrain(X, k.sink, 0, s);
+}
+
+}  // namespace
+
+int fcc_plan_create(fcc_ctx *ctx, const fcc_tensor *a, const int32_t *a_modes, const int32_t *a_batch,
+                    const fcc_tensor *b, const int32_t *b_modes, const int32_t *b_batch, int32_t n_contr,
+                    int32_t n_batch, const fcc_options *options, fcc_plan **plan) {
+    if (!ctx) return FCC_ERR_INVALID;
+    if (!plan) return fail(ctx, FCC_ERR_INVALID, "plan is NULL");
+    *plan = nullptr;
+    std::string msg;
+    int rc = validate(a, "a", msg);
+    if (!rc) rc = validate(b, "b", msg);
+    std::vector<int32_t> ka, kb;
+    if (!rc) rc = check_modes(a, b, a_modes, b_modes, n_contr, a_batch, b_batch, n_batch, ka, kb, msg);
+    if (rc) return fail(ctx, rc, msg);
+    std::unique_ptr<fcc_plan> P(new fcc_plan);
+    P->ctx = ctx;
+    P->c.reset(new Contraction);
+    Contraction &c = *P->c;
+    c.ctx = ctx;
+    c.s = ctx->stream;
+    fcc_options_default(&c.o);
+    if (options) c.o = *options;
+    if (c.o.left_tile_stride < 1) c.o.left_tile_stride = 1;
+    c.keep = true;
+    fcc_result first;
+    memset(&first, 0, sizeof(first));
+    c.out = &first;
+    P->self_values = a == b || (a->coords == b->coords && a->vals == b->vals && a->order == b->order &&
+                                a->nnz == b->nnz && a->on_device == b->on_device);
+    P->nA = a->nnz;
+    P->nB = b->nnz;
+    hipStream_t s = ctx->stream;
+    try {
+        FCC_HIP(hipSetDevice(ctx->device));
+        c.run(a, ka.data(), b, kb.data(), n_contr + n_batch, n_batch);
+        Contraction::Kept &k = c.k;
+        DevPool &pool = ctx->pool;
+        if (!k.empty && k.mode == 1) {
+            // learned run: its per-unit output counts become exact offsets
+            Tmp<unsigned long long> tot(pool, 1);
+            scan_u64(pool, (const uint64_t *)k.unit_cur.get(), (uint64_t *)k.unit_off.get(), k.U.n,
+                     (uint64_t *)tot.get(), s);
+            k.X.unit_off = k.unit_off;
+            k.X.capacity = k.sink ? ~0ull : k.stride;
+            k.mode = 0;
+        }
+        if (!k.empty && first.stats.ghash_entries * 2 > k.X.gmask + 1) {
+            uint64_t cap = pow2_ceil(first.stats.ghash_entries * 2);
+            k.gk = Tmp<unsigned long long>(pool, cap);
+            k.gv = Tmp<double>(pool, cap);
+            k.gu = Tmp<uint32_t>(pool, cap);
+            k.X.gkeys = k.gk;
+            k.X.gvals = k.gv;
+            k.X.gunit = k.gu;
+            k.X.gmask = cap - 1;
+        }
+        P->a_stage = Tmp<double>(pool, (size_t)std::max<int64_t>(P->nA, 1));
+        P->b_stage = Tmp<double>(pool, (size_t)std::max<int64_t>(P->nB, 1));
+        // plan summary (exact count, bounds, shape)
+        fcc_plan_info &I = P->info;
+        I.out_nnz = first.nnz;
+        I.products = first.stats.products;
+        I.left_tiles = first.stats.left_tiles;
+        I.units = first.stats.units;
+        I.units_dense = first.stats.units_dense;
+        I.units_lhash = first.stats.units_lhash;
+        I.units_ghash = first.stats.units_ghash;
+        I.row_tile = first.stats.row_tile;
+        I.col_window = first.stats.col_window;
+        I.order = first.order;
+        memcpy(I.extents, first.extents, sizeof(I.extents));
+        I.ms_plan = first.stats.ms_total;
+        const uint32_t T = k.ix.natiles;
+        if (!k.empty && T) {
+            std::vector<unsigned long long> mg(T), pr(T);
+            FCC_HIP(hipMemcpyAsync(mg.data(), k.ix.st_maxgp, 8ull * T, hipMemcpyDeviceToHost, s));
+            FCC_HIP(hipMemcpyAsync(pr.data(), k.ix.st_products, 8ull * T, hipMemcpyDeviceToHost, s));
+            FCC_HIP(hipStreamSynchronize(s));
+            const double area = (double)k.ix.TL * (double)k.g.spanB;
+            for (uint32_t t = 0; t < T; t++) {
+                I.nnz_lower_bound += mg[t];
+                I.nnz_upper_bound += (uint64_t)std::min<double>((double)pr[t], area);
+            }
+        }
+        P->stats = first.stats;
+        P->order = first.order;
+        memcpy(P->extents, first.extents, sizeof(P->extents));
+        *plan = P.release();
+        ctx->err.clear();
+        return FCC_OK;
+    } catch (HipError &e) {
+        (void)hipGetLastError();
+        return fail(ctx, FCC_ERR_HIP, e.where + ": " + hipGetErrorString(e.code));
+    } catch (LibError &e) {
+        return fail(ctx, e.code, e.msg);
+    } catch (std::bad_alloc &) {
+        return fail(ctx, FCC_ERR_OOM, "host allocation failed");
+    }
+}
+
+int fcc_plan_info_get(const fcc_plan *plan, fcc_plan_info *info) {
+    if (!plan || !info) return FCC_ERR_INVALID;
+    *info = plan->info;
+    return FCC_OK;
+}
+
+int64_t fcc_plan_tile_count(const fcc_plan *plan) {
+    if (!plan || plan->c->k.empty) return 0;
+    return (int64_t)plan->c->k.ix.natiles;
+}
+
+int fcc_plan_tiles(fcc_plan *plan, uint64_t *nnz, uint64_t *products, float *predicted, int64_t n) {
+    if (!plan) return FCC_ERR_INVALID;
+    fcc_ctx *ctx = plan->ctx;
+    Contraction::Kept &k = plan->c->k;
+    if (k.empty) return n == 0 ? FCC_OK : fail(ctx, FCC_ERR_INVALID, "plan has no tiles");
+    const uint32_t T = k.ix.natiles;
+    if (n != (int64_t)T) return fail(ctx, FCC_ERR_INVALID, "n must equal fcc_plan_tile_count");
+    hipStream_t s = ctx->stream;
+    try {
+        if (nnz) {
+            // per-unit exact counts: the count pass (exact planner) or the
+            // units' emitted outputs of the last numeric pass
+            Tmp<unsigned long long> tn(ctx->pool, T);
+            fill_u64(tn, T, 0, s);
+            const unsigned long long *uc = plan->c->o.planner == FCC_PLAN_LEARNED ? k.unit_cur.get() : k.unit_count.get();
+            k_unit_to_tile<<<grid256(k.U.n), 256, 0, s>>>(uc, k.U.tile, k.U.n, tn);
+            FCC_HIP(hipMemcpyAsync(nnz, tn.get(), 8ull * T, hipMemcpyDeviceToHost, s));
+            FCC_HIP(hipStreamSynchronize(s));
+        }
+        if (products) FCC_HIP(hipMemcpyAsync(products, k.ix.st_products, 8ull * T, hipMemcpyDeviceToHost, s));
+        if (predicted) {
+            if (k.pred.get()) FCC_HIP(hipMemcpyAsync(predicted, k.pred.get(), 4ull * T, hipMemcpyDeviceToHost, s));
+            else
+                for (uint32_t t = 0; t < T; t++) predicted[t] = -1.0f;
+        }
+        FCC_HIP(hipStreamSynchronize(s));
+    } catch (HipError &e) {
+        (void)hipGetLastError();
+        return fail(ctx, FCC_ERR_HIP, e.where);
+    } catch (LibError &e) {
+        return fail(ctx, e.code, e.msg);
+    }
+    return FCC_OK;
+}
+
+int fcc_plan_execute(fcc_plan *plan, const double *a_vals, const double *b_vals, int32_t on_device,
+                     int32_t use_graph, fcc_result *out) {
+    if (!plan) return FCC_ERR_INVALID;
+    fcc_ctx *ctx = plan->ctx;
+    if (!out) return fail(ctx, FCC_ERR_INVALID, "out is NULL");
+    memset(out, 0, sizeof(*out));
+    const bool self = plan->self_values;
+    if ((plan->nA && !a_vals) || (!self && plan->nB && !b_vals)) return fail(ctx, FCC_ERR_INVALID, "NULL values");
+    Contraction::Kept &k = plan->c->k;
+    out->order = plan->order;
+    memcpy(out->extents, plan->extents, sizeof(out->extents));
+    out->stats = plan->stats;
+    if (k.empty) return FCC_OK;
+    hipStream_t s = ctx->stream;
+    hipEvent_t *ev = ctx->ev;
+    try {
+        FCC_HIP(hipSetDevice(ctx->device));
+        FCC_HIP(hipEventRecord(ev[0], s));
+        const double *asrc = a_vals, *bsrc = self ? a_vals : b_vals;
+        if (use_graph || !on_device) {
+            // stage into the plan's fixed buffers
+            const hipMemcpyKind kind = on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
+            if (plan->nA && a_vals != plan->a_stage.get())
+                FCC_HIP(hipMemcpyAsync(plan->a_stage.get(), a_vals, 8ull * plan->nA, kind, s));
+            asrc = plan->a_stage.get();
+            if (self) {
+                bsrc = asrc;
+            } else {
+                if (plan->nB && b_vals != plan->b_stage.get())
+                    FCC_HIP(hipMemcpyAsync(plan->b_stage.get(), b_vals, 8ull * plan->nB, kind, s));
+                bsrc = plan->b_stage.get();
+            }
+        }
+        FCC_HIP(hipEventRecord(ev[4], s));
+        if (use_graph) {
+            if (!plan->exec) {
+                hipGraph_t g = nullptr;
+                FCC_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+                try {
+                    plan_sequence(plan, asrc, bsrc, s);
+                } catch (...) {
+                    (void)hipStreamEndCapture(s, &g);
+                    if (g) (void)hipGraphDestroy(g);
+                    throw;
+                }
+                FCC_HIP(hipStreamEndCapture(s, &g));
+                plan->graph = g;
+                FCC_HIP(hipGraphInstantiate(&plan->exec, g, nullptr, nullptr, 0));
+            }
+            FCC_HIP(hipGraphLaunch(plan->exec, s));
+        } else {
+            plan_sequence(plan, asrc, bsrc, s);
+        }
+        FCC_HIP(hipEventRecord(ev[5], s));
+        uint32_t hf[2];
+        FCC_HIP(hipMemcpyAsync(hf, k.flags.get(), 8, hipMemcpyDeviceToHost, s));
+        unsigned long long sk[4];
+        FCC_HIP(hipMemcpyAsync(sk, k.sinkbuf.get(), 32, hipMemcpyDeviceToHost, s));
+        FCC_HIP(hipStreamSynchronize(s));
+        if (hf[0]) throw LibError{FCC_ERR_OVERFLOW, "global hash overflow in the numeric pass"};
+        if (hf[1]) throw LibError{FCC_ERR_OVERFLOW, "numeric pass disagreed with the plan"};
+        out->nnz = (int64_t)k.nnz_out;
+        out->coord_stride = (int64_t)k.stride;
+        if (k.sink) {
+            out->nnz = (int64_t)sk[0];
+            out->coord_hash = sk[0] ? sk[1] : 0;
+            memcpy(&out->val_sum, &sk[2], 8);
+            memcpy(&out->val_wsum, &sk[3], 8);
+        } else {
+            out->coords = k.out_c.get();
+            out->vals = k.out_v.get();
+        }
+        fcc_stats &st = out->stats;
+        st.ms_index = 0;
+        st.ms_plan = 0;
+        st.ms_symbolic = 0;
+        st.ms_numeric = ev_ms(ev[4], ev[5]);
+        st.ms_numeric_kernel = st.ms_numeric;
+        st.ms_overflow = 0;
+        st.ms_total = ev_ms(ev[0], ev[5]);
+        st.ms_done = st.ms_total;
+        st.reruns = 0;
+        ctx->err.clear();
+        return FCC_OK;
+    } catch (HipError &e) {
+        (void)hipGetLastError();
+        return fail(ctx, FCC_ERR_HIP, e.where + ": " + hipGetErrorString(e.code));
+    } catch (LibError &e) {
+        return fail(ctx, e.code, e.msg);
+    }
+}
+
+void fcc_plan_destroy(fcc_plan *plan) {
+    if (!plan) return;
+    (void)hipSetDevice(plan->ctx->device);
+    delete plan;
 }
 
 }  // extern "C"

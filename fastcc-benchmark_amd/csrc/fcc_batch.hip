@@ -144,12 +144,23 @@ __global__ __launch_bounds__(256) void k_split_scatter(const uint32_t *coords, u
     }
 }
 
+// units arrive in tile order, so a wavefront's units mostly share a group:
+// one atomic per wavefront then
 __global__ void k_group_units(const uint32_t *unit_tile, const uint32_t *atile_id, uint32_t nunits, int logTL,
                               uint64_t group_rows, unsigned int *left) {
     const uint32_t u = blockIdx.x * blockDim.x + threadIdx.x;
-    if (u >= nunits) return;
-    const uint64_t row0 = (uint64_t)atile_id[unit_tile[u]] << logTL;
-    atomicAdd(&left[row0 / group_rows], 1u);
+    const bool t = u < nunits;
+    const uint32_t g = t ? (uint32_t)(((uint64_t)atile_id[unit_tile[u]] << logTL) / group_rows) : 0u;
+    const uint32_t lane = threadIdx.x & 63;
+    const uint64_t act = __ballot(t);
+    if (!act) return;
+    const int first = __ffsll((unsigned long long)act) - 1;
+    const uint32_t g0 = __shfl(g, first, 64);
+    if (__ballot(t && g != g0) == 0) {
+        if (lane == (uint32_t)first) atomicAdd(&left[g0], (unsigned int)__popcll(act));
+    } else if (t) {
+        atomicAdd(&left[g], 1u);
+    }
 }
 
 }  // namespace

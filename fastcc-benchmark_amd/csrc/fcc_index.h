@@ -7,6 +7,8 @@ namespace fcc {
 void fill_u32(uint32_t *p, size_t n, uint32_t v, hipStream_t s);
 void fill_u64(unsigned long long *p, size_t n, unsigned long long v, hipStream_t s);
 void fill_f64(double *p, size_t n, double v, hipStream_t s);
+// dst[p] = src[perm[p]]
+void perm_gather(const double *src, const uint32_t *perm, size_t n, double *dst, hipStream_t s);
 
 // out[m] = max coordinate of mode m (m < order); out[order] |= 1 when a value
 // is zero or tiny. out must be zeroed by the caller.

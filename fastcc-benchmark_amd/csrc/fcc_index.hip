@@ -25,6 +25,15 @@ static unsigned fill_grid(size_t n) {
     size_t b = div_up(n ? n : 1, 256);
     return (unsigned)(b > 8192 ? 8192 : b);
 }
+__global__ void k_perm_gather(const double *src, const uint32_t *perm, size_t n, double *dst) {
+    for (size_t p = blockIdx.x * (size_t)blockDim.x + threadIdx.x; p < n; p += (size_t)gridDim.x * blockDim.x)
+        dst[p] = src[perm[p]];
+}
+
+void perm_gather(const double *src, const uint32_t *perm, size_t n, double *dst, hipStream_t s) {
+    if (n) k_perm_gather<<<(unsigned)std::min<size_t>((n + 255) / 256, 8192), 256, 0, s>>>(src, perm, n, dst);
+}
+
 void fill_u32(uint32_t *p, size_t n, uint32_t v, hipStream_t s) {
     if (n) k_fill_u32<<<fill_grid(n), 256, 0, s>>>(p, n, v);
 }
@@ -245,6 +254,31 @@ __global__ void k_gdesc(const uint32_t *group_ptr, const uint32_t *group_fiber, 
     gdesc[g] = make_uint4(fp0, fiber_ptr[f + 1] - fp0, e0, group_ptr[g + 1] - e0);
 }
 
+// segments per group: heavy groups are cut every HSEG fiber entries, light
+// groups have none
+__global__ void k_hseg_count(const uint4 *gdesc, const uint8_t *glight, uint32_t ngroups, uint32_t *nseg) {
+    uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= ngroups) return;
+    nseg[g] = glight[g] ? 0u : (gdesc[g].y + HSEG - 1) / HSEG;
+}
+
+__global__ void k_hseg_emit(const uint4 *gdesc, const uint32_t *off, const uint32_t *nseg, uint32_t ngroups,
+                            uint4 *hdesc) {
+    uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= ngroups) return;
+    const uint4 d = gdesc[g];
+    const uint32_t n = nseg[g], o = off[g];
+    for (uint32_t k = 0; k < n; k++) {
+        const uint32_t s0 = k * HSEG;
+        hdesc[o + k] = make_uint4(d.x + s0, min(HSEG, d.y - s0), d.z, d.w);
+    }
+}
+
+__global__ void k_tile_hseg(const uint32_t *atile_g0, const uint32_t *off, uint32_t natiles, uint32_t *atile_h0) {
+    uint32_t a = blockIdx.x * blockDim.x + threadIdx.x;
+    if (a <= natiles) atile_h0[a] = off[atile_g0[a]];
+}
+
 __global__ void k_tile_light_init(const uint32_t *atile_g0, uint32_t natiles, uint32_t *atile_lg) {
     uint32_t a = blockIdx.x * blockDim.x + threadIdx.x;
     if (a < natiles) atile_lg[a] = atile_g0[a + 1];
@@ -412,6 +446,10 @@ void build_right(DevPool &pool, const Geom &g, const Operand &B, Index &ix, std:
     ix.Bval = (double *)keep(n * 8);
     Tmp<uint32_t> flag(pool, n), fid(pool, n);
     k_gather_B<<<grid_for(n), 256, 0, s>>>(ks, vs, B.vals, n, g.cbits, ix.Bcol, ix.Bval, flag);
+    if (ix.keep_perm) {
+        ix.permB = (uint32_t *)keep(n * 4);
+        if (n) FCC_HIP(hipMemcpyAsync(ix.permB, vs, n * 4, hipMemcpyDeviceToDevice, s));
+    }
     Tmp<uint32_t> total(pool, 1);
     scan_u32(pool, flag, fid, n, total, s);
     ix.fiber_ptr = (uint32_t *)keep((n + 1) * 4);
@@ -551,6 +589,10 @@ static void left_from_sorted(DevPool &pool, const Operand &A, const uint64_t *ks
     if (m)
         k_gather_A<<<grid_for(m), 256, 0, s>>>(ks, vs, A.vals, ix.fiber_ptr, m, logTL, fbits, ix.Ali, ix.Aval, ix.Afib,
                                                ix.Afp, gflag, tflag);
+    if (ix.keep_perm) {
+        ix.permA = (uint32_t *)keep(m * 4);
+        if (m) FCC_HIP(hipMemcpyAsync(ix.permA, vs, m * 4, hipMemcpyDeviceToDevice, s));
+    }
     scan_u32(pool, gflag, gid, m, tot.get(), s);
     scan_u32(pool, tflag, aid, m, tot.get() + 1, s);
     uint32_t h[2];
@@ -572,6 +614,19 @@ static void left_from_sorted(DevPool &pool, const Operand &A, const uint64_t *ks
         k_gdesc<<<grid_for(ix.ngroups), 256, 0, s>>>(ix.group_ptr, ix.group_fiber, ix.fiber_ptr, ix.ngroups, ix.gdesc);
         k_tile_light_init<<<grid_for(ix.natiles), 256, 0, s>>>(ix.atile_g0, ix.natiles, ix.atile_lg);
         k_tile_light<<<grid_for(ix.ngroups), 256, 0, s>>>(glight, ix.group_atile, ix.atile_g0, ix.ngroups, ix.atile_lg);
+    }
+    // heavy segments
+    {
+        Tmp<uint32_t> nseg(pool, (size_t)ix.ngroups + 1), off(pool, (size_t)ix.ngroups + 1), nh(pool, 1);
+        if (ix.ngroups) k_hseg_count<<<grid_for(ix.ngroups), 256, 0, s>>>(ix.gdesc, glight, ix.ngroups, nseg);
+        fill_u32(nseg.get() + ix.ngroups, 1, 0, s);
+        scan_u32(pool, nseg, off, (size_t)ix.ngroups + 1, nh, s);
+        FCC_HIP(hipMemcpyAsync(&ix.nhseg, nh.get(), 4, hipMemcpyDeviceToHost, s));
+        FCC_HIP(hipStreamSynchronize(s));
+        ix.hdesc = (uint4 *)keep((size_t)ix.nhseg * 16);
+        ix.atile_h0 = (uint32_t *)keep(((size_t)ix.natiles + 1) * 4);
+        if (ix.ngroups) k_hseg_emit<<<grid_for(ix.ngroups), 256, 0, s>>>(ix.gdesc, off, nseg, ix.ngroups, ix.hdesc);
+        k_tile_hseg<<<grid_for((size_t)ix.natiles + 1), 256, 0, s>>>(ix.atile_g0, off, ix.natiles, ix.atile_h0);
     }
     // statistics
     size_t T = ix.natiles;

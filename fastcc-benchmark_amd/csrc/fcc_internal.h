@@ -121,6 +121,7 @@ constexpr uint32_t SYM_SLOTS = 16384;   // LDS hash-set slots of the count pass 
 constexpr uint32_t SYM_MAX = 12288;
 constexpr uint32_t HASH_TARGET = 2048;  // planned distinct outputs per hash unit
 constexpr int ACC_THREADS = 512;
+constexpr uint32_t HSEG = 512;  // fiber entries per heavy segment
 
 // Device-side index of one contraction: B's CSR keyed by contraction key
 // ("fibers") and A's entries grouped by (left tile, fiber).
@@ -144,6 +145,12 @@ struct Index {
     uint32_t *atile_id = nullptr;     // tile index of active tile a
     uint32_t *atile_g0 = nullptr;     // natiles+1: first group of active tile a
     uint32_t *atile_lg = nullptr;     // first light group of active tile a
+    // heavy groups cut into segments of at most HSEG fiber entries, so long
+    // fibers spread over many wavefronts; tile a's segments are
+    // [atile_h0[a], atile_h0[a + 1])
+    uint32_t nhseg = 0;
+    uint4 *hdesc = nullptr;           // per segment: fiber start, length, first entry, entries
+    uint32_t *atile_h0 = nullptr;     // natiles+1
     // per active tile statistics (planner features)
     uint64_t *st_products = nullptr;  // sum over groups of m * |fiber|
     uint64_t *st_sumfib = nullptr;    // sum over groups of |fiber|
@@ -156,6 +163,11 @@ struct Index {
     uint32_t TL = 1;
     uint64_t products = 0;
     uint32_t heavy_min = HEAVY_MIN;  // fiber length from which groups are processed wave-per-group
+    // kept for plans (new values on the same coordinates): the input entry
+    // behind each sorted position, so Bval/Aval can be gathered again
+    bool keep_perm = false;
+    uint32_t *permB = nullptr;  // nB
+    uint32_t *permA = nullptr;  // nAv
 };
 
 struct Units {
@@ -178,6 +190,8 @@ struct AccArgs {
     const uint2 *Afp;
     const uint32_t *group_ptr;
     const uint4 *gdesc;
+    const uint4 *hdesc;
+    const uint32_t *atile_h0;
     const uint32_t *group_fiber;
     const uint32_t *atile_id;
     const uint32_t *atile_g0;

@@ -114,6 +114,71 @@ class Result:
             pass
 
 
+def _vals_arg(v, keep):
+    """(pointer, on_device) for a float64 numpy array or a CUDA tensor."""
+    if v is None:
+        return None, 0
+    if hasattr(v, "is_cuda") and v.is_cuda:
+        assert v.is_contiguous() and v.element_size() == 8
+        keep.append(v)
+        return v.data_ptr(), 1
+    a = np.ascontiguousarray(v, dtype=np.float64)
+    keep.append(a)
+    return a.ctypes.data, 0
+
+
+class Plan:
+    """A contraction planned for one coordinate pattern (fcc_plan_create):
+    execute() reruns only the numeric pass on new values, optionally as a
+    replayed hipGraph; info() and tiles() answer the symbolic queries."""
+
+    def __init__(self, ctx: "Context", h):
+        self._ctx = ctx
+        self.h = h
+
+    def info(self) -> dict:
+        i = _abi.FccPlanInfo()
+        self._ctx._check(self._ctx.lib.fcc_plan_info_get(self.h, C.byref(i)))
+        d = {name: getattr(i, name) for name, _ in _abi.FccPlanInfo._fields_ if name != "extents"}
+        d["extents"] = tuple(i.extents[m] for m in range(i.order))
+        return d
+
+    def tiles(self):
+        """(exact nnz, products, predicted nnz or -1) per active left tile."""
+        n = int(self._ctx.lib.fcc_plan_tile_count(self.h))
+        nnz = np.zeros(n, dtype=np.uint64)
+        prod = np.zeros(n, dtype=np.uint64)
+        pred = np.zeros(n, dtype=np.float32)
+        if n:
+            self._ctx._check(self._ctx.lib.fcc_plan_tiles(self.h, nnz.ctypes.data, prod.ctypes.data,
+                                                          pred.ctypes.data, n))
+        return nnz, prod, pred
+
+    def execute(self, a_vals, b_vals=None, graph=False) -> "Result":
+        """New values in the entry order given at planning; the result views
+        the plan's output buffers until the next execute."""
+        keep = []
+        pa, da = _vals_arg(a_vals, keep)
+        pb, db = _vals_arg(b_vals, keep)
+        if pb is not None and da != db:
+            raise FastccError(_abi.FCC_ERR_INVALID, "a_vals and b_vals must both be host or both device")
+        raw = _abi.FccResult()
+        rc = self._ctx.lib.fcc_plan_execute(self.h, pa, pb, da, 1 if graph else 0, C.byref(raw))
+        self._ctx._check(rc)
+        return Result(self._ctx, raw)
+
+    def close(self):
+        if getattr(self, "h", None) and self._ctx.h:
+            self._ctx.lib.fcc_plan_destroy(self.h)
+        self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 class Context:
     """One GPU, one HIP stream, a caching device pool and the MLP planner."""
 
@@ -184,6 +249,25 @@ class Context:
             rc = self.lib.fcc_contract(self.h, C.byref(ta), am, C.byref(tb), bm, r, C.byref(o), C.byref(raw))
         self._check(rc)
         return Result(self, raw)
+
+    def plan(self, a, a_modes, b, b_modes, a_batch=(), b_batch=(), **opts) -> Plan:
+        """fcc_plan_create: plans (and performs once) the contraction."""
+        keep = []
+        ta = _as_fcc_tensor(a, keep)
+        tb = ta if b is a else _as_fcc_tensor(b, keep)
+        r, nb = len(a_modes), len(a_batch)
+        if len(b_modes) != r or len(b_batch) != nb:
+            raise FastccError(_abi.FCC_ERR_INVALID, "mode lists differ in length")
+        am = (C.c_int32 * max(r, 1))(*a_modes)
+        bm = (C.c_int32 * max(r, 1))(*b_modes)
+        ab = (C.c_int32 * max(nb, 1))(*a_batch)
+        bb = (C.c_int32 * max(nb, 1))(*b_batch)
+        o = self.options(**opts)
+        h = C.c_void_p()
+        rc = self.lib.fcc_plan_create(self.h, C.byref(ta), am, ab, C.byref(tb), bm, bb, r, nb, C.byref(o),
+                                      C.byref(h))
+        self._check(rc)
+        return Plan(self, h)
 
     def contract_batch(self, pairs, a_modes, b_modes, **opts):
         keep = []

@@ -18,7 +18,8 @@ FCC_MLP_IN, FCC_MLP_HID, FCC_MLP_OUT = 6, 64, 2
 # every symbol include/fcc.h declares
 EXPORTS = (
     "fcc_create", "fcc_destroy", "fcc_last_error", "fcc_stream", "fcc_options_default", "fcc_contract",
-    "fcc_contract_ex",
+    "fcc_contract_ex", "fcc_plan_create", "fcc_plan_info_get", "fcc_plan_tile_count", "fcc_plan_tiles",
+    "fcc_plan_execute", "fcc_plan_destroy",
     "fcc_contract_batch", "fcc_result_to_host", "fcc_result_free", "fcc_mlp_param_count", "fcc_mlp_set_params",
     "fcc_mlp_get_params", "fcc_tile_stats_count", "fcc_tile_stats_copy", "fcc_mlp_train",
     "fcc_mlp_forward_backward", "fcc_density_bucket", "fcc_tune_set", "fcc_tune_get",
@@ -91,6 +92,25 @@ class FccResult(C.Structure):
     ]
 
 
+class FccPlanInfo(C.Structure):
+    _fields_ = [
+        ("out_nnz", C.c_int64),
+        ("products", C.c_uint64),
+        ("left_tiles", C.c_uint64),
+        ("units", C.c_uint64),
+        ("units_dense", C.c_uint64),
+        ("units_lhash", C.c_uint64),
+        ("units_ghash", C.c_uint64),
+        ("row_tile", C.c_int32),
+        ("col_window", C.c_int32),
+        ("order", C.c_int32),
+        ("extents", C.c_uint32 * FCC_MAX_ORDER),
+        ("nnz_lower_bound", C.c_uint64),
+        ("nnz_upper_bound", C.c_uint64),
+        ("ms_plan", C.c_double),
+    ]
+
+
 class FccTuneEntry(C.Structure):
     _fields_ = [
         ("bucket", C.c_int32),
@@ -160,6 +180,20 @@ def load(path: str | None = None):
     lib.fcc_tune_set.restype = C.c_int
     lib.fcc_tune_get.argtypes = [P, C.POINTER(FccTuneEntry), C.c_int32, C.POINTER(C.c_int32)]
     lib.fcc_tune_get.restype = C.c_int
+    lib.fcc_plan_create.argtypes = [P, C.POINTER(FccTensor), C.POINTER(C.c_int32), C.POINTER(C.c_int32),
+                                    C.POINTER(FccTensor), C.POINTER(C.c_int32), C.POINTER(C.c_int32), C.c_int32,
+                                    C.c_int32, C.POINTER(FccOptions), C.POINTER(P)]
+    lib.fcc_plan_create.restype = C.c_int
+    lib.fcc_plan_info_get.argtypes = [P, C.POINTER(FccPlanInfo)]
+    lib.fcc_plan_info_get.restype = C.c_int
+    lib.fcc_plan_tile_count.argtypes = [P]
+    lib.fcc_plan_tile_count.restype = C.c_int64
+    lib.fcc_plan_tiles.argtypes = [P, P, P, P, C.c_int64]
+    lib.fcc_plan_tiles.restype = C.c_int
+    lib.fcc_plan_execute.argtypes = [P, P, P, C.c_int32, C.c_int32, C.POINTER(FccResult)]
+    lib.fcc_plan_execute.restype = C.c_int
+    lib.fcc_plan_destroy.argtypes = [P]
+    lib.fcc_plan_destroy.restype = None
     if path is None:
         _lib = lib
     return lib

@@ -154,6 +154,52 @@ int fcc_contract_batch(fcc_ctx *ctx, int32_t n, const fcc_tensor *a, const int32
 int fcc_result_to_host(fcc_ctx *ctx, const fcc_result *r, uint32_t *coords, double *vals);
 void fcc_result_free(fcc_ctx *ctx, fcc_result *r);
 
+/* ---- plans: one coordinate pattern, many value sets --------------------- */
+/* A plan fixes everything that depends only on the coordinates of a and b:
+ * the right operand's index, the left tiles, each unit's accumulator kind and
+ * output offsets. fcc_plan_execute then reruns only the numeric pass on new
+ * values for the same coordinates, optionally as a captured hipGraph. The
+ * reference redoes all of fastcc_multiply per call (contract.hpp:824-912);
+ * its symbolic queries (SymbolicTensor::output_shape / count_ops,
+ * index_tensors.hpp:287-353; bound_output_nnz_*, contract.hpp:180-281) are
+ * answered by fcc_plan_info_get and fcc_plan_tiles. */
+typedef struct fcc_plan fcc_plan;
+
+typedef struct {
+    int64_t out_nnz;          /* exact output nonzeros */
+    uint64_t products;        /* multiply-adds per execution */
+    uint64_t left_tiles, units, units_dense, units_lhash, units_ghash;
+    int32_t row_tile, col_window;
+    int32_t order;
+    uint32_t extents[FCC_MAX_ORDER];
+    uint64_t nnz_lower_bound; /* sum over tiles of the largest single-key product m*|fiber| */
+    uint64_t nnz_upper_bound; /* sum over tiles of min(products, rows * right span) */
+    double ms_plan;           /* device time of the planning call (one full contraction) */
+} fcc_plan_info;
+
+/* Plans (and performs once) the contraction of fcc_contract_ex. options as
+ * there; the learned planner's first run supplies the exact per-unit counts. */
+int fcc_plan_create(fcc_ctx *ctx, const fcc_tensor *a, const int32_t *a_modes, const int32_t *a_batch,
+                    const fcc_tensor *b, const int32_t *b_modes, const int32_t *b_batch, int32_t n_contr,
+                    int32_t n_batch, const fcc_options *options, fcc_plan **plan);
+int fcc_plan_info_get(const fcc_plan *plan, fcc_plan_info *info);
+/* Per active left tile: exact output nonzeros, products, the MLP's predicted
+ * outputs (-1 when the plan used the exact planner). n = fcc_plan_tile_count. */
+int64_t fcc_plan_tile_count(const fcc_plan *plan);
+int fcc_plan_tiles(fcc_plan *plan, uint64_t *nnz, uint64_t *products, float *predicted, int64_t n);
+/* The numeric pass on new values: a_vals holds a->nnz values and b_vals
+ * b->nnz, in the entry order given to fcc_plan_create (b_vals is ignored when
+ * a and b were the same tensor). on_device: 1 device pointers, 0 host.
+ * use_graph: 1 = the pass (value gather, counter resets, accumulate, drain) is
+ * captured into a hipGraph on the first such call and replayed afterwards.
+ * out is a view of the plan's own output buffers, valid until the next
+ * execute or fcc_plan_destroy (fcc_result_free on it releases nothing).
+ * Executions of one plan must not overlap. */
+int fcc_plan_execute(fcc_plan *plan, const double *a_vals, const double *b_vals, int32_t on_device,
+                     int32_t use_graph, fcc_result *out);
+/* Destroy plans before their context. */
+void fcc_plan_destroy(fcc_plan *plan);
+
 /* ---- learned planner (MLP 6 -> 64 -> 64 -> 2, FP32) -------------------- */
 #define FCC_MLP_IN 6
 #define FCC_MLP_HID 64

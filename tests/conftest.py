@@ -82,6 +82,20 @@ def assert_same_result(gc, gv, rc, rv, rel=1e-12, abs_floor=1e-14, what="", absu
                            f"{np.max(np.abs(gv - rv) / np.maximum(np.abs(rv), 1e-300))}")
 
 
+@pytest.fixture(scope="session", autouse=True)
+def _torch_runtime_first(request):
+    """torch ships its own HIP runtime, which only initialises when it comes
+    first in the process; GPU sessions start it before any library context."""
+    if any(item.get_closest_marker("gpu") for item in request.session.items):
+        try:
+            import torch
+            if torch.cuda.is_available():
+                torch.cuda.init()
+        except ImportError:
+            pass
+    yield
+
+
 @pytest.fixture(scope="session")
 def gpu_ctx():
     from fastcc_amd import Context
