@@ -53,6 +53,10 @@ def main():
     ap.add_argument("--tile-hint", type=int, default=0, help="rows per left tile (0 = planner)")
     ap.add_argument("--col-window", type=int, default=0, help="columns per dense unit (0 = planner)")
     ap.add_argument("--workgroups", type=int, default=0, help="persistent workgroups (0 = planner)")
+    ap.add_argument("--force-acc", default="auto", choices=["auto", "dense", "lhash", "ghash"],
+                    help="force one accumulator kind for every unit (profiling per kind)")
+    ap.add_argument("--autotune", action="store_true",
+                    help="run the autotuner on the workload before timing (untimed); the tuned shape is used")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a device-to-device copy of MIB MiB (HBM counter calibration) and exit")
     args = ap.parse_args()
@@ -96,7 +100,12 @@ def main():
     ctx = Context(0)
     ctx.set_mlp_params(np.load(MLP_PARAMS))
     sink = args.workload == "c3"  # ~3.3e10 outputs do not fit in HBM: count + checksum
-    opts = dict(tile_hint=args.tile_hint, col_window=args.col_window, workgroups=args.workgroups, sink=sink)
+    force = {"auto": 0, "dense": 1, "lhash": 2, "ghash": 3}[args.force_acc]
+    opts = dict(tile_hint=args.tile_hint, col_window=args.col_window, workgroups=args.workgroups, sink=sink,
+                force_acc=force)
+    tuned = None
+    if args.autotune:
+        tuned = ctx.autotune(da, list(wl.a_modes), db, list(wl.b_modes), reps=2, planner=args.planner, sink=sink)
 
     def run(planner):
         def step():
@@ -166,7 +175,8 @@ def main():
         "data": "synthetic (seeded generator, csrc/fcc_gen.c)",
         "config": {"workload": wl.name, "description": wl.description, "planner": args.planner,
                    "output": "count + checksum" if sink else "materialised SoA", "output_nnz": int(nnz),
-                   "products": int(products)},
+                   "products": int(products), "row_tile": stats[-1]["row_tile"],
+                   "col_window": stats[-1]["col_window"], "autotuned": tuned, "force_acc": args.force_acc},
         "hbm_gbs_alg": bytes_alg / dt / 1e9,
         "device_ms": dev_ms,
         "phases_ms": {k: float(np.mean([s[k] for s in stats])) for k in

@@ -142,10 +142,67 @@ __device__ __forceinline__ int lds_probe(uint32_t *keys, uint32_t *cnt, uint32_t
 
 // ------------------------------------------------------------- per unit
 
+struct SinkAcc {
+    uint64_t count = 0, hash = 0;
+    double vsum = 0.0, wsum = 0.0;
+    uint64_t group = 0;  // row group this accumulator belongs to
+};
+
+__device__ __forceinline__ double hash_weight(uint64_t z) { return (double)(z >> 11) * 0x1.0p-53; }
+
+__device__ __forceinline__ void sink_atomic(const AccArgs &A, uint64_t g, unsigned long long c, unsigned long long h,
+                                            double vs, double ws) {
+    atomicAdd(&A.sink[4 * g], c);
+    atomicAdd(&A.sink[4 * g + 1], h);
+    unsafeAtomicAdd(&A.sinkf[4 * g], vs);
+    unsafeAtomicAdd(&A.sinkf[4 * g + 1], ws);
+}
+
+__device__ __forceinline__ void emit_output(const AccArgs &A, uint64_t pos, uint64_t row, uint64_t col, double v,
+                                            bool sink, SinkAcc &sk) {
+    if (sink) {
+        if (A.group_rows) {
+            const uint64_t g = row / A.group_rows;
+            row -= g * A.group_rows;
+            if (g != sk.group) {  // a unit straddling two groups: add this output directly
+                uint64_t z = mix64(row * A.spanB + col);
+                sink_atomic(A, g, 1ull, z, v, v * hash_weight(z));
+                return;
+            }
+        }
+        uint64_t z = mix64(row * A.spanB + col);
+        sk.count++;
+        sk.hash += z;
+        sk.vsum += v;
+        sk.wsum += v * hash_weight(z);
+        return;
+    }
+    if (pos >= A.capacity) {  // a count/fill disagreement: flag it, never write past the output
+        atomicOr(A.fail_flag, 2u);
+        return;
+    }
+    uint32_t r = (uint32_t)row, c = (uint32_t)col;
+    for (int m = A.nxa - 1; m >= 0; m--) {
+        uint32_t E = A.xaE[m];
+        uint32_t q = r / E;
+        A.ocoords[(uint64_t)m * A.ostride + pos] = r - q * E;
+        r = q;
+    }
+    for (int m = A.nxb - 1; m >= 0; m--) {
+        uint32_t E = A.xbE[m];
+        uint32_t q = c / E;
+        A.ocoords[(uint64_t)(A.nxa + m) * A.ostride + pos] = c - q * E;
+        c = q;
+    }
+    A.ovals[pos] = v;
+}
+
+
 struct UnitCtx {
     uint32_t u, a, tile, c0, W, kind, multi;
     uint32_t Wp;  // dense row stride: W rounded up to a multiple of 32 (hash kinds: W)
     uint64_t row0;
+    uint64_t obase;  // DIRECT units: first output position of the unit
 };
 
 template <bool NUM, bool BITMAP>
@@ -155,6 +212,8 @@ struct Acc {
     uint32_t *keys;   // LHASH keys (numeric) or key set (count)
     uint32_t *bits;   // DENSE bitmap (count pass, or numeric BITMAP mode)
     uint32_t *cnt;
+    SinkAcc *sk;      // DIRECT units: count + checksum outputs go here
+    bool sink;
 };
 
 template <bool NUM, bool BITMAP>
@@ -173,6 +232,24 @@ __device__ __forceinline__ void accum(const AccArgs &A, const UnitCtx &U, const 
             uint64_t gk = (U.row0 + li) * A.spanB + (U.c0 + col);
             if (NUM) ghash_add(A, gk, v, U.u);
             else gset_insert(A, gk, U.u);
+        }
+    } else if (U.kind == KIND_DIRECT) {
+        // a distinct output: written straight out, positions reserved per
+        // wavefront from the unit's LDS counter (order within the unit is free)
+        if (NUM) {
+            const uint64_t row = U.row0 + li, c = (uint64_t)U.c0 + col;
+            if (S.sink) {
+                emit_output(A, 0, row, c, v, true, *S.sk);
+            } else {
+                const uint32_t lane = threadIdx.x & 63;
+                const uint64_t act = __ballot(1);
+                const int first = __ffsll((unsigned long long)act) - 1;
+                uint32_t b = 0;
+                if (lane == (uint32_t)first) b = atomicAdd(S.cnt, (uint32_t)__popcll(act));
+                b = __shfl(b, first, 64);
+                const uint64_t lt = lane ? (~0ull >> (64 - lane)) : 0ull;
+                emit_output(A, U.obase + b + (uint32_t)__popcll(act & lt), row, c, v, false, *S.sk);
+            }
         }
     } else {
         uint64_t gk = (U.row0 + li) * A.spanB + (U.c0 + col);
@@ -404,10 +481,18 @@ __device__ __forceinline__ void heavy_group(const AccArgs &A, const UnitCtx &U, 
 template <bool NUM, bool BITMAP>
 __device__ void heavy_claimed(const AccArgs &A, const UnitCtx &U, const Acc<NUM, BITMAP> &S, uint32_t *s_gnext) {
     const uint32_t lane = threadIdx.x & 63;
-    const uint32_t g0 = A.atile_h0[U.a], gl = A.atile_h0[U.a + 1];
+    // A tile with few heavy groups works on fiber segments (long fibers spread
+    // over the block's wavefronts, claims sized so every wavefront gets some);
+    // a tile with many works on whole groups, 64 per claim (fewer descriptors
+    // to clip against column windows).
+    const uint32_t ngh = A.atile_lg[U.a] - A.atile_g0[U.a];
+    const bool seg = ngh < A.seg_below;
+    const uint4 *desc = seg ? A.hdesc : A.gdesc;
+    const uint32_t g0 = seg ? A.atile_h0[U.a] : A.atile_g0[U.a];
+    const uint32_t gl = seg ? A.atile_h0[U.a + 1] : A.atile_lg[U.a];
     const uint32_t cend = U.c0 + U.W;
-    // claim size: enough claims for every wavefront of the block, at most 64
-    const uint32_t claim = max(1u, min(64u, (gl - g0 + 2 * NWAVES - 1) / (2 * NWAVES)));
+    const uint32_t claim =
+        (seg && A.claim_div) ? max(1u, min(64u, (gl - g0 + A.claim_div - 1) / A.claim_div)) : 64u;
     for (;;) {
         uint32_t bs = 0;
         if (lane == 0) bs = atomicAdd(s_gnext, claim);
@@ -416,7 +501,7 @@ __device__ void heavy_claimed(const AccArgs &A, const UnitCtx &U, const Acc<NUM,
         const uint32_t bg = g0 + bs;
         const uint32_t nb = min(claim, gl - bg);
         uint4 d = make_uint4(0, 0, 0, 0);
-        if (lane < nb) d = A.hdesc[bg + lane];
+        if (lane < nb) d = desc[bg + lane];
         uint32_t s_l = d.x, e_l = d.x + d.y;
         if (U.multi && lane < nb && s_l < e_l) {
             // segments wholly outside the window are dropped on their end columns
@@ -620,61 +705,6 @@ __device__ uint32_t block_count(uint32_t n, uint32_t *s_w, Pred pred) {
     return tot;
 }
 
-struct SinkAcc {
-    uint64_t count = 0, hash = 0;
-    double vsum = 0.0, wsum = 0.0;
-    uint64_t group = 0;  // row group this accumulator belongs to
-};
-
-__device__ __forceinline__ double hash_weight(uint64_t z) { return (double)(z >> 11) * 0x1.0p-53; }
-
-__device__ __forceinline__ void sink_atomic(const AccArgs &A, uint64_t g, unsigned long long c, unsigned long long h,
-                                            double vs, double ws) {
-    atomicAdd(&A.sink[4 * g], c);
-    atomicAdd(&A.sink[4 * g + 1], h);
-    unsafeAtomicAdd(&A.sinkf[4 * g], vs);
-    unsafeAtomicAdd(&A.sinkf[4 * g + 1], ws);
-}
-
-__device__ __forceinline__ void emit_output(const AccArgs &A, uint64_t pos, uint64_t row, uint64_t col, double v,
-                                            bool sink, SinkAcc &sk) {
-    if (sink) {
-        if (A.group_rows) {
-            const uint64_t g = row / A.group_rows;
-            row -= g * A.group_rows;
-            if (g != sk.group) {  // a unit straddling two groups: add this output directly
-                uint64_t z = mix64(row * A.spanB + col);
-                sink_atomic(A, g, 1ull, z, v, v * hash_weight(z));
-                return;
-            }
-        }
-        uint64_t z = mix64(row * A.spanB + col);
-        sk.count++;
-        sk.hash += z;
-        sk.vsum += v;
-        sk.wsum += v * hash_weight(z);
-        return;
-    }
-    if (pos >= A.capacity) {  // a count/fill disagreement: flag it, never write past the output
-        atomicOr(A.fail_flag, 2u);
-        return;
-    }
-    uint32_t r = (uint32_t)row, c = (uint32_t)col;
-    for (int m = A.nxa - 1; m >= 0; m--) {
-        uint32_t E = A.xaE[m];
-        uint32_t q = r / E;
-        A.ocoords[(uint64_t)m * A.ostride + pos] = r - q * E;
-        r = q;
-    }
-    for (int m = A.nxb - 1; m >= 0; m--) {
-        uint32_t E = A.xbE[m];
-        uint32_t q = c / E;
-        A.ocoords[(uint64_t)(A.nxa + m) * A.ostride + pos] = c - q * E;
-        c = q;
-    }
-    A.ovals[pos] = v;
-}
-
 __device__ void sink_flush(const AccArgs &A, SinkAcc &sk, unsigned long long *s_u, double *s_d) {
     const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     for (int d = 32; d > 0; d >>= 1) {
@@ -712,12 +742,14 @@ __global__ __launch_bounds__(ACC_THREADS) void k_numeric(AccArgs A) {
     __shared__ unsigned long long s_base;
     __shared__ unsigned long long s_su[2 * NWAVES];
     __shared__ double s_sd[2 * NWAVES];
+    SinkAcc sk;
     Acc<true, BITMAP> S;
     S.fv = s_f64;
     S.keys = reinterpret_cast<uint32_t *>(s_f64 + LH_SLOTS);
     S.bits = s_bits;
     S.cnt = &s_cnt;
-    SinkAcc sk;
+    S.sk = &sk;
+    S.sink = SINK;
     for (;;) {
         if (threadIdx.x == 0) s_unit = atomicAdd(A.work, 1u);
         __syncthreads();
@@ -756,17 +788,35 @@ __global__ __launch_bounds__(ACC_THREADS) void k_numeric(AccArgs A) {
         if (threadIdx.x == 0) {
             s_cnt = 0;
             s_gnext = 0;
+            if (U.kind == KIND_DIRECT && !SINK) {
+                // the unit's outputs are exactly its products (one window)
+                const unsigned long long n = A.st_products[U.a];
+                unsigned long long b;
+                if (MODE == 1) {
+                    b = atomicAdd(A.cursor, n);
+                    if (b + n > A.capacity) {
+                        atomicOr(A.fail_flag, 1u);
+                        b = ~0ull;
+                    }
+                } else {
+                    b = A.unit_off[U.u];
+                }
+                s_base = b;
+                if (b != ~0ull) A.unit_cursor[U.u] = n;
+            }
         }
         __syncthreads();
+        if (U.kind == KIND_DIRECT && !SINK && s_base == ~0ull) continue;  // over capacity: re-run
+        U.obase = s_base;
         // flattening stage: LDS the unit's accumulator leaves free (LHASH uses
         // 4096 values + 4096 keys = 6144 doubles' worth, GHASH none)
         uint32_t *stage = nullptr;
         if (U.kind == KIND_LHASH) stage = reinterpret_cast<uint32_t *>(s_f64 + 6144);
-        else if (U.kind == KIND_GHASH) stage = reinterpret_cast<uint32_t *>(s_f64);
+        else if (U.kind == KIND_GHASH || U.kind == KIND_DIRECT) stage = reinterpret_cast<uint32_t *>(s_f64);
         else if (nslots + ACC_THREADS <= DENSE_SLOTS) stage = reinterpret_cast<uint32_t *>(s_f64 + DENSE_SLOTS - ACC_THREADS);
         process_unit<true, BITMAP>(A, U, S, &s_gnext, stage, s_w);
         __syncthreads();
-        if (U.kind != KIND_GHASH) {
+        if (U.kind == KIND_DENSE || U.kind == KIND_LHASH) {
             auto pred = [&](uint32_t s) -> bool {
                 if (U.kind == KIND_DENSE) {
                     if (BITMAP) return (s_bits[s >> 5] >> (s & 31)) & 1u;
@@ -774,33 +824,64 @@ __global__ __launch_bounds__(ACC_THREADS) void k_numeric(AccArgs A) {
                 }
                 return S.keys[s] < LSEALED;
             };
-            bool skip = false;
-            if (MODE == 1 && !SINK) {
-                uint32_t tot = block_count(nslots, s_w, pred);
+            // Each wavefront owns a contiguous run of slots (whole 64-slot
+            // chunks), so the unit's outputs keep slot order: count per
+            // wavefront, one barrier for the offsets, then emit without
+            // further barriers. Count + checksum outputs need no offsets.
+            const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+            const uint32_t per = ((nslots + NWAVES * 64 - 1) / (NWAVES * 64)) * 64;
+            const uint32_t w0 = min(nslots, wave * per), w1 = min(nslots, w0 + per);
+            auto emit = [&](uint32_t s, uint64_t pos) {
+                const uint32_t key = U.kind == KIND_DENSE ? s : S.keys[s];
+                const uint32_t li = key / U.Wp, cl = key - li * U.Wp;
+                emit_output(A, pos, U.row0 + li, (uint64_t)U.c0 + cl, s_f64[s], SINK, sk);
+            };
+            if (SINK) {
+                for (uint32_t s = w0 + lane; s < w1; s += 64)
+                    if (pred(s)) emit(s, 0);
+            } else {
+                const uint64_t lt = lane ? (~0ull >> (64 - lane)) : 0ull;
+                uint32_t cnt = 0;
+                for (uint32_t b = w0; b < w1; b += 64) {
+                    const bool t = b + lane < w1 && pred(b + lane);
+                    cnt += (uint32_t)__popcll(__ballot(t));
+                }
+                if (lane == 0) s_w[wave] = cnt;
+                __syncthreads();
+                uint32_t pre = 0, tot = 0;
+#pragma unroll
+                for (int w = 0; w < NWAVES; w++) {
+                    const uint32_t c = s_w[w];
+                    pre += (w < (int)wave) ? c : 0;
+                    tot += c;
+                }
                 if (threadIdx.x == 0) {
-                    unsigned long long b = atomicAdd(A.cursor, (unsigned long long)tot);
-                    if (b + tot > A.capacity) {
-                        atomicOr(A.fail_flag, 1u);
-                        b = ~0ull;
+                    unsigned long long b;
+                    if (MODE == 1) {
+                        b = atomicAdd(A.cursor, (unsigned long long)tot);
+                        if (b + tot > A.capacity) {
+                            atomicOr(A.fail_flag, 1u);
+                            b = ~0ull;
+                        }
+                    } else {
+                        b = A.unit_off[U.u];
                     }
                     s_base = b;
+                    if (b != ~0ull) A.unit_cursor[U.u] = tot;
                 }
                 __syncthreads();
-                skip = s_base == ~0ull;
-            } else if (MODE == 0 && !SINK) {
-                if (threadIdx.x == 0) s_base = A.unit_off[U.u];
-                __syncthreads();
-            }
-            if (!skip) {
                 const uint64_t base = s_base;
-                uint32_t emitted = block_compact(nslots, s_w, pred, [&](uint32_t s, uint32_t rank) {
-                    uint32_t key = U.kind == KIND_DENSE ? s : S.keys[s];
-                    uint32_t li = key / U.Wp, cl = key - li * U.Wp;
-                    emit_output(A, base + rank, U.row0 + li, (uint64_t)U.c0 + cl, s_f64[s], SINK, sk);
-                });
-                if (!SINK && threadIdx.x == 0) A.unit_cursor[U.u] = emitted;
+                if (base != ~0ull) {
+                    uint32_t run = pre;
+                    for (uint32_t b = w0; b < w1; b += 64) {
+                        const bool t = b + lane < w1 && pred(b + lane);
+                        const uint64_t bal = __ballot(t);
+                        if (t) emit(b + lane, base + run + (uint32_t)__popcll(bal & lt));
+                        run += (uint32_t)__popcll(bal);
+                    }
+                }
             }
-        } else if (!SINK && threadIdx.x == 0) {
+        } else if (U.kind == KIND_GHASH && !SINK && threadIdx.x == 0) {
             A.unit_cursor[U.u] = 0;
         }
         if (A.group_left && threadIdx.x == 0) {
@@ -823,6 +904,8 @@ __global__ __launch_bounds__(ACC_THREADS) void k_count(AccArgs A) {
     S.keys = s_set;
     S.bits = s_set;
     S.cnt = &s_cnt;
+    S.sk = nullptr;
+    S.sink = false;
     for (;;) {
         if (threadIdx.x == 0) s_unit = atomicAdd(A.work, 1u);
         __syncthreads();
@@ -838,6 +921,12 @@ __global__ __launch_bounds__(ACC_THREADS) void k_count(AccArgs A) {
         U.multi = A.unit_multi[U.u];
         U.Wp = U.kind == KIND_DENSE ? ((U.W + 31) & ~31u) : U.W;
         U.row0 = (uint64_t)U.tile << A.logTL;
+        if (U.kind == KIND_DIRECT) {  // every product is a distinct output
+            if (threadIdx.x == 0) A.unit_count[U.u] = A.st_products[U.a];
+            // every thread has read s_unit before thread 0 claims the next unit
+            __syncthreads();
+            continue;
+        }
         const uint32_t TL = 1u << A.logTL;
         const uint32_t nwords = U.kind == KIND_DENSE ? TL * U.Wp / 32 : 0;
         if (U.kind == KIND_DENSE) {
@@ -956,10 +1045,17 @@ __global__ __launch_bounds__(256) void k_gdrain(AccArgs A) {
 
 // Per active tile: accumulator kind and number of column windows.
 // est = exact planner: the products bound; learned planner: predicted outputs.
-__global__ void k_plan_tiles(const uint64_t *st_products, const uint64_t *st_maxgp, const float *pred,
-                             uint32_t natiles, PlanParams P, uint32_t *t_nwin, uint32_t *t_W, uint8_t *t_kind) {
+__global__ void k_plan_tiles(const uint64_t *st_products, const uint64_t *st_maxgp, const uint8_t *st_direct,
+                             const float *pred, uint32_t natiles, PlanParams P, uint32_t *t_nwin, uint32_t *t_W,
+                             uint8_t *t_kind) {
     uint32_t a = blockIdx.x * blockDim.x + threadIdx.x;
     if (a >= natiles) return;
+    if (!P.force && st_direct && st_direct[a]) {  // distinct outputs: one window, no accumulator
+        t_nwin[a] = 1;
+        t_W[a] = (uint32_t)P.spanB;
+        t_kind[a] = KIND_DIRECT;
+        return;
+    }
     const double prod = (double)st_products[a];
     const double lb = (double)st_maxgp[a];  // one group's products are distinct outputs
     const double S = (double)P.spanB;
@@ -1039,8 +1135,8 @@ void launch_plan_tiles(const Index &ix, const float *pred, const PlanParams &P, 
                        uint8_t *t_kind, hipStream_t s) {
     if (!ix.natiles) return;
     k_plan_tiles<<<(unsigned)div_up(ix.natiles, 256), 256, 0, s>>>((const uint64_t *)ix.st_products,
-                                                                     (const uint64_t *)ix.st_maxgp, pred, ix.natiles,
-                                                                     P, t_nwin, t_W, t_kind);
+                                                                     (const uint64_t *)ix.st_maxgp, ix.st_direct, pred,
+                                                                     ix.natiles, P, t_nwin, t_W, t_kind);
 }
 
 void launch_emit_units(const uint32_t *t_nwin, const uint32_t *t_W, const uint8_t *t_kind, const uint32_t *t_ubase,

@@ -149,7 +149,7 @@ int validate(const fcc_tensor *t, const char *name, std::string &msg) {
 // table, or the caller's hint (the reference's tile_size, rounded up to a
 // power of two like contract.hpp:844-846).
 void choose_tiles(fcc_ctx *ctx, const fcc_options &o, const Geom &g, uint64_t products, uint32_t nvalid, int *logTL,
-                  uint32_t *Wd) {
+                  uint32_t *Wd, int32_t *bucket_out, int32_t *wg_out) {
     double cells = (double)g.spanA * (double)g.spanB;
     double dens = cells > 0 ? (double)products / cells : 0.0;
     double rows = (double)g.spanA * (1.0 - std::exp(-(double)nvalid / std::max(1.0, (double)g.spanA)));
@@ -160,6 +160,8 @@ void choose_tiles(fcc_ctx *ctx, const fcc_options &o, const Geom &g, uint64_t pr
     const fcc_tune_entry *te = nullptr;
     for (auto &e : ctx->tune)
         if (e.bucket == bucket) te = &e;
+    *bucket_out = bucket;
+    *wg_out = (te && o.tile_hint <= 0 && te->workgroups > 0) ? te->workgroups : 0;
     if (o.tile_hint > 0) {
         TL = pow2_ceil((uint64_t)o.tile_hint);
     } else if (te && te->row_tile > 0) {
@@ -229,9 +231,9 @@ __global__ void k_unit_to_tile(const unsigned long long *unit_count, const uint3
 }
 
 __global__ void k_kind_hist(const uint8_t *kind, uint32_t n, unsigned long long *h) {
-    uint32_t c[4] = {0, 0, 0, 0};
-    for (uint32_t u = blockIdx.x * blockDim.x + threadIdx.x; u < n; u += gridDim.x * blockDim.x) c[kind[u] & 3]++;
-    for (int k = 0; k < 4; k++) {
+    uint32_t c[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    for (uint32_t u = blockIdx.x * blockDim.x + threadIdx.x; u < n; u += gridDim.x * blockDim.x) c[kind[u] & 7]++;
+    for (int k = 0; k < 8; k++) {
         uint32_t v = c[k];
         for (int d = 32; d > 0; d >>= 1) v += __shfl_xor(v, d, 64);
         if ((threadIdx.x & 63) == 0 && v) atomicAdd(&h[k], (unsigned long long)v);
@@ -306,12 +308,13 @@ __global__ void k_mape(const float *pred, const unsigned long long *actual, uint
 }
 
 // Sum of predicted tile outputs, each capped by its exact bounds (products, area).
-__global__ void k_pred_sum(const float *pred, const uint64_t *st_products, const uint64_t *st_maxgp, double area,
-                           uint32_t n, unsigned long long *sum) {
+__global__ void k_pred_sum(const float *pred, const uint64_t *st_products, const uint64_t *st_maxgp,
+                           const uint8_t *st_direct, double area, uint32_t n, unsigned long long *sum) {
     uint32_t a = blockIdx.x * blockDim.x + threadIdx.x;
     unsigned long long v = 0ull;
     if (a < n) {
-        double p = ceil((double)pred[a]);
+        // DIRECT tiles: exactly their products
+        double p = st_direct[a] ? (double)st_products[a] : ceil((double)pred[a]);
         p = p < (double)st_maxgp[a] ? (double)st_maxgp[a] : p;
         p = p > (double)st_products[a] ? (double)st_products[a] : p;
         p = p > area ? area : p;
@@ -359,6 +362,8 @@ struct Contraction {
     // fused batch: the leading batch coordinate is a group index in [0, groups)
     // with extent groups; summaries and completion times are kept per group
     uint32_t groups = 0;
+    int32_t bucket = 0;    // density bucket of the last run (autotuner key)
+    int32_t tuned_wg = 0;  // workgroups from the tuner table, 0 = default
     std::vector<unsigned long long> group_sink;  // 4 per group: count, hash, val_sum, val_wsum bits
     std::vector<double> group_ms;                // per group: call start -> output complete
     // plans: keep what the numeric pass needs after run(), keep the sort
@@ -510,8 +515,9 @@ int Contraction::run(const fcc_tensor *ta, const int32_t *am, const fcc_tensor *
     st.products = products;
     int logTL;
     uint32_t Wd;
-    choose_tiles(ctx, o, g, products, nvalid, &logTL, &Wd);
+    choose_tiles(ctx, o, g, products, nvalid, &logTL, &Wd, &bucket, &tuned_wg);
     if (const char *hm = getenv("FCC_HEAVY_MIN")) ix.heavy_min = (uint32_t)std::max(1, atoi(hm));
+    if (const char *hs = getenv("FCC_HSEG")) ix.hseg = (uint32_t)std::max(64, atoi(hs));
     if (self && o.left_tile_stride <= 1) build_left_self(pool, g, A, sj, logTL, ix, hold, s);
     else build_left(pool, g, A, rowA, fidA, nvalid, logTL, ix, hold, s);
     bfib.reset();
@@ -587,8 +593,13 @@ int Contraction::run(const fcc_tensor *ta, const int32_t *am, const fcc_tensor *
     X.atile_id = ix.atile_id;
     X.atile_g0 = ix.atile_g0;
     X.atile_lg = ix.atile_lg;
+    X.st_products = (const uint64_t *)ix.st_products;
     X.unit_order = nullptr;
     X.row_own = getenv("FCC_ROW_OWN") ? 1 : 0;
+    X.claim_div = 2 * (ACC_THREADS / 64);
+    if (const char *cd = getenv("FCC_CLAIM_DIV")) X.claim_div = (uint32_t)std::max(0, atoi(cd));
+    X.seg_below = 2 * (ACC_THREADS / 64);
+    if (const char *sb = getenv("FCC_SEG_BELOW")) X.seg_below = (uint32_t)std::max(0, atoi(sb));
     X.logTL = ix.logTL;
     X.spanB = g.spanB;
     X.nxa = g.nxa;
@@ -660,7 +671,7 @@ int Contraction::run(const fcc_tensor *ta, const int32_t *am, const fcc_tensor *
         X.unit_count = unit_count;
         X.unit_off = unit_off;
         X.unit_cursor = unit_cur;
-        grid = (unsigned)(o.workgroups > 0 ? o.workgroups : ctx->num_cus * 2);
+        grid = (unsigned)(o.workgroups > 0 ? o.workgroups : (tuned_wg > 0 ? tuned_wg : ctx->num_cus * 2));
         grid = std::max(1u, std::min<unsigned>(grid, U.n));
         st.units = U.n;
     };
@@ -698,7 +709,7 @@ int Contraction::run(const fcc_tensor *ta, const int32_t *am, const fcc_tensor *
         Tmp<unsigned long long> psum(pool, 1);
         fill_u64(psum, 1, 0, s);
         k_pred_sum<<<grid256(T), 256, 0, s>>>(pred_t, (const uint64_t *)ix.st_products,
-                                               (const uint64_t *)ix.st_maxgp, (double)ix.TL * (double)g.spanB, T,
+                                               (const uint64_t *)ix.st_maxgp, ix.st_direct, (double)ix.TL * (double)g.spanB, T,
                                                psum);
         uint64_t predicted = d2h(psum.get(), s);
         // never more than the products bound: no tile can emit more outputs than products
@@ -878,15 +889,16 @@ int Contraction::run(const fcc_tensor *ta, const int32_t *am, const fcc_tensor *
     }
     // stats
     {
-        Tmp<unsigned long long> kh(pool, 4);
-        fill_u64(kh, 4, 0, s);
+        Tmp<unsigned long long> kh(pool, 8);
+        fill_u64(kh, 8, 0, s);
         k_kind_hist<<<std::min(grid256(U.n), 1024u), 256, 0, s>>>(U.kind, U.n, kh);
-        unsigned long long h[4];
-        FCC_HIP(hipMemcpyAsync(h, kh.get(), 32, hipMemcpyDeviceToHost, s));
+        unsigned long long h[8];
+        FCC_HIP(hipMemcpyAsync(h, kh.get(), 64, hipMemcpyDeviceToHost, s));
         FCC_HIP(hipStreamSynchronize(s));
         st.units_dense = h[KIND_DENSE];
         st.units_lhash = h[KIND_LHASH];
         st.units_ghash = h[KIND_GHASH];
+        st.units_direct = h[KIND_DIRECT];
     }
     st.ms_index = ev_ms(ev[0], ev[1]);
     st.ms_plan = ev_ms(ev[1], ev[2]);
@@ -1446,6 +1458,102 @@ int fcc_contract_batch(fcc_ctx *ctx, int32_t n, const fcc_tensor *a, const int32
     }
     for (int32_t k = 0; k < n; k++) fcc_result_free(ctx, &outs[k]);
     return rc;
+}
+
+// ------------------------------------------------------------- autotuner
+
+int fcc_autotune(fcc_ctx *ctx, const fcc_tensor *a, const int32_t *a_modes, const fcc_tensor *b,
+                 const int32_t *b_modes, int32_t n_contr, const fcc_options *options, int32_t reps,
+                 fcc_tune_entry *best) {
+    if (!ctx) return FCC_ERR_INVALID;
+    if (!best) return fail(ctx, FCC_ERR_INVALID, "best is NULL");
+    std::string msg;
+    int rc = validate(a, "a", msg);
+    if (!rc) rc = validate(b, "b", msg);
+    std::vector<int32_t> ka, kb;
+    if (!rc) rc = check_modes(a, b, a_modes, b_modes, n_contr, nullptr, nullptr, 0, ka, kb, msg);
+    if (rc) return fail(ctx, rc, msg);
+    fcc_options base;
+    fcc_options_default(&base);
+    if (options) base = *options;
+    if (base.left_tile_stride < 1) base.left_tile_stride = 1;
+    reps = std::max(1, reps);
+    // time one candidate: the fastest of reps full contractions (ms_total)
+    auto trial = [&](int32_t tl, int32_t w, int32_t wg, int32_t *bucket, int32_t *tl_used, int32_t *w_used,
+                     uint64_t *dense_units) -> double {
+        double bestms = 1e300;
+        for (int i = 0; i < reps; i++) {
+            Contraction c;
+            c.ctx = ctx;
+            c.s = ctx->stream;
+            c.o = base;
+            c.o.tile_hint = tl;
+            c.o.col_window = w;
+            c.o.workgroups = wg;
+            fcc_result r;
+            memset(&r, 0, sizeof(r));
+            c.out = &r;
+            c.run(a, ka.data(), b, kb.data(), n_contr, 0);
+            bestms = std::min(bestms, r.stats.ms_total);
+            *bucket = c.bucket;
+            *tl_used = r.stats.row_tile;
+            *w_used = r.stats.col_window;
+            *dense_units = r.stats.units_dense;
+            fcc_result_free(ctx, &r);
+        }
+        return bestms;
+    };
+    try {
+        FCC_HIP(hipSetDevice(ctx->device));
+        fcc_tune_entry win{0, 0, 0, 0, 1e300};
+        uint64_t win_dense = 0;
+        int32_t bucket = 0;
+        // rows per tile: every power of two the dense tile can hold a column of
+        for (int32_t tl = 1; tl <= 1024; tl *= 2) {
+            int32_t tu, wu;
+            uint64_t du;
+            double ms = trial(tl, 0, 0, &bucket, &tu, &wu, &du);
+            if (ms < win.ms) {
+                win = fcc_tune_entry{bucket, tu, 0, 0, ms};
+                win_dense = du;
+            }
+        }
+        // dense windows narrower than the widest that fits, then the grid size
+        if (win_dense) {
+            const int32_t wmax = (int32_t)(DENSE_SLOTS / std::max(1, win.row_tile)) & ~31;
+            for (int32_t w : {wmax / 2, wmax / 4}) {
+                if (w < 32) continue;
+                int32_t tu, wu;
+                uint64_t du;
+                double ms = trial(win.row_tile, w & ~31, 0, &bucket, &tu, &wu, &du);
+                if (ms < win.ms) win = fcc_tune_entry{bucket, win.row_tile, wu, 0, ms};
+            }
+        }
+        for (int32_t f : {1, 3, 4}) {
+            int32_t tu, wu;
+            uint64_t du;
+            double ms = trial(win.row_tile, win.col_window, f * ctx->num_cus, &bucket, &tu, &wu, &du);
+            if (ms < win.ms) win = fcc_tune_entry{bucket, win.row_tile, win.col_window, f * ctx->num_cus, ms};
+        }
+        win.bucket = bucket;
+        bool found = false;
+        for (auto &e : ctx->tune)
+            if (e.bucket == win.bucket) {
+                e = win;
+                found = true;
+            }
+        if (!found) ctx->tune.push_back(win);
+        *best = win;
+        ctx->err.clear();
+        return FCC_OK;
+    } catch (HipError &e) {
+        (void)hipGetLastError();
+        return fail(ctx, FCC_ERR_HIP, e.where + ": " + hipGetErrorString(e.code));
+    } catch (LibError &e) {
+        return fail(ctx, e.code, e.msg);
+    } catch (std::bad_alloc &) {
+        return fail(ctx, FCC_ERR_OOM, "host allocation failed");
+    }
 }
 
 // ---------------------------------------------------------------- plans

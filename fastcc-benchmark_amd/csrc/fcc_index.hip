@@ -256,27 +256,52 @@ __global__ void k_gdesc(const uint32_t *group_ptr, const uint32_t *group_fiber, 
 
 // segments per group: heavy groups are cut every HSEG fiber entries, light
 // groups have none
-__global__ void k_hseg_count(const uint4 *gdesc, const uint8_t *glight, uint32_t ngroups, uint32_t *nseg) {
+__global__ void k_hseg_count(const uint4 *gdesc, const uint8_t *glight, uint32_t ngroups, uint32_t hseg,
+                             uint32_t *nseg) {
     uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
     if (g >= ngroups) return;
-    nseg[g] = glight[g] ? 0u : (gdesc[g].y + HSEG - 1) / HSEG;
+    nseg[g] = glight[g] ? 0u : (gdesc[g].y + hseg - 1) / hseg;
 }
 
 __global__ void k_hseg_emit(const uint4 *gdesc, const uint32_t *off, const uint32_t *nseg, uint32_t ngroups,
-                            uint4 *hdesc) {
+                            uint32_t hseg, uint4 *hdesc) {
     uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
     if (g >= ngroups) return;
     const uint4 d = gdesc[g];
     const uint32_t n = nseg[g], o = off[g];
     for (uint32_t k = 0; k < n; k++) {
-        const uint32_t s0 = k * HSEG;
-        hdesc[o + k] = make_uint4(d.x + s0, min(HSEG, d.y - s0), d.z, d.w);
+        const uint32_t s0 = k * hseg;
+        hdesc[o + k] = make_uint4(d.x + s0, min(hseg, d.y - s0), d.z, d.w);
     }
 }
 
 __global__ void k_tile_hseg(const uint32_t *atile_g0, const uint32_t *off, uint32_t natiles, uint32_t *atile_h0) {
     uint32_t a = blockIdx.x * blockDim.x + threadIdx.x;
     if (a <= natiles) atile_h0[a] = off[atile_g0[a]];
+}
+
+// A tile is DIRECT-able when no two of its entries share a row (then no two
+// products meet at one output). Checked for tiles of at most 64 rows holding
+// at most one entry per row.
+__global__ void k_tile_direct(const uint32_t *atile_g0, const uint32_t *group_ptr, const uint32_t *Ali,
+                              const uint32_t *st_nnz, uint32_t natiles, uint32_t TL, uint8_t *direct) {
+    uint32_t a = blockIdx.x * blockDim.x + threadIdx.x;
+    if (a >= natiles) return;
+    uint8_t ok = 0;
+    if (TL <= DIRECT_MAX_TL && st_nnz[a] <= TL) {
+        const uint32_t e0 = group_ptr[atile_g0[a]], e1 = group_ptr[atile_g0[a + 1]];
+        uint64_t seen = 0;
+        ok = 1;
+        for (uint32_t e = e0; e < e1; e++) {
+            const uint64_t bit = 1ull << Ali[e];
+            if (seen & bit) {
+                ok = 0;
+                break;
+            }
+            seen |= bit;
+        }
+    }
+    direct[a] = ok;
 }
 
 __global__ void k_tile_light_init(const uint32_t *atile_g0, uint32_t natiles, uint32_t *atile_lg) {
@@ -618,14 +643,15 @@ static void left_from_sorted(DevPool &pool, const Operand &A, const uint64_t *ks
     // heavy segments
     {
         Tmp<uint32_t> nseg(pool, (size_t)ix.ngroups + 1), off(pool, (size_t)ix.ngroups + 1), nh(pool, 1);
-        if (ix.ngroups) k_hseg_count<<<grid_for(ix.ngroups), 256, 0, s>>>(ix.gdesc, glight, ix.ngroups, nseg);
+        if (ix.ngroups) k_hseg_count<<<grid_for(ix.ngroups), 256, 0, s>>>(ix.gdesc, glight, ix.ngroups, ix.hseg, nseg);
         fill_u32(nseg.get() + ix.ngroups, 1, 0, s);
         scan_u32(pool, nseg, off, (size_t)ix.ngroups + 1, nh, s);
         FCC_HIP(hipMemcpyAsync(&ix.nhseg, nh.get(), 4, hipMemcpyDeviceToHost, s));
         FCC_HIP(hipStreamSynchronize(s));
         ix.hdesc = (uint4 *)keep((size_t)ix.nhseg * 16);
         ix.atile_h0 = (uint32_t *)keep(((size_t)ix.natiles + 1) * 4);
-        if (ix.ngroups) k_hseg_emit<<<grid_for(ix.ngroups), 256, 0, s>>>(ix.gdesc, off, nseg, ix.ngroups, ix.hdesc);
+        if (ix.ngroups) k_hseg_emit<<<grid_for(ix.ngroups), 256, 0, s>>>(ix.gdesc, off, nseg, ix.ngroups, ix.hseg,
+                                                            ix.hdesc);
         k_tile_hseg<<<grid_for((size_t)ix.natiles + 1), 256, 0, s>>>(ix.atile_g0, off, ix.natiles, ix.atile_h0);
     }
     // statistics
@@ -649,6 +675,10 @@ static void left_from_sorted(DevPool &pool, const Operand &A, const uint64_t *ks
                                                           (unsigned long long *)ix.st_products, ix.st_sketch,
                                                           (unsigned long long *)ix.st_maxgp);
     }
+    ix.st_direct = (uint8_t *)keep(T + 1);
+    if (T)
+        k_tile_direct<<<grid_for(T), 256, 0, s>>>(ix.atile_g0, ix.group_ptr, ix.Ali, ix.st_nnz, (uint32_t)T, ix.TL,
+                                                   ix.st_direct);
 }
 
 void build_left(DevPool &pool, const Geom &g, const Operand &A, const uint32_t *rowA, const uint32_t *fidA,

@@ -111,7 +111,10 @@ struct Operand {
 };
 
 // Accumulator kinds (match FCC_ACC_*).
-enum : uint8_t { KIND_DENSE = 1, KIND_LHASH = 2, KIND_GHASH = 3 };
+// DIRECT: a tile whose entries all sit on distinct rows; every product is
+// then a distinct output and is written straight out, no accumulator.
+enum : uint8_t { KIND_DENSE = 1, KIND_LHASH = 2, KIND_GHASH = 3, KIND_DIRECT = 4 };
+constexpr uint32_t DIRECT_MAX_TL = 64;  // rows per tile up to which DIRECT tiles are detected
 
 constexpr uint32_t HEAVY_MIN = 32;      // default: fibers at least this long are processed wave-per-group
 constexpr uint32_t DENSE_SLOTS = 9984;  // f64 LDS accumulator slots (78 KB: two workgroups per CU)
@@ -149,6 +152,7 @@ struct Index {
     // fibers spread over many wavefronts; tile a's segments are
     // [atile_h0[a], atile_h0[a + 1])
     uint32_t nhseg = 0;
+    uint32_t hseg = HSEG;             // fiber entries per segment
     uint4 *hdesc = nullptr;           // per segment: fiber start, length, first entry, entries
     uint32_t *atile_h0 = nullptr;     // natiles+1
     // per active tile statistics (planner features)
@@ -159,6 +163,7 @@ struct Index {
     uint32_t *st_rows = nullptr;      // distinct rows in tile
     uint32_t *st_sketch = nullptr;    // distinct-column sketch (KMV estimate)
     uint64_t *st_maxgp = nullptr;     // largest single-group product m * |fiber| (lower bound on outputs)
+    uint8_t *st_direct = nullptr;     // 1: the tile's entries sit on distinct rows (DIRECT-able)
     int logTL = 0;
     uint32_t TL = 1;
     uint64_t products = 0;
@@ -196,6 +201,7 @@ struct AccArgs {
     const uint32_t *atile_id;
     const uint32_t *atile_g0;
     const uint32_t *atile_lg;
+    const uint64_t *st_products;  // per active tile (DIRECT units: their exact output count)
     uint32_t nunits;
     const uint32_t *unit_tile, *unit_c0, *unit_W;
     const uint8_t *unit_kind, *unit_multi;
@@ -235,6 +241,8 @@ struct AccArgs {
     unsigned long long *clock_start;     // wall clock when the first unit started
     unsigned int *work;        // persistent work counter
     int row_own;               // dense units: row-owned plain updates (experimental)
+    uint32_t claim_div;        // heavy claims: segments / claim_div per claim (0: 64 per claim)
+    uint32_t seg_below;        // tiles with fewer heavy groups than this work on segments
 };
 
 }  // namespace fcc

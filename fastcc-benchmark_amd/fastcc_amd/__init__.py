@@ -328,6 +328,23 @@ class Context:
                                                       C.byref(loss), grad.ctypes.data))
         return pred, float(loss.value), grad
 
+    # ---- autotuner
+    def autotune(self, a, a_modes, b, b_modes, reps=2, **opts) -> dict:
+        """fcc_autotune: time tile shapes on this contraction and record the
+        fastest for its density bucket in the context's table."""
+        keep = []
+        ta = _as_fcc_tensor(a, keep)
+        tb = ta if b is a else _as_fcc_tensor(b, keep)
+        r = len(a_modes)
+        am = (C.c_int32 * max(r, 1))(*a_modes)
+        bm = (C.c_int32 * max(r, 1))(*b_modes)
+        o = self.options(**opts)
+        e = _abi.FccTuneEntry()
+        self._check(self.lib.fcc_autotune(self.h, C.byref(ta), am, C.byref(tb), bm, r, C.byref(o), int(reps),
+                                          C.byref(e)))
+        return {"bucket": e.bucket, "row_tile": e.row_tile, "col_window": e.col_window,
+                "workgroups": e.workgroups, "ms": e.ms}
+
     # ---- autotuner table
     def set_tune(self, entries):
         arr = (_abi.FccTuneEntry * max(len(entries), 1))()

@@ -83,6 +83,8 @@ typedef struct {
     double ms_done;       /* time from the call's start until this contraction's output was
                              complete: ms_total for fcc_contract; per pair for fcc_contract_batch
                              (device clock at the pair's last accumulate unit, plus the drain) */
+    uint64_t units_direct; /* units of tiles whose entries sit on distinct rows: products written
+                              straight out, no accumulator */
 } fcc_stats;
 
 typedef struct {
@@ -237,6 +239,17 @@ typedef struct {
 int32_t fcc_density_bucket(double products_per_output_cell);
 int fcc_tune_set(fcc_ctx *ctx, const fcc_tune_entry *entries, int32_t n);
 int fcc_tune_get(fcc_ctx *ctx, fcc_tune_entry *entries, int32_t cap, int32_t *n);
+/* The autotuner: times this contraction (the fastest of `reps` runs each)
+ * over rows per tile 1, 2, 4, ..., 1024, then narrower dense windows and
+ * 1, 3, 4 workgroups per CU for the winner, and records the fastest in the
+ * context's table under the contraction's density bucket (replacing an
+ * entry of that bucket); *best receives it. Calls that leave tile_hint,
+ * col_window and workgroups at 0 then use the entry. No reference
+ * counterpart: the reference's scripts sweep tile_size by hand
+ * (scripts/*.sh). */
+int fcc_autotune(fcc_ctx *ctx, const fcc_tensor *a, const int32_t *a_modes, const fcc_tensor *b,
+                 const int32_t *b_modes, int32_t n_contr, const fcc_options *options, int32_t reps,
+                 fcc_tune_entry *best);
 
 #ifdef __cplusplus
 }

@@ -171,3 +171,31 @@ def test_readme_example_binary(tmp_path):
     ref = oracle.contract_bruteforce(a, (1, 3), b, (1, 2))
     assert nnz == ref.nnz
     assert s == pytest.approx(float(np.sum(ref.vals)), rel=1e-9, abs=1e-9)
+
+
+@pytest.mark.parametrize("planner", ["exact", "learned"])
+@pytest.mark.parametrize("sink", [False, True])
+def test_direct_tiles(gpu_ctx, planner, sink):
+    """Tiles whose entries sit on distinct rows write every product straight
+    out (no accumulator). A sparse left operand (most rows one entry) mixes such
+    tiles with accumulated ones; the result must not change."""
+    import os
+    from conftest import ROOT
+    print(f"planner={planner} sink={sink}", flush=True)
+    gpu_ctx.set_mlp_params(np.load(os.path.join(ROOT, "fastcc-benchmark_amd", "fastcc_amd", "mlp_params.npy")))
+    rng = np.random.default_rng(5150)
+    a = rand_tensor(rng, (400, 300, 20), 9000, "uniform")   # rows (0, 1): 120k, ~0.07 entries per row
+    b = rand_tensor(rng, (20, 500), 4000, "uniform")
+    for th in (1, 4, 64):
+        r = gpu_ctx.contract(a, [2], b, [0], planner=planner, sink=sink, tile_hint=th)
+        assert r.stats["units_direct"] > 0, th
+        if sink:
+            ref = oracle.contract_tiled(a, (2,), b, (0,), tile_size=64, sink=True)
+            assert r.nnz == ref.nnz and r.coord_hash == ref.coord_hash
+            assert abs(r.val_sum - ref.val_sum) <= 1e-9 * max(1.0, abs(ref.val_sum))
+        else:
+            ref = oracle.contract_bruteforce(a, (2,), b, (0,))
+            c, v = r.to_host()
+            assert_same_result(c, v, ref.coords, ref.vals, rel=REL, abs_floor=ABS, what=f"direct tile_hint {th}",
+                               absum=ref.absum, nterms=ref.nterms)
+        r.free()
