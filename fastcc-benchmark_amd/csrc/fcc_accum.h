@@ -11,6 +11,8 @@ struct PlanParams {
     uint32_t Wd;         // dense column window when TL*spanB exceeds the LDS tile
     float margin;        // learned planner: relative safety margin on predictions
     float dense_ratio;   // dense when estimated outputs >= dense_ratio * TL * spanB
+    uint64_t run;        // products per unit of consecutive DIRECT tiles (0: one tile per unit)
+    uint64_t group_rows; // DIRECT runs break at multiples of this row (0: no groups)
 };
 
 void launch_plan_tiles(const Index &ix, const float *pred, const PlanParams &P, uint32_t *t_nwin, uint32_t *t_W,

@@ -216,6 +216,26 @@ struct Acc {
     bool sink;
 };
 
+// A distinct output written straight out: count + checksum, or a position
+// reserved per wavefront from the unit's LDS counter (order within the unit
+// is free).
+template <bool NUM, bool BITMAP>
+__device__ __forceinline__ void direct_emit(const AccArgs &A, const UnitCtx &U, const Acc<NUM, BITMAP> &S,
+                                            uint64_t row, uint64_t col, double v) {
+    if (S.sink) {
+        emit_output(A, 0, row, col, v, true, *S.sk);
+        return;
+    }
+    const uint32_t lane = threadIdx.x & 63;
+    const uint64_t act = __ballot(1);
+    const int first = __ffsll((unsigned long long)act) - 1;
+    uint32_t b = 0;
+    if (lane == (uint32_t)first) b = atomicAdd(S.cnt, (uint32_t)__popcll(act));
+    b = __shfl(b, first, 64);
+    const uint64_t lt = lane ? (~0ull >> (64 - lane)) : 0ull;
+    emit_output(A, U.obase + b + (uint32_t)__popcll(act & lt), row, col, v, false, *S.sk);
+}
+
 template <bool NUM, bool BITMAP>
 __device__ __forceinline__ void accum(const AccArgs &A, const UnitCtx &U, const Acc<NUM, BITMAP> &S, uint32_t li,
                                       uint32_t col, double v) {
@@ -234,23 +254,7 @@ __device__ __forceinline__ void accum(const AccArgs &A, const UnitCtx &U, const 
             else gset_insert(A, gk, U.u);
         }
     } else if (U.kind == KIND_DIRECT) {
-        // a distinct output: written straight out, positions reserved per
-        // wavefront from the unit's LDS counter (order within the unit is free)
-        if (NUM) {
-            const uint64_t row = U.row0 + li, c = (uint64_t)U.c0 + col;
-            if (S.sink) {
-                emit_output(A, 0, row, c, v, true, *S.sk);
-            } else {
-                const uint32_t lane = threadIdx.x & 63;
-                const uint64_t act = __ballot(1);
-                const int first = __ffsll((unsigned long long)act) - 1;
-                uint32_t b = 0;
-                if (lane == (uint32_t)first) b = atomicAdd(S.cnt, (uint32_t)__popcll(act));
-                b = __shfl(b, first, 64);
-                const uint64_t lt = lane ? (~0ull >> (64 - lane)) : 0ull;
-                emit_output(A, U.obase + b + (uint32_t)__popcll(act & lt), row, c, v, false, *S.sk);
-            }
-        }
+        if (NUM) direct_emit<NUM, BITMAP>(A, U, S, U.row0 + li, (uint64_t)U.c0 + col, v);
     } else {
         uint64_t gk = (U.row0 + li) * A.spanB + (U.c0 + col);
         if (NUM) ghash_add(A, gk, v, U.u);
@@ -649,6 +653,56 @@ __device__ void light_entries(const AccArgs &A, const UnitCtx &U, const Acc<NUM,
 
 static_assert(LH_SLOTS + LH_SLOTS / 2 + ACC_THREADS <= DENSE_SLOTS, "LHASH flattening stage must fit in the LDS tile");
 
+// A run of DIRECT tiles: left entries [e0, e1) (whole tiles, one window),
+// every product a distinct output. Products are flattened over the block like
+// light entries (scan of fiber lengths staged in LDS, owner by binary search).
+template <bool BITMAP>
+__device__ void direct_run(const AccArgs &A, const UnitCtx &U, const Acc<true, BITMAP> &S, uint32_t e0, uint32_t e1,
+                           uint32_t *stage, uint32_t *s_w) {
+    const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    uint32_t *s_inc = stage, *s_s0 = stage + ACC_THREADS;
+    for (uint32_t eb = e0; eb < e1; eb += ACC_THREADS) {
+        const uint32_t e = eb + threadIdx.x;
+        uint32_t s0 = 0, len = 0;
+        if (e < e1) {
+            const uint2 fp = A.Afp[e];
+            s0 = fp.x;
+            len = fp.y - fp.x;
+        }
+        uint32_t inc = len;
+        for (int d = 1; d < 64; d <<= 1) {
+            uint32_t t = __shfl_up(inc, d, 64);
+            if (lane >= (uint32_t)d) inc += t;
+        }
+        if (lane == 63) s_w[wave] = inc;
+        __syncthreads();
+        uint32_t pre = 0, P = 0;
+#pragma unroll
+        for (int w = 0; w < NWAVES; w++) {
+            const uint32_t c = s_w[w];
+            pre += (w < (int)wave) ? c : 0;
+            P += c;
+        }
+        s_inc[threadIdx.x] = inc + pre;
+        s_s0[threadIdx.x] = s0;
+        __syncthreads();
+        const uint32_t nent = min(e1 - eb, (uint32_t)ACC_THREADS);
+        for (uint32_t q = threadIdx.x; q < P; q += ACC_THREADS) {
+            uint32_t lo = 0, hi = nent - 1;  // s_inc[nent - 1] == P > q
+            while (lo < hi) {
+                const uint32_t mid = (lo + hi) >> 1;
+                if (s_inc[mid] > q) hi = mid;
+                else lo = mid + 1;
+            }
+            const uint32_t exc = lo ? s_inc[lo - 1] : 0u;
+            const uint32_t p = s_s0[lo] + (q - exc);
+            const uint32_t ee = eb + lo;
+            direct_emit<true, BITMAP>(A, U, S, A.Arow[ee], A.Bcol[p], A.Aval[ee] * A.Bval[p]);
+        }
+        __syncthreads();  // s_inc / s_s0 / s_w are rewritten by the next chunk
+    }
+}
+
 // One unit: heavy groups (a wavefront per group), then the light entries.
 template <bool NUM, bool BITMAP>
 __device__ void process_unit(const AccArgs &A, const UnitCtx &U, const Acc<NUM, BITMAP> &S, uint32_t *s_gnext,
@@ -789,8 +843,9 @@ __global__ __launch_bounds__(ACC_THREADS) void k_numeric(AccArgs A) {
             s_cnt = 0;
             s_gnext = 0;
             if (U.kind == KIND_DIRECT && !SINK) {
-                // the unit's outputs are exactly its products (one window)
-                const unsigned long long n = A.st_products[U.a];
+                // the unit's outputs are exactly the products of its run of tiles
+                const uint32_t t1 = U.u + 1 < A.nunits ? A.unit_tile[U.u + 1] : A.natiles;
+                const unsigned long long n = A.st_pprefix[t1] - A.st_pprefix[U.a];
                 unsigned long long b;
                 if (MODE == 1) {
                     b = atomicAdd(A.cursor, n);
@@ -814,7 +869,12 @@ __global__ __launch_bounds__(ACC_THREADS) void k_numeric(AccArgs A) {
         if (U.kind == KIND_LHASH) stage = reinterpret_cast<uint32_t *>(s_f64 + 6144);
         else if (U.kind == KIND_GHASH || U.kind == KIND_DIRECT) stage = reinterpret_cast<uint32_t *>(s_f64);
         else if (nslots + ACC_THREADS <= DENSE_SLOTS) stage = reinterpret_cast<uint32_t *>(s_f64 + DENSE_SLOTS - ACC_THREADS);
-        process_unit<true, BITMAP>(A, U, S, &s_gnext, stage, s_w);
+        if (U.kind == KIND_DIRECT) {
+            const uint32_t t1 = U.u + 1 < A.nunits ? A.unit_tile[U.u + 1] : A.natiles;
+            direct_run<BITMAP>(A, U, S, A.group_ptr[A.atile_g0[U.a]], A.group_ptr[A.atile_g0[t1]], stage, s_w);
+        } else {
+            process_unit<true, BITMAP>(A, U, S, &s_gnext, stage, s_w);
+        }
         __syncthreads();
         if (U.kind == KIND_DENSE || U.kind == KIND_LHASH) {
             auto pred = [&](uint32_t s) -> bool {
@@ -922,7 +982,10 @@ __global__ __launch_bounds__(ACC_THREADS) void k_count(AccArgs A) {
         U.Wp = U.kind == KIND_DENSE ? ((U.W + 31) & ~31u) : U.W;
         U.row0 = (uint64_t)U.tile << A.logTL;
         if (U.kind == KIND_DIRECT) {  // every product is a distinct output
-            if (threadIdx.x == 0) A.unit_count[U.u] = A.st_products[U.a];
+            if (threadIdx.x == 0) {
+                const uint32_t t1 = U.u + 1 < A.nunits ? A.unit_tile[U.u + 1] : A.natiles;
+                A.unit_count[U.u] = A.st_pprefix[t1] - A.st_pprefix[U.a];
+            }
             // every thread has read s_unit before thread 0 claims the next unit
             __syncthreads();
             continue;
@@ -1050,7 +1113,9 @@ __global__ void k_plan_tiles(const uint64_t *st_products, const uint64_t *st_max
                              uint8_t *t_kind) {
     uint32_t a = blockIdx.x * blockDim.x + threadIdx.x;
     if (a >= natiles) return;
-    if (!P.force && st_direct && st_direct[a]) {  // distinct outputs: one window, no accumulator
+    // distinct outputs: one window, no accumulator (the unit's LDS output
+    // counter is 32-bit, so a DIRECT tile holds fewer than 2^31 products)
+    if (!P.force && st_direct && st_direct[a] && st_products[a] < (1ull << 31)) {
         t_nwin[a] = 1;
         t_W[a] = (uint32_t)P.spanB;
         t_kind[a] = KIND_DIRECT;
@@ -1100,6 +1165,22 @@ __global__ void k_plan_tiles(const uint64_t *st_products, const uint64_t *st_max
     t_kind[a] = (uint8_t)kind;
 }
 
+// Consecutive DIRECT tiles form one unit of about DIRECT_RUN products (no LDS
+// is involved, so units can be large); a run breaks at a non-DIRECT tile, at
+// each multiple of `run` cumulative products and at a row-group boundary.
+__global__ void k_direct_runs(const uint8_t *t_kind, const uint64_t *pp, const uint32_t *atile_id, uint32_t natiles,
+                              int logTL, uint64_t group_rows, uint64_t run, uint32_t *t_nwin) {
+    uint32_t a = blockIdx.x * blockDim.x + threadIdx.x;
+    if (a >= natiles || t_kind[a] != KIND_DIRECT) return;
+    bool start = true;
+    if (run && a > 0 && t_kind[a - 1] == KIND_DIRECT) {
+        start = pp[a] / run != pp[a - 1] / run;
+        if (!start && group_rows)
+            start = ((uint64_t)atile_id[a] << logTL) / group_rows != ((uint64_t)atile_id[a - 1] << logTL) / group_rows;
+    }
+    t_nwin[a] = start ? 1u : 0u;
+}
+
 __global__ void k_emit_units(const uint32_t *t_nwin, const uint32_t *t_W, const uint8_t *t_kind,
                              const uint32_t *t_ubase, uint32_t natiles, uint64_t spanB, const float *pred,
                              uint32_t *u_tile, uint32_t *u_c0, uint32_t *u_W, uint8_t *u_kind, uint8_t *u_multi,
@@ -1137,6 +1218,9 @@ void launch_plan_tiles(const Index &ix, const float *pred, const PlanParams &P, 
     k_plan_tiles<<<(unsigned)div_up(ix.natiles, 256), 256, 0, s>>>((const uint64_t *)ix.st_products,
                                                                      (const uint64_t *)ix.st_maxgp, ix.st_direct, pred,
                                                                      ix.natiles, P, t_nwin, t_W, t_kind);
+    k_direct_runs<<<(unsigned)div_up(ix.natiles, 256), 256, 0, s>>>(t_kind, (const uint64_t *)ix.st_pprefix,
+                                                                      ix.atile_id, ix.natiles, ix.logTL, P.group_rows,
+                                                                      P.run, t_nwin);
 }
 
 void launch_emit_units(const uint32_t *t_nwin, const uint32_t *t_W, const uint8_t *t_kind, const uint32_t *t_ubase,

@@ -224,10 +224,18 @@ __global__ void k_count_valid(const uint32_t *fidA, const uint32_t *fiber_ptr, s
     }
 }
 
-__global__ void k_unit_to_tile(const unsigned long long *unit_count, const uint32_t *unit_tile, uint32_t n,
-                               unsigned long long *tile_count) {
+// Per-tile output counts from per-unit counts; a unit of DIRECT tiles may
+// span several tiles, so DIRECT tiles take their products instead.
+__global__ void k_unit_to_tile(const unsigned long long *unit_count, const uint32_t *unit_tile,
+                               const uint8_t *unit_kind, uint32_t n, unsigned long long *tile_count) {
     uint32_t u = blockIdx.x * blockDim.x + threadIdx.x;
-    if (u < n && unit_count[u]) atomicAdd(&tile_count[unit_tile[u]], unit_count[u]);
+    if (u < n && unit_kind[u] != KIND_DIRECT && unit_count[u]) atomicAdd(&tile_count[unit_tile[u]], unit_count[u]);
+}
+
+__global__ void k_direct_tile_count(const uint8_t *st_direct, const uint64_t *st_products, uint32_t T, int force,
+                                    unsigned long long *tile_count) {
+    uint32_t a = blockIdx.x * blockDim.x + threadIdx.x;
+    if (a < T && !force && st_direct[a]) tile_count[a] = st_products[a];
 }
 
 __global__ void k_kind_hist(const uint8_t *kind, uint32_t n, unsigned long long *h) {
@@ -395,6 +403,7 @@ struct Contraction {
         size_t nsink = 4;
         uint64_t stride = 0, nnz_out = 0;
         int mode = 0;  // placement of the last numeric pass: 0 exact offsets, 1 atomic cursor
+        int plan_force = 0;  // forced accumulator kind of the plan (0: planner's choice)
     } k;
 
     // ka/kb: r key pairs, the first nb of them batch pairs
@@ -568,6 +577,10 @@ int Contraction::run(const fcc_tensor *ta, const int32_t *am, const fcc_tensor *
     P.spanB = g.spanB;
     P.Wd = Wd;
     P.margin = 0.15f;
+    // per-tile counts are wanted when collecting planner statistics: no runs then
+    P.run = o.collect_stats ? 0 : DIRECT_RUN;
+    k.plan_force = P.force;
+    P.group_rows = groups ? g.spanA / groups : 0;
     Tmp<uint32_t> t_nwin(pool, T), t_W(pool, T), t_ubase(pool, T), d_nunits(pool, 1);
     Tmp<uint8_t> t_kind(pool, T);
     Units &U = k.U;
@@ -594,6 +607,9 @@ int Contraction::run(const fcc_tensor *ta, const int32_t *am, const fcc_tensor *
     X.atile_g0 = ix.atile_g0;
     X.atile_lg = ix.atile_lg;
     X.st_products = (const uint64_t *)ix.st_products;
+    X.st_pprefix = (const uint64_t *)ix.st_pprefix;
+    X.Arow = ix.Arow;
+    X.natiles = ix.natiles;
     X.unit_order = nullptr;
     X.row_own = getenv("FCC_ROW_OWN") ? 1 : 0;
     X.claim_div = 2 * (ACC_THREADS / 64);
@@ -762,7 +778,9 @@ int Contraction::run(const fcc_tensor *ta, const int32_t *am, const fcc_tensor *
             // tile-level MAPE against the actual per-unit outputs
             Tmp<unsigned long long> tn(pool, T);
             fill_u64(tn, T, 0, s);
-            k_unit_to_tile<<<grid256(U.n), 256, 0, s>>>(unit_cur, U.tile, U.n, tn);
+            k_unit_to_tile<<<grid256(U.n), 256, 0, s>>>(unit_cur, U.tile, U.kind, U.n, tn);
+            k_direct_tile_count<<<grid256(T), 256, 0, s>>>(ix.st_direct, (const uint64_t *)ix.st_products, T,
+                                                           P.force, tn);
             Tmp<double> mp(pool, 1);
             fill_f64(mp, 1, 0.0, s);
             k_mape<<<grid256(T), 256, 0, s>>>(pred_t, tn, T, mp);
@@ -797,7 +815,9 @@ int Contraction::run(const fcc_tensor *ta, const int32_t *am, const fcc_tensor *
             // labels: exact tile nnz and distinct output columns
             Tmp<unsigned long long> tn(pool, T), tc(pool, T);
             fill_u64(tn, T, 0, s);
-            k_unit_to_tile<<<grid256(U.n), 256, 0, s>>>(unit_count, U.tile, U.n, tn);
+            k_unit_to_tile<<<grid256(U.n), 256, 0, s>>>(unit_count, U.tile, U.kind, U.n, tn);
+            k_direct_tile_count<<<grid256(T), 256, 0, s>>>(ix.st_direct, (const uint64_t *)ix.st_products, T,
+                                                           P.force, tn);
             k_tile_cols<<<std::min<unsigned>(T, 4096), 256, 0, s>>>(ix.atile_g0, ix.group_fiber, ix.fiber_ptr,
                                                                      ix.Bcol, T, g.spanB, tc);
             Tmp<float> lab(pool, (size_t)T * 2);
@@ -1723,7 +1743,9 @@ int fcc_plan_tiles(fcc_plan *plan, uint64_t *nnz, uint64_t *products, float *pre
             Tmp<unsigned long long> tn(ctx->pool, T);
             fill_u64(tn, T, 0, s);
             const unsigned long long *uc = plan->c->o.planner == FCC_PLAN_LEARNED ? k.unit_cur.get() : k.unit_count.get();
-            k_unit_to_tile<<<grid256(k.U.n), 256, 0, s>>>(uc, k.U.tile, k.U.n, tn);
+            k_unit_to_tile<<<grid256(k.U.n), 256, 0, s>>>(uc, k.U.tile, k.U.kind, k.U.n, tn);
+            k_direct_tile_count<<<grid256(T), 256, 0, s>>>(k.ix.st_direct, (const uint64_t *)k.ix.st_products, T,
+                                                           k.plan_force, tn);
             FCC_HIP(hipMemcpyAsync(nnz, tn.get(), 8ull * T, hipMemcpyDeviceToHost, s));
             FCC_HIP(hipStreamSynchronize(s));
         }

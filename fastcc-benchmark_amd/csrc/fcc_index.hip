@@ -205,12 +205,13 @@ __global__ void k_comp_A(const uint32_t *rowA, const uint32_t *fidA, const uint3
 }
 
 __global__ void k_gather_A(const uint64_t *comp, const uint32_t *idx, const double *vals, const uint32_t *fiber_ptr,
-                           size_t n, int logTL, int fbits, uint32_t *Ali, double *Aval, uint32_t *Afib, uint2 *Afp,
-                           uint32_t *gflag, uint32_t *tflag) {
+                           size_t n, int logTL, int fbits, uint32_t *Ali, uint32_t *Arow, double *Aval, uint32_t *Afib,
+                           uint2 *Afp, uint32_t *gflag, uint32_t *tflag) {
     size_t p = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
     if (p >= n) return;
     uint64_t c = comp[p];
     Ali[p] = (uint32_t)(c & ((1ull << logTL) - 1));
+    Arow[p] = (uint32_t)(((c >> (1 + fbits + logTL)) << logTL) | (c & ((1ull << logTL) - 1)));
     uint32_t f = (uint32_t)((c >> logTL) & ((1ull << fbits) - 1));
     Afib[p] = f;
     Afp[p] = make_uint2(fiber_ptr[f], fiber_ptr[f + 1]);
@@ -607,12 +608,14 @@ static void left_from_sorted(DevPool &pool, const Operand &A, const uint64_t *ks
     };
     size_t m = nvalid;
     ix.Ali = (uint32_t *)keep(m * 4);
+    ix.Arow = (uint32_t *)keep(m * 4);
     ix.Aval = (double *)keep(m * 8);
     ix.Afib = (uint32_t *)keep(m * 4);
     Tmp<uint32_t> gflag(pool, m), tflag(pool, m), gid(pool, m), aid(pool, m), tot(pool, 2);
     ix.Afp = (uint2 *)keep(m * 8);
     if (m)
-        k_gather_A<<<grid_for(m), 256, 0, s>>>(ks, vs, A.vals, ix.fiber_ptr, m, logTL, fbits, ix.Ali, ix.Aval, ix.Afib,
+        k_gather_A<<<grid_for(m), 256, 0, s>>>(ks, vs, A.vals, ix.fiber_ptr, m, logTL, fbits, ix.Ali, ix.Arow, ix.Aval,
+                                               ix.Afib,
                                                ix.Afp, gflag, tflag);
     if (ix.keep_perm) {
         ix.permA = (uint32_t *)keep(m * 4);
@@ -679,6 +682,10 @@ static void left_from_sorted(DevPool &pool, const Operand &A, const uint64_t *ks
     if (T)
         k_tile_direct<<<grid_for(T), 256, 0, s>>>(ix.atile_g0, ix.group_ptr, ix.Ali, ix.st_nnz, (uint32_t)T, ix.TL,
                                                    ix.st_direct);
+    // exclusive prefix of the tiles' products (the output count of a run of DIRECT tiles)
+    ix.st_pprefix = (uint64_t *)keep((T + 1) * 8);
+    if (T) scan_u64(pool, ix.st_products, ix.st_pprefix, T, ix.st_pprefix + T, s);
+    else fill_u64((unsigned long long *)ix.st_pprefix, 1, 0, s);
 }
 
 void build_left(DevPool &pool, const Geom &g, const Operand &A, const uint32_t *rowA, const uint32_t *fidA,

@@ -115,6 +115,7 @@ struct Operand {
 // then a distinct output and is written straight out, no accumulator.
 enum : uint8_t { KIND_DENSE = 1, KIND_LHASH = 2, KIND_GHASH = 3, KIND_DIRECT = 4 };
 constexpr uint32_t DIRECT_MAX_TL = 64;  // rows per tile up to which DIRECT tiles are detected
+constexpr uint64_t DIRECT_RUN = 65536;  // products per unit made of consecutive DIRECT tiles
 
 constexpr uint32_t HEAVY_MIN = 32;      // default: fibers at least this long are processed wave-per-group
 constexpr uint32_t DENSE_SLOTS = 9984;  // f64 LDS accumulator slots (78 KB: two workgroups per CU)
@@ -138,6 +139,7 @@ struct Index {
     // A: nAv entries with a matching fiber, sorted by (tile, class, fiber, row-in-tile)
     uint32_t nAv = 0, ngroups = 0, natiles = 0;
     uint32_t *Ali = nullptr;   // row within tile
+    uint32_t *Arow = nullptr;  // global row
     double *Aval = nullptr;
     uint32_t *Afib = nullptr;  // fiber id per entry
     uint2 *Afp = nullptr;      // fiber [start, end) per entry
@@ -164,6 +166,7 @@ struct Index {
     uint32_t *st_sketch = nullptr;    // distinct-column sketch (KMV estimate)
     uint64_t *st_maxgp = nullptr;     // largest single-group product m * |fiber| (lower bound on outputs)
     uint8_t *st_direct = nullptr;     // 1: the tile's entries sit on distinct rows (DIRECT-able)
+    uint64_t *st_pprefix = nullptr;   // natiles+1: exclusive prefix sum of st_products
     int logTL = 0;
     uint32_t TL = 1;
     uint64_t products = 0;
@@ -201,7 +204,10 @@ struct AccArgs {
     const uint32_t *atile_id;
     const uint32_t *atile_g0;
     const uint32_t *atile_lg;
-    const uint64_t *st_products;  // per active tile (DIRECT units: their exact output count)
+    const uint64_t *st_products;  // per active tile
+    const uint64_t *st_pprefix;   // natiles+1 exclusive prefix of st_products (DIRECT run output counts)
+    const uint32_t *Arow;         // global row per left entry
+    uint32_t natiles;
     uint32_t nunits;
     const uint32_t *unit_tile, *unit_c0, *unit_W;
     const uint8_t *unit_kind, *unit_multi;
