@@ -73,42 +73,63 @@ __device__ __forceinline__ bool gtable_full(const AccArgs &A, int probe) {
     return c > ((A.gmask + 1) >> 3) * 7;
 }
 
+// New keys are counted once per wavefront (ballot after the probe loop, where
+// the lanes that entered reconverge): one global atomic per wavefront instead
+// of one per key on a single counter. Every lane of a wavefront works on the
+// same unit.
+__device__ __forceinline__ void count_new_keys(const AccArgs &A, bool inserted, unsigned long long *unit_counter) {
+    const uint64_t bal = __ballot(inserted);
+    const uint32_t lane = threadIdx.x & 63;
+    if (bal && lane == (uint32_t)(__ffsll((unsigned long long)bal) - 1)) {
+        const unsigned long long n = (unsigned long long)__popcll(bal);
+        atomicAdd(A.gcount, n);
+        if (unit_counter) atomicAdd(unit_counter, n);
+    }
+}
+
 __device__ bool ghash_add(const AccArgs &A, uint64_t key, double v, uint32_t unit) {
     uint64_t h = mix64(key) & A.gmask;
+    bool done = false, inserted = false;
     for (int probe = 0; probe < 8192; probe++) {
         if (gtable_full(A, probe)) break;
         unsigned long long old = atomicCAS(&A.gkeys[h], GEMPTY, (unsigned long long)key);
         if (old == GEMPTY) {
             A.gunit[h] = unit;
-            atomicAdd(A.gcount, 1ull);
             unsafeAtomicAdd(&A.gvals[h], v);
-            return true;
+            done = inserted = true;
+            break;
         }
         if (old == key) {
             unsafeAtomicAdd(&A.gvals[h], v);
-            return true;
+            done = true;
+            break;
         }
         h = (h + 1) & A.gmask;
     }
-    atomicOr(A.gflag, 1u);
-    return false;
+    count_new_keys(A, inserted, nullptr);
+    if (!done) atomicOr(A.gflag, 1u);
+    return done;
 }
 
 // count pass: set semantics, a new key adds one to its unit's count
 __device__ void gset_insert(const AccArgs &A, uint64_t key, uint32_t unit) {
     uint64_t h = mix64(key) & A.gmask;
+    bool done = false, inserted = false;
     for (int probe = 0; probe < 8192; probe++) {
         if (gtable_full(A, probe)) break;
         unsigned long long old = atomicCAS(&A.gkeys[h], GEMPTY, (unsigned long long)key);
         if (old == GEMPTY) {
-            atomicAdd(&A.unit_count[unit], 1ull);
-            atomicAdd(A.gcount, 1ull);
-            return;
+            done = inserted = true;
+            break;
         }
-        if (old == key) return;
+        if (old == key) {
+            done = true;
+            break;
+        }
         h = (h + 1) & A.gmask;
     }
-    atomicOr(A.gflag, 1u);
+    count_new_keys(A, inserted, &A.unit_count[unit]);
+    if (!done) atomicOr(A.gflag, 1u);
 }
 
 // ------------------------------------------------------------ LDS hash
@@ -417,28 +438,17 @@ __device__ __forceinline__ void load_first_chunks(const AccArgs &A, uint32_t lan
     }
 }
 
+// One 64-column chunk of a group's fiber (lane = column) against the group's
+// rows li_l/av_l (lane k holds row k, k < mk).
 template <bool NUM, bool BITMAP>
-__device__ __forceinline__ void heavy_group(const AccArgs &A, const UnitCtx &U, const Acc<NUM, BITMAP> &S,
-                                            uint32_t lane, uint32_t cs, uint32_t ce, uint32_t cep0, uint32_t cm,
-                                            uint32_t cli, double cav, uint32_t ccol, double cbv) {
-    uint32_t ncol = ccol;
-    double nbv = cbv;
-    for (uint32_t cb = cs; cb < ce; cb += 64) {
-        uint32_t col = ncol;
-        double b = nbv;
-        const bool ok = cb + lane < ce;
-        if (cb + 64 < ce) {  // next chunk in flight while this one accumulates
-            ncol = 0;
-            nbv = 0.0;
-            if (cb + 64 + lane < ce) {
-                ncol = A.Bcol[cb + 64 + lane];
-                if (NUM) nbv = A.Bval[cb + 64 + lane];
-            }
-        }
-        col -= U.c0;
+__device__ __forceinline__ void heavy_chunk(const AccArgs &A, const UnitCtx &U, const Acc<NUM, BITMAP> &S,
+                                            uint32_t lane, uint32_t col, double b, bool ok, uint32_t li_l,
+                                            double av_l, uint32_t mk) {
+    col -= U.c0;
+    if (U.kind == KIND_DENSE) {
         uint32_t wc = 0, bm = 0;
         bool leader = false;
-        if (U.kind == KIND_DENSE && (!NUM || BITMAP)) {
+        if (!NUM || BITMAP) {
             // The chunk's columns are ascending, so lanes sharing a 32-column
             // bitmap word are contiguous; OR their bits together and let one
             // lane per word touch the bitmap.
@@ -452,6 +462,55 @@ __device__ __forceinline__ void heavy_group(const AccArgs &A, const UnitCtx &U, 
             uint32_t pw = __shfl_up(wc, 1, 64);
             leader = ok && (lane == 0 || pw != wc);
         }
+        const uint32_t wpr = U.Wp >> 5;
+        for (uint32_t kk = 0; kk < mk; kk++) {
+            const uint32_t li = rlane(li_l, kk);
+            if ((!NUM || BITMAP) && leader) atomicOr(&S.bits[li * wpr + wc], bm);
+            if (NUM && ok) lds_add_f64(&S.fv[li * U.Wp + col], rlane_f64(av_l, kk) * b);
+        }
+    } else {
+        for (uint32_t kk = 0; kk < mk; kk++) {
+            const uint32_t li = rlane(li_l, kk);
+            const double av = NUM ? rlane_f64(av_l, kk) : 0.0;
+            if (ok) accum<NUM, BITMAP>(A, U, S, li, col, av * b);
+        }
+    }
+}
+
+template <bool NUM, bool BITMAP>
+__device__ __forceinline__ void heavy_group(const AccArgs &A, const UnitCtx &U, const Acc<NUM, BITMAP> &S,
+                                            uint32_t lane, uint32_t cs, uint32_t ce, uint32_t cep0, uint32_t cm,
+                                            uint32_t cli, double cav, uint32_t ccol, double cbv) {
+    if (cm <= 64) {
+        // All rows already in registers: nothing is loaded between the next
+        // chunk's prefetch and this chunk's use, so the prefetch stays in flight.
+        uint32_t col = ccol;
+        double b = cbv;
+        for (uint32_t cb = cs; cb < ce; cb += 64) {
+            uint32_t ncol = 0;
+            double nbv = 0.0;
+            if (cb + 64 + lane < ce) {
+                ncol = A.Bcol[cb + 64 + lane];
+                if (NUM) nbv = A.Bval[cb + 64 + lane];
+            }
+            heavy_chunk<NUM, BITMAP>(A, U, S, lane, col, b, cb + lane < ce, cli, cav, cm);
+            col = ncol;
+            b = nbv;
+        }
+        return;
+    }
+    // groups with more than 64 rows: rows reloaded 64 at a time per chunk
+    for (uint32_t cb = cs; cb < ce; cb += 64) {
+        uint32_t col = ccol;
+        double b = cbv;
+        if (cb != cs) {
+            col = 0;
+            b = 0.0;
+            if (cb + lane < ce) {
+                col = A.Bcol[cb + lane];
+                if (NUM) b = A.Bval[cb + lane];
+            }
+        }
         for (uint32_t kb = 0; kb < cm; kb += 64) {
             uint32_t li_l = cli;
             double av_l = cav;
@@ -463,24 +522,22 @@ __device__ __forceinline__ void heavy_group(const AccArgs &A, const UnitCtx &U, 
                     if (NUM) av_l = A.Aval[cep0 + kb + lane];
                 }
             }
-            const uint32_t mk = min(64u, cm - kb);
-            if (U.kind == KIND_DENSE) {
-                const uint32_t wpr = U.Wp >> 5;
-                for (uint32_t kk = 0; kk < mk; kk++) {
-                    const uint32_t li = rlane(li_l, kk);
-                    if ((!NUM || BITMAP) && leader) atomicOr(&S.bits[li * wpr + wc], bm);
-                    if (NUM && ok) lds_add_f64(&S.fv[li * U.Wp + col], rlane_f64(av_l, kk) * b);
-                }
-            } else {
-                for (uint32_t kk = 0; kk < mk; kk++) {
-                    const uint32_t li = rlane(li_l, kk);
-                    const double av = NUM ? rlane_f64(av_l, kk) : 0.0;
-                    if (ok) accum<NUM, BITMAP>(A, U, S, li, col, av * b);
-                }
-            }
+            heavy_chunk<NUM, BITMAP>(A, U, S, lane, col, b, cb + lane < ce, li_l, av_l, min(64u, cm - kb));
         }
     }
 }
+
+// Heavy groups (or segments) are claimed in batches per wavefront; lane l
+// holds descriptor l (fiber range clipped to the unit's window, first entry,
+// entries), read back as scalars with readlane.
+//
+// The batch is walked as one stream of (group, 64-column chunk) items with a
+// ring of HRING items in flight: the fiber chunk (lane = column) and, at a
+// group's first chunk, its rows (lane = row) are loaded HRING items before they
+// accumulate. Loads are unconditional on clamped addresses so the compiler's
+// memory counters stay exact across the ring. Batches holding a group with
+// more than 64 rows take the per-group path (heavy_group).
+constexpr int HRING = 4;
 
 template <bool NUM, bool BITMAP>
 __device__ void heavy_claimed(const AccArgs &A, const UnitCtx &U, const Acc<NUM, BITMAP> &S, uint32_t *s_gnext) {
@@ -516,39 +573,71 @@ __device__ void heavy_claimed(const AccArgs &A, const UnitCtx &U, const Acc<NUM,
                 e_l = lane_lower_bound(A.Bcol, s_l, e_l, cend);
             }
         }
-        uint32_t li[PIPE], col[PIPE];
-        double av[PIPE], bv[PIPE];
-#pragma unroll
-        for (int q = 0; q < PIPE; q++) {
-            const uint32_t j = q;  // lanes >= nb hold empty descriptors
-            load_first_chunks<NUM>(A, lane, rlane(s_l, j), rlane(e_l, j), rlane(d.z, j), rlane(d.w, j), li[q], av[q],
-                                   col[q], bv[q]);
-        }
-        for (uint32_t jb = 0; jb < nb; jb += PIPE) {
-            uint32_t cli[PIPE], ccol[PIPE];
-            double cav[PIPE], cbv[PIPE];
-#pragma unroll
-            for (int q = 0; q < PIPE; q++) {
-                cli[q] = li[q];
-                ccol[q] = col[q];
-                cav[q] = av[q];
-                cbv[q] = bv[q];
+        if (__ballot(lane < nb && d.w > 64) != 0) {
+            // rows beyond one wavefront: group by group
+            for (uint32_t j = 0; j < nb; j++) {
+                uint32_t li, col;
+                double av, bv;
+                load_first_chunks<NUM>(A, lane, rlane(s_l, j), rlane(e_l, j), rlane(d.z, j), rlane(d.w, j), li, av,
+                                       col, bv);
+                heavy_group<NUM, BITMAP>(A, U, S, lane, rlane(s_l, j), rlane(e_l, j), rlane(d.z, j), rlane(d.w, j),
+                                         li, av, col, bv);
             }
-            if (jb + PIPE < nb) {
+            continue;
+        }
+        const uint32_t nc_l = (lane < nb && e_l > s_l) ? (e_l - s_l + 63) / 64 : 0u;
+        uint32_t ij = 0, ic = 0;  // issue cursor: group, chunk (wave-uniform)
+        while (ij < nb && rlane(nc_l, ij) == 0) ij++;
+        uint32_t rcol[HRING], rli[HRING], rj[HRING], rc[HRING];
+        double rb[HRING], rav[HRING];
+        bool rv[HRING];
+        auto issue = [&](int k) {
+            rv[k] = ij < nb;
+            rcol[k] = 0;
+            rb[k] = 0.0;
+            rli[k] = 0;
+            rav[k] = 0.0;
+            if (!rv[k]) return;
+            const uint32_t s0 = rlane(s_l, ij) + ic * 64, e = rlane(e_l, ij);
+            const uint32_t p = min(s0 + lane, e - 1);
+            rcol[k] = A.Bcol[p];
+            if (NUM) rb[k] = A.Bval[p];
+            if (ic == 0) {
+                const uint32_t ep0 = rlane(d.z, ij), m = rlane(d.w, ij);
+                const uint32_t q = ep0 + min(lane, m - 1);
+                rli[k] = A.Ali[q];
+                if (NUM) rav[k] = A.Aval[q];
+            }
+            rj[k] = ij;
+            rc[k] = ic;
+            if (++ic == rlane(nc_l, ij)) {
+                ic = 0;
+                ij++;
+                while (ij < nb && rlane(nc_l, ij) == 0) ij++;
+            }
+        };
 #pragma unroll
-                for (int q = 0; q < PIPE; q++) {
-                    const uint32_t j = jb + PIPE + q;
-                    load_first_chunks<NUM>(A, lane, rlane(s_l, j & 63), rlane(e_l, j & 63), rlane(d.z, j & 63),
-                                           j < 64 ? rlane(d.w, j & 63) : 0u, li[q], av[q], col[q], bv[q]);
+        for (int k = 0; k < HRING; k++) issue(k);
+        uint32_t gli = 0, gm = 0;
+        double gav = 0.0;
+        for (bool more = rv[0]; more;) {
+#pragma unroll
+            for (int k = 0; k < HRING; k++) {
+                if (rv[k]) {
+                    const uint32_t j = rj[k], c = rc[k];
+                    if (c == 0) {
+                        gli = rli[k];
+                        gav = rav[k];
+                        gm = rlane(d.w, j);
+                    }
+                    const uint32_t s0 = rlane(s_l, j) + c * 64, e = rlane(e_l, j);
+                    const uint32_t col = rcol[k];
+                    const double b = rb[k];
+                    issue(k);  // refill: the item HRING ahead
+                    heavy_chunk<NUM, BITMAP>(A, U, S, lane, col, b, s0 + lane < e, gli, gav, gm);
                 }
             }
-#pragma unroll
-            for (int q = 0; q < PIPE; q++) {
-                const uint32_t j = jb + q;
-                if (j < nb)
-                    heavy_group<NUM, BITMAP>(A, U, S, lane, rlane(s_l, j), rlane(e_l, j), rlane(d.z, j), rlane(d.w, j),
-                                             cli[q], cav[q], ccol[q], cbv[q]);
-            }
+            more = rv[0];
         }
     }
 }
@@ -599,18 +688,47 @@ __device__ void light_entries(const AccArgs &A, const UnitCtx &U, const Acc<NUM,
             s_s0[threadIdx.x] = s0;
             __syncthreads();
             const uint32_t nent = min(le1 - eb, (uint32_t)ACC_THREADS);
-            for (uint32_t q = threadIdx.x; q < P; q += ACC_THREADS) {
+            // product q's entry and fiber position; the loads for the next
+            // product of this lane are issued before the current one accumulates
+            auto locate = [&](uint32_t q, uint32_t &p, uint32_t &ee) {
                 uint32_t lo = 0, hi = nent - 1;  // s_inc[nent - 1] == P > q
                 while (lo < hi) {
                     const uint32_t mid = (lo + hi) >> 1;
                     if (s_inc[mid] > q) hi = mid;
                     else lo = mid + 1;
                 }
-                const uint32_t exc = lo ? s_inc[lo - 1] : 0u;
-                const uint32_t p = s_s0[lo] + (q - exc);
-                const uint32_t ee = eb + lo;
-                const uint32_t col = A.Bcol[p];
-                accum<NUM, BITMAP>(A, U, S, A.Ali[ee], col - U.c0, NUM ? A.Aval[ee] * A.Bval[p] : 0.0);
+                p = s_s0[lo] + (q - (lo ? s_inc[lo - 1] : 0u));
+                ee = eb + lo;
+            };
+            uint32_t q = threadIdx.x, p = 0, ee = 0, col = 0, li = 0;
+            double av = 0.0, b = 0.0;
+            if (q < P) {
+                locate(q, p, ee);
+                col = A.Bcol[p];
+                li = A.Ali[ee];
+                if (NUM) {
+                    b = A.Bval[p];
+                    av = A.Aval[ee];
+                }
+            }
+            for (; q < P; q += ACC_THREADS) {
+                uint32_t ncol = 0, nli = 0;
+                double nav = 0.0, nb = 0.0;
+                if (q + ACC_THREADS < P) {
+                    uint32_t np, nee;
+                    locate(q + ACC_THREADS, np, nee);
+                    ncol = A.Bcol[np];
+                    nli = A.Ali[nee];
+                    if (NUM) {
+                        nb = A.Bval[np];
+                        nav = A.Aval[nee];
+                    }
+                }
+                accum<NUM, BITMAP>(A, U, S, li, col - U.c0, NUM ? av * b : 0.0);
+                col = ncol;
+                li = nli;
+                av = nav;
+                b = nb;
             }
             __syncthreads();  // s_inc / s_s0 / s_w are rewritten by the next chunk
         }
@@ -687,17 +805,43 @@ __device__ void direct_run(const AccArgs &A, const UnitCtx &U, const Acc<true, B
         s_s0[threadIdx.x] = s0;
         __syncthreads();
         const uint32_t nent = min(e1 - eb, (uint32_t)ACC_THREADS);
-        for (uint32_t q = threadIdx.x; q < P; q += ACC_THREADS) {
+        auto locate = [&](uint32_t q, uint32_t &p, uint32_t &ee) {
             uint32_t lo = 0, hi = nent - 1;  // s_inc[nent - 1] == P > q
             while (lo < hi) {
                 const uint32_t mid = (lo + hi) >> 1;
                 if (s_inc[mid] > q) hi = mid;
                 else lo = mid + 1;
             }
-            const uint32_t exc = lo ? s_inc[lo - 1] : 0u;
-            const uint32_t p = s_s0[lo] + (q - exc);
-            const uint32_t ee = eb + lo;
-            direct_emit<true, BITMAP>(A, U, S, A.Arow[ee], A.Bcol[p], A.Aval[ee] * A.Bval[p]);
+            p = s_s0[lo] + (q - (lo ? s_inc[lo - 1] : 0u));
+            ee = eb + lo;
+        };
+        // the next product's loads are in flight while the current one is written
+        uint32_t q = threadIdx.x, row = 0, col = 0;
+        double av = 0.0, b = 0.0;
+        if (q < P) {
+            uint32_t p, ee;
+            locate(q, p, ee);
+            row = A.Arow[ee];
+            col = A.Bcol[p];
+            av = A.Aval[ee];
+            b = A.Bval[p];
+        }
+        for (; q < P; q += ACC_THREADS) {
+            uint32_t nrow = 0, ncol = 0;
+            double nav = 0.0, nb = 0.0;
+            if (q + ACC_THREADS < P) {
+                uint32_t np, nee;
+                locate(q + ACC_THREADS, np, nee);
+                nrow = A.Arow[nee];
+                ncol = A.Bcol[np];
+                nav = A.Aval[nee];
+                nb = A.Bval[np];
+            }
+            direct_emit<true, BITMAP>(A, U, S, row, col, av * b);
+            row = nrow;
+            col = ncol;
+            av = nav;
+            b = nb;
         }
         __syncthreads();  // s_inc / s_s0 / s_w are rewritten by the next chunk
     }
