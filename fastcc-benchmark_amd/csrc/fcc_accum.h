@@ -22,6 +22,8 @@ void launch_emit_units(const uint32_t *t_nwin, const uint32_t *t_W, const uint8_
 void launch_refine_kinds(const unsigned long long *unit_count, uint32_t n, int force, uint8_t *u_kind,
                          unsigned long long *gtotal, hipStream_t s);
 void launch_count(const AccArgs &A, unsigned grid, hipStream_t s);
+// window cuts of every fiber for dense windows of width W (n = windows + 1)
+void launch_fiber_cuts(const Index &ix, uint32_t n, uint32_t W, uint32_t *fcut, hipStream_t s);
 void launch_numeric(const AccArgs &A, unsigned grid, bool bitmap, bool sink, int mode, hipStream_t s);
 void launch_gdrain(const AccArgs &A, bool sink, int mode, hipStream_t s);
 

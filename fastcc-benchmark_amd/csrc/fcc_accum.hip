@@ -564,7 +564,14 @@ __device__ void heavy_claimed(const AccArgs &A, const UnitCtx &U, const Acc<NUM,
         uint4 d = make_uint4(0, 0, 0, 0);
         if (lane < nb) d = desc[bg + lane];
         uint32_t s_l = d.x, e_l = d.x + d.y;
-        if (U.multi && lane < nb && s_l < e_l) {
+        const bool cuts = U.multi && !seg && A.fcut && U.kind == KIND_DENSE;
+        if (cuts && lane < nb) {
+            // precomputed window cuts of the group's fiber
+            const uint32_t w = U.c0 / A.fcut_W;
+            const uint32_t *fc = A.fcut + (size_t)A.group_fiber[bg + lane] * A.fcut_n + w;
+            s_l = fc[0];
+            e_l = fc[1];
+        } else if (U.multi && lane < nb && s_l < e_l) {
             // segments wholly outside the window are dropped on their end columns
             if (A.Bcol[s_l] >= cend || A.Bcol[e_l - 1] < U.c0) {
                 e_l = s_l;
@@ -654,12 +661,21 @@ __device__ void light_entries(const AccArgs &A, const UnitCtx &U, const Acc<NUM,
                               uint32_t le1, uint32_t *stage, uint32_t *s_w) {
     const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const uint32_t cend = U.c0 + U.W;
+    const bool cuts = U.multi && A.fcut && U.kind == KIND_DENSE;
     auto span_of = [&](uint32_t e, uint32_t &s0, uint32_t &len) {
-        const uint2 fp = A.Afp[e];
-        uint32_t a = fp.x, b = fp.y;
-        if (U.multi) {
-            a = lane_lower_bound(A.Bcol, a, b, U.c0);
-            b = lane_lower_bound(A.Bcol, a, b, cend);
+        uint32_t a, b;
+        if (cuts) {
+            const uint32_t *fc = A.fcut + (size_t)A.Afib[e] * A.fcut_n + U.c0 / A.fcut_W;
+            a = fc[0];
+            b = fc[1];
+        } else {
+            const uint2 fp = A.Afp[e];
+            a = fp.x;
+            b = fp.y;
+            if (U.multi) {
+                a = lane_lower_bound(A.Bcol, a, b, U.c0);
+                b = lane_lower_bound(A.Bcol, a, b, cend);
+            }
         }
         s0 = a;
         len = b - a;
@@ -1354,7 +1370,34 @@ __global__ void k_refine_kinds(const unsigned long long *unit_count, uint32_t n,
     if (u_kind[u] == KIND_GHASH) atomicAdd(gtotal, unit_count[u]);
 }
 
+// fcut[f * n + w] = first position of fiber f whose column is >= w * W
+// (w = 0..n-1; w = n-1 is the fiber's end when (n-1) * W >= spanB)
+__global__ void k_fiber_cuts(const uint32_t *fiber_ptr, const uint32_t *Bcol, uint32_t nfib, uint32_t n,
+                             uint32_t W, uint32_t *fcut) {
+    const uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+    if (i >= (uint64_t)nfib * n) return;
+    const uint32_t f = (uint32_t)(i / n), w = (uint32_t)(i - (uint64_t)f * n);
+    uint32_t lo = fiber_ptr[f], hi = fiber_ptr[f + 1];
+    const uint64_t x = (uint64_t)w * W;
+    if (x > 0xFFFFFFFFull) {
+        fcut[i] = hi;
+        return;
+    }
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (Bcol[mid] < (uint32_t)x) lo = mid + 1;
+        else hi = mid;
+    }
+    fcut[i] = lo;
+}
+
 // ------------------------------------------------------------ launchers
+
+void launch_fiber_cuts(const Index &ix, uint32_t n, uint32_t W, uint32_t *fcut, hipStream_t s) {
+    const uint64_t tot = (uint64_t)ix.nfib * n;
+    if (!tot) return;
+    k_fiber_cuts<<<(unsigned)div_up(tot, 256), 256, 0, s>>>(ix.fiber_ptr, ix.Bcol, ix.nfib, n, W, fcut);
+}
 
 void launch_plan_tiles(const Index &ix, const float *pred, const PlanParams &P, uint32_t *t_nwin, uint32_t *t_W,
                        uint8_t *t_kind, hipStream_t s) {

@@ -170,9 +170,11 @@ void choose_tiles(fcc_ctx *ctx, const fcc_options &o, const Geom &g, uint64_t pr
         const uint64_t sp = (g.spanB + 31) & ~31ull;  // dense rows are padded to 32 columns
         uint64_t tf = 1;
         while (tf * 2 * sp <= DENSE_SLOTS) tf *= 2;
-        // dense units use row ownership (one row class per wavefront): at least 16 rows
+        // rows fit whole: as many rows as fit (16..64); else column windows,
+        // fewer rows when the rows are long (each window rescans the tile's
+        // groups, so wide outputs favour fewer, wider windows)
         if (tf * sp <= DENSE_SLOTS && tf >= 16) TL = std::min<uint64_t>(tf, 64);
-        else TL = 16;
+        else TL = g.spanB > 8192 ? 8 : 16;
     } else {
         double t = (double)HASH_TARGET / std::max(1.0, prow);
         TL = 1;
@@ -397,6 +399,7 @@ struct Contraction {
         Tmp<uint32_t> out_c;
         Tmp<double> out_v;
         Tmp<float> pred;  // per-tile MLP predictions (learned planner)
+        Tmp<uint32_t> fcut;  // window cuts of every fiber (dense column windows)
         AccArgs X;
         unsigned grid = 1;
         bool bitmap = false, sink = false, empty = true;
@@ -598,6 +601,20 @@ int Contraction::run(const fcc_tensor *ta, const int32_t *am, const fcc_tensor *
     X.Ali = ix.Ali;
     X.Aval = ix.Aval;
     X.Afp = ix.Afp;
+    X.Afib = ix.Afib;
+    // dense column windows: every fiber's window cuts, once, instead of a
+    // binary search per group per window unit (bounded table size)
+    auto &fcut = k.fcut;
+    if (g.spanB > Wd) {
+        const uint64_t nw = (g.spanB + Wd - 1) / Wd + 1;
+        if ((uint64_t)ix.nfib * nw <= (1ull << 28)) {
+            fcut = Tmp<uint32_t>(pool, (size_t)ix.nfib * nw);
+            launch_fiber_cuts(ix, (uint32_t)nw, Wd, fcut, s);
+            X.fcut = fcut;
+            X.fcut_n = (uint32_t)nw;
+            X.fcut_W = Wd;
+        }
+    }
     X.group_ptr = ix.group_ptr;
     X.gdesc = ix.gdesc;
     X.hdesc = ix.hdesc;
@@ -1103,6 +1120,41 @@ int fcc_result_to_host(fcc_ctx *ctx, const fcc_result *r, uint32_t *coords, doub
         (void)hipGetLastError();
         return fail(ctx, FCC_ERR_HIP, e.where);
     }
+    return FCC_OK;
+}
+
+int fcc_result_as_tensor(fcc_ctx *ctx, fcc_result *r, fcc_tensor *t) {
+    if (!ctx) return FCC_ERR_INVALID;
+    if (!r || !t) return fail(ctx, FCC_ERR_INVALID, "NULL result or tensor");
+    if (r->nnz > 0 && (!r->coords || !r->vals))
+        return fail(ctx, FCC_ERR_INVALID, "result holds no materialised output");
+    if (r->order < 1) return fail(ctx, FCC_ERR_INVALID, "an order-0 result is no operand");
+    try {
+        if (r->nnz > 0 && r->coord_stride != r->nnz) {
+            ResultPriv *rp = (ResultPriv *)r->priv;
+            if (!rp || rp->block) return fail(ctx, FCC_ERR_INVALID, "a view with a padded stride cannot be compacted");
+            FCC_HIP(hipSetDevice(ctx->device));
+            uint32_t *nc = (uint32_t *)ctx->pool.get(4ull * r->order * r->nnz);
+            FCC_HIP(hipMemcpy2DAsync(nc, 4ull * r->nnz, r->coords, 4ull * r->coord_stride, 4ull * r->nnz, r->order,
+                                     hipMemcpyDeviceToDevice, ctx->stream));
+            FCC_HIP(hipStreamSynchronize(ctx->stream));
+            ctx->pool.put(rp->coords);
+            rp->coords = nc;
+            r->coords = nc;
+            r->coord_stride = r->nnz;
+        }
+    } catch (HipError &e) {
+        (void)hipGetLastError();
+        return fail(ctx, FCC_ERR_HIP, e.where);
+    } catch (LibError &e) {
+        return fail(ctx, e.code, e.msg);
+    }
+    t->order = r->order;
+    t->nnz = r->nnz;
+    t->coords = r->coords;
+    t->vals = r->vals;
+    t->extents = r->extents;
+    t->on_device = 1;
     return FCC_OK;
 }
 

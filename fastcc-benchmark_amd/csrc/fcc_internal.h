@@ -196,6 +196,12 @@ struct AccArgs {
     const uint32_t *Ali;
     const double *Aval;
     const uint2 *Afp;
+    const uint32_t *Afib;         // fiber per left entry
+    // dense column windows of width fcut_W: fcut[f * fcut_n + w] = first
+    // position of fiber f at a column >= w * fcut_W (w = 0..fcut_n-1), or null
+    const uint32_t *fcut;
+    uint32_t fcut_n;
+    uint32_t fcut_W;
     const uint32_t *group_ptr;
     const uint4 *gdesc;
     const uint4 *hdesc;

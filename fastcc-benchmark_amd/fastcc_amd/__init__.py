@@ -36,6 +36,7 @@ class DeviceTensor:
     coords_ptr: int   # device pointer, uint32 SoA [order][nnz]
     vals_ptr: int     # device pointer, float64 [nnz]
     keepalive: object = None
+    given_extents: tuple = None  # per-mode extents, or None: max+1 inferred
 
     @staticmethod
     def from_torch(coords, vals):
@@ -53,6 +54,10 @@ def _as_fcc_tensor(t, keep):
         ft.coords, ft.vals = t.coords_ptr, t.vals_ptr
         ft.on_device = 1
         ft.extents = None
+        if t.given_extents is not None:
+            e = (C.c_uint32 * ft.order)(*[int(x) for x in t.given_extents])
+            keep.append(e)
+            ft.extents = C.cast(e, C.POINTER(C.c_uint32))
         return ft
     coords = np.ascontiguousarray(t.coords, dtype=np.uint32)
     vals = np.ascontiguousarray(t.vals, dtype=np.float64)
@@ -101,6 +106,15 @@ class Result:
                                                   vals.ctypes.data)
             self._ctx._check(rc)
         return coords, vals
+
+    def as_tensor(self) -> "DeviceTensor":
+        """fcc_result_as_tensor: this result as an operand of a later
+        contraction, still on the device (valid until free())."""
+        t = _abi.FccTensor()
+        self._ctx._check(self._ctx.lib.fcc_result_as_tensor(self._ctx.h, C.byref(self._raw), C.byref(t)))
+        self.coord_stride = self._raw.coord_stride
+        self.coords_ptr = self._raw.coords or 0
+        return DeviceTensor(t.order, t.nnz, t.coords or 0, t.vals or 0, self, tuple(self.extents))
 
     def free(self):
         if self._raw is not None and self._ctx.h:

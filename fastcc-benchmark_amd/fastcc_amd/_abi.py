@@ -19,7 +19,7 @@ FCC_MLP_IN, FCC_MLP_HID, FCC_MLP_OUT = 6, 64, 2
 EXPORTS = (
     "fcc_create", "fcc_destroy", "fcc_last_error", "fcc_stream", "fcc_options_default", "fcc_contract",
     "fcc_contract_ex", "fcc_plan_create", "fcc_plan_info_get", "fcc_plan_tile_count", "fcc_plan_tiles",
-    "fcc_plan_execute", "fcc_plan_destroy", "fcc_autotune",
+    "fcc_plan_execute", "fcc_plan_destroy", "fcc_autotune", "fcc_result_as_tensor",
     "fcc_contract_batch", "fcc_result_to_host", "fcc_result_free", "fcc_mlp_param_count", "fcc_mlp_set_params",
     "fcc_mlp_get_params", "fcc_tile_stats_count", "fcc_tile_stats_copy", "fcc_mlp_train",
     "fcc_mlp_forward_backward", "fcc_density_bucket", "fcc_tune_set", "fcc_tune_get",
@@ -181,6 +181,8 @@ def load(path: str | None = None):
     lib.fcc_tune_set.restype = C.c_int
     lib.fcc_tune_get.argtypes = [P, C.POINTER(FccTuneEntry), C.c_int32, C.POINTER(C.c_int32)]
     lib.fcc_tune_get.restype = C.c_int
+    lib.fcc_result_as_tensor.argtypes = [P, C.POINTER(FccResult), C.POINTER(FccTensor)]
+    lib.fcc_result_as_tensor.restype = C.c_int
     lib.fcc_autotune.argtypes = [P, C.POINTER(FccTensor), C.POINTER(C.c_int32), C.POINTER(FccTensor),
                                  C.POINTER(C.c_int32), C.c_int32, C.POINTER(FccOptions), C.c_int32,
                                  C.POINTER(FccTuneEntry)]

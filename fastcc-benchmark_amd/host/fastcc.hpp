@@ -115,6 +115,14 @@ template <class DT> class ListTensor {
         return hv_[n];
     }
     const fcc_result &raw() const { return res_->r; }
+    // The result as an operand of a later contraction, still on the device
+    // (fcc_result_as_tensor); valid while this ListTensor lives.
+    fcc_tensor device_tensor() const {
+        fcc_tensor t{};
+        int rc = fcc_result_as_tensor(res_->ctx, &res_->r, &t);
+        if (rc != FCC_OK) throw fastcc_error(rc, fcc_last_error(res_->ctx));
+        return t;
+    }
 };
 
 template <class DT> class Tensor {

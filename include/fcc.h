@@ -155,6 +155,16 @@ int fcc_contract_batch(fcc_ctx *ctx, int32_t n, const fcc_tensor *a, const int32
  * Stands in for walking the reference's ListTensor nodes (index_tensors.hpp:80-126). */
 int fcc_result_to_host(fcc_ctx *ctx, const fcc_result *r, uint32_t *coords, double *vals);
 void fcc_result_free(fcc_ctx *ctx, fcc_result *r);
+/* Chaining: a materialised result as the operand of a later contraction,
+ * without leaving the device (the reference converts through a host Tensor,
+ * CompactTensor::to_tensor, contract.hpp:1824-1832). Fills *t with the
+ * result's device arrays (on_device = 1, extents = r->extents). A result whose
+ * coord_stride exceeds its nnz (learned planner) is first compacted: its
+ * coordinates move to a stride of nnz. *t views the result's memory and the
+ * r->extents array: valid until fcc_result_free(r). Plan outputs (results of
+ * fcc_plan_execute) are views of the plan and must already have
+ * coord_stride == nnz. */
+int fcc_result_as_tensor(fcc_ctx *ctx, fcc_result *r, fcc_tensor *t);
 
 /* ---- plans: one coordinate pattern, many value sets --------------------- */
 /* A plan fixes everything that depends only on the coordinates of a and b:
