@@ -380,6 +380,7 @@ int oc_contract_tiled_ex(const oc_tensor *a, const int32_t *a_modes, const int32
         else tindex_build(&Rx, b, &g, 0, T, hw);
     }
     omp_set_max_active_levels(nested_prev);
+    out->index_seconds = now_s() - t0;
     if (L.err || Rx.err || !L.tiles || !Rx.tiles) { tindex_free(&L); tindex_free(&Rx); return -2; }
 
     oc_tls *tls = (oc_tls *)calloc(hw, sizeof(oc_tls));

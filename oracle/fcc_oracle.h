@@ -32,6 +32,7 @@ typedef struct {
     double val_wsum;      /* sum of value * u(linear index), u in [0,1) from mix64 */
     uint64_t products;    /* multiply-adds performed */
     double seconds;       /* wall time of the contraction proper (index + tile loop + lists) */
+    double index_seconds; /* of which: building both tile indices (not sampled) */
     int32_t threads;      /* OpenMP threads used */
 } oc_result;
 

@@ -30,6 +30,7 @@ class _OcResult(C.Structure):
         ("val_wsum", C.c_double),
         ("products", C.c_uint64),
         ("seconds", C.c_double),
+        ("index_seconds", C.c_double),
         ("threads", C.c_int32),
     ]
 
@@ -71,11 +72,12 @@ def _load():
 
 class OracleResult:
     def __init__(self, order, coords, vals, coord_hash, val_sum, val_wsum, products, seconds, threads, nnz,
-                 absum=None, nterms=None):
+                 absum=None, nterms=None, index_seconds=0.0):
         self.order, self.coords, self.vals = order, coords, vals
         self.absum, self.nterms = absum, nterms
         self.coord_hash, self.val_sum, self.val_wsum = coord_hash, val_sum, val_wsum
         self.products, self.seconds, self.threads, self.nnz = products, seconds, threads, nnz
+        self.index_seconds = index_seconds
 
 
 def _mk(t, keep, ext=None):
@@ -108,7 +110,7 @@ def _take(lib, r: _OcResult, sort: bool) -> OracleResult:
             absum = np.ctypeslib.as_array(r.absum, shape=(max(n, 1),))[:n].copy()
             nterms = np.ctypeslib.as_array(r.nterms, shape=(max(n, 1),))[:n].copy()
     out = OracleResult(r.order, coords, vals, r.coord_hash, r.val_sum, r.val_wsum, r.products, r.seconds,
-                       r.threads, r.nnz, absum, nterms)
+                       r.threads, r.nnz, absum, nterms, r.index_seconds)
     lib.oc_result_free(C.byref(r))
     return out
 
