@@ -440,12 +440,12 @@ __device__ __forceinline__ void load_first_chunks(const AccArgs &A, uint32_t lan
 
 // One 64-column chunk of a group's fiber (lane = column) against the group's
 // rows li_l/av_l (lane k holds row k, k < mk).
-template <bool NUM, bool BITMAP>
+template <bool NUM, bool BITMAP, bool DONLY = false>
 __device__ __forceinline__ void heavy_chunk(const AccArgs &A, const UnitCtx &U, const Acc<NUM, BITMAP> &S,
                                             uint32_t lane, uint32_t col, double b, bool ok, uint32_t li_l,
                                             double av_l, uint32_t mk) {
     col -= U.c0;
-    if (U.kind == KIND_DENSE) {
+    if (DONLY || U.kind == KIND_DENSE) {
         uint32_t wc = 0, bm = 0;
         bool leader = false;
         if (!NUM || BITMAP) {
@@ -537,7 +537,69 @@ __device__ __forceinline__ void heavy_group(const AccArgs &A, const UnitCtx &U, 
 // accumulate. Loads are unconditional on clamped addresses so the compiler's
 // memory counters stay exact across the ring. Batches holding a group with
 // more than 64 rows take the per-group path (heavy_group).
-constexpr int HRING = 4;
+constexpr int HRING_DENSE = 8;  // dense units: a small chunk body, a deep ring
+constexpr int HRING_HASH = 4;   // hash units: the chunk body holds the hash probes
+
+
+// The (group, chunk) stream of one claimed batch, RING items in flight.
+template <bool NUM, bool BITMAP, int RING, bool DONLY>
+__device__ __forceinline__ void heavy_ring(const AccArgs &A, const UnitCtx &U, const Acc<NUM, BITMAP> &S,
+                                           uint32_t lane, uint32_t nb, uint32_t s_l, uint32_t e_l, uint4 d) {
+    const uint32_t nc_l = (lane < nb && e_l > s_l) ? (e_l - s_l + 63) / 64 : 0u;
+    uint32_t ij = 0, ic = 0;  // issue cursor: group, chunk (wave-uniform)
+    while (ij < nb && rlane(nc_l, ij) == 0) ij++;
+    uint32_t rcol[RING], rli[RING], rj[RING], rc[RING];
+    double rb[RING], rav[RING];
+    bool rv[RING];
+    auto issue = [&](int k) {
+        rv[k] = ij < nb;
+        rcol[k] = 0;
+        rb[k] = 0.0;
+        rli[k] = 0;
+        rav[k] = 0.0;
+        if (!rv[k]) return;
+        const uint32_t s0 = rlane(s_l, ij) + ic * 64, e = rlane(e_l, ij);
+        const uint32_t p = min(s0 + lane, e - 1);
+        rcol[k] = A.Bcol[p];
+        if (NUM) rb[k] = A.Bval[p];
+        if (ic == 0) {
+            const uint32_t ep0 = rlane(d.z, ij), m = rlane(d.w, ij);
+            const uint32_t q = ep0 + min(lane, m - 1);
+            rli[k] = A.Ali[q];
+            if (NUM) rav[k] = A.Aval[q];
+        }
+        rj[k] = ij;
+        rc[k] = ic;
+        if (++ic == rlane(nc_l, ij)) {
+            ic = 0;
+            ij++;
+            while (ij < nb && rlane(nc_l, ij) == 0) ij++;
+        }
+    };
+#pragma unroll
+    for (int k = 0; k < RING; k++) issue(k);
+    uint32_t gli = 0, gm = 0;
+    double gav = 0.0;
+    for (bool more = rv[0]; more;) {
+#pragma unroll
+        for (int k = 0; k < RING; k++) {
+            if (rv[k]) {
+                const uint32_t j = rj[k], c = rc[k];
+                if (c == 0) {
+                    gli = rli[k];
+                    gav = rav[k];
+                    gm = rlane(d.w, j);
+                }
+                const uint32_t s0 = rlane(s_l, j) + c * 64, e = rlane(e_l, j);
+                const uint32_t col = rcol[k];
+                const double b = rb[k];
+                issue(k);  // refill: the item RING ahead
+                heavy_chunk<NUM, BITMAP, DONLY>(A, U, S, lane, col, b, s0 + lane < e, gli, gav, gm);
+            }
+        }
+        more = rv[0];
+    }
+}
 
 template <bool NUM, bool BITMAP>
 __device__ void heavy_claimed(const AccArgs &A, const UnitCtx &U, const Acc<NUM, BITMAP> &S, uint32_t *s_gnext) {
@@ -592,60 +654,10 @@ __device__ void heavy_claimed(const AccArgs &A, const UnitCtx &U, const Acc<NUM,
             }
             continue;
         }
-        const uint32_t nc_l = (lane < nb && e_l > s_l) ? (e_l - s_l + 63) / 64 : 0u;
-        uint32_t ij = 0, ic = 0;  // issue cursor: group, chunk (wave-uniform)
-        while (ij < nb && rlane(nc_l, ij) == 0) ij++;
-        uint32_t rcol[HRING], rli[HRING], rj[HRING], rc[HRING];
-        double rb[HRING], rav[HRING];
-        bool rv[HRING];
-        auto issue = [&](int k) {
-            rv[k] = ij < nb;
-            rcol[k] = 0;
-            rb[k] = 0.0;
-            rli[k] = 0;
-            rav[k] = 0.0;
-            if (!rv[k]) return;
-            const uint32_t s0 = rlane(s_l, ij) + ic * 64, e = rlane(e_l, ij);
-            const uint32_t p = min(s0 + lane, e - 1);
-            rcol[k] = A.Bcol[p];
-            if (NUM) rb[k] = A.Bval[p];
-            if (ic == 0) {
-                const uint32_t ep0 = rlane(d.z, ij), m = rlane(d.w, ij);
-                const uint32_t q = ep0 + min(lane, m - 1);
-                rli[k] = A.Ali[q];
-                if (NUM) rav[k] = A.Aval[q];
-            }
-            rj[k] = ij;
-            rc[k] = ic;
-            if (++ic == rlane(nc_l, ij)) {
-                ic = 0;
-                ij++;
-                while (ij < nb && rlane(nc_l, ij) == 0) ij++;
-            }
-        };
-#pragma unroll
-        for (int k = 0; k < HRING; k++) issue(k);
-        uint32_t gli = 0, gm = 0;
-        double gav = 0.0;
-        for (bool more = rv[0]; more;) {
-#pragma unroll
-            for (int k = 0; k < HRING; k++) {
-                if (rv[k]) {
-                    const uint32_t j = rj[k], c = rc[k];
-                    if (c == 0) {
-                        gli = rli[k];
-                        gav = rav[k];
-                        gm = rlane(d.w, j);
-                    }
-                    const uint32_t s0 = rlane(s_l, j) + c * 64, e = rlane(e_l, j);
-                    const uint32_t col = rcol[k];
-                    const double b = rb[k];
-                    issue(k);  // refill: the item HRING ahead
-                    heavy_chunk<NUM, BITMAP>(A, U, S, lane, col, b, s0 + lane < e, gli, gav, gm);
-                }
-            }
-            more = rv[0];
-        }
+        if (U.kind == KIND_DENSE)
+            heavy_ring<NUM, BITMAP, HRING_DENSE, true>(A, U, S, lane, nb, s_l, e_l, d);
+        else
+            heavy_ring<NUM, BITMAP, HRING_HASH, false>(A, U, S, lane, nb, s_l, e_l, d);
     }
 }
 
