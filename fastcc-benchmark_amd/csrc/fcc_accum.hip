@@ -627,7 +627,7 @@ __device__ void heavy_claimed(const AccArgs &A, const UnitCtx &U, const Acc<NUM,
         if (lane < nb) d = desc[bg + lane];
         uint32_t s_l = d.x, e_l = d.x + d.y;
         const bool cuts = U.multi && !seg && A.fcut && U.kind == KIND_DENSE;
-        if (cuts && lane < nb) {
+        if (cuts && lane < nb && d.y) {
             // precomputed window cuts of the group's fiber
             const uint32_t w = U.c0 / A.fcut_W;
             const uint32_t *fc = A.fcut + (size_t)A.group_fiber[bg + lane] * A.fcut_n + w;
@@ -680,6 +680,10 @@ __device__ void light_entries(const AccArgs &A, const UnitCtx &U, const Acc<NUM,
             const uint32_t *fc = A.fcut + (size_t)A.Afib[e] * A.fcut_n + U.c0 / A.fcut_W;
             a = fc[0];
             b = fc[1];
+            if (A.dk_tile) {  // entries of dense keys carry an empty fiber range
+                const uint2 fp = A.Afp[e];
+                if (fp.x == fp.y) b = a;
+            }
         } else {
             const uint2 fp = A.Afp[e];
             a = fp.x;
@@ -875,11 +879,84 @@ __device__ void direct_run(const AccArgs &A, const UnitCtx &U, const Acc<true, B
     }
 }
 
+// ------------------------------------------------------- dense-key block
+// Presence bits of the dense-key block on `row`, columns [col, col + 32); col
+// need not be word-aligned (the map carries a spare word at its end).
+__device__ __forceinline__ uint32_t dk_word(const AccArgs &A, uint64_t row, uint64_t col) {
+    const uint32_t *pr = A.dk_P + row * A.dk_pw + (col >> 5);
+    const uint32_t sh = (uint32_t)(col & 31);
+    uint32_t w = pr[0];
+    if (sh) w = (uint32_t)(((((uint64_t)pr[1]) << 32) | w) >> sh);
+    return w;
+}
+
+// Bitmap word w of a dense unit (rows of U.Wp slots) from the block.
+__device__ __forceinline__ uint32_t dk_dense_word(const AccArgs &A, const UnitCtx &U, uint32_t w) {
+    const uint32_t wpr = U.Wp >> 5;
+    const uint32_t li = w / wpr, cl0 = (w - li * wpr) * 32;
+    const uint64_t row = U.row0 + li;
+    if (row >= A.spanA || cl0 >= U.W) return 0u;
+    uint32_t b = dk_word(A, row, (uint64_t)U.c0 + cl0);
+    if (U.W - cl0 < 32) b &= (1u << (U.W - cl0)) - 1u;
+    return b;
+}
+
+// Dense unit accumulator start: block values where present, else the empty
+// marker (+0 with a bitmap, -0 without; a present -0 is stored as +0).
+template <bool BITMAP>
+__device__ void dk_seed_dense(const AccArgs &A, const UnitCtx &U, double *fv, uint32_t *bits, uint32_t nslots) {
+    const double init = BITMAP ? 0.0 : -0.0;
+    for (uint32_t s = threadIdx.x; s < nslots; s += ACC_THREADS) {
+        const uint32_t li = s / U.Wp, cl = s - li * U.Wp;
+        const uint64_t row = U.row0 + li, col = (uint64_t)U.c0 + cl;
+        double v = init;
+        if (cl < U.W && row < A.spanA && ((A.dk_P[row * A.dk_pw + (col >> 5)] >> (col & 31)) & 1u)) {
+            v = A.dk_C[row * A.dk_rb + col];
+            if (!BITMAP) v = v + 0.0;
+        }
+        fv[s] = v;
+    }
+    if (BITMAP)
+        for (uint32_t w = threadIdx.x; w < nslots / 32; w += ACC_THREADS) bits[w] = dk_dense_word(A, U, w);
+}
+
+// Hash unit start: every present cell of the unit's rows x window goes in
+// like a product (count pass: the key; numeric pass: the block value).
+template <bool NUM, bool BITMAP>
+__device__ void dk_seed_cells(const AccArgs &A, const UnitCtx &U, const Acc<NUM, BITMAP> &S) {
+    const uint32_t wpr = (U.W + 31) / 32;
+    const uint32_t n = (1u << A.logTL) * wpr;
+    for (uint32_t i = threadIdx.x; i < n; i += ACC_THREADS) {
+        const uint32_t li = i / wpr, cl0 = (i - li * wpr) * 32;
+        const uint64_t row = U.row0 + li;
+        if (row >= A.spanA) continue;
+        uint32_t w = dk_word(A, row, (uint64_t)U.c0 + cl0);
+        if (U.W - cl0 < 32) w &= (1u << (U.W - cl0)) - 1u;
+        while (w) {
+            const uint32_t cl = cl0 + (uint32_t)__builtin_ctz(w);
+            w &= w - 1;
+            const uint64_t gk = row * A.spanB + (U.c0 + cl);
+            if (U.kind == KIND_LHASH) {
+                const uint32_t key = li * U.W + cl;
+                const int slot =
+                    NUM ? lds_probe<12>(S.keys, S.cnt, LH_MAX, key) : lds_probe<14>(S.keys, S.cnt, SYM_MAX, key);
+                if (slot >= 0) {
+                    if (NUM) lds_add_f64(&S.fv[slot], A.dk_C[row * A.dk_rb + U.c0 + cl]);
+                    continue;
+                }
+            }
+            if (NUM) ghash_add(A, gk, A.dk_C[row * A.dk_rb + U.c0 + cl], U.u);
+            else gset_insert(A, gk, U.u);
+        }
+    }
+}
+
 // One unit: heavy groups (a wavefront per group), then the light entries.
 template <bool NUM, bool BITMAP>
 __device__ void process_unit(const AccArgs &A, const UnitCtx &U, const Acc<NUM, BITMAP> &S, uint32_t *s_gnext,
                              uint32_t *stage, uint32_t *s_w) {
     const uint32_t gl = A.atile_lg[U.a], g1 = A.atile_g0[U.a + 1];
+    if (U.kind != KIND_DENSE && A.dk_tile && A.dk_tile[U.a]) dk_seed_cells<NUM, BITMAP>(A, U, S);
     if (A.row_own && U.kind == KIND_DENSE && (1u << A.logTL) >= (uint32_t)NWAVES) {
         dense_heavy_owned<NUM, BITMAP>(A, U, S);
         __syncthreads();  // plain updates above, atomic updates below
@@ -1000,7 +1077,9 @@ __global__ __launch_bounds__(ACC_THREADS) void k_numeric(AccArgs A) {
             sk.group = U.row0 / A.group_rows;
         }
         if (A.clock_start && ui == 0 && threadIdx.x == 0) *A.clock_start = wall_clock64();
-        if (U.kind == KIND_DENSE) {
+        if (U.kind == KIND_DENSE && A.dk_tile && A.dk_tile[U.a]) {
+            dk_seed_dense<BITMAP>(A, U, s_f64, s_bits, nslots);
+        } else if (U.kind == KIND_DENSE) {
             const double init = BITMAP ? 0.0 : -0.0;
             for (uint32_t s = threadIdx.x; s < nslots; s += ACC_THREADS) s_f64[s] = init;
             if (BITMAP)
@@ -1165,7 +1244,9 @@ __global__ __launch_bounds__(ACC_THREADS) void k_count(AccArgs A) {
         const uint32_t TL = 1u << A.logTL;
         const uint32_t nwords = U.kind == KIND_DENSE ? TL * U.Wp / 32 : 0;
         if (U.kind == KIND_DENSE) {
-            for (uint32_t s = threadIdx.x; s < nwords; s += ACC_THREADS) s_set[s] = 0;
+            const bool seeded = A.dk_tile && A.dk_tile[U.a];
+            for (uint32_t s = threadIdx.x; s < nwords; s += ACC_THREADS)
+                s_set[s] = seeded ? dk_dense_word(A, U, s) : 0u;
         } else if (U.kind == KIND_LHASH) {
             for (uint32_t s = threadIdx.x; s < SYM_SLOTS; s += ACC_THREADS) s_set[s] = LEMPTY;
         }

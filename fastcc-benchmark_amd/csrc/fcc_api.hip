@@ -23,6 +23,7 @@
 #include "../../include/fcc.h"
 #include "fcc_accum.h"
 #include "fcc_batch.h"
+#include "fcc_dense.h"
 #include "fcc_mlp.h"
 
 using namespace fcc;
@@ -148,8 +149,8 @@ int validate(const fcc_tensor *t, const char *name, std::string &msg) {
 // Rows per left tile (and dense window) from global statistics, the tuner
 // table, or the caller's hint (the reference's tile_size, rounded up to a
 // power of two like contract.hpp:844-846).
-void choose_tiles(fcc_ctx *ctx, const fcc_options &o, const Geom &g, uint64_t products, uint32_t nvalid, int *logTL,
-                  uint32_t *Wd, int32_t *bucket_out, int32_t *wg_out) {
+void choose_tiles(fcc_ctx *ctx, const fcc_options &o, const Geom &g, uint64_t products, uint32_t nvalid,
+                  bool dense_keys, int *logTL, uint32_t *Wd, int32_t *bucket_out, int32_t *wg_out) {
     double cells = (double)g.spanA * (double)g.spanB;
     double dens = cells > 0 ? (double)products / cells : 0.0;
     double rows = (double)g.spanA * (1.0 - std::exp(-(double)nvalid / std::max(1.0, (double)g.spanA)));
@@ -166,6 +167,11 @@ void choose_tiles(fcc_ctx *ctx, const fcc_options &o, const Geom &g, uint64_t pr
         TL = pow2_ceil((uint64_t)o.tile_hint);
     } else if (te && te->row_tile > 0) {
         TL = (uint64_t)te->row_tile;
+    } else if (dense_keys) {
+        // the dense keys' products are gone: what stays is many short
+        // fibers, and every column window rescans the tile's entries, so
+        // two-row tiles with the widest windows
+        TL = 2;
     } else if (dens >= 0.25) {
         const uint64_t sp = (g.spanB + 31) & ~31ull;  // dense rows are padded to 32 columns
         uint64_t tf = 1;
@@ -400,6 +406,7 @@ struct Contraction {
         Tmp<double> out_v;
         Tmp<float> pred;  // per-tile MLP predictions (learned planner)
         Tmp<uint32_t> fcut;  // window cuts of every fiber (dense column windows)
+        DenseKeys dk;        // dense-key block (dk.nh == 0: none)
         AccArgs X;
         unsigned grid = 1;
         bool bitmap = false, sink = false, empty = true;
@@ -525,9 +532,25 @@ int Contraction::run(const fcc_tensor *ta, const int32_t *am, const fcc_tensor *
         products = h[1];
     }
     st.products = products;
+
+    // ---- dense keys: keys dense on both sides leave the tile traversal and
+    // are contracted by one FP64 GEMM; units start from its block
+    DenseKeys &dk = k.dk;
+    dk = DenseKeys();
+    if (!groups && o.dense_keys >= 0.0f && products) {
+        size_t mem_free = 0, mem_total = 0;
+        FCC_HIP(hipMemGetInfo(&mem_free, &mem_total));
+        const uint64_t max_cells = std::min<uint64_t>(1ull << 29, mem_free / 32);
+        // the default threshold also asks the block to pay for itself; an
+        // explicit one is taken as given
+        const bool dflt = o.dense_keys == 0.0f;
+        const bool self_fib = self && o.left_tile_stride <= 1;  // left fibers = right fibers
+        dense_keys_select(pool, g, ix, self_fib ? nullptr : fidA.get(), (size_t)A.nnz, dflt ? 0.02f : o.dense_keys,
+                          max_cells, dflt, dk, hold, s);
+    }
     int logTL;
     uint32_t Wd;
-    choose_tiles(ctx, o, g, products, nvalid, &logTL, &Wd, &bucket, &tuned_wg);
+    choose_tiles(ctx, o, g, products, nvalid, dk.nh > 0, &logTL, &Wd, &bucket, &tuned_wg);
     if (const char *hm = getenv("FCC_HEAVY_MIN")) ix.heavy_min = (uint32_t)std::max(1, atoi(hm));
     if (const char *hs = getenv("FCC_HSEG")) ix.hseg = (uint32_t)std::max(64, atoi(hs));
     if (self && o.left_tile_stride <= 1) build_left_self(pool, g, A, sj, logTL, ix, hold, s);
@@ -539,6 +562,13 @@ int Contraction::run(const fcc_tensor *ta, const int32_t *am, const fcc_tensor *
     st.row_tile = (int32_t)ix.TL;
     st.col_window = (int32_t)Wd;
     st.left_tiles = ix.natiles;
+
+    dense_keys_build(pool, g, ix, dk, hold, s);
+    FCC_HIP(hipEventRecord(ev[7], s));
+    dense_keys_values(ix, dk, s);
+    FCC_HIP(hipEventRecord(ev[8], s));
+    st.dense_keys = dk.nh;
+    st.dense_key_products = dk.products;
     FCC_HIP(hipEventRecord(ev[1], s));
 
     const bool learned = o.planner == FCC_PLAN_LEARNED;
@@ -635,6 +665,14 @@ int Contraction::run(const fcc_tensor *ta, const int32_t *am, const fcc_tensor *
     if (const char *sb = getenv("FCC_SEG_BELOW")) X.seg_below = (uint32_t)std::max(0, atoi(sb));
     X.logTL = ix.logTL;
     X.spanB = g.spanB;
+    X.spanA = g.spanA;
+    if (dk.nh) {
+        X.dk_P = dk.P;
+        X.dk_C = dk.C;
+        X.dk_tile = dk.tile_dk;
+        X.dk_pw = dk.rb / 32;
+        X.dk_rb = dk.rb;
+    }
     X.nxa = g.nxa;
     X.nxb = g.nxb;
     for (int m = 0; m < kMaxOrder; m++) {
@@ -938,6 +976,7 @@ int Contraction::run(const fcc_tensor *ta, const int32_t *am, const fcc_tensor *
         st.units_direct = h[KIND_DIRECT];
     }
     st.ms_index = ev_ms(ev[0], ev[1]);
+    st.ms_dense = ev_ms(ev[7], ev[8]);
     st.ms_plan = ev_ms(ev[1], ev[2]);
     st.ms_symbolic = (learned && !use_exact) ? 0.0 : ev_ms(ev[2], ev[3]);
     st.ms_numeric = ev_ms(ev[4], ev[5]);
@@ -1659,6 +1698,7 @@ void plan_sequence(fcc_plan *P, const double *asrc, const double *bsrc, hipStrea
     AccArgs &X = k.X;
     perm_gather(bsrc, k.ix.permB, k.ix.nB, k.ix.Bval, s);
     perm_gather(asrc, k.ix.permA, k.ix.nAv, k.ix.Aval, s);
+    dense_keys_values(k.ix, k.dk, s);
     fill_u64(X.gkeys, X.gmask + 1, ~0ull, s);
     fill_f64(X.gvals, X.gmask + 1, 0.0, s);
     fill_u64(k.gmeta, 4, 0, s);

@@ -1,0 +1,72 @@
+// Dense contraction keys (fcc_dense.hip).
+//
+// A contraction key k whose left fiber A(:, k) and right fiber B(k, :) are
+// both dense contributes nA(k) * nB(k) products that cover a large part of
+// the output block spanA x spanB. Such keys are taken out of the sparse
+// traversal and contracted together as one dense FP64 GEMM on the matrix
+// cores:
+//     C_dk[row][col] = sum over selected keys h of AT[h][row] * Bd[h][col]
+// Structural support stays exact: bit h of rowbits[row] says A(row, key h)
+// is stored, bit h of colbits[col] says B(key h, col) is stored, and the
+// presence map P[row][col] = (rowbits[row] & colbits[col]) != 0. Every unit
+// of work then starts its accumulator from (P, C_dk) over its rows and column
+// window instead of from zero.
+#pragma once
+#include "fcc_internal.h"
+
+namespace fcc {
+
+constexpr uint32_t DK_BM = 128;  // GEMM tile rows (left rows)
+constexpr uint32_t DK_BN = 128;  // GEMM tile columns (right columns)
+constexpr uint32_t DK_BK = 16;   // GEMM k step (selected keys)
+
+struct DenseKeys {
+    uint32_t nh = 0;          // selected keys (0: the path is off)
+    uint32_t nw = 0;          // 64-bit words per key bitset
+    uint32_t kp = 0;          // nh rounded up to DK_BK
+    uint64_t spanA = 0, spanB = 0;
+    uint64_t ra = 0, rb = 0;  // spanA, spanB rounded up to DK_BM, DK_BN
+    uint64_t products = 0;    // products of the selected keys
+    float tau = 0;            // density threshold used
+    uint32_t *hkey = nullptr;              // per fiber: selected key index, or ~0
+    uint32_t *hfib = nullptr;              // per selected key: its fiber
+    uint32_t ndg = 0;
+    uint32_t *dgrp = nullptr;              // groups (tile, fiber) of the selected keys
+    uint8_t *tile_dk = nullptr;            // per active tile: 1 if it holds a selected key
+    unsigned long long *rowbits = nullptr; // [spanA][nw]
+    unsigned long long *colbits = nullptr; // [nw][rb] (word-major: coalesced per column)
+    uint32_t *P = nullptr;                 // presence [ra][rb / 32] (+2 words)
+    double *AT = nullptr;                  // [kp][ra]
+    double *Bd = nullptr;                  // [kp][rb]
+    double *C = nullptr;                   // [ra][rb]
+};
+
+// Chooses the dense keys once the right fibers and the left join are known
+// (before the left tiles are built): keys with nA * nB >= tau * spanA *
+// spanB, at most 16384 of them, the block within max_cells cells; with
+// require_gain only when their products average four or more per block cell.
+// fidA: the join's fiber per left entry (nA entries), or null for a
+// self-contraction (left fibers = right fibers). Fills nh, products, hkey,
+// hfib and the sizes; synchronises. Leaves dk.nh = 0 when none qualify.
+void dense_keys_select(DevPool &pool, const Geom &g, const Index &ix, const uint32_t *fidA, size_t nA, float tau,
+                       uint64_t max_cells, bool require_gain, DenseKeys &dk, std::vector<std::shared_ptr<void>> &hold,
+                       hipStream_t s);
+
+// After the left tiles: takes the selected keys out of the sparse traversal
+// (group descriptors, light-entry fiber ranges, DIRECT eligibility; heavy
+// segments rebuilt), builds the key bitsets and the presence map, and
+// allocates the value panels and the block. Synchronises.
+void dense_keys_build(DevPool &pool, const Geom &g, Index &ix, DenseKeys &dk, std::vector<std::shared_ptr<void>> &hold,
+                      hipStream_t s);
+
+// Values: scatters the selected keys' current values (ix.Aval, ix.Bval) into
+// the dense panels and runs the GEMM into dk.C. No memset nodes and no
+// allocation: safe to capture into a graph.
+void dense_keys_values(const Index &ix, const DenseKeys &dk, hipStream_t s);
+
+// C[ra][rb] = AT^T * Bd for row-major AT[kp][ra], Bd[kp][rb]; ra, rb
+// multiples of DK_BM / DK_BN, kp a multiple of DK_BK.
+void launch_dgemm_tn(const double *AT, const double *Bd, double *C, uint64_t ra, uint64_t rb, uint32_t kp,
+                     hipStream_t s);
+
+}  // namespace fcc

@@ -1,0 +1,393 @@
+// Dense contraction keys: selection, masking out of the sparse traversal,
+// key bitsets, presence map, value panels and the FP64 MFMA GEMM.
+//
+// The reference contracts every key through its tile accumulators
+// (contract.hpp:850-990: for each left tile, for each contraction index, every
+// pair of left and right entries is multiplied into the accumulator). The
+// products of a key whose left and right fibers are both dense form a rank-1
+// update of a dense block; many such keys together are a dense GEMM, which
+// the matrix cores run at the FP64 peak instead of one LDS atomic per
+// product. The result, the set of outputs and their values, is the same.
+#include "fcc_dense.h"
+
+#include <algorithm>
+
+#include "fcc_index.h"
+
+namespace fcc {
+
+namespace {
+
+inline unsigned grid_of(uint64_t n, unsigned b = 256) { return (unsigned)std::max<uint64_t>(1, div_up(n, b)); }
+
+typedef double f64x4 __attribute__((ext_vector_type(4)));
+
+// left entries per fiber from the join (fiber per left entry, ~0 when
+// unmatched); only fibers long enough on the right to reach the threshold
+// (nB >= nb_min) are counted, which keeps the atomics off the many short ones
+__global__ void k_dk_na(const uint32_t *fidA, size_t n, const uint32_t *fiber_ptr, uint32_t nb_min, uint32_t *na) {
+    const size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t f = fidA[i];
+    if (f != 0xFFFFFFFFu && fiber_ptr[f + 1] - fiber_ptr[f] >= nb_min) atomicAdd(&na[f], 1u);
+}
+
+// self-contraction: the left fibers are the right ones
+__global__ void k_dk_na_self(const uint32_t *fiber_ptr, uint32_t nfib, uint32_t *na) {
+    const uint32_t f = blockIdx.x * blockDim.x + threadIdx.x;
+    if (f < nfib) na[f] = fiber_ptr[f + 1] - fiber_ptr[f];
+}
+
+// a fiber is selected when nA * nB >= tau * spanA * spanB; tot[0] counts the
+// selected fibers, tot[1] sums their products (wave-aggregated)
+__global__ void k_dk_flag(const uint32_t *fiber_ptr, const uint32_t *na, uint32_t nfib, double inv_area, double tau,
+                          uint32_t *flag, unsigned long long *tot) {
+    const uint32_t f = blockIdx.x * blockDim.x + threadIdx.x;
+    unsigned long long c = 0, prod = 0;
+    if (f < nfib) {
+        const uint64_t a = na[f], b = fiber_ptr[f + 1] - fiber_ptr[f];
+        const bool sel = a && (double)a * (double)b * inv_area >= tau;
+        flag[f] = sel ? 1u : 0u;
+        c = sel ? 1ull : 0ull;
+        prod = sel ? a * b : 0ull;
+    }
+    for (int d = 32; d > 0; d >>= 1) {
+        c += __shfl_xor(c, d, 64);
+        prod += __shfl_xor(prod, d, 64);
+    }
+    if ((threadIdx.x & 63) == 0 && c) {
+        atomicAdd(&tot[0], c);
+        atomicAdd(&tot[1], prod);
+    }
+}
+
+__global__ void k_dk_keys(const uint32_t *flag, const uint32_t *idx, uint32_t nfib, uint32_t *hkey, uint32_t *hfib) {
+    const uint32_t f = blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= nfib) return;
+    const bool sel = flag[f] != 0;
+    hkey[f] = sel ? idx[f] : 0xFFFFFFFFu;
+    if (sel) hfib[idx[f]] = f;
+}
+
+__global__ void k_dk_gflag(const uint32_t *group_fiber, const uint32_t *hkey, uint32_t ngroups, uint32_t *gflag) {
+    const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g < ngroups) gflag[g] = hkey[group_fiber[g]] != 0xFFFFFFFFu ? 1u : 0u;
+}
+
+__global__ void k_dk_glist(const uint32_t *gflag, const uint32_t *gidx, uint32_t ngroups, uint32_t *dgrp) {
+    const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g < ngroups && gflag[g]) dgrp[gidx[g]] = g;
+}
+
+// Takes the selected keys' groups out of the sparse traversal: a heavy group
+// gets an empty fiber in its descriptor, a light group's entries get empty
+// fiber ranges; their tiles lose DIRECT eligibility (their outputs meet the
+// dense block). Sets the row bitsets.
+__global__ void k_dk_mask(const uint32_t *dgrp, uint32_t ndg, const uint32_t *group_ptr, const uint32_t *group_fiber,
+                          const uint32_t *group_atile, const uint8_t *glight, const uint32_t *hkey,
+                          const uint32_t *Arow, uint32_t nw, uint4 *gdesc, uint2 *Afp, uint8_t *st_direct,
+                          uint8_t *tile_dk, unsigned long long *rowbits) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= ndg) return;
+    const uint32_t g = dgrp[i];
+    const uint32_t h = hkey[group_fiber[g]], a = group_atile[g];
+    const uint32_t e0 = group_ptr[g], e1 = group_ptr[g + 1];
+    if (glight[g]) {
+        for (uint32_t e = e0; e < e1; e++) {
+            uint2 fp = Afp[e];
+            fp.y = fp.x;
+            Afp[e] = fp;
+        }
+    } else {
+        uint4 d = gdesc[g];
+        d.y = 0;
+        gdesc[g] = d;
+    }
+    st_direct[a] = 0;
+    tile_dk[a] = 1;
+    const unsigned long long bit = 1ull << (h & 63);
+    for (uint32_t e = e0; e < e1; e++) atomicOr(&rowbits[(uint64_t)Arow[e] * nw + (h >> 6)], bit);
+}
+
+// one workgroup per selected key: column bitsets
+__global__ __launch_bounds__(256) void k_dk_colbits(const uint32_t *hfib, const uint32_t *fiber_ptr,
+                                                   const uint32_t *Bcol, uint64_t rb, unsigned long long *colbits) {
+    const uint32_t h = blockIdx.x, f = hfib[h];
+    const unsigned long long bit = 1ull << (h & 63);
+    unsigned long long *row = colbits + (uint64_t)(h >> 6) * rb;
+    for (uint32_t p = fiber_ptr[f] + threadIdx.x; p < fiber_ptr[f + 1]; p += blockDim.x) atomicOr(&row[Bcol[p]], bit);
+}
+
+// P[row][c >> 5] bit (c & 31) = (rowbits[row] & colbits[c]) != 0.
+// Workgroup: 64 rows x 64 columns; wavefront: 16 rows, a lane per column,
+// its column's bitset words held in registers 8 at a time; the 64 rows'
+// bitset words are staged in LDS 8 words per row at a time (broadcast reads).
+__global__ __launch_bounds__(256) void k_dk_presence(const unsigned long long *rowbits,
+                                                    const unsigned long long *colbits, uint64_t spanA, uint64_t rb,
+                                                    uint32_t nw, uint32_t *P) {
+    __shared__ unsigned long long s_rw[64][8];
+    const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const uint64_t c = (uint64_t)blockIdx.x * 64 + lane;
+    const uint64_t rb0 = (uint64_t)blockIdx.y * 64;
+    const uint32_t lr0 = wave * 16;
+    uint32_t hit = 0;
+    for (uint32_t w0 = 0; w0 < nw; w0 += 8) {
+        __syncthreads();
+        for (uint32_t q = threadIdx.x; q < 64 * 8; q += 256) {
+            const uint32_t rr = q >> 3, j = q & 7;
+            const uint64_t row = rb0 + rr;
+            s_rw[rr][j] = (row < spanA && w0 + j < nw) ? rowbits[row * nw + w0 + j] : 0ull;
+        }
+        __syncthreads();
+        unsigned long long cb[8];
+#pragma unroll
+        for (int j = 0; j < 8; j++) cb[j] = w0 + j < nw ? colbits[(uint64_t)(w0 + j) * rb + c] : 0ull;
+#pragma unroll 4
+        for (int i = 0; i < 16; i++) {
+            unsigned long long acc = 0;
+#pragma unroll
+            for (int j = 0; j < 8; j++) acc |= s_rw[lr0 + i][j] & cb[j];
+            if (acc) hit |= 1u << i;
+        }
+    }
+    const uint64_t r0 = rb0 + lr0;
+    const uint64_t pw = rb / 32;
+    for (int i = 0; i < 16; i++) {
+        const unsigned long long bal = __ballot((hit >> i) & 1u);
+        if (lane == 0) {
+            uint32_t *pr = P + (r0 + i) * pw + (uint64_t)blockIdx.x * 2;
+            pr[0] = (uint32_t)bal;
+            pr[1] = (uint32_t)(bal >> 32);
+        }
+    }
+}
+
+__global__ void k_dk_scatter_A(const uint32_t *dgrp, uint32_t ndg, const uint32_t *group_ptr,
+                               const uint32_t *group_fiber, const uint32_t *hkey, const uint32_t *Arow,
+                               const double *Aval, uint64_t ra, double *AT) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= ndg) return;
+    const uint32_t g = dgrp[i];
+    double *col = AT + (uint64_t)hkey[group_fiber[g]] * ra;
+    for (uint32_t e = group_ptr[g]; e < group_ptr[g + 1]; e++) atomicAdd(&col[Arow[e]], Aval[e]);
+}
+
+__global__ __launch_bounds__(256) void k_dk_scatter_B(const uint32_t *hfib, const uint32_t *fiber_ptr,
+                                                     const uint32_t *Bcol, const double *Bval, uint64_t rb,
+                                                     double *Bd) {
+    const uint32_t h = blockIdx.x, f = hfib[h];
+    double *row = Bd + (uint64_t)h * rb;
+    for (uint32_t p = fiber_ptr[f] + threadIdx.x; p < fiber_ptr[f + 1]; p += blockDim.x)
+        atomicAdd(&row[Bcol[p]], Bval[p]);
+}
+
+// C = AT^T * Bd. Workgroup tile 128 x 128, four wavefronts of 64 x 64, each a
+// 4 x 4 grid of v_mfma_f64_16x16x4 blocks (16 f64x4 accumulators). The k
+// loop steps 16 keys at a time through double-buffered LDS panels
+// sA[k][row], sB[k][col]; the next panels are loaded into registers while the
+// current ones feed the matrix cores, one barrier per step.
+// MFMA operand layout (16x16x4, f64): lane l holds A[i = l % 16][k = l / 16]
+// and B[k = l / 16][j = l % 16]; the result block D[i][j] sits in lane
+// j + 16 * (i % 4), element i / 4.
+__global__ __launch_bounds__(256, 2) void k_dgemm_tn(const double *__restrict__ AT, const double *__restrict__ Bd,
+                                                     double *__restrict__ C, uint32_t nbm, uint32_t nbn, uint64_t ra,
+                                                     uint64_t rb, uint32_t kp) {
+    __shared__ __align__(16) double sA[2][DK_BK][DK_BM];
+    __shared__ __align__(16) double sB[2][DK_BK][DK_BN];
+    // Workgroups are dealt round-robin over the 8 XCDs (one L2 each): give
+    // every XCD a contiguous run of logical tiles, and walk logical tiles down
+    // bands of 8 row tiles so neighbours share AT rows and a Bd column panel.
+    const uint32_t nblk = nbm * nbn;
+    const uint32_t x = blockIdx.x & 7, jx = blockIdx.x >> 3;
+    const uint32_t q = nblk >> 3, rem = nblk & 7;
+    const uint32_t L = x * q + min(x, rem) + jx;
+    const uint32_t band = 8, per_band = band * nbn;
+    const uint32_t bnd = L / per_band, w = L - bnd * per_band;
+    const uint32_t rows_in = min(band, nbm - bnd * band);
+    const uint32_t tm = bnd * band + w % rows_in, tn = w / rows_in;
+
+    const uint32_t t = threadIdx.x, lane = t & 63, wave = t >> 6;
+    const uint32_t wm = (wave >> 1) * 64, wn = (wave & 1) * 64;
+    const uint32_t lr = t >> 4, lc = (t & 15) * 8;  // panel copy: k row lr, 8 doubles from lc
+    const double *gA = AT + (uint64_t)lr * ra + (uint64_t)tm * DK_BM + lc;
+    const double *gB = Bd + (uint64_t)lr * rb + (uint64_t)tn * DK_BN + lc;
+    f64x4 acc[4][4];
+#pragma unroll
+    for (int i = 0; i < 4; i++)
+#pragma unroll
+        for (int j = 0; j < 4; j++) acc[i][j] = f64x4{0.0, 0.0, 0.0, 0.0};
+    double2 pa[4], pb[4];
+#pragma unroll
+    for (int v = 0; v < 4; v++) {
+        pa[v] = *reinterpret_cast<const double2 *>(gA + 2 * v);
+        pb[v] = *reinterpret_cast<const double2 *>(gB + 2 * v);
+    }
+#pragma unroll
+    for (int v = 0; v < 4; v++) {
+        *reinterpret_cast<double2 *>(&sA[0][lr][lc + 2 * v]) = pa[v];
+        *reinterpret_cast<double2 *>(&sB[0][lr][lc + 2 * v]) = pb[v];
+    }
+    __syncthreads();
+    const uint32_t nk = kp / DK_BK;
+    const uint32_t kr = lane >> 4, ci = lane & 15;
+    for (uint32_t kb = 0; kb < nk; kb++) {
+        const uint32_t cur = kb & 1;
+        // the next panels (the last step reloads its own: no branch, so the
+        // staging registers stay registers; the idle buffer takes the copy)
+        const uint32_t kn = kb + 1 < nk ? kb + 1 : kb;
+        {
+            const uint64_t oa = (uint64_t)kn * DK_BK * ra, ob = (uint64_t)kn * DK_BK * rb;
+#pragma unroll
+            for (int v = 0; v < 4; v++) {
+                pa[v] = *reinterpret_cast<const double2 *>(gA + oa + 2 * v);
+                pb[v] = *reinterpret_cast<const double2 *>(gB + ob + 2 * v);
+            }
+        }
+#pragma unroll
+        for (uint32_t kk = 0; kk < DK_BK; kk += 4) {
+            double a[4], b[4];
+#pragma unroll
+            for (int i = 0; i < 4; i++) a[i] = sA[cur][kk + kr][wm + i * 16 + ci];
+#pragma unroll
+            for (int j = 0; j < 4; j++) b[j] = sB[cur][kk + kr][wn + j * 16 + ci];
+#pragma unroll
+            for (int i = 0; i < 4; i++)
+#pragma unroll
+                for (int j = 0; j < 4; j++)
+                    acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i], b[j], acc[i][j], 0, 0, 0);
+        }
+#pragma unroll
+        for (int v = 0; v < 4; v++) {
+            *reinterpret_cast<double2 *>(&sA[cur ^ 1][lr][lc + 2 * v]) = pa[v];
+            *reinterpret_cast<double2 *>(&sB[cur ^ 1][lr][lc + 2 * v]) = pb[v];
+        }
+        __syncthreads();
+    }
+    const uint64_t row0 = (uint64_t)tm * DK_BM + wm + kr, col0 = (uint64_t)tn * DK_BN + wn + ci;
+#pragma unroll
+    for (int i = 0; i < 4; i++)
+#pragma unroll
+        for (int j = 0; j < 4; j++)
+#pragma unroll
+            for (int r = 0; r < 4; r++) C[(row0 + i * 16 + 4 * r) * rb + col0 + j * 16] = acc[i][j][r];
+}
+
+}  // namespace
+
+void launch_dgemm_tn(const double *AT, const double *Bd, double *C, uint64_t ra, uint64_t rb, uint32_t kp,
+                     hipStream_t s) {
+    const uint32_t nbm = (uint32_t)(ra / DK_BM), nbn = (uint32_t)(rb / DK_BN);
+    if (!nbm || !nbn || !kp) return;
+    k_dgemm_tn<<<nbm * nbn, 256, 0, s>>>(AT, Bd, C, nbm, nbn, ra, rb, kp);
+}
+
+void dense_keys_select(DevPool &pool, const Geom &g, const Index &ix, const uint32_t *fidA, size_t nA, float tau,
+                       uint64_t max_cells, bool require_gain, DenseKeys &dk, std::vector<std::shared_ptr<void>> &hold,
+                       hipStream_t s) {
+    dk = DenseKeys();
+    if (!ix.nfib || tau <= 0.0f) return;
+    const uint64_t ra = div_up(g.spanA, DK_BM) * DK_BM, rb = div_up(g.spanB, DK_BN) * DK_BN;
+    if (ra * rb > max_cells || ra / 64 > 65535) return;
+    auto keep = [&](size_t bytes) -> void * {
+        void *p = pool.get(bytes + 16);
+        DevPool *pl = &pool;
+        hold.emplace_back(p, [pl](void *q) { pl->put(q); });
+        return p;
+    };
+    const uint32_t nfib = ix.nfib;
+    Tmp<uint32_t> na(pool, nfib), flag(pool, nfib), idx(pool, nfib), dtot(pool, 1);
+    Tmp<unsigned long long> tot(pool, 2);
+    if (fidA) {
+        // nA <= spanA, so a selected key has nB >= tau * spanB
+        const double nbm = std::floor((double)tau * (double)g.spanB);
+        fill_u32(na, nfib, 0, s);
+        if (nA)
+            k_dk_na<<<grid_of(nA), 256, 0, s>>>(fidA, nA, ix.fiber_ptr,
+                                                 (uint32_t)std::min(nbm, 4294967295.0), na);
+    } else {
+        k_dk_na_self<<<grid_of(nfib), 256, 0, s>>>(ix.fiber_ptr, nfib, na);
+    }
+    const double inv_area = 1.0 / ((double)g.spanA * (double)g.spanB);
+    // at most this many keys, and their panels within 2 GiB
+    const uint64_t max_keys = std::min<uint64_t>(16384, (2ull << 30) / (8 * (ra + rb)));
+    double t = tau;
+    unsigned long long h[2] = {0, 0};
+    for (int it = 0; it < 40; it++) {
+        fill_u64(tot, 2, 0, s);
+        k_dk_flag<<<grid_of(nfib), 256, 0, s>>>(ix.fiber_ptr, na, nfib, inv_area, t, flag, tot);
+        FCC_HIP(hipMemcpyAsync(h, tot.get(), 16, hipMemcpyDeviceToHost, s));
+        FCC_HIP(hipStreamSynchronize(s));
+        if (h[0] <= max_keys) break;
+        t *= 1.25;
+    }
+    if (!h[0] || h[0] > max_keys) return;
+    if (require_gain && (double)h[1] < 4.0 * (double)g.spanA * (double)g.spanB) return;
+    dk.nh = (uint32_t)h[0];
+    dk.products = h[1];
+    dk.tau = (float)t;
+    dk.nw = (dk.nh + 63) / 64;
+    dk.kp = (uint32_t)div_up(dk.nh, DK_BK) * DK_BK;
+    dk.spanA = g.spanA;
+    dk.spanB = g.spanB;
+    dk.ra = ra;
+    dk.rb = rb;
+    scan_u32(pool, flag, idx, nfib, dtot, s);
+    dk.hkey = (uint32_t *)keep((size_t)nfib * 4);
+    dk.hfib = (uint32_t *)keep((size_t)dk.nh * 4);
+    k_dk_keys<<<grid_of(nfib), 256, 0, s>>>(flag, idx, nfib, dk.hkey, dk.hfib);
+}
+
+void dense_keys_build(DevPool &pool, const Geom &g, Index &ix, DenseKeys &dk, std::vector<std::shared_ptr<void>> &hold,
+                      hipStream_t s) {
+    if (!dk.nh) return;
+    auto keep = [&](size_t bytes) -> void * {
+        void *p = pool.get(bytes + 16);
+        DevPool *pl = &pool;
+        hold.emplace_back(p, [pl](void *q) { pl->put(q); });
+        return p;
+    };
+    const uint64_t ra = dk.ra, rb = dk.rb;
+    // the selected keys' groups
+    if (ix.ngroups) {
+        Tmp<uint32_t> gflag(pool, ix.ngroups), gidx(pool, ix.ngroups), gtot(pool, 1);
+        k_dk_gflag<<<grid_of(ix.ngroups), 256, 0, s>>>(ix.group_fiber, dk.hkey, ix.ngroups, gflag);
+        scan_u32(pool, gflag, gidx, ix.ngroups, gtot, s);
+        FCC_HIP(hipMemcpyAsync(&dk.ndg, gtot.get(), 4, hipMemcpyDeviceToHost, s));
+        FCC_HIP(hipStreamSynchronize(s));
+        dk.dgrp = (uint32_t *)keep((size_t)dk.ndg * 4);
+        k_dk_glist<<<grid_of(ix.ngroups), 256, 0, s>>>(gflag, gidx, ix.ngroups, dk.dgrp);
+    }
+    dk.tile_dk = (uint8_t *)keep((size_t)ix.natiles + 1);
+    fill_u32(reinterpret_cast<uint32_t *>(dk.tile_dk), ((size_t)ix.natiles + 1 + 3) / 4, 0, s);
+    dk.rowbits = (unsigned long long *)keep((size_t)g.spanA * dk.nw * 8);
+    dk.colbits = (unsigned long long *)keep((size_t)dk.nw * rb * 8);
+    fill_u64(dk.rowbits, (size_t)g.spanA * dk.nw, 0, s);
+    fill_u64(dk.colbits, (size_t)dk.nw * rb, 0, s);
+    if (dk.ndg)
+        k_dk_mask<<<grid_of(dk.ndg), 256, 0, s>>>(dk.dgrp, dk.ndg, ix.group_ptr, ix.group_fiber, ix.group_atile,
+                                                   ix.glight, dk.hkey, ix.Arow, dk.nw, ix.gdesc, ix.Afp, ix.st_direct,
+                                                   dk.tile_dk, dk.rowbits);
+    k_dk_colbits<<<dk.nh, 256, 0, s>>>(dk.hfib, ix.fiber_ptr, ix.Bcol, rb, dk.colbits);
+    dk.P = (uint32_t *)keep((size_t)(ra * (rb / 32) + 2) * 4);
+    fill_u32(dk.P + ra * (rb / 32), 2, 0, s);
+    k_dk_presence<<<dim3((unsigned)(rb / 64), (unsigned)(ra / 64)), 256, 0, s>>>(dk.rowbits, dk.colbits, g.spanA, rb,
+                                                                                 dk.nw, dk.P);
+    dk.AT = (double *)keep((size_t)dk.kp * ra * 8);
+    dk.Bd = (double *)keep((size_t)dk.kp * rb * 8);
+    dk.C = (double *)keep((size_t)ra * rb * 8);
+    // heavy segments from the masked descriptors
+    build_hseg(pool, ix, hold, s);
+}
+
+void dense_keys_values(const Index &ix, const DenseKeys &dk, hipStream_t s) {
+    if (!dk.nh) return;
+    fill_f64(dk.AT, (size_t)dk.kp * dk.ra, 0.0, s);
+    fill_f64(dk.Bd, (size_t)dk.kp * dk.rb, 0.0, s);
+    if (dk.ndg)
+        k_dk_scatter_A<<<grid_of(dk.ndg), 256, 0, s>>>(dk.dgrp, dk.ndg, ix.group_ptr, ix.group_fiber, dk.hkey, ix.Arow,
+                                                        ix.Aval, dk.ra, dk.AT);
+    k_dk_scatter_B<<<dk.nh, 256, 0, s>>>(dk.hfib, ix.fiber_ptr, ix.Bcol, ix.Bval, dk.rb, dk.Bd);
+    launch_dgemm_tn(dk.AT, dk.Bd, dk.C, dk.ra, dk.rb, dk.kp, s);
+}
+
+}  // namespace fcc

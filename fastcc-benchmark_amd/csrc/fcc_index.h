@@ -41,4 +41,8 @@ void build_left_self(DevPool &pool, const Geom &g, const Operand &A, const SelfJ
 void build_left(DevPool &pool, const Geom &g, const Operand &A, const uint32_t *rowA, const uint32_t *fidA,
                 uint32_t nvalid, int logTL, Index &ix, std::vector<std::shared_ptr<void>> &hold, hipStream_t s);
 
+// Heavy-group segments (hdesc, atile_h0) from the group descriptors; run
+// again after descriptors change (new buffers are kept in hold).
+void build_hseg(DevPool &pool, Index &ix, std::vector<std::shared_ptr<void>> &hold, hipStream_t s);
+
 }  // namespace fcc

@@ -597,6 +597,26 @@ void join_left(DevPool &pool, const Geom &g, const Operand &A, const Index &ix, 
 
 // Left tiles from the sorted composite keys (tile | class | fiber | row-in-tile)
 // and the original entry index of each sorted position.
+void build_hseg(DevPool &pool, Index &ix, std::vector<std::shared_ptr<void>> &hold, hipStream_t s) {
+    auto keep = [&](size_t bytes) -> void * {
+        void *p = pool.get(bytes + 16);
+        DevPool *pl = &pool;
+        hold.emplace_back(p, [pl](void *q) { pl->put(q); });
+        return p;
+    };
+    Tmp<uint32_t> nseg(pool, (size_t)ix.ngroups + 1), off(pool, (size_t)ix.ngroups + 1), nh(pool, 1);
+    if (ix.ngroups) k_hseg_count<<<grid_for(ix.ngroups), 256, 0, s>>>(ix.gdesc, ix.glight, ix.ngroups, ix.hseg, nseg);
+    fill_u32(nseg.get() + ix.ngroups, 1, 0, s);
+    scan_u32(pool, nseg, off, (size_t)ix.ngroups + 1, nh, s);
+    FCC_HIP(hipMemcpyAsync(&ix.nhseg, nh.get(), 4, hipMemcpyDeviceToHost, s));
+    FCC_HIP(hipStreamSynchronize(s));
+    ix.hdesc = (uint4 *)keep((size_t)ix.nhseg * 16);
+    ix.atile_h0 = (uint32_t *)keep(((size_t)ix.natiles + 1) * 4);
+    if (ix.ngroups)
+        k_hseg_emit<<<grid_for(ix.ngroups), 256, 0, s>>>(ix.gdesc, off, nseg, ix.ngroups, ix.hseg, ix.hdesc);
+    k_tile_hseg<<<grid_for((size_t)ix.natiles + 1), 256, 0, s>>>(ix.atile_g0, off, ix.natiles, ix.atile_h0);
+}
+
 static void left_from_sorted(DevPool &pool, const Operand &A, const uint64_t *ks, const uint32_t *vs, uint32_t nvalid,
                              int logTL, int fbits, Index &ix, std::vector<std::shared_ptr<void>> &hold,
                              hipStream_t s) {
@@ -632,7 +652,8 @@ static void left_from_sorted(DevPool &pool, const Operand &A, const uint64_t *ks
     ix.group_fiber = (uint32_t *)keep((size_t)ix.ngroups * 4);
     ix.group_atile = (uint32_t *)keep((size_t)ix.ngroups * 4);
     ix.gdesc = (uint4 *)keep((size_t)ix.ngroups * 16);
-    Tmp<uint8_t> glight(pool, ix.ngroups + 1);
+    ix.glight = (uint8_t *)keep((size_t)ix.ngroups + 1);
+    uint8_t *glight = ix.glight;
     ix.atile_id = (uint32_t *)keep((size_t)ix.natiles * 4);
     ix.atile_g0 = (uint32_t *)keep((size_t)(ix.natiles + 1) * 4);
     ix.atile_lg = (uint32_t *)keep((size_t)ix.natiles * 4);
@@ -643,20 +664,7 @@ static void left_from_sorted(DevPool &pool, const Operand &A, const uint64_t *ks
         k_tile_light_init<<<grid_for(ix.natiles), 256, 0, s>>>(ix.atile_g0, ix.natiles, ix.atile_lg);
         k_tile_light<<<grid_for(ix.ngroups), 256, 0, s>>>(glight, ix.group_atile, ix.atile_g0, ix.ngroups, ix.atile_lg);
     }
-    // heavy segments
-    {
-        Tmp<uint32_t> nseg(pool, (size_t)ix.ngroups + 1), off(pool, (size_t)ix.ngroups + 1), nh(pool, 1);
-        if (ix.ngroups) k_hseg_count<<<grid_for(ix.ngroups), 256, 0, s>>>(ix.gdesc, glight, ix.ngroups, ix.hseg, nseg);
-        fill_u32(nseg.get() + ix.ngroups, 1, 0, s);
-        scan_u32(pool, nseg, off, (size_t)ix.ngroups + 1, nh, s);
-        FCC_HIP(hipMemcpyAsync(&ix.nhseg, nh.get(), 4, hipMemcpyDeviceToHost, s));
-        FCC_HIP(hipStreamSynchronize(s));
-        ix.hdesc = (uint4 *)keep((size_t)ix.nhseg * 16);
-        ix.atile_h0 = (uint32_t *)keep(((size_t)ix.natiles + 1) * 4);
-        if (ix.ngroups) k_hseg_emit<<<grid_for(ix.ngroups), 256, 0, s>>>(ix.gdesc, off, nseg, ix.ngroups, ix.hseg,
-                                                            ix.hdesc);
-        k_tile_hseg<<<grid_for((size_t)ix.natiles + 1), 256, 0, s>>>(ix.atile_g0, off, ix.natiles, ix.atile_h0);
-    }
+    build_hseg(pool, ix, hold, s);
     // statistics
     size_t T = ix.natiles;
     ix.st_nnz = (uint32_t *)keep(T * 4);

@@ -147,6 +147,7 @@ struct Index {
     uint4 *gdesc = nullptr;           // per group: fiber start, fiber length, first entry, entries
     uint32_t *group_fiber = nullptr;
     uint32_t *group_atile = nullptr;
+    uint8_t *glight = nullptr;        // per group: 1 = light class (short fiber)
     uint32_t *atile_id = nullptr;     // tile index of active tile a
     uint32_t *atile_g0 = nullptr;     // natiles+1: first group of active tile a
     uint32_t *atile_lg = nullptr;     // first light group of active tile a
@@ -255,6 +256,15 @@ struct AccArgs {
     int row_own;               // dense units: row-owned plain updates (experimental)
     uint32_t claim_div;        // heavy claims: segments / claim_div per claim (0: 64 per claim)
     uint32_t seg_below;        // tiles with fewer heavy groups than this work on segments
+    // dense-key block over left rows x right columns: presence bits dk_P
+    // (row pitch dk_pw words) and values dk_C (row pitch dk_rb); units of the
+    // active tiles flagged in dk_tile start their accumulator from it.
+    // dk_tile == null: no block.
+    const uint32_t *dk_P;
+    const double *dk_C;
+    const uint8_t *dk_tile;
+    uint64_t dk_pw, dk_rb;
+    uint64_t spanA;
 };
 
 }  // namespace fcc

@@ -226,7 +226,7 @@ class Context:
 
     @staticmethod
     def options(planner="exact", tile_hint=0, sink=False, force_acc=FCC_ACC_AUTO, col_window=0, workgroups=0,
-                collect_stats=False, left_tile_stride=1, summarize=False):
+                collect_stats=False, left_tile_stride=1, summarize=False, dense_keys=0.0):
         o = _abi.FccOptions()
         _abi.load().fcc_options_default(C.byref(o))
         o.planner = FCC_PLAN_LEARNED if planner == "learned" else FCC_PLAN_EXACT
@@ -238,6 +238,8 @@ class Context:
         o.collect_stats = 1 if collect_stats else 0
         o.left_tile_stride = int(left_tile_stride)
         o.summarize = 1 if summarize else 0
+        # dense-key GEMM threshold: 0 default, < 0 off, > 0 the threshold
+        o.dense_keys = float(dense_keys)
         return o
 
     def contract(self, a, a_modes, b, b_modes, a_batch=(), b_batch=(), **opts) -> Result:

@@ -48,6 +48,7 @@ class FccOptions(C.Structure):
         ("collect_stats", C.c_int32),
         ("left_tile_stride", C.c_int32),
         ("summarize", C.c_int32),
+        ("dense_keys", C.c_float),
     ]
 
 
@@ -74,6 +75,9 @@ class FccStats(C.Structure):
         ("ms_numeric_kernel", C.c_double),
         ("ms_done", C.c_double),
         ("units_direct", C.c_uint64),
+        ("dense_keys", C.c_uint64),
+        ("dense_key_products", C.c_uint64),
+        ("ms_dense", C.c_double),
     ]
 
 

@@ -60,6 +60,11 @@ typedef struct {
                                  (x / T) % s == 0, T = tile_hint rounded up to a power of
                                  two (128 when 0): the sampled baseline of the largest shapes */
     int32_t summarize;   /* 1: also fill coord_hash / val_sum / val_wsum for a materialised result */
+    float dense_keys;    /* contraction keys with nA(k) * nB(k) >= dense_keys * spanA * spanB are
+                            contracted as one dense FP64 GEMM (matrix cores) instead of through the
+                            tile accumulators; same outputs. 0 = default threshold (0.02, and only
+                            when their products average four or more per block cell), > 0 = this
+                            threshold unconditionally, < 0 = never */
 } fcc_options;
 
 typedef struct {
@@ -85,6 +90,9 @@ typedef struct {
                              (device clock at the pair's last accumulate unit, plus the drain) */
     uint64_t units_direct; /* units of tiles whose entries sit on distinct rows: products written
                               straight out, no accumulator */
+    uint64_t dense_keys;   /* contraction keys taken by the dense GEMM (0: none) */
+    uint64_t dense_key_products; /* their products (part of products) */
+    double ms_dense;       /* value panels + dense GEMM */
 } fcc_stats;
 
 typedef struct {
