@@ -890,6 +890,14 @@ __device__ __forceinline__ uint32_t dk_word(const AccArgs &A, uint64_t row, uint
     return w;
 }
 
+// Block value at (row, col): the sum of the GEMM's k-split partial blocks.
+__device__ __forceinline__ double dk_value(const AccArgs &A, uint64_t row, uint64_t col) {
+    const double *p = A.dk_C + row * A.dk_rb + col;
+    double v = p[0];
+    for (uint32_t z = 1; z < A.dk_splits; z++) v += p[(uint64_t)z * A.dk_slab];
+    return v;
+}
+
 // Bitmap word w of a dense unit (rows of U.Wp slots) from the block.
 __device__ __forceinline__ uint32_t dk_dense_word(const AccArgs &A, const UnitCtx &U, uint32_t w) {
     const uint32_t wpr = U.Wp >> 5;
@@ -911,7 +919,7 @@ __device__ void dk_seed_dense(const AccArgs &A, const UnitCtx &U, double *fv, ui
         const uint64_t row = U.row0 + li, col = (uint64_t)U.c0 + cl;
         double v = init;
         if (cl < U.W && row < A.spanA && ((A.dk_P[row * A.dk_pw + (col >> 5)] >> (col & 31)) & 1u)) {
-            v = A.dk_C[row * A.dk_rb + col];
+            v = dk_value(A, row, col);
             if (!BITMAP) v = v + 0.0;
         }
         fv[s] = v;
@@ -941,11 +949,11 @@ __device__ void dk_seed_cells(const AccArgs &A, const UnitCtx &U, const Acc<NUM,
                 const int slot =
                     NUM ? lds_probe<12>(S.keys, S.cnt, LH_MAX, key) : lds_probe<14>(S.keys, S.cnt, SYM_MAX, key);
                 if (slot >= 0) {
-                    if (NUM) lds_add_f64(&S.fv[slot], A.dk_C[row * A.dk_rb + U.c0 + cl]);
+                    if (NUM) lds_add_f64(&S.fv[slot], dk_value(A, row, U.c0 + cl));
                     continue;
                 }
             }
-            if (NUM) ghash_add(A, gk, A.dk_C[row * A.dk_rb + U.c0 + cl], U.u);
+            if (NUM) ghash_add(A, gk, dk_value(A, row, U.c0 + cl), U.u);
             else gset_insert(A, gk, U.u);
         }
     }

@@ -563,7 +563,7 @@ int Contraction::run(const fcc_tensor *ta, const int32_t *am, const fcc_tensor *
     st.col_window = (int32_t)Wd;
     st.left_tiles = ix.natiles;
 
-    dense_keys_build(pool, g, ix, dk, hold, s);
+    dense_keys_build(pool, g, ix, dk, ctx->num_cus, hold, s);
     FCC_HIP(hipEventRecord(ev[7], s));
     dense_keys_values(ix, dk, s);
     FCC_HIP(hipEventRecord(ev[8], s));
@@ -672,6 +672,8 @@ int Contraction::run(const fcc_tensor *ta, const int32_t *am, const fcc_tensor *
         X.dk_tile = dk.tile_dk;
         X.dk_pw = dk.rb / 32;
         X.dk_rb = dk.rb;
+        X.dk_splits = dk.splits;
+        X.dk_slab = dk.ra * dk.rb;
     }
     X.nxa = g.nxa;
     X.nxb = g.nxb;

@@ -28,17 +28,19 @@ struct DenseKeys {
     uint64_t ra = 0, rb = 0;  // spanA, spanB rounded up to DK_BM, DK_BN
     uint64_t products = 0;    // products of the selected keys
     float tau = 0;            // density threshold used
+    bool self = false;        // self-contraction: AT is Bd, row bitsets are column bitsets
+    uint32_t splits = 1;      // GEMM k splits: C holds this many ra x rb slabs, summed by the readers
     uint32_t *hkey = nullptr;              // per fiber: selected key index, or ~0
     uint32_t *hfib = nullptr;              // per selected key: its fiber
     uint32_t ndg = 0;
     uint32_t *dgrp = nullptr;              // groups (tile, fiber) of the selected keys
     uint8_t *tile_dk = nullptr;            // per active tile: 1 if it holds a selected key
-    unsigned long long *rowbits = nullptr; // [spanA][nw]
+    unsigned long long *rowbits = nullptr; // [spanA][nw] (null for a self-contraction)
     unsigned long long *colbits = nullptr; // [nw][rb] (word-major: coalesced per column)
     uint32_t *P = nullptr;                 // presence [ra][rb / 32] (+2 words)
     double *AT = nullptr;                  // [kp][ra]
     double *Bd = nullptr;                  // [kp][rb]
-    double *C = nullptr;                   // [ra][rb]
+    double *C = nullptr;                   // [splits][ra][rb]
 };
 
 // Chooses the dense keys once the right fibers and the left join are known
@@ -56,17 +58,20 @@ void dense_keys_select(DevPool &pool, const Geom &g, const Index &ix, const uint
 // (group descriptors, light-entry fiber ranges, DIRECT eligibility; heavy
 // segments rebuilt), builds the key bitsets and the presence map, and
 // allocates the value panels and the block. Synchronises.
-void dense_keys_build(DevPool &pool, const Geom &g, Index &ix, DenseKeys &dk, std::vector<std::shared_ptr<void>> &hold,
-                      hipStream_t s);
+void dense_keys_build(DevPool &pool, const Geom &g, Index &ix, DenseKeys &dk, int num_cus,
+                      std::vector<std::shared_ptr<void>> &hold, hipStream_t s);
 
 // Values: scatters the selected keys' current values (ix.Aval, ix.Bval) into
 // the dense panels and runs the GEMM into dk.C. No memset nodes and no
 // allocation: safe to capture into a graph.
 void dense_keys_values(const Index &ix, const DenseKeys &dk, hipStream_t s);
 
-// C[ra][rb] = AT^T * Bd for row-major AT[kp][ra], Bd[kp][rb]; ra, rb
-// multiples of DK_BM / DK_BN, kp a multiple of DK_BK.
+// C[z][ra][rb] (z < splits) = partial sums of AT^T * Bd for row-major
+// AT[kp][ra], Bd[kp][rb]; ra, rb multiples of DK_BM / DK_BN, kp a multiple
+// of DK_BK; split z sums its share of the k steps.
 void launch_dgemm_tn(const double *AT, const double *Bd, double *C, uint64_t ra, uint64_t rb, uint32_t kp,
-                     hipStream_t s);
+                     uint32_t splits, hipStream_t s);
+// k splits that fill the GPU's workgroup slots for a small grid (1 for large).
+uint32_t dgemm_splits(uint64_t ra, uint64_t rb, uint32_t kp, int num_cus);
 
 }  // namespace fcc

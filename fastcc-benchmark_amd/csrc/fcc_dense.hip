@@ -105,6 +105,7 @@ __global__ void k_dk_mask(const uint32_t *dgrp, uint32_t ndg, const uint32_t *gr
     }
     st_direct[a] = 0;
     tile_dk[a] = 1;
+    if (!rowbits) return;  // self-contraction: the row bitsets are the column bitsets
     const unsigned long long bit = 1ull << (h & 63);
     for (uint32_t e = e0; e < e1; e++) atomicOr(&rowbits[(uint64_t)Arow[e] * nw + (h >> 6)], bit);
 }
@@ -122,6 +123,8 @@ __global__ __launch_bounds__(256) void k_dk_colbits(const uint32_t *hfib, const 
 // Workgroup: 64 rows x 64 columns; wavefront: 16 rows, a lane per column,
 // its column's bitset words held in registers 8 at a time; the 64 rows'
 // bitset words are staged in LDS 8 words per row at a time (broadcast reads).
+// rowbits == null: a self-contraction, whose row bitsets are its column
+// bitsets (colbits word-major, read transposed).
 __global__ __launch_bounds__(256) void k_dk_presence(const unsigned long long *rowbits,
                                                     const unsigned long long *colbits, uint64_t spanA, uint64_t rb,
                                                     uint32_t nw, uint32_t *P) {
@@ -136,7 +139,10 @@ __global__ __launch_bounds__(256) void k_dk_presence(const unsigned long long *r
         for (uint32_t q = threadIdx.x; q < 64 * 8; q += 256) {
             const uint32_t rr = q >> 3, j = q & 7;
             const uint64_t row = rb0 + rr;
-            s_rw[rr][j] = (row < spanA && w0 + j < nw) ? rowbits[row * nw + w0 + j] : 0ull;
+            unsigned long long v = 0ull;
+            if (row < spanA && w0 + j < nw)
+                v = rowbits ? rowbits[row * nw + w0 + j] : colbits[(uint64_t)(w0 + j) * rb + row];
+            s_rw[rr][j] = v;
         }
         __syncthreads();
         unsigned long long cb[8];
@@ -191,16 +197,19 @@ __global__ __launch_bounds__(256) void k_dk_scatter_B(const uint32_t *hfib, cons
 // j + 16 * (i % 4), element i / 4.
 __global__ __launch_bounds__(256, 2) void k_dgemm_tn(const double *__restrict__ AT, const double *__restrict__ Bd,
                                                      double *__restrict__ C, uint32_t nbm, uint32_t nbn, uint64_t ra,
-                                                     uint64_t rb, uint32_t kp) {
+                                                     uint64_t rb, uint32_t kp, uint32_t splits) {
     __shared__ __align__(16) double sA[2][DK_BK][DK_BM];
     __shared__ __align__(16) double sB[2][DK_BK][DK_BN];
     // Workgroups are dealt round-robin over the 8 XCDs (one L2 each): give
     // every XCD a contiguous run of logical tiles, and walk logical tiles down
     // bands of 8 row tiles so neighbours share AT rows and a Bd column panel.
-    const uint32_t nblk = nbm * nbn;
+    // With split k (small grids) workgroup z of a tile sums k steps
+    // [z * nk / splits, (z + 1) * nk / splits) into its own slab of C.
+    const uint32_t nblk = nbm * nbn, total = nblk * splits;
     const uint32_t x = blockIdx.x & 7, jx = blockIdx.x >> 3;
-    const uint32_t q = nblk >> 3, rem = nblk & 7;
-    const uint32_t L = x * q + min(x, rem) + jx;
+    const uint32_t q = total >> 3, rem = total & 7;
+    const uint32_t Lz = x * q + min(x, rem) + jx;
+    const uint32_t z = Lz / nblk, L = Lz - z * nblk;
     const uint32_t band = 8, per_band = band * nbn;
     const uint32_t bnd = L / per_band, w = L - bnd * per_band;
     const uint32_t rows_in = min(band, nbm - bnd * band);
@@ -209,8 +218,11 @@ __global__ __launch_bounds__(256, 2) void k_dgemm_tn(const double *__restrict__ 
     const uint32_t t = threadIdx.x, lane = t & 63, wave = t >> 6;
     const uint32_t wm = (wave >> 1) * 64, wn = (wave & 1) * 64;
     const uint32_t lr = t >> 4, lc = (t & 15) * 8;  // panel copy: k row lr, 8 doubles from lc
-    const double *gA = AT + (uint64_t)lr * ra + (uint64_t)tm * DK_BM + lc;
-    const double *gB = Bd + (uint64_t)lr * rb + (uint64_t)tn * DK_BN + lc;
+    const uint32_t nk = kp / DK_BK;
+    const uint32_t kb0 = (uint32_t)((uint64_t)z * nk / splits), kb1 = (uint32_t)((uint64_t)(z + 1) * nk / splits);
+    const double *gA = AT + ((uint64_t)kb0 * DK_BK + lr) * ra + (uint64_t)tm * DK_BM + lc;
+    const double *gB = Bd + ((uint64_t)kb0 * DK_BK + lr) * rb + (uint64_t)tn * DK_BN + lc;
+    C += (uint64_t)z * ra * rb;
     f64x4 acc[4][4];
 #pragma unroll
     for (int i = 0; i < 4; i++)
@@ -228,13 +240,13 @@ __global__ __launch_bounds__(256, 2) void k_dgemm_tn(const double *__restrict__ 
         *reinterpret_cast<double2 *>(&sB[0][lr][lc + 2 * v]) = pb[v];
     }
     __syncthreads();
-    const uint32_t nk = kp / DK_BK;
     const uint32_t kr = lane >> 4, ci = lane & 15;
-    for (uint32_t kb = 0; kb < nk; kb++) {
+    const uint32_t nks = kb1 - kb0;
+    for (uint32_t kb = 0; kb < nks; kb++) {
         const uint32_t cur = kb & 1;
         // the next panels (the last step reloads its own: no branch, so the
         // staging registers stay registers; the idle buffer takes the copy)
-        const uint32_t kn = kb + 1 < nk ? kb + 1 : kb;
+        const uint32_t kn = kb + 1 < nks ? kb + 1 : kb;
         {
             const uint64_t oa = (uint64_t)kn * DK_BK * ra, ob = (uint64_t)kn * DK_BK * rb;
 #pragma unroll
@@ -275,10 +287,29 @@ __global__ __launch_bounds__(256, 2) void k_dgemm_tn(const double *__restrict__ 
 }  // namespace
 
 void launch_dgemm_tn(const double *AT, const double *Bd, double *C, uint64_t ra, uint64_t rb, uint32_t kp,
-                     hipStream_t s) {
+                     uint32_t splits, hipStream_t s) {
     const uint32_t nbm = (uint32_t)(ra / DK_BM), nbn = (uint32_t)(rb / DK_BN);
     if (!nbm || !nbn || !kp) return;
-    k_dgemm_tn<<<nbm * nbn, 256, 0, s>>>(AT, Bd, C, nbm, nbn, ra, rb, kp);
+    k_dgemm_tn<<<nbm * nbn * splits, 256, 0, s>>>(AT, Bd, C, nbm, nbn, ra, rb, kp, splits);
+}
+
+uint32_t dgemm_splits(uint64_t ra, uint64_t rb, uint32_t kp, int num_cus) {
+    // two workgroups per CU; small grids split k so the last wave of
+    // workgroups is nearly full, keeping at least 8 k steps per split
+    const uint64_t nblk = (ra / DK_BM) * (rb / DK_BN), slots = 2ull * (uint64_t)std::max(1, num_cus);
+    const uint32_t nk = kp / DK_BK;
+    if (nblk >= 4 * slots) return 1;
+    uint32_t best = 1;
+    double best_eff = 0;
+    for (uint32_t sp = 1; sp <= 8 && nk / sp >= 8; sp++) {
+        const uint64_t t = nblk * sp;
+        const double eff = (double)t / (double)(div_up(t, slots) * slots);
+        if (eff > best_eff + 0.02) {
+            best_eff = eff;
+            best = sp;
+        }
+    }
+    return best;
 }
 
 void dense_keys_select(DevPool &pool, const Geom &g, const Index &ix, const uint32_t *fidA, size_t nA, float tau,
@@ -331,14 +362,15 @@ void dense_keys_select(DevPool &pool, const Geom &g, const Index &ix, const uint
     dk.spanB = g.spanB;
     dk.ra = ra;
     dk.rb = rb;
+    dk.self = fidA == nullptr;
     scan_u32(pool, flag, idx, nfib, dtot, s);
     dk.hkey = (uint32_t *)keep((size_t)nfib * 4);
     dk.hfib = (uint32_t *)keep((size_t)dk.nh * 4);
     k_dk_keys<<<grid_of(nfib), 256, 0, s>>>(flag, idx, nfib, dk.hkey, dk.hfib);
 }
 
-void dense_keys_build(DevPool &pool, const Geom &g, Index &ix, DenseKeys &dk, std::vector<std::shared_ptr<void>> &hold,
-                      hipStream_t s) {
+void dense_keys_build(DevPool &pool, const Geom &g, Index &ix, DenseKeys &dk, int num_cus,
+                      std::vector<std::shared_ptr<void>> &hold, hipStream_t s) {
     if (!dk.nh) return;
     auto keep = [&](size_t bytes) -> void * {
         void *p = pool.get(bytes + 16);
@@ -359,10 +391,12 @@ void dense_keys_build(DevPool &pool, const Geom &g, Index &ix, DenseKeys &dk, st
     }
     dk.tile_dk = (uint8_t *)keep((size_t)ix.natiles + 1);
     fill_u32(reinterpret_cast<uint32_t *>(dk.tile_dk), ((size_t)ix.natiles + 1 + 3) / 4, 0, s);
-    dk.rowbits = (unsigned long long *)keep((size_t)g.spanA * dk.nw * 8);
     dk.colbits = (unsigned long long *)keep((size_t)dk.nw * rb * 8);
-    fill_u64(dk.rowbits, (size_t)g.spanA * dk.nw, 0, s);
     fill_u64(dk.colbits, (size_t)dk.nw * rb, 0, s);
+    if (!dk.self) {
+        dk.rowbits = (unsigned long long *)keep((size_t)g.spanA * dk.nw * 8);
+        fill_u64(dk.rowbits, (size_t)g.spanA * dk.nw, 0, s);
+    }
     if (dk.ndg)
         k_dk_mask<<<grid_of(dk.ndg), 256, 0, s>>>(dk.dgrp, dk.ndg, ix.group_ptr, ix.group_fiber, ix.group_atile,
                                                    ix.glight, dk.hkey, ix.Arow, dk.nw, ix.gdesc, ix.Afp, ix.st_direct,
@@ -372,22 +406,23 @@ void dense_keys_build(DevPool &pool, const Geom &g, Index &ix, DenseKeys &dk, st
     fill_u32(dk.P + ra * (rb / 32), 2, 0, s);
     k_dk_presence<<<dim3((unsigned)(rb / 64), (unsigned)(ra / 64)), 256, 0, s>>>(dk.rowbits, dk.colbits, g.spanA, rb,
                                                                                  dk.nw, dk.P);
-    dk.AT = (double *)keep((size_t)dk.kp * ra * 8);
     dk.Bd = (double *)keep((size_t)dk.kp * rb * 8);
-    dk.C = (double *)keep((size_t)ra * rb * 8);
+    dk.AT = dk.self ? dk.Bd : (double *)keep((size_t)dk.kp * ra * 8);  // self: the same panel
+    dk.splits = dgemm_splits(ra, rb, dk.kp, num_cus);
+    dk.C = (double *)keep((size_t)dk.splits * ra * rb * 8);
     // heavy segments from the masked descriptors
     build_hseg(pool, ix, hold, s);
 }
 
 void dense_keys_values(const Index &ix, const DenseKeys &dk, hipStream_t s) {
     if (!dk.nh) return;
-    fill_f64(dk.AT, (size_t)dk.kp * dk.ra, 0.0, s);
     fill_f64(dk.Bd, (size_t)dk.kp * dk.rb, 0.0, s);
-    if (dk.ndg)
+    if (!dk.self) fill_f64(dk.AT, (size_t)dk.kp * dk.ra, 0.0, s);
+    if (dk.ndg && !dk.self)
         k_dk_scatter_A<<<grid_of(dk.ndg), 256, 0, s>>>(dk.dgrp, dk.ndg, ix.group_ptr, ix.group_fiber, dk.hkey, ix.Arow,
                                                         ix.Aval, dk.ra, dk.AT);
     k_dk_scatter_B<<<dk.nh, 256, 0, s>>>(dk.hfib, ix.fiber_ptr, ix.Bcol, ix.Bval, dk.rb, dk.Bd);
-    launch_dgemm_tn(dk.AT, dk.Bd, dk.C, dk.ra, dk.rb, dk.kp, s);
+    launch_dgemm_tn(dk.AT, dk.Bd, dk.C, dk.ra, dk.rb, dk.kp, dk.splits, s);
 }
 
 }  // namespace fcc

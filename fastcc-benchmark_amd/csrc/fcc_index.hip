@@ -327,6 +327,24 @@ __device__ inline uint32_t hash32(uint32_t x) {
     return x;
 }
 
+// Per fiber: two min-hashes over up to 4 evenly spaced columns (fibers are
+// never empty). A group's sketch is its fiber's, so it is computed once per
+// fiber rather than once per (tile, fiber) group.
+__global__ void k_fiber_sketch(const uint32_t *fiber_ptr, const uint32_t *Bcol, uint32_t nfib, uint2 *fsk) {
+    const uint32_t f = blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= nfib) return;
+    const uint32_t f0 = fiber_ptr[f], f1 = fiber_ptr[f + 1];
+    const uint32_t step = (f1 - f0 + 3) / 4;
+    uint32_t h1 = 0xFFFFFFFFu, h2 = 0xFFFFFFFFu;
+    for (uint32_t p = f0; p < f1; p += step) {
+        const uint32_t c = Bcol[p];
+        const uint32_t x1 = hash32(c * 2u + 1u), x2 = hash32(c ^ 0x5bd1e995u);
+        h1 = x1 < h1 ? x1 : h1;
+        h2 = x2 < h2 ? x2 : h2;
+    }
+    fsk[f] = make_uint2(h1, h2);
+}
+
 // Per active tile statistics, one thread per group, wave-aggregated atomics.
 //  nnz      A entries of the tile          (num_nnz_in_tile, index_tensors.hpp:562)
 //  keys     distinct contraction keys      (num_active_columns, index_tensors.hpp:604)
@@ -335,7 +353,7 @@ __device__ inline uint32_t hash32(uint32_t x) {
 //  sketch   two min-hashes over sampled fiber columns (distinct-column estimate)
 __global__ __launch_bounds__(256) void k_tile_stats(const uint32_t *group_ptr, const uint32_t *group_fiber,
                                                    const uint32_t *group_atile, const uint32_t *fiber_ptr,
-                                                   const uint32_t *Bcol, uint32_t ngroups, uint32_t *st_nnz,
+                                                   const uint2 *fsk, uint32_t ngroups, uint32_t *st_nnz,
                                                    uint32_t *st_keys, unsigned long long *st_sumfib,
                                                    unsigned long long *st_products, uint32_t *st_sketch,
                                                    unsigned long long *st_maxgp) {
@@ -351,16 +369,10 @@ __global__ __launch_bounds__(256) void k_tile_stats(const uint32_t *group_ptr, c
     if (ok) {
         m = group_ptr[g + 1] - group_ptr[g];
         uint32_t f = group_fiber[g];
-        uint32_t f0 = fiber_ptr[f], f1 = fiber_ptr[f + 1];
-        nf = f1 - f0;
-        // sample up to 4 evenly spaced columns of the fiber (fibers are never empty)
-        uint32_t step = (uint32_t)((nf + 3) / 4);
-        for (uint32_t p = f0; p < f1; p += step) {
-            uint32_t c = Bcol[p];
-            uint32_t x1 = hash32(c * 2u + 1u), x2 = hash32(c ^ 0x5bd1e995u);
-            h1 = x1 < h1 ? x1 : h1;
-            h2 = x2 < h2 ? x2 : h2;
-        }
+        nf = fiber_ptr[f + 1] - fiber_ptr[f];
+        const uint2 hh = fsk[f];
+        h1 = hh.x;
+        h2 = hh.y;
     }
     if (threadIdx.x == 0) s_a0 = a;
     __syncthreads();
@@ -680,8 +692,10 @@ static void left_from_sorted(DevPool &pool, const Operand &A, const uint64_t *ks
         fill_u64((unsigned long long *)ix.st_products, T, 0, s);
         fill_u32(ix.st_sketch, 2 * T, 0xFFFFFFFFu, s);
         fill_u64((unsigned long long *)ix.st_maxgp, T, 0, s);
+        Tmp<uint2> fsk(pool, ix.nfib);
+        if (ix.nfib) k_fiber_sketch<<<grid_for(ix.nfib), 256, 0, s>>>(ix.fiber_ptr, ix.Bcol, ix.nfib, fsk);
         k_tile_stats<<<grid_for(ix.ngroups), 256, 0, s>>>(ix.group_ptr, ix.group_fiber, ix.group_atile, ix.fiber_ptr,
-                                                          ix.Bcol, ix.ngroups, ix.st_nnz, ix.st_keys,
+                                                          fsk, ix.ngroups, ix.st_nnz, ix.st_keys,
                                                           (unsigned long long *)ix.st_sumfib,
                                                           (unsigned long long *)ix.st_products, ix.st_sketch,
                                                           (unsigned long long *)ix.st_maxgp);

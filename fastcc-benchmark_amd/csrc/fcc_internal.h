@@ -264,6 +264,8 @@ struct AccArgs {
     const double *dk_C;
     const uint8_t *dk_tile;
     uint64_t dk_pw, dk_rb;
+    uint64_t dk_slab;    // dk_C holds dk_splits partial blocks dk_slab doubles apart; a value is their sum
+    uint32_t dk_splits;
     uint64_t spanA;
 };
 
