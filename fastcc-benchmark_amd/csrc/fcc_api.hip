@@ -31,6 +31,7 @@ using namespace fcc;
 struct fcc_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
+    hipStream_t side = nullptr;  // second stream (default priority): the dense-key block beside the left tiles
     DevPool pool;
     std::string err;
     int num_cus = 256;
@@ -548,6 +549,28 @@ int Contraction::run(const fcc_tensor *ta, const int32_t *am, const fcc_tensor *
         dense_keys_select(pool, g, ix, self_fib ? nullptr : fidA.get(), (size_t)A.nnz, dflt ? 0.02f : o.dense_keys,
                           max_cells, dflt, dk, hold, s);
     }
+    // The block depends only on the right fibers and the join: it runs on
+    // the side stream while the left tiles are built. Its buffers are all
+    // allocated before the fork; the join arrays live until the join below
+    // (side_join synchronises the side stream on any early exit first).
+    struct SideJoin {
+        hipStream_t st = nullptr;
+        ~SideJoin() {
+            if (st) (void)hipStreamSynchronize(st);
+        }
+    } side_join;
+    if (dk.nh) {
+        dense_keys_alloc(pool, g, dk, ctx->num_cus, hold);
+        FCC_HIP(hipEventRecord(ev[9], s));
+        FCC_HIP(hipStreamWaitEvent(ctx->side, ev[9], 0));
+        side_join.st = ctx->side;
+        FCC_HIP(hipEventRecord(ev[7], ctx->side));
+        dense_keys_block(ix, dk, fidA.get(), rowA.get(), A.vals, (size_t)A.nnz, ctx->side);
+        FCC_HIP(hipEventRecord(ev[8], ctx->side));
+    } else {
+        FCC_HIP(hipEventRecord(ev[7], s));
+        FCC_HIP(hipEventRecord(ev[8], s));
+    }
     int logTL;
     uint32_t Wd;
     choose_tiles(ctx, o, g, products, nvalid, dk.nh > 0, &logTL, &Wd, &bucket, &tuned_wg);
@@ -557,16 +580,14 @@ int Contraction::run(const fcc_tensor *ta, const int32_t *am, const fcc_tensor *
     else build_left(pool, g, A, rowA, fidA, nvalid, logTL, ix, hold, s);
     bfib.reset();
     vsB.reset();
-    rowA.reset();
-    fidA.reset();
     st.row_tile = (int32_t)ix.TL;
     st.col_window = (int32_t)Wd;
     st.left_tiles = ix.natiles;
 
-    dense_keys_build(pool, g, ix, dk, ctx->num_cus, hold, s);
-    FCC_HIP(hipEventRecord(ev[7], s));
-    dense_keys_values(ix, dk, s);
-    FCC_HIP(hipEventRecord(ev[8], s));
+    dense_keys_mask(pool, ix, dk, hold, s);
+    if (dk.nh) FCC_HIP(hipStreamWaitEvent(s, ev[8], 0));  // join the block
+    rowA.reset();
+    fidA.reset();
     st.dense_keys = dk.nh;
     st.dense_key_products = dk.products;
     FCC_HIP(hipEventRecord(ev[1], s));
@@ -1031,6 +1052,7 @@ int fcc_create(fcc_ctx **out, int device) {
     try {
         FCC_HIP(hipSetDevice(device));
         FCC_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+        FCC_HIP(hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking));
         for (auto &e : c->ev) FCC_HIP(hipEventCreate(&e));
         hipDeviceProp_t prop;
         FCC_HIP(hipGetDeviceProperties(&prop, device));
@@ -1043,6 +1065,7 @@ int fcc_create(fcc_ctx **out, int device) {
         for (auto &ev : c->ev)
             if (ev) (void)hipEventDestroy(ev);
         if (c->stream) (void)hipStreamDestroy(c->stream);
+        if (c->side) (void)hipStreamDestroy(c->side);
         delete c;
         return FCC_ERR_HIP;
     }
@@ -1054,11 +1077,13 @@ void fcc_destroy(fcc_ctx *c) {
     if (!c) return;
     (void)hipSetDevice(c->device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
+    if (c->side) (void)hipStreamSynchronize(c->side);
     c->pool.release_all();
     if (c->mlp_dev) (void)hipFree(c->mlp_dev);
     for (auto &e : c->ev)
         if (e) (void)hipEventDestroy(e);
     if (c->stream) (void)hipStreamDestroy(c->stream);
+    if (c->side) (void)hipStreamDestroy(c->side);
     delete c;
 }
 

@@ -54,15 +54,27 @@ void dense_keys_select(DevPool &pool, const Geom &g, const Index &ix, const uint
                        uint64_t max_cells, bool require_gain, DenseKeys &dk, std::vector<std::shared_ptr<void>> &hold,
                        hipStream_t s);
 
+// Allocates the bitsets, presence map, value panels and block (host only,
+// no kernels: everything the block needs exists before it is launched, so it
+// can run on another stream while the left tiles are built).
+void dense_keys_alloc(DevPool &pool, const Geom &g, DenseKeys &dk, int num_cus,
+                      std::vector<std::shared_ptr<void>> &hold);
+
+// The block from the right fibers and the left join (fiber and row per left
+// entry, nA entries; unused for a self-contraction): column (and row)
+// bitsets, presence map, value panels, GEMM. Reads nothing the left-tile
+// build writes; fidA/rowA/Avals must stay valid until it completes.
+void dense_keys_block(const Index &ix, const DenseKeys &dk, const uint32_t *fidA, const uint32_t *rowA,
+                      const double *Avals, size_t nA, hipStream_t s);
+
 // After the left tiles: takes the selected keys out of the sparse traversal
 // (group descriptors, light-entry fiber ranges, DIRECT eligibility; heavy
-// segments rebuilt), builds the key bitsets and the presence map, and
-// allocates the value panels and the block. Synchronises.
-void dense_keys_build(DevPool &pool, const Geom &g, Index &ix, DenseKeys &dk, int num_cus,
-                      std::vector<std::shared_ptr<void>> &hold, hipStream_t s);
+// segments rebuilt) and flags the tiles that hold them. Synchronises.
+void dense_keys_mask(DevPool &pool, Index &ix, DenseKeys &dk, std::vector<std::shared_ptr<void>> &hold,
+                     hipStream_t s);
 
-// Values: scatters the selected keys' current values (ix.Aval, ix.Bval) into
-// the dense panels and runs the GEMM into dk.C. No memset nodes and no
+// Values again (plans): scatters the selected keys' current values (ix.Aval,
+// ix.Bval, index order) into the panels and runs the GEMM into dk.C. No memset nodes and no
 // allocation: safe to capture into a graph.
 void dense_keys_values(const Index &ix, const DenseKeys &dk, hipStream_t s);
 

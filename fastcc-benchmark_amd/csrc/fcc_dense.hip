@@ -168,6 +168,21 @@ __global__ __launch_bounds__(256) void k_dk_presence(const unsigned long long *r
     }
 }
 
+// From the left join (before the left tiles exist): row bitsets and the
+// left panel, one thread per left entry.
+__global__ void k_dk_join_A(const uint32_t *fidA, const uint32_t *rowA, const double *vals, size_t n,
+                            const uint32_t *hkey, uint32_t nw, uint64_t ra, unsigned long long *rowbits, double *AT) {
+    const size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+    if (e >= n) return;
+    const uint32_t f = fidA[e];
+    if (f == 0xFFFFFFFFu) return;
+    const uint32_t h = hkey[f];
+    if (h == 0xFFFFFFFFu) return;
+    const uint32_t row = rowA[e];
+    atomicOr(&rowbits[(uint64_t)row * nw + (h >> 6)], 1ull << (h & 63));
+    atomicAdd(&AT[(uint64_t)h * ra + row], vals[e]);
+}
+
 __global__ void k_dk_scatter_A(const uint32_t *dgrp, uint32_t ndg, const uint32_t *group_ptr,
                                const uint32_t *group_fiber, const uint32_t *hkey, const uint32_t *Arow,
                                const double *Aval, uint64_t ra, double *AT) {
@@ -369,8 +384,8 @@ void dense_keys_select(DevPool &pool, const Geom &g, const Index &ix, const uint
     k_dk_keys<<<grid_of(nfib), 256, 0, s>>>(flag, idx, nfib, dk.hkey, dk.hfib);
 }
 
-void dense_keys_build(DevPool &pool, const Geom &g, Index &ix, DenseKeys &dk, int num_cus,
-                      std::vector<std::shared_ptr<void>> &hold, hipStream_t s) {
+void dense_keys_alloc(DevPool &pool, const Geom &g, DenseKeys &dk, int num_cus,
+                      std::vector<std::shared_ptr<void>> &hold) {
     if (!dk.nh) return;
     auto keep = [&](size_t bytes) -> void * {
         void *p = pool.get(bytes + 16);
@@ -379,6 +394,44 @@ void dense_keys_build(DevPool &pool, const Geom &g, Index &ix, DenseKeys &dk, in
         return p;
     };
     const uint64_t ra = dk.ra, rb = dk.rb;
+    dk.colbits = (unsigned long long *)keep((size_t)dk.nw * rb * 8);
+    if (!dk.self) dk.rowbits = (unsigned long long *)keep((size_t)g.spanA * dk.nw * 8);
+    dk.P = (uint32_t *)keep((size_t)(ra * (rb / 32) + 2) * 4);
+    dk.Bd = (double *)keep((size_t)dk.kp * rb * 8);
+    dk.AT = dk.self ? dk.Bd : (double *)keep((size_t)dk.kp * ra * 8);  // self: the same panel
+    dk.splits = dgemm_splits(ra, rb, dk.kp, num_cus);
+    dk.C = (double *)keep((size_t)dk.splits * ra * rb * 8);
+}
+
+void dense_keys_block(const Index &ix, const DenseKeys &dk, const uint32_t *fidA, const uint32_t *rowA,
+                      const double *Avals, size_t nA, hipStream_t s) {
+    if (!dk.nh) return;
+    const uint64_t ra = dk.ra, rb = dk.rb;
+    fill_u64(dk.colbits, (size_t)dk.nw * rb, 0, s);
+    fill_u32(dk.P + ra * (rb / 32), 2, 0, s);
+    fill_f64(dk.Bd, (size_t)dk.kp * rb, 0.0, s);
+    k_dk_colbits<<<dk.nh, 256, 0, s>>>(dk.hfib, ix.fiber_ptr, ix.Bcol, rb, dk.colbits);
+    k_dk_scatter_B<<<dk.nh, 256, 0, s>>>(dk.hfib, ix.fiber_ptr, ix.Bcol, ix.Bval, rb, dk.Bd);
+    if (!dk.self) {
+        fill_u64(dk.rowbits, (size_t)dk.spanA * dk.nw, 0, s);
+        fill_f64(dk.AT, (size_t)dk.kp * ra, 0.0, s);
+        if (nA)
+            k_dk_join_A<<<grid_of(nA), 256, 0, s>>>(fidA, rowA, Avals, nA, dk.hkey, dk.nw, ra, dk.rowbits, dk.AT);
+    }
+    k_dk_presence<<<dim3((unsigned)(rb / 64), (unsigned)(ra / 64)), 256, 0, s>>>(dk.rowbits, dk.colbits, dk.spanA, rb,
+                                                                                 dk.nw, dk.P);
+    launch_dgemm_tn(dk.AT, dk.Bd, dk.C, ra, rb, dk.kp, dk.splits, s);
+}
+
+void dense_keys_mask(DevPool &pool, Index &ix, DenseKeys &dk, std::vector<std::shared_ptr<void>> &hold,
+                     hipStream_t s) {
+    if (!dk.nh) return;
+    auto keep = [&](size_t bytes) -> void * {
+        void *p = pool.get(bytes + 16);
+        DevPool *pl = &pool;
+        hold.emplace_back(p, [pl](void *q) { pl->put(q); });
+        return p;
+    };
     // the selected keys' groups
     if (ix.ngroups) {
         Tmp<uint32_t> gflag(pool, ix.ngroups), gidx(pool, ix.ngroups), gtot(pool, 1);
@@ -391,25 +444,10 @@ void dense_keys_build(DevPool &pool, const Geom &g, Index &ix, DenseKeys &dk, in
     }
     dk.tile_dk = (uint8_t *)keep((size_t)ix.natiles + 1);
     fill_u32(reinterpret_cast<uint32_t *>(dk.tile_dk), ((size_t)ix.natiles + 1 + 3) / 4, 0, s);
-    dk.colbits = (unsigned long long *)keep((size_t)dk.nw * rb * 8);
-    fill_u64(dk.colbits, (size_t)dk.nw * rb, 0, s);
-    if (!dk.self) {
-        dk.rowbits = (unsigned long long *)keep((size_t)g.spanA * dk.nw * 8);
-        fill_u64(dk.rowbits, (size_t)g.spanA * dk.nw, 0, s);
-    }
     if (dk.ndg)
         k_dk_mask<<<grid_of(dk.ndg), 256, 0, s>>>(dk.dgrp, dk.ndg, ix.group_ptr, ix.group_fiber, ix.group_atile,
                                                    ix.glight, dk.hkey, ix.Arow, dk.nw, ix.gdesc, ix.Afp, ix.st_direct,
-                                                   dk.tile_dk, dk.rowbits);
-    k_dk_colbits<<<dk.nh, 256, 0, s>>>(dk.hfib, ix.fiber_ptr, ix.Bcol, rb, dk.colbits);
-    dk.P = (uint32_t *)keep((size_t)(ra * (rb / 32) + 2) * 4);
-    fill_u32(dk.P + ra * (rb / 32), 2, 0, s);
-    k_dk_presence<<<dim3((unsigned)(rb / 64), (unsigned)(ra / 64)), 256, 0, s>>>(dk.rowbits, dk.colbits, g.spanA, rb,
-                                                                                 dk.nw, dk.P);
-    dk.Bd = (double *)keep((size_t)dk.kp * rb * 8);
-    dk.AT = dk.self ? dk.Bd : (double *)keep((size_t)dk.kp * ra * 8);  // self: the same panel
-    dk.splits = dgemm_splits(ra, rb, dk.kp, num_cus);
-    dk.C = (double *)keep((size_t)dk.splits * ra * rb * 8);
+                                                   dk.tile_dk, nullptr);
     // heavy segments from the masked descriptors
     build_hseg(pool, ix, hold, s);
 }
