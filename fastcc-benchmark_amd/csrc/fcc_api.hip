@@ -170,9 +170,13 @@ void choose_tiles(fcc_ctx *ctx, const fcc_options &o, const Geom &g, uint64_t pr
         TL = (uint64_t)te->row_tile;
     } else if (dense_keys) {
         // the dense keys' products are gone: what stays is many short
-        // fibers, and every column window rescans the tile's entries, so
-        // two-row tiles with the widest windows
-        TL = 2;
+        // fibers, and every column window rescans the tile's entries: as many
+        // rows as fit with whole rows in one window, else two-row tiles with
+        // the widest windows
+        const uint64_t sp = (g.spanB + 31) & ~31ull;
+        TL = 1;
+        while (TL * 2 * sp <= DENSE_SLOTS && TL < 16) TL *= 2;
+        if (TL < 2) TL = 2;
     } else if (dens >= 0.25) {
         const uint64_t sp = (g.spanB + 31) & ~31ull;  // dense rows are padded to 32 columns
         uint64_t tf = 1;
@@ -197,6 +201,9 @@ void choose_tiles(fcc_ctx *ctx, const fcc_options &o, const Geom &g, uint64_t pr
     if (W == 0) {
         W = 32;
         while (W * 2 * TL <= DENSE_SLOTS) W *= 2;
+        // whole (padded) rows in one window when they fit
+        const uint64_t sp = (g.spanB + 31) & ~31ull;
+        if (sp > W && sp * TL <= DENSE_SLOTS) W = sp;
     }
     W = (W + 31) & ~31ull;  // dense windows cover whole 32-column bitmap words
     while (W > 32 && W * TL > DENSE_SLOTS) W -= 32;
@@ -457,11 +464,14 @@ int Contraction::run(const fcc_tensor *ta, const int32_t *am, const fcc_tensor *
     // ---- extents (contract.hpp:124-144) and the zero/tiny value flag
     Tmp<uint32_t> dext(pool, 2 * (kMaxOrder + 1));
     fill_u32(dext, 2 * (kMaxOrder + 1), 0, s);
+    // one operand on both sides (a self-contraction) is scanned once
+    const bool same_operand = A.coords == B.coords && A.vals == B.vals && A.order == B.order && A.nnz == B.nnz;
     launch_extents(A, dext.get(), s);
-    launch_extents(B, dext.get() + kMaxOrder + 1, s);
+    if (!same_operand) launch_extents(B, dext.get() + kMaxOrder + 1, s);
     uint32_t hext[2 * (kMaxOrder + 1)];
     FCC_HIP(hipMemcpyAsync(hext, dext.get(), sizeof(hext), hipMemcpyDeviceToHost, s));
     FCC_HIP(hipStreamSynchronize(s));
+    if (same_operand) memcpy(hext + kMaxOrder + 1, hext, (kMaxOrder + 1) * sizeof(uint32_t));
     uint32_t EA[kMaxOrder], EB[kMaxOrder];
     for (int m = 0; m < ta->order; m++) EA[m] = ta->extents ? ta->extents[m] : (ta->nnz ? hext[m] + 1 : 0);
     for (int m = 0; m < tb->order; m++)
@@ -506,7 +516,10 @@ int Contraction::run(const fcc_tensor *ta, const int32_t *am, const fcc_tensor *
     for (int i = 0; i < r && self; i++) self = am[i] == bm[i];
     for (int m = 0; m < ta->order && self; m++) self = EA[m] == EB[m];
     Tmp<uint32_t> bfib, vsB;
-    SelfJoin sj{rowA, fidA, nullptr, nullptr, 0};
+    // the per-entry row and fiber of a self-contraction are only read when
+    // left rows are sampled (the tiles come from the right operand's order)
+    const bool need_join = o.left_tile_stride > 1;
+    SelfJoin sj{need_join ? rowA.get() : nullptr, need_join ? fidA.get() : nullptr, nullptr, nullptr, 0};
     if (self) {
         bfib = Tmp<uint32_t>(pool, (size_t)A.nnz);
         vsB = Tmp<uint32_t>(pool, (size_t)A.nnz);
@@ -565,11 +578,10 @@ int Contraction::run(const fcc_tensor *ta, const int32_t *am, const fcc_tensor *
         FCC_HIP(hipStreamWaitEvent(ctx->side, ev[9], 0));
         side_join.st = ctx->side;
         FCC_HIP(hipEventRecord(ev[7], ctx->side));
-        dense_keys_block(ix, dk, fidA.get(), rowA.get(), A.vals, (size_t)A.nnz, ctx->side);
+        dense_keys_block(ix, dk, fidA.get(), rowA.get(), A.vals, (size_t)A.nnz, ctx->side, ev[10], ev[11]);
         FCC_HIP(hipEventRecord(ev[8], ctx->side));
     } else {
-        FCC_HIP(hipEventRecord(ev[7], s));
-        FCC_HIP(hipEventRecord(ev[8], s));
+        for (int e : {7, 8, 10, 11}) FCC_HIP(hipEventRecord(ev[e], s));
     }
     int logTL;
     uint32_t Wd;
@@ -1000,6 +1012,12 @@ int Contraction::run(const fcc_tensor *ta, const int32_t *am, const fcc_tensor *
     }
     st.ms_index = ev_ms(ev[0], ev[1]);
     st.ms_dense = ev_ms(ev[7], ev[8]);
+    st.ms_dense_gemm = ev_ms(ev[10], ev[11]);
+    if (dk.nh) {
+        st.dense_flops = 2ull * dk.ra * dk.rb * dk.kp;
+        // the tiles' units read the block's cells once (every k split) plus the presence words
+        st.dense_block_bytes = dk.splits * dk.spanA * dk.spanB * 8 + dk.spanA * dk.spanB / 8;
+    }
     st.ms_plan = ev_ms(ev[1], ev[2]);
     st.ms_symbolic = (learned && !use_exact) ? 0.0 : ev_ms(ev[2], ev[3]);
     st.ms_numeric = ev_ms(ev[4], ev[5]);

@@ -64,8 +64,10 @@ void dense_keys_alloc(DevPool &pool, const Geom &g, DenseKeys &dk, int num_cus,
 // entry, nA entries; unused for a self-contraction): column (and row)
 // bitsets, presence map, value panels, GEMM. Reads nothing the left-tile
 // build writes; fidA/rowA/Avals must stay valid until it completes.
+// gemm0 / gemm1 (optional) are recorded around the GEMM launch.
 void dense_keys_block(const Index &ix, const DenseKeys &dk, const uint32_t *fidA, const uint32_t *rowA,
-                      const double *Avals, size_t nA, hipStream_t s);
+                      const double *Avals, size_t nA, hipStream_t s, hipEvent_t gemm0 = nullptr,
+                      hipEvent_t gemm1 = nullptr);
 
 // After the left tiles: takes the selected keys out of the sparse traversal
 // (group descriptors, light-entry fiber ranges, DIRECT eligibility; heavy

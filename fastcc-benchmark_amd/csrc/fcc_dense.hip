@@ -404,7 +404,7 @@ void dense_keys_alloc(DevPool &pool, const Geom &g, DenseKeys &dk, int num_cus,
 }
 
 void dense_keys_block(const Index &ix, const DenseKeys &dk, const uint32_t *fidA, const uint32_t *rowA,
-                      const double *Avals, size_t nA, hipStream_t s) {
+                      const double *Avals, size_t nA, hipStream_t s, hipEvent_t gemm0, hipEvent_t gemm1) {
     if (!dk.nh) return;
     const uint64_t ra = dk.ra, rb = dk.rb;
     fill_u64(dk.colbits, (size_t)dk.nw * rb, 0, s);
@@ -420,7 +420,9 @@ void dense_keys_block(const Index &ix, const DenseKeys &dk, const uint32_t *fidA
     }
     k_dk_presence<<<dim3((unsigned)(rb / 64), (unsigned)(ra / 64)), 256, 0, s>>>(dk.rowbits, dk.colbits, dk.spanA, rb,
                                                                                  dk.nw, dk.P);
+    if (gemm0) FCC_HIP(hipEventRecord(gemm0, s));
     launch_dgemm_tn(dk.AT, dk.Bd, dk.C, ra, rb, dk.kp, dk.splits, s);
+    if (gemm1) FCC_HIP(hipEventRecord(gemm1, s));
 }
 
 void dense_keys_mask(DevPool &pool, Index &ix, DenseKeys &dk, std::vector<std::shared_ptr<void>> &hold,

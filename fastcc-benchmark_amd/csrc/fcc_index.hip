@@ -449,8 +449,10 @@ __global__ __launch_bounds__(256) void k_self_join(const uint32_t *vs, const uin
     for (size_t p = blockIdx.x * (size_t)blockDim.x + threadIdx.x; p < n; p += (size_t)gridDim.x * blockDim.x) {
         uint32_t f = fid[p] + flag[p] - 1;
         uint32_t o = vs[p];
-        rowA[o] = Bcol[p];
-        fidA[o] = f;
+        if (rowA) {  // per original entry (random writes): only when asked for
+            rowA[o] = Bcol[p];
+            fidA[o] = f;
+        }
         bfib[p] = f;
         vsB[p] = o;
         prod += fiber_ptr[f + 1] - fiber_ptr[f];

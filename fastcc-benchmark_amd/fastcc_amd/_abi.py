@@ -78,6 +78,9 @@ class FccStats(C.Structure):
         ("dense_keys", C.c_uint64),
         ("dense_key_products", C.c_uint64),
         ("ms_dense", C.c_double),
+        ("ms_dense_gemm", C.c_double),
+        ("dense_flops", C.c_uint64),
+        ("dense_block_bytes", C.c_uint64),
     ]
 
 

@@ -92,7 +92,11 @@ typedef struct {
                               straight out, no accumulator */
     uint64_t dense_keys;   /* contraction keys taken by the dense GEMM (0: none) */
     uint64_t dense_key_products; /* their products (part of products) */
-    double ms_dense;       /* value panels + dense GEMM */
+    double ms_dense;       /* the dense-key block: bitsets, presence map, value panels, GEMM
+                              (on a second stream, beside the left-tile build) */
+    double ms_dense_gemm;  /* the dense GEMM kernel alone */
+    uint64_t dense_flops;  /* flops the dense GEMM executes (2 * padded rows * columns * keys) */
+    uint64_t dense_block_bytes; /* bytes of the block and presence map the accumulate kernels read */
 } fcc_stats;
 
 typedef struct {
