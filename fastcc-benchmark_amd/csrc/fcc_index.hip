@@ -733,27 +733,20 @@ void build_left(DevPool &pool, const Geom &g, const Operand &A, const uint32_t *
 
 // Self-contraction: the right operand is already sorted by (fiber, column)
 // and column == left row, so a stable sort on (tile, class) alone yields the
-// left order (tile, class, fiber, row-in-tile).
-__global__ void k_self_tile_key(const uint32_t *Bcol, const uint32_t *bfib, const uint32_t *fiber_ptr, size_t n,
-                                int logTL, uint32_t heavy_min, uint64_t *key, uint32_t *pos) {
+// left order (tile, class, fiber, row-in-tile). The whole composite key and
+// the entry index are written here in the right operand's order (coalesced
+// reads) and the sort touches only the (tile, class) bits, so nothing has to
+// be gathered back through the sort permutation afterwards.
+__global__ void k_self_composite(const uint32_t *Bcol, const uint32_t *bfib, const uint32_t *vsB,
+                                 const uint32_t *fiber_ptr, size_t n, int logTL, int fbits, uint32_t heavy_min,
+                                 uint64_t *ks, uint32_t *vs) {
     size_t p = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
     if (p >= n) return;
-    uint32_t f = bfib[p];
-    uint64_t cls = (fiber_ptr[f + 1] - fiber_ptr[f]) >= heavy_min ? 0 : 1;
-    key[p] = ((uint64_t)(Bcol[p] >> logTL) << 1) | cls;
-    pos[p] = (uint32_t)p;
-}
-
-__global__ void k_self_composite(const uint64_t *tkey, const uint32_t *pos, const uint32_t *Bcol,
-                                 const uint32_t *bfib, const uint32_t *vsB, size_t n, int logTL, int fbits,
-                                 uint64_t *ks, uint32_t *vs) {
-    size_t q = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
-    if (q >= n) return;
-    uint32_t p = pos[q];
-    uint64_t tc = tkey[q];
-    uint64_t li = Bcol[p] & ((1u << logTL) - 1);
-    ks[q] = ((tc >> 1) << (1 + fbits + logTL)) | ((tc & 1) << (fbits + logTL)) | ((uint64_t)bfib[p] << logTL) | li;
-    vs[q] = vsB[p];
+    const uint32_t f = bfib[p], c = Bcol[p];
+    const uint64_t cls = (fiber_ptr[f + 1] - fiber_ptr[f]) >= heavy_min ? 0 : 1;
+    const uint64_t tile = c >> logTL, li = c & ((1u << logTL) - 1);
+    ks[p] = (tile << (1 + fbits + logTL)) | (cls << (fbits + logTL)) | ((uint64_t)f << logTL) | li;
+    vs[p] = vsB[p];
 }
 
 void build_left_self(DevPool &pool, const Geom &g, const Operand &A, const SelfJoin &sj, int logTL, Index &ix,
@@ -767,13 +760,11 @@ void build_left_self(DevPool &pool, const Geom &g, const Operand &A, const SelfJ
     if (tbits + 1 + fbits + logTL + 1 > 64) throw LibError{ERR_INVALID, "left composite key exceeds 64 bits"};
     Tmp<uint64_t> k1(pool, n), k2(pool, n);
     Tmp<uint32_t> v1(pool, n), v2(pool, n);
-    k_self_tile_key<<<grid_for(n), 256, 0, s>>>(ix.Bcol, sj.bfib, ix.fiber_ptr, n, logTL, ix.heavy_min, k1, v1);
-    uint64_t *tk;
-    uint32_t *tp;
-    radix_sort_pairs(pool, k1, v1, k2, v2, n, tbits + 1, s, &tk, &tp);
-    uint64_t *ks = tk == k1.get() ? k2.get() : k1.get();
-    uint32_t *vs = tp == v1.get() ? v2.get() : v1.get();
-    k_self_composite<<<grid_for(n), 256, 0, s>>>(tk, tp, ix.Bcol, sj.bfib, sj.vsB, n, logTL, fbits, ks, vs);
+    k_self_composite<<<grid_for(n), 256, 0, s>>>(ix.Bcol, sj.bfib, sj.vsB, ix.fiber_ptr, n, logTL, fbits,
+                                                 ix.heavy_min, k1, v1);
+    uint64_t *ks;
+    uint32_t *vs;
+    radix_sort_pairs(pool, k1, v1, k2, v2, n, tbits + 1, s, &ks, &vs, fbits + logTL);
     left_from_sorted(pool, A, ks, vs, (uint32_t)n, logTL, fbits, ix, hold, s);
 }
 

@@ -96,11 +96,13 @@ void scan_u32(DevPool &pool, const uint32_t *in, uint32_t *out, size_t n, uint32
 void scan_u64(DevPool &pool, const uint64_t *in, uint64_t *out, size_t n, uint64_t *total_dev,
               hipStream_t s);
 
-// Stable LSD radix sort of (key, value) pairs on key bits [0, nbits).
-// Returns pointers to the sorted arrays (either the inputs or the alternates).
+// Stable LSD radix sort of (key, value) pairs on key bits [lo_bit, lo_bit +
+// nbits) (8-bit digits; key bits above lo_bit + nbits must be zero, bits below
+// lo_bit keep their input order). Returns pointers to the sorted arrays
+// (either the inputs or the alternates).
 void radix_sort_pairs(DevPool &pool, uint64_t *keys, uint32_t *vals, uint64_t *keys_alt,
                       uint32_t *vals_alt, size_t n, int nbits, hipStream_t s,
-                      uint64_t **keys_out, uint32_t **vals_out);
+                      uint64_t **keys_out, uint32_t **vals_out, int lo_bit = 0);
 
 // ------------------------------------------------------------------- index
 struct Operand {

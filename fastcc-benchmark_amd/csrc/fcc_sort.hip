@@ -264,7 +264,7 @@ __global__ __launch_bounds__(RS_THREADS) void k_rs_scatter(const uint64_t *keys,
 }
 
 void radix_sort_pairs(DevPool &pool, uint64_t *keys, uint32_t *vals, uint64_t *keys_alt, uint32_t *vals_alt,
-                      size_t n, int nbits, hipStream_t s, uint64_t **keys_out, uint32_t **vals_out) {
+                      size_t n, int nbits, hipStream_t s, uint64_t **keys_out, uint32_t **vals_out, int lo_bit) {
     if (n >= (1ull << 32)) throw LibError{ERR_INVALID, "radix sort limited to < 2^32 elements"};
     uint64_t *ka = keys, *kb = keys_alt;
     uint32_t *va = vals, *vb = vals_alt;
@@ -272,7 +272,7 @@ void radix_sort_pairs(DevPool &pool, uint64_t *keys, uint32_t *vals, uint64_t *k
         unsigned nblocks = (unsigned)div_up(n, RS_TILE);
         size_t hn = (size_t)256 * nblocks;
         Tmp<uint32_t> hist(pool, hn);
-        for (int shift = 0; shift < nbits; shift += 8) {
+        for (int shift = lo_bit; shift < lo_bit + nbits; shift += 8) {
             k_rs_hist<<<nblocks, RS_THREADS, 0, s>>>(ka, n, shift, hist, nblocks);
             scan_u32(pool, hist, hist, hn, nullptr, s);
             k_rs_scatter<<<nblocks, RS_THREADS, 0, s>>>(ka, va, kb, vb, n, shift, hist, nblocks);
