@@ -142,6 +142,12 @@ def main():
     dev_ms = float(np.mean([s["ms_total"] for s in stats]))
     roof = numeric_roofline(stats[-1], k_ms)
     roof_dense = dense_roofline(stats)
+    if args.planner == "exact" and args.force_acc == "auto" and not args.tile_hint and not args.col_window:
+        for r, kern in ((roof, "k_numeric"), (roof_dense, "k_dgemm_tn")):
+            if r is not None:
+                r["traffic"], src = measured_traffic(wl.name, kern)
+                if src:
+                    r["traffic_source"] = src
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -212,6 +218,18 @@ def roofline(products, bytes_alg, k_ms, kernel="k_numeric", note=""):
     achieved = bytes_alg / (k_ms * 1e-3) / 1e9
     return {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": kernel, "note": note}
+
+
+def measured_traffic(workload, kernel):
+    """HBM bytes per launch (FETCH_SIZE + WRITE_SIZE, copy-calibrated) of a
+    kernel on a workload, from the committed rocprofv3 summaries
+    (profiles/hbm_traffic.json, written with each profiles/*.md summary)."""
+    p = os.path.join(ROOT, "profiles", "hbm_traffic.json")
+    try:
+        t = json.load(open(p))[workload][kernel]
+    except (OSError, KeyError, ValueError):
+        return None, None
+    return t["fetch_bytes"] + t["write_bytes"], "profiles/" + t["source"]
 
 
 def numeric_roofline(st, k_ms):

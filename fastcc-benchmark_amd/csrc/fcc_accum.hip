@@ -601,7 +601,10 @@ __device__ __forceinline__ void heavy_ring(const AccArgs &A, const UnitCtx &U, c
     }
 }
 
-template <bool NUM, bool BITMAP>
+// DEEP: dense units keep HRING_DENSE items in flight; without it (kernels
+// whose other paths already need many registers) they keep HRING_HASH, so
+// the kernel stays within 128 VGPRs and two workgroups fit per CU.
+template <bool NUM, bool BITMAP, bool DEEP = true>
 __device__ void heavy_claimed(const AccArgs &A, const UnitCtx &U, const Acc<NUM, BITMAP> &S, uint32_t *s_gnext) {
     const uint32_t lane = threadIdx.x & 63;
     // A tile with few heavy groups works on fiber segments (long fibers spread
@@ -655,7 +658,7 @@ __device__ void heavy_claimed(const AccArgs &A, const UnitCtx &U, const Acc<NUM,
             continue;
         }
         if (U.kind == KIND_DENSE)
-            heavy_ring<NUM, BITMAP, HRING_DENSE, true>(A, U, S, lane, nb, s_l, e_l, d);
+            heavy_ring<NUM, BITMAP, DEEP ? HRING_DENSE : HRING_HASH, true>(A, U, S, lane, nb, s_l, e_l, d);
         else
             heavy_ring<NUM, BITMAP, HRING_HASH, false>(A, U, S, lane, nb, s_l, e_l, d);
     }
@@ -960,7 +963,7 @@ __device__ void dk_seed_cells(const AccArgs &A, const UnitCtx &U, const Acc<NUM,
 }
 
 // One unit: heavy groups (a wavefront per group), then the light entries.
-template <bool NUM, bool BITMAP>
+template <bool NUM, bool BITMAP, bool DEEP = true>
 __device__ void process_unit(const AccArgs &A, const UnitCtx &U, const Acc<NUM, BITMAP> &S, uint32_t *s_gnext,
                              uint32_t *stage, uint32_t *s_w) {
     const uint32_t gl = A.atile_lg[U.a], g1 = A.atile_g0[U.a + 1];
@@ -969,7 +972,7 @@ __device__ void process_unit(const AccArgs &A, const UnitCtx &U, const Acc<NUM, 
         dense_heavy_owned<NUM, BITMAP>(A, U, S);
         __syncthreads();  // plain updates above, atomic updates below
     } else {
-        heavy_claimed<NUM, BITMAP>(A, U, S, s_gnext);
+        heavy_claimed<NUM, BITMAP, DEEP>(A, U, S, s_gnext);
     }
     light_entries<NUM, BITMAP>(A, U, S, A.group_ptr[gl], A.group_ptr[g1], stage, s_w);
 }
@@ -1046,7 +1049,7 @@ __device__ void sink_flush(const AccArgs &A, SinkAcc &sk, unsigned long long *s_
 
 // MODE: 0 exact placement, 1 learned placement (cursor)
 template <bool BITMAP, bool SINK, int MODE>
-__global__ __launch_bounds__(ACC_THREADS) void k_numeric(AccArgs A) {
+__global__ __launch_bounds__(ACC_THREADS, 4) void k_numeric(AccArgs A) {
     __shared__ __align__(16) double s_f64[DENSE_SLOTS];
     __shared__ uint32_t s_bits[BITMAP ? DENSE_SLOTS / 32 : 1];
     __shared__ uint32_t s_cnt, s_gnext, s_unit, s_w[NWAVES];
@@ -1132,7 +1135,9 @@ __global__ __launch_bounds__(ACC_THREADS) void k_numeric(AccArgs A) {
             const uint32_t t1 = U.u + 1 < A.nunits ? A.unit_tile[U.u + 1] : A.natiles;
             direct_run<BITMAP>(A, U, S, A.group_ptr[A.atile_g0[U.a]], A.group_ptr[A.atile_g0[t1]], stage, s_w);
         } else {
-            process_unit<true, BITMAP>(A, U, S, &s_gnext, stage, s_w);
+            // count + checksum and bitmap variants carry more state: the
+            // shallow dense ring keeps them at two workgroups per CU
+            process_unit<true, BITMAP, !SINK && !BITMAP>(A, U, S, &s_gnext, stage, s_w);
         }
         __syncthreads();
         if (U.kind == KIND_DENSE || U.kind == KIND_LHASH) {
