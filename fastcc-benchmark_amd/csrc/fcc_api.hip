@@ -588,6 +588,7 @@ int Contraction::run(const fcc_tensor *ta, const int32_t *am, const fcc_tensor *
     choose_tiles(ctx, o, g, products, nvalid, dk.nh > 0, &logTL, &Wd, &bucket, &tuned_wg);
     if (const char *hm = getenv("FCC_HEAVY_MIN")) ix.heavy_min = (uint32_t)std::max(1, atoi(hm));
     if (const char *hs = getenv("FCC_HSEG")) ix.hseg = (uint32_t)std::max(64, atoi(hs));
+    ix.defer_hseg = dk.nh > 0;  // built once, after the dense keys are masked
     if (self && o.left_tile_stride <= 1) build_left_self(pool, g, A, sj, logTL, ix, hold, s);
     else build_left(pool, g, A, rowA, fidA, nvalid, logTL, ix, hold, s);
     bfib.reset();
@@ -809,6 +810,12 @@ int Contraction::run(const fcc_tensor *ta, const int32_t *am, const fcc_tensor *
     auto &out_v = k.out_v;
     uint64_t stride = 0;
     bool use_exact = !learned;
+    // statistics read back with the results (one synchronisation at the
+    // end): [0, 8) units per accumulator kind, [8] the learned planner's
+    // summed tile error (double bits)
+    Tmp<unsigned long long> tail(pool, 9);
+    fill_u64(tail, 9, 0, s);
+    bool mape_pending = false;
 
     if (learned) {
         // ---- learned planning: predicted capacity, single numeric pass
@@ -871,10 +878,8 @@ int Contraction::run(const fcc_tensor *ta, const int32_t *am, const fcc_tensor *
             k_unit_to_tile<<<grid256(U.n), 256, 0, s>>>(unit_cur, U.tile, U.kind, U.n, tn);
             k_direct_tile_count<<<grid256(T), 256, 0, s>>>(ix.st_direct, (const uint64_t *)ix.st_products, T,
                                                            P.force, tn);
-            Tmp<double> mp(pool, 1);
-            fill_f64(mp, 1, 0.0, s);
-            k_mape<<<grid256(T), 256, 0, s>>>(pred_t, tn, T, mp);
-            if (!sink) st.mlp_mape = d2h(mp.get(), s) / (double)T;
+            k_mape<<<grid256(T), 256, 0, s>>>(pred_t, tn, T, reinterpret_cast<double *>(tail.get() + 8));
+            mape_pending = !sink;
         }
     }
     if (use_exact) {
@@ -944,8 +949,10 @@ int Contraction::run(const fcc_tensor *ta, const int32_t *am, const fcc_tensor *
     }
 
     // ---- results
-    unsigned long long sk[4];
+    k_kind_hist<<<std::min(grid256(U.n), 1024u), 256, 0, s>>>(U.kind, U.n, tail);
+    unsigned long long sk[4], th[9];
     FCC_HIP(hipMemcpyAsync(sk, sinkbuf.get(), 32, hipMemcpyDeviceToHost, s));
+    FCC_HIP(hipMemcpyAsync(th, tail.get(), sizeof(th), hipMemcpyDeviceToHost, s));
     std::vector<unsigned long long> gdone;
     if (groups) {
         group_sink.resize(nsink);
@@ -954,6 +961,15 @@ int Contraction::run(const fcc_tensor *ta, const int32_t *am, const fcc_tensor *
         FCC_HIP(hipMemcpyAsync(gdone.data(), g_done.get(), 8 * (groups + 1), hipMemcpyDeviceToHost, s));
     }
     FCC_HIP(hipStreamSynchronize(s));
+    st.units_dense = th[KIND_DENSE];
+    st.units_lhash = th[KIND_LHASH];
+    st.units_ghash = th[KIND_GHASH];
+    st.units_direct = th[KIND_DIRECT];
+    if (mape_pending) {
+        double m;
+        memcpy(&m, &th[8], 8);
+        st.mlp_mape = m / (double)T;
+    }
     if (sink) {
         nnz_out = sk[0];
         out->coord_hash = sk[0] ? sk[1] : 0;
@@ -996,19 +1012,6 @@ int Contraction::run(const fcc_tensor *ta, const int32_t *am, const fcc_tensor *
             out->val_sum = dv[0];
             out->val_wsum = dv[1];
         }
-    }
-    // stats
-    {
-        Tmp<unsigned long long> kh(pool, 8);
-        fill_u64(kh, 8, 0, s);
-        k_kind_hist<<<std::min(grid256(U.n), 1024u), 256, 0, s>>>(U.kind, U.n, kh);
-        unsigned long long h[8];
-        FCC_HIP(hipMemcpyAsync(h, kh.get(), 64, hipMemcpyDeviceToHost, s));
-        FCC_HIP(hipStreamSynchronize(s));
-        st.units_dense = h[KIND_DENSE];
-        st.units_lhash = h[KIND_LHASH];
-        st.units_ghash = h[KIND_GHASH];
-        st.units_direct = h[KIND_DIRECT];
     }
     st.ms_index = ev_ms(ev[0], ev[1]);
     st.ms_dense = ev_ms(ev[7], ev[8]);

@@ -32,8 +32,6 @@ struct DenseKeys {
     uint32_t splits = 1;      // GEMM k splits: C holds this many ra x rb slabs, summed by the readers
     uint32_t *hkey = nullptr;              // per fiber: selected key index, or ~0
     uint32_t *hfib = nullptr;              // per selected key: its fiber
-    uint32_t ndg = 0;
-    uint32_t *dgrp = nullptr;              // groups (tile, fiber) of the selected keys
     uint8_t *tile_dk = nullptr;            // per active tile: 1 if it holds a selected key
     unsigned long long *rowbits = nullptr; // [spanA][nw] (null for a self-contraction)
     unsigned long long *colbits = nullptr; // [nw][rb] (word-major: coalesced per column)
@@ -71,7 +69,8 @@ void dense_keys_block(const Index &ix, const DenseKeys &dk, const uint32_t *fidA
 
 // After the left tiles: takes the selected keys out of the sparse traversal
 // (group descriptors, light-entry fiber ranges, DIRECT eligibility; heavy
-// segments rebuilt) and flags the tiles that hold them. Synchronises.
+// segments built here, the left-tile build having deferred them) and flags the
+// tiles that hold them.
 void dense_keys_mask(DevPool &pool, Index &ix, DenseKeys &dk, std::vector<std::shared_ptr<void>> &hold,
                      hipStream_t s);
 

@@ -69,28 +69,19 @@ __global__ void k_dk_keys(const uint32_t *flag, const uint32_t *idx, uint32_t nf
     if (sel) hfib[idx[f]] = f;
 }
 
-__global__ void k_dk_gflag(const uint32_t *group_fiber, const uint32_t *hkey, uint32_t ngroups, uint32_t *gflag) {
-    const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
-    if (g < ngroups) gflag[g] = hkey[group_fiber[g]] != 0xFFFFFFFFu ? 1u : 0u;
-}
-
-__global__ void k_dk_glist(const uint32_t *gflag, const uint32_t *gidx, uint32_t ngroups, uint32_t *dgrp) {
-    const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
-    if (g < ngroups && gflag[g]) dgrp[gidx[g]] = g;
-}
-
 // Takes the selected keys' groups out of the sparse traversal: a heavy group
 // gets an empty fiber in its descriptor, a light group's entries get empty
 // fiber ranges; their tiles lose DIRECT eligibility (their outputs meet the
 // dense block). Sets the row bitsets.
-__global__ void k_dk_mask(const uint32_t *dgrp, uint32_t ndg, const uint32_t *group_ptr, const uint32_t *group_fiber,
+__global__ void k_dk_mask(uint32_t ngroups, const uint32_t *group_ptr, const uint32_t *group_fiber,
                           const uint32_t *group_atile, const uint8_t *glight, const uint32_t *hkey,
                           const uint32_t *Arow, uint32_t nw, uint4 *gdesc, uint2 *Afp, uint8_t *st_direct,
                           uint8_t *tile_dk, unsigned long long *rowbits) {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= ndg) return;
-    const uint32_t g = dgrp[i];
-    const uint32_t h = hkey[group_fiber[g]], a = group_atile[g];
+    const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= ngroups) return;
+    const uint32_t h = hkey[group_fiber[g]];
+    if (h == 0xFFFFFFFFu) return;
+    const uint32_t a = group_atile[g];
     const uint32_t e0 = group_ptr[g], e1 = group_ptr[g + 1];
     if (glight[g]) {
         for (uint32_t e = e0; e < e1; e++) {
@@ -183,13 +174,14 @@ __global__ void k_dk_join_A(const uint32_t *fidA, const uint32_t *rowA, const do
     atomicAdd(&AT[(uint64_t)h * ra + row], vals[e]);
 }
 
-__global__ void k_dk_scatter_A(const uint32_t *dgrp, uint32_t ndg, const uint32_t *group_ptr,
-                               const uint32_t *group_fiber, const uint32_t *hkey, const uint32_t *Arow,
-                               const double *Aval, uint64_t ra, double *AT) {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= ndg) return;
-    const uint32_t g = dgrp[i];
-    double *col = AT + (uint64_t)hkey[group_fiber[g]] * ra;
+__global__ void k_dk_scatter_A(uint32_t ngroups, const uint32_t *group_ptr, const uint32_t *group_fiber,
+                               const uint32_t *hkey, const uint32_t *Arow, const double *Aval, uint64_t ra,
+                               double *AT) {
+    const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= ngroups) return;
+    const uint32_t h = hkey[group_fiber[g]];
+    if (h == 0xFFFFFFFFu) return;
+    double *col = AT + (uint64_t)h * ra;
     for (uint32_t e = group_ptr[g]; e < group_ptr[g + 1]; e++) atomicAdd(&col[Arow[e]], Aval[e]);
 }
 
@@ -434,23 +426,14 @@ void dense_keys_mask(DevPool &pool, Index &ix, DenseKeys &dk, std::vector<std::s
         hold.emplace_back(p, [pl](void *q) { pl->put(q); });
         return p;
     };
-    // the selected keys' groups
-    if (ix.ngroups) {
-        Tmp<uint32_t> gflag(pool, ix.ngroups), gidx(pool, ix.ngroups), gtot(pool, 1);
-        k_dk_gflag<<<grid_of(ix.ngroups), 256, 0, s>>>(ix.group_fiber, dk.hkey, ix.ngroups, gflag);
-        scan_u32(pool, gflag, gidx, ix.ngroups, gtot, s);
-        FCC_HIP(hipMemcpyAsync(&dk.ndg, gtot.get(), 4, hipMemcpyDeviceToHost, s));
-        FCC_HIP(hipStreamSynchronize(s));
-        dk.dgrp = (uint32_t *)keep((size_t)dk.ndg * 4);
-        k_dk_glist<<<grid_of(ix.ngroups), 256, 0, s>>>(gflag, gidx, ix.ngroups, dk.dgrp);
-    }
     dk.tile_dk = (uint8_t *)keep((size_t)ix.natiles + 1);
     fill_u32(reinterpret_cast<uint32_t *>(dk.tile_dk), ((size_t)ix.natiles + 1 + 3) / 4, 0, s);
-    if (dk.ndg)
-        k_dk_mask<<<grid_of(dk.ndg), 256, 0, s>>>(dk.dgrp, dk.ndg, ix.group_ptr, ix.group_fiber, ix.group_atile,
-                                                   ix.glight, dk.hkey, ix.Arow, dk.nw, ix.gdesc, ix.Afp, ix.st_direct,
-                                                   dk.tile_dk, nullptr);
-    // heavy segments from the masked descriptors
+    if (ix.ngroups)
+        k_dk_mask<<<grid_of(ix.ngroups), 256, 0, s>>>(ix.ngroups, ix.group_ptr, ix.group_fiber, ix.group_atile,
+                                                       ix.glight, dk.hkey, ix.Arow, dk.nw, ix.gdesc, ix.Afp,
+                                                       ix.st_direct, dk.tile_dk, nullptr);
+    // heavy segments from the masked descriptors (the left-tile build
+    // deferred them: ix.defer_hseg)
     build_hseg(pool, ix, hold, s);
 }
 
@@ -458,9 +441,9 @@ void dense_keys_values(const Index &ix, const DenseKeys &dk, hipStream_t s) {
     if (!dk.nh) return;
     fill_f64(dk.Bd, (size_t)dk.kp * dk.rb, 0.0, s);
     if (!dk.self) fill_f64(dk.AT, (size_t)dk.kp * dk.ra, 0.0, s);
-    if (dk.ndg && !dk.self)
-        k_dk_scatter_A<<<grid_of(dk.ndg), 256, 0, s>>>(dk.dgrp, dk.ndg, ix.group_ptr, ix.group_fiber, dk.hkey, ix.Arow,
-                                                        ix.Aval, dk.ra, dk.AT);
+    if (ix.ngroups && !dk.self)
+        k_dk_scatter_A<<<grid_of(ix.ngroups), 256, 0, s>>>(ix.ngroups, ix.group_ptr, ix.group_fiber, dk.hkey, ix.Arow,
+                                                            ix.Aval, dk.ra, dk.AT);
     k_dk_scatter_B<<<dk.nh, 256, 0, s>>>(dk.hfib, ix.fiber_ptr, ix.Bcol, ix.Bval, dk.rb, dk.Bd);
     launch_dgemm_tn(dk.AT, dk.Bd, dk.C, dk.ra, dk.rb, dk.kp, dk.splits, s);
 }

@@ -678,7 +678,7 @@ static void left_from_sorted(DevPool &pool, const Operand &A, const uint64_t *ks
         k_tile_light_init<<<grid_for(ix.natiles), 256, 0, s>>>(ix.atile_g0, ix.natiles, ix.atile_lg);
         k_tile_light<<<grid_for(ix.ngroups), 256, 0, s>>>(glight, ix.group_atile, ix.atile_g0, ix.ngroups, ix.atile_lg);
     }
-    build_hseg(pool, ix, hold, s);
+    if (!ix.defer_hseg) build_hseg(pool, ix, hold, s);
     // statistics
     size_t T = ix.natiles;
     ix.st_nnz = (uint32_t *)keep(T * 4);

@@ -158,6 +158,7 @@ struct Index {
     // [atile_h0[a], atile_h0[a + 1])
     uint32_t nhseg = 0;
     uint32_t hseg = HSEG;             // fiber entries per segment
+    bool defer_hseg = false;          // the left-tile build leaves hdesc to a later build_hseg
     uint4 *hdesc = nullptr;           // per segment: fiber start, length, first entry, entries
     uint32_t *atile_h0 = nullptr;     // natiles+1
     // per active tile statistics (planner features)
