@@ -664,6 +664,29 @@ __device__ void heavy_claimed(const AccArgs &A, const UnitCtx &U, const Acc<NUM,
     }
 }
 
+// Smallest i >= from with inc[i] > q, where inc is nondecreasing and
+// inc[n - 1] > q (inclusive prefix sums of products per entry, in LDS). A
+// lane's product index only grows, so it searches forward from its last
+// entry: one probe while it stays inside a long fiber, a galloping search
+// (log of the distance) when it skips short ones -- instead of a full binary
+// search over the chunk per product.
+__device__ __forceinline__ uint32_t gallop_upper(const uint32_t *inc, uint32_t from, uint32_t n, uint32_t q) {
+    if (inc[from] > q) return from;
+    uint32_t lo = from, step = 1, hi = from + 1;
+    while (hi < n - 1 && inc[hi] <= q) {
+        lo = hi;
+        step <<= 1;
+        hi = min(lo + step, n - 1);
+    }
+    uint32_t a = lo + 1, b = hi;  // inc[lo] <= q < inc[hi]
+    while (a < b) {
+        const uint32_t mid = (a + b) >> 1;
+        if (inc[mid] > q) b = mid;
+        else a = mid + 1;
+    }
+    return a;
+}
+
 // Light entries [le0, le1) of a unit: the products of every entry (its right
 // fiber, clipped to the unit's column window) are flattened and spread over
 // all lanes, so a tile of a few entries with long fibers keeps the block busy.
@@ -725,13 +748,10 @@ __device__ void light_entries(const AccArgs &A, const UnitCtx &U, const Acc<NUM,
             const uint32_t nent = min(le1 - eb, (uint32_t)ACC_THREADS);
             // product q's entry and fiber position; the loads for the next
             // product of this lane are issued before the current one accumulates
+            uint32_t cur = 0;  // this lane's last entry (its products come in increasing order)
             auto locate = [&](uint32_t q, uint32_t &p, uint32_t &ee) {
-                uint32_t lo = 0, hi = nent - 1;  // s_inc[nent - 1] == P > q
-                while (lo < hi) {
-                    const uint32_t mid = (lo + hi) >> 1;
-                    if (s_inc[mid] > q) hi = mid;
-                    else lo = mid + 1;
-                }
+                const uint32_t lo = gallop_upper(s_inc, cur, nent, q);  // s_inc[nent - 1] == P > q
+                cur = lo;
                 p = s_s0[lo] + (q - (lo ? s_inc[lo - 1] : 0u));
                 ee = eb + lo;
             };
@@ -840,13 +860,10 @@ __device__ void direct_run(const AccArgs &A, const UnitCtx &U, const Acc<true, B
         s_s0[threadIdx.x] = s0;
         __syncthreads();
         const uint32_t nent = min(e1 - eb, (uint32_t)ACC_THREADS);
+        uint32_t cur = 0;  // this lane's last entry (its products come in increasing order)
         auto locate = [&](uint32_t q, uint32_t &p, uint32_t &ee) {
-            uint32_t lo = 0, hi = nent - 1;  // s_inc[nent - 1] == P > q
-            while (lo < hi) {
-                const uint32_t mid = (lo + hi) >> 1;
-                if (s_inc[mid] > q) hi = mid;
-                else lo = mid + 1;
-            }
+            const uint32_t lo = gallop_upper(s_inc, cur, nent, q);  // s_inc[nent - 1] == P > q
+            cur = lo;
             p = s_s0[lo] + (q - (lo ? s_inc[lo - 1] : 0u));
             ee = eb + lo;
         };

@@ -748,12 +748,16 @@ int Contraction::run(const fcc_tensor *ta, const int32_t *am, const fcc_tensor *
     unsigned &grid = k.grid;
     grid = 1;
 
-    // units from the exact bounds (pred == null) or from predictions
-    auto make_units = [&](const float *pred) {
+    // units from the exact bounds (pred == null) or from predictions; the
+    // unit count comes back with the predicted output total (psum, when
+    // given) in one synchronisation
+    auto make_units = [&](const float *pred, const unsigned long long *psum, uint64_t *psum_h) {
         P.dense_ratio = pred ? 0.15f : 0.5f;
         launch_plan_tiles(ix, pred, P, t_nwin, t_W, t_kind, s);
         scan_u32(pool, t_nwin, t_ubase, T, d_nunits, s);
-        U.n = d2h(d_nunits.get(), s);
+        FCC_HIP(hipMemcpyAsync(&U.n, d_nunits.get(), 4, hipMemcpyDeviceToHost, s));
+        if (psum) FCC_HIP(hipMemcpyAsync(psum_h, psum, 8, hipMemcpyDeviceToHost, s));
+        FCC_HIP(hipStreamSynchronize(s));
         u_tile = Tmp<uint32_t>(pool, U.n);
         u_c0 = Tmp<uint32_t>(pool, U.n);
         u_W = Tmp<uint32_t>(pool, U.n);
@@ -782,7 +786,17 @@ int Contraction::run(const fcc_tensor *ta, const int32_t *am, const fcc_tensor *
         grid = std::max(1u, std::min<unsigned>(grid, U.n));
         st.units = U.n;
     };
-    make_units(learned ? pred_t.get() : nullptr);
+    // learned: predicted outputs summed over tiles (each capped by its exact
+    // bounds) -- the output capacity
+    uint64_t predicted = 0;
+    Tmp<unsigned long long> psum(pool, 1);
+    if (learned) {
+        fill_u64(psum, 1, 0, s);
+        k_pred_sum<<<grid256(T), 256, 0, s>>>(pred_t, (const uint64_t *)ix.st_products,
+                                               (const uint64_t *)ix.st_maxgp, ix.st_direct,
+                                               (double)ix.TL * (double)g.spanB, T, psum);
+    }
+    make_units(learned ? pred_t.get() : nullptr, learned ? psum.get() : nullptr, &predicted);
     FCC_HIP(hipEventRecord(ev[2], s));
 
     auto &gk = k.gk;
@@ -798,37 +812,55 @@ int Contraction::run(const fcc_tensor *ta, const int32_t *am, const fcc_tensor *
         X.gunit = gu;
         X.gmask = cap - 1;
     };
-    auto reset_ghash = [&]() {
-        fill_u64(X.gkeys, X.gmask + 1, ~0ull, s);
-        fill_f64(X.gvals, X.gmask + 1, 0.0, s);
-        fill_u64(gmeta, 4, 0, s);
-        fill_u32(flags, 4, 0, s);
-        fill_u32(work, 4, 0, s);
+    // statistics read back with the results: [0, 8) units per accumulator
+    // kind, [8] the learned planner's summed tile error (double bits)
+    Tmp<unsigned long long> tail(pool, 9);
+    // counters, flags, the global hash and the statistics: one launch
+    auto reset_ghash = [&](bool with_sink) {
+        FillSet f;
+        f.add(X.gkeys, X.gmask + 1, ~0ull, 8);
+        f.add(X.gvals, X.gmask + 1, 0ull, 8);  // +0.0
+        f.add(gmeta.get(), 4, 0, 8);
+        f.add(flags.get(), 4, 0, 4);
+        f.add(work.get(), 4, 0, 4);
+        f.add(tail.get(), 9, 0, 8);
+        if (with_sink) f.add(sinkbuf.get(), nsink, 0, 8);
+        fill_many(f, s);
+    };
+    // After a numeric pass: the statistics kernels and every small read-back
+    // (flags, global-hash meta, summaries, statistics, group completion) are
+    // queued together and covered by one synchronisation.
+    unsigned long long sk[4] = {}, th[9] = {}, hm[4] = {};
+    uint32_t hf[2] = {};
+    std::vector<unsigned long long> gdone;
+    auto fetch_tail = [&]() {
+        k_kind_hist<<<std::min(grid256(U.n), 1024u), 256, 0, s>>>(U.kind, U.n, tail);
+        FCC_HIP(hipMemcpyAsync(hf, flags.get(), 8, hipMemcpyDeviceToHost, s));
+        FCC_HIP(hipMemcpyAsync(hm, gmeta.get(), 32, hipMemcpyDeviceToHost, s));
+        FCC_HIP(hipMemcpyAsync(sk, sinkbuf.get(), 32, hipMemcpyDeviceToHost, s));
+        FCC_HIP(hipMemcpyAsync(th, tail.get(), sizeof(th), hipMemcpyDeviceToHost, s));
+        if (groups) {
+            group_sink.resize(nsink);
+            gdone.resize(groups + 1);
+            FCC_HIP(hipMemcpyAsync(group_sink.data(), sinkbuf.get(), 8 * nsink, hipMemcpyDeviceToHost, s));
+            FCC_HIP(hipMemcpyAsync(gdone.data(), g_done.get(), 8 * (groups + 1), hipMemcpyDeviceToHost, s));
+        }
+        FCC_HIP(hipStreamSynchronize(s));
     };
 
     auto &out_c = k.out_c;
     auto &out_v = k.out_v;
     uint64_t stride = 0;
     bool use_exact = !learned;
-    // statistics read back with the results (one synchronisation at the
-    // end): [0, 8) units per accumulator kind, [8] the learned planner's
-    // summed tile error (double bits)
-    Tmp<unsigned long long> tail(pool, 9);
-    fill_u64(tail, 9, 0, s);
     bool mape_pending = false;
 
     if (learned) {
         // ---- learned planning: predicted capacity, single numeric pass
-        Tmp<unsigned long long> psum(pool, 1);
-        fill_u64(psum, 1, 0, s);
-        k_pred_sum<<<grid256(T), 256, 0, s>>>(pred_t, (const uint64_t *)ix.st_products,
-                                               (const uint64_t *)ix.st_maxgp, ix.st_direct, (double)ix.TL * (double)g.spanB, T,
-                                               psum);
-        uint64_t predicted = d2h(psum.get(), s);
         // never more than the products bound: no tile can emit more outputs than products
         uint64_t capacity = std::min<uint64_t>((uint64_t)((double)predicted * 1.10) + 4096, products + 16);
         alloc_ghash(1u << 20);
         FCC_HIP(hipEventRecord(ev[3], s));
+        Tmp<unsigned long long> tn(pool, T);
         for (int attempt = 0;; attempt++) {
             if (!sink) {
                 out_c = Tmp<uint32_t>(pool, (size_t)g.dC * capacity);
@@ -839,18 +871,21 @@ int Contraction::run(const fcc_tensor *ta, const int32_t *am, const fcc_tensor *
             X.ovals = out_v.get();
             X.ostride = stride;
             X.capacity = sink ? ~0ull : capacity;
-            reset_ghash();
-            if (sink) fill_u64(sinkbuf, nsink, 0, s);
+            reset_ghash(sink);
             arm_groups();
             if (attempt == 0) FCC_HIP(hipEventRecord(ev[4], s));
             launch_numeric(X, grid, bitmap, sink, 1, s);
             if (attempt == 0) FCC_HIP(hipEventRecord(ev[5], s));
             launch_gdrain(X, sink, 1, s);
-            uint32_t hf[2];
-            FCC_HIP(hipMemcpyAsync(hf, flags.get(), 8, hipMemcpyDeviceToHost, s));
-            unsigned long long hm[4];
-            FCC_HIP(hipMemcpyAsync(hm, gmeta.get(), 32, hipMemcpyDeviceToHost, s));
-            FCC_HIP(hipStreamSynchronize(s));
+            FCC_HIP(hipEventRecord(ev[6], s));
+            // tile-level MAPE against the actual per-unit outputs (kept only
+            // when this pass is the final one)
+            fill_u64(tn, T, 0, s);
+            k_unit_to_tile<<<grid256(U.n), 256, 0, s>>>(unit_cur, U.tile, U.kind, U.n, tn);
+            k_direct_tile_count<<<grid256(T), 256, 0, s>>>(ix.st_direct, (const uint64_t *)ix.st_products, T,
+                                                           P.force, tn);
+            k_mape<<<grid256(T), 256, 0, s>>>(pred_t, tn, T, reinterpret_cast<double *>(tail.get() + 8));
+            fetch_tail();
             st.ghash_entries = hm[0];
             if (hf[0]) {
                 // the predictions left more spill than the global hash holds:
@@ -868,42 +903,33 @@ int Contraction::run(const fcc_tensor *ta, const int32_t *am, const fcc_tensor *
                 continue;
             }
             nnz_out = sink ? 0 : hm[1];
-            break;
-        }
-        if (!use_exact) {
-            FCC_HIP(hipEventRecord(ev[6], s));
-            // tile-level MAPE against the actual per-unit outputs
-            Tmp<unsigned long long> tn(pool, T);
-            fill_u64(tn, T, 0, s);
-            k_unit_to_tile<<<grid256(U.n), 256, 0, s>>>(unit_cur, U.tile, U.kind, U.n, tn);
-            k_direct_tile_count<<<grid256(T), 256, 0, s>>>(ix.st_direct, (const uint64_t *)ix.st_products, T,
-                                                           P.force, tn);
-            k_mape<<<grid256(T), 256, 0, s>>>(pred_t, tn, T, reinterpret_cast<double *>(tail.get() + 8));
             mape_pending = !sink;
+            break;
         }
     }
     if (use_exact) {
         if (learned) {
-            make_units(nullptr);
+            make_units(nullptr, nullptr, nullptr);
             fill_u64(sinkbuf, nsink, 0, s);
         }
         // ---- exact symbolic planning: count, scan, then fill
         alloc_ghash(1u << 16);
         for (int attempt = 0;; attempt++) {
-            reset_ghash();
+            reset_ghash(false);
             fill_u64(unit_count, U.n, 0, s);
             launch_count(X, grid, s);
-            uint32_t gf = d2h(flags.get(), s);
+            launch_refine_kinds(unit_count, U.n, o.force_acc, U.kind, gmeta.get() + 2, s);
+            scan_u64(pool, (const uint64_t *)unit_count.get(), (uint64_t *)unit_off.get(), U.n,
+                     (uint64_t *)d_total.get(), s);
+            uint32_t gf = 0;
+            FCC_HIP(hipMemcpyAsync(&gf, flags.get(), 4, hipMemcpyDeviceToHost, s));
+            FCC_HIP(hipMemcpyAsync(hm, gmeta.get(), 32, hipMemcpyDeviceToHost, s));
+            FCC_HIP(hipMemcpyAsync(&nnz_out, d_total.get(), 8, hipMemcpyDeviceToHost, s));
+            FCC_HIP(hipStreamSynchronize(s));
             if (!gf) break;
             if (attempt >= 3) throw LibError{FCC_ERR_OVERFLOW, "count-pass global set overflow"};
             alloc_ghash((X.gmask + 1) * 16);
         }
-        launch_refine_kinds(unit_count, U.n, o.force_acc, U.kind, gmeta.get() + 2, s);
-        scan_u64(pool, (const uint64_t *)unit_count.get(), (uint64_t *)unit_off.get(), U.n, (uint64_t *)d_total.get(),
-                 s);
-        unsigned long long hm[4];
-        FCC_HIP(hipMemcpyAsync(hm, gmeta.get(), 32, hipMemcpyDeviceToHost, s));
-        nnz_out = d2h(d_total.get(), s);
         uint64_t gtotal = hm[2];
         FCC_HIP(hipEventRecord(ev[3], s));
         if (o.collect_stats) {
@@ -933,34 +959,20 @@ int Contraction::run(const fcc_tensor *ta, const int32_t *am, const fcc_tensor *
         X.ostride = stride;
         X.capacity = nnz_out;
         if (gtotal > (X.gmask + 1) / 2) alloc_ghash(2 * gtotal);
-        reset_ghash();
+        reset_ghash(false);
         arm_groups();
         FCC_HIP(hipEventRecord(ev[4], s));
         launch_numeric(X, grid, bitmap, sink, 0, s);
         FCC_HIP(hipEventRecord(ev[5], s));
         launch_gdrain(X, sink, 0, s);
         FCC_HIP(hipEventRecord(ev[6], s));
-        uint32_t hf[2];
-        FCC_HIP(hipMemcpyAsync(hf, flags.get(), 8, hipMemcpyDeviceToHost, s));
-        FCC_HIP(hipStreamSynchronize(s));
+        fetch_tail();
         if (hf[0]) throw LibError{FCC_ERR_OVERFLOW, "global hash overflow in the numeric pass"};
         if (hf[1]) throw LibError{FCC_ERR_OVERFLOW, "numeric pass disagreed with the count pass"};
-        st.ghash_entries = d2h(gmeta.get(), s);
+        st.ghash_entries = hm[0];
     }
 
-    // ---- results
-    k_kind_hist<<<std::min(grid256(U.n), 1024u), 256, 0, s>>>(U.kind, U.n, tail);
-    unsigned long long sk[4], th[9];
-    FCC_HIP(hipMemcpyAsync(sk, sinkbuf.get(), 32, hipMemcpyDeviceToHost, s));
-    FCC_HIP(hipMemcpyAsync(th, tail.get(), sizeof(th), hipMemcpyDeviceToHost, s));
-    std::vector<unsigned long long> gdone;
-    if (groups) {
-        group_sink.resize(nsink);
-        gdone.resize(groups + 1);
-        FCC_HIP(hipMemcpyAsync(group_sink.data(), sinkbuf.get(), 8 * nsink, hipMemcpyDeviceToHost, s));
-        FCC_HIP(hipMemcpyAsync(gdone.data(), g_done.get(), 8 * (groups + 1), hipMemcpyDeviceToHost, s));
-    }
-    FCC_HIP(hipStreamSynchronize(s));
+    // ---- results (read back by fetch_tail)
     st.units_dense = th[KIND_DENSE];
     st.units_lhash = th[KIND_LHASH];
     st.units_ghash = th[KIND_GHASH];

@@ -7,6 +7,21 @@ namespace fcc {
 void fill_u32(uint32_t *p, size_t n, uint32_t v, hipStream_t s);
 void fill_u64(unsigned long long *p, size_t n, unsigned long long v, hipStream_t s);
 void fill_f64(double *p, size_t n, double v, hipStream_t s);
+// up to 8 fills (4- or 8-byte elements) in one launch
+struct FillOp {
+    void *p;
+    size_t n;
+    unsigned long long v;  // the element's bits
+    uint32_t bytes;        // 4 or 8
+};
+struct FillSet {
+    FillOp op[8];
+    int n = 0;
+    void add(void *p, size_t count, unsigned long long v, uint32_t bytes) {
+        if (p && count) op[n++] = FillOp{p, count, v, bytes};
+    }
+};
+void fill_many(const FillSet &f, hipStream_t s);
 // dst[p] = src[perm[p]]
 void perm_gather(const double *src, const uint32_t *perm, size_t n, double *dst, hipStream_t s);
 

@@ -37,6 +37,23 @@ void perm_gather(const double *src, const uint32_t *perm, size_t n, double *dst,
 void fill_u32(uint32_t *p, size_t n, uint32_t v, hipStream_t s) {
     if (n) k_fill_u32<<<fill_grid(n), 256, 0, s>>>(p, n, v);
 }
+
+// Several fills in one launch: blockIdx.y picks the buffer (a short chain of
+// small fills otherwise costs one launch each, exposed after a host wait).
+__global__ void k_fill_many(FillSet f) {
+    const FillOp op = f.op[blockIdx.y];
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < op.n; i += (size_t)gridDim.x * blockDim.x) {
+        if (op.bytes == 8) static_cast<unsigned long long *>(op.p)[i] = op.v;
+        else static_cast<uint32_t *>(op.p)[i] = (uint32_t)op.v;
+    }
+}
+
+void fill_many(const FillSet &f, hipStream_t s) {
+    if (f.n <= 0) return;
+    size_t most = 0;
+    for (int i = 0; i < f.n; i++) most = std::max<size_t>(most, f.op[i].n);
+    k_fill_many<<<dim3(fill_grid(most), (unsigned)f.n), 256, 0, s>>>(f);
+}
 void fill_u64(unsigned long long *p, size_t n, unsigned long long v, hipStream_t s) {
     if (n) k_fill_u64<<<fill_grid(n), 256, 0, s>>>(p, n, v);
 }
