@@ -16,6 +16,7 @@
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
+#include <functional>
 #include <random>
 #include <string>
 #include <vector>
@@ -563,16 +564,19 @@ int Contraction::run(const fcc_tensor *ta, const int32_t *am, const fcc_tensor *
                           max_cells, dflt, dk, hold, s);
     }
     // The block depends only on the right fibers and the join: it runs on
-    // the side stream while the left tiles are built. Its buffers are all
-    // allocated before the fork; the join arrays live until the join below
-    // (side_join synchronises the side stream on any early exit first).
+    // the side stream beside the left-tile build, forked once the left sort
+    // is queued (the sort and the GEMM both want the LDS; what follows the
+    // sort -- gathers, groups, statistics -- is latency-bound and shares the
+    // CUs well). Its buffers are all allocated before the fork; the join
+    // arrays live until the join below (side_join synchronises the side
+    // stream on any early exit first).
     struct SideJoin {
         hipStream_t st = nullptr;
         ~SideJoin() {
             if (st) (void)hipStreamSynchronize(st);
         }
     } side_join;
-    if (dk.nh) {
+    std::function<void()> fork_dense = [&]() {
         dense_keys_alloc(pool, g, dk, ctx->num_cus, hold);
         FCC_HIP(hipEventRecord(ev[9], s));
         FCC_HIP(hipStreamWaitEvent(ctx->side, ev[9], 0));
@@ -580,17 +584,19 @@ int Contraction::run(const fcc_tensor *ta, const int32_t *am, const fcc_tensor *
         FCC_HIP(hipEventRecord(ev[7], ctx->side));
         dense_keys_block(ix, dk, fidA.get(), rowA.get(), A.vals, (size_t)A.nnz, ctx->side, ev[10], ev[11]);
         FCC_HIP(hipEventRecord(ev[8], ctx->side));
-    } else {
+    };
+    if (!dk.nh)
         for (int e : {7, 8, 10, 11}) FCC_HIP(hipEventRecord(ev[e], s));
-    }
     int logTL;
     uint32_t Wd;
     choose_tiles(ctx, o, g, products, nvalid, dk.nh > 0, &logTL, &Wd, &bucket, &tuned_wg);
     if (const char *hm = getenv("FCC_HEAVY_MIN")) ix.heavy_min = (uint32_t)std::max(1, atoi(hm));
     if (const char *hs = getenv("FCC_HSEG")) ix.hseg = (uint32_t)std::max(64, atoi(hs));
     ix.defer_hseg = dk.nh > 0;  // built once, after the dense keys are masked
-    if (self && o.left_tile_stride <= 1) build_left_self(pool, g, A, sj, logTL, ix, hold, s);
-    else build_left(pool, g, A, rowA, fidA, nvalid, logTL, ix, hold, s);
+    const std::function<void()> *after_sort = dk.nh ? &fork_dense : nullptr;
+    if (self && o.left_tile_stride <= 1) build_left_self(pool, g, A, sj, logTL, ix, hold, s, after_sort);
+    else build_left(pool, g, A, rowA, fidA, nvalid, logTL, ix, hold, s, after_sort);
+    if (dk.nh && !side_join.st) throw LibError{FCC_ERR_INVALID, "dense-key block was not started"};
     bfib.reset();
     vsB.reset();
     st.row_tile = (int32_t)ix.TL;

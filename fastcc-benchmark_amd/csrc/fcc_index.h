@@ -1,5 +1,7 @@
 // Index-build entry points (fcc_index.hip).
 #pragma once
+#include <functional>
+
 #include "fcc_internal.h"
 
 namespace fcc {
@@ -49,12 +51,17 @@ void join_left(DevPool &pool, const Geom &g, const Operand &A, const Index &ix, 
                uint64_t *products, uint32_t *nvalid, hipStream_t s);
 
 // Self-contraction: left tiles from the right operand's order (2 radix passes).
+// after_sort (optional) runs once the left entries are sorted, before the
+// rest of the build is queued (a point to start independent work on another
+// stream that should not compete with the sort).
 void build_left_self(DevPool &pool, const Geom &g, const Operand &A, const SelfJoin &sj, int logTL, Index &ix,
-                     std::vector<std::shared_ptr<void>> &hold, hipStream_t s);
+                     std::vector<std::shared_ptr<void>> &hold, hipStream_t s,
+                     const std::function<void()> *after_sort = nullptr);
 
 // Left operand tiles of 2^logTL rows: sort, groups, active tiles, statistics.
 void build_left(DevPool &pool, const Geom &g, const Operand &A, const uint32_t *rowA, const uint32_t *fidA,
-                uint32_t nvalid, int logTL, Index &ix, std::vector<std::shared_ptr<void>> &hold, hipStream_t s);
+                uint32_t nvalid, int logTL, Index &ix, std::vector<std::shared_ptr<void>> &hold, hipStream_t s,
+                const std::function<void()> *after_sort = nullptr);
 
 // Heavy-group segments (hdesc, atile_h0) from the group descriptors; run
 // again after descriptors change (new buffers are kept in hold).

@@ -730,7 +730,8 @@ static void left_from_sorted(DevPool &pool, const Operand &A, const uint64_t *ks
 }
 
 void build_left(DevPool &pool, const Geom &g, const Operand &A, const uint32_t *rowA, const uint32_t *fidA,
-                uint32_t nvalid, int logTL, Index &ix, std::vector<std::shared_ptr<void>> &hold, hipStream_t s) {
+                uint32_t nvalid, int logTL, Index &ix, std::vector<std::shared_ptr<void>> &hold, hipStream_t s,
+                const std::function<void()> *after_sort) {
     size_t n = (size_t)A.nnz;
     ix.logTL = logTL;
     ix.TL = 1u << logTL;
@@ -745,6 +746,7 @@ void build_left(DevPool &pool, const Geom &g, const Operand &A, const uint32_t *
     uint64_t *ks;
     uint32_t *vs;
     radix_sort_pairs(pool, k1, v1, k2, v2, n, std::min(nbits, 64), s, &ks, &vs);
+    if (after_sort) (*after_sort)();
     left_from_sorted(pool, A, ks, vs, nvalid, logTL, fbits, ix, hold, s);
 }
 
@@ -767,7 +769,7 @@ __global__ void k_self_composite(const uint32_t *Bcol, const uint32_t *bfib, con
 }
 
 void build_left_self(DevPool &pool, const Geom &g, const Operand &A, const SelfJoin &sj, int logTL, Index &ix,
-                     std::vector<std::shared_ptr<void>> &hold, hipStream_t s) {
+                     std::vector<std::shared_ptr<void>> &hold, hipStream_t s, const std::function<void()> *after_sort) {
     size_t n = (size_t)A.nnz;
     ix.logTL = logTL;
     ix.TL = 1u << logTL;
@@ -782,6 +784,7 @@ void build_left_self(DevPool &pool, const Geom &g, const Operand &A, const SelfJ
     uint64_t *ks;
     uint32_t *vs;
     radix_sort_pairs(pool, k1, v1, k2, v2, n, tbits + 1, s, &ks, &vs, fbits + logTL);
+    if (after_sort) (*after_sort)();
     left_from_sorted(pool, A, ks, vs, (uint32_t)n, logTL, fbits, ix, hold, s);
 }
 
