@@ -301,11 +301,14 @@ void launch_dgemm_tn(const double *AT, const double *Bd, double *C, uint64_t ra,
 }
 
 uint32_t dgemm_splits(uint64_t ra, uint64_t rb, uint32_t kp, int num_cus) {
-    // two workgroups per CU; small grids split k so the last wave of
-    // workgroups is nearly full, keeping at least 8 k steps per split
+    // two workgroups per CU. The GEMM runs beside the left-tile build, and
+    // every split is one more partial block each unit reads back, so k is
+    // split only when the tiles cannot fill half the workgroup slots; then
+    // the split makes the last wave of workgroups nearly full, keeping at
+    // least 8 k steps per split
     const uint64_t nblk = (ra / DK_BM) * (rb / DK_BN), slots = 2ull * (uint64_t)std::max(1, num_cus);
     const uint32_t nk = kp / DK_BK;
-    if (nblk >= 4 * slots) return 1;
+    if (2 * nblk >= slots) return 1;
     uint32_t best = 1;
     double best_eff = 0;
     for (uint32_t sp = 1; sp <= 8 && nk / sp >= 8; sp++) {
