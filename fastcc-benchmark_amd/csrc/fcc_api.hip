@@ -36,6 +36,7 @@ struct fcc_ctx {
     DevPool pool;
     std::string err;
     int num_cus = 256;
+    size_t mem_total = 0;  // device memory (read once: a driver query per call costs tens of microseconds)
     int clock_khz = 100000;  // wall_clock64 rate
     // learned planner
     std::vector<float> mlp_host;
@@ -553,9 +554,8 @@ int Contraction::run(const fcc_tensor *ta, const int32_t *am, const fcc_tensor *
     DenseKeys &dk = k.dk;
     dk = DenseKeys();
     if (!groups && o.dense_keys >= 0.0f && products) {
-        size_t mem_free = 0, mem_total = 0;
-        FCC_HIP(hipMemGetInfo(&mem_free, &mem_total));
-        const uint64_t max_cells = std::min<uint64_t>(1ull << 29, mem_free / 32);
+        // the block and its panels within an eighth of device memory
+        const uint64_t max_cells = std::min<uint64_t>(1ull << 29, ctx->mem_total / 64);
         // the default threshold also asks the block to pay for itself; an
         // explicit one is taken as given
         const bool dflt = o.dense_keys == 0.0f;
@@ -1096,6 +1096,7 @@ int fcc_create(fcc_ctx **out, int device) {
         hipDeviceProp_t prop;
         FCC_HIP(hipGetDeviceProperties(&prop, device));
         c->num_cus = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 256;
+        c->mem_total = prop.totalGlobalMem;
         int khz = 0;
         if (hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, device) == hipSuccess && khz > 0)
             c->clock_khz = khz;

@@ -263,15 +263,6 @@ __global__ void k_groups(const uint64_t *comp, const uint32_t *gflag, const uint
     }
 }
 
-__global__ void k_gdesc(const uint32_t *group_ptr, const uint32_t *group_fiber, const uint32_t *fiber_ptr,
-                        uint32_t ngroups, uint4 *gdesc) {
-    uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
-    if (g >= ngroups) return;
-    uint32_t f = group_fiber[g];
-    uint32_t fp0 = fiber_ptr[f], e0 = group_ptr[g];
-    gdesc[g] = make_uint4(fp0, fiber_ptr[f + 1] - fp0, e0, group_ptr[g + 1] - e0);
-}
-
 // segments per group: heavy groups are cut every HSEG fiber entries, light
 // groups have none
 __global__ void k_hseg_count(const uint4 *gdesc, const uint8_t *glight, uint32_t ngroups, uint32_t hseg,
@@ -362,7 +353,9 @@ __global__ void k_fiber_sketch(const uint32_t *fiber_ptr, const uint32_t *Bcol, 
     fsk[f] = make_uint2(h1, h2);
 }
 
-// Per active tile statistics, one thread per group, wave-aggregated atomics.
+// Per group: its descriptor (fiber start, fiber length, first entry, entries)
+// and, aggregated per active tile (block-wide when the block's groups share a
+// tile, else per group), the tile statistics:
 //  nnz      A entries of the tile          (num_nnz_in_tile, index_tensors.hpp:562)
 //  keys     distinct contraction keys      (num_active_columns, index_tensors.hpp:604)
 //  sumfib   summed right-fiber lengths
@@ -370,7 +363,7 @@ __global__ void k_fiber_sketch(const uint32_t *fiber_ptr, const uint32_t *Bcol, 
 //  sketch   two min-hashes over sampled fiber columns (distinct-column estimate)
 __global__ __launch_bounds__(256) void k_tile_stats(const uint32_t *group_ptr, const uint32_t *group_fiber,
                                                    const uint32_t *group_atile, const uint32_t *fiber_ptr,
-                                                   const uint2 *fsk, uint32_t ngroups, uint32_t *st_nnz,
+                                                   const uint2 *fsk, uint32_t ngroups, uint4 *gdesc, uint32_t *st_nnz,
                                                    uint32_t *st_keys, unsigned long long *st_sumfib,
                                                    unsigned long long *st_products, uint32_t *st_sketch,
                                                    unsigned long long *st_maxgp) {
@@ -384,9 +377,12 @@ __global__ __launch_bounds__(256) void k_tile_stats(const uint32_t *group_ptr, c
     uint64_t m = 0, nf = 0;
     uint32_t h1 = 0xFFFFFFFFu, h2 = 0xFFFFFFFFu;
     if (ok) {
-        m = group_ptr[g + 1] - group_ptr[g];
+        const uint32_t e0 = group_ptr[g];
+        m = group_ptr[g + 1] - e0;
         uint32_t f = group_fiber[g];
-        nf = fiber_ptr[f + 1] - fiber_ptr[f];
+        const uint32_t fp0 = fiber_ptr[f];
+        nf = fiber_ptr[f + 1] - fp0;
+        gdesc[g] = make_uint4(fp0, (uint32_t)nf, e0, (uint32_t)m);
         const uint2 hh = fsk[f];
         h1 = hh.x;
         h2 = hh.y;
@@ -691,12 +687,10 @@ static void left_from_sorted(DevPool &pool, const Operand &A, const uint64_t *ks
     if (m) {
         k_groups<<<grid_for(m), 256, 0, s>>>(ks, gflag, gid, tflag, aid, m, logTL, fbits, ix.group_ptr, ix.group_fiber,
                                              ix.group_atile, glight, ix.atile_id, ix.atile_g0);
-        k_gdesc<<<grid_for(ix.ngroups), 256, 0, s>>>(ix.group_ptr, ix.group_fiber, ix.fiber_ptr, ix.ngroups, ix.gdesc);
         k_tile_light_init<<<grid_for(ix.natiles), 256, 0, s>>>(ix.atile_g0, ix.natiles, ix.atile_lg);
         k_tile_light<<<grid_for(ix.ngroups), 256, 0, s>>>(glight, ix.group_atile, ix.atile_g0, ix.ngroups, ix.atile_lg);
     }
-    if (!ix.defer_hseg) build_hseg(pool, ix, hold, s);
-    // statistics
+    // descriptors + statistics (one pass over the groups)
     size_t T = ix.natiles;
     ix.st_nnz = (uint32_t *)keep(T * 4);
     ix.st_keys = (uint32_t *)keep(T * 4);
@@ -705,20 +699,23 @@ static void left_from_sorted(DevPool &pool, const Operand &A, const uint64_t *ks
     ix.st_sketch = (uint32_t *)keep(T * 8);
     ix.st_maxgp = (uint64_t *)keep(T * 8);
     if (T) {
-        fill_u32(ix.st_nnz, T, 0, s);
-        fill_u32(ix.st_keys, T, 0, s);
-        fill_u64((unsigned long long *)ix.st_sumfib, T, 0, s);
-        fill_u64((unsigned long long *)ix.st_products, T, 0, s);
-        fill_u32(ix.st_sketch, 2 * T, 0xFFFFFFFFu, s);
-        fill_u64((unsigned long long *)ix.st_maxgp, T, 0, s);
+        FillSet f;
+        f.add(ix.st_nnz, T, 0, 4);
+        f.add(ix.st_keys, T, 0, 4);
+        f.add(ix.st_sumfib, T, 0, 8);
+        f.add(ix.st_products, T, 0, 8);
+        f.add(ix.st_sketch, 2 * T, 0xFFFFFFFFull, 4);
+        f.add(ix.st_maxgp, T, 0, 8);
+        fill_many(f, s);
         Tmp<uint2> fsk(pool, ix.nfib);
         if (ix.nfib) k_fiber_sketch<<<grid_for(ix.nfib), 256, 0, s>>>(ix.fiber_ptr, ix.Bcol, ix.nfib, fsk);
         k_tile_stats<<<grid_for(ix.ngroups), 256, 0, s>>>(ix.group_ptr, ix.group_fiber, ix.group_atile, ix.fiber_ptr,
-                                                          fsk, ix.ngroups, ix.st_nnz, ix.st_keys,
+                                                          fsk, ix.ngroups, ix.gdesc, ix.st_nnz, ix.st_keys,
                                                           (unsigned long long *)ix.st_sumfib,
                                                           (unsigned long long *)ix.st_products, ix.st_sketch,
                                                           (unsigned long long *)ix.st_maxgp);
     }
+    if (!ix.defer_hseg) build_hseg(pool, ix, hold, s);
     ix.st_direct = (uint8_t *)keep(T + 1);
     if (T)
         k_tile_direct<<<grid_for(T), 256, 0, s>>>(ix.atile_g0, ix.group_ptr, ix.Ali, ix.st_nnz, (uint32_t)T, ix.TL,
