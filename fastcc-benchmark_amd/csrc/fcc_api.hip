@@ -593,7 +593,12 @@ int Contraction::run(const fcc_tensor *ta, const int32_t *am, const fcc_tensor *
     if (const char *hm = getenv("FCC_HEAVY_MIN")) ix.heavy_min = (uint32_t)std::max(1, atoi(hm));
     if (const char *hs = getenv("FCC_HSEG")) ix.hseg = (uint32_t)std::max(64, atoi(hs));
     ix.defer_hseg = dk.nh > 0;  // built once, after the dense keys are masked
-    const std::function<void()> *after_sort = dk.nh ? &fork_dense : nullptr;
+    // A large GEMM (C4-sized: hundreds of GFLOP) outlasts the whole left-tile
+    // build, so it starts before the left sort; a small one waits for the
+    // sort to be queued (they compete for the LDS).
+    const bool early_fork = dk.nh && 2.0 * (double)dk.ra * (double)dk.rb * (double)dk.kp > 1e11;
+    if (early_fork) fork_dense();
+    const std::function<void()> *after_sort = (dk.nh && !early_fork) ? &fork_dense : nullptr;
     if (self && o.left_tile_stride <= 1) build_left_self(pool, g, A, sj, logTL, ix, hold, s, after_sort);
     else build_left(pool, g, A, rowA, fidA, nvalid, logTL, ix, hold, s, after_sort);
     if (dk.nh && !side_join.st) throw LibError{FCC_ERR_INVALID, "dense-key block was not started"};

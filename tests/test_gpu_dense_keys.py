@@ -123,3 +123,16 @@ def test_plan_new_values(gpu_ctx, use_graph):
         r = plan.execute(a2.vals, b2.vals, graph=use_graph)
         _check(r, ref, False, f"plan it {it} graph {use_graph}")
     plan.close()
+
+
+def test_sampled_left_rows(gpu_ctx):
+    """Sampled left tiles (the large-shape baseline mode) with dense keys:
+    the keys are counted from the sampled rows only."""
+    wl = W.c2(0.05)
+    a, _ = wl.tensors()
+    ref = oracle.contract_tiled(a, wl.a_modes, a, wl.b_modes, tile_size=8, sample_stride=3)
+    r = gpu_ctx.contract(a, list(wl.a_modes), a, list(wl.b_modes), tile_hint=8, left_tile_stride=3,
+                         dense_keys=0.005)
+    assert r.stats["dense_keys"] > 0
+    _check(r, ref, False, "sampled dense keys")
+    r.free()
