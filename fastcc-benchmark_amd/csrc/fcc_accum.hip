@@ -137,8 +137,14 @@ __device__ void gset_insert(const AccArgs &A, uint64_t key, uint32_t unit) {
 // by a new key is sealed instead of claimed, and any key whose probe meets a
 // sealed slot lives in the global hash. A key claimed earlier always sits
 // before any slot sealed on its probe path, so it is never duplicated.
+// The key count is added per wavefront after the probe (lds_probe_counted):
+// one LDS atomic per wavefront instead of 64 on one address when most
+// products are new keys. The count may lag by one probe round of every
+// wavefront of the block (at most ACC_THREADS keys), so a table holds at most
+// limit + ACC_THREADS keys, which stays below its slot count.
 template <int BITS>
-__device__ __forceinline__ int lds_probe(uint32_t *keys, uint32_t *cnt, uint32_t limit, uint32_t key) {
+__device__ __forceinline__ int lds_probe(uint32_t *keys, const uint32_t *cnt, uint32_t limit, uint32_t key,
+                                         bool &claimed) {
     // returns slot index, or -1 when the key must go to the global hash
     uint32_t h = (key * 2654435761u) >> (32 - BITS);
     for (uint32_t probe = 0; probe < (1u << BITS); probe++) {
@@ -150,7 +156,7 @@ __device__ __forceinline__ int lds_probe(uint32_t *keys, uint32_t *cnt, uint32_t
             uint32_t old = atomicCAS(&keys[h], LEMPTY, want);
             if (old == LEMPTY) {
                 if (want == LSEALED) return -1;
-                atomicAdd(cnt, 1u);
+                claimed = true;
                 return (int)h;
             }
             if (old == key) return (int)h;
@@ -159,6 +165,23 @@ __device__ __forceinline__ int lds_probe(uint32_t *keys, uint32_t *cnt, uint32_t
         h = (h + 1) & ((1u << BITS) - 1);
     }
     return -1;
+}
+
+static_assert(LH_SLOTS >= LH_MAX + ACC_THREADS && SYM_SLOTS >= SYM_MAX + ACC_THREADS,
+              "an LDS table must hold its limit plus one probe round of every wavefront");
+
+// Every active lane calls this together (divergent code is fine: the ballot
+// covers the lanes that are here).
+template <int BITS>
+__device__ __forceinline__ int lds_probe_counted(uint32_t *keys, uint32_t *cnt, uint32_t limit, uint32_t key) {
+    bool claimed = false;
+    const int slot = lds_probe<BITS>(keys, cnt, limit, key, claimed);
+    const uint64_t bal = __ballot(claimed);
+    if (bal) {
+        const uint32_t lane = threadIdx.x & 63;
+        if (lane == (uint32_t)(__ffsll((unsigned long long)bal) - 1)) atomicAdd(cnt, (uint32_t)__popcll(bal));
+    }
+    return slot;
 }
 
 // ------------------------------------------------------------- per unit
@@ -266,7 +289,8 @@ __device__ __forceinline__ void accum(const AccArgs &A, const UnitCtx &U, const 
         if (NUM) lds_add_f64(&S.fv[slot], v);
     } else if (U.kind == KIND_LHASH) {
         uint32_t key = li * U.W + col;
-        int slot = NUM ? lds_probe<12>(S.keys, S.cnt, LH_MAX, key) : lds_probe<14>(S.keys, S.cnt, SYM_MAX, key);
+        int slot = NUM ? lds_probe_counted<12>(S.keys, S.cnt, LH_MAX, key)
+                       : lds_probe_counted<14>(S.keys, S.cnt, SYM_MAX, key);
         if (slot >= 0) {
             if (NUM) lds_add_f64(&S.fv[slot], v);
         } else {
@@ -967,7 +991,8 @@ __device__ void dk_seed_cells(const AccArgs &A, const UnitCtx &U, const Acc<NUM,
             if (U.kind == KIND_LHASH) {
                 const uint32_t key = li * U.W + cl;
                 const int slot =
-                    NUM ? lds_probe<12>(S.keys, S.cnt, LH_MAX, key) : lds_probe<14>(S.keys, S.cnt, SYM_MAX, key);
+                    NUM ? lds_probe_counted<12>(S.keys, S.cnt, LH_MAX, key)
+                        : lds_probe_counted<14>(S.keys, S.cnt, SYM_MAX, key);
                 if (slot >= 0) {
                     if (NUM) lds_add_f64(&S.fv[slot], dk_value(A, row, U.c0 + cl));
                     continue;
