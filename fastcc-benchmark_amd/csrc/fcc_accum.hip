@@ -251,10 +251,12 @@ struct UnitCtx {
 
 template <bool NUM, bool BITMAP>
 struct Acc {
-    // NUM: DENSE f64 slots / LHASH vals+keys; SYM: DENSE bitmap / HASH key set
+    // NUM: DENSE f64 slots / LHASH vals+keys; SYM: DENSE byte map / HASH key set
     double *fv;       // numeric values
     uint32_t *keys;   // LHASH keys (numeric) or key set (count)
-    uint32_t *bits;   // DENSE bitmap (count pass, or numeric BITMAP mode)
+    uint32_t *bits;   // DENSE bitmap (numeric BITMAP mode)
+    uint8_t *seen;    // DENSE count pass: one byte per slot, set to 1 by plain
+                      // stores (idempotent: no atomics, no per-word merging)
     uint32_t *cnt;
     SinkAcc *sk;      // DIRECT units: count + checksum outputs go here
     bool sink;
@@ -285,7 +287,8 @@ __device__ __forceinline__ void accum(const AccArgs &A, const UnitCtx &U, const 
                                       uint32_t col, double v) {
     if (U.kind == KIND_DENSE) {
         uint32_t slot = li * U.Wp + col;
-        if (!NUM || BITMAP) atomicOr(&S.bits[slot >> 5], 1u << (slot & 31));
+        if (!NUM) S.seen[slot] = 1;
+        if (NUM && BITMAP) atomicOr(&S.bits[slot >> 5], 1u << (slot & 31));
         if (NUM) lds_add_f64(&S.fv[slot], v);
     } else if (U.kind == KIND_LHASH) {
         uint32_t key = li * U.W + col;
@@ -337,7 +340,7 @@ __device__ void dense_heavy_owned(const AccArgs &A, const UnitCtx &U, const Acc<
     const uint32_t g0 = A.atile_g0[U.a], gl = A.atile_lg[U.a];
     const uint32_t cend = U.c0 + U.W;
     const uint32_t wpr = U.Wp >> 5;
-    constexpr bool BITS = !NUM || BITMAP;
+    constexpr bool BITS = NUM && BITMAP;
     for (uint32_t bg = g0; bg < gl; bg += 64) {
         const uint32_t nb = min(64u, gl - bg);
         uint4 d = make_uint4(0, 0, 0, 0);
@@ -425,6 +428,7 @@ __device__ void dense_heavy_owned(const AccArgs &A, const UnitCtx &U, const Acc<
                         own &= own - 1;
                         const uint32_t li = rlane(li_l, kk);
                         if (BITS && leader) S.bits[li * wpr + wc] |= bm;
+                        if (!NUM && ok) S.seen[li * U.Wp + col] = 1;
                         if (NUM && ok) {
                             double *slot = &S.fv[li * U.Wp + col];
                             *slot = *slot + rlane_f64(av_l, kk) * b;
@@ -472,7 +476,7 @@ __device__ __forceinline__ void heavy_chunk(const AccArgs &A, const UnitCtx &U, 
     if (DONLY || U.kind == KIND_DENSE) {
         uint32_t wc = 0, bm = 0;
         bool leader = false;
-        if (!NUM || BITMAP) {
+        if (NUM && BITMAP) {
             // The chunk's columns are ascending, so lanes sharing a 32-column
             // bitmap word are contiguous; OR their bits together and let one
             // lane per word touch the bitmap.
@@ -489,7 +493,8 @@ __device__ __forceinline__ void heavy_chunk(const AccArgs &A, const UnitCtx &U, 
         const uint32_t wpr = U.Wp >> 5;
         for (uint32_t kk = 0; kk < mk; kk++) {
             const uint32_t li = rlane(li_l, kk);
-            if ((!NUM || BITMAP) && leader) atomicOr(&S.bits[li * wpr + wc], bm);
+            if (NUM && BITMAP && leader) atomicOr(&S.bits[li * wpr + wc], bm);
+            if (!NUM && ok) S.seen[li * U.Wp + col] = 1;
             if (NUM && ok) lds_add_f64(&S.fv[li * U.Wp + col], rlane_f64(av_l, kk) * b);
         }
     } else {
@@ -1103,6 +1108,7 @@ __global__ __launch_bounds__(ACC_THREADS, 4) void k_numeric(AccArgs A) {
     S.fv = s_f64;
     S.keys = reinterpret_cast<uint32_t *>(s_f64 + LH_SLOTS);
     S.bits = s_bits;
+    S.seen = nullptr;
     S.cnt = &s_cnt;
     S.sk = &sk;
     S.sink = SINK;
@@ -1260,6 +1266,8 @@ __global__ __launch_bounds__(ACC_THREADS, 4) void k_numeric(AccArgs A) {
     if (SINK) sink_flush(A, sk, s_su, s_sd);
 }
 
+static_assert(DENSE_SLOTS <= 4 * SYM_SLOTS, "the count pass's dense byte map must fit its key set");
+
 // Count pass: structural support only.
 __global__ __launch_bounds__(ACC_THREADS) void k_count(AccArgs A) {
     __shared__ __align__(16) uint32_t s_set[SYM_SLOTS];
@@ -1268,7 +1276,8 @@ __global__ __launch_bounds__(ACC_THREADS) void k_count(AccArgs A) {
     Acc<false, false> S;
     S.fv = nullptr;
     S.keys = s_set;
-    S.bits = s_set;
+    S.bits = nullptr;
+    S.seen = reinterpret_cast<uint8_t *>(s_set);
     S.cnt = &s_cnt;
     S.sk = nullptr;
     S.sink = false;
@@ -1297,11 +1306,23 @@ __global__ __launch_bounds__(ACC_THREADS) void k_count(AccArgs A) {
             continue;
         }
         const uint32_t TL = 1u << A.logTL;
-        const uint32_t nwords = U.kind == KIND_DENSE ? TL * U.Wp / 32 : 0;
+        // dense units: a byte per slot, read back four slots per word
+        const uint32_t nwords = U.kind == KIND_DENSE ? TL * U.Wp / 4 : 0;
         if (U.kind == KIND_DENSE) {
             const bool seeded = A.dk_tile && A.dk_tile[U.a];
-            for (uint32_t s = threadIdx.x; s < nwords; s += ACC_THREADS)
-                s_set[s] = seeded ? dk_dense_word(A, U, s) : 0u;
+            if (seeded) {
+                // presence bits of the block, one nibble per byte word
+                for (uint32_t w = threadIdx.x; w < nwords / 8; w += ACC_THREADS) {
+                    const uint32_t b = dk_dense_word(A, U, w);
+#pragma unroll
+                    for (int j = 0; j < 8; j++) {
+                        const uint32_t n = (b >> (4 * j)) & 0xFu;
+                        s_set[8 * w + j] = (n & 1u) | ((n & 2u) << 7) | ((n & 4u) << 14) | ((n & 8u) << 21);
+                    }
+                }
+            } else {
+                for (uint32_t s = threadIdx.x; s < nwords; s += ACC_THREADS) s_set[s] = 0u;
+            }
         } else if (U.kind == KIND_LHASH) {
             for (uint32_t s = threadIdx.x; s < SYM_SLOTS; s += ACC_THREADS) s_set[s] = LEMPTY;
         }

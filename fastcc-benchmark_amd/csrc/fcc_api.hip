@@ -609,7 +609,13 @@ int Contraction::run(const fcc_tensor *ta, const int32_t *am, const fcc_tensor *
     st.left_tiles = ix.natiles;
 
     dense_keys_mask(pool, ix, dk, hold, s);
-    if (dk.nh) FCC_HIP(hipStreamWaitEvent(s, ev[8], 0));  // join the block
+    // Planning and the count pass read only the block's presence map (ready
+    // before the GEMM starts, event 10); the numeric pass reads its values and
+    // joins the GEMM itself (event 8), so the GEMM runs on beside the count.
+    if (dk.nh) FCC_HIP(hipStreamWaitEvent(s, ev[10], 0));
+    auto join_block = [&]() {
+        if (dk.nh) FCC_HIP(hipStreamWaitEvent(s, ev[8], 0));
+    };
     rowA.reset();
     fidA.reset();
     st.dense_keys = dk.nh;
@@ -870,6 +876,7 @@ int Contraction::run(const fcc_tensor *ta, const int32_t *am, const fcc_tensor *
         // never more than the products bound: no tile can emit more outputs than products
         uint64_t capacity = std::min<uint64_t>((uint64_t)((double)predicted * 1.10) + 4096, products + 16);
         alloc_ghash(1u << 20);
+        join_block();
         FCC_HIP(hipEventRecord(ev[3], s));
         Tmp<unsigned long long> tn(pool, T);
         for (int attempt = 0;; attempt++) {
@@ -942,6 +949,7 @@ int Contraction::run(const fcc_tensor *ta, const int32_t *am, const fcc_tensor *
             alloc_ghash((X.gmask + 1) * 16);
         }
         uint64_t gtotal = hm[2];
+        join_block();
         FCC_HIP(hipEventRecord(ev[3], s));
         if (o.collect_stats) {
             // labels: exact tile nnz and distinct output columns

@@ -435,14 +435,37 @@ __global__ __launch_bounds__(256) void k_tile_stats(const uint32_t *group_ptr, c
         }
         return;
     }
-    if (ok) {
-        atomicAdd(&st_nnz[a], (uint32_t)m);
-        atomicAdd(&st_keys[a], 1u);
-        atomicAdd(&st_sumfib[a], (unsigned long long)nf);
-        atomicAdd(&st_products[a], (unsigned long long)(m * nf));
+    // Small tiles: groups are in tile order, so the lanes of one tile are
+    // contiguous. A segmented reduction towards each run's first lane (the
+    // partner d lanes on belongs to the same run exactly when its tile is
+    // equal) leaves one set of atomics per tile per wavefront.
+    const uint32_t lane = threadIdx.x & 63;
+    uint32_t sm = (uint32_t)m, sk = ok ? 1u : 0u;
+    uint64_t snf = nf, sp = m * nf, mx = m * nf;
+    for (int d = 1; d < 64; d <<= 1) {
+        const uint32_t oa = __shfl_down(a, d, 64);
+        const uint32_t om = __shfl_down(sm, d, 64), ok2 = __shfl_down(sk, d, 64);
+        const uint64_t onf = __shfl_down(snf, d, 64), op = __shfl_down(sp, d, 64), omx = __shfl_down(mx, d, 64);
+        const uint32_t o1 = __shfl_down(h1, d, 64), o2 = __shfl_down(h2, d, 64);
+        if (lane + d < 64 && oa == a) {
+            sm += om;
+            sk += ok2;
+            snf += onf;
+            sp += op;
+            mx = omx > mx ? omx : mx;
+            h1 = o1 < h1 ? o1 : h1;
+            h2 = o2 < h2 ? o2 : h2;
+        }
+    }
+    const uint32_t pa = __shfl_up(a, 1, 64);
+    if (ok && (lane == 0 || pa != a)) {
+        atomicAdd(&st_nnz[a], sm);
+        atomicAdd(&st_keys[a], sk);
+        atomicAdd(&st_sumfib[a], (unsigned long long)snf);
+        atomicAdd(&st_products[a], (unsigned long long)sp);
         atomicMin(&st_sketch[2 * a], h1);
         atomicMin(&st_sketch[2 * a + 1], h2);
-        atomicMax(&st_maxgp[a], (unsigned long long)(m * nf));
+        atomicMax(&st_maxgp[a], (unsigned long long)mx);
     }
 }
 
