@@ -202,11 +202,18 @@ __global__ __launch_bounds__(256) void k_dk_scatter_B(const uint32_t *hfib, cons
 // MFMA operand layout (16x16x4, f64): lane l holds A[i = l % 16][k = l / 16]
 // and B[k = l / 16][j = l % 16]; the result block D[i][j] sits in lane
 // j + 16 * (i % 4), element i / 4.
+// LDS banks: an operand read takes 16 consecutive doubles from each of four
+// k rows; k rows are DK_LDS_ROW = 144 doubles apart (128 B past a multiple of
+// 256 B), so two rows read in the same cycle fall on disjoint banks. Panel
+// stores write 256 contiguous bytes per 16 lanes.
+constexpr uint32_t DK_LDS_ROW = DK_BM + 16;
+static_assert(DK_BM == DK_BN, "one padded LDS row layout for both panels");
+
 __global__ __launch_bounds__(256, 2) void k_dgemm_tn(const double *__restrict__ AT, const double *__restrict__ Bd,
                                                      double *__restrict__ C, uint32_t nbm, uint32_t nbn, uint64_t ra,
                                                      uint64_t rb, uint32_t kp, uint32_t splits) {
-    __shared__ __align__(16) double sA[2][DK_BK][DK_BM];
-    __shared__ __align__(16) double sB[2][DK_BK][DK_BN];
+    __shared__ __align__(16) double sA[2][DK_BK][DK_LDS_ROW];
+    __shared__ __align__(16) double sB[2][DK_BK][DK_LDS_ROW];
     // Workgroups are dealt round-robin over the 8 XCDs (one L2 each): give
     // every XCD a contiguous run of logical tiles, and walk logical tiles down
     // bands of 8 row tiles so neighbours share AT rows and a Bd column panel.
@@ -224,7 +231,8 @@ __global__ __launch_bounds__(256, 2) void k_dgemm_tn(const double *__restrict__ 
 
     const uint32_t t = threadIdx.x, lane = t & 63, wave = t >> 6;
     const uint32_t wm = (wave >> 1) * 64, wn = (wave & 1) * 64;
-    const uint32_t lr = t >> 4, lc = (t & 15) * 8;  // panel copy: k row lr, 8 doubles from lc
+    // panel copy: k row lr, doubles lc + 32 v (v = 0..3) as pairs
+    const uint32_t lr = t >> 4, lc = (t & 15) * 2;
     const uint32_t nk = kp / DK_BK;
     const uint32_t kb0 = (uint32_t)((uint64_t)z * nk / splits), kb1 = (uint32_t)((uint64_t)(z + 1) * nk / splits);
     const double *gA = AT + ((uint64_t)kb0 * DK_BK + lr) * ra + (uint64_t)tm * DK_BM + lc;
@@ -238,13 +246,13 @@ __global__ __launch_bounds__(256, 2) void k_dgemm_tn(const double *__restrict__ 
     double2 pa[4], pb[4];
 #pragma unroll
     for (int v = 0; v < 4; v++) {
-        pa[v] = *reinterpret_cast<const double2 *>(gA + 2 * v);
-        pb[v] = *reinterpret_cast<const double2 *>(gB + 2 * v);
+        pa[v] = *reinterpret_cast<const double2 *>(gA + 32 * v);
+        pb[v] = *reinterpret_cast<const double2 *>(gB + 32 * v);
     }
 #pragma unroll
     for (int v = 0; v < 4; v++) {
-        *reinterpret_cast<double2 *>(&sA[0][lr][lc + 2 * v]) = pa[v];
-        *reinterpret_cast<double2 *>(&sB[0][lr][lc + 2 * v]) = pb[v];
+        *reinterpret_cast<double2 *>(&sA[0][lr][lc + 32 * v]) = pa[v];
+        *reinterpret_cast<double2 *>(&sB[0][lr][lc + 32 * v]) = pb[v];
     }
     __syncthreads();
     const uint32_t kr = lane >> 4, ci = lane & 15;
@@ -258,8 +266,8 @@ __global__ __launch_bounds__(256, 2) void k_dgemm_tn(const double *__restrict__ 
             const uint64_t oa = (uint64_t)kn * DK_BK * ra, ob = (uint64_t)kn * DK_BK * rb;
 #pragma unroll
             for (int v = 0; v < 4; v++) {
-                pa[v] = *reinterpret_cast<const double2 *>(gA + oa + 2 * v);
-                pb[v] = *reinterpret_cast<const double2 *>(gB + ob + 2 * v);
+                pa[v] = *reinterpret_cast<const double2 *>(gA + oa + 32 * v);
+                pb[v] = *reinterpret_cast<const double2 *>(gB + ob + 32 * v);
             }
         }
 #pragma unroll
@@ -277,8 +285,8 @@ __global__ __launch_bounds__(256, 2) void k_dgemm_tn(const double *__restrict__ 
         }
 #pragma unroll
         for (int v = 0; v < 4; v++) {
-            *reinterpret_cast<double2 *>(&sA[cur ^ 1][lr][lc + 2 * v]) = pa[v];
-            *reinterpret_cast<double2 *>(&sB[cur ^ 1][lr][lc + 2 * v]) = pb[v];
+            *reinterpret_cast<double2 *>(&sA[cur ^ 1][lr][lc + 32 * v]) = pa[v];
+            *reinterpret_cast<double2 *>(&sB[cur ^ 1][lr][lc + 32 * v]) = pb[v];
         }
         __syncthreads();
     }

@@ -172,11 +172,15 @@ __global__ __launch_bounds__(RS_THREADS) void k_rs_hist(const uint64_t *keys, si
     h[threadIdx.x] = 0;
     __syncthreads();
     size_t t0 = (size_t)blockIdx.x * RS_TILE;
-#pragma unroll 4
+    uint32_t dg[RS_ITEMS];  // all loads in flight before the first atomic
+#pragma unroll
     for (int j = 0; j < RS_ITEMS; j++) {
         size_t idx = t0 + (size_t)j * RS_THREADS + threadIdx.x;
-        if (idx < n) atomicAdd(&h[(uint32_t)(keys[idx] >> shift) & 255u], 1u);
+        dg[j] = idx < n ? (uint32_t)(keys[idx] >> shift) & 255u : 256u;
     }
+#pragma unroll
+    for (int j = 0; j < RS_ITEMS; j++)
+        if (dg[j] < 256u) atomicAdd(&h[dg[j]], 1u);
     __syncthreads();
     hist[(size_t)threadIdx.x * nblocks + blockIdx.x] = h[threadIdx.x];
 }
@@ -196,10 +200,23 @@ __global__ __launch_bounds__(RS_THREADS) void k_rs_scatter(const uint64_t *keys,
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const size_t t0 = (size_t)blockIdx.x * RS_TILE;
     const uint32_t tn = (uint32_t)(n - t0 < (size_t)RS_TILE ? n - t0 : (size_t)RS_TILE);
+    // The whole tile is loaded into registers up front (item j of a thread is
+    // tile element j * RS_THREADS + tid): one memory latency per tile instead
+    // of one per ranking round, and no second read for the local histogram.
+    uint64_t rk[RS_ITEMS];
+    uint32_t rv[RS_ITEMS];
+#pragma unroll
+    for (int j = 0; j < RS_ITEMS; j++) {
+        const uint32_t li = (uint32_t)j * RS_THREADS + tid;
+        rk[j] = li < tn ? keys[t0 + li] : 0ull;
+        rv[j] = li < tn ? vals[t0 + li] : 0u;
+    }
     // local histogram -> exclusive scan (one digit per thread)
     s_run[tid] = 0;
     __syncthreads();
-    for (uint32_t i = tid; i < tn; i += RS_THREADS) atomicAdd(&s_run[(uint32_t)(keys[t0 + i] >> shift) & 255u], 1u);
+#pragma unroll
+    for (int j = 0; j < RS_ITEMS; j++)
+        if ((uint32_t)j * RS_THREADS + tid < tn) atomicAdd(&s_run[(uint32_t)(rk[j] >> shift) & 255u], 1u);
     __syncthreads();
     {
         uint32_t c = s_run[tid];
@@ -222,11 +239,12 @@ __global__ __launch_bounds__(RS_THREADS) void k_rs_scatter(const uint64_t *keys,
     }
     __syncthreads();
     const uint64_t lt_mask = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+#pragma unroll
     for (int j = 0; j < RS_ITEMS; j++) {
         uint32_t li = (uint32_t)j * RS_THREADS + tid;
         bool valid = li < tn;
-        uint64_t k = valid ? keys[t0 + li] : 0;
-        uint32_t v = valid ? vals[t0 + li] : 0;
+        const uint64_t k = rk[j];
+        const uint32_t v = rv[j];
         uint32_t d = (uint32_t)(k >> shift) & 255u;
         uint64_t peers = __ballot(valid);
 #pragma unroll
