@@ -213,13 +213,6 @@ void choose_tiles(fcc_ctx *ctx, const fcc_options &o, const Geom &g, uint64_t pr
     *Wd = (uint32_t)W;
 }
 
-template <typename T> T d2h(const T *dev, hipStream_t s) {
-    T h;
-    FCC_HIP(hipMemcpyAsync(&h, dev, sizeof(T), hipMemcpyDeviceToHost, s));
-    FCC_HIP(hipStreamSynchronize(s));
-    return h;
-}
-
 __global__ void k_sample_rows(const uint32_t *rowA, uint32_t *fidA, size_t n, int logT, uint32_t stride) {
     size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
     if (i < n && ((rowA[i] >> logT) % stride) != 0) fidA[i] = 0xFFFFFFFFu;
@@ -433,6 +426,7 @@ struct Contraction {
 int Contraction::run(const fcc_tensor *ta, const int32_t *am, const fcc_tensor *tb, const int32_t *bm, int r,
                      int nb) {
     DevPool &pool = ctx->pool;
+    pool.discard();  // read-backs left queued by a call that failed
     hipEvent_t *ev = ctx->ev;
     FCC_HIP(hipEventRecord(ev[0], s));
 
@@ -471,8 +465,8 @@ int Contraction::run(const fcc_tensor *ta, const int32_t *am, const fcc_tensor *
     launch_extents(A, dext.get(), s);
     if (!same_operand) launch_extents(B, dext.get() + kMaxOrder + 1, s);
     uint32_t hext[2 * (kMaxOrder + 1)];
-    FCC_HIP(hipMemcpyAsync(hext, dext.get(), sizeof(hext), hipMemcpyDeviceToHost, s));
-    FCC_HIP(hipStreamSynchronize(s));
+    pool.read(hext, dext.get(), sizeof(hext), s);
+    pool.sync(s);
     if (same_operand) memcpy(hext + kMaxOrder + 1, hext, (kMaxOrder + 1) * sizeof(uint32_t));
     uint32_t EA[kMaxOrder], EB[kMaxOrder];
     for (int m = 0; m < ta->order; m++) EA[m] = ta->extents ? ta->extents[m] : (ta->nnz ? hext[m] + 1 : 0);
@@ -542,8 +536,8 @@ int Contraction::run(const fcc_tensor *ta, const int32_t *am, const fcc_tensor *
         fill_u64(cv, 2, 0, s);
         k_count_valid<<<grid256(A.nnz), 256, 0, s>>>(fidA, ix.fiber_ptr, (size_t)A.nnz, cv);
         unsigned long long h[2];
-        FCC_HIP(hipMemcpyAsync(h, cv.get(), 16, hipMemcpyDeviceToHost, s));
-        FCC_HIP(hipStreamSynchronize(s));
+        pool.read(h, cv.get(), 16, s);
+        pool.sync(s);
         nvalid = (uint32_t)h[0];
         products = h[1];
     }
@@ -772,9 +766,9 @@ int Contraction::run(const fcc_tensor *ta, const int32_t *am, const fcc_tensor *
         P.dense_ratio = pred ? 0.15f : 0.5f;
         launch_plan_tiles(ix, pred, P, t_nwin, t_W, t_kind, s);
         scan_u32(pool, t_nwin, t_ubase, T, d_nunits, s);
-        FCC_HIP(hipMemcpyAsync(&U.n, d_nunits.get(), 4, hipMemcpyDeviceToHost, s));
-        if (psum) FCC_HIP(hipMemcpyAsync(psum_h, psum, 8, hipMemcpyDeviceToHost, s));
-        FCC_HIP(hipStreamSynchronize(s));
+        pool.read(&U.n, d_nunits.get(), 4, s);
+        if (psum) pool.read(psum_h, psum, 8, s);
+        pool.sync(s);
         u_tile = Tmp<uint32_t>(pool, U.n);
         u_c0 = Tmp<uint32_t>(pool, U.n);
         u_W = Tmp<uint32_t>(pool, U.n);
@@ -852,17 +846,17 @@ int Contraction::run(const fcc_tensor *ta, const int32_t *am, const fcc_tensor *
     std::vector<unsigned long long> gdone;
     auto fetch_tail = [&]() {
         k_kind_hist<<<std::min(grid256(U.n), 1024u), 256, 0, s>>>(U.kind, U.n, tail);
-        FCC_HIP(hipMemcpyAsync(hf, flags.get(), 8, hipMemcpyDeviceToHost, s));
-        FCC_HIP(hipMemcpyAsync(hm, gmeta.get(), 32, hipMemcpyDeviceToHost, s));
-        FCC_HIP(hipMemcpyAsync(sk, sinkbuf.get(), 32, hipMemcpyDeviceToHost, s));
-        FCC_HIP(hipMemcpyAsync(th, tail.get(), sizeof(th), hipMemcpyDeviceToHost, s));
+        pool.read(hf, flags.get(), 8, s);
+        pool.read(hm, gmeta.get(), 32, s);
+        pool.read(sk, sinkbuf.get(), 32, s);
+        pool.read(th, tail.get(), sizeof(th), s);
         if (groups) {
             group_sink.resize(nsink);
             gdone.resize(groups + 1);
-            FCC_HIP(hipMemcpyAsync(group_sink.data(), sinkbuf.get(), 8 * nsink, hipMemcpyDeviceToHost, s));
-            FCC_HIP(hipMemcpyAsync(gdone.data(), g_done.get(), 8 * (groups + 1), hipMemcpyDeviceToHost, s));
+            pool.read(group_sink.data(), sinkbuf.get(), 8 * nsink, s);
+            pool.read(gdone.data(), g_done.get(), 8 * (groups + 1), s);
         }
-        FCC_HIP(hipStreamSynchronize(s));
+        pool.sync(s);
     };
 
     auto &out_c = k.out_c;
@@ -940,10 +934,10 @@ int Contraction::run(const fcc_tensor *ta, const int32_t *am, const fcc_tensor *
             scan_u64(pool, (const uint64_t *)unit_count.get(), (uint64_t *)unit_off.get(), U.n,
                      (uint64_t *)d_total.get(), s);
             uint32_t gf = 0;
-            FCC_HIP(hipMemcpyAsync(&gf, flags.get(), 4, hipMemcpyDeviceToHost, s));
-            FCC_HIP(hipMemcpyAsync(hm, gmeta.get(), 32, hipMemcpyDeviceToHost, s));
-            FCC_HIP(hipMemcpyAsync(&nnz_out, d_total.get(), 8, hipMemcpyDeviceToHost, s));
-            FCC_HIP(hipStreamSynchronize(s));
+            pool.read(&gf, flags.get(), 4, s);
+            pool.read(hm, gmeta.get(), 32, s);
+            pool.read(&nnz_out, d_total.get(), 8, s);
+            pool.sync(s);
             if (!gf) break;
             if (attempt >= 3) throw LibError{FCC_ERR_OVERFLOW, "count-pass global set overflow"};
             alloc_ghash((X.gmask + 1) * 16);
@@ -1036,9 +1030,9 @@ int Contraction::run(const fcc_tensor *ta, const int32_t *am, const fcc_tensor *
             k_summarize<<<grid256(nnz_out), 256, 0, s>>>(out->coords, out->vals, nnz_out, stride, g.dC, E, hs, vs);
             unsigned long long hh;
             double dv[2];
-            FCC_HIP(hipMemcpyAsync(&hh, hs.get(), 8, hipMemcpyDeviceToHost, s));
-            FCC_HIP(hipMemcpyAsync(dv, vs.get(), 16, hipMemcpyDeviceToHost, s));
-            FCC_HIP(hipStreamSynchronize(s));
+            pool.read(&hh, hs.get(), 8, s);
+            pool.read(dv, vs.get(), 16, s);
+            pool.sync(s);
             out->coord_hash = hh;
             out->val_sum = dv[0];
             out->val_wsum = dv[1];

@@ -364,8 +364,8 @@ void dense_keys_select(DevPool &pool, const Geom &g, const Index &ix, const uint
     for (int it = 0; it < 40; it++) {
         fill_u64(tot, 2, 0, s);
         k_dk_flag<<<grid_of(nfib), 256, 0, s>>>(ix.fiber_ptr, na, nfib, inv_area, t, flag, tot);
-        FCC_HIP(hipMemcpyAsync(h, tot.get(), 16, hipMemcpyDeviceToHost, s));
-        FCC_HIP(hipStreamSynchronize(s));
+        pool.read(h, tot.get(), 16, s);
+        pool.sync(s);
         if (h[0] <= max_keys) break;
         t *= 1.25;
     }

@@ -536,10 +536,10 @@ void build_right(DevPool &pool, const Geom &g, const Operand &B, Index &ix, std:
         fill_u64(prod, 1, 0, s);
         k_self_join<<<std::min(grid_for(n), 2048u), 256, 0, s>>>(vs, flag, fid, ix.Bcol, ix.fiber_ptr, n, self->rowA,
                                                                   self->fidA, self->bfib, self->vsB, prod);
-        FCC_HIP(hipMemcpyAsync(&self->products, prod.get(), 8, hipMemcpyDeviceToHost, s));
+        pool.read(&self->products, prod.get(), 8, s);
     }
-    FCC_HIP(hipMemcpyAsync(&ix.nfib, total.get(), 4, hipMemcpyDeviceToHost, s));
-    FCC_HIP(hipStreamSynchronize(s));
+    pool.read(&ix.nfib, total.get(), 4, s);
+    pool.sync(s);
 }
 
 // key -> fiber hash map for key spaces too large for a direct table
@@ -639,8 +639,8 @@ void join_left(DevPool &pool, const Geom &g, const Operand &A, const Index &ix, 
                                                                     (unsigned int *)(cnt.get() + 1));
     }
     unsigned long long h[2];
-    FCC_HIP(hipMemcpyAsync(h, cnt.get(), 16, hipMemcpyDeviceToHost, s));
-    FCC_HIP(hipStreamSynchronize(s));
+    pool.read(h, cnt.get(), 16, s);
+    pool.sync(s);
     *products = h[0];
     *nvalid = (uint32_t)(h[1] & 0xFFFFFFFFu);
 }
@@ -658,8 +658,8 @@ void build_hseg(DevPool &pool, Index &ix, std::vector<std::shared_ptr<void>> &ho
     if (ix.ngroups) k_hseg_count<<<grid_for(ix.ngroups), 256, 0, s>>>(ix.gdesc, ix.glight, ix.ngroups, ix.hseg, nseg);
     fill_u32(nseg.get() + ix.ngroups, 1, 0, s);
     scan_u32(pool, nseg, off, (size_t)ix.ngroups + 1, nh, s);
-    FCC_HIP(hipMemcpyAsync(&ix.nhseg, nh.get(), 4, hipMemcpyDeviceToHost, s));
-    FCC_HIP(hipStreamSynchronize(s));
+    pool.read(&ix.nhseg, nh.get(), 4, s);
+    pool.sync(s);
     ix.hdesc = (uint4 *)keep((size_t)ix.nhseg * 16);
     ix.atile_h0 = (uint32_t *)keep(((size_t)ix.natiles + 1) * 4);
     if (ix.ngroups)
@@ -694,8 +694,8 @@ static void left_from_sorted(DevPool &pool, const Operand &A, const uint64_t *ks
     scan_u32(pool, gflag, gid, m, tot.get(), s);
     scan_u32(pool, tflag, aid, m, tot.get() + 1, s);
     uint32_t h[2];
-    FCC_HIP(hipMemcpyAsync(h, tot.get(), 8, hipMemcpyDeviceToHost, s));
-    FCC_HIP(hipStreamSynchronize(s));
+    pool.read(h, tot.get(), 8, s);
+    pool.sync(s);
     ix.ngroups = h[0];
     ix.natiles = h[1];
     ix.group_ptr = (uint32_t *)keep((size_t)(ix.ngroups + 1) * 4);

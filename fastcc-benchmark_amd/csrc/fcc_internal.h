@@ -39,10 +39,29 @@ class DevPool {
     void release_all();
     size_t bytes_cached() const { return cached_; }
 
+    // Small device-to-host reads staged through pinned host memory (a
+    // pageable destination makes every copy a slower synchronous one):
+    // read() queues a copy on stream s, sync(s) waits for s and delivers every
+    // queued read to its destination; discard() drops queued reads.
+    void read(void *dst, const void *src, size_t bytes, hipStream_t s);
+    void sync(hipStream_t s);
+    void discard() {
+        pend_.clear();
+        pin_used_ = 0;
+    }
+
   private:
     std::multimap<size_t, void *> free_;
     std::map<void *, size_t> used_;
     size_t cached_ = 0;
+    static constexpr size_t kPinBytes = 1 << 16;
+    struct Pending {
+        void *dst;
+        size_t off, bytes;
+    };
+    uint8_t *pin_ = nullptr;
+    size_t pin_used_ = 0;
+    std::vector<Pending> pend_;
 };
 
 // RAII temporary from the pool.

@@ -6,11 +6,39 @@
 // sorting the linearised composite keys instead of hashing them.
 #include "fcc_internal.h"
 
+#include <cstring>
+
 namespace fcc {
 
 // ------------------------------------------------------------ caching pool
 
-DevPool::~DevPool() { release_all(); }
+DevPool::~DevPool() {
+    release_all();
+    if (pin_) (void)hipHostFree(pin_);
+}
+
+void DevPool::read(void *dst, const void *src, size_t bytes, hipStream_t s) {
+    if (!pin_ && hipHostMalloc((void **)&pin_, kPinBytes, hipHostMallocDefault) != hipSuccess) {
+        (void)hipGetLastError();
+        pin_ = nullptr;
+    }
+    const size_t off = (pin_used_ + 15) & ~size_t(15);
+    if (!pin_ || off + bytes > kPinBytes) {  // no room: a direct copy
+        FCC_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, s));
+        return;
+    }
+    FCC_HIP(hipMemcpyAsync(pin_ + off, src, bytes, hipMemcpyDeviceToHost, s));
+    pend_.push_back(Pending{dst, off, bytes});
+    pin_used_ = off + bytes;
+}
+
+void DevPool::sync(hipStream_t s) {
+    const hipError_t e = hipStreamSynchronize(s);
+    if (e == hipSuccess)
+        for (const Pending &p : pend_) memcpy(p.dst, pin_ + p.off, p.bytes);
+    discard();
+    if (e != hipSuccess) throw HipError{e, "hipStreamSynchronize (read-back)"};
+}
 
 void *DevPool::get(size_t bytes) {
     if (bytes == 0) bytes = 16;
