@@ -1268,9 +1268,12 @@ __global__ __launch_bounds__(ACC_THREADS, 4) void k_numeric(AccArgs A) {
 
 static_assert(DENSE_SLOTS <= 4 * SYM_SLOTS, "the count pass's dense byte map must fit its key set");
 
-// Count pass: structural support only.
+// Count pass: structural support only. The key set of hash units (or the
+// byte map of dense units) is dynamic LDS of A.set_words words: a launch with
+// no LDS-hash units sizes it for the dense byte map alone (DENSE_SLOTS bytes),
+// so twice as many workgroups share a CU.
 __global__ __launch_bounds__(ACC_THREADS) void k_count(AccArgs A) {
-    __shared__ __align__(16) uint32_t s_set[SYM_SLOTS];
+    extern __shared__ __align__(16) uint32_t s_set[];
     __shared__ uint32_t s_stage[2 * ACC_THREADS];
     __shared__ uint32_t s_cnt, s_gnext, s_unit, s_w[NWAVES];
     Acc<false, false> S;
@@ -1293,6 +1296,9 @@ __global__ __launch_bounds__(ACC_THREADS) void k_count(AccArgs A) {
         U.c0 = A.unit_c0[U.u];
         U.W = A.unit_W[U.u];
         U.kind = A.unit_kind[U.u];
+        // an LDS-hash unit in a launch without room for its key set counts
+        // through the global set (the same count)
+        if (U.kind == KIND_LHASH && A.set_words < SYM_SLOTS) U.kind = KIND_GHASH;
         U.multi = A.unit_multi[U.u];
         U.Wp = U.kind == KIND_DENSE ? ((U.W + 31) & ~31u) : U.W;
         U.row0 = (uint64_t)U.tile << A.logTL;
@@ -1594,7 +1600,9 @@ void launch_refine_kinds(const unsigned long long *unit_count, uint32_t n, int f
 
 void launch_count(const AccArgs &A, unsigned grid, hipStream_t s) {
     if (!A.nunits) return;
-    k_count<<<grid, ACC_THREADS, 0, s>>>(A);
+    if (A.set_words != SYM_SLOTS && A.set_words != DENSE_SLOTS / 4)
+        throw LibError{ERR_INVALID, "count pass: unsupported key-set size"};
+    k_count<<<grid, ACC_THREADS, (size_t)A.set_words * 4, s>>>(A);
 }
 
 void launch_numeric(const AccArgs &A, unsigned grid, bool bitmap, bool sink, int mode, hipStream_t s) {

@@ -762,11 +762,18 @@ int Contraction::run(const fcc_tensor *ta, const int32_t *am, const fcc_tensor *
     // units from the exact bounds (pred == null) or from predictions; the
     // unit count comes back with the predicted output total (psum, when
     // given) in one synchronisation
+    // tiles per accumulator kind: with no LDS-hash tile the count pass needs
+    // LDS only for the dense byte map
+    Tmp<unsigned long long> tk_hist(pool, 8);
+    unsigned long long tkh[8] = {};
     auto make_units = [&](const float *pred, const unsigned long long *psum, uint64_t *psum_h) {
         P.dense_ratio = pred ? 0.15f : 0.5f;
         launch_plan_tiles(ix, pred, P, t_nwin, t_W, t_kind, s);
+        fill_u64(tk_hist, 8, 0, s);
+        k_kind_hist<<<std::min(grid256(T), 1024u), 256, 0, s>>>(t_kind, T, tk_hist);
         scan_u32(pool, t_nwin, t_ubase, T, d_nunits, s);
         pool.read(&U.n, d_nunits.get(), 4, s);
+        pool.read(tkh, tk_hist.get(), sizeof(tkh), s);
         if (psum) pool.read(psum_h, psum, 8, s);
         pool.sync(s);
         u_tile = Tmp<uint32_t>(pool, U.n);
@@ -926,10 +933,17 @@ int Contraction::run(const fcc_tensor *ta, const int32_t *am, const fcc_tensor *
         }
         // ---- exact symbolic planning: count, scan, then fill
         alloc_ghash(1u << 16);
+        // the key set's LDS (see make_units) and, when it is only the dense
+        // byte map, a default grid of four workgroups per CU
+        const bool small_set = tkh[KIND_LHASH] == 0;
+        X.set_words = small_set ? DENSE_SLOTS / 4 : SYM_SLOTS;
+        unsigned cgrid = grid;
+        if (small_set && o.workgroups <= 0 && tuned_wg <= 0)
+            cgrid = std::max(1u, std::min<unsigned>((unsigned)ctx->num_cus * 4, U.n));
         for (int attempt = 0;; attempt++) {
             reset_ghash(false);
             fill_u64(unit_count, U.n, 0, s);
-            launch_count(X, grid, s);
+            launch_count(X, cgrid, s);
             launch_refine_kinds(unit_count, U.n, o.force_acc, U.kind, gmeta.get() + 2, s);
             scan_u64(pool, (const uint64_t *)unit_count.get(), (uint64_t *)unit_off.get(), U.n,
                      (uint64_t *)d_total.get(), s);

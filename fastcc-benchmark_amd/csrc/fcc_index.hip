@@ -71,7 +71,8 @@ __global__ __launch_bounds__(256) void k_extents(const uint32_t *coords, const d
     __shared__ uint32_t s_mx[4], s_fl[4];
     const int m = blockIdx.y;
     uint32_t mx = 0, flag = 0;
-    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < nnz; i += (size_t)gridDim.x * blockDim.x) {
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    auto take = [&](size_t i) {
         if (m < order) {
             uint32_t c = coords[(size_t)m * nnz + i];
             mx = c > mx ? c : mx;
@@ -80,7 +81,14 @@ __global__ __launch_bounds__(256) void k_extents(const uint32_t *coords, const d
             double a = fabs(v);
             if (!(a >= 0x1.0p-511) && !isnan(v)) flag = 1;
         }
+    };
+    size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+    // four independent loads in flight per thread
+    for (; i + 3 * stride < nnz; i += 4 * stride) {
+#pragma unroll
+        for (int u = 0; u < 4; u++) take(i + u * stride);
     }
+    for (; i < nnz; i += stride) take(i);
     for (int d = 32; d > 0; d >>= 1) {
         uint32_t o = __shfl_xor(mx, d, 64);
         mx = o > mx ? o : mx;
@@ -103,7 +111,7 @@ __global__ __launch_bounds__(256) void k_extents(const uint32_t *coords, const d
 
 void launch_extents(const Operand &t, uint32_t *dev_out, hipStream_t s) {
     if (t.nnz == 0) return;
-    unsigned gx = (unsigned)std::min<size_t>(div_up((size_t)t.nnz, 256), 512);
+    unsigned gx = (unsigned)std::min<size_t>(div_up((size_t)t.nnz, 256 * 4), 2048);
     dim3 grid(gx, t.order + 1);
     k_extents<<<grid, 256, 0, s>>>(t.coords, t.vals, t.order, (size_t)t.nnz, dev_out);
 }
@@ -712,6 +720,11 @@ static void left_from_sorted(DevPool &pool, const Operand &A, const uint64_t *ks
                                              ix.group_atile, glight, ix.atile_id, ix.atile_g0);
         k_tile_light_init<<<grid_for(ix.natiles), 256, 0, s>>>(ix.atile_g0, ix.natiles, ix.atile_lg);
         k_tile_light<<<grid_for(ix.ngroups), 256, 0, s>>>(glight, ix.group_atile, ix.atile_g0, ix.ngroups, ix.atile_lg);
+    } else {
+        // no valid left entry: no groups, no tiles; the one-entry offset
+        // arrays still hold their end (read by the segment table below)
+        fill_u32(ix.atile_g0, 1, 0, s);
+        fill_u32(ix.group_ptr, 1, 0, s);
     }
     // descriptors + statistics (one pass over the groups)
     size_t T = ix.natiles;

@@ -278,6 +278,7 @@ struct AccArgs {
     int row_own;               // dense units: row-owned plain updates (experimental)
     uint32_t claim_div;        // heavy claims: segments / claim_div per claim (0: 64 per claim)
     uint32_t seg_below;        // tiles with fewer heavy groups than this work on segments
+    uint32_t set_words;        // count pass: LDS words of its key set / byte map (launch size)
     // dense-key block over left rows x right columns: presence bits dk_P
     // (row pitch dk_pw words) and values dk_C (row pitch dk_rb); units of the
     // active tiles flagged in dk_tile start their accumulator from it.
