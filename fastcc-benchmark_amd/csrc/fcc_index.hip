@@ -231,7 +231,7 @@ __global__ void k_comp_A(const uint32_t *rowA, const uint32_t *fidA, const uint3
 
 __global__ void k_gather_A(const uint64_t *comp, const uint32_t *idx, const double *vals, const uint32_t *fiber_ptr,
                            size_t n, int logTL, int fbits, uint32_t *Ali, uint32_t *Arow, double *Aval, uint32_t *Afib,
-                           uint2 *Afp, uint32_t *gflag, uint32_t *tflag) {
+                           uint2 *Afp, uint64_t *gtflag) {
     size_t p = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
     if (p >= n) return;
     uint64_t c = comp[p];
@@ -242,32 +242,37 @@ __global__ void k_gather_A(const uint64_t *comp, const uint32_t *idx, const doub
     Afp[p] = make_uint2(fiber_ptr[f], fiber_ptr[f + 1]);
     Aval[p] = vals[idx[p]];
     uint64_t prev = p ? comp[p - 1] : ~0ull;
-    gflag[p] = (p == 0 || (c >> logTL) != (prev >> logTL)) ? 1u : 0u;
-    tflag[p] = (p == 0 || (c >> (1 + fbits + logTL)) != (prev >> (1 + fbits + logTL))) ? 1u : 0u;
+    // group start (low word) and tile start (high word): one scan of the
+    // packed pair numbers both (each half stays below 2^32)
+    const uint64_t gf = (p == 0 || (c >> logTL) != (prev >> logTL)) ? 1u : 0u;
+    const uint64_t tf = (p == 0 || (c >> (1 + fbits + logTL)) != (prev >> (1 + fbits + logTL))) ? 1u : 0u;
+    gtflag[p] = (tf << 32) | gf;
 }
 
-__global__ void k_groups(const uint64_t *comp, const uint32_t *gflag, const uint32_t *gid, const uint32_t *tflag,
-                         const uint32_t *aid, size_t n, int logTL, int fbits, uint32_t *group_ptr,
-                         uint32_t *group_fiber, uint32_t *group_atile, uint8_t *group_light, uint32_t *atile_id,
-                         uint32_t *atile_g0) {
+__global__ void k_groups(const uint64_t *comp, const uint64_t *gtflag, const uint64_t *gtid, size_t n, int logTL,
+                         int fbits, uint32_t *group_ptr, uint32_t *group_fiber, uint32_t *group_atile,
+                         uint8_t *group_light, uint32_t *atile_id, uint32_t *atile_g0) {
     size_t p = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
     if (p >= n) return;
     uint64_t c = comp[p];
-    uint32_t a = aid[p] + tflag[p] - 1;
-    if (gflag[p]) {
-        uint32_t g = gid[p];
+    const uint64_t fl = gtflag[p], id = gtid[p];
+    const uint32_t gflag = (uint32_t)fl, tflag = (uint32_t)(fl >> 32);
+    const uint32_t gid = (uint32_t)id, aid = (uint32_t)(id >> 32);
+    uint32_t a = aid + tflag - 1;
+    if (gflag) {
+        uint32_t g = gid;
         group_ptr[g] = (uint32_t)p;
         group_fiber[g] = (uint32_t)((c >> logTL) & ((1ull << fbits) - 1));
         group_atile[g] = a;
         group_light[g] = (uint8_t)((c >> (fbits + logTL)) & 1);
     }
-    if (tflag[p]) {
+    if (tflag) {
         atile_id[a] = (uint32_t)(c >> (1 + fbits + logTL));
-        atile_g0[a] = gid[p];
+        atile_g0[a] = gid;
     }
     if (p == n - 1) {
-        group_ptr[gid[p] + gflag[p]] = (uint32_t)n;
-        atile_g0[a + 1] = gid[p] + gflag[p];
+        group_ptr[gid + gflag] = (uint32_t)n;
+        atile_g0[a + 1] = gid + gflag;
     }
 }
 
@@ -689,23 +694,22 @@ static void left_from_sorted(DevPool &pool, const Operand &A, const uint64_t *ks
     ix.Arow = (uint32_t *)keep(m * 4);
     ix.Aval = (double *)keep(m * 8);
     ix.Afib = (uint32_t *)keep(m * 4);
-    Tmp<uint32_t> gflag(pool, m), tflag(pool, m), gid(pool, m), aid(pool, m), tot(pool, 2);
+    Tmp<uint64_t> gtflag(pool, m), gtid(pool, m), tot(pool, 1);
     ix.Afp = (uint2 *)keep(m * 8);
     if (m)
         k_gather_A<<<grid_for(m), 256, 0, s>>>(ks, vs, A.vals, ix.fiber_ptr, m, logTL, fbits, ix.Ali, ix.Arow, ix.Aval,
                                                ix.Afib,
-                                               ix.Afp, gflag, tflag);
+                                               ix.Afp, gtflag);
     if (ix.keep_perm) {
         ix.permA = (uint32_t *)keep(m * 4);
         if (m) FCC_HIP(hipMemcpyAsync(ix.permA, vs, m * 4, hipMemcpyDeviceToDevice, s));
     }
-    scan_u32(pool, gflag, gid, m, tot.get(), s);
-    scan_u32(pool, tflag, aid, m, tot.get() + 1, s);
-    uint32_t h[2];
-    pool.read(h, tot.get(), 8, s);
+    scan_u64(pool, gtflag, gtid, m, tot.get(), s);
+    uint64_t h = 0;
+    pool.read(&h, tot.get(), 8, s);
     pool.sync(s);
-    ix.ngroups = h[0];
-    ix.natiles = h[1];
+    ix.ngroups = (uint32_t)h;
+    ix.natiles = (uint32_t)(h >> 32);
     ix.group_ptr = (uint32_t *)keep((size_t)(ix.ngroups + 1) * 4);
     ix.group_fiber = (uint32_t *)keep((size_t)ix.ngroups * 4);
     ix.group_atile = (uint32_t *)keep((size_t)ix.ngroups * 4);
@@ -716,7 +720,7 @@ static void left_from_sorted(DevPool &pool, const Operand &A, const uint64_t *ks
     ix.atile_g0 = (uint32_t *)keep((size_t)(ix.natiles + 1) * 4);
     ix.atile_lg = (uint32_t *)keep((size_t)ix.natiles * 4);
     if (m) {
-        k_groups<<<grid_for(m), 256, 0, s>>>(ks, gflag, gid, tflag, aid, m, logTL, fbits, ix.group_ptr, ix.group_fiber,
+        k_groups<<<grid_for(m), 256, 0, s>>>(ks, gtflag, gtid, m, logTL, fbits, ix.group_ptr, ix.group_fiber,
                                              ix.group_atile, glight, ix.atile_id, ix.atile_g0);
         k_tile_light_init<<<grid_for(ix.natiles), 256, 0, s>>>(ix.atile_g0, ix.natiles, ix.atile_lg);
         k_tile_light<<<grid_for(ix.ngroups), 256, 0, s>>>(glight, ix.group_atile, ix.atile_g0, ix.ngroups, ix.atile_lg);
