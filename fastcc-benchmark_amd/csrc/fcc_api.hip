@@ -172,13 +172,15 @@ void choose_tiles(fcc_ctx *ctx, const fcc_options &o, const Geom &g, uint64_t pr
         TL = (uint64_t)te->row_tile;
     } else if (dense_keys) {
         // the dense keys' products are gone: what stays is many short
-        // fibers, and every column window rescans the tile's entries: as many
-        // rows as fit with whole rows in one window, else two-row tiles with
-        // the widest windows
+        // fibers, and every column window rescans the tile's entries: whole
+        // rows in one window, as many rows as fit while the tiles still number
+        // at least twice the accumulate kernels' workgroup slots (every unit
+        // is a dense block-seeded tile: fewer, larger ones leave the last
+        // workgroups waiting); else two-row tiles with the widest windows
         const uint64_t sp = (g.spanB + 31) & ~31ull;
-        TL = 1;
-        while (TL * 2 * sp <= DENSE_SLOTS && TL < 16) TL *= 2;
-        if (TL < 2) TL = 2;
+        const uint64_t min_tiles = 4ull * (uint64_t)std::max(1, ctx->num_cus);
+        TL = 2;
+        while (TL * 2 * sp <= DENSE_SLOTS && TL < 16 && g.spanA / (TL * 2) >= min_tiles) TL *= 2;
     } else if (dens >= 0.25) {
         const uint64_t sp = (g.spanB + 31) & ~31ull;  // dense rows are padded to 32 columns
         uint64_t tf = 1;
