@@ -111,7 +111,9 @@ __global__ __launch_bounds__(256) void k_extents(const uint32_t *coords, const d
 
 void launch_extents(const Operand &t, uint32_t *dev_out, hipStream_t s) {
     if (t.nnz == 0) return;
-    unsigned gx = (unsigned)std::min<size_t>(div_up((size_t)t.nnz, 256 * 4), 2048);
+    // at most 512 workgroups per mode: each ends in one atomic on the mode's
+    // maximum, and same-address atomics serialise in L2
+    unsigned gx = (unsigned)std::min<size_t>(div_up((size_t)t.nnz, 256 * 4), 512);
     dim3 grid(gx, t.order + 1);
     k_extents<<<grid, 256, 0, s>>>(t.coords, t.vals, t.order, (size_t)t.nnz, dev_out);
 }
