@@ -1318,13 +1318,30 @@ __global__ __launch_bounds__(ACC_THREADS) void k_count(AccArgs A) {
             const bool seeded = A.dk_tile && A.dk_tile[U.a];
             if (seeded) {
                 // presence bits of the block, one nibble per byte word
+                uint32_t have = 0;
                 for (uint32_t w = threadIdx.x; w < nwords / 8; w += ACC_THREADS) {
                     const uint32_t b = dk_dense_word(A, U, w);
+                    have += (uint32_t)__popc(b);
 #pragma unroll
                     for (int j = 0; j < 8; j++) {
                         const uint32_t n = (b >> (4 * j)) & 0xFu;
                         s_set[8 * w + j] = (n & 1u) | ((n & 2u) << 7) | ((n & 4u) << 14) | ((n & 8u) << 21);
                     }
+                }
+                // A unit whose every cell the block already holds has nothing
+                // left to count: its sparse products can only land on present
+                // cells. Its count is the cell count, without the traversal.
+                for (int d = 32; d > 0; d >>= 1) have += __shfl_xor(have, d, 64);
+                if ((threadIdx.x & 63) == 0) s_w[threadIdx.x >> 6] = have;
+                __syncthreads();
+                uint32_t tot = 0;
+#pragma unroll
+                for (int w = 0; w < NWAVES; w++) tot += s_w[w];
+                __syncthreads();  // s_w is written again below
+                const uint64_t rows_in = min((uint64_t)TL, A.spanA - U.row0);
+                if ((uint64_t)tot == rows_in * U.W) {
+                    if (threadIdx.x == 0) atomicAdd(&A.unit_count[U.u], (unsigned long long)tot);
+                    continue;  // every thread has read s_unit (barriers above)
                 }
             } else {
                 for (uint32_t s = threadIdx.x; s < nwords; s += ACC_THREADS) s_set[s] = 0u;
