@@ -589,6 +589,7 @@ int Contraction::run(const fcc_tensor *ta, const int32_t *am, const fcc_tensor *
     if (const char *hm = getenv("FCC_HEAVY_MIN")) ix.heavy_min = (uint32_t)std::max(1, atoi(hm));
     if (const char *hs = getenv("FCC_HSEG")) ix.hseg = (uint32_t)std::max(64, atoi(hs));
     ix.defer_hseg = dk.nh > 0;  // built once, after the dense keys are masked
+    ix.features = o.planner == FCC_PLAN_LEARNED || o.collect_stats;
     // A large GEMM (C4-sized: hundreds of GFLOP) outlasts the whole left-tile
     // build, so it starts before the left sort; a small one waits for the
     // sort to be queued (they compete for the LDS).

@@ -251,9 +251,12 @@ __global__ void k_gather_A(const uint64_t *comp, const uint32_t *idx, const doub
     gtflag[p] = (tf << 32) | gf;
 }
 
+// Also each tile's first light group (atile_lg): light groups follow the heavy
+// ones inside a tile, so it is the group where the class turns light, or the
+// tile's group end when its last group is heavy.
 __global__ void k_groups(const uint64_t *comp, const uint64_t *gtflag, const uint64_t *gtid, size_t n, int logTL,
                          int fbits, uint32_t *group_ptr, uint32_t *group_fiber, uint32_t *group_atile,
-                         uint8_t *group_light, uint32_t *atile_id, uint32_t *atile_g0) {
+                         uint8_t *group_light, uint32_t *atile_id, uint32_t *atile_g0, uint32_t *atile_lg) {
     size_t p = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
     if (p >= n) return;
     uint64_t c = comp[p];
@@ -272,6 +275,10 @@ __global__ void k_groups(const uint64_t *comp, const uint64_t *gtflag, const uin
         atile_id[a] = (uint32_t)(c >> (1 + fbits + logTL));
         atile_g0[a] = gid;
     }
+    const bool light = (c >> (fbits + logTL)) & 1;
+    if (gflag && light && (tflag || !((comp[p - 1] >> (fbits + logTL)) & 1))) atile_lg[a] = gid;
+    const bool tile_end = p == n - 1 || (gtflag[p + 1] >> 32) != 0;
+    if (tile_end && !light) atile_lg[a] = gid + gflag;
     if (p == n - 1) {
         group_ptr[gid + gflag] = (uint32_t)n;
         atile_g0[a + 1] = gid + gflag;
@@ -326,19 +333,6 @@ __global__ void k_tile_direct(const uint32_t *atile_g0, const uint32_t *group_pt
         }
     }
     direct[a] = ok;
-}
-
-__global__ void k_tile_light_init(const uint32_t *atile_g0, uint32_t natiles, uint32_t *atile_lg) {
-    uint32_t a = blockIdx.x * blockDim.x + threadIdx.x;
-    if (a < natiles) atile_lg[a] = atile_g0[a + 1];
-}
-
-__global__ void k_tile_light(const uint8_t *group_light, const uint32_t *group_atile, const uint32_t *atile_g0,
-                             uint32_t ngroups, uint32_t *atile_lg) {
-    uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
-    if (g >= ngroups || !group_light[g]) return;
-    uint32_t a = group_atile[g];
-    if (g == atile_g0[a] || !group_light[g - 1]) atile_lg[a] = g;
 }
 
 __device__ inline uint32_t hash32(uint32_t x) {
@@ -398,9 +392,11 @@ __global__ __launch_bounds__(256) void k_tile_stats(const uint32_t *group_ptr, c
         const uint32_t fp0 = fiber_ptr[f];
         nf = fiber_ptr[f + 1] - fp0;
         gdesc[g] = make_uint4(fp0, (uint32_t)nf, e0, (uint32_t)m);
-        const uint2 hh = fsk[f];
-        h1 = hh.x;
-        h2 = hh.y;
+        if (fsk) {
+            const uint2 hh = fsk[f];
+            h1 = hh.x;
+            h2 = hh.y;
+        }
     }
     if (threadIdx.x == 0) s_a0 = a;
     __syncthreads();
@@ -442,11 +438,13 @@ __global__ __launch_bounds__(256) void k_tile_stats(const uint32_t *group_ptr, c
             }
             atomicMax(&st_maxgp[a0], (unsigned long long)mx);
             atomicAdd(&st_nnz[a0], (uint32_t)sm);
-            atomicAdd(&st_keys[a0], (uint32_t)sk);
-            atomicAdd(&st_sumfib[a0], (unsigned long long)snf);
             atomicAdd(&st_products[a0], (unsigned long long)sp);
-            atomicMin(&st_sketch[2 * a0], h1);
-            atomicMin(&st_sketch[2 * a0 + 1], h2);
+            if (fsk) {  // planner features
+                atomicAdd(&st_keys[a0], (uint32_t)sk);
+                atomicAdd(&st_sumfib[a0], (unsigned long long)snf);
+                atomicMin(&st_sketch[2 * a0], h1);
+                atomicMin(&st_sketch[2 * a0 + 1], h2);
+            }
         }
         return;
     }
@@ -475,12 +473,14 @@ __global__ __launch_bounds__(256) void k_tile_stats(const uint32_t *group_ptr, c
     const uint32_t pa = __shfl_up(a, 1, 64);
     if (ok && (lane == 0 || pa != a)) {
         atomicAdd(&st_nnz[a], sm);
-        atomicAdd(&st_keys[a], sk);
-        atomicAdd(&st_sumfib[a], (unsigned long long)snf);
         atomicAdd(&st_products[a], (unsigned long long)sp);
-        atomicMin(&st_sketch[2 * a], h1);
-        atomicMin(&st_sketch[2 * a + 1], h2);
         atomicMax(&st_maxgp[a], (unsigned long long)mx);
+        if (fsk) {  // planner features
+            atomicAdd(&st_keys[a], sk);
+            atomicAdd(&st_sumfib[a], (unsigned long long)snf);
+            atomicMin(&st_sketch[2 * a], h1);
+            atomicMin(&st_sketch[2 * a + 1], h2);
+        }
     }
 }
 
@@ -723,9 +723,7 @@ static void left_from_sorted(DevPool &pool, const Operand &A, const uint64_t *ks
     ix.atile_lg = (uint32_t *)keep((size_t)ix.natiles * 4);
     if (m) {
         k_groups<<<grid_for(m), 256, 0, s>>>(ks, gtflag, gtid, m, logTL, fbits, ix.group_ptr, ix.group_fiber,
-                                             ix.group_atile, glight, ix.atile_id, ix.atile_g0);
-        k_tile_light_init<<<grid_for(ix.natiles), 256, 0, s>>>(ix.atile_g0, ix.natiles, ix.atile_lg);
-        k_tile_light<<<grid_for(ix.ngroups), 256, 0, s>>>(glight, ix.group_atile, ix.atile_g0, ix.ngroups, ix.atile_lg);
+                                             ix.group_atile, glight, ix.atile_id, ix.atile_g0, ix.atile_lg);
     } else {
         // no valid left entry: no groups, no tiles; the one-entry offset
         // arrays still hold their end (read by the segment table below)
@@ -749,10 +747,13 @@ static void left_from_sorted(DevPool &pool, const Operand &A, const uint64_t *ks
         f.add(ix.st_sketch, 2 * T, 0xFFFFFFFFull, 4);
         f.add(ix.st_maxgp, T, 0, 8);
         fill_many(f, s);
-        Tmp<uint2> fsk(pool, ix.nfib);
-        if (ix.nfib) k_fiber_sketch<<<grid_for(ix.nfib), 256, 0, s>>>(ix.fiber_ptr, ix.Bcol, ix.nfib, fsk);
+        // the planner features (key counts, fiber sums, column sketches) only
+        // when a learned planner or statistics collection reads them
+        Tmp<uint2> fsk(pool, ix.features ? ix.nfib : 0);
+        if (ix.features && ix.nfib)
+            k_fiber_sketch<<<grid_for(ix.nfib), 256, 0, s>>>(ix.fiber_ptr, ix.Bcol, ix.nfib, fsk);
         k_tile_stats<<<grid_for(ix.ngroups), 256, 0, s>>>(ix.group_ptr, ix.group_fiber, ix.group_atile, ix.fiber_ptr,
-                                                          fsk, ix.ngroups, ix.gdesc, ix.st_nnz, ix.st_keys,
+                                                          ix.features ? fsk.get() : nullptr, ix.ngroups, ix.gdesc, ix.st_nnz, ix.st_keys,
                                                           (unsigned long long *)ix.st_sumfib,
                                                           (unsigned long long *)ix.st_products, ix.st_sketch,
                                                           (unsigned long long *)ix.st_maxgp);

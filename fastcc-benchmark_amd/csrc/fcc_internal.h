@@ -194,6 +194,7 @@ struct Index {
     uint32_t TL = 1;
     uint64_t products = 0;
     uint32_t heavy_min = HEAVY_MIN;  // fiber length from which groups are processed wave-per-group
+    bool features = true;            // per-tile planner features (keys, fiber sums, sketches) wanted
     // kept for plans (new values on the same coordinates): the input entry
     // behind each sorted position, so Bval/Aval can be gathered again
     bool keep_perm = false;
