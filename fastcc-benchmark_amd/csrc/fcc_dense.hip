@@ -32,20 +32,15 @@ __global__ void k_dk_na(const uint32_t *fidA, size_t n, const uint32_t *fiber_pt
     if (f != 0xFFFFFFFFu && fiber_ptr[f + 1] - fiber_ptr[f] >= nb_min) atomicAdd(&na[f], 1u);
 }
 
-// self-contraction: the left fibers are the right ones
-__global__ void k_dk_na_self(const uint32_t *fiber_ptr, uint32_t nfib, uint32_t *na) {
-    const uint32_t f = blockIdx.x * blockDim.x + threadIdx.x;
-    if (f < nfib) na[f] = fiber_ptr[f + 1] - fiber_ptr[f];
-}
-
 // a fiber is selected when nA * nB >= tau * spanA * spanB; tot[0] counts the
-// selected fibers, tot[1] sums their products (wave-aggregated)
+// selected fibers, tot[1] sums their products (wave-aggregated). na == null:
+// a self-contraction, whose left fibers are the right ones (nA = nB).
 __global__ void k_dk_flag(const uint32_t *fiber_ptr, const uint32_t *na, uint32_t nfib, double inv_area, double tau,
                           uint32_t *flag, unsigned long long *tot) {
     const uint32_t f = blockIdx.x * blockDim.x + threadIdx.x;
     unsigned long long c = 0, prod = 0;
     if (f < nfib) {
-        const uint64_t a = na[f], b = fiber_ptr[f + 1] - fiber_ptr[f];
+        const uint64_t b = fiber_ptr[f + 1] - fiber_ptr[f], a = na ? na[f] : b;
         const bool sel = a && (double)a * (double)b * inv_area >= tau;
         flag[f] = sel ? 1u : 0u;
         c = sel ? 1ull : 0ull;
@@ -61,12 +56,15 @@ __global__ void k_dk_flag(const uint32_t *fiber_ptr, const uint32_t *na, uint32_
     }
 }
 
-__global__ void k_dk_keys(const uint32_t *flag, const uint32_t *idx, uint32_t nfib, uint32_t *hkey, uint32_t *hfib) {
+// hkey[f] = the selected key index of fiber f (or none), hfib[h] = its fiber;
+// hfib holds `cap` entries (a selection over the cap is discarded by the host)
+__global__ void k_dk_keys(const uint32_t *flag, const uint32_t *idx, uint32_t nfib, uint32_t cap, uint32_t *hkey,
+                          uint32_t *hfib) {
     const uint32_t f = blockIdx.x * blockDim.x + threadIdx.x;
     if (f >= nfib) return;
     const bool sel = flag[f] != 0;
     hkey[f] = sel ? idx[f] : 0xFFFFFFFFu;
-    if (sel) hfib[idx[f]] = f;
+    if (sel && idx[f] < cap) hfib[idx[f]] = f;
 }
 
 // Takes the selected keys' groups out of the sparse traversal: a heavy group
@@ -344,7 +342,7 @@ void dense_keys_select(DevPool &pool, const Geom &g, const Index &ix, const uint
         return p;
     };
     const uint32_t nfib = ix.nfib;
-    Tmp<uint32_t> na(pool, nfib), flag(pool, nfib), idx(pool, nfib), dtot(pool, 1);
+    Tmp<uint32_t> na(pool, fidA ? nfib : 0), flag(pool, nfib), idx(pool, nfib), dtot(pool, 1);
     Tmp<unsigned long long> tot(pool, 2);
     if (fidA) {
         // nA <= spanA, so a selected key has nB >= tau * spanB
@@ -353,17 +351,21 @@ void dense_keys_select(DevPool &pool, const Geom &g, const Index &ix, const uint
         if (nA)
             k_dk_na<<<grid_of(nA), 256, 0, s>>>(fidA, nA, ix.fiber_ptr,
                                                  (uint32_t)std::min(nbm, 4294967295.0), na);
-    } else {
-        k_dk_na_self<<<grid_of(nfib), 256, 0, s>>>(ix.fiber_ptr, nfib, na);
     }
     const double inv_area = 1.0 / ((double)g.spanA * (double)g.spanB);
     // at most this many keys, and their panels within 2 GiB
     const uint64_t max_keys = std::min<uint64_t>(16384, (2ull << 30) / (8 * (ra + rb)));
+    // the key numbering is queued with the selection, before the host waits
+    // for the count: a selection within max_keys needs no second pass
+    uint32_t *hkey = (uint32_t *)keep((size_t)nfib * 4);
+    uint32_t *hfib = (uint32_t *)keep((size_t)max_keys * 4);
     double t = tau;
     unsigned long long h[2] = {0, 0};
     for (int it = 0; it < 40; it++) {
         fill_u64(tot, 2, 0, s);
-        k_dk_flag<<<grid_of(nfib), 256, 0, s>>>(ix.fiber_ptr, na, nfib, inv_area, t, flag, tot);
+        k_dk_flag<<<grid_of(nfib), 256, 0, s>>>(ix.fiber_ptr, fidA ? na.get() : nullptr, nfib, inv_area, t, flag, tot);
+        scan_u32(pool, flag, idx, nfib, dtot, s);
+        k_dk_keys<<<grid_of(nfib), 256, 0, s>>>(flag, idx, nfib, (uint32_t)max_keys, hkey, hfib);
         pool.read(h, tot.get(), 16, s);
         pool.sync(s);
         if (h[0] <= max_keys) break;
@@ -381,10 +383,8 @@ void dense_keys_select(DevPool &pool, const Geom &g, const Index &ix, const uint
     dk.ra = ra;
     dk.rb = rb;
     dk.self = fidA == nullptr;
-    scan_u32(pool, flag, idx, nfib, dtot, s);
-    dk.hkey = (uint32_t *)keep((size_t)nfib * 4);
-    dk.hfib = (uint32_t *)keep((size_t)dk.nh * 4);
-    k_dk_keys<<<grid_of(nfib), 256, 0, s>>>(flag, idx, nfib, dk.hkey, dk.hfib);
+    dk.hkey = hkey;
+    dk.hfib = hfib;
 }
 
 void dense_keys_alloc(DevPool &pool, const Geom &g, DenseKeys &dk, int num_cus,
