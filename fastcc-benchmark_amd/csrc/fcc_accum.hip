@@ -1340,7 +1340,10 @@ __global__ __launch_bounds__(ACC_THREADS) void k_count(AccArgs A) {
                 __syncthreads();  // s_w is written again below
                 const uint64_t rows_in = min((uint64_t)TL, A.spanA - U.row0);
                 if ((uint64_t)tot == rows_in * U.W) {
-                    if (threadIdx.x == 0) atomicAdd(&A.unit_count[U.u], (unsigned long long)tot);
+                    if (threadIdx.x == 0) {
+                        atomicAdd(&A.unit_count[U.u], (unsigned long long)tot);
+                        atomicAdd(A.block_counted, 1ull);
+                    }
                     continue;  // every thread has read s_unit (barriers above)
                 }
             } else {

@@ -730,7 +730,7 @@ int Contraction::run(const fcc_tensor *ta, const int32_t *am, const fcc_tensor *
         X.xbE[m] = g.xbE[m];
     }
     auto &work = k.work;
-    auto &gmeta = k.gmeta;  // [0]=gcount [1]=cursor [2]=gtotal
+    auto &gmeta = k.gmeta;  // [0]=gcount [1]=cursor [2]=gtotal [3]=units counted from the dense block
     auto &flags = k.flags;  // [0]=gflag [1]=fail
     work = Tmp<unsigned int>(pool, 4);
     gmeta = Tmp<unsigned long long>(pool, 4);
@@ -738,6 +738,7 @@ int Contraction::run(const fcc_tensor *ta, const int32_t *am, const fcc_tensor *
     X.work = work;
     X.gcount = gmeta.get();
     X.cursor = gmeta.get() + 1;
+    X.block_counted = gmeta.get() + 3;
     X.gflag = flags.get();
     X.fail_flag = flags.get() + 1;
     X.sink = sinkbuf.get();
@@ -960,6 +961,7 @@ int Contraction::run(const fcc_tensor *ta, const int32_t *am, const fcc_tensor *
             alloc_ghash((X.gmask + 1) * 16);
         }
         uint64_t gtotal = hm[2];
+        st.units_block_counted = hm[3];
         join_block();
         FCC_HIP(hipEventRecord(ev[3], s));
         if (o.collect_stats) {

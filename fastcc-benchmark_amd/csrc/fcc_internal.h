@@ -280,6 +280,7 @@ struct AccArgs {
     uint32_t claim_div;        // heavy claims: segments / claim_div per claim (0: 64 per claim)
     uint32_t seg_below;        // tiles with fewer heavy groups than this work on segments
     uint32_t set_words;        // count pass: LDS words of its key set / byte map (launch size)
+    unsigned long long *block_counted;  // count pass: units counted from the dense-key block
     // dense-key block over left rows x right columns: presence bits dk_P
     // (row pitch dk_pw words) and values dk_C (row pitch dk_rb); units of the
     // active tiles flagged in dk_tile start their accumulator from it.

@@ -81,6 +81,7 @@ class FccStats(C.Structure):
         ("ms_dense_gemm", C.c_double),
         ("dense_flops", C.c_uint64),
         ("dense_block_bytes", C.c_uint64),
+        ("units_block_counted", C.c_uint64),
     ]
 
 

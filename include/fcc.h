@@ -97,6 +97,8 @@ typedef struct {
     double ms_dense_gemm;  /* the dense GEMM kernel alone */
     uint64_t dense_flops;  /* flops the dense GEMM executes (2 * padded rows * columns * keys) */
     uint64_t dense_block_bytes; /* bytes of the block and presence map the accumulate kernels read */
+    uint64_t units_block_counted; /* exact planner: dense units the dense-key block covers
+                                     entirely, counted from it without a traversal */
 } fcc_stats;
 
 typedef struct {

@@ -54,6 +54,10 @@ def test_all_keys_dense(gpu_ctx, case, values, force):
     st = r.stats
     assert st["dense_keys"] > 0
     assert st["dense_key_products"] == st["products"] == ref.products
+    # only dense units can be counted from the block
+    assert st["units_block_counted"] <= st["units_dense"]
+    if force in (FCC_ACC_LHASH, FCC_ACC_GHASH):
+        assert st["units_block_counted"] == 0
     _check(r, ref, values != "uniform", f"case {case} {values} force {force}")
     r.free()
 
@@ -85,6 +89,7 @@ def test_default_threshold(gpu_ctx, name, mk, sink):
     ref = oracle.contract_tiled(a, wl.a_modes, a, wl.b_modes, tile_size=128, high_precision=True, sink=sink)
     r = gpu_ctx.contract(a, list(wl.a_modes), a, list(wl.b_modes), sink=sink, summarize=True)
     assert r.stats["dense_keys"] > 0
+    assert r.stats["units_block_counted"] <= r.stats["units_dense"]
     assert r.nnz == ref.nnz and r.coord_hash == ref.coord_hash
     assert r.val_sum == pytest.approx(ref.val_sum, rel=1e-10, abs=1e-10)
     assert r.val_wsum == pytest.approx(ref.val_wsum, rel=1e-10, abs=1e-10)
@@ -94,6 +99,30 @@ def test_default_threshold(gpu_ctx, name, mk, sink):
     assert off.stats["dense_keys"] == 0
     assert off.nnz == r.nnz and off.coord_hash == r.coord_hash
     off.free()
+    r.free()
+
+
+@pytest.mark.parametrize("values", ["uniform", "smallint"])
+@pytest.mark.parametrize("shape", [((40, 30), (30, 50)), ((300, 37), (37, 200)), ((9, 13, 20), (13, 9, 150))])
+def test_block_covered_units_counted_from_block(gpu_ctx, values, shape):
+    """Fully stored operands: every output cell is reached by a dense key, so
+    the exact planner counts every dense unit from the block's presence map
+    instead of traversing it; sizes and values still match the oracle."""
+    print(f"shape={shape} values={values}", flush=True)
+    rng = np.random.default_rng(4500 + len(shape[0]))
+    da, db = shape
+    nA, nB = int(np.prod(da)), int(np.prod(db))
+    a = rand_tensor(rng, da, nA, values)  # every coordinate stored
+    b = rand_tensor(rng, db, nB, values)
+    am = (1,) if len(da) == 2 else (1, 0)
+    bm = (0,) if len(db) == 2 else (0, 1)
+    ref = oracle.contract_bruteforce(a, am, b, bm)
+    r = gpu_ctx.contract(a, list(am), b, list(bm), dense_keys=ALL_KEYS, force_acc=FCC_ACC_DENSE)
+    st = r.stats
+    assert st["dense_keys"] > 0 and st["units_dense"] > 0
+    assert st["units_block_counted"] == st["units_dense"]
+    assert r.nnz == ref.nnz
+    _check(r, ref, values != "uniform", f"covered {shape} {values}")
     r.free()
 
 
