@@ -146,6 +146,25 @@ def test_fused_batch_sink(gpu_ctx, planner):
         r.free()
 
 
+def test_fused_batch_many_pairs_sink(gpu_ctx):
+    """2100 small pairs in one launch: the per-pair summaries read back at the
+    end (4 x 8 bytes per pair) exceed the library's pinned read-back buffer and
+    take its direct-copy path; every pair still matches the oracle."""
+    rng = np.random.default_rng(3500)
+    specs = [((5, 4, 6), int(rng.integers(1, 40)), (4, 7), int(rng.integers(1, 20)), "smallint")
+             for _ in range(2100)]
+    pairs = _pairs(3501, specs)
+    ea, eb = _bbox([p[0] for p in pairs]), _bbox([p[1] for p in pairs])
+    rs = gpu_ctx.contract_batch(pairs, [1], [0], sink=True)
+    assert len(rs) == len(pairs)
+    for i, ((a, b), r) in enumerate(zip(pairs, rs)):
+        ref = oracle.contract_tiled(a, (1,), b, (0,), tile_size=8, sink=True, a_extents=ea, b_extents=eb)
+        assert r.nnz == ref.nnz, f"pair {i}"
+        assert r.coord_hash == ref.coord_hash, f"pair {i}"
+        assert r.val_sum == ref.val_sum and r.val_wsum == pytest.approx(ref.val_wsum, rel=1e-12, abs=1e-12), f"pair {i}"
+        r.free()
+
+
 def test_fused_batch_matches_single_calls(gpu_ctx):
     """Scaled uneven pairs of the order-4 x order-3 batch shape: each fused
     pair equals its own fcc_contract call (same coordinates, values within the
