@@ -104,8 +104,13 @@ __global__ __launch_bounds__(256) void k_extents(const uint32_t *coords, const d
             mx = s_mx[w] > mx ? s_mx[w] : mx;
             flag |= s_fl[w];
         }
-        if (m < order) atomicMax(&out[m], mx);
-        else if (flag) atomicOr(&out[order], 1u);
+        // most blocks hold the maximum the first ones already stored: read
+        // first, so same-address atomics (serialised in L2) stay few
+        if (m < order) {
+            if (mx > __atomic_load_n(&out[m], __ATOMIC_RELAXED)) atomicMax(&out[m], mx);
+        } else if (flag) {
+            atomicOr(&out[order], 1u);
+        }
     }
 }
 
