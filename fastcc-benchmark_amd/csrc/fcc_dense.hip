@@ -195,23 +195,26 @@ __global__ __launch_bounds__(256) void k_dk_scatter_B(const uint32_t *hfib, cons
 // C = AT^T * Bd. Workgroup tile 128 x 128, four wavefronts of 64 x 64, each a
 // 4 x 4 grid of v_mfma_f64_16x16x4 blocks (16 f64x4 accumulators). The k
 // loop steps 16 keys at a time through double-buffered LDS panels
-// sA[k][row], sB[k][col]; the next panels are loaded into registers while the
-// current ones feed the matrix cores, one barrier per step.
+// sA[k][row], sB[k][col]; the next panels stream from global memory straight
+// into LDS while the current ones feed the matrix cores, one barrier per step.
 // MFMA operand layout (16x16x4, f64): lane l holds A[i = l % 16][k = l / 16]
 // and B[k = l / 16][j = l % 16]; the result block D[i][j] sits in lane
 // j + 16 * (i % 4), element i / 4.
 // LDS banks: an operand read takes 16 consecutive doubles from each of four
 // k rows; k rows are DK_LDS_ROW = 144 doubles apart (128 B past a multiple of
-// 256 B), so two rows read in the same cycle fall on disjoint banks. Panel
-// stores write 256 contiguous bytes per 16 lanes.
+// 256 B), so two rows read in the same cycle fall on disjoint banks. A panel
+// load writes one whole k row (1 KiB, 16 bytes per lane) per instruction.
 constexpr uint32_t DK_LDS_ROW = DK_BM + 16;
 static_assert(DK_BM == DK_BN, "one padded LDS row layout for both panels");
 
 __global__ __launch_bounds__(256, 2) void k_dgemm_tn(const double *__restrict__ AT, const double *__restrict__ Bd,
                                                      double *__restrict__ C, uint32_t nbm, uint32_t nbn, uint64_t ra,
                                                      uint64_t rb, uint32_t kp, uint32_t splits) {
-    __shared__ __align__(16) double sA[2][DK_BK][DK_LDS_ROW];
-    __shared__ __align__(16) double sB[2][DK_BK][DK_LDS_ROW];
+    // two buffers as four distinct LDS objects: the compiler then knows that
+    // reads of one buffer do not depend on panel loads still landing in the
+    // other, and waits for those loads only at the step's end
+    __shared__ __align__(16) double sA0[DK_BK][DK_LDS_ROW], sA1[DK_BK][DK_LDS_ROW];
+    __shared__ __align__(16) double sB0[DK_BK][DK_LDS_ROW], sB1[DK_BK][DK_LDS_ROW];
     // Workgroups are dealt round-robin over the 8 XCDs (one L2 each): give
     // every XCD a contiguous run of logical tiles, and walk logical tiles down
     // bands of 8 row tiles so neighbours share AT rows and a Bd column panel.
@@ -229,65 +232,70 @@ __global__ __launch_bounds__(256, 2) void k_dgemm_tn(const double *__restrict__ 
 
     const uint32_t t = threadIdx.x, lane = t & 63, wave = t >> 6;
     const uint32_t wm = (wave >> 1) * 64, wn = (wave & 1) * 64;
-    // panel copy: k row lr, doubles lc + 32 v (v = 0..3) as pairs
-    const uint32_t lr = t >> 4, lc = (t & 15) * 2;
     const uint32_t nk = kp / DK_BK;
     const uint32_t kb0 = (uint32_t)((uint64_t)z * nk / splits), kb1 = (uint32_t)((uint64_t)(z + 1) * nk / splits);
-    const double *gA = AT + ((uint64_t)kb0 * DK_BK + lr) * ra + (uint64_t)tm * DK_BM + lc;
-    const double *gB = Bd + ((uint64_t)kb0 * DK_BK + lr) * rb + (uint64_t)tn * DK_BN + lc;
     C += (uint64_t)z * ra * rb;
     f64x4 acc[4][4];
 #pragma unroll
     for (int i = 0; i < 4; i++)
 #pragma unroll
         for (int j = 0; j < 4; j++) acc[i][j] = f64x4{0.0, 0.0, 0.0, 0.0};
-    double2 pa[4], pb[4];
+    // Panels go from global memory straight into LDS (global_load_lds_dwordx4:
+    // lane l's 16 bytes land at byte 16 l of the row), one k step ahead of
+    // their use: one instruction moves a k row of 128 doubles, each wavefront
+    // moves 4 rows of each panel. No staging registers, and the loads of step
+    // k + 1 complete under step k's matrix ops.
+    const double *gA = AT + (uint64_t)tm * DK_BM + 2 * lane;
+    const double *gB = Bd + (uint64_t)tn * DK_BN + 2 * lane;
+    using Panel = double[DK_BK][DK_LDS_ROW];
+    auto issue_panel = [&](uint32_t k, Panel &pA, Panel &pB) {
+        const uint64_t k0 = (uint64_t)(kb0 + k) * DK_BK;
 #pragma unroll
-    for (int v = 0; v < 4; v++) {
-        pa[v] = *reinterpret_cast<const double2 *>(gA + 32 * v);
-        pb[v] = *reinterpret_cast<const double2 *>(gB + 32 * v);
-    }
-#pragma unroll
-    for (int v = 0; v < 4; v++) {
-        *reinterpret_cast<double2 *>(&sA[0][lr][lc + 32 * v]) = pa[v];
-        *reinterpret_cast<double2 *>(&sB[0][lr][lc + 32 * v]) = pb[v];
-    }
-    __syncthreads();
-    const uint32_t kr = lane >> 4, ci = lane & 15;
-    const uint32_t nks = kb1 - kb0;
-    for (uint32_t kb = 0; kb < nks; kb++) {
-        const uint32_t cur = kb & 1;
-        // the next panels (the last step reloads its own: no branch, so the
-        // staging registers stay registers; the idle buffer takes the copy)
-        const uint32_t kn = kb + 1 < nks ? kb + 1 : kb;
-        {
-            const uint64_t oa = (uint64_t)kn * DK_BK * ra, ob = (uint64_t)kn * DK_BK * rb;
-#pragma unroll
-            for (int v = 0; v < 4; v++) {
-                pa[v] = *reinterpret_cast<const double2 *>(gA + oa + 32 * v);
-                pb[v] = *reinterpret_cast<const double2 *>(gB + ob + 32 * v);
-            }
+        for (int r = 0; r < 4; r++) {
+            const uint32_t row = wave * 4 + r;
+            __builtin_amdgcn_global_load_lds((const void *)(gA + (k0 + row) * ra),
+                                             (__attribute__((address_space(3))) void *)&pA[row][0], 16, 0, 0);
+            __builtin_amdgcn_global_load_lds((const void *)(gB + (k0 + row) * rb),
+                                             (__attribute__((address_space(3))) void *)&pB[row][0], 16, 0, 0);
         }
+    };
+    const uint32_t kr = lane >> 4, ci = lane & 15;
+    auto compute = [&](const Panel &pA, const Panel &pB) {
 #pragma unroll
         for (uint32_t kk = 0; kk < DK_BK; kk += 4) {
             double a[4], b[4];
 #pragma unroll
-            for (int i = 0; i < 4; i++) a[i] = sA[cur][kk + kr][wm + i * 16 + ci];
+            for (int i = 0; i < 4; i++) a[i] = pA[kk + kr][wm + i * 16 + ci];
 #pragma unroll
-            for (int j = 0; j < 4; j++) b[j] = sB[cur][kk + kr][wn + j * 16 + ci];
+            for (int j = 0; j < 4; j++) b[j] = pB[kk + kr][wn + j * 16 + ci];
 #pragma unroll
             for (int i = 0; i < 4; i++)
 #pragma unroll
                 for (int j = 0; j < 4; j++)
                     acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i], b[j], acc[i][j], 0, 0, 0);
         }
-#pragma unroll
-        for (int v = 0; v < 4; v++) {
-            *reinterpret_cast<double2 *>(&sA[cur ^ 1][lr][lc + 32 * v]) = pa[v];
-            *reinterpret_cast<double2 *>(&sB[cur ^ 1][lr][lc + 32 * v]) = pb[v];
-        }
+    };
+    // this wavefront's panel loads have landed, then every wavefront's
+    auto land = [] {
+        __builtin_amdgcn_s_waitcnt(0);
         __syncthreads();
+    };
+    // Step k reads buffer k % 2 while step k + 1's panels load into the other,
+    // whose last readers passed the previous step's barrier. Two steps per
+    // trip keep each buffer a fixed object.
+    const uint32_t nks = kb1 - kb0;
+    if (nks) issue_panel(0, sA0, sB0);
+    land();
+    uint32_t kb = 0;
+    for (; kb + 2 <= nks; kb += 2) {
+        issue_panel(kb + 1, sA1, sB1);
+        compute(sA0, sB0);
+        land();
+        if (kb + 2 < nks) issue_panel(kb + 2, sA0, sB0);
+        compute(sA1, sB1);
+        land();
     }
+    if (kb < nks) compute(sA0, sB0);
     const uint64_t row0 = (uint64_t)tm * DK_BM + wm + kr, col0 = (uint64_t)tn * DK_BN + wn + ci;
 #pragma unroll
     for (int i = 0; i < 4; i++)
