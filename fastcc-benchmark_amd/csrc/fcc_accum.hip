@@ -963,13 +963,18 @@ __device__ __forceinline__ uint32_t dk_dense_word(const AccArgs &A, const UnitCt
 template <bool BITMAP>
 __device__ void dk_seed_dense(const AccArgs &A, const UnitCtx &U, double *fv, uint32_t *bits, uint32_t nslots) {
     const double init = BITMAP ? 0.0 : -0.0;
+    // the presence word and the value are loaded together (every cell of the
+    // unit's rows x window lies inside the padded block), several cells per
+    // thread in flight: the loop is bound by load latency, not bandwidth
+#pragma unroll 4
     for (uint32_t s = threadIdx.x; s < nslots; s += ACC_THREADS) {
         const uint32_t li = s / U.Wp, cl = s - li * U.Wp;
         const uint64_t row = U.row0 + li, col = (uint64_t)U.c0 + cl;
         double v = init;
-        if (cl < U.W && row < A.spanA && ((A.dk_P[row * A.dk_pw + (col >> 5)] >> (col & 31)) & 1u)) {
-            v = dk_value(A, row, col);
-            if (!BITMAP) v = v + 0.0;
+        if (cl < U.W && row < A.spanA) {
+            const uint32_t pw = A.dk_P[row * A.dk_pw + (col >> 5)];
+            const double c = dk_value(A, row, col);
+            if ((pw >> (col & 31)) & 1u) v = BITMAP ? c : c + 0.0;
         }
         fv[s] = v;
     }

@@ -1061,7 +1061,9 @@ int Contraction::run(const fcc_tensor *ta, const int32_t *am, const fcc_tensor *
     st.ms_dense = ev_ms(ev[7], ev[8]);
     st.ms_dense_gemm = ev_ms(ev[10], ev[11]);
     if (dk.nh) {
-        st.dense_flops = 2ull * dk.ra * dk.rb * dk.kp;
+        // the tiles the GEMM computes: all, or (self) those on and above the diagonal
+        const uint64_t nbm = dk.ra / DK_BM, tiles = dk.self ? nbm * (nbm + 1) / 2 : nbm * (dk.rb / DK_BN);
+        st.dense_flops = 2ull * tiles * DK_BM * DK_BN * dk.kp;
         // the tiles' units read the block's cells once (every k split) plus the presence words
         st.dense_block_bytes = dk.splits * dk.spanA * dk.spanB * 8 + dk.spanA * dk.spanB / 8;
     }

@@ -81,10 +81,12 @@ void dense_keys_values(const Index &ix, const DenseKeys &dk, hipStream_t s);
 
 // C[z][ra][rb] (z < splits) = partial sums of AT^T * Bd for row-major
 // AT[kp][ra], Bd[kp][rb]; ra, rb multiples of DK_BM / DK_BN, kp a multiple
-// of DK_BK; split z sums its share of the k steps.
+// of DK_BK; split z sums its share of the k steps. sym (AT == Bd, ra == rb:
+// C is symmetric) computes the tiles on and above the diagonal and mirrors
+// them, half the matrix-core work.
 void launch_dgemm_tn(const double *AT, const double *Bd, double *C, uint64_t ra, uint64_t rb, uint32_t kp,
-                     uint32_t splits, hipStream_t s);
+                     uint32_t splits, bool sym, hipStream_t s);
 // k splits that fill the GPU's workgroup slots for a small grid (1 for large).
-uint32_t dgemm_splits(uint64_t ra, uint64_t rb, uint32_t kp, int num_cus);
+uint32_t dgemm_splits(uint64_t ra, uint64_t rb, uint32_t kp, bool sym, int num_cus);
 
 }  // namespace fcc

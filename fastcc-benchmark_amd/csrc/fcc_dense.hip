@@ -209,7 +209,7 @@ static_assert(DK_BM == DK_BN, "one padded LDS row layout for both panels");
 
 __global__ __launch_bounds__(256, 2) void k_dgemm_tn(const double *__restrict__ AT, const double *__restrict__ Bd,
                                                      double *__restrict__ C, uint32_t nbm, uint32_t nbn, uint64_t ra,
-                                                     uint64_t rb, uint32_t kp, uint32_t splits) {
+                                                     uint64_t rb, uint32_t kp, uint32_t splits, bool sym) {
     // two buffers as four distinct LDS objects: the compiler then knows that
     // reads of one buffer do not depend on panel loads still landing in the
     // other, and waits for those loads only at the step's end
@@ -220,15 +220,45 @@ __global__ __launch_bounds__(256, 2) void k_dgemm_tn(const double *__restrict__ 
     // bands of 8 row tiles so neighbours share AT rows and a Bd column panel.
     // With split k (small grids) workgroup z of a tile sums k steps
     // [z * nk / splits, (z + 1) * nk / splits) into its own slab of C.
-    const uint32_t nblk = nbm * nbn, total = nblk * splits;
+    // sym (AT is Bd, so C is symmetric): only tiles tm <= tn, each band
+    // holding the columns from its own first row on; an off-diagonal tile is
+    // also stored transposed.
+    const uint32_t band = 8;
+    const uint32_t nblk = sym ? nbm * (nbm + 1) / 2 : nbm * nbn, total = nblk * splits;
     const uint32_t x = blockIdx.x & 7, jx = blockIdx.x >> 3;
     const uint32_t q = total >> 3, rem = total & 7;
     const uint32_t Lz = x * q + min(x, rem) + jx;
     const uint32_t z = Lz / nblk, L = Lz - z * nblk;
-    const uint32_t band = 8, per_band = band * nbn;
-    const uint32_t bnd = L / per_band, w = L - bnd * per_band;
-    const uint32_t rows_in = min(band, nbm - bnd * band);
-    const uint32_t tm = bnd * band + w % rows_in, tn = w / rows_in;
+    uint32_t tm, tn;
+    if (sym) {
+        // band r0 / band: R rows, columns r0 .. nbm - 1, column r0 + j holding
+        // min(R, j + 1) of them (the first R columns a triangle)
+        uint32_t r0 = 0, w = L, R = min(band, nbm);
+        for (;;) {
+            const uint32_t cnt = R * (R + 1) / 2 + (nbm - r0 - R) * R;
+            if (w < cnt) break;
+            w -= cnt;
+            r0 += band;
+            R = min(band, nbm - r0);
+        }
+        const uint32_t tri = R * (R + 1) / 2;
+        if (w < tri) {
+            uint32_t j = 0;
+            while ((j + 1) * (j + 2) / 2 <= w) j++;
+            tm = r0 + w - j * (j + 1) / 2;
+            tn = r0 + j;
+        } else {
+            w -= tri;
+            tm = r0 + w % R;
+            tn = r0 + R + w / R;
+        }
+    } else {
+        const uint32_t per_band = band * nbn;
+        const uint32_t bnd = L / per_band, w = L - bnd * per_band;
+        const uint32_t rows_in = min(band, nbm - bnd * band);
+        tm = bnd * band + w % rows_in;
+        tn = w / rows_in;
+    }
 
     const uint32_t t = threadIdx.x, lane = t & 63, wave = t >> 6;
     const uint32_t wm = (wave >> 1) * 64, wn = (wave & 1) * 64;
@@ -303,26 +333,59 @@ __global__ __launch_bounds__(256, 2) void k_dgemm_tn(const double *__restrict__ 
         for (int j = 0; j < 4; j++)
 #pragma unroll
             for (int r = 0; r < 4; r++) C[(row0 + i * 16 + 4 * r) * rb + col0 + j * 16] = acc[i][j][r];
+    if (sym && tm != tn) {
+        // The mirror tile (ra == rb), transposed through LDS so that every
+        // store writes whole 512-byte column runs: each wavefront takes one
+        // of the four panel buffers (free once all have passed the barrier)
+        // for 32 of its 64 columns at a time, columns TS doubles apart.
+        constexpr uint32_t TS = 66;
+        static_assert(32 * TS <= DK_BK * DK_LDS_ROW, "a mirror half fits one panel buffer");
+        __syncthreads();
+        double *T = wave == 0 ? &sA0[0][0] : wave == 1 ? &sA1[0][0] : wave == 2 ? &sB0[0][0] : &sB1[0][0];
+        const uint64_t mrow0 = (uint64_t)tn * DK_BN + wn, mcol0 = (uint64_t)tm * DK_BM + wm;
+#pragma unroll
+        for (int h = 0; h < 2; h++) {
+#pragma unroll
+            for (int jj = 0; jj < 2; jj++)
+#pragma unroll
+                for (int i = 0; i < 4; i++)
+#pragma unroll
+                    for (int r = 0; r < 4; r++) T[(jj * 16 + ci) * TS + i * 16 + kr + 4 * r] = acc[i][2 * h + jj][r];
+            __syncthreads();
+            const uint32_t rr = (lane & 31) * 2;
+#pragma unroll
+            for (int it = 0; it < 16; it++) {
+                const uint32_t c = 2 * it + (lane >> 5);
+                const double2 v = *reinterpret_cast<const double2 *>(&T[c * TS + rr]);
+                *reinterpret_cast<double2 *>(&C[(mrow0 + 32 * h + c) * rb + mcol0 + rr]) = v;
+            }
+            __syncthreads();
+        }
+    }
 }
 
 }  // namespace
 
 void launch_dgemm_tn(const double *AT, const double *Bd, double *C, uint64_t ra, uint64_t rb, uint32_t kp,
-                     uint32_t splits, hipStream_t s) {
+                     uint32_t splits, bool sym, hipStream_t s) {
     const uint32_t nbm = (uint32_t)(ra / DK_BM), nbn = (uint32_t)(rb / DK_BN);
     if (!nbm || !nbn || !kp) return;
-    k_dgemm_tn<<<nbm * nbn * splits, 256, 0, s>>>(AT, Bd, C, nbm, nbn, ra, rb, kp, splits);
+    if (sym && (AT != Bd || ra != rb)) throw LibError{ERR_INVALID, "launch_dgemm_tn: sym needs AT == Bd and ra == rb"};
+    const uint32_t nblk = sym ? nbm * (nbm + 1) / 2 : nbm * nbn;
+    k_dgemm_tn<<<nblk * splits, 256, 0, s>>>(AT, Bd, C, nbm, nbn, ra, rb, kp, splits, sym);
 }
 
-uint32_t dgemm_splits(uint64_t ra, uint64_t rb, uint32_t kp, int num_cus) {
+uint32_t dgemm_splits(uint64_t ra, uint64_t rb, uint32_t kp, bool sym, int num_cus) {
     // two workgroups per CU. The GEMM runs beside the left-tile build, and
-    // every split is one more partial block each unit reads back, so k is
-    // split only when the tiles cannot fill half the workgroup slots; then
-    // the split makes the last wave of workgroups nearly full, keeping at
-    // least 8 k steps per split
-    const uint64_t nblk = (ra / DK_BM) * (rb / DK_BN), slots = 2ull * (uint64_t)std::max(1, num_cus);
+    // every split is one more partial block each unit reads back (measured:
+    // 210 tiles split in two on 256 CUs made the accumulate kernel 13% slower
+    // and the GEMM no faster), so k is split only for small blocks, tiles on
+    // under a quarter of the CUs; then the split makes the last wave of
+    // workgroups nearly full, keeping at least 8 k steps per split
+    const uint64_t nbm = ra / DK_BM, nblk = sym ? nbm * (nbm + 1) / 2 : nbm * (rb / DK_BN);
+    const uint64_t slots = 2ull * (uint64_t)std::max(1, num_cus);
     const uint32_t nk = kp / DK_BK;
-    if (2 * nblk >= slots) return 1;
+    if (4 * nblk >= (uint64_t)std::max(1, num_cus)) return 1;
     uint32_t best = 1;
     double best_eff = 0;
     for (uint32_t sp = 1; sp <= 8 && nk / sp >= 8; sp++) {
@@ -410,7 +473,7 @@ void dense_keys_alloc(DevPool &pool, const Geom &g, DenseKeys &dk, int num_cus,
     dk.P = (uint32_t *)keep((size_t)(ra * (rb / 32) + 2) * 4);
     dk.Bd = (double *)keep((size_t)dk.kp * rb * 8);
     dk.AT = dk.self ? dk.Bd : (double *)keep((size_t)dk.kp * ra * 8);  // self: the same panel
-    dk.splits = dgemm_splits(ra, rb, dk.kp, num_cus);
+    dk.splits = dgemm_splits(ra, rb, dk.kp, dk.self, num_cus);
     dk.C = (double *)keep((size_t)dk.splits * ra * rb * 8);
 }
 
@@ -432,7 +495,7 @@ void dense_keys_block(const Index &ix, const DenseKeys &dk, const uint32_t *fidA
     k_dk_presence<<<dim3((unsigned)(rb / 64), (unsigned)(ra / 64)), 256, 0, s>>>(dk.rowbits, dk.colbits, dk.spanA, rb,
                                                                                  dk.nw, dk.P);
     if (gemm0) FCC_HIP(hipEventRecord(gemm0, s));
-    launch_dgemm_tn(dk.AT, dk.Bd, dk.C, ra, rb, dk.kp, dk.splits, s);
+    launch_dgemm_tn(dk.AT, dk.Bd, dk.C, ra, rb, dk.kp, dk.splits, dk.self, s);
     if (gemm1) FCC_HIP(hipEventRecord(gemm1, s));
 }
 
@@ -464,7 +527,7 @@ void dense_keys_values(const Index &ix, const DenseKeys &dk, hipStream_t s) {
         k_dk_scatter_A<<<grid_of(ix.ngroups), 256, 0, s>>>(ix.ngroups, ix.group_ptr, ix.group_fiber, dk.hkey, ix.Arow,
                                                             ix.Aval, dk.ra, dk.AT);
     k_dk_scatter_B<<<dk.nh, 256, 0, s>>>(dk.hfib, ix.fiber_ptr, ix.Bcol, ix.Bval, dk.rb, dk.Bd);
-    launch_dgemm_tn(dk.AT, dk.Bd, dk.C, dk.ra, dk.rb, dk.kp, dk.splits, s);
+    launch_dgemm_tn(dk.AT, dk.Bd, dk.C, dk.ra, dk.rb, dk.kp, dk.splits, dk.self, s);
 }
 
 }  // namespace fcc

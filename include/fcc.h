@@ -95,7 +95,7 @@ typedef struct {
     double ms_dense;       /* the dense-key block: bitsets, presence map, value panels, GEMM
                               (on a second stream, beside the left-tile build) */
     double ms_dense_gemm;  /* the dense GEMM kernel alone */
-    uint64_t dense_flops;  /* flops the dense GEMM executes (2 * padded rows * columns * keys) */
+    uint64_t dense_flops;  /* flops the dense GEMM executes (2 * padded rows * columns * keys; a self-contraction computes the tiles on and above the diagonal) */
     uint64_t dense_block_bytes; /* bytes of the block and presence map the accumulate kernels read */
     uint64_t units_block_counted; /* exact planner: dense units the dense-key block covers
                                      entirely, counted from it without a traversal */

@@ -126,6 +126,55 @@ def test_block_covered_units_counted_from_block(gpu_ctx, values, shape):
     r.free()
 
 
+# Self-contractions (one tensor, the same modes on both sides): the GEMM's
+# operands are one panel, so the block is symmetric; it computes the tiles on
+# and above the diagonal and mirrors them. Spans of one tile, a partial band
+# of tiles (3, 5), two bands (9 tiles: 8 + 1); 300 keys over 5 tiles
+# (15 computed) give a split k.
+SELF_CASES = [
+    ((40, 90), 3000, (0,)),
+    ((300, 600), 20000, (0,)),
+    ((20, 15, 30), 6000, (1,)),
+    ((60, 1100), 30000, (0,)),
+    ((6, 5, 40, 9), 5000, (0, 1)),
+]
+
+
+@pytest.mark.parametrize("case", range(len(SELF_CASES)))
+@pytest.mark.parametrize("values", ["uniform", "smallint"])
+def test_self_contraction_symmetric_block(gpu_ctx, case, values):
+    print(f"case={case} values={values}", flush=True)
+    rng = np.random.default_rng(4600 + case)
+    dims, nnz, modes = SELF_CASES[case]
+    a = rand_tensor(rng, dims, nnz, values)
+    ref = oracle.contract_bruteforce(a, modes, a, modes)
+    r = gpu_ctx.contract(a, list(modes), a, list(modes), dense_keys=ALL_KEYS)
+    st = r.stats
+    assert st["dense_keys"] > 0
+    assert st["dense_key_products"] == st["products"] == ref.products
+    span = int(np.prod([d for m, d in enumerate(dims) if m not in modes]))
+    nbm, kp = -(-span // 128), -(-st["dense_keys"] // 16) * 16
+    assert st["dense_flops"] == 2 * (nbm * (nbm + 1) // 2) * 128 * 128 * kp  # the upper triangle of tiles
+    _check(r, ref, values != "uniform", f"self case {case} {values}")
+    r.free()
+
+
+@pytest.mark.parametrize("use_graph", [False, True])
+def test_self_plan_new_values(gpu_ctx, use_graph):
+    """A self-contraction plan re-runs the half GEMM on new values (two tiles
+    of rows, so one mirrored tile)."""
+    rng = np.random.default_rng(4700)
+    a = rand_tensor(rng, (50, 200), 4000, "uniform")
+    plan = gpu_ctx.plan(a, [0], a, [0], dense_keys=ALL_KEYS)
+    for it in range(3):
+        a2 = SparseTensor(a.coords, rng.uniform(-1, 1, a.nnz))
+        ref = oracle.contract_bruteforce(a2, (0,), a2, (0,))
+        r = plan.execute(a2.vals, graph=use_graph)
+        assert r.stats["dense_keys"] > 0
+        _check(r, ref, False, f"self plan it {it} graph {use_graph}")
+    plan.close()
+
+
 def test_batch_modes(gpu_ctx):
     """Batch pairs lead rows and keys: a key is dense within its batch."""
     rng = np.random.default_rng(4300)

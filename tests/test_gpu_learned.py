@@ -77,19 +77,28 @@ def test_any_prediction_stays_exact(bias):
 
 
 def test_learned_skips_count_pass_and_is_faster(trained_ctx):
-    """What the trained network gains: the count pass disappears, end-to-end
-    device time drops on a mid-size order-4 self-contraction."""
+    """What the trained network gains on a mid-size order-4 self-contraction:
+    the count pass disappears, and with every product through the tiles
+    (dense keys off) end-to-end device time drops. With the dense-key block
+    most units are counted from its presence map instead of traversed, so
+    there the two planners must run level (learned within 5% of exact)."""
     ctx, _, _ = trained_ctx
     wl = W.c2(0.5)
     a, _ = wl.tensors()
-    times = {}
-    for pl in ("exact", "learned"):
-        ts = []
-        for _ in range(4):
-            r = ctx.contract(a, list(wl.a_modes), a, list(wl.b_modes), planner=pl)
-            ts.append(r.stats["ms_total"])
-            n = r.nnz
-            r.free()
-        times[pl] = min(ts[1:])
-        print(f"{pl}: {times[pl]:.3f} ms, nnz {n}")
-    assert times["learned"] < times["exact"]
+    for dk in (-1.0, 0.0):
+        times = {}
+        for pl in ("exact", "learned"):
+            ts = []
+            for _ in range(4):
+                r = ctx.contract(a, list(wl.a_modes), a, list(wl.b_modes), planner=pl, dense_keys=dk)
+                ts.append(r.stats["ms_total"])
+                if pl == "learned":
+                    assert r.stats["ms_symbolic"] == 0.0
+                n = r.nnz
+                r.free()
+            times[pl] = min(ts[1:])
+            print(f"dense_keys {dk} {pl}: {times[pl]:.3f} ms, nnz {n}")
+        if dk < 0:
+            assert times["learned"] < times["exact"]
+        else:
+            assert times["learned"] < 1.05 * times["exact"]
