@@ -590,10 +590,14 @@ int Contraction::run(const fcc_tensor *ta, const int32_t *am, const fcc_tensor *
     if (const char *hs = getenv("FCC_HSEG")) ix.hseg = (uint32_t)std::max(64, atoi(hs));
     ix.defer_hseg = dk.nh > 0;  // built once, after the dense keys are masked
     ix.features = o.planner == FCC_PLAN_LEARNED || o.collect_stats;
-    // A large GEMM (C4-sized: hundreds of GFLOP) outlasts the whole left-tile
-    // build, so it starts before the left sort; a small one waits for the
-    // sort to be queued (they compete for the LDS).
-    const bool early_fork = dk.nh && 2.0 * (double)dk.ra * (double)dk.rb * (double)dk.kp > 1e11;
+    // A very large GEMM outlasts the whole left-tile build, so it starts
+    // before the left sort; a smaller one waits for the sort to be queued
+    // (they compete for the LDS: beside the GEMM a sort pass ran 6x slower).
+    // Measured on the order-5 self-contraction: its full 340 GFLOP GEMM was
+    // better forked early, its symmetric 170 GFLOP half 0.7 ms better late.
+    const uint64_t dk_nbm = dk.ra / DK_BM;
+    const double dk_tiles = dk.self ? (double)(dk_nbm * (dk_nbm + 1) / 2) : (double)dk_nbm * (double)(dk.rb / DK_BN);
+    const bool early_fork = dk.nh && 2.0 * dk_tiles * DK_BM * DK_BN * (double)dk.kp > 2.5e11;
     if (early_fork) fork_dense();
     const std::function<void()> *after_sort = (dk.nh && !early_fork) ? &fork_dense : nullptr;
     if (self && o.left_tile_stride <= 1) build_left_self(pool, g, A, sj, logTL, ix, hold, s, after_sort);
