@@ -90,7 +90,7 @@ void DevPool::release_all() {
 // ------------------------------------------------------------------- scan
 
 constexpr int SCAN_THREADS = 256;
-constexpr int SCAN_ITEMS = 8;
+constexpr int SCAN_ITEMS = 16;
 constexpr int SCAN_TILE = SCAN_THREADS * SCAN_ITEMS;
 
 template <typename T> __device__ inline T wave_incl_scan(T v) {
@@ -135,24 +135,44 @@ template <typename T> __global__ __launch_bounds__(SCAN_THREADS) void k_block_re
     if (threadIdx.x == 0) sums[blockIdx.x] = tot;
 }
 
+// A tile is staged through LDS: coalesced global loads and stores, and each
+// thread scans SCAN_ITEMS consecutive elements read back from LDS. One pad
+// slot per SCAN_ITEMS elements spreads a thread's run over the banks
+// (thread t's run starts at t * (SCAN_ITEMS + 1)). in may equal out: a block
+// reads its whole tile before it writes.
+__device__ __forceinline__ int scan_pad(int e) { return e + e / SCAN_ITEMS; }
+
 template <typename T>
 __global__ __launch_bounds__(SCAN_THREADS) void k_block_scan(const T *in, T *out, size_t n, const T *offsets,
                                                              T *total) {
-    size_t base = (size_t)blockIdx.x * SCAN_TILE + (size_t)threadIdx.x * SCAN_ITEMS;
-    T v[SCAN_ITEMS];
-    T s = 0;
+    __shared__ T s[SCAN_TILE + SCAN_THREADS];
+    const size_t base = (size_t)blockIdx.x * SCAN_TILE;
+    const int t = threadIdx.x;
 #pragma unroll
     for (int i = 0; i < SCAN_ITEMS; i++) {
-        T x = (base + i < n) ? in[base + i] : T(0);
-        v[i] = s;
-        s += x;
+        const int e = i * SCAN_THREADS + t;
+        s[scan_pad(e)] = base + e < n ? in[base + e] : T(0);
+    }
+    __syncthreads();
+    T v[SCAN_ITEMS];
+    T sum = 0;
+#pragma unroll
+    for (int j = 0; j < SCAN_ITEMS; j++) {
+        const T x = s[t * (SCAN_ITEMS + 1) + j];
+        v[j] = sum;
+        sum += x;
     }
     T tot;
-    T pre = block_excl_scan(s, &tot);
-    T off = (offsets ? offsets[blockIdx.x] : T(0)) + pre;
+    const T pre = block_excl_scan(sum, &tot);  // its barriers also end every read of s above
+    const T off = (offsets ? offsets[blockIdx.x] : T(0)) + pre;
 #pragma unroll
-    for (int i = 0; i < SCAN_ITEMS; i++)
-        if (base + i < n) out[base + i] = v[i] + off;
+    for (int j = 0; j < SCAN_ITEMS; j++) s[t * (SCAN_ITEMS + 1) + j] = v[j] + off;
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < SCAN_ITEMS; i++) {
+        const int e = i * SCAN_THREADS + t;
+        if (base + e < n) out[base + e] = s[scan_pad(e)];
+    }
     if (total && blockIdx.x == gridDim.x - 1 && threadIdx.x == 0)
         *total = (offsets ? offsets[blockIdx.x] : T(0)) + tot;
 }
