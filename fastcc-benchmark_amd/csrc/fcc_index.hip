@@ -72,23 +72,54 @@ __global__ __launch_bounds__(256) void k_extents(const uint32_t *coords, const d
     const int m = blockIdx.y;
     uint32_t mx = 0, flag = 0;
     const size_t stride = (size_t)gridDim.x * blockDim.x;
-    auto take = [&](size_t i) {
-        if (m < order) {
-            uint32_t c = coords[(size_t)m * nnz + i];
-            mx = c > mx ? c : mx;
-        } else {
-            double v = vals[i];
-            double a = fabs(v);
-            if (!(a >= 0x1.0p-511) && !isnan(v)) flag = 1;
-        }
-    };
-    size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
-    // four independent loads in flight per thread
-    for (; i + 3 * stride < nnz; i += 4 * stride) {
+    const size_t gid = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+    // 16-byte loads over the aligned body of the mode's column (or of the
+    // values), four in flight per thread; the few elements before the first
+    // 16-byte boundary and after the last whole vector one at a time
+    if (m < order) {
+        const uint32_t *p = coords + (size_t)m * nnz;
+        const size_t head = std::min<size_t>(nnz, ((16 - ((uintptr_t)p & 15)) & 15) / 4);
+        const uint4 *q = reinterpret_cast<const uint4 *>(p + head);
+        const size_t nq = (nnz - head) / 4;
+        auto take4 = [&](uint4 w) {
+            mx = max(mx, max(max(w.x, w.y), max(w.z, w.w)));
+        };
+        size_t j = gid;
+        for (; j + 3 * stride < nq; j += 4 * stride) {
+            uint4 w[4];
 #pragma unroll
-        for (int u = 0; u < 4; u++) take(i + u * stride);
+            for (int u = 0; u < 4; u++) w[u] = q[j + u * stride];
+#pragma unroll
+            for (int u = 0; u < 4; u++) take4(w[u]);
+        }
+        for (; j < nq; j += stride) take4(q[j]);
+        if (gid < head) mx = max(mx, p[gid]);
+        for (size_t i = head + 4 * nq + gid; i < nnz; i += stride) mx = max(mx, p[i]);
+    } else {
+        auto take = [&](double v) {
+            if (!(fabs(v) >= 0x1.0p-511) && !isnan(v)) flag = 1;
+        };
+        const size_t head = std::min<size_t>(nnz, ((16 - ((uintptr_t)vals & 15)) & 15) / 8);
+        const double2 *q = reinterpret_cast<const double2 *>(vals + head);
+        const size_t nq = (nnz - head) / 2;
+        size_t j = gid;
+        for (; j + 3 * stride < nq; j += 4 * stride) {
+            double2 w[4];
+#pragma unroll
+            for (int u = 0; u < 4; u++) w[u] = q[j + u * stride];
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+                take(w[u].x);
+                take(w[u].y);
+            }
+        }
+        for (; j < nq; j += stride) {
+            take(q[j].x);
+            take(q[j].y);
+        }
+        if (gid < head) take(vals[gid]);
+        for (size_t i = head + 2 * nq + gid; i < nnz; i += stride) take(vals[i]);
     }
-    for (; i < nnz; i += stride) take(i);
     for (int d = 32; d > 0; d >>= 1) {
         uint32_t o = __shfl_xor(mx, d, 64);
         mx = o > mx ? o : mx;
@@ -118,7 +149,7 @@ void launch_extents(const Operand &t, uint32_t *dev_out, hipStream_t s) {
     if (t.nnz == 0) return;
     // at most 512 workgroups per mode: each ends in one atomic on the mode's
     // maximum, and same-address atomics serialise in L2
-    unsigned gx = (unsigned)std::min<size_t>(div_up((size_t)t.nnz, 256 * 4), 512);
+    unsigned gx = (unsigned)std::min<size_t>(div_up((size_t)t.nnz, 256 * 16), 512);
     dim3 grid(gx, t.order + 1);
     k_extents<<<grid, 256, 0, s>>>(t.coords, t.vals, t.order, (size_t)t.nnz, dev_out);
 }
