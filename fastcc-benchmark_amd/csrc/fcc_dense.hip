@@ -113,11 +113,16 @@ __global__ __launch_bounds__(256) void k_dk_colbits(const uint32_t *hfib, const 
 // its column's bitset words held in registers 8 at a time; the 64 rows'
 // bitset words are staged in LDS 8 words per row at a time (broadcast reads).
 // rowbits == null: a self-contraction, whose row bitsets are its column
-// bitsets (colbits word-major, read transposed).
+// bitsets (colbits word-major, read transposed). Its map is symmetric: only
+// blocks on and above the diagonal are computed, each also writing its
+// mirror block (a 64 x 64 bit transpose through LDS).
 __global__ __launch_bounds__(256) void k_dk_presence(const unsigned long long *rowbits,
                                                     const unsigned long long *colbits, uint64_t spanA, uint64_t rb,
                                                     uint32_t nw, uint32_t *P) {
+    const bool sym = rowbits == nullptr;
+    if (sym && blockIdx.y > blockIdx.x) return;  // written by block (x, y)'s mirror
     __shared__ unsigned long long s_rw[64][8];
+    __shared__ uint32_t s_h[64][4];
     const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const uint64_t c = (uint64_t)blockIdx.x * 64 + lane;
     const uint64_t rb0 = (uint64_t)blockIdx.y * 64;
@@ -153,6 +158,17 @@ __global__ __launch_bounds__(256) void k_dk_presence(const unsigned long long *r
             uint32_t *pr = P + (r0 + i) * pw + (uint64_t)blockIdx.x * 2;
             pr[0] = (uint32_t)bal;
             pr[1] = (uint32_t)(bal >> 32);
+        }
+    }
+    if (sym && blockIdx.y != blockIdx.x) {
+        // mirror row c: its bits for columns rb0 .. rb0 + 63 are this block's
+        // column c, wavefront w holding rows rb0 + 16 w .. + 15
+        s_h[lane][wave] = hit;
+        __syncthreads();
+        if (wave == 0) {
+            uint32_t *pr = P + c * pw + rb0 / 32;
+            pr[0] = s_h[lane][0] | (s_h[lane][1] << 16);
+            pr[1] = s_h[lane][2] | (s_h[lane][3] << 16);
         }
     }
 }
