@@ -256,7 +256,8 @@ def dense_roofline(stats):
     return {"bound": "mfma", "achieved": ach, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
             "frac": ach / FP64_PEAK_TFLOPS, "traffic": None, "kernel": "k_dgemm_tn", "ms": g_ms,
             "dense_keys": st["dense_keys"], "useful_tflops": useful,
-            "note": "v_mfma_f64_16x16x4 GEMM of the dense keys; achieved counts padded panels"}
+            "note": "v_mfma_f64_16x16x4 GEMM of the dense keys; achieved counts the flops executed on padded "
+                    "panels (a self-contraction computes the tiles on and above the diagonal and mirrors them)"}
 
 
 def bench_batch(args):
