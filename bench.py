@@ -62,7 +62,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "
-                          "(multi-GPU sharding is described in DESIGN.md, not implemented)"}))
+                          "(sharding by left tiles over several GPUs is not implemented)"}))
         sys.exit(2)
 
     import numpy as np
