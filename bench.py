@@ -4,13 +4,17 @@ One step = one full contraction (index build, planning, count pass for the
 exact planner, accumulate, compaction into the SoA output) of the synthetic
 workload with its inputs already resident in HBM.
 
-Default workload: the order-4 self-contraction T(6186x24x77x32, 5.3M nnz,
-mode-0 Zipf(1.1), lognormal values) on modes {0,1}, exact-symbolic planning,
-with the learned planner timed beside it on the same inputs.
+Default workload: the order-5 self-contraction T(165427x11374x2x100x89, 26M
+nnz, modes 0 and 1 Zipf(1.05), uniform [-1,1] values) on modes {0,1}, the
+largest shape whose output (3.17e8 nonzeros, 10.1 GB) fits one GPU: learned
+planning with the autotuner's tile shape for its density bucket, the
+exact-symbolic planner timed beside it on the same inputs.
+--workload c1|c2|c3: the other single-contraction shapes (c3 keeps count +
+checksum outputs: its 3.3e10 outputs do not fit in HBM).
 --workload c5: 256 uneven independent pairs A_b(1024x128x1024x64) x
 B_b(2048x128x64) in one fused launch (learned planner, count + checksum
 outputs), reporting aggregate output nnz/s and per-pair latency percentiles.
-Prints one JSON line.
+Prints one JSON line (progress goes to stderr).
 """
 import argparse
 import json
@@ -37,26 +41,111 @@ def cpu_model():
     return platform.processor() or "unknown"
 
 
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def host_cpus():
+    """What the host-core baseline runs on: the logical CPUs this process may
+    use (sched_getaffinity), the machine's count (nproc), and the OpenMP
+    thread count the oracle's parallel loops take by default."""
+    try:
+        affinity = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        affinity = os.cpu_count() or 1
+    return {"affinity": affinity, "nproc": os.cpu_count() or 1,
+            "omp_num_threads": os.environ.get("OMP_NUM_THREADS"), "model": cpu_model()}
+
+
+def left_span(t, modes):
+    ext = t.extents()
+    span = 1
+    for m in range(t.order):
+        if m not in modes:
+            span *= int(ext[m])
+    return span
+
+
+def cpu_baseline_protocol(oracle, ta, tb, wl, nnz_full, products_full, sink, fraction, threads, budget_s=45.0):
+    """The reference benchmark's protocol (benchmark.cpp:68-73,95-115;
+    scripts/chicago01.sh:6-12) on the oracle's restatement of the reference
+    algorithm: a tile-size sweep T = 32..2048 (one call each), then the
+    median of 3 calls at the best T and at T = 128. Shapes too large for the
+    host run a deterministic sample of left tiles (about `fraction` of the
+    rows) and extrapolate linearly in products: t = index + loop x (full
+    products / sampled products). Returns the cpu_baseline object."""
+    import numpy as np
+    span = left_span(ta, wl.a_modes)
+
+    def call(T):
+        ntiles = -(-span // T)
+        stride = 1 if fraction >= 1.0 else max(1, min(ntiles, int(round(1.0 / fraction))))
+        r = oracle.contract_tiled(ta, wl.a_modes, tb, wl.b_modes, tile_size=T, threads=threads,
+                                  high_precision=False, sample_stride=stride, sink=sink, sort=False)
+        loop = r.seconds - r.index_seconds
+        scale = products_full / r.products if (stride > 1 and r.products) else 1.0
+        return {"T": T, "stride": stride, "seconds": r.seconds, "index_s": r.index_seconds,
+                "full_s": r.index_seconds + loop * scale, "sampled_products": int(r.products),
+                "sampled_nnz": int(r.nnz), "threads": int(r.threads)}
+
+    t_start = time.perf_counter()
+    sweep = []
+    for T in (128, 32, 64, 256, 512, 1024, 2048):
+        if sweep and time.perf_counter() - t_start > budget_s * 0.6:
+            break
+        sweep.append(call(T))
+        log(f"cpu baseline T={T}: {sweep[-1]['full_s']:.3f} s (stride {sweep[-1]['stride']})")
+    best = min(sweep, key=lambda c: c["full_s"])
+
+    def median3(T, first):
+        runs = [first] + [call(T) for _ in range(2)]
+        return float(np.median([c["full_s"] for c in runs])), runs
+
+    best_s, best_runs = median3(best["T"], best)
+    t128 = next(c for c in sweep if c["T"] == 128)
+    t128_s = best_s if best["T"] == 128 else median3(128, t128)[0]
+    cores = best["threads"]
+    sampled = best["stride"] > 1
+    return {
+        "value": nnz_full / best_s, "unit": "output nnz/s", "cores": cores, "kind": "port",
+        "sample": (f"oracle restatement of fastcc_multiply, TileAccumulator (dense=1), median of 3 calls at the "
+                   f"best tile size T={best['T']} of a sweep 32..2048" +
+                   (f"; left tiles i % {best['stride']} == 0 sampled ({best['sampled_products']} of "
+                    f"{products_full} products) and extrapolated linearly in products" if sampled else
+                    "; whole contraction") +
+                   f"; {cores} OpenMP threads on {host_cpus()['model']}"),
+        "seconds": best_s,
+        "best_T": best["T"],
+        "value_T128": nnz_full / t128_s,
+        "seconds_T128": t128_s,
+        "extrapolated": sampled,
+        "sweep": [{"T": c["T"], "seconds": round(c["full_s"], 4), "stride": c["stride"]} for c in sweep],
+        "host": host_cpus(),
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", default="c2", choices=["c1", "c2", "c3", "c4", "c5"])
+    ap.add_argument("--workload", default="c4", choices=["c1", "c2", "c3", "c4", "c5"])
     ap.add_argument("--planner", default=None, choices=["exact", "learned"],
-                    help="default: exact (c1-c4), learned (c5)")
+                    help="default: learned (c3, c4, c5), exact (c1, c2)")
     ap.add_argument("--no-compare", action="store_true", help="skip timing the other planner")
     ap.add_argument("--pairs", type=int, default=256, help="c5: number of pairs")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-stride", type=int, default=0, help="left-tile sample stride for the CPU baseline")
-    ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--cpu-fraction", type=float, default=0.0,
+                    help="fraction of left rows the CPU baseline samples (0 = per workload)")
+    ap.add_argument("--cpu-threads", type=int, default=0, help="0 = the OpenMP default (all threads granted)")
     ap.add_argument("--tile-hint", type=int, default=0, help="rows per left tile (0 = planner)")
     ap.add_argument("--col-window", type=int, default=0, help="columns per dense unit (0 = planner)")
     ap.add_argument("--workgroups", type=int, default=0, help="persistent workgroups (0 = planner)")
     ap.add_argument("--force-acc", default="auto", choices=["auto", "dense", "lhash", "ghash"],
                     help="force one accumulator kind for every unit (profiling per kind)")
-    ap.add_argument("--autotune", action="store_true",
-                    help="run the autotuner on the workload before timing (untimed); the tuned shape is used")
+    ap.add_argument("--autotune", default=None, choices=["on", "off"],
+                    help="run the autotuner on the workload before timing (untimed) and use its entry; "
+                         "default on for c4")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a device-to-device copy of MIB MiB (HBM counter calibration) and exit")
     args = ap.parse_args()
@@ -80,10 +169,13 @@ def main():
     from fastcc_amd import workloads as W
 
     if args.planner is None:
-        args.planner = "learned" if args.workload == "c5" else "exact"
+        args.planner = "learned" if args.workload in ("c3", "c4", "c5") else "exact"
+    if args.autotune is None:
+        args.autotune = "on" if args.workload == "c4" else "off"
     if args.workload == "c5":
         return bench_batch(args)
     wl = W.FULL[args.workload]()
+    log(f"generating {wl.name}")
     ta, tb = wl.tensors()
     dev = torch.device("cuda:0")
     ca = torch.from_numpy(ta.coords.view(np.int32)).to(dev).contiguous()
@@ -104,8 +196,11 @@ def main():
     opts = dict(tile_hint=args.tile_hint, col_window=args.col_window, workgroups=args.workgroups, sink=sink,
                 force_acc=force)
     tuned = None
-    if args.autotune:
+    if args.autotune == "on":
+        t0 = time.perf_counter()
         tuned = ctx.autotune(da, list(wl.a_modes), db, list(wl.b_modes), reps=2, planner=args.planner, sink=sink)
+        tuned["seconds"] = time.perf_counter() - t0
+        log(f"autotuned: {tuned}")
 
     def run(planner):
         def step():
@@ -124,7 +219,9 @@ def main():
             nnz, st = step()
             stats.append(st)
         torch.cuda.synchronize()
-        return nnz, stats, (time.perf_counter() - t0) / max(args.steps, 1)
+        dt = (time.perf_counter() - t0) / max(args.steps, 1)
+        log(f"{planner}: {dt * 1e3:.3f} ms per step")
+        return nnz, stats, dt
 
     nnz, stats, dt = run(args.planner)
     compare = None
@@ -132,9 +229,13 @@ def main():
         other = "exact" if args.planner == "learned" else "learned"
         _, ostats, odt = run(other)
         ms = {args.planner: dt * 1e3, other: odt * 1e3}
+        lst = stats if args.planner == "learned" else ostats
+        est = ostats if args.planner == "learned" else stats
         compare = {"exact_ms_per_step": ms["exact"], "learned_ms_per_step": ms["learned"],
                    "learned_speedup": ms["exact"] / ms["learned"],
-                   "learned_mlp_mape": (stats if args.planner == "learned" else ostats)[-1]["mlp_mape"]}
+                   "learned_mlp_mape": lst[-1]["mlp_mape"],
+                   "exact_phases_ms": phases(est), "learned_phases_ms": phases(lst),
+                   "exact_reruns": int(est[-1]["reruns"]), "learned_reruns": int(lst[-1]["reruns"])}
 
     products = stats[-1]["products"]
     bytes_alg = stats[-1]["bytes_alg"]
@@ -142,35 +243,20 @@ def main():
     dev_ms = float(np.mean([s["ms_total"] for s in stats]))
     roof = numeric_roofline(stats[-1], k_ms)
     roof_dense = dense_roofline(stats)
-    if args.planner == "exact" and args.force_acc == "auto" and not args.tile_hint and not args.col_window:
+    if args.force_acc == "auto" and not args.tile_hint and not args.col_window:
         for r, kern in ((roof, "k_numeric"), (roof_dense, "k_dgemm_tn")):
             if r is not None:
-                r["traffic"], src = measured_traffic(wl.name, kern)
+                r["traffic"], src = measured_traffic(wl.name, kern, args.planner)
                 if src:
                     r["traffic_source"] = src
 
     cpu = None
     if not args.no_cpu_baseline:
         from oracle import oracle
-        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
-        stride = args.cpu_stride or {"c1": 1, "c2": 1, "c3": 4096, "c4": 32}[args.workload]
-        r = oracle.contract_tiled(ta, wl.a_modes, tb, wl.b_modes, tile_size=128, threads=threads,
-                                  high_precision=False, sample_stride=stride, sink=args.workload in ("c3", "c4"),
-                                  sort=False)
-        if stride > 1:
-            # the index build is whole in a sampled run; only the tile loop is
-            # sampled, so it alone is scaled up to the full contraction
-            t_full = r.index_seconds + (r.seconds - r.index_seconds) * stride
-            value = nnz / t_full
-            cpu_note = (f"extrapolated: index {r.index_seconds:.2f} s + tile loop "
-                        f"{r.seconds - r.index_seconds:.2f} s x {stride} = {t_full:.1f} s for {nnz} outputs")
-        else:
-            value = r.nnz / r.seconds
-            cpu_note = "full contraction"
-        cpu = {"value": value, "unit": "output nnz/s", "cores": r.threads, "kind": "port",
-               "sample": f"left tiles t % {stride} == 0 at T=128 ({r.nnz} outputs, {r.products} products, "
-                         f"{r.seconds:.2f} s; {cpu_note}); CPU {cpu_model()}",
-               "seconds": r.seconds}
+        fraction = args.cpu_fraction or {"c1": 1.0, "c2": 1.0, "c3": 1.0 / 4096, "c4": 1.0 / 32}[args.workload]
+        log("cpu baseline (oracle restatement of the reference algorithm)")
+        cpu = cpu_baseline_protocol(oracle, ta, tb, wl, nnz, products, args.workload in ("c3", "c4"), fraction,
+                                    args.cpu_threads)
 
     line = {
         "metric": "output nnz/s (exact-symbolic planning)" if args.planner == "exact" else
@@ -192,17 +278,24 @@ def main():
                    "col_window": stats[-1]["col_window"], "autotuned": tuned, "force_acc": args.force_acc,
                    "dense_keys": stats[-1]["dense_keys"], "dense_key_products": int(stats[-1]["dense_key_products"])},
         "hbm_gbs_alg": bytes_alg / dt / 1e9,
+        "hbm_frac_alg": bytes_alg / dt / 1e9 / HBM_PEAK_GBS,
         "device_ms": dev_ms,
-        "phases_ms": {k: float(np.mean([s[k] for s in stats])) for k in
-                      ("ms_index", "ms_plan", "ms_symbolic", "ms_numeric", "ms_overflow", "ms_dense",
-                       "ms_dense_gemm")},
+        "phases_ms": phases(stats),
         "planner_comparison": compare,
         "roofline": roof,
         "roofline_dense": roof_dense,
         "cpu_baseline": cpu,
     }
+    if cpu:
+        line["gpu_over_cpu"] = line["value"] / cpu["value"]
     print(json.dumps(line), flush=True)
     ctx.close()
+
+
+def phases(stats):
+    import numpy as np
+    return {k: float(np.mean([s[k] for s in stats])) for k in
+            ("ms_index", "ms_plan", "ms_symbolic", "ms_numeric", "ms_overflow", "ms_dense", "ms_dense_gemm")}
 
 
 def roofline(products, bytes_alg, k_ms, kernel="k_numeric", note=""):
@@ -220,13 +313,16 @@ def roofline(products, bytes_alg, k_ms, kernel="k_numeric", note=""):
             "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": kernel, "note": note}
 
 
-def measured_traffic(workload, kernel):
+def measured_traffic(workload, kernel, planner):
     """HBM bytes per launch (FETCH_SIZE + WRITE_SIZE, copy-calibrated) of a
-    kernel on a workload, from the committed rocprofv3 summaries
-    (profiles/hbm_traffic.json, written with each profiles/*.md summary)."""
+    kernel on a workload under a planner, from the committed rocprofv3
+    summaries (profiles/hbm_traffic.json, written with each profiles/*.md
+    summary)."""
     p = os.path.join(ROOT, "profiles", "hbm_traffic.json")
     try:
         t = json.load(open(p))[workload][kernel]
+        if t.get("planner", "exact") != planner:
+            return None, None
     except (OSError, KeyError, ValueError):
         return None, None
     return t["fetch_bytes"] + t["write_bytes"], "profiles/" + t["source"]
@@ -317,8 +413,8 @@ def bench_batch(args):
     cpu = None
     if not args.no_cpu_baseline:
         from oracle import oracle
-        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
-        stride = args.cpu_stride or 8
+        threads = args.cpu_threads
+        stride = 8
         n_s, sec, prod = 0, 0.0, 0
         for i in range(0, len(host), stride):
             ta, tb = host[i]
@@ -327,10 +423,10 @@ def bench_batch(args):
             n_s += r.nnz
             sec += r.seconds
             prod += r.products
-        cpu = {"value": n_s / sec, "unit": "output nnz/s", "cores": threads, "kind": "port",
+        cpu = {"value": n_s / sec, "unit": "output nnz/s", "cores": r.threads, "kind": "port",
                "sample": f"every {stride}th pair, each contracted whole at T=128 with count + checksum outputs "
                          f"({n_s} outputs, {prod} products, {sec:.2f} s); CPU {cpu_model()}",
-               "seconds": sec}
+               "seconds": sec, "host": host_cpus()}
 
     line = {
         "metric": f"aggregate output nnz/s ({args.planner} planning, {len(pairs)} pairs in one fused launch)",
