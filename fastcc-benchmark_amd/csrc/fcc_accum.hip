@@ -76,14 +76,17 @@ __device__ __forceinline__ bool gtable_full(const AccArgs &A, int probe) {
 // New keys are counted once per wavefront (ballot after the probe loop, where
 // the lanes that entered reconverge): one global atomic per wavefront instead
 // of one per key on a single counter. Every lane of a wavefront works on the
-// same unit.
-__device__ __forceinline__ void count_new_keys(const AccArgs &A, bool inserted, unsigned long long *unit_counter) {
+// same unit. The unit's count takes a key twice when it stands for two
+// outputs (mirror mode, off the diagonal).
+__device__ __forceinline__ void count_new_keys(const AccArgs &A, bool inserted, bool twice,
+                                               unsigned long long *unit_counter) {
     const uint64_t bal = __ballot(inserted);
+    const uint64_t bal2 = __ballot(inserted && twice);
     const uint32_t lane = threadIdx.x & 63;
     if (bal && lane == (uint32_t)(__ffsll((unsigned long long)bal) - 1)) {
         const unsigned long long n = (unsigned long long)__popcll(bal);
         atomicAdd(A.gcount, n);
-        if (unit_counter) atomicAdd(unit_counter, n);
+        if (unit_counter) atomicAdd(unit_counter, n + (unsigned long long)__popcll(bal2));
     }
 }
 
@@ -106,13 +109,14 @@ __device__ bool ghash_add(const AccArgs &A, uint64_t key, double v, uint32_t uni
         }
         h = (h + 1) & A.gmask;
     }
-    count_new_keys(A, inserted, nullptr);
+    count_new_keys(A, inserted, false, nullptr);
     if (!done) atomicOr(A.gflag, 1u);
     return done;
 }
 
-// count pass: set semantics, a new key adds one to its unit's count
-__device__ void gset_insert(const AccArgs &A, uint64_t key, uint32_t unit) {
+// count pass: set semantics, a new key adds one to its unit's count (two when
+// `twice`: a mirrored cell off the diagonal)
+__device__ void gset_insert(const AccArgs &A, uint64_t key, uint32_t unit, bool twice) {
     uint64_t h = mix64(key) & A.gmask;
     bool done = false, inserted = false;
     for (int probe = 0; probe < 8192; probe++) {
@@ -128,7 +132,7 @@ __device__ void gset_insert(const AccArgs &A, uint64_t key, uint32_t unit) {
         }
         h = (h + 1) & A.gmask;
     }
-    count_new_keys(A, inserted, &A.unit_count[unit]);
+    count_new_keys(A, inserted, twice, &A.unit_count[unit]);
     if (!done) atomicOr(A.gflag, 1u);
 }
 
@@ -172,14 +176,22 @@ static_assert(LH_SLOTS >= LH_MAX + ACC_THREADS && SYM_SLOTS >= SYM_MAX + ACC_THR
 
 // Every active lane calls this together (divergent code is fine: the ballot
 // covers the lanes that are here).
+// dcnt (optional, mirror-mode count pass): claimed keys on the diagonal
+// (`diag`) are also counted there.
 template <int BITS>
-__device__ __forceinline__ int lds_probe_counted(uint32_t *keys, uint32_t *cnt, uint32_t limit, uint32_t key) {
+__device__ __forceinline__ int lds_probe_counted(uint32_t *keys, uint32_t *cnt, uint32_t limit, uint32_t key,
+                                                 uint32_t *dcnt = nullptr, bool diag = false) {
     bool claimed = false;
     const int slot = lds_probe<BITS>(keys, cnt, limit, key, claimed);
     const uint64_t bal = __ballot(claimed);
     if (bal) {
         const uint32_t lane = threadIdx.x & 63;
-        if (lane == (uint32_t)(__ffsll((unsigned long long)bal) - 1)) atomicAdd(cnt, (uint32_t)__popcll(bal));
+        const bool first = lane == (uint32_t)(__ffsll((unsigned long long)bal) - 1);
+        if (first) atomicAdd(cnt, (uint32_t)__popcll(bal));
+        if (dcnt) {
+            const uint64_t bd = __ballot(claimed && diag);
+            if (bd && first) atomicAdd(dcnt, (uint32_t)__popcll(bd));
+        }
     }
     return slot;
 }
@@ -258,6 +270,7 @@ struct Acc {
     uint8_t *seen;    // DENSE count pass: one byte per slot, set to 1 by plain
                       // stores (idempotent: no atomics, no per-word merging)
     uint32_t *cnt;
+    uint32_t *dcnt;   // mirror-mode count pass: LDS-hash keys on the diagonal (else null)
     SinkAcc *sk;      // DIRECT units: count + checksum outputs go here
     bool sink;
 };
@@ -265,21 +278,30 @@ struct Acc {
 // A distinct output written straight out: count + checksum, or a position
 // reserved per wavefront from the unit's LDS counter (order within the unit
 // is free).
+// In mirror mode the caller passes cells with col >= row only; off the
+// diagonal the transposed cell is written too (after the wavefront's direct
+// cells, so each store instruction writes consecutive positions).
 template <bool NUM, bool BITMAP>
 __device__ __forceinline__ void direct_emit(const AccArgs &A, const UnitCtx &U, const Acc<NUM, BITMAP> &S,
                                             uint64_t row, uint64_t col, double v) {
+    const bool twice = A.mirror && col > row;
     if (S.sink) {
         emit_output(A, 0, row, col, v, true, *S.sk);
+        if (twice) emit_output(A, 0, col, row, v, true, *S.sk);
         return;
     }
     const uint32_t lane = threadIdx.x & 63;
     const uint64_t act = __ballot(1);
+    const uint64_t act2 = __ballot(twice);
     const int first = __ffsll((unsigned long long)act) - 1;
     uint32_t b = 0;
-    if (lane == (uint32_t)first) b = atomicAdd(S.cnt, (uint32_t)__popcll(act));
+    if (lane == (uint32_t)first) b = atomicAdd(S.cnt, (uint32_t)(__popcll(act) + __popcll(act2)));
     b = __shfl(b, first, 64);
     const uint64_t lt = lane ? (~0ull >> (64 - lane)) : 0ull;
     emit_output(A, U.obase + b + (uint32_t)__popcll(act & lt), row, col, v, false, *S.sk);
+    if (twice)
+        emit_output(A, U.obase + b + (uint32_t)__popcll(act) + (uint32_t)__popcll(act2 & lt), col, row, v, false,
+                    *S.sk);
 }
 
 template <bool NUM, bool BITMAP>
@@ -290,23 +312,29 @@ __device__ __forceinline__ void accum(const AccArgs &A, const UnitCtx &U, const 
         if (!NUM) S.seen[slot] = 1;
         if (NUM && BITMAP) atomicOr(&S.bits[slot >> 5], 1u << (slot & 31));
         if (NUM) lds_add_f64(&S.fv[slot], v);
-    } else if (U.kind == KIND_LHASH) {
+        return;
+    }
+    // hash kinds: in mirror mode cells below the diagonal are not kept (a
+    // dense slot exists anyway and is dropped when the tile is written out)
+    const uint32_t w = A.mirror ? mirror_weight(U.row0 + li, (uint64_t)U.c0 + col) : 1u;
+    if (w == 0) return;
+    if (U.kind == KIND_LHASH) {
         uint32_t key = li * U.W + col;
         int slot = NUM ? lds_probe_counted<12>(S.keys, S.cnt, LH_MAX, key)
-                       : lds_probe_counted<14>(S.keys, S.cnt, SYM_MAX, key);
+                       : lds_probe_counted<14>(S.keys, S.cnt, SYM_MAX, key, S.dcnt, w == 1);
         if (slot >= 0) {
             if (NUM) lds_add_f64(&S.fv[slot], v);
         } else {
             uint64_t gk = (U.row0 + li) * A.spanB + (U.c0 + col);
             if (NUM) ghash_add(A, gk, v, U.u);
-            else gset_insert(A, gk, U.u);
+            else gset_insert(A, gk, U.u, w == 2);
         }
     } else if (U.kind == KIND_DIRECT) {
         if (NUM) direct_emit<NUM, BITMAP>(A, U, S, U.row0 + li, (uint64_t)U.c0 + col, v);
     } else {
         uint64_t gk = (U.row0 + li) * A.spanB + (U.c0 + col);
         if (NUM) ghash_add(A, gk, v, U.u);
-        else gset_insert(A, gk, U.u);
+        else gset_insert(A, gk, U.u, w == 2);
     }
 }
 
@@ -335,7 +363,7 @@ __device__ __forceinline__ uint32_t lane_lower_bound(const uint32_t *arr, uint32
 // is updated with plain LDS read-modify-write instead of atomics. Within a
 // wavefront the 64 lanes hold distinct columns of one row at a time.
 template <bool NUM, bool BITMAP>
-__device__ void dense_heavy_owned(const AccArgs &A, const UnitCtx &U, const Acc<NUM, BITMAP> &S) {
+__device__ __forceinline__ void dense_heavy_owned(const AccArgs &A, const UnitCtx &U, const Acc<NUM, BITMAP> &S) {
     const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const uint32_t g0 = A.atile_g0[U.a], gl = A.atile_lg[U.a];
     const uint32_t cend = U.c0 + U.W;
@@ -634,7 +662,7 @@ __device__ __forceinline__ void heavy_ring(const AccArgs &A, const UnitCtx &U, c
 // whose other paths already need many registers) they keep HRING_HASH, so
 // the kernel stays within 128 VGPRs and two workgroups fit per CU.
 template <bool NUM, bool BITMAP, bool DEEP = true>
-__device__ void heavy_claimed(const AccArgs &A, const UnitCtx &U, const Acc<NUM, BITMAP> &S, uint32_t *s_gnext) {
+__device__ __forceinline__ void heavy_claimed(const AccArgs &A, const UnitCtx &U, const Acc<NUM, BITMAP> &S, uint32_t *s_gnext) {
     const uint32_t lane = threadIdx.x & 63;
     // A tile with few heavy groups works on fiber segments (long fibers spread
     // over the block's wavefronts, claims sized so every wavefront gets some);
@@ -674,6 +702,9 @@ __device__ void heavy_claimed(const AccArgs &A, const UnitCtx &U, const Acc<NUM,
                 e_l = lane_lower_bound(A.Bcol, s_l, e_l, cend);
             }
         }
+        // mirror mode: no cell left of the tile's first row is kept
+        if (A.mirror && U.row0 > U.c0 && lane < nb && s_l < e_l)
+            s_l = lane_lower_bound(A.Bcol, s_l, e_l, (uint32_t)U.row0);
         if (__ballot(lane < nb && d.w > 64) != 0) {
             // rows beyond one wavefront: group by group
             for (uint32_t j = 0; j < nb; j++) {
@@ -724,7 +755,7 @@ __device__ __forceinline__ uint32_t gallop_upper(const uint32_t *inc, uint32_t f
 // product finds its entry by binary search in LDS. Without it each wavefront
 // flattens its own 64 entries in registers (owner found by shuffles).
 template <bool NUM, bool BITMAP>
-__device__ void light_entries(const AccArgs &A, const UnitCtx &U, const Acc<NUM, BITMAP> &S, uint32_t le0,
+__device__ __forceinline__ void light_entries(const AccArgs &A, const UnitCtx &U, const Acc<NUM, BITMAP> &S, uint32_t le0,
                               uint32_t le1, uint32_t *stage, uint32_t *s_w) {
     const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const uint32_t cend = U.c0 + U.W;
@@ -748,6 +779,8 @@ __device__ void light_entries(const AccArgs &A, const UnitCtx &U, const Acc<NUM,
                 b = lane_lower_bound(A.Bcol, a, b, cend);
             }
         }
+        // mirror mode: no cell left of the tile's first row is kept
+        if (A.mirror && U.row0 > U.c0) a = lane_lower_bound(A.Bcol, a, b, (uint32_t)U.row0);
         s0 = a;
         len = b - a;
     };
@@ -859,7 +892,7 @@ static_assert(LH_SLOTS + LH_SLOTS / 2 + ACC_THREADS <= DENSE_SLOTS, "LHASH flatt
 // every product a distinct output. Products are flattened over the block like
 // light entries (scan of fiber lengths staged in LDS, owner by binary search).
 template <bool BITMAP>
-__device__ void direct_run(const AccArgs &A, const UnitCtx &U, const Acc<true, BITMAP> &S, uint32_t e0, uint32_t e1,
+__device__ __forceinline__ void direct_run(const AccArgs &A, const UnitCtx &U, const Acc<true, BITMAP> &S, uint32_t e0, uint32_t e1,
                            uint32_t *stage, uint32_t *s_w) {
     const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     uint32_t *s_inc = stage, *s_s0 = stage + ACC_THREADS;
@@ -869,7 +902,10 @@ __device__ void direct_run(const AccArgs &A, const UnitCtx &U, const Acc<true, B
         if (e < e1) {
             const uint2 fp = A.Afp[e];
             s0 = fp.x;
-            len = fp.y - fp.x;
+            // mirror mode: only the entry's columns at or right of its row
+            // (its row is distinct within the tile, so this is exact)
+            if (A.mirror) s0 = lane_lower_bound(A.Bcol, fp.x, fp.y, A.Arow[e]);
+            len = fp.y - s0;
         }
         uint32_t inc = len;
         for (int d = 1; d < 64; d <<= 1) {
@@ -961,7 +997,7 @@ __device__ __forceinline__ uint32_t dk_dense_word(const AccArgs &A, const UnitCt
 // Dense unit accumulator start: block values where present, else the empty
 // marker (+0 with a bitmap, -0 without; a present -0 is stored as +0).
 template <bool BITMAP>
-__device__ void dk_seed_dense(const AccArgs &A, const UnitCtx &U, double *fv, uint32_t *bits, uint32_t nslots) {
+__device__ __forceinline__ void dk_seed_dense(const AccArgs &A, const UnitCtx &U, double *fv, uint32_t *bits, uint32_t nslots) {
     const double init = BITMAP ? 0.0 : -0.0;
     // the presence word and the value are loaded together (every cell of the
     // unit's rows x window lies inside the padded block), several cells per
@@ -985,7 +1021,7 @@ __device__ void dk_seed_dense(const AccArgs &A, const UnitCtx &U, double *fv, ui
 // Hash unit start: every present cell of the unit's rows x window goes in
 // like a product (count pass: the key; numeric pass: the block value).
 template <bool NUM, bool BITMAP>
-__device__ void dk_seed_cells(const AccArgs &A, const UnitCtx &U, const Acc<NUM, BITMAP> &S) {
+__device__ __forceinline__ void dk_seed_cells(const AccArgs &A, const UnitCtx &U, const Acc<NUM, BITMAP> &S) {
     const uint32_t wpr = (U.W + 31) / 32;
     const uint32_t n = (1u << A.logTL) * wpr;
     for (uint32_t i = threadIdx.x; i < n; i += ACC_THREADS) {
@@ -997,26 +1033,28 @@ __device__ void dk_seed_cells(const AccArgs &A, const UnitCtx &U, const Acc<NUM,
         while (w) {
             const uint32_t cl = cl0 + (uint32_t)__builtin_ctz(w);
             w &= w - 1;
+            const uint32_t mw = A.mirror ? mirror_weight(row, (uint64_t)U.c0 + cl) : 1u;
+            if (mw == 0) continue;  // mirror mode: below the diagonal
             const uint64_t gk = row * A.spanB + (U.c0 + cl);
             if (U.kind == KIND_LHASH) {
                 const uint32_t key = li * U.W + cl;
                 const int slot =
                     NUM ? lds_probe_counted<12>(S.keys, S.cnt, LH_MAX, key)
-                        : lds_probe_counted<14>(S.keys, S.cnt, SYM_MAX, key);
+                        : lds_probe_counted<14>(S.keys, S.cnt, SYM_MAX, key, S.dcnt, mw == 1);
                 if (slot >= 0) {
                     if (NUM) lds_add_f64(&S.fv[slot], dk_value(A, row, U.c0 + cl));
                     continue;
                 }
             }
             if (NUM) ghash_add(A, gk, dk_value(A, row, U.c0 + cl), U.u);
-            else gset_insert(A, gk, U.u);
+            else gset_insert(A, gk, U.u, mw == 2);
         }
     }
 }
 
 // One unit: heavy groups (a wavefront per group), then the light entries.
 template <bool NUM, bool BITMAP, bool DEEP = true>
-__device__ void process_unit(const AccArgs &A, const UnitCtx &U, const Acc<NUM, BITMAP> &S, uint32_t *s_gnext,
+__device__ __forceinline__ void process_unit(const AccArgs &A, const UnitCtx &U, const Acc<NUM, BITMAP> &S, uint32_t *s_gnext,
                              uint32_t *stage, uint32_t *s_w) {
     const uint32_t gl = A.atile_lg[U.a], g1 = A.atile_g0[U.a + 1];
     if (U.kind != KIND_DENSE && A.dk_tile && A.dk_tile[U.a]) dk_seed_cells<NUM, BITMAP>(A, U, S);
@@ -1071,7 +1109,7 @@ __device__ uint32_t block_count(uint32_t n, uint32_t *s_w, Pred pred) {
     return tot;
 }
 
-__device__ void sink_flush(const AccArgs &A, SinkAcc &sk, unsigned long long *s_u, double *s_d) {
+__device__ __forceinline__ void sink_flush(const AccArgs &A, SinkAcc &sk, unsigned long long *s_u, double *s_d) {
     const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     for (int d = 32; d > 0; d >>= 1) {
         sk.count += __shfl_xor(sk.count, d, 64);
@@ -1115,6 +1153,7 @@ __global__ __launch_bounds__(ACC_THREADS, 4) void k_numeric(AccArgs A) {
     S.bits = s_bits;
     S.seen = nullptr;
     S.cnt = &s_cnt;
+    S.dcnt = nullptr;
     S.sk = &sk;
     S.sink = SINK;
     for (;;) {
@@ -1157,6 +1196,10 @@ __global__ __launch_bounds__(ACC_THREADS, 4) void k_numeric(AccArgs A) {
         if (threadIdx.x == 0) {
             s_cnt = 0;
             s_gnext = 0;
+            if (U.kind == KIND_DIRECT && SINK) {
+                const uint32_t t1 = U.u + 1 < A.nunits ? A.unit_tile[U.u + 1] : A.natiles;
+                A.unit_cursor[U.u] = A.st_pprefix[t1] - A.st_pprefix[U.a];  // the unit's outputs
+            }
             if (U.kind == KIND_DIRECT && !SINK) {
                 // the unit's outputs are exactly the products of its run of tiles
                 const uint32_t t1 = U.u + 1 < A.nunits ? A.unit_tile[U.u + 1] : A.natiles;
@@ -1201,27 +1244,56 @@ __global__ __launch_bounds__(ACC_THREADS, 4) void k_numeric(AccArgs A) {
                 }
                 return S.keys[s] < LSEALED;
             };
+            auto cell = [&](uint32_t s, uint64_t &r, uint64_t &c) {
+                const uint32_t key = U.kind == KIND_DENSE ? s : S.keys[s];
+                const uint32_t li = key / U.Wp, cl = key - li * U.Wp;
+                r = U.row0 + li;
+                c = (uint64_t)U.c0 + cl;
+            };
+            // outputs slot s stands for: 0 untouched; in mirror mode 0 below
+            // the diagonal, 1 on it, 2 (the cell and its transpose) above
+            auto weight = [&](uint32_t s) -> uint32_t {
+                if (!pred(s)) return 0u;
+                if (!A.mirror) return 1u;
+                uint64_t r, c;
+                cell(s, r, c);
+                return mirror_weight(r, c);
+            };
             // Each wavefront owns a contiguous run of slots (whole 64-slot
             // chunks), so the unit's outputs keep slot order: count per
             // wavefront, one barrier for the offsets, then emit without
             // further barriers. Count + checksum outputs need no offsets.
+            // A 64-slot chunk writes its cells, then their transposes.
             const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
             const uint32_t per = ((nslots + NWAVES * 64 - 1) / (NWAVES * 64)) * 64;
             const uint32_t w0 = min(nslots, wave * per), w1 = min(nslots, w0 + per);
-            auto emit = [&](uint32_t s, uint64_t pos) {
-                const uint32_t key = U.kind == KIND_DENSE ? s : S.keys[s];
-                const uint32_t li = key / U.Wp, cl = key - li * U.Wp;
-                emit_output(A, pos, U.row0 + li, (uint64_t)U.c0 + cl, s_f64[s], SINK, sk);
-            };
             if (SINK) {
-                for (uint32_t s = w0 + lane; s < w1; s += 64)
-                    if (pred(s)) emit(s, 0);
+                uint32_t n = 0;
+                for (uint32_t s = w0 + lane; s < w1; s += 64) {
+                    const uint32_t w = weight(s);
+                    if (!w) continue;
+                    n += w;
+                    uint64_t r, c;
+                    cell(s, r, c);
+                    emit_output(A, 0, r, c, s_f64[s], true, sk);
+                    if (w == 2) emit_output(A, 0, c, r, s_f64[s], true, sk);
+                }
+                // the unit's outputs (plan queries, planner error)
+                for (int d = 32; d > 0; d >>= 1) n += __shfl_xor(n, d, 64);
+                if (lane == 0) s_w[wave] = n;
+                __syncthreads();
+                if (threadIdx.x == 0) {
+                    uint32_t tot = 0;
+#pragma unroll
+                    for (int w = 0; w < NWAVES; w++) tot += s_w[w];
+                    A.unit_cursor[U.u] = tot;
+                }
             } else {
                 const uint64_t lt = lane ? (~0ull >> (64 - lane)) : 0ull;
                 uint32_t cnt = 0;
                 for (uint32_t b = w0; b < w1; b += 64) {
-                    const bool t = b + lane < w1 && pred(b + lane);
-                    cnt += (uint32_t)__popcll(__ballot(t));
+                    const uint32_t w = b + lane < w1 ? weight(b + lane) : 0u;
+                    cnt += (uint32_t)__popcll(__ballot(w != 0)) + (uint32_t)__popcll(__ballot(w == 2));
                 }
                 if (lane == 0) s_w[wave] = cnt;
                 __syncthreads();
@@ -1251,15 +1323,23 @@ __global__ __launch_bounds__(ACC_THREADS, 4) void k_numeric(AccArgs A) {
                 if (base != ~0ull) {
                     uint32_t run = pre;
                     for (uint32_t b = w0; b < w1; b += 64) {
-                        const bool t = b + lane < w1 && pred(b + lane);
-                        const uint64_t bal = __ballot(t);
-                        if (t) emit(b + lane, base + run + (uint32_t)__popcll(bal & lt));
-                        run += (uint32_t)__popcll(bal);
+                        const uint32_t w = b + lane < w1 ? weight(b + lane) : 0u;
+                        const uint64_t bal = __ballot(w != 0), bal2 = __ballot(w == 2);
+                        const uint32_t n1 = (uint32_t)__popcll(bal);
+                        if (w) {
+                            uint64_t r, c;
+                            cell(b + lane, r, c);
+                            const double v = s_f64[b + lane];
+                            emit_output(A, base + run + (uint32_t)__popcll(bal & lt), r, c, v, false, sk);
+                            if (w == 2)
+                                emit_output(A, base + run + n1 + (uint32_t)__popcll(bal2 & lt), c, r, v, false, sk);
+                        }
+                        run += n1 + (uint32_t)__popcll(bal2);
                     }
                 }
             }
-        } else if (U.kind == KIND_GHASH && !SINK && threadIdx.x == 0) {
-            A.unit_cursor[U.u] = 0;
+        } else if (U.kind == KIND_GHASH && threadIdx.x == 0) {
+            A.unit_cursor[U.u] = 0;  // the global-hash drain counts the unit's outputs
         }
         if (A.group_left && threadIdx.x == 0) {
             const uint64_t g = U.row0 / A.group_rows;
@@ -1280,13 +1360,14 @@ static_assert(DENSE_SLOTS <= 4 * SYM_SLOTS, "the count pass's dense byte map mus
 __global__ __launch_bounds__(ACC_THREADS) void k_count(AccArgs A) {
     extern __shared__ __align__(16) uint32_t s_set[];
     __shared__ uint32_t s_stage[2 * ACC_THREADS];
-    __shared__ uint32_t s_cnt, s_gnext, s_unit, s_w[NWAVES];
+    __shared__ uint32_t s_cnt, s_dcnt, s_gnext, s_unit, s_w[NWAVES];
     Acc<false, false> S;
     S.fv = nullptr;
     S.keys = s_set;
     S.bits = nullptr;
     S.seen = reinterpret_cast<uint8_t *>(s_set);
     S.cnt = &s_cnt;
+    S.dcnt = A.mirror ? &s_dcnt : nullptr;
     S.sk = nullptr;
     S.sink = false;
     for (;;) {
@@ -1346,7 +1427,18 @@ __global__ __launch_bounds__(ACC_THREADS) void k_count(AccArgs A) {
                 const uint64_t rows_in = min((uint64_t)TL, A.spanA - U.row0);
                 if ((uint64_t)tot == rows_in * U.W) {
                     if (threadIdx.x == 0) {
-                        atomicAdd(&A.unit_count[U.u], (unsigned long long)tot);
+                        unsigned long long n = tot;
+                        if (A.mirror) {
+                            // every cell present: per row, the cells at or right
+                            // of the diagonal, twice off it
+                            n = 0;
+                            const uint64_t ce = (uint64_t)U.c0 + U.W;
+                            for (uint64_t r = U.row0; r < U.row0 + rows_in; r++) {
+                                const uint64_t lo = r > U.c0 ? r : U.c0;
+                                if (lo < ce) n += 2 * (ce - lo) - (r >= U.c0 ? 1 : 0);
+                            }
+                        }
+                        atomicAdd(&A.unit_count[U.u], n);
                         atomicAdd(A.block_counted, 1ull);
                     }
                     continue;  // every thread has read s_unit (barriers above)
@@ -1359,6 +1451,7 @@ __global__ __launch_bounds__(ACC_THREADS) void k_count(AccArgs A) {
         }
         if (threadIdx.x == 0) {
             s_cnt = 0;
+            s_dcnt = 0;
             s_gnext = 0;
         }
         __syncthreads();
@@ -1367,14 +1460,30 @@ __global__ __launch_bounds__(ACC_THREADS) void k_count(AccArgs A) {
         uint32_t c;
         if (U.kind == KIND_DENSE) {
             uint32_t part = 0;
-            for (uint32_t s = threadIdx.x; s < nwords; s += ACC_THREADS) part += __popc(s_set[s]);
+            if (!A.mirror) {
+                for (uint32_t s = threadIdx.x; s < nwords; s += ACC_THREADS) part += __popc(s_set[s]);
+            } else {
+                // the four slots of a word share a row (rows hold a multiple of
+                // 32 slots); each touched slot counts its mirror weight
+                for (uint32_t s = threadIdx.x; s < nwords; s += ACC_THREADS) {
+                    const uint32_t wd = s_set[s];
+                    if (!wd) continue;
+                    const uint32_t li = (4 * s) / U.Wp, cl = 4 * s - li * U.Wp;
+                    const uint64_t r = U.row0 + li, c0 = (uint64_t)U.c0 + cl;
+#pragma unroll
+                    for (int j = 0; j < 4; j++)
+                        if ((wd >> (8 * j)) & 1u) part += mirror_weight(r, c0 + j);
+                }
+            }
             for (int d = 32; d > 0; d >>= 1) part += __shfl_xor(part, d, 64);
             if ((threadIdx.x & 63) == 0) s_w[threadIdx.x >> 6] = part;
             __syncthreads();
             c = 0;
             for (int w = 0; w < NWAVES; w++) c += s_w[w];
         } else {
-            c = s_cnt;
+            // LDS-hash keys (mirror mode: all at or right of the diagonal,
+            // twice except the diagonal ones)
+            c = A.mirror ? 2 * s_cnt - s_dcnt : s_cnt;
         }
         if (threadIdx.x == 0 && c) atomicAdd(&A.unit_count[U.u], (unsigned long long)c);
         __syncthreads();
@@ -1398,6 +1507,9 @@ __global__ __launch_bounds__(256) void k_gdrain(AccArgs A) {
         bool t = key != GEMPTY;
         uint64_t row = t ? key / A.spanB : 0, col = t ? key - row * A.spanB : 0;
         double v = t ? A.gvals[h] : 0.0;
+        // mirror mode: every key is at or right of the diagonal; off it the
+        // transposed cell is written too
+        const bool twice = t && A.mirror && col > row;
         if (SINK) {
             if (A.group_rows) {
                 // per-group summaries: one set of atomics per wave when the
@@ -1421,20 +1533,24 @@ __global__ __launch_bounds__(256) void k_gdrain(AccArgs A) {
                 } else if (t) {
                     sink_atomic(A, g, 1ull, z, v, v * hash_weight(z));
                 }
+                if (t) atomicAdd(&A.unit_cursor[A.gunit[h]], 1ull);  // per-unit outputs
                 continue;
             }
             if (t) emit_output(A, 0, row, col, v, true, sk);
+            if (twice) emit_output(A, 0, col, row, v, true, sk);
+            if (t) atomicAdd(&A.unit_cursor[A.gunit[h]], twice ? 2ull : 1ull);  // per-unit outputs
             continue;
         }
         if (MODE == 0) {
             if (t) {
                 uint32_t u = A.gunit[h];
-                unsigned long long r = atomicAdd(&A.unit_cursor[u], 1ull);
+                unsigned long long r = atomicAdd(&A.unit_cursor[u], twice ? 2ull : 1ull);
                 emit_output(A, A.unit_off[u] + r, row, col, v, false, sk);
+                if (twice) emit_output(A, A.unit_off[u] + r + 1, col, row, v, false, sk);
             }
         } else {
-            uint64_t bal = __ballot(t);
-            if (lane == 0) s_w[wave] = (uint32_t)__popcll(bal);
+            const uint64_t bal = __ballot(t), bal2 = __ballot(twice);
+            if (lane == 0) s_w[wave] = (uint32_t)(__popcll(bal) + __popcll(bal2));
             __syncthreads();
             if (threadIdx.x == 0) {
                 uint32_t tot = s_w[0] + s_w[1] + s_w[2] + s_w[3];
@@ -1448,8 +1564,14 @@ __global__ __launch_bounds__(256) void k_gdrain(AccArgs A) {
             __syncthreads();
             uint32_t woff = 0;
             for (uint32_t w = 0; w < wave; w++) woff += s_w[w];
-            if (t && s_base != ~0ull) emit_output(A, s_base + woff + (uint32_t)__popcll(bal & lt), row, col, v, false, sk);
-            if (t) atomicAdd(&A.unit_cursor[A.gunit[h]], 1ull);  // per-unit actual count (planner error)
+            if (t && s_base != ~0ull) {
+                emit_output(A, s_base + woff + (uint32_t)__popcll(bal & lt), row, col, v, false, sk);
+                if (twice)
+                    emit_output(A, s_base + woff + (uint32_t)__popcll(bal) + (uint32_t)__popcll(bal2 & lt), col, row,
+                                v, false, sk);
+            }
+            // per-unit actual count (planner error)
+            if (t) atomicAdd(&A.unit_cursor[A.gunit[h]], twice ? 2ull : 1ull);
             __syncthreads();
         }
     }
@@ -1468,11 +1590,14 @@ __global__ __launch_bounds__(256) void k_gdrain(AccArgs A) {
 
 // Per active tile: accumulator kind and number of column windows.
 // est = exact planner: the products bound; learned planner: predicted outputs.
+// In mirror mode the windows wholly left of the tile's first row hold no kept
+// cell and are skipped: t_w0 is the first window kept.
 __global__ void k_plan_tiles(const uint64_t *st_products, const uint64_t *st_maxgp, const uint8_t *st_direct,
                              const float *pred, uint32_t natiles, PlanParams P, uint32_t *t_nwin, uint32_t *t_W,
-                             uint8_t *t_kind) {
+                             uint8_t *t_kind, uint32_t *t_w0) {
     uint32_t a = blockIdx.x * blockDim.x + threadIdx.x;
     if (a >= natiles) return;
+    t_w0[a] = 0;
     // distinct outputs: one window, no accumulator (the unit's LDS output
     // counter is 32-bit, so a DIRECT tile holds fewer than 2^31 products)
     if (!P.force && st_direct && st_direct[a] && st_products[a] < (1ull << 31)) {
@@ -1520,6 +1645,12 @@ __global__ void k_plan_tiles(const uint64_t *st_products, const uint64_t *st_max
         }
         nwin = (uint32_t)((P.spanB + W - 1) / W);
     }
+    if (P.mirror && nwin > 1) {
+        const uint64_t row0 = (uint64_t)P.atile_id[a] << P.logTL;
+        const uint32_t w0 = (uint32_t)(row0 / W);  // window holding column row0 (row0 < spanB)
+        t_w0[a] = w0;
+        nwin -= w0;
+    }
     t_nwin[a] = nwin;
     t_W[a] = (uint32_t)W;
     t_kind[a] = (uint8_t)kind;
@@ -1542,31 +1673,36 @@ __global__ void k_direct_runs(const uint8_t *t_kind, const uint64_t *pp, const u
 }
 
 __global__ void k_emit_units(const uint32_t *t_nwin, const uint32_t *t_W, const uint8_t *t_kind,
-                             const uint32_t *t_ubase, uint32_t natiles, uint64_t spanB, const float *pred,
-                             uint32_t *u_tile, uint32_t *u_c0, uint32_t *u_W, uint8_t *u_kind, uint8_t *u_multi,
-                             float *u_pred) {
+                             const uint32_t *t_ubase, const uint32_t *t_w0, uint32_t natiles, uint64_t spanB,
+                             const float *pred, uint32_t *u_tile, uint32_t *u_c0, uint32_t *u_W, uint8_t *u_kind,
+                             uint8_t *u_multi, float *u_pred) {
     uint32_t a = blockIdx.x * blockDim.x + threadIdx.x;
     if (a >= natiles) return;
-    uint32_t nw = t_nwin[a], W = t_W[a], base = t_ubase[a];
+    uint32_t nw = t_nwin[a], W = t_W[a], base = t_ubase[a], w0 = t_w0[a];
     for (uint32_t w = 0; w < nw; w++) {
-        uint64_t c0 = (uint64_t)w * W;
+        uint64_t c0 = (uint64_t)(w0 + w) * W;
         uint64_t wid = spanB - c0 < W ? spanB - c0 : W;
         u_tile[base + w] = a;
         u_c0[base + w] = (uint32_t)c0;
         u_W[base + w] = (uint32_t)wid;
         u_kind[base + w] = t_kind[a];
-        u_multi[base + w] = nw > 1;
+        u_multi[base + w] = wid < spanB;
         if (u_pred) u_pred[base + w] = pred ? pred[a] / (float)nw : 0.0f;
     }
 }
 
 // Exact planner, after the count pass: hash units whose exact count exceeds
 // the LDS hash go to the global hash; returns their total through *gtotal.
-__global__ void k_refine_kinds(const unsigned long long *unit_count, uint32_t n, int force, uint8_t *u_kind,
-                               unsigned long long *gtotal) {
+// In mirror mode a unit's count takes each kept cell twice except those on
+// the diagonal (at most one per row), so its distinct cells are at most
+// (count + min(TL, W)) / 2.
+__global__ void k_refine_kinds(const unsigned long long *unit_count, const uint32_t *u_W, uint32_t n, int force,
+                               int mirror, uint32_t TL, uint8_t *u_kind, unsigned long long *gtotal) {
     uint32_t u = blockIdx.x * blockDim.x + threadIdx.x;
     if (u >= n) return;
-    if (u_kind[u] == KIND_LHASH && unit_count[u] > LH_MAX && force != KIND_LHASH) u_kind[u] = KIND_GHASH;
+    unsigned long long cells = unit_count[u];
+    if (mirror) cells = (cells + min(TL, u_W[u])) / 2;
+    if (u_kind[u] == KIND_LHASH && cells > LH_MAX && force != KIND_LHASH) u_kind[u] = KIND_GHASH;
     if (u_kind[u] == KIND_GHASH) atomicAdd(gtotal, unit_count[u]);
 }
 
@@ -1600,27 +1736,28 @@ void launch_fiber_cuts(const Index &ix, uint32_t n, uint32_t W, uint32_t *fcut, 
 }
 
 void launch_plan_tiles(const Index &ix, const float *pred, const PlanParams &P, uint32_t *t_nwin, uint32_t *t_W,
-                       uint8_t *t_kind, hipStream_t s) {
+                       uint8_t *t_kind, uint32_t *t_w0, hipStream_t s) {
     if (!ix.natiles) return;
     k_plan_tiles<<<(unsigned)div_up(ix.natiles, 256), 256, 0, s>>>((const uint64_t *)ix.st_products,
                                                                      (const uint64_t *)ix.st_maxgp, ix.st_direct, pred,
-                                                                     ix.natiles, P, t_nwin, t_W, t_kind);
+                                                                     ix.natiles, P, t_nwin, t_W, t_kind, t_w0);
     k_direct_runs<<<(unsigned)div_up(ix.natiles, 256), 256, 0, s>>>(t_kind, (const uint64_t *)ix.st_pprefix,
                                                                       ix.atile_id, ix.natiles, ix.logTL, P.group_rows,
                                                                       P.run, t_nwin);
 }
 
 void launch_emit_units(const uint32_t *t_nwin, const uint32_t *t_W, const uint8_t *t_kind, const uint32_t *t_ubase,
-                       uint32_t natiles, uint64_t spanB, const float *pred, Units &U, hipStream_t s) {
+                       const uint32_t *t_w0, uint32_t natiles, uint64_t spanB, const float *pred, Units &U,
+                       hipStream_t s) {
     if (!natiles) return;
-    k_emit_units<<<(unsigned)div_up(natiles, 256), 256, 0, s>>>(t_nwin, t_W, t_kind, t_ubase, natiles, spanB, pred,
-                                                                  U.tile, U.c0, U.W, U.kind, U.multi, U.pred);
+    k_emit_units<<<(unsigned)div_up(natiles, 256), 256, 0, s>>>(t_nwin, t_W, t_kind, t_ubase, t_w0, natiles, spanB,
+                                                                  pred, U.tile, U.c0, U.W, U.kind, U.multi, U.pred);
 }
 
-void launch_refine_kinds(const unsigned long long *unit_count, uint32_t n, int force, uint8_t *u_kind,
-                         unsigned long long *gtotal, hipStream_t s) {
+void launch_refine_kinds(const unsigned long long *unit_count, const uint32_t *u_W, uint32_t n, int force, int mirror,
+                         uint32_t TL, uint8_t *u_kind, unsigned long long *gtotal, hipStream_t s) {
     if (!n) return;
-    k_refine_kinds<<<(unsigned)div_up(n, 256), 256, 0, s>>>(unit_count, n, force, u_kind, gtotal);
+    k_refine_kinds<<<(unsigned)div_up(n, 256), 256, 0, s>>>(unit_count, u_W, n, force, mirror, TL, u_kind, gtotal);
 }
 
 void launch_count(const AccArgs &A, unsigned grid, hipStream_t s) {

@@ -245,10 +245,10 @@ __global__ void k_unit_to_tile(const unsigned long long *unit_count, const uint3
     if (u < n && unit_kind[u] != KIND_DIRECT && unit_count[u]) atomicAdd(&tile_count[unit_tile[u]], unit_count[u]);
 }
 
-__global__ void k_direct_tile_count(const uint8_t *st_direct, const uint64_t *st_products, uint32_t T, int force,
+__global__ void k_direct_tile_count(const uint8_t *st_direct, const uint64_t *st_outd, uint32_t T, int force,
                                     unsigned long long *tile_count) {
     uint32_t a = blockIdx.x * blockDim.x + threadIdx.x;
-    if (a < T && !force && st_direct[a]) tile_count[a] = st_products[a];
+    if (a < T && !force && st_direct[a]) tile_count[a] = st_outd[a];
 }
 
 __global__ void k_kind_hist(const uint8_t *kind, uint32_t n, unsigned long long *h) {
@@ -330,12 +330,14 @@ __global__ void k_mape(const float *pred, const unsigned long long *actual, uint
 
 // Sum of predicted tile outputs, each capped by its exact bounds (products, area).
 __global__ void k_pred_sum(const float *pred, const uint64_t *st_products, const uint64_t *st_maxgp,
-                           const uint8_t *st_direct, double area, uint32_t n, unsigned long long *sum) {
+                           const uint8_t *st_direct, const uint64_t *st_outd, double area, uint32_t n,
+                           unsigned long long *sum) {
     uint32_t a = blockIdx.x * blockDim.x + threadIdx.x;
     unsigned long long v = 0ull;
-    if (a < n) {
-        // DIRECT tiles: exactly their products
-        double p = st_direct[a] ? (double)st_products[a] : ceil((double)pred[a]);
+    if (a < n && st_direct[a]) {
+        v = st_outd[a];  // DIRECT tiles: exactly their outputs
+    } else if (a < n) {
+        double p = ceil((double)pred[a]);
         p = p < (double)st_maxgp[a] ? (double)st_maxgp[a] : p;
         p = p > (double)st_products[a] ? (double)st_products[a] : p;
         p = p > area ? area : p;
@@ -586,6 +588,11 @@ int Contraction::run(const fcc_tensor *ta, const int32_t *am, const fcc_tensor *
     int logTL;
     uint32_t Wd;
     choose_tiles(ctx, o, g, products, nvalid, dk.nh > 0, &logTL, &Wd, &bucket, &tuned_wg);
+    // mirror mode: one tensor, the same mode lists, every left tile (a
+    // sample of rows has no symmetric output) and no statistics collection
+    // (its labels are whole-tile output counts)
+    const bool mirror = self && o.left_tile_stride <= 1 && !o.collect_stats && o.mirror >= 0 && !groups;
+    ix.mirror = mirror;
     if (const char *hm = getenv("FCC_HEAVY_MIN")) ix.heavy_min = (uint32_t)std::max(1, atoi(hm));
     if (const char *hs = getenv("FCC_HSEG")) ix.hseg = (uint32_t)std::max(64, atoi(hs));
     ix.defer_hseg = dk.nh > 0;  // built once, after the dense keys are masked
@@ -608,6 +615,7 @@ int Contraction::run(const fcc_tensor *ta, const int32_t *am, const fcc_tensor *
     st.row_tile = (int32_t)ix.TL;
     st.col_window = (int32_t)Wd;
     st.left_tiles = ix.natiles;
+    st.mirrored = mirror ? 1 : 0;
 
     dense_keys_mask(pool, ix, dk, hold, s);
     // Planning and the count pass read only the block's presence map (ready
@@ -666,7 +674,10 @@ int Contraction::run(const fcc_tensor *ta, const int32_t *am, const fcc_tensor *
     P.run = o.collect_stats ? 0 : DIRECT_RUN;
     k.plan_force = P.force;
     P.group_rows = groups ? g.spanA / groups : 0;
-    Tmp<uint32_t> t_nwin(pool, T), t_W(pool, T), t_ubase(pool, T), d_nunits(pool, 1);
+    P.mirror = mirror ? 1 : 0;
+    P.atile_id = ix.atile_id;
+    P.logTL = ix.logTL;
+    Tmp<uint32_t> t_nwin(pool, T), t_W(pool, T), t_ubase(pool, T), t_w0(pool, T), d_nunits(pool, 1);
     Tmp<uint8_t> t_kind(pool, T);
     Units &U = k.U;
     auto &u_tile = k.u_tile, &u_c0 = k.u_c0, &u_W = k.u_W;
@@ -710,7 +721,8 @@ int Contraction::run(const fcc_tensor *ta, const int32_t *am, const fcc_tensor *
     X.Arow = ix.Arow;
     X.natiles = ix.natiles;
     X.unit_order = nullptr;
-    X.row_own = getenv("FCC_ROW_OWN") ? 1 : 0;
+    X.row_own = (getenv("FCC_ROW_OWN") && !mirror) ? 1 : 0;
+    X.mirror = mirror ? 1 : 0;
     X.claim_div = 2 * (ACC_THREADS / 64);
     if (const char *cd = getenv("FCC_CLAIM_DIV")) X.claim_div = (uint32_t)std::max(0, atoi(cd));
     X.seg_below = 2 * (ACC_THREADS / 64);
@@ -776,7 +788,7 @@ int Contraction::run(const fcc_tensor *ta, const int32_t *am, const fcc_tensor *
     unsigned long long tkh[8] = {};
     auto make_units = [&](const float *pred, const unsigned long long *psum, uint64_t *psum_h) {
         P.dense_ratio = pred ? 0.15f : 0.5f;
-        launch_plan_tiles(ix, pred, P, t_nwin, t_W, t_kind, s);
+        launch_plan_tiles(ix, pred, P, t_nwin, t_W, t_kind, t_w0, s);
         fill_u64(tk_hist, 8, 0, s);
         k_kind_hist<<<std::min(grid256(T), 1024u), 256, 0, s>>>(t_kind, T, tk_hist);
         scan_u32(pool, t_nwin, t_ubase, T, d_nunits, s);
@@ -795,7 +807,7 @@ int Contraction::run(const fcc_tensor *ta, const int32_t *am, const fcc_tensor *
         U.kind = u_kind;
         U.multi = u_multi;
         U.pred = nullptr;
-        launch_emit_units(t_nwin, t_W, t_kind, t_ubase, T, g.spanB, pred, U, s);
+        launch_emit_units(t_nwin, t_W, t_kind, t_ubase, t_w0, T, g.spanB, pred, U, s);
         unit_count = Tmp<unsigned long long>(pool, U.n);
         unit_off = Tmp<unsigned long long>(pool, U.n);
         unit_cur = Tmp<unsigned long long>(pool, U.n);
@@ -820,7 +832,8 @@ int Contraction::run(const fcc_tensor *ta, const int32_t *am, const fcc_tensor *
         fill_u64(psum, 1, 0, s);
         k_pred_sum<<<grid256(T), 256, 0, s>>>(pred_t, (const uint64_t *)ix.st_products,
                                                (const uint64_t *)ix.st_maxgp, ix.st_direct,
-                                               (double)ix.TL * (double)g.spanB, T, psum);
+                                               (const uint64_t *)ix.st_outd, (double)ix.TL * (double)g.spanB, T,
+                                               psum);
     }
     make_units(learned ? pred_t.get() : nullptr, learned ? psum.get() : nullptr, &predicted);
     FCC_HIP(hipEventRecord(ev[2], s));
@@ -909,7 +922,7 @@ int Contraction::run(const fcc_tensor *ta, const int32_t *am, const fcc_tensor *
             // when this pass is the final one)
             fill_u64(tn, T, 0, s);
             k_unit_to_tile<<<grid256(U.n), 256, 0, s>>>(unit_cur, U.tile, U.kind, U.n, tn);
-            k_direct_tile_count<<<grid256(T), 256, 0, s>>>(ix.st_direct, (const uint64_t *)ix.st_products, T,
+            k_direct_tile_count<<<grid256(T), 256, 0, s>>>(ix.st_direct, (const uint64_t *)ix.st_outd, T,
                                                            P.force, tn);
             k_mape<<<grid256(T), 256, 0, s>>>(pred_t, tn, T, reinterpret_cast<double *>(tail.get() + 8));
             fetch_tail();
@@ -930,7 +943,7 @@ int Contraction::run(const fcc_tensor *ta, const int32_t *am, const fcc_tensor *
                 continue;
             }
             nnz_out = sink ? 0 : hm[1];
-            mape_pending = !sink;
+            mape_pending = !mirror;  // per-tile outputs = the whole tile band only without mirroring
             break;
         }
     }
@@ -952,7 +965,8 @@ int Contraction::run(const fcc_tensor *ta, const int32_t *am, const fcc_tensor *
             reset_ghash(false);
             fill_u64(unit_count, U.n, 0, s);
             launch_count(X, cgrid, s);
-            launch_refine_kinds(unit_count, U.n, o.force_acc, U.kind, gmeta.get() + 2, s);
+            launch_refine_kinds(unit_count, U.W, U.n, o.force_acc, mirror ? 1 : 0, ix.TL, U.kind, gmeta.get() + 2,
+                                s);
             scan_u64(pool, (const uint64_t *)unit_count.get(), (uint64_t *)unit_off.get(), U.n,
                      (uint64_t *)d_total.get(), s);
             uint32_t gf = 0;
@@ -973,7 +987,7 @@ int Contraction::run(const fcc_tensor *ta, const int32_t *am, const fcc_tensor *
             Tmp<unsigned long long> tn(pool, T), tc(pool, T);
             fill_u64(tn, T, 0, s);
             k_unit_to_tile<<<grid256(U.n), 256, 0, s>>>(unit_count, U.tile, U.kind, U.n, tn);
-            k_direct_tile_count<<<grid256(T), 256, 0, s>>>(ix.st_direct, (const uint64_t *)ix.st_products, T,
+            k_direct_tile_count<<<grid256(T), 256, 0, s>>>(ix.st_direct, (const uint64_t *)ix.st_outd, T,
                                                            P.force, tn);
             k_tile_cols<<<std::min<unsigned>(T, 4096), 256, 0, s>>>(ix.atile_g0, ix.group_fiber, ix.fiber_ptr,
                                                                      ix.Bcol, T, g.spanB, tc);
@@ -1935,7 +1949,7 @@ int fcc_plan_tiles(fcc_plan *plan, uint64_t *nnz, uint64_t *products, float *pre
             fill_u64(tn, T, 0, s);
             const unsigned long long *uc = plan->c->o.planner == FCC_PLAN_LEARNED ? k.unit_cur.get() : k.unit_count.get();
             k_unit_to_tile<<<grid256(k.U.n), 256, 0, s>>>(uc, k.U.tile, k.U.kind, k.U.n, tn);
-            k_direct_tile_count<<<grid256(T), 256, 0, s>>>(k.ix.st_direct, (const uint64_t *)k.ix.st_products, T,
+            k_direct_tile_count<<<grid256(T), 256, 0, s>>>(k.ix.st_direct, (const uint64_t *)k.ix.st_outd, T,
                                                            k.plan_force, tn);
             FCC_HIP(hipMemcpyAsync(nnz, tn.get(), 8ull * T, hipMemcpyDeviceToHost, s));
             FCC_HIP(hipStreamSynchronize(s));

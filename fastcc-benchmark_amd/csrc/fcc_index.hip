@@ -371,6 +371,35 @@ __global__ void k_tile_direct(const uint32_t *atile_g0, const uint32_t *group_pt
     direct[a] = ok;
 }
 
+// Mirror mode: outputs of each DIRECT tile. Its entries sit on distinct rows,
+// so every product is a distinct cell; entry (row r, fiber f) keeps the fiber
+// columns c >= r (fibers are sorted by column: a lower bound), each written
+// twice except the diagonal one. Other tiles keep their products.
+__global__ void k_direct_outputs(const uint32_t *atile_g0, const uint32_t *group_ptr, const uint32_t *Arow,
+                                 const uint2 *Afp, const uint32_t *Bcol, const uint8_t *st_direct,
+                                 const uint64_t *st_products, uint32_t natiles, uint64_t *st_outd) {
+    uint32_t a = blockIdx.x * blockDim.x + threadIdx.x;
+    if (a >= natiles) return;
+    if (!st_direct[a]) {
+        st_outd[a] = st_products[a];
+        return;
+    }
+    const uint32_t e0 = group_ptr[atile_g0[a]], e1 = group_ptr[atile_g0[a + 1]];
+    uint64_t n = 0;
+    for (uint32_t e = e0; e < e1; e++) {
+        const uint2 fp = Afp[e];
+        const uint32_t r = Arow[e];
+        uint32_t lo = fp.x, hi = fp.y;
+        while (lo < hi) {
+            const uint32_t mid = (lo + hi) >> 1;
+            if (Bcol[mid] < r) lo = mid + 1;
+            else hi = mid;
+        }
+        if (lo < fp.y) n += 2ull * (fp.y - lo) - (Bcol[lo] == r ? 1u : 0u);
+    }
+    st_outd[a] = n;
+}
+
 __device__ inline uint32_t hash32(uint32_t x) {
     x ^= x >> 16;
     x *= 0x7feb352dU;
@@ -799,9 +828,18 @@ static void left_from_sorted(DevPool &pool, const Operand &A, const uint64_t *ks
     if (T)
         k_tile_direct<<<grid_for(T), 256, 0, s>>>(ix.atile_g0, ix.group_ptr, ix.Ali, ix.st_nnz, (uint32_t)T, ix.TL,
                                                    ix.st_direct);
-    // exclusive prefix of the tiles' products (the output count of a run of DIRECT tiles)
+    // DIRECT tiles' outputs: their products, or in mirror mode the products
+    // at or above the diagonal counted twice off it
+    ix.st_outd = ix.st_products;
+    if (ix.mirror) {
+        ix.st_outd = (uint64_t *)keep(T * 8 + 8);
+        if (T)
+            k_direct_outputs<<<grid_for(T), 256, 0, s>>>(ix.atile_g0, ix.group_ptr, ix.Arow, ix.Afp, ix.Bcol,
+                                                          ix.st_direct, ix.st_products, (uint32_t)T, ix.st_outd);
+    }
+    // exclusive prefix of st_outd (the output count of a run of DIRECT tiles)
     ix.st_pprefix = (uint64_t *)keep((T + 1) * 8);
-    if (T) scan_u64(pool, ix.st_products, ix.st_pprefix, T, ix.st_pprefix + T, s);
+    if (T) scan_u64(pool, ix.st_outd, ix.st_pprefix, T, ix.st_pprefix + T, s);
     else fill_u64((unsigned long long *)ix.st_pprefix, 1, 0, s);
 }
 

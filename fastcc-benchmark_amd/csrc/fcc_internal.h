@@ -189,12 +189,19 @@ struct Index {
     uint32_t *st_sketch = nullptr;    // distinct-column sketch (KMV estimate)
     uint64_t *st_maxgp = nullptr;     // largest single-group product m * |fiber| (lower bound on outputs)
     uint8_t *st_direct = nullptr;     // 1: the tile's entries sit on distinct rows (DIRECT-able)
-    uint64_t *st_pprefix = nullptr;   // natiles+1: exclusive prefix sum of st_products
+    uint64_t *st_outd = nullptr;      // outputs of the tile when DIRECT: st_products, or in mirror
+                                      // mode its products at or above the diagonal, counted twice
+                                      // off it (aliases st_products when not mirrored)
+    uint64_t *st_pprefix = nullptr;   // natiles+1: exclusive prefix sum of st_outd
     int logTL = 0;
     uint32_t TL = 1;
     uint64_t products = 0;
     uint32_t heavy_min = HEAVY_MIN;  // fiber length from which groups are processed wave-per-group
     bool features = true;            // per-tile planner features (keys, fiber sums, sketches) wanted
+    // mirror mode: a self-contraction (one tensor, the same mode lists on both
+    // sides) has a symmetric output C(x, y) = C(y, x); each unordered pair is
+    // computed once, at its cell with column >= row, and written twice
+    bool mirror = false;
     // kept for plans (new values on the same coordinates): the input entry
     // behind each sorted position, so Bval/Aval can be gathered again
     bool keep_perm = false;
@@ -292,6 +299,14 @@ struct AccArgs {
     uint64_t dk_slab;    // dk_C holds dk_splits partial blocks dk_slab doubles apart; a value is their sum
     uint32_t dk_splits;
     uint64_t spanA;
+    // mirror mode (self-contractions, spanA == spanB): only cells (r, c) with
+    // c >= r are accumulated; each is written as (r, c) and, off the
+    // diagonal, as (c, r)
+    int mirror;
 };
+
+// Mirror mode: outputs written for cell (r, c): none below the diagonal, one
+// on it, two (the cell and its transpose) above it.
+__device__ __forceinline__ uint32_t mirror_weight(uint64_t r, uint64_t c) { return c < r ? 0u : (c == r ? 1u : 2u); }
 
 }  // namespace fcc

@@ -49,6 +49,7 @@ class FccOptions(C.Structure):
         ("left_tile_stride", C.c_int32),
         ("summarize", C.c_int32),
         ("dense_keys", C.c_float),
+        ("mirror", C.c_int32),
     ]
 
 
@@ -82,6 +83,7 @@ class FccStats(C.Structure):
         ("dense_flops", C.c_uint64),
         ("dense_block_bytes", C.c_uint64),
         ("units_block_counted", C.c_uint64),
+        ("mirrored", C.c_int32),
     ]
 
 

@@ -65,6 +65,10 @@ typedef struct {
                             tile accumulators; same outputs. 0 = default threshold (0.02, and only
                             when their products average four or more per block cell), > 0 = this
                             threshold unconditionally, < 0 = never */
+    int32_t mirror;      /* self-contractions (a == b, same mode lists): the output is symmetric,
+                            C(x, y) = C(y, x), so 0 (default) computes each unordered pair once and
+                            writes it at both coordinates; -1 computes every output. Same outputs
+                            either way. Off with left_tile_stride > 1 or collect_stats. */
 } fcc_options;
 
 typedef struct {
@@ -99,6 +103,7 @@ typedef struct {
     uint64_t dense_block_bytes; /* bytes of the block and presence map the accumulate kernels read */
     uint64_t units_block_counted; /* exact planner: dense units the dense-key block covers
                                      entirely, counted from it without a traversal */
+    int32_t mirrored;      /* 1: the symmetric output was computed once per unordered pair */
 } fcc_stats;
 
 typedef struct {

@@ -1,0 +1,147 @@
+"""Mirror mode: a self-contraction (one tensor on both sides, the same mode
+lists) has a symmetric output, C(x, y) = C(y, x) -- both are
+sum_k T(k, x) * T(k, y). The library computes each unordered pair once, at
+its cell with column >= row, and writes it at both coordinates. These tests
+check that the outputs are those of the oracle (bit-exact coordinate sets,
+values within max(1e-12 |ref|, 1e-14), exact for small-integer inputs) for
+every accumulator kind, both planners, several tile sizes, both output sinks,
+and that mirror mode on and off agree."""
+import functools
+import os
+
+import numpy as np
+import pytest
+
+from conftest import ROOT, assert_same_result, rand_tensor
+
+from oracle import oracle
+from fastcc_amd import FCC_ACC_AUTO, FCC_ACC_DENSE, FCC_ACC_GHASH, FCC_ACC_LHASH
+from fastcc_amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+REL, ABS = 1e-12, 1e-14
+MLP = os.path.join(ROOT, "fastcc-benchmark_amd", "fastcc_amd", "mlp_params.npy")
+
+# (dims, nnz, contraction modes): T contracted with itself on the same modes
+CASES = [
+    ((6, 5, 4), 60, (0,)),
+    ((5, 4, 6, 3), 90, (1, 3)),
+    ((40, 9, 30), 2500, (1,)),
+    ((300, 7, 8), 6000, (1, 2)),
+    ((12, 10), 80, ()),            # outer product T(x) T(y)
+    ((4, 5), 15, (0, 1)),          # full contraction: one diagonal cell
+    ((30, 6, 20), 3000, (0,)),
+    ((400, 300, 20), 9000, (2,)),  # 120k rows, most hold one entry: DIRECT tiles
+]
+
+
+@pytest.fixture(scope="module")
+def ctx(gpu_ctx):
+    gpu_ctx.set_mlp_params(np.load(MLP))
+    return gpu_ctx
+
+
+@functools.lru_cache(maxsize=None)
+def case_data(case, values):
+    rng = np.random.default_rng(4200 + 17 * case + len(values))
+    dims, nnz, modes = CASES[case]
+    t = rand_tensor(rng, dims, nnz, values)
+    return t, oracle.contract_bruteforce(t, modes, t, modes)
+
+
+def run(ctx, t, modes, **kw):
+    r = ctx.contract(t, list(modes), t, list(modes), **kw)
+    st = r.stats
+    c, v = r.to_host()
+    r.free()
+    return c, v, st
+
+
+def check(c, v, ref, values, what):
+    exact = values != "uniform"
+    assert_same_result(c, v, ref.coords, ref.vals, rel=0 if exact else REL, abs_floor=0 if exact else ABS,
+                       what=what, absum=None if exact else ref.absum, nterms=None if exact else ref.nterms)
+
+
+@pytest.mark.parametrize("case", range(len(CASES)))
+@pytest.mark.parametrize("values", ["uniform", "smallint", "zeros"])
+@pytest.mark.parametrize("force", [FCC_ACC_AUTO, FCC_ACC_DENSE, FCC_ACC_LHASH, FCC_ACC_GHASH])
+@pytest.mark.parametrize("planner", ["exact", "learned"])
+def test_mirror_small(ctx, case, values, force, planner):
+    print(f"mirror case={case} values={values} force={force} planner={planner}", flush=True)
+    t, ref = case_data(case, values)
+    modes = CASES[case][2]
+    c, v, st = run(ctx, t, modes, force_acc=force, planner=planner)
+    assert st["mirrored"] == 1
+    check(c, v, ref, values, f"mirror case {case} {values} force {force} {planner}")
+    assert st["products"] == ref.products
+
+
+@pytest.mark.parametrize("case", [2, 3, 6, 7])
+@pytest.mark.parametrize("tile", [1, 2, 4, 16, 128])
+@pytest.mark.parametrize("planner", ["exact", "learned"])
+def test_mirror_tile_sizes(ctx, case, tile, planner):
+    print(f"mirror tiles case={case} tile={tile} planner={planner}", flush=True)
+    t, ref = case_data(case, "smallint")
+    c, v, st = run(ctx, t, CASES[case][2], tile_hint=tile, planner=planner)
+    assert st["mirrored"] == 1 and st["row_tile"] == min(tile, st["row_tile"])
+    check(c, v, ref, "smallint", f"mirror case {case} tile {tile} {planner}")
+
+
+@pytest.mark.parametrize("case", range(len(CASES)))
+@pytest.mark.parametrize("planner", ["exact", "learned"])
+def test_mirror_sink(ctx, case, planner):
+    """Count + checksum outputs: the mirrored cells enter the summaries."""
+    print(f"mirror sink case={case} planner={planner}", flush=True)
+    t, _ = case_data(case, "uniform")
+    modes = CASES[case][2]
+    ref = oracle.contract_tiled(t, modes, t, modes, tile_size=16, sink=True)
+    r = ctx.contract(t, list(modes), t, list(modes), sink=True, planner=planner)
+    assert r.stats["mirrored"] == 1
+    assert r.nnz == ref.nnz and r.coord_hash == ref.coord_hash
+    assert r.val_sum == pytest.approx(ref.val_sum, rel=1e-10, abs=1e-10)
+    assert r.val_wsum == pytest.approx(ref.val_wsum, rel=1e-10, abs=1e-10)
+    r.free()
+
+
+def test_mirror_off_and_other_operands(ctx):
+    """mirror=False computes every cell; a different tensor object with the
+    same values, or sampled left tiles, never mirror; all give the same
+    outputs."""
+    t, ref = case_data(3, "smallint")
+    modes = CASES[3][2]
+    c, v, st = run(ctx, t, modes, mirror=False)
+    assert st["mirrored"] == 0
+    check(c, v, ref, "smallint", "mirror off")
+    from fastcc_amd.workloads import SparseTensor
+    u = SparseTensor(t.coords.copy(), t.vals.copy())
+    r = ctx.contract(t, list(modes), u, list(modes))
+    assert r.stats["mirrored"] == 0
+    c, v = r.to_host()
+    r.free()
+    check(c, v, ref, "smallint", "two tensor objects")
+
+
+SCALED = [("c2", lambda: W.c2(0.25)), ("c3", lambda: W.c3(0.02)), ("c4", lambda: W.c4(0.05))]
+
+
+@pytest.mark.parametrize("name,mk", SCALED)
+@pytest.mark.parametrize("planner", ["exact", "learned"])
+def test_mirror_on_off_scaled(ctx, name, mk, planner):
+    """Scaled self-contraction workloads (dense keys, DIRECT and hash tiles):
+    mirror on and off give the same coordinate set; values agree within the
+    fp64 summation-order bound (the two modes sum in different orders)."""
+    wl = mk()
+    print(f"mirror on/off {wl.name} {planner}", flush=True)
+    a, _ = wl.tensors()
+    outs = []
+    for m in (True, False):
+        r = ctx.contract(a, list(wl.a_modes), a, list(wl.b_modes), planner=planner, mirror=m)
+        assert r.stats["mirrored"] == (1 if m else 0)
+        outs.append(r.to_host())
+        r.free()
+    ref = oracle.contract_tiled(a, wl.a_modes, a, wl.b_modes, tile_size=128, high_precision=True)
+    for (c, v), m in zip(outs, (True, False)):
+        assert_same_result(c, v, ref.coords, ref.vals, rel=REL, abs_floor=ABS, what=f"{wl.name} mirror={m}",
+                           absum=ref.absum, nterms=ref.nterms)
