@@ -1704,6 +1704,9 @@ __global__ void k_refine_kinds(const unsigned long long *unit_count, const uint3
     if (mirror) cells = (cells + min(TL, u_W[u])) / 2;
     if (u_kind[u] == KIND_LHASH && cells > LH_MAX && force != KIND_LHASH) u_kind[u] = KIND_GHASH;
     if (u_kind[u] == KIND_GHASH) atomicAdd(gtotal, unit_count[u]);
+    // a forced LDS-hash unit over the limit seals its table once it holds
+    // LH_MAX keys: the rest of its cells go to the global hash, sized for them
+    else if (u_kind[u] == KIND_LHASH && cells > LH_MAX) atomicAdd(gtotal, cells - LH_MAX);
 }
 
 // fcut[f * n + w] = first position of fiber f whose column is >= w * W

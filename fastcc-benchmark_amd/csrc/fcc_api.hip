@@ -24,6 +24,7 @@
 #include "../../include/fcc.h"
 #include "fcc_accum.h"
 #include "fcc_batch.h"
+#include "fcc_chunk.h"
 #include "fcc_dense.h"
 #include "fcc_mlp.h"
 
@@ -106,26 +107,32 @@ Geom make_geom(const fcc_tensor *a, const int32_t *ka, const fcc_tensor *b, cons
             if (l[i] == m) return true;
         return false;
     };
+    // spans saturate at 2^64 - 1 instead of wrapping (an external index
+    // span of 2^32 or more is rejected by the caller)
+    auto mul_sat = [](uint64_t x, uint64_t y) {
+        uint64_t z;
+        return __builtin_mul_overflow(x, y, &z) ? ~0ull : z;
+    };
     g.spanA = 1;
     g.spanB = 1;
     for (int i = 0; i < nb; i++) {
         g.xa[g.nxa] = ka[i];
         g.xaE[g.nxa] = (uint32_t)g.R[i];
-        g.spanA *= g.R[i];
+        g.spanA = mul_sat(g.spanA, g.R[i]);
         g.nxa++;
     }
     for (int m = 0; m < a->order; m++)
         if (!in(ka, r, m)) {
             g.xa[g.nxa] = m;
             g.xaE[g.nxa] = EA[m];
-            g.spanA *= EA[m];
+            g.spanA = mul_sat(g.spanA, EA[m]);
             g.nxa++;
         }
     for (int m = 0; m < b->order; m++)
         if (!in(kb, r, m)) {
             g.xb[g.nxb] = m;
             g.xbE[g.nxb] = EB[m];
-            g.spanB *= EB[m];
+            g.spanB = mul_sat(g.spanB, EB[m]);
             g.nxb++;
         }
     g.K = 1;
@@ -135,8 +142,8 @@ Geom make_geom(const fcc_tensor *a, const int32_t *ka, const fcc_tensor *b, cons
         g.K *= g.R[i];
     }
     if (K >= 9.0e18L) g.K = ~0ull;
-    g.rbits = std::max(1, ceil_log2(g.spanA));
-    g.cbits = std::max(1, ceil_log2(g.spanB));
+    g.rbits = std::max(1, ceil_log2(std::min<uint64_t>(g.spanA, 1ull << 63)));
+    g.cbits = std::max(1, ceil_log2(std::min<uint64_t>(g.spanB, 1ull << 63)));
     g.kbits = std::max(1, ceil_log2(g.K));
     return g;
 }
@@ -476,7 +483,9 @@ int Contraction::run(const fcc_tensor *ta, const int32_t *am, const fcc_tensor *
     for (int m = 0; m < ta->order; m++) EA[m] = ta->extents ? ta->extents[m] : (ta->nnz ? hext[m] + 1 : 0);
     for (int m = 0; m < tb->order; m++)
         EB[m] = tb->extents ? tb->extents[m] : (tb->nnz ? hext[kMaxOrder + 1 + m] + 1 : 0);
-    const bool bitmap = keep || (hext[ta->order] | hext[kMaxOrder + 1 + tb->order]) != 0;
+    const uint32_t vflags = hext[ta->order] | hext[kMaxOrder + 1 + tb->order];
+    const bool bitmap = keep || (vflags & 1u) != 0;
+    const bool nonfinite = (vflags & 2u) != 0;
     k.bitmap = bitmap;
     if (ta->extents)
         for (int m = 0; m < ta->order; m++)
@@ -551,7 +560,8 @@ int Contraction::run(const fcc_tensor *ta, const int32_t *am, const fcc_tensor *
     // are contracted by one FP64 GEMM; units start from its block
     DenseKeys &dk = k.dk;
     dk = DenseKeys();
-    if (!groups && o.dense_keys >= 0.0f && products) {
+    // (not with infinite or NaN values: the GEMM's zero padding would spread them)
+    if (!groups && o.dense_keys >= 0.0f && products && !nonfinite) {
         // the block and its panels within an eighth of device memory
         const uint64_t max_cells = std::min<uint64_t>(1ull << 29, ctx->mem_total / 64);
         // the default threshold also asks the block to pay for itself; an
@@ -1175,6 +1185,19 @@ void fcc_destroy(fcc_ctx *c) {
 
 const char *fcc_last_error(const fcc_ctx *c) { return c ? c->err.c_str() : "no context"; }
 
+int fcc_set_memory_budget(fcc_ctx *c, uint64_t bytes) {
+    if (!c) return FCC_ERR_INVALID;
+    c->pool.set_budget((size_t)bytes);
+    return FCC_OK;
+}
+
+int fcc_memory_stats(const fcc_ctx *c, uint64_t *held, uint64_t *in_use) {
+    if (!c) return FCC_ERR_INVALID;
+    if (held) *held = c->pool.bytes_cached();
+    if (in_use) *in_use = c->pool.bytes_in_use();
+    return FCC_OK;
+}
+
 void *fcc_stream(fcc_ctx *c) { return c ? (void *)c->stream : nullptr; }
 
 namespace {
@@ -1517,8 +1540,53 @@ int fcc_contract_batch(fcc_ctx *ctx, int32_t n, const fcc_tensor *a, const int32
     hipStream_t s = ctx->stream;
     try {
         FCC_HIP(hipSetDevice(ctx->device));
-        for (int32_t c0 = 0; c0 < n; c0 += (int32_t)kMaxBatchPairs) {
-            const uint32_t cnt = (uint32_t)std::min<int32_t>(n - c0, (int32_t)kMaxBatchPairs);
+        // per-pair extents (maximum coordinate + 1, or the caller's, whichever
+        // is larger) decide how many pairs one fused launch can hold
+        std::vector<uint32_t> pext[2];
+        std::vector<uint64_t> pnnz[2];
+        for (int k = 0; k < 2; k++) {
+            const fcc_tensor *src = k ? b : a;
+            const int d = k ? dB : dA;
+            pext[k].assign((size_t)n * d, 0);
+            pnnz[k].assign(n, 0);
+            std::vector<PairSrc> dev;
+            std::vector<int32_t> dev_idx;
+            for (int32_t i = 0; i < n; i++) {
+                const fcc_tensor &t = src[i];
+                pnnz[k][i] = (uint64_t)t.nnz;
+                if (t.nnz && t.on_device) {
+                    dev.push_back(PairSrc{t.coords, t.vals, (uint64_t)t.nnz});
+                    dev_idx.push_back(i);
+                } else if (t.nnz) {
+                    for (int m = 0; m < d; m++) {
+                        uint32_t mx = 0;
+                        for (int64_t j = 0; j < t.nnz; j++) mx = std::max(mx, t.coords[(size_t)m * t.nnz + j]);
+                        pext[k][(size_t)i * d + m] = mx + 1;
+                    }
+                }
+            }
+            if (!dev.empty()) {
+                Tmp<unsigned char> dps(pool, sizeof(PairSrc) * dev.size());
+                Tmp<uint32_t> dmx(pool, dev.size() * d);
+                FCC_HIP(hipMemcpyAsync(dps.get(), dev.data(), sizeof(PairSrc) * dev.size(), hipMemcpyHostToDevice, s));
+                pair_max((const PairSrc *)dps.get(), (uint32_t)dev.size(), d, dmx, s);
+                std::vector<uint32_t> hm(dev.size() * d);
+                FCC_HIP(hipMemcpyAsync(hm.data(), dmx.get(), 4 * hm.size(), hipMemcpyDeviceToHost, s));
+                FCC_HIP(hipStreamSynchronize(s));
+                for (size_t q = 0; q < dev.size(); q++)
+                    for (int m = 0; m < d; m++) pext[k][(size_t)dev_idx[q] * d + m] = hm[q * d + m] + 1;
+            }
+            for (int32_t i = 0; i < n; i++)
+                if (src[i].extents)
+                    for (int m = 0; m < d; m++)
+                        pext[k][(size_t)i * d + m] = std::max(pext[k][(size_t)i * d + m], src[i].extents[m]);
+        }
+        const std::vector<uint32_t> ends = batch_chunks((uint32_t)n, dA, dB, a_modes, b_modes, n_contr,
+                                                        pext[0].data(), pext[1].data(), pnnz[0].data(),
+                                                        pnnz[1].data(), kMaxBatchPairs);
+        for (size_t ci = 0; ci < ends.size(); ci++) {
+            const int32_t c0 = ci ? (int32_t)ends[ci - 1] : 0;
+            const uint32_t cnt = ends[ci] - (uint32_t)c0;
             const fcc_tensor *pa = a + c0, *pb = b + c0;
             fcc_result *po = outs + c0;
             // ---- concatenate each operand with a trailing pair-index mode

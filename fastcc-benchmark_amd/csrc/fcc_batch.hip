@@ -35,6 +35,21 @@ __device__ __forceinline__ uint32_t wave_add(uint32_t *s_h, bool t, uint32_t key
     return t ? atomicAdd(&s_h[key], 1u) : 0u;
 }
 
+// One workgroup per pair: the maximum coordinate of each mode (0 when empty).
+__global__ __launch_bounds__(256) void k_pair_max(const PairSrc *src, int order, uint32_t *out) {
+    __shared__ uint32_t s_m[4];
+    const PairSrc ps = src[blockIdx.x];
+    for (int m = 0; m < order; m++) {
+        uint32_t v = 0;
+        for (uint64_t j = threadIdx.x; j < ps.nnz; j += 256) v = max(v, ps.coords[(uint64_t)m * ps.nnz + j]);
+        for (int d = 32; d > 0; d >>= 1) v = max(v, (uint32_t)__shfl_xor((int)v, d, 64));
+        if ((threadIdx.x & 63) == 0) s_m[threadIdx.x >> 6] = v;
+        __syncthreads();
+        if (threadIdx.x == 0) out[(uint64_t)blockIdx.x * order + m] = max(max(s_m[0], s_m[1]), max(s_m[2], s_m[3]));
+        __syncthreads();
+    }
+}
+
 __global__ __launch_bounds__(256) void k_gather_pairs(const PairSrc *src, const unsigned long long *offs, uint32_t n,
                                                       int order, uint64_t total, uint32_t *dst, double *dv,
                                                       uint32_t *dmax) {
@@ -164,6 +179,10 @@ __global__ void k_group_units(const uint32_t *unit_tile, const uint32_t *atile_i
 }
 
 }  // namespace
+
+void pair_max(const PairSrc *src, uint32_t n, int order, uint32_t *out, hipStream_t s) {
+    if (n) k_pair_max<<<n, 256, 0, s>>>(src, order, out);
+}
 
 void gather_pairs(const PairSrc *src, const unsigned long long *offs, uint32_t n, int order, uint64_t total,
                   uint32_t *dst_coords, double *dst_vals, uint32_t *dmax, hipStream_t s) {

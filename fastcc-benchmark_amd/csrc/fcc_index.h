@@ -28,7 +28,8 @@ void fill_many(const FillSet &f, hipStream_t s);
 void perm_gather(const double *src, const uint32_t *perm, size_t n, double *dst, hipStream_t s);
 
 // out[m] = max coordinate of mode m (m < order); out[order] |= 1 when a value
-// is zero or tiny. out must be zeroed by the caller.
+// is zero or tiny, |= 2 when a value is infinite or NaN. out must be zeroed by
+// the caller.
 void launch_extents(const Operand &t, uint32_t *dev_out, hipStream_t s);
 
 // Left join read off the right operand's sort (self-contraction).

@@ -65,7 +65,9 @@ void fill_f64(double *p, size_t n, double v, hipStream_t s) {
 // out[m] = max coordinate of mode m (the reference's shape, contract.hpp:124-144);
 // out[order] |= 1 if some value is zero, subnormal, or smaller than 2^-511 in
 // magnitude (then a product can round to zero and the accumulators must track
-// touched slots explicitly).
+// touched slots explicitly); |= 2 if some value is infinite or NaN (then no key
+// may go through the dense-key GEMM, whose zero padding would turn inf * 0
+// into NaN at cells the reference never touches with that value).
 __global__ __launch_bounds__(256) void k_extents(const uint32_t *coords, const double *vals, int order, size_t nnz,
                                                 uint32_t *out) {
     __shared__ uint32_t s_mx[4], s_fl[4];
@@ -97,7 +99,8 @@ __global__ __launch_bounds__(256) void k_extents(const uint32_t *coords, const d
         for (size_t i = head + 4 * nq + gid; i < nnz; i += stride) mx = max(mx, p[i]);
     } else {
         auto take = [&](double v) {
-            if (!(fabs(v) >= 0x1.0p-511) && !isnan(v)) flag = 1;
+            if (!(fabs(v) >= 0x1.0p-511) && !isnan(v)) flag |= 1;
+            if (!isfinite(v)) flag |= 2;
         };
         const size_t head = std::min<size_t>(nnz, ((16 - ((uintptr_t)vals & 15)) & 15) / 8);
         const double2 *q = reinterpret_cast<const double2 *>(vals + head);
@@ -140,7 +143,7 @@ __global__ __launch_bounds__(256) void k_extents(const uint32_t *coords, const d
         if (m < order) {
             if (mx > __atomic_load_n(&out[m], __ATOMIC_RELAXED)) atomicMax(&out[m], mx);
         } else if (flag) {
-            atomicOr(&out[order], 1u);
+            atomicOr(&out[order], flag);
         }
     }
 }

@@ -38,6 +38,11 @@ class DevPool {
     void put(void *p);
     void release_all();
     size_t bytes_cached() const { return cached_; }
+    size_t bytes_in_use() const { return in_use_; }
+    // device bytes the pool may hold (cached + in use); 0 = no limit. An
+    // allocation beyond it first returns cached blocks, then fails with
+    // ERR_OOM, as a failed hipMalloc does.
+    void set_budget(size_t bytes) { budget_ = bytes; }
 
     // Small device-to-host reads staged through pinned host memory (a
     // pageable destination makes every copy a slower synchronous one):
@@ -54,6 +59,8 @@ class DevPool {
     std::multimap<size_t, void *> free_;
     std::map<void *, size_t> used_;
     size_t cached_ = 0;
+    size_t in_use_ = 0;
+    size_t budget_ = 0;
     static constexpr size_t kPinBytes = 1 << 16;
     struct Pending {
         void *dst;

@@ -49,7 +49,19 @@ void *DevPool::get(size_t bytes) {
         size_t sz = it->first;
         free_.erase(it);
         used_[p] = sz;
+        in_use_ += sz;
         return p;
+    }
+    if (budget_ && cached_ + bytes > budget_) {
+        // over the budget: give cached blocks back first
+        for (auto &kv : free_) {
+            (void)hipFree(kv.second);
+            cached_ -= kv.first;
+        }
+        free_.clear();
+        if (cached_ + bytes > budget_)
+            throw LibError{ERR_OOM, "device memory budget of " + std::to_string(budget_) + " bytes exceeded (" +
+                                        std::to_string(cached_) + " held, " + std::to_string(bytes) + " asked)"};
     }
     void *p = nullptr;
     hipError_t e = hipMalloc(&p, bytes);
@@ -69,6 +81,7 @@ void *DevPool::get(size_t bytes) {
     }
     cached_ += bytes;
     used_[p] = bytes;
+    in_use_ += bytes;
     return p;
 }
 
@@ -76,6 +89,7 @@ void DevPool::put(void *p) {
     auto it = used_.find(p);
     if (it == used_.end()) return;
     free_.emplace(it->second, p);
+    in_use_ -= it->second;
     used_.erase(it);
 }
 
@@ -85,6 +99,7 @@ void DevPool::release_all() {
     free_.clear();
     used_.clear();
     cached_ = 0;
+    in_use_ = 0;
 }
 
 // ------------------------------------------------------------------- scan

@@ -304,6 +304,16 @@ class Context:
         self._check(rc)
         return [Result(self, raws[i]) for i in range(n)]
 
+    # ---- device memory
+    def set_memory_budget(self, nbytes: int):
+        """Cap on the device bytes the context's pool may hold (0: none)."""
+        self._check(self.lib.fcc_set_memory_budget(self.h, int(nbytes)))
+
+    def memory_stats(self) -> dict:
+        held, used = C.c_uint64(), C.c_uint64()
+        self._check(self.lib.fcc_memory_stats(self.h, C.byref(held), C.byref(used)))
+        return {"held": int(held.value), "in_use": int(used.value)}
+
     # ---- learned planner
     @property
     def mlp_param_count(self) -> int:

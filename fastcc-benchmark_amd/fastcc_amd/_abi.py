@@ -23,6 +23,7 @@ EXPORTS = (
     "fcc_contract_batch", "fcc_result_to_host", "fcc_result_free", "fcc_mlp_param_count", "fcc_mlp_set_params",
     "fcc_mlp_get_params", "fcc_tile_stats_count", "fcc_tile_stats_copy", "fcc_mlp_train",
     "fcc_mlp_forward_backward", "fcc_density_bucket", "fcc_tune_set", "fcc_tune_get",
+    "fcc_set_memory_budget", "fcc_memory_stats",
 )
 
 
@@ -211,6 +212,10 @@ def load(path: str | None = None):
     lib.fcc_plan_execute.restype = C.c_int
     lib.fcc_plan_destroy.argtypes = [P]
     lib.fcc_plan_destroy.restype = None
+    lib.fcc_set_memory_budget.argtypes = [P, C.c_uint64]
+    lib.fcc_set_memory_budget.restype = C.c_int
+    lib.fcc_memory_stats.argtypes = [P, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]
+    lib.fcc_memory_stats.restype = C.c_int
     if path is None:
         _lib = lib
     return lib

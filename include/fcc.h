@@ -133,6 +133,16 @@ void *fcc_stream(fcc_ctx *ctx);
 
 void fcc_options_default(fcc_options *opt);
 
+/* Device memory of a context: all device temporaries and results come from a
+ * caching pool. fcc_set_memory_budget caps what the pool may hold (cached and
+ * in use; 0 = no cap, the default): a call that would need more fails with
+ * FCC_ERR_OOM and leaves nothing allocated beyond what was held before it.
+ * Stands in for the reference's fixed-size per-thread heaps, which exit(1)
+ * when exhausted (timer.hpp:76-101). fcc_memory_stats reports the bytes the
+ * pool holds and how many of them are in use (results not yet freed, plans). */
+int fcc_set_memory_budget(fcc_ctx *ctx, uint64_t bytes);
+int fcc_memory_stats(const fcc_ctx *ctx, uint64_t *held, uint64_t *in_use);
+
 /* Replaces Tensor<double>::fastcc_multiply<TileAccumulator<double>,double>(
  *     Tensor<double>& other, CoOrdinate left_contr, CoOrdinate right_contr,
  *     int tile_size) -- contract.hpp:824-827. Here a = *this, b = other,
@@ -158,7 +168,9 @@ int fcc_contract_ex(fcc_ctx *ctx, const fcc_tensor *a, const int32_t *a_modes, c
  * (benchmark.cpp:68-73). The pairs run as ONE batch-mode contraction whose
  * batch pair is a pair index appended to both operands, so every pair's tiles
  * share one persistent work-stealing accumulate launch (up to 4096 pairs per
- * launch; more are taken 4096 at a time). outs[i] is pair i's result, with the
+ * launch, fewer when the fused index would exceed the limits of one
+ * contraction: the pair index multiplies the left span and the key span;
+ * consecutive pairs are taken as many at a time as fit). outs[i] is pair i's result, with the
  * same coordinates and values as fcc_contract on that pair, except that:
  *   - extents are the per-mode maximum over all pairs (the summaries
  *     coord_hash / val_wsum linearise with these extents);

@@ -53,6 +53,19 @@ def sort_result(coords, vals):
     return coords[:, o], vals[o]
 
 
+PARITY_LOG = os.environ.get("FCC_PARITY_LOG")
+
+
+def record_parity(what, **fields):
+    """Appends one JSON line per checked result to $FCC_PARITY_LOG (when set):
+    the error figures the parity tables are made from."""
+    if not PARITY_LOG:
+        return
+    import json
+    with open(PARITY_LOG, "a") as f:
+        f.write(json.dumps({"what": what, **fields}) + "\n")
+
+
 def assert_same_result(gc, gv, rc, rv, rel=1e-12, abs_floor=1e-14, what="", absum=None, nterms=None):
     """Sorted coordinate sets bit-exact; values |g - r| <= max(rel*|r|, abs_floor).
 
@@ -60,12 +73,24 @@ def assert_same_result(gc, gv, rc, rv, rel=1e-12, abs_floor=1e-14, what="", absu
     terms, an output whose sum cancels may instead meet the worst-case error of
     an fp64 sum in an arbitrary order, (n+1) * 2^-53 * sum|terms| (one rounding
     per product and per addition); the number of outputs that needed this is
-    printed, and must stay a small fraction."""
+    printed, and must stay a small fraction. Non-finite values must match
+    exactly (the same infinities and NaN positions). Returns the maximum
+    relative error, the outputs outside the 1e-12 bound and those that met
+    only the summation-order bound."""
     gc, gv = sort_result(np.asarray(gc), np.asarray(gv))
     o = sort_order(np.asarray(rc))
     rc, rv = np.asarray(rc)[:, o], np.asarray(rv)[o]
     assert gc.shape == rc.shape, f"{what}: nnz/order differ {gc.shape} vs {rc.shape}"
     assert np.array_equal(gc, rc), f"{what}: coordinate sets differ"
+    fin = np.isfinite(rv)
+    assert np.array_equal(np.isfinite(gv), fin), f"{what}: non-finite outputs differ"
+    assert np.array_equal(np.isnan(gv), np.isnan(rv)), f"{what}: NaN outputs differ"
+    assert np.array_equal(gv[np.isinf(rv)], rv[np.isinf(rv)]), f"{what}: infinite outputs differ"
+    gv, rv = gv[fin], rv[fin]
+    if absum is not None:
+        absum = np.asarray(absum)[o][fin]
+        nterms = np.asarray(nterms)[o][fin]
+        o = np.arange(rv.size)
     err = np.abs(gv - rv)
     tol = np.maximum(rel * np.abs(rv), abs_floor)
     strict_bad = err > tol
@@ -78,8 +103,16 @@ def assert_same_result(gc, gv, rc, rv, rel=1e-12, abs_floor=1e-14, what="", absu
         assert n_relaxed <= max(10, rv.size // 10000), f"{what}: too many cancelling outputs ({n_relaxed})"
     else:
         bad = strict_bad
-    assert not bad.any(), (f"{what}: {bad.sum()} values out of tolerance; worst "
-                           f"{np.max(np.abs(gv - rv) / np.maximum(np.abs(rv), 1e-300))}")
+    relerr = np.abs(gv - rv) / np.maximum(np.abs(rv), 1e-300)
+    worst = float(np.max(relerr)) if rv.size else 0.0
+    assert not bad.any(), f"{what}: {bad.sum()} values out of tolerance; worst {worst}"
+    stats = {"outputs": int(rv.size), "max_rel_err": worst,
+             # where the relative bound, not the absolute floor, is the tolerance
+             "max_rel_err_rel_bound": float(np.max(relerr[rel * np.abs(rv) >= abs_floor]))
+             if (rel * np.abs(rv) >= abs_floor).any() else 0.0,
+             "outside_1e-12": int(strict_bad.sum()), "order_bound_only": int(strict_bad.sum() - bad.sum())}
+    record_parity(what, **stats)
+    return stats
 
 
 @pytest.fixture(scope="session", autouse=True)
