@@ -30,6 +30,12 @@
 
 using namespace fcc;
 
+// Device scratch of fcc_contract_noalloc: every device temporary of such a
+// call is carved from it. Each device the library runs on holds its own copy
+// until the process ends; host code reaches it only through the address
+// hipGetSymbolAddress returns for the current device (fcc_ctx::scratch).
+static __device__ __align__(256) unsigned char scratch[FCC_NOALLOC_SCRATCH_BYTES];
+
 struct fcc_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
@@ -48,6 +54,7 @@ struct fcc_ctx {
     // statistics of the last collect_stats contraction
     std::vector<float> feat, labels;
     hipEvent_t ev[12] = {};
+    void *scratch = nullptr;  // this device's copy of the no-allocation scratch (null: lookup failed)
 };
 
 namespace {
@@ -400,6 +407,11 @@ struct Contraction {
     // permutations (values can be gathered again) and use explicit-bitmap
     // dense accumulators (later values may hold zeros)
     bool keep = false;
+    // caller-owned outputs (the no-allocation entry point): SoA coordinates
+    // with row stride user_cap and user_cap values; null = library-owned
+    uint32_t *user_c = nullptr;
+    double *user_v = nullptr;
+    uint64_t user_cap = 0;
     struct Kept {
         std::vector<std::shared_ptr<void>> hold;
         Index ix;
@@ -907,18 +919,19 @@ int Contraction::run(const fcc_tensor *ta, const int32_t *am, const fcc_tensor *
         // ---- learned planning: predicted capacity, single numeric pass
         // never more than the products bound: no tile can emit more outputs than products
         uint64_t capacity = std::min<uint64_t>((uint64_t)((double)predicted * 1.10) + 4096, products + 16);
+        if (user_c || user_v) capacity = user_cap;  // the caller's buffers, whole
         alloc_ghash(1u << 20);
         join_block();
         FCC_HIP(hipEventRecord(ev[3], s));
         Tmp<unsigned long long> tn(pool, T);
         for (int attempt = 0;; attempt++) {
-            if (!sink) {
+            if (!sink && !user_v) {
                 out_c = Tmp<uint32_t>(pool, (size_t)g.dC * capacity);
                 out_v = Tmp<double>(pool, (size_t)capacity);
             }
             stride = capacity;
-            X.ocoords = out_c.get();
-            X.ovals = out_v.get();
+            X.ocoords = user_v ? user_c : out_c.get();
+            X.ovals = user_v ? user_v : out_v.get();
             X.ostride = stride;
             X.capacity = sink ? ~0ull : capacity;
             reset_ghash(sink);
@@ -947,6 +960,11 @@ int Contraction::run(const fcc_tensor *ta, const int32_t *am, const fcc_tensor *
                 break;
             }
             if (hf[1]) {
+                if (user_v) {
+                    out->nnz = (int64_t)hm[1];
+                    throw LibError{FCC_ERR_OVERFLOW, "output capacity " + std::to_string(user_cap) + " below the " +
+                                                         std::to_string(hm[1]) + " outputs (out->nnz)"};
+                }
                 capacity = hm[1];
                 st.reruns++;
                 if (attempt >= 4) throw LibError{FCC_ERR_OVERFLOW, "learned capacity re-run did not converge"};
@@ -1009,13 +1027,18 @@ int Contraction::run(const fcc_tensor *ta, const int32_t *am, const fcc_tensor *
             FCC_HIP(hipMemcpyAsync(ctx->labels.data(), lab.get(), ctx->labels.size() * 4, hipMemcpyDeviceToHost, s));
             FCC_HIP(hipStreamSynchronize(s));
         }
-        if (!sink) {
+        if (!sink && user_v && nnz_out > user_cap) {
+            out->nnz = (int64_t)nnz_out;
+            throw LibError{FCC_ERR_OVERFLOW, "output capacity " + std::to_string(user_cap) + " below the " +
+                                                 std::to_string(nnz_out) + " outputs (out->nnz)"};
+        }
+        if (!sink && !user_v) {
             out_c = Tmp<uint32_t>(pool, (size_t)g.dC * nnz_out);
             out_v = Tmp<double>(pool, (size_t)nnz_out);
         }
-        stride = nnz_out;
-        X.ocoords = out_c.get();
-        X.ovals = out_v.get();
+        stride = user_v ? user_cap : nnz_out;
+        X.ocoords = user_v ? user_c : out_c.get();
+        X.ovals = user_v ? user_v : out_v.get();
         X.ostride = stride;
         X.capacity = nnz_out;
         if (gtotal > (X.gmask + 1) / 2) alloc_ghash(2 * gtotal);
@@ -1057,9 +1080,9 @@ int Contraction::run(const fcc_tensor *ta, const int32_t *am, const fcc_tensor *
     k.mode = (learned && !use_exact) ? 1 : 0;
     k.empty = false;
     if (!sink) {
-        out->coords = out_c.p;
-        out->vals = out_v.p;
-        if (!keep) {  // the result owns the output; a plan keeps it
+        out->coords = user_v ? user_c : out_c.p;
+        out->vals = user_v ? user_v : out_v.p;
+        if (!keep && !user_v) {  // the result owns the output; a plan keeps it
             ResultPriv *rp = new ResultPriv;
             rp->coords = out_c.p;
             rp->vals = out_v.p;
@@ -1157,6 +1180,12 @@ int fcc_create(fcc_ctx **out, int device) {
         if (hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, device) == hipSuccess && khz > 0)
             c->clock_khz = khz;
         FCC_HIP(hipMalloc(&c->mlp_dev, sizeof(float) * MLP_PARAMS));
+        c->pool.init_pinned();
+        // the no-allocation scratch on this device (the device is current)
+        if (hipGetSymbolAddress(&c->scratch, HIP_SYMBOL(scratch)) != hipSuccess) {
+            (void)hipGetLastError();
+            c->scratch = nullptr;
+        }
     } catch (HipError &e) {
         for (auto &ev : c->ev)
             if (ev) (void)hipEventDestroy(ev);
@@ -1272,6 +1301,67 @@ int contract_impl(fcc_ctx *ctx, const fcc_tensor *a, const int32_t *a_modes, con
 int fcc_contract(fcc_ctx *ctx, const fcc_tensor *a, const int32_t *a_modes, const fcc_tensor *b,
                  const int32_t *b_modes, int32_t n_contr, const fcc_options *options, fcc_result *out) {
     return contract_impl(ctx, a, a_modes, nullptr, b, b_modes, nullptr, n_contr, 0, options, out);
+}
+
+int fcc_contract_noalloc(fcc_ctx *ctx, const fcc_tensor *a, const int32_t *a_modes, const fcc_tensor *b,
+                         const int32_t *b_modes, int32_t n_contr, const fcc_options *options, uint32_t *out_coords,
+                         double *out_vals, int64_t out_capacity, fcc_result *out) {
+    if (!ctx) return FCC_ERR_INVALID;
+    if (!out) return fail(ctx, FCC_ERR_INVALID, "out is NULL");
+    memset(out, 0, sizeof(*out));
+    std::string msg;
+    int rc = validate(a, "a", msg);
+    if (!rc) rc = validate(b, "b", msg);
+    std::vector<int32_t> ka, kb;
+    if (!rc) rc = check_modes(a, b, a_modes, b_modes, n_contr, nullptr, nullptr, 0, ka, kb, msg);
+    if (rc) return fail(ctx, rc, msg);
+    fcc_options o;
+    fcc_options_default(&o);
+    if (options) o = *options;
+    if (o.left_tile_stride < 1) o.left_tile_stride = 1;
+    if (o.collect_stats) return fail(ctx, FCC_ERR_INVALID, "collect_stats needs host storage: not without allocation");
+    // the dense-key block would need spanA x spanB cells of scratch
+    o.dense_keys = -1.0f;
+    const bool sink = o.sink == FCC_SINK_COUNT_CHECKSUM;
+    const int dC = a->order + b->order - 2 * n_contr;
+    if (!sink && (out_capacity < 0 || (out_capacity > 0 && (!out_vals || (dC > 0 && !out_coords)))))
+        return fail(ctx, FCC_ERR_INVALID, "materialised outputs need out_vals, out_coords and out_capacity >= 0");
+    if (!ctx->scratch) return fail(ctx, FCC_ERR_HIP, "no-allocation scratch not found on this device");
+    DevPool &pool = ctx->pool;
+    struct ArenaEnd {  // declared before the contraction: its temporaries go first
+        DevPool &p;
+        ~ArenaEnd() { p.end_arena(); }
+    } arena_end{pool};
+    pool.begin_arena(ctx->scratch, FCC_NOALLOC_SCRATCH_BYTES);
+    Contraction c;
+    c.ctx = ctx;
+    c.s = ctx->stream;
+    c.o = o;
+    c.out = out;
+    if (!sink) {
+        c.user_c = out_coords;
+        c.user_v = out_vals ? out_vals : reinterpret_cast<double *>(ctx->scratch);  // capacity 0: never written
+        c.user_cap = (uint64_t)out_capacity;
+    }
+    try {
+        FCC_HIP(hipSetDevice(ctx->device));
+        rc = c.run(a, ka.data(), b, kb.data(), n_contr, 0);
+        if (!sink && out->nnz == 0) {
+            out->coords = nullptr;
+            out->vals = nullptr;
+        }
+        ctx->err.clear();
+        return rc;
+    } catch (HipError &e) {
+        (void)hipGetLastError();
+        return fail(ctx, FCC_ERR_HIP, e.where + ": " + hipGetErrorString(e.code));
+    } catch (LibError &e) {
+        out->coords = nullptr;
+        out->vals = nullptr;
+        return fail(ctx, e.code, e.msg);
+    } catch (std::bad_alloc &) {
+        return fail(ctx, FCC_ERR_OOM, "host allocation failed");
+    }
 }
 
 int fcc_contract_ex(fcc_ctx *ctx, const fcc_tensor *a, const int32_t *a_modes, const int32_t *a_batch,

@@ -43,6 +43,19 @@ class DevPool {
     // allocation beyond it first returns cached blocks, then fails with
     // ERR_OOM, as a failed hipMalloc does.
     void set_budget(size_t bytes) { budget_ = bytes; }
+    // Arena mode: every get() is carved from [base, base + bytes) in order
+    // (no device allocation; put() of such a block is a no-op, the arena is
+    // reused whole by the next begin_arena). Exceeding it throws ERR_INVALID.
+    void begin_arena(void *base, size_t bytes) {
+        arena_ = (uint8_t *)base;
+        arena_bytes_ = bytes;
+        arena_used_ = 0;
+    }
+    void end_arena() { arena_ = nullptr; }
+    bool in_arena() const { return arena_ != nullptr; }
+    size_t arena_peak() const { return arena_used_; }
+    // the pinned staging buffer of read(), allocated up front (context creation)
+    void init_pinned();
 
     // Small device-to-host reads staged through pinned host memory (a
     // pageable destination makes every copy a slower synchronous one):
@@ -61,6 +74,8 @@ class DevPool {
     size_t cached_ = 0;
     size_t in_use_ = 0;
     size_t budget_ = 0;
+    uint8_t *arena_ = nullptr;
+    size_t arena_bytes_ = 0, arena_used_ = 0;
     static constexpr size_t kPinBytes = 1 << 16;
     struct Pending {
         void *dst;

@@ -17,11 +17,15 @@ DevPool::~DevPool() {
     if (pin_) (void)hipHostFree(pin_);
 }
 
-void DevPool::read(void *dst, const void *src, size_t bytes, hipStream_t s) {
+void DevPool::init_pinned() {
     if (!pin_ && hipHostMalloc((void **)&pin_, kPinBytes, hipHostMallocDefault) != hipSuccess) {
         (void)hipGetLastError();
         pin_ = nullptr;
     }
+}
+
+void DevPool::read(void *dst, const void *src, size_t bytes, hipStream_t s) {
+    if (!arena_) init_pinned();  // arena mode allocates nothing: without staging, direct copies
     const size_t off = (pin_used_ + 15) & ~size_t(15);
     if (!pin_ || off + bytes > kPinBytes) {  // no room: a direct copy
         FCC_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, s));
@@ -43,6 +47,15 @@ void DevPool::sync(hipStream_t s) {
 void *DevPool::get(size_t bytes) {
     if (bytes == 0) bytes = 16;
     bytes = (bytes + 255) & ~size_t(255);
+    if (arena_) {
+        if (bytes > arena_bytes_ - arena_used_)
+            throw LibError{ERR_INVALID, "the contraction needs more than the " + std::to_string(arena_bytes_ >> 20) +
+                                            " MiB no-allocation scratch (" + std::to_string(arena_used_) +
+                                            " bytes taken, " + std::to_string(bytes) + " more asked)"};
+        void *p = arena_ + arena_used_;
+        arena_used_ += bytes;
+        return p;
+    }
     auto it = free_.lower_bound(bytes);
     if (it != free_.end() && it->first <= bytes * 2 + (1u << 20)) {
         void *p = it->second;

@@ -23,8 +23,9 @@ EXPORTS = (
     "fcc_contract_batch", "fcc_result_to_host", "fcc_result_free", "fcc_mlp_param_count", "fcc_mlp_set_params",
     "fcc_mlp_get_params", "fcc_tile_stats_count", "fcc_tile_stats_copy", "fcc_mlp_train",
     "fcc_mlp_forward_backward", "fcc_density_bucket", "fcc_tune_set", "fcc_tune_get",
-    "fcc_set_memory_budget", "fcc_memory_stats",
+    "fcc_set_memory_budget", "fcc_memory_stats", "fcc_contract_noalloc",
 )
+FCC_NOALLOC_SCRATCH_BYTES = 64 << 20
 
 
 class FccTensor(C.Structure):
@@ -212,6 +213,10 @@ def load(path: str | None = None):
     lib.fcc_plan_execute.restype = C.c_int
     lib.fcc_plan_destroy.argtypes = [P]
     lib.fcc_plan_destroy.restype = None
+    lib.fcc_contract_noalloc.argtypes = [P, C.POINTER(FccTensor), C.POINTER(C.c_int32), C.POINTER(FccTensor),
+                                         C.POINTER(C.c_int32), C.c_int32, C.POINTER(FccOptions), P, P, C.c_int64,
+                                         C.POINTER(FccResult)]
+    lib.fcc_contract_noalloc.restype = C.c_int
     lib.fcc_set_memory_budget.argtypes = [P, C.c_uint64]
     lib.fcc_set_memory_budget.restype = C.c_int
     lib.fcc_memory_stats.argtypes = [P, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]

@@ -150,6 +150,26 @@ int fcc_memory_stats(const fcc_ctx *ctx, uint64_t *held, uint64_t *in_use);
 int fcc_contract(fcc_ctx *ctx, const fcc_tensor *a, const int32_t *a_modes, const fcc_tensor *b,
                  const int32_t *b_modes, int32_t n_contr, const fcc_options *options, fcc_result *out);
 
+/* No counterpart in the reference. For callers that cannot allocate device
+ * memory -- code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc: fcc_contract that allocates nothing. Every device
+ * temporary comes from a fixed scratch array of FCC_NOALLOC_SCRATCH_BYTES
+ * inside the library (one per device, held until the process ends); a
+ * contraction that needs more is rejected with FCC_ERR_INVALID and a message
+ * naming the limit. Outputs go to the caller's device buffers: out_coords
+ * holds (output order) rows of out_capacity entries (SoA, coord_stride ==
+ * out_capacity), out_vals out_capacity values; more outputs than that give
+ * FCC_ERR_OVERFLOW with the needed count in out->nnz. In count + checksum
+ * mode they may be NULL. The dense-key GEMM and collect_stats are not used
+ * here; outputs equal fcc_contract's. out views the caller's buffers
+ * (fcc_result_free on it releases nothing). Calls that overlap on one device
+ * (from any context) share the scratch and race; the scratch keeps the last
+ * call's contents and every call initialises what it reads. */
+#define FCC_NOALLOC_SCRATCH_BYTES (64u << 20)
+int fcc_contract_noalloc(fcc_ctx *ctx, const fcc_tensor *a, const int32_t *a_modes, const fcc_tensor *b,
+                         const int32_t *b_modes, int32_t n_contr, const fcc_options *options, uint32_t *out_coords,
+                         double *out_vals, int64_t out_capacity, fcc_result *out);
+
 /* Replaces Tensor<double>::multiply<double,double>(Tensor<double>& other,
  *     CoOrdinate left_contr, CoOrdinate left_batch, CoOrdinate right_contr,
  *     CoOrdinate right_batch) -- contract.hpp:285-367 -- on the FaSTCC tiled
