@@ -19,6 +19,7 @@
 #pragma once
 
 #include <cstdint>
+#include <cstdlib>
 #include <fstream>
 #include <initializer_list>
 #include <memory>
@@ -137,8 +138,10 @@ template <class DT> class Tensor {
     Tensor(int order, std::vector<uint32_t> coords_soa, std::vector<DT> vals)
         : order_(order), coords_(std::move(coords_soa)), vals_(std::move(vals)) {}
     // read.hpp:58-106: FROSTT text; lines not starting with a digit are
-    // skipped, has_header skips the first numeric line, the last token is the
-    // value, coordinates are kept as written.
+    // skipped, has_header skips the first numeric line, fields are separated
+    // by single spaces, the last field is the value, coordinates are kept as
+    // written. A field that is empty (a doubled or trailing space) or not a
+    // number is rejected, where the reference's std::stoi / std::stod throw.
     explicit Tensor(const std::string &fname, bool has_header = false) {
         std::ifstream in(fname);
         if (!in) throw fastcc_error(FCC_ERR_INVALID, "cannot open " + fname);
@@ -151,18 +154,35 @@ template <class DT> class Tensor {
                 skipped = true;
                 continue;
             }
-            std::istringstream ss(line);
             std::vector<std::string> tok;
-            std::string t;
-            while (ss >> t) tok.push_back(t);
-            if (tok.size() < 2) continue;
+            for (size_t b = 0;;) {
+                const size_t e = line.find(' ', b);
+                tok.push_back(line.substr(b, e == std::string::npos ? std::string::npos : e - b));
+                if (e == std::string::npos) break;
+                b = e + 1;
+            }
+            auto bad = [&](const std::string &why) {
+                return fastcc_error(FCC_ERR_INVALID, fname + ": " + why + ": \"" + line + "\"");
+            };
+            if (tok.size() < 2) throw bad("a nonzero needs coordinates and a value");
             int d = (int)tok.size() - 1;
             if (order_ == 0) order_ = d;
-            if (d != order_) throw fastcc_error(FCC_ERR_INVALID, "inconsistent order in " + fname);
+            if (d != order_) throw bad("inconsistent order");
             std::vector<uint32_t> c(d);
-            for (int m = 0; m < d; m++) c[m] = (uint32_t)std::stoul(tok[m]);
+            for (int m = 0; m < d; m++) {
+                const std::string &f = tok[m];
+                if (f.empty() || f.size() > 10 || f.find_first_not_of("0123456789") != std::string::npos)
+                    throw bad("coordinate field is not an unsigned integer");
+                const unsigned long long x = std::stoull(f);
+                if (x > 0xFFFFFFFFull) throw bad("coordinate exceeds 32 bits");
+                c[m] = (uint32_t)x;
+            }
+            const std::string &vf = tok[d];
+            char *end = nullptr;
+            const double v = vf.empty() ? 0.0 : std::strtod(vf.c_str(), &end);
+            if (vf.empty() || end != vf.c_str() + vf.size()) throw bad("value field is not a number");
             rows.push_back(std::move(c));
-            vals_.push_back(std::stod(tok[d]));
+            vals_.push_back(v);
         }
         coords_.resize((size_t)order_ * rows.size());
         for (size_t n = 0; n < rows.size(); n++)

@@ -76,33 +76,46 @@ def test_write_read_round_trip():
     assert np.allclose(r6.vals, vals, rtol=1e-5, atol=0) and not np.array_equal(r6.vals, vals)
 
 
-def test_cpp_header_reads_the_same():
-    tmp = tempfile.mkdtemp(prefix="fcc_tns_")
-    p = _write(tmp, SAMPLE)
-    src = os.path.join(tmp, "rd.cpp")
-    with open(src, "w") as f:
-        f.write(r'''
+MALFORMED = ["1  2 3.0\n", "1 2 3.0 \n", "1 x 3.0\n", "1 2 3.0\n1 2 3 4.0\n", "1 2 3.0x\n", "1 2\n3 4 5.0\n"]
+
+
+@pytest.fixture(scope="module")
+def cpp_reader(tmp_path_factory):
+    tmp = tmp_path_factory.mktemp("rd")
+    src = tmp / "rd.cpp"
+    src.write_text(r'''
 #include <cstdio>
 #include "fastcc.hpp"
 int main(int argc, char **argv) {
-    Tensor<double> t(argv[1], true);
-    const auto &c = t.coords_soa();
-    const auto &v = t.values();
-    size_t n = v.size();
-    std::printf("%d %zu\n", t.get_dimensionality(), n);
-    for (size_t i = 0; i < n; i++) {
-        for (int m = 0; m < t.get_dimensionality(); m++) std::printf("%u ", c[(size_t)m * n + i]);
-        std::printf("%.17g\n", v[i]);
+    try {
+        Tensor<double> t(argv[1], argv[2][0] == '1');
+        const auto &c = t.coords_soa();
+        const auto &v = t.values();
+        size_t n = v.size();
+        std::printf("%d %zu\n", t.get_dimensionality(), n);
+        for (size_t i = 0; i < n; i++) {
+            for (int m = 0; m < t.get_dimensionality(); m++) std::printf("%u ", c[(size_t)m * n + i]);
+            std::printf("%.17g\n", v[i]);
+        }
+    } catch (const fastcc_error &e) {
+        std::fprintf(stderr, "%s\n", e.what());
+        return 2;
     }
     return 0;
 }
 ''')
-    exe = os.path.join(tmp, "rd")
+    exe = tmp / "rd"
     host = os.path.join(ROOT, "fastcc-benchmark_amd", "host")
     lib = os.path.join(ROOT, "fastcc-benchmark_amd")
-    subprocess.run(["g++", "-std=c++20", "-I", os.path.join(ROOT, "include"), "-I", host, "-o", exe, src, "-L", lib,
-                    "-lfastcc_hip", f"-Wl,-rpath,{lib}"], check=True)
-    out = subprocess.run([exe, p], capture_output=True, text=True, check=True).stdout.split("\n")
+    subprocess.run(["g++", "-std=c++20", "-I", os.path.join(ROOT, "include"), "-I", host, "-o", str(exe), str(src),
+                    "-L", lib, "-lfastcc_hip", f"-Wl,-rpath,{lib}"], check=True)
+    return str(exe)
+
+
+def test_cpp_header_reads_the_same(cpp_reader):
+    tmp = tempfile.mkdtemp(prefix="fcc_tns_")
+    p = _write(tmp, SAMPLE)
+    out = subprocess.run([cpp_reader, p, "1"], capture_output=True, text=True, check=True).stdout.split("\n")
     order, n = map(int, out[0].split())
     t = read_tns(p, has_header=True)
     assert order == t.order and n == t.nnz
@@ -110,3 +123,16 @@ int main(int argc, char **argv) {
         f = out[1 + i].split()
         assert [int(x) for x in f[:-1]] == t.coords[:, i].tolist()
         assert float(f[-1]) == t.vals[i]
+
+
+@pytest.mark.parametrize("text", MALFORMED)
+def test_cpp_header_rejects_malformed_lines(cpp_reader, text):
+    """Doubled and trailing spaces, non-numeric fields and a changing order:
+    the C++ reader refuses them like the Python reader (and the reference's
+    std::stoi / std::stod)."""
+    tmp = tempfile.mkdtemp(prefix="fcc_tns_")
+    p = _write(tmp, text)
+    with pytest.raises(ValueError):
+        read_tns(p)
+    r = subprocess.run([cpp_reader, p, "0"], capture_output=True, text=True)
+    assert r.returncode == 2, (text, r.stdout, r.stderr)
