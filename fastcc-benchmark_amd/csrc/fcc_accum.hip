@@ -688,11 +688,12 @@ __device__ __forceinline__ void heavy_claimed(const AccArgs &A, const UnitCtx &U
         uint32_t s_l = d.x, e_l = d.x + d.y;
         const bool cuts = U.multi && !seg && A.fcut && U.kind == KIND_DENSE;
         if (cuts && lane < nb && d.y) {
-            // precomputed window cuts of the group's fiber
+            // precomputed window cuts of the group's fiber (in mirror mode the
+            // descriptor may start later: no cell left of the tile's first row)
             const uint32_t w = U.c0 / A.fcut_W;
             const uint32_t *fc = A.fcut + (size_t)A.group_fiber[bg + lane] * A.fcut_n + w;
-            s_l = fc[0];
             e_l = fc[1];
+            s_l = min(max(fc[0], d.x), e_l);
         } else if (U.multi && lane < nb && s_l < e_l) {
             // segments wholly outside the window are dropped on their end columns
             if (A.Bcol[s_l] >= cend || A.Bcol[e_l - 1] < U.c0) {
@@ -702,9 +703,7 @@ __device__ __forceinline__ void heavy_claimed(const AccArgs &A, const UnitCtx &U
                 e_l = lane_lower_bound(A.Bcol, s_l, e_l, cend);
             }
         }
-        // mirror mode: no cell left of the tile's first row is kept
-        if (A.mirror && U.row0 > U.c0 && lane < nb && s_l < e_l)
-            s_l = lane_lower_bound(A.Bcol, s_l, e_l, (uint32_t)U.row0);
+
         if (__ballot(lane < nb && d.w > 64) != 0) {
             // rows beyond one wavefront: group by group
             for (uint32_t j = 0; j < nb; j++) {
@@ -766,9 +765,12 @@ __device__ __forceinline__ void light_entries(const AccArgs &A, const UnitCtx &U
             const uint32_t *fc = A.fcut + (size_t)A.Afib[e] * A.fcut_n + U.c0 / A.fcut_W;
             a = fc[0];
             b = fc[1];
-            if (A.dk_tile) {  // entries of dense keys carry an empty fiber range
+            if (A.dk_tile || A.mirror) {
+                // entries of dense keys carry an empty fiber range; in mirror
+                // mode the range starts at the entry's own column
                 const uint2 fp = A.Afp[e];
                 if (fp.x == fp.y) b = a;
+                if (A.mirror) a = min(max(a, fp.x), b);
             }
         } else {
             const uint2 fp = A.Afp[e];
@@ -779,8 +781,6 @@ __device__ __forceinline__ void light_entries(const AccArgs &A, const UnitCtx &U
                 b = lane_lower_bound(A.Bcol, a, b, cend);
             }
         }
-        // mirror mode: no cell left of the tile's first row is kept
-        if (A.mirror && U.row0 > U.c0) a = lane_lower_bound(A.Bcol, a, b, (uint32_t)U.row0);
         s0 = a;
         len = b - a;
     };
@@ -901,10 +901,7 @@ __device__ __forceinline__ void direct_run(const AccArgs &A, const UnitCtx &U, c
         uint32_t s0 = 0, len = 0;
         if (e < e1) {
             const uint2 fp = A.Afp[e];
-            s0 = fp.x;
-            // mirror mode: only the entry's columns at or right of its row
-            // (its row is distinct within the tile, so this is exact)
-            if (A.mirror) s0 = lane_lower_bound(A.Bcol, fp.x, fp.y, A.Arow[e]);
+            s0 = fp.x;  // mirror mode: the range starts at the entry's own column
             len = fp.y - s0;
         }
         uint32_t inc = len;

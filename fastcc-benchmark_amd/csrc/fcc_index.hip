@@ -25,6 +25,11 @@ static unsigned fill_grid(size_t n) {
     size_t b = div_up(n ? n : 1, 256);
     return (unsigned)(b > 8192 ? 8192 : b);
 }
+__global__ void k_map_u32(const uint32_t *map, const uint32_t *idx, size_t n, uint32_t *out) {
+    const size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+    if (i < n) out[i] = map[idx[i]];
+}
+
 __global__ void k_perm_gather(const double *src, const uint32_t *perm, size_t n, double *dst) {
     for (size_t p = blockIdx.x * (size_t)blockDim.x + threadIdx.x; p < n; p += (size_t)gridDim.x * blockDim.x)
         dst[p] = src[perm[p]];
@@ -270,9 +275,13 @@ __global__ void k_comp_A(const uint32_t *rowA, const uint32_t *fidA, const uint3
     idx[n] = (uint32_t)n;
 }
 
+// Mirror mode (self-contraction): idx is the entry's own position in the right
+// operand's (key, column) order, i.e. inside its fiber at the column equal to
+// its row, so its fiber range starts there: the cells left of the diagonal are
+// cut without a search.
 __global__ void k_gather_A(const uint64_t *comp, const uint32_t *idx, const double *vals, const uint32_t *fiber_ptr,
-                           size_t n, int logTL, int fbits, uint32_t *Ali, uint32_t *Arow, double *Aval, uint32_t *Afib,
-                           uint2 *Afp, uint64_t *gtflag) {
+                           size_t n, int logTL, int fbits, bool mirror, uint32_t *Ali, uint32_t *Arow, double *Aval,
+                           uint32_t *Afib, uint2 *Afp, uint64_t *gtflag) {
     size_t p = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
     if (p >= n) return;
     uint64_t c = comp[p];
@@ -280,7 +289,7 @@ __global__ void k_gather_A(const uint64_t *comp, const uint32_t *idx, const doub
     Arow[p] = (uint32_t)(((c >> (1 + fbits + logTL)) << logTL) | (c & ((1ull << logTL) - 1)));
     uint32_t f = (uint32_t)((c >> logTL) & ((1ull << fbits) - 1));
     Afib[p] = f;
-    Afp[p] = make_uint2(fiber_ptr[f], fiber_ptr[f + 1]);
+    Afp[p] = make_uint2(mirror ? idx[p] : fiber_ptr[f], fiber_ptr[f + 1]);
     Aval[p] = vals[idx[p]];
     uint64_t prev = p ? comp[p - 1] : ~0ull;
     // group start (low word) and tile start (high word): one scan of the
@@ -376,8 +385,8 @@ __global__ void k_tile_direct(const uint32_t *atile_g0, const uint32_t *group_pt
 
 // Mirror mode: outputs of each DIRECT tile. Its entries sit on distinct rows,
 // so every product is a distinct cell; entry (row r, fiber f) keeps the fiber
-// columns c >= r (fibers are sorted by column: a lower bound), each written
-// twice except the diagonal one. Other tiles keep their products.
+// columns c >= r (its fiber range already starts there), each written twice
+// except the diagonal one. Other tiles keep their products.
 __global__ void k_direct_outputs(const uint32_t *atile_g0, const uint32_t *group_ptr, const uint32_t *Arow,
                                  const uint2 *Afp, const uint32_t *Bcol, const uint8_t *st_direct,
                                  const uint64_t *st_products, uint32_t natiles, uint64_t *st_outd) {
@@ -390,15 +399,8 @@ __global__ void k_direct_outputs(const uint32_t *atile_g0, const uint32_t *group
     const uint32_t e0 = group_ptr[atile_g0[a]], e1 = group_ptr[atile_g0[a + 1]];
     uint64_t n = 0;
     for (uint32_t e = e0; e < e1; e++) {
-        const uint2 fp = Afp[e];
-        const uint32_t r = Arow[e];
-        uint32_t lo = fp.x, hi = fp.y;
-        while (lo < hi) {
-            const uint32_t mid = (lo + hi) >> 1;
-            if (Bcol[mid] < r) lo = mid + 1;
-            else hi = mid;
-        }
-        if (lo < fp.y) n += 2ull * (fp.y - lo) - (Bcol[lo] == r ? 1u : 0u);
+        const uint2 fp = Afp[e];  // starts at the entry's own column (its row)
+        if (fp.x < fp.y) n += 2ull * (fp.y - fp.x) - (Bcol[fp.x] == Arow[e] ? 1u : 0u);
     }
     st_outd[a] = n;
 }
@@ -438,9 +440,15 @@ __global__ void k_fiber_sketch(const uint32_t *fiber_ptr, const uint32_t *Bcol, 
 //  sumfib   summed right-fiber lengths
 //  products sum of m * |fiber| = an upper bound on the tile's output count
 //  sketch   two min-hashes over sampled fiber columns (distinct-column estimate)
+// Mirror mode (Afp_clip != null): a group's descriptor starts at its first
+// entry's clipped fiber start. The group's entries are the fiber's only
+// elements in the tile's columns (a self-contraction: fiber k holds exactly
+// the rows with key k), so that is the fiber's first column at or right of
+// the tile's first row. The statistics keep the whole fibers.
 __global__ __launch_bounds__(256) void k_tile_stats(const uint32_t *group_ptr, const uint32_t *group_fiber,
                                                    const uint32_t *group_atile, const uint32_t *fiber_ptr,
-                                                   const uint2 *fsk, uint32_t ngroups, uint4 *gdesc, uint32_t *st_nnz,
+                                                   const uint2 *fsk, const uint2 *Afp_clip, uint32_t ngroups,
+                                                   uint4 *gdesc, uint32_t *st_nnz,
                                                    uint32_t *st_keys, unsigned long long *st_sumfib,
                                                    unsigned long long *st_products, uint32_t *st_sketch,
                                                    unsigned long long *st_maxgp) {
@@ -457,9 +465,10 @@ __global__ __launch_bounds__(256) void k_tile_stats(const uint32_t *group_ptr, c
         const uint32_t e0 = group_ptr[g];
         m = group_ptr[g + 1] - e0;
         uint32_t f = group_fiber[g];
-        const uint32_t fp0 = fiber_ptr[f];
-        nf = fiber_ptr[f + 1] - fp0;
-        gdesc[g] = make_uint4(fp0, (uint32_t)nf, e0, (uint32_t)m);
+        const uint32_t fp0 = fiber_ptr[f], fp1 = fiber_ptr[f + 1];
+        nf = fp1 - fp0;
+        const uint32_t c0 = Afp_clip ? Afp_clip[e0].x : fp0;
+        gdesc[g] = make_uint4(c0, fp1 - c0, e0, (uint32_t)m);
         if (fsk) {
             const uint2 hh = fsk[f];
             h1 = hh.x;
@@ -750,9 +759,12 @@ void build_hseg(DevPool &pool, Index &ix, std::vector<std::shared_ptr<void>> &ho
     k_tile_hseg<<<grid_for((size_t)ix.natiles + 1), 256, 0, s>>>(ix.atile_g0, off, ix.natiles, ix.atile_h0);
 }
 
-static void left_from_sorted(DevPool &pool, const Operand &A, const uint64_t *ks, const uint32_t *vs, uint32_t nvalid,
-                             int logTL, int fbits, Index &ix, std::vector<std::shared_ptr<void>> &hold,
-                             hipStream_t s) {
+// vals: the values vs indexes (A's, or for a self-contraction the right
+// operand's sorted values, vs then holding right positions); perm_map (when
+// not null) maps vs to A's entry order for plans.
+static void left_from_sorted(DevPool &pool, const double *vals, const uint64_t *ks, const uint32_t *vs,
+                             const uint32_t *perm_map, uint32_t nvalid, int logTL, int fbits, Index &ix,
+                             std::vector<std::shared_ptr<void>> &hold, hipStream_t s) {
     auto keep = [&](size_t bytes) -> void * {
         void *p = pool.get(bytes + 16);
         DevPool *pl = &pool;
@@ -767,12 +779,12 @@ static void left_from_sorted(DevPool &pool, const Operand &A, const uint64_t *ks
     Tmp<uint64_t> gtflag(pool, m), gtid(pool, m), tot(pool, 1);
     ix.Afp = (uint2 *)keep(m * 8);
     if (m)
-        k_gather_A<<<grid_for(m), 256, 0, s>>>(ks, vs, A.vals, ix.fiber_ptr, m, logTL, fbits, ix.Ali, ix.Arow, ix.Aval,
-                                               ix.Afib,
-                                               ix.Afp, gtflag);
+        k_gather_A<<<grid_for(m), 256, 0, s>>>(ks, vs, vals, ix.fiber_ptr, m, logTL, fbits, ix.mirror, ix.Ali, ix.Arow,
+                                               ix.Aval, ix.Afib, ix.Afp, gtflag);
     if (ix.keep_perm) {
         ix.permA = (uint32_t *)keep(m * 4);
-        if (m) FCC_HIP(hipMemcpyAsync(ix.permA, vs, m * 4, hipMemcpyDeviceToDevice, s));
+        if (m && perm_map) k_map_u32<<<grid_for(m), 256, 0, s>>>(perm_map, vs, m, ix.permA);
+        else if (m) FCC_HIP(hipMemcpyAsync(ix.permA, vs, m * 4, hipMemcpyDeviceToDevice, s));
     }
     scan_u64(pool, gtflag, gtid, m, tot.get(), s);
     uint64_t h = 0;
@@ -821,7 +833,9 @@ static void left_from_sorted(DevPool &pool, const Operand &A, const uint64_t *ks
         if (ix.features && ix.nfib)
             k_fiber_sketch<<<grid_for(ix.nfib), 256, 0, s>>>(ix.fiber_ptr, ix.Bcol, ix.nfib, fsk);
         k_tile_stats<<<grid_for(ix.ngroups), 256, 0, s>>>(ix.group_ptr, ix.group_fiber, ix.group_atile, ix.fiber_ptr,
-                                                          ix.features ? fsk.get() : nullptr, ix.ngroups, ix.gdesc, ix.st_nnz, ix.st_keys,
+                                                          ix.features ? fsk.get() : nullptr,
+                                                          ix.mirror ? ix.Afp : nullptr, ix.ngroups, ix.gdesc,
+                                                          ix.st_nnz, ix.st_keys,
                                                           (unsigned long long *)ix.st_sumfib,
                                                           (unsigned long long *)ix.st_products, ix.st_sketch,
                                                           (unsigned long long *)ix.st_maxgp);
@@ -864,25 +878,24 @@ void build_left(DevPool &pool, const Geom &g, const Operand &A, const uint32_t *
     uint32_t *vs;
     radix_sort_pairs(pool, k1, v1, k2, v2, n, std::min(nbits, 64), s, &ks, &vs);
     if (after_sort) (*after_sort)();
-    left_from_sorted(pool, A, ks, vs, nvalid, logTL, fbits, ix, hold, s);
+    left_from_sorted(pool, A.vals, ks, vs, nullptr, nvalid, logTL, fbits, ix, hold, s);
 }
 
 // Self-contraction: the right operand is already sorted by (fiber, column)
 // and column == left row, so a stable sort on (tile, class) alone yields the
 // left order (tile, class, fiber, row-in-tile). The whole composite key and
-// the entry index are written here in the right operand's order (coalesced
-// reads) and the sort touches only the (tile, class) bits, so nothing has to
-// be gathered back through the sort permutation afterwards.
-__global__ void k_self_composite(const uint32_t *Bcol, const uint32_t *bfib, const uint32_t *vsB,
-                                 const uint32_t *fiber_ptr, size_t n, int logTL, int fbits, uint32_t heavy_min,
-                                 uint64_t *ks, uint32_t *vs) {
+// the entry's position in the right operand's order are written here in that
+// order (coalesced reads) and the sort touches only the (tile, class) bits;
+// values are then read from the right operand's sorted values.
+__global__ void k_self_composite(const uint32_t *Bcol, const uint32_t *bfib, const uint32_t *fiber_ptr, size_t n,
+                                 int logTL, int fbits, uint32_t heavy_min, uint64_t *ks, uint32_t *vs) {
     size_t p = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
     if (p >= n) return;
     const uint32_t f = bfib[p], c = Bcol[p];
     const uint64_t cls = (fiber_ptr[f + 1] - fiber_ptr[f]) >= heavy_min ? 0 : 1;
     const uint64_t tile = c >> logTL, li = c & ((1u << logTL) - 1);
     ks[p] = (tile << (1 + fbits + logTL)) | (cls << (fbits + logTL)) | ((uint64_t)f << logTL) | li;
-    vs[p] = vsB[p];
+    vs[p] = (uint32_t)p;
 }
 
 void build_left_self(DevPool &pool, const Geom &g, const Operand &A, const SelfJoin &sj, int logTL, Index &ix,
@@ -896,13 +909,13 @@ void build_left_self(DevPool &pool, const Geom &g, const Operand &A, const SelfJ
     if (tbits + 1 + fbits + logTL + 1 > 64) throw LibError{ERR_INVALID, "left composite key exceeds 64 bits"};
     Tmp<uint64_t> k1(pool, n), k2(pool, n);
     Tmp<uint32_t> v1(pool, n), v2(pool, n);
-    k_self_composite<<<grid_for(n), 256, 0, s>>>(ix.Bcol, sj.bfib, sj.vsB, ix.fiber_ptr, n, logTL, fbits,
-                                                 ix.heavy_min, k1, v1);
+    k_self_composite<<<grid_for(n), 256, 0, s>>>(ix.Bcol, sj.bfib, ix.fiber_ptr, n, logTL, fbits, ix.heavy_min, k1,
+                                                 v1);
     uint64_t *ks;
     uint32_t *vs;
     radix_sort_pairs(pool, k1, v1, k2, v2, n, tbits + 1, s, &ks, &vs, fbits + logTL);
     if (after_sort) (*after_sort)();
-    left_from_sorted(pool, A, ks, vs, (uint32_t)n, logTL, fbits, ix, hold, s);
+    left_from_sorted(pool, ix.Bval, ks, vs, sj.vsB, (uint32_t)n, logTL, fbits, ix, hold, s);
 }
 
 }  // namespace fcc
