@@ -231,9 +231,15 @@ def main():
         ms = {args.planner: dt * 1e3, other: odt * 1e3}
         lst = stats if args.planner == "learned" else ostats
         est = ostats if args.planner == "learned" else stats
+        # the exact planner judges the network's tile-level error (untimed):
+        # with mirroring, an extra exact count over a sample of tiles
+        r = ctx.contract(da, list(wl.a_modes), db, list(wl.b_modes), planner="learned", judge=True, **opts)
+        judged = r.stats
+        r.free()
         compare = {"exact_ms_per_step": ms["exact"], "learned_ms_per_step": ms["learned"],
                    "learned_speedup": ms["exact"] / ms["learned"],
-                   "learned_mlp_mape": lst[-1]["mlp_mape"],
+                   "learned_mlp_mape": judged["mlp_mape"], "mape_tiles": int(judged["mape_tiles"]),
+                   "left_tiles": int(judged["left_tiles"]),
                    "exact_phases_ms": phases(est), "learned_phases_ms": phases(lst),
                    "exact_reruns": int(est[-1]["reruns"]), "learned_reruns": int(lst[-1]["reruns"])}
 

@@ -686,6 +686,7 @@ __device__ __forceinline__ void heavy_claimed(const AccArgs &A, const UnitCtx &U
         uint4 d = make_uint4(0, 0, 0, 0);
         if (lane < nb) d = desc[bg + lane];
         uint32_t s_l = d.x, e_l = d.x + d.y;
+        if (A.unclipped && lane < nb && d.y) s_l = A.fiber_ptr[A.group_fiber[bg + lane]];
         const bool cuts = U.multi && !seg && A.fcut && U.kind == KIND_DENSE;
         if (cuts && lane < nb && d.y) {
             // precomputed window cuts of the group's fiber (in mirror mode the
@@ -693,7 +694,7 @@ __device__ __forceinline__ void heavy_claimed(const AccArgs &A, const UnitCtx &U
             const uint32_t w = U.c0 / A.fcut_W;
             const uint32_t *fc = A.fcut + (size_t)A.group_fiber[bg + lane] * A.fcut_n + w;
             e_l = fc[1];
-            s_l = min(max(fc[0], d.x), e_l);
+            s_l = A.mirror ? min(max(fc[0], d.x), e_l) : fc[0];
         } else if (U.multi && lane < nb && s_l < e_l) {
             // segments wholly outside the window are dropped on their end columns
             if (A.Bcol[s_l] >= cend || A.Bcol[e_l - 1] < U.c0) {
@@ -774,7 +775,7 @@ __device__ __forceinline__ void light_entries(const AccArgs &A, const UnitCtx &U
             }
         } else {
             const uint2 fp = A.Afp[e];
-            a = fp.x;
+            a = (A.unclipped && fp.x != fp.y) ? A.fiber_ptr[A.Afib[e]] : fp.x;
             b = fp.y;
             if (U.multi) {
                 a = lane_lower_bound(A.Bcol, a, b, U.c0);

@@ -342,6 +342,33 @@ __global__ void k_mape(const float *pred, const unsigned long long *actual, uint
     if ((threadIdx.x & 63) == 0 && e != 0.0) unsafeAtomicAdd(acc, e);
 }
 
+// The same over every S-th tile (acc[1] counts them).
+__global__ void k_mape_sampled(const float *pred, const unsigned long long *actual, uint32_t n, uint32_t S,
+                               double *acc) {
+    uint32_t a = blockIdx.x * blockDim.x + threadIdx.x;
+    double e = 0.0, c = 0.0;
+    if (a < n && a % S == 0) {
+        const double act = (double)actual[a];
+        e = fabs((double)pred[a] - act) / (act > 1.0 ? act : 1.0);
+        c = 1.0;
+    }
+    for (int d = 32; d > 0; d >>= 1) {
+        e += __shfl_xor(e, d, 64);
+        c += __shfl_xor(c, d, 64);
+    }
+    if ((threadIdx.x & 63) == 0 && c != 0.0) {
+        unsafeAtomicAdd(&acc[0], e);
+        unsafeAtomicAdd(&acc[1], c);
+    }
+}
+
+// Keeps the windows of every S-th tile (DIRECT tiles excepted: their count is
+// their products), clears the rest: the units of a sampled exact count.
+__global__ void k_sample_tiles(uint32_t *t_nwin, const uint8_t *t_kind, uint32_t n, uint32_t S) {
+    uint32_t a = blockIdx.x * blockDim.x + threadIdx.x;
+    if (a < n && (a % S != 0 || t_kind[a] == KIND_DIRECT)) t_nwin[a] = 0;
+}
+
 // Sum of predicted tile outputs, each capped by its exact bounds (products, area).
 __global__ void k_pred_sum(const float *pred, const uint64_t *st_products, const uint64_t *st_maxgp,
                            const uint8_t *st_direct, const uint64_t *st_outd, double area, uint32_t n,
@@ -596,6 +623,19 @@ int Contraction::run(const fcc_tensor *ta, const int32_t *am, const fcc_tensor *
             if (st) (void)hipStreamSynchronize(st);
         }
     } side_join;
+    // dense column windows: every fiber's window cuts, once, instead of a
+    // binary search per group per window unit (bounded table size). They
+    // depend on the right index alone and are built ahead of the left tiles
+    // (with a dense-key block the main stream has time to spare: the
+    // accumulate pass waits for the block's GEMM on the second stream).
+    uint32_t fcut_n = 0;
+    auto make_fiber_cuts = [&](uint32_t Wd) {
+        if (g.spanB <= Wd) return;
+        const uint64_t nw = (g.spanB + Wd - 1) / Wd + 1;
+        if ((uint64_t)ix.nfib * nw > (1ull << 28)) return;
+        k.fcut = Tmp<uint32_t>(pool, (size_t)ix.nfib * nw);
+        fcut_n = (uint32_t)nw;
+    };
     std::function<void()> fork_dense = [&]() {
         dense_keys_alloc(pool, g, dk, ctx->num_cus, hold);
         FCC_HIP(hipEventRecord(ev[9], s));
@@ -610,6 +650,8 @@ int Contraction::run(const fcc_tensor *ta, const int32_t *am, const fcc_tensor *
     int logTL;
     uint32_t Wd;
     choose_tiles(ctx, o, g, products, nvalid, dk.nh > 0, &logTL, &Wd, &bucket, &tuned_wg);
+    make_fiber_cuts(Wd);
+    if (k.fcut.get()) launch_fiber_cuts(ix, fcut_n, Wd, k.fcut, s);
     // mirror mode: one tensor, the same mode lists, every left tile (a
     // sample of rows has no symmetric output) and no statistics collection
     // (its labels are whole-tile output counts)
@@ -617,7 +659,9 @@ int Contraction::run(const fcc_tensor *ta, const int32_t *am, const fcc_tensor *
     ix.mirror = mirror;
     if (const char *hm = getenv("FCC_HEAVY_MIN")) ix.heavy_min = (uint32_t)std::max(1, atoi(hm));
     if (const char *hs = getenv("FCC_HSEG")) ix.hseg = (uint32_t)std::max(64, atoi(hs));
-    ix.defer_hseg = dk.nh > 0;  // built once, after the dense keys are masked
+    ix.dk_hkey = dk.nh ? dk.hkey : nullptr;  // the left build takes these keys out of the tiles
+    ix.seg_below = 2 * (ACC_THREADS / 64);
+    if (const char *sb = getenv("FCC_SEG_BELOW")) ix.seg_below = (uint32_t)std::max(0, atoi(sb));
     ix.features = o.planner == FCC_PLAN_LEARNED || o.collect_stats;
     // A very large GEMM outlasts the whole left-tile build, so it starts
     // before the left sort; a smaller one waits for the sort to be queued
@@ -639,7 +683,7 @@ int Contraction::run(const fcc_tensor *ta, const int32_t *am, const fcc_tensor *
     st.left_tiles = ix.natiles;
     st.mirrored = mirror ? 1 : 0;
 
-    dense_keys_mask(pool, ix, dk, hold, s);
+    dk.tile_dk = ix.tile_dk;
     // Planning and the count pass read only the block's presence map (ready
     // before the GEMM starts, event 10); the numeric pass reads its values and
     // joins the GEMM itself (event 8), so the GEMM runs on beside the count.
@@ -717,18 +761,11 @@ int Contraction::run(const fcc_tensor *ta, const int32_t *am, const fcc_tensor *
     X.Aval = ix.Aval;
     X.Afp = ix.Afp;
     X.Afib = ix.Afib;
-    // dense column windows: every fiber's window cuts, once, instead of a
-    // binary search per group per window unit (bounded table size)
     auto &fcut = k.fcut;
-    if (g.spanB > Wd) {
-        const uint64_t nw = (g.spanB + Wd - 1) / Wd + 1;
-        if ((uint64_t)ix.nfib * nw <= (1ull << 28)) {
-            fcut = Tmp<uint32_t>(pool, (size_t)ix.nfib * nw);
-            launch_fiber_cuts(ix, (uint32_t)nw, Wd, fcut, s);
-            X.fcut = fcut;
-            X.fcut_n = (uint32_t)nw;
-            X.fcut_W = Wd;
-        }
+    if (fcut.get()) {
+        X.fcut = fcut;
+        X.fcut_n = fcut_n;
+        X.fcut_W = Wd;
     }
     X.group_ptr = ix.group_ptr;
     X.gdesc = ix.gdesc;
@@ -747,8 +784,7 @@ int Contraction::run(const fcc_tensor *ta, const int32_t *am, const fcc_tensor *
     X.mirror = mirror ? 1 : 0;
     X.claim_div = 2 * (ACC_THREADS / 64);
     if (const char *cd = getenv("FCC_CLAIM_DIV")) X.claim_div = (uint32_t)std::max(0, atoi(cd));
-    X.seg_below = 2 * (ACC_THREADS / 64);
-    if (const char *sb = getenv("FCC_SEG_BELOW")) X.seg_below = (uint32_t)std::max(0, atoi(sb));
+    X.seg_below = ix.seg_below;
     X.logTL = ix.logTL;
     X.spanB = g.spanB;
     X.spanA = g.spanA;
@@ -943,11 +979,13 @@ int Contraction::run(const fcc_tensor *ta, const int32_t *am, const fcc_tensor *
             FCC_HIP(hipEventRecord(ev[6], s));
             // tile-level MAPE against the actual per-unit outputs (kept only
             // when this pass is the final one)
-            fill_u64(tn, T, 0, s);
-            k_unit_to_tile<<<grid256(U.n), 256, 0, s>>>(unit_cur, U.tile, U.kind, U.n, tn);
-            k_direct_tile_count<<<grid256(T), 256, 0, s>>>(ix.st_direct, (const uint64_t *)ix.st_outd, T,
-                                                           P.force, tn);
-            k_mape<<<grid256(T), 256, 0, s>>>(pred_t, tn, T, reinterpret_cast<double *>(tail.get() + 8));
+            if (!mirror) {
+                fill_u64(tn, T, 0, s);
+                k_unit_to_tile<<<grid256(U.n), 256, 0, s>>>(unit_cur, U.tile, U.kind, U.n, tn);
+                k_direct_tile_count<<<grid256(T), 256, 0, s>>>(ix.st_direct, (const uint64_t *)ix.st_outd, T,
+                                                               P.force, tn);
+                k_mape<<<grid256(T), 256, 0, s>>>(pred_t, tn, T, reinterpret_cast<double *>(tail.get() + 8));
+            }
             fetch_tail();
             st.ghash_entries = hm[0];
             if (hf[0]) {
@@ -973,6 +1011,75 @@ int Contraction::run(const fcc_tensor *ta, const int32_t *am, const fcc_tensor *
             nnz_out = sink ? 0 : hm[1];
             mape_pending = !mirror;  // per-tile outputs = the whole tile band only without mirroring
             break;
+        }
+        if (!use_exact && mirror && o.judge) {
+            // A mirrored tile's outputs are not its whole band, the quantity
+            // the network predicts: the exact planner judges the predictions
+            // on a sample of about 256 tiles instead, with a count pass that
+            // counts every output of their bands.
+            const uint32_t S = std::max<uint32_t>(1, T / 256);
+            PlanParams P2 = P;
+            P2.mirror = 0;
+            P2.run = 0;
+            P2.dense_ratio = 0.5f;
+            Tmp<uint32_t> n2(pool, T), W2(pool, T), b2(pool, T), w2(pool, T), tot2(pool, 1);
+            Tmp<uint8_t> k2(pool, T);
+            launch_plan_tiles(ix, nullptr, P2, n2, W2, k2, w2, s);
+            k_sample_tiles<<<grid256(T), 256, 0, s>>>(n2, k2, T, S);
+            scan_u32(pool, n2, b2, T, tot2, s);
+            uint32_t nu = 0;
+            pool.read(&nu, tot2.get(), 4, s);
+            pool.sync(s);
+            Units U2;
+            Tmp<uint32_t> u2t(pool, nu), u2c(pool, nu), u2w(pool, nu);
+            Tmp<uint8_t> u2k(pool, nu), u2m(pool, nu);
+            U2.n = nu;
+            U2.tile = u2t;
+            U2.c0 = u2c;
+            U2.W = u2w;
+            U2.kind = u2k;
+            U2.multi = u2m;
+            Tmp<unsigned long long> cnt2(pool, nu), tn2(pool, T);
+            Tmp<unsigned int> work2(pool, 4);
+            Tmp<double> acc2(pool, 2);
+            if (nu) launch_emit_units(n2, W2, k2, b2, w2, T, g.spanB, nullptr, U2, s);
+            AccArgs XS = X;
+            XS.mirror = 0;
+            XS.unclipped = 1;
+            XS.seg_below = 0;  // no heavy segments: they were cut from clipped ranges
+            XS.nunits = nu;
+            XS.unit_tile = U2.tile;
+            XS.unit_c0 = U2.c0;
+            XS.unit_W = U2.W;
+            XS.unit_kind = U2.kind;
+            XS.unit_multi = U2.multi;
+            XS.unit_order = nullptr;
+            XS.unit_count = cnt2;
+            XS.work = work2;
+            XS.set_words = SYM_SLOTS;
+            FillSet f;
+            f.add(XS.gkeys, XS.gmask + 1, ~0ull, 8);
+            f.add(gmeta.get(), 4, 0, 8);
+            f.add(flags.get(), 4, 0, 4);
+            f.add(work2.get(), 4, 0, 4);
+            f.add(cnt2.get(), nu, 0, 8);
+            f.add(tn2.get(), T, 0, 8);
+            f.add(acc2.get(), 2, 0, 8);
+            fill_many(f, s);
+            if (nu) launch_count(XS, std::max(1u, std::min<unsigned>((unsigned)ctx->num_cus * 2, nu)), s);
+            if (nu) k_unit_to_tile<<<grid256(nu), 256, 0, s>>>(cnt2, U2.tile, U2.kind, nu, tn2);
+            k_direct_tile_count<<<grid256(T), 256, 0, s>>>(ix.st_direct, (const uint64_t *)ix.st_products, T,
+                                                           P.force, tn2);
+            k_mape_sampled<<<grid256(T), 256, 0, s>>>(pred_t, tn2, T, S, acc2);
+            double m2[2] = {0.0, 0.0};
+            uint32_t gf2 = 0;
+            pool.read(m2, acc2.get(), 16, s);
+            pool.read(&gf2, flags.get(), 4, s);
+            pool.sync(s);
+            if (!gf2 && m2[1] > 0) {
+                st.mlp_mape = m2[0] / m2[1];
+                st.mape_tiles = (uint64_t)m2[1];
+            }
         }
     }
     if (use_exact) {
@@ -1064,6 +1171,7 @@ int Contraction::run(const fcc_tensor *ta, const int32_t *am, const fcc_tensor *
         double m;
         memcpy(&m, &th[8], 8);
         st.mlp_mape = m / (double)T;
+        st.mape_tiles = T;
     }
     if (sink) {
         nnz_out = sk[0];

@@ -67,13 +67,6 @@ void dense_keys_block(const Index &ix, const DenseKeys &dk, const uint32_t *fidA
                       const double *Avals, size_t nA, hipStream_t s, hipEvent_t gemm0 = nullptr,
                       hipEvent_t gemm1 = nullptr);
 
-// After the left tiles: takes the selected keys out of the sparse traversal
-// (group descriptors, light-entry fiber ranges, DIRECT eligibility; heavy
-// segments built here, the left-tile build having deferred them) and flags the
-// tiles that hold them.
-void dense_keys_mask(DevPool &pool, Index &ix, DenseKeys &dk, std::vector<std::shared_ptr<void>> &hold,
-                     hipStream_t s);
-
 // Values again (plans): scatters the selected keys' current values (ix.Aval,
 // ix.Bval, index order) into the panels and runs the GEMM into dk.C. No memset nodes and no
 // allocation: safe to capture into a graph.

@@ -67,38 +67,6 @@ __global__ void k_dk_keys(const uint32_t *flag, const uint32_t *idx, uint32_t nf
     if (sel && idx[f] < cap) hfib[idx[f]] = f;
 }
 
-// Takes the selected keys' groups out of the sparse traversal: a heavy group
-// gets an empty fiber in its descriptor, a light group's entries get empty
-// fiber ranges; their tiles lose DIRECT eligibility (their outputs meet the
-// dense block). Sets the row bitsets.
-__global__ void k_dk_mask(uint32_t ngroups, const uint32_t *group_ptr, const uint32_t *group_fiber,
-                          const uint32_t *group_atile, const uint8_t *glight, const uint32_t *hkey,
-                          const uint32_t *Arow, uint32_t nw, uint4 *gdesc, uint2 *Afp, uint8_t *st_direct,
-                          uint8_t *tile_dk, unsigned long long *rowbits) {
-    const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
-    if (g >= ngroups) return;
-    const uint32_t h = hkey[group_fiber[g]];
-    if (h == 0xFFFFFFFFu) return;
-    const uint32_t a = group_atile[g];
-    const uint32_t e0 = group_ptr[g], e1 = group_ptr[g + 1];
-    if (glight[g]) {
-        for (uint32_t e = e0; e < e1; e++) {
-            uint2 fp = Afp[e];
-            fp.y = fp.x;
-            Afp[e] = fp;
-        }
-    } else {
-        uint4 d = gdesc[g];
-        d.y = 0;
-        gdesc[g] = d;
-    }
-    st_direct[a] = 0;
-    tile_dk[a] = 1;
-    if (!rowbits) return;  // self-contraction: the row bitsets are the column bitsets
-    const unsigned long long bit = 1ull << (h & 63);
-    for (uint32_t e = e0; e < e1; e++) atomicOr(&rowbits[(uint64_t)Arow[e] * nw + (h >> 6)], bit);
-}
-
 // one workgroup per selected key: column bitsets
 __global__ __launch_bounds__(256) void k_dk_colbits(const uint32_t *hfib, const uint32_t *fiber_ptr,
                                                    const uint32_t *Bcol, uint64_t rb, unsigned long long *colbits) {
@@ -513,26 +481,6 @@ void dense_keys_block(const Index &ix, const DenseKeys &dk, const uint32_t *fidA
     if (gemm0) FCC_HIP(hipEventRecord(gemm0, s));
     launch_dgemm_tn(dk.AT, dk.Bd, dk.C, ra, rb, dk.kp, dk.splits, dk.self, s);
     if (gemm1) FCC_HIP(hipEventRecord(gemm1, s));
-}
-
-void dense_keys_mask(DevPool &pool, Index &ix, DenseKeys &dk, std::vector<std::shared_ptr<void>> &hold,
-                     hipStream_t s) {
-    if (!dk.nh) return;
-    auto keep = [&](size_t bytes) -> void * {
-        void *p = pool.get(bytes + 16);
-        DevPool *pl = &pool;
-        hold.emplace_back(p, [pl](void *q) { pl->put(q); });
-        return p;
-    };
-    dk.tile_dk = (uint8_t *)keep((size_t)ix.natiles + 1);
-    fill_u32(reinterpret_cast<uint32_t *>(dk.tile_dk), ((size_t)ix.natiles + 1 + 3) / 4, 0, s);
-    if (ix.ngroups)
-        k_dk_mask<<<grid_of(ix.ngroups), 256, 0, s>>>(ix.ngroups, ix.group_ptr, ix.group_fiber, ix.group_atile,
-                                                       ix.glight, dk.hkey, ix.Arow, dk.nw, ix.gdesc, ix.Afp,
-                                                       ix.st_direct, dk.tile_dk, nullptr);
-    // heavy segments from the masked descriptors (the left-tile build
-    // deferred them: ix.defer_hseg)
-    build_hseg(pool, ix, hold, s);
 }
 
 void dense_keys_values(const Index &ix, const DenseKeys &dk, hipStream_t s) {

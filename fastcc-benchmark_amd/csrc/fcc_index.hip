@@ -333,41 +333,47 @@ __global__ void k_groups(const uint64_t *comp, const uint64_t *gtflag, const uin
     }
 }
 
-// segments per group: heavy groups are cut every HSEG fiber entries, light
-// groups have none
-__global__ void k_hseg_count(const uint4 *gdesc, const uint8_t *glight, uint32_t ngroups, uint32_t hseg,
-                             uint32_t *nseg) {
-    uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
-    if (g >= ngroups) return;
-    nseg[g] = glight[g] ? 0u : (gdesc[g].y + hseg - 1) / hseg;
-}
-
-__global__ void k_hseg_emit(const uint4 *gdesc, const uint32_t *off, const uint32_t *nseg, uint32_t ngroups,
-                            uint32_t hseg, uint4 *hdesc) {
-    uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
-    if (g >= ngroups) return;
-    const uint4 d = gdesc[g];
-    const uint32_t n = nseg[g], o = off[g];
-    for (uint32_t k = 0; k < n; k++) {
-        const uint32_t s0 = k * hseg;
-        hdesc[o + k] = make_uint4(d.x + s0, min(hseg, d.y - s0), d.z, d.w);
+// Heavy segments per active tile: a tile with fewer than seg_below heavy
+// groups has its heavy groups cut every hseg fiber entries (its accumulate
+// units spread long fibers over many wavefronts); other tiles get none.
+__global__ void k_hseg_tile_count(const uint32_t *atile_g0, const uint32_t *atile_lg, const uint4 *gdesc,
+                                  uint32_t natiles, uint32_t hseg, uint32_t seg_below, uint32_t *nseg) {
+    const uint32_t a = blockIdx.x * blockDim.x + threadIdx.x;
+    if (a > natiles) return;
+    uint32_t n = 0;
+    if (a < natiles) {
+        const uint32_t g0 = atile_g0[a], gl = atile_lg[a];
+        if (gl - g0 < seg_below)
+            for (uint32_t g = g0; g < gl; g++) n += (gdesc[g].y + hseg - 1) / hseg;
     }
+    nseg[a] = n;  // nseg[natiles] = 0: the scan's total lands in atile_h0[natiles]
 }
 
-__global__ void k_tile_hseg(const uint32_t *atile_g0, const uint32_t *off, uint32_t natiles, uint32_t *atile_h0) {
-    uint32_t a = blockIdx.x * blockDim.x + threadIdx.x;
-    if (a <= natiles) atile_h0[a] = off[atile_g0[a]];
+__global__ void k_hseg_tile_emit(const uint32_t *atile_g0, const uint32_t *atile_lg, const uint4 *gdesc,
+                                 const uint32_t *atile_h0, uint32_t natiles, uint32_t hseg, uint32_t seg_below,
+                                 uint4 *hdesc) {
+    const uint32_t a = blockIdx.x * blockDim.x + threadIdx.x;
+    if (a >= natiles) return;
+    const uint32_t g0 = atile_g0[a], gl = atile_lg[a];
+    if (gl - g0 >= seg_below) return;
+    uint32_t o = atile_h0[a];
+    for (uint32_t g = g0; g < gl; g++) {
+        const uint4 d = gdesc[g];
+        for (uint32_t s0 = 0; s0 < d.y; s0 += hseg) hdesc[o++] = make_uint4(d.x + s0, min(hseg, d.y - s0), d.z, d.w);
+    }
 }
 
 // A tile is DIRECT-able when no two of its entries share a row (then no two
 // products meet at one output). Checked for tiles of at most 64 rows holding
 // at most one entry per row.
+// (never a tile holding a dense key: its outputs meet the dense block)
 __global__ void k_tile_direct(const uint32_t *atile_g0, const uint32_t *group_ptr, const uint32_t *Ali,
-                              const uint32_t *st_nnz, uint32_t natiles, uint32_t TL, uint8_t *direct) {
+                              const uint32_t *st_nnz, const uint8_t *tile_dk, uint32_t natiles, uint32_t TL,
+                              uint8_t *direct) {
     uint32_t a = blockIdx.x * blockDim.x + threadIdx.x;
     if (a >= natiles) return;
     uint8_t ok = 0;
-    if (TL <= DIRECT_MAX_TL && st_nnz[a] <= TL) {
+    if (TL <= DIRECT_MAX_TL && st_nnz[a] <= TL && !(tile_dk && tile_dk[a])) {
         const uint32_t e0 = group_ptr[atile_g0[a]], e1 = group_ptr[atile_g0[a + 1]];
         uint64_t seen = 0;
         ok = 1;
@@ -445,10 +451,15 @@ __global__ void k_fiber_sketch(const uint32_t *fiber_ptr, const uint32_t *Bcol, 
 // elements in the tile's columns (a self-contraction: fiber k holds exactly
 // the rows with key k), so that is the fiber's first column at or right of
 // the tile's first row. The statistics keep the whole fibers.
+// Dense keys (dk_hkey != null): a group whose key the dense GEMM takes leaves
+// the tile traversal -- a heavy group gets an empty fiber in its descriptor, a
+// light group's entries get empty fiber ranges -- and its tile is flagged in
+// tile_dk (its outputs meet the dense block). The statistics keep it.
 __global__ __launch_bounds__(256) void k_tile_stats(const uint32_t *group_ptr, const uint32_t *group_fiber,
                                                    const uint32_t *group_atile, const uint32_t *fiber_ptr,
-                                                   const uint2 *fsk, const uint2 *Afp_clip, uint32_t ngroups,
-                                                   uint4 *gdesc, uint32_t *st_nnz,
+                                                   const uint2 *fsk, bool mirror, uint2 *Afp,
+                                                   const uint32_t *dk_hkey, const uint8_t *glight, uint8_t *tile_dk,
+                                                   uint32_t ngroups, uint4 *gdesc, uint32_t *st_nnz,
                                                    uint32_t *st_keys, unsigned long long *st_sumfib,
                                                    unsigned long long *st_products, uint32_t *st_sketch,
                                                    unsigned long long *st_maxgp) {
@@ -467,8 +478,18 @@ __global__ __launch_bounds__(256) void k_tile_stats(const uint32_t *group_ptr, c
         uint32_t f = group_fiber[g];
         const uint32_t fp0 = fiber_ptr[f], fp1 = fiber_ptr[f + 1];
         nf = fp1 - fp0;
-        const uint32_t c0 = Afp_clip ? Afp_clip[e0].x : fp0;
-        gdesc[g] = make_uint4(c0, fp1 - c0, e0, (uint32_t)m);
+        const uint32_t c0 = mirror ? Afp[e0].x : fp0;
+        const bool dense = dk_hkey && dk_hkey[f] != 0xFFFFFFFFu;
+        gdesc[g] = make_uint4(c0, dense ? 0u : fp1 - c0, e0, (uint32_t)m);
+        if (dense) {
+            tile_dk[a] = 1;
+            if (glight[g])
+                for (uint32_t e = e0; e < e0 + (uint32_t)m; e++) {
+                    uint2 fp = Afp[e];
+                    fp.y = fp.x;
+                    Afp[e] = fp;
+                }
+        }
         if (fsk) {
             const uint2 hh = fsk[f];
             h1 = hh.x;
@@ -746,17 +767,18 @@ void build_hseg(DevPool &pool, Index &ix, std::vector<std::shared_ptr<void>> &ho
         hold.emplace_back(p, [pl](void *q) { pl->put(q); });
         return p;
     };
-    Tmp<uint32_t> nseg(pool, (size_t)ix.ngroups + 1), off(pool, (size_t)ix.ngroups + 1), nh(pool, 1);
-    if (ix.ngroups) k_hseg_count<<<grid_for(ix.ngroups), 256, 0, s>>>(ix.gdesc, ix.glight, ix.ngroups, ix.hseg, nseg);
-    fill_u32(nseg.get() + ix.ngroups, 1, 0, s);
-    scan_u32(pool, nseg, off, (size_t)ix.ngroups + 1, nh, s);
+    const size_t T = ix.natiles;
+    Tmp<uint32_t> nseg(pool, T + 1), nh(pool, 1);
+    ix.atile_h0 = (uint32_t *)keep((T + 1) * 4);
+    k_hseg_tile_count<<<grid_for(T + 1), 256, 0, s>>>(ix.atile_g0, ix.atile_lg, ix.gdesc, (uint32_t)T, ix.hseg,
+                                                      ix.seg_below, nseg);
+    scan_u32(pool, nseg, ix.atile_h0, T + 1, nh, s);
     pool.read(&ix.nhseg, nh.get(), 4, s);
     pool.sync(s);
     ix.hdesc = (uint4 *)keep((size_t)ix.nhseg * 16);
-    ix.atile_h0 = (uint32_t *)keep(((size_t)ix.natiles + 1) * 4);
-    if (ix.ngroups)
-        k_hseg_emit<<<grid_for(ix.ngroups), 256, 0, s>>>(ix.gdesc, off, nseg, ix.ngroups, ix.hseg, ix.hdesc);
-    k_tile_hseg<<<grid_for((size_t)ix.natiles + 1), 256, 0, s>>>(ix.atile_g0, off, ix.natiles, ix.atile_h0);
+    if (T && ix.nhseg)
+        k_hseg_tile_emit<<<grid_for(T), 256, 0, s>>>(ix.atile_g0, ix.atile_lg, ix.gdesc, ix.atile_h0, (uint32_t)T,
+                                                     ix.hseg, ix.seg_below, ix.hdesc);
 }
 
 // vals: the values vs indexes (A's, or for a self-contraction the right
@@ -812,6 +834,10 @@ static void left_from_sorted(DevPool &pool, const double *vals, const uint64_t *
     }
     // descriptors + statistics (one pass over the groups)
     size_t T = ix.natiles;
+    if (ix.dk_hkey) {
+        ix.tile_dk = (uint8_t *)keep(T + 4);
+        fill_u32(reinterpret_cast<uint32_t *>(ix.tile_dk), (T + 4) / 4, 0, s);
+    }
     ix.st_nnz = (uint32_t *)keep(T * 4);
     ix.st_keys = (uint32_t *)keep(T * 4);
     ix.st_sumfib = (uint64_t *)keep(T * 8);
@@ -833,18 +859,18 @@ static void left_from_sorted(DevPool &pool, const double *vals, const uint64_t *
         if (ix.features && ix.nfib)
             k_fiber_sketch<<<grid_for(ix.nfib), 256, 0, s>>>(ix.fiber_ptr, ix.Bcol, ix.nfib, fsk);
         k_tile_stats<<<grid_for(ix.ngroups), 256, 0, s>>>(ix.group_ptr, ix.group_fiber, ix.group_atile, ix.fiber_ptr,
-                                                          ix.features ? fsk.get() : nullptr,
-                                                          ix.mirror ? ix.Afp : nullptr, ix.ngroups, ix.gdesc,
+                                                          ix.features ? fsk.get() : nullptr, ix.mirror, ix.Afp,
+                                                          ix.dk_hkey, ix.glight, ix.tile_dk, ix.ngroups, ix.gdesc,
                                                           ix.st_nnz, ix.st_keys,
                                                           (unsigned long long *)ix.st_sumfib,
                                                           (unsigned long long *)ix.st_products, ix.st_sketch,
                                                           (unsigned long long *)ix.st_maxgp);
     }
-    if (!ix.defer_hseg) build_hseg(pool, ix, hold, s);
+    build_hseg(pool, ix, hold, s);
     ix.st_direct = (uint8_t *)keep(T + 1);
     if (T)
-        k_tile_direct<<<grid_for(T), 256, 0, s>>>(ix.atile_g0, ix.group_ptr, ix.Ali, ix.st_nnz, (uint32_t)T, ix.TL,
-                                                   ix.st_direct);
+        k_tile_direct<<<grid_for(T), 256, 0, s>>>(ix.atile_g0, ix.group_ptr, ix.Ali, ix.st_nnz, ix.tile_dk,
+                                                   (uint32_t)T, ix.TL, ix.st_direct);
     // DIRECT tiles' outputs: their products, or in mirror mode the products
     // at or above the diagonal counted twice off it
     ix.st_outd = ix.st_products;

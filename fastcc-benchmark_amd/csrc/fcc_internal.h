@@ -199,7 +199,13 @@ struct Index {
     // [atile_h0[a], atile_h0[a + 1])
     uint32_t nhseg = 0;
     uint32_t hseg = HSEG;             // fiber entries per segment
-    bool defer_hseg = false;          // the left-tile build leaves hdesc to a later build_hseg
+    uint32_t seg_below = 16;          // only tiles with fewer heavy groups than this get segments
+                                      // (the accumulate kernels walk the others group by group)
+    // dense keys (dk_hkey[fiber] != ~0u: contracted by the dense GEMM): the
+    // left build takes their groups out of the tile traversal and flags
+    // their tiles in tile_dk (natiles + 1 bytes); null: no dense keys
+    const uint32_t *dk_hkey = nullptr;
+    uint8_t *tile_dk = nullptr;
     uint4 *hdesc = nullptr;           // per segment: fiber start, length, first entry, entries
     uint32_t *atile_h0 = nullptr;     // natiles+1
     // per active tile statistics (planner features)
@@ -325,6 +331,10 @@ struct AccArgs {
     // c >= r are accumulated; each is written as (r, c) and, off the
     // diagonal, as (c, r)
     int mirror;
+    // the index was built for mirroring (fiber ranges start at the entries'
+    // own columns) but this pass wants whole fibers: ranges start at the
+    // fiber's first entry (the sampled exact count that judges the planner)
+    int unclipped;
 };
 
 // Mirror mode: outputs written for cell (r, c): none below the diagonal, one

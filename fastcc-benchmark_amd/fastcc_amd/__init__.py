@@ -226,7 +226,8 @@ class Context:
 
     @staticmethod
     def options(planner="exact", tile_hint=0, sink=False, force_acc=FCC_ACC_AUTO, col_window=0, workgroups=0,
-                collect_stats=False, left_tile_stride=1, summarize=False, dense_keys=0.0, mirror=True):
+                collect_stats=False, left_tile_stride=1, summarize=False, dense_keys=0.0, mirror=True,
+                judge=False):
         o = _abi.FccOptions()
         _abi.load().fcc_options_default(C.byref(o))
         o.planner = FCC_PLAN_LEARNED if planner == "learned" else FCC_PLAN_EXACT
@@ -242,6 +243,8 @@ class Context:
         o.dense_keys = float(dense_keys)
         # self-contractions: compute each unordered output pair once (default) or every output
         o.mirror = 0 if mirror else -1
+        # learned planner with mirroring: judge predictions by a sampled exact count
+        o.judge = 1 if judge else 0
         return o
 
     def contract(self, a, a_modes, b, b_modes, a_batch=(), b_batch=(), **opts) -> Result:

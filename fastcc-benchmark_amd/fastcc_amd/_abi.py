@@ -52,6 +52,7 @@ class FccOptions(C.Structure):
         ("summarize", C.c_int32),
         ("dense_keys", C.c_float),
         ("mirror", C.c_int32),
+        ("judge", C.c_int32),
     ]
 
 
@@ -86,6 +87,7 @@ class FccStats(C.Structure):
         ("dense_block_bytes", C.c_uint64),
         ("units_block_counted", C.c_uint64),
         ("mirrored", C.c_int32),
+        ("mape_tiles", C.c_uint64),
     ]
 
 

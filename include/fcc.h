@@ -69,6 +69,10 @@ typedef struct {
                             C(x, y) = C(y, x), so 0 (default) computes each unordered pair once and
                             writes it at both coordinates; -1 computes every output. Same outputs
                             either way. Off with left_tile_stride > 1 or collect_stats. */
+    int32_t judge;       /* learned planner with mirroring: 1 = judge the predictions with an extra
+                            exact count pass over a sample of about 256 tiles (stats.mlp_mape,
+                            mape_tiles); 0 = not (mlp_mape = -1). Without mirroring the tile-level
+                            MAPE is always taken from the pass's own per-tile outputs. */
 } fcc_options;
 
 typedef struct {
@@ -104,6 +108,9 @@ typedef struct {
     uint64_t units_block_counted; /* exact planner: dense units the dense-key block covers
                                      entirely, counted from it without a traversal */
     int32_t mirrored;      /* 1: the symmetric output was computed once per unordered pair */
+    uint64_t mape_tiles;   /* tiles mlp_mape is taken over: all, or with mirroring (whose per-tile
+                              outputs are not whole-tile counts) a sample of about 256 tiles whose
+                              outputs an extra exact count pass counts */
 } fcc_stats;
 
 typedef struct {

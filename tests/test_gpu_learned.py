@@ -43,7 +43,7 @@ def test_learned_equals_oracle(trained_ctx, mk):
     a, b = wl.tensors()
     b = a if wl.self_contraction else b
     ref = oracle.contract_tiled(a, wl.a_modes, b, wl.b_modes, tile_size=128)
-    r = ctx.contract(a, list(wl.a_modes), b, list(wl.b_modes), planner="learned")
+    r = ctx.contract(a, list(wl.a_modes), b, list(wl.b_modes), planner="learned", judge=True)
     c, v = r.to_host()
     assert_same_result(c, v, ref.coords, ref.vals, rel=REL, abs_floor=ABS, what=wl.name, absum=ref.absum,
                            nterms=ref.nterms)
@@ -66,7 +66,7 @@ def test_any_prediction_stays_exact(bias):
     wl = W.c1(0.25)
     a, b = wl.tensors()
     ref = oracle.contract_tiled(a, wl.a_modes, b, wl.b_modes, tile_size=64)
-    r = ctx.contract(a, list(wl.a_modes), b, list(wl.b_modes), planner="learned")
+    r = ctx.contract(a, list(wl.a_modes), b, list(wl.b_modes), planner="learned", judge=True)
     c, v = r.to_host()
     assert_same_result(c, v, ref.coords, ref.vals, rel=REL, abs_floor=ABS, what=f"bias {bias}", absum=ref.absum,
                        nterms=ref.nterms)

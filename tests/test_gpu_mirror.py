@@ -145,3 +145,24 @@ def test_mirror_on_off_scaled(ctx, name, mk, planner):
     for (c, v), m in zip(outs, (True, False)):
         assert_same_result(c, v, ref.coords, ref.vals, rel=REL, abs_floor=ABS, what=f"{wl.name} mirror={m}",
                            absum=ref.absum, nterms=ref.nterms)
+
+
+@pytest.mark.parametrize("name,mk", [("c2", lambda: W.c2(0.1)), ("c3", lambda: W.c3(0.005)),
+                                     ("c4", lambda: W.c4(0.02))])
+def test_mirror_learned_mape_is_the_exact_judge(ctx, name, mk):
+    """With mirroring the learned planner's error is judged by a sampled exact
+    count pass (whole tile bands). When every tile is in the sample it must
+    equal the error taken from the unmirrored pass's own per-tile outputs."""
+    wl = mk()
+    a, _ = wl.tensors()
+    st = {}
+    for m in (True, False):
+        r = ctx.contract(a, list(wl.a_modes), a, list(wl.b_modes), planner="learned", mirror=m, sink=(name == "c3"),
+                         judge=True)
+        st[m] = r.stats
+        r.free()
+    print(name, {m: (st[m]["mlp_mape"], st[m]["mape_tiles"], st[m]["left_tiles"]) for m in st}, flush=True)
+    assert st[False]["mape_tiles"] == st[False]["left_tiles"] and st[False]["mlp_mape"] >= 0
+    assert 0 < st[True]["mape_tiles"] <= st[True]["left_tiles"] and st[True]["mlp_mape"] >= 0
+    if st[True]["mape_tiles"] == st[True]["left_tiles"]:
+        assert st[True]["mlp_mape"] == pytest.approx(st[False]["mlp_mape"], rel=1e-6, abs=1e-9)
