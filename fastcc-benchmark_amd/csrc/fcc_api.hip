@@ -625,10 +625,10 @@ int Contraction::run(const fcc_tensor *ta, const int32_t *am, const fcc_tensor *
     } side_join;
     // dense column windows: every fiber's window cuts, once, instead of a
     // binary search per group per window unit (bounded table size). They
-    // depend on the right index alone and are built ahead of the left tiles
-    // (with a dense-key block the main stream has time to spare: the
-    // accumulate pass waits for the block's GEMM on the second stream).
-    uint32_t fcut_n = 0;
+    // depend on the right index alone; with a dense-key block they are queued
+    // right after its fork, so the block starts as early as it can (the
+    // accumulate pass waits for the block's GEMM anyway).
+    uint32_t fcut_n = 0, fcut_W = 0;
     auto make_fiber_cuts = [&](uint32_t Wd) {
         if (g.spanB <= Wd) return;
         const uint64_t nw = (g.spanB + Wd - 1) / Wd + 1;
@@ -644,6 +644,7 @@ int Contraction::run(const fcc_tensor *ta, const int32_t *am, const fcc_tensor *
         FCC_HIP(hipEventRecord(ev[7], ctx->side));
         dense_keys_block(ix, dk, fidA.get(), rowA.get(), A.vals, (size_t)A.nnz, ctx->side, ev[10], ev[11]);
         FCC_HIP(hipEventRecord(ev[8], ctx->side));
+        if (k.fcut.get()) launch_fiber_cuts(ix, fcut_n, fcut_W, k.fcut, s);
     };
     if (!dk.nh)
         for (int e : {7, 8, 10, 11}) FCC_HIP(hipEventRecord(ev[e], s));
@@ -651,7 +652,8 @@ int Contraction::run(const fcc_tensor *ta, const int32_t *am, const fcc_tensor *
     uint32_t Wd;
     choose_tiles(ctx, o, g, products, nvalid, dk.nh > 0, &logTL, &Wd, &bucket, &tuned_wg);
     make_fiber_cuts(Wd);
-    if (k.fcut.get()) launch_fiber_cuts(ix, fcut_n, Wd, k.fcut, s);
+    fcut_W = Wd;
+    if (k.fcut.get() && !dk.nh) launch_fiber_cuts(ix, fcut_n, Wd, k.fcut, s);
     // mirror mode: one tensor, the same mode lists, every left tile (a
     // sample of rows has no symmetric output) and no statistics collection
     // (its labels are whole-tile output counts)

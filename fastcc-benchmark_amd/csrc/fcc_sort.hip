@@ -233,20 +233,26 @@ void scan_u64(DevPool &pool, const uint64_t *in, uint64_t *out, size_t n, uint64
 
 // -------------------------------------------------------------- radix sort
 // One pass = histogram per 4096-element tile, exclusive scan of the
-// digit-major histogram table, stable scatter. Stability inside a tile: the
-// tile is walked in 16 rounds of 256 consecutive elements; within a round a
-// wavefront ranks equal digits with 8 ballots (lane order), and the four
-// wavefronts are ordered through per-wave digit counts in LDS.
+// digit-major histogram table, stable scatter. Inside a tile each wavefront
+// owns 1024 consecutive elements and ranks them in 16 rounds of 64 (lane
+// order = element order): equal digits within a round are found with 8
+// ballots, and a wavefront-private LDS histogram carries the ranks across its
+// rounds without block barriers. One block-wide scan over (digit, wavefront)
+// then places every element; the tile is written out digit run by digit run.
 
 constexpr int RS_THREADS = 256;
 constexpr int RS_ITEMS = 16;
 constexpr int RS_TILE = RS_THREADS * RS_ITEMS;
+constexpr int RS_WAVES = RS_THREADS / 64;
+constexpr int RS_WAVE_ELEMS = RS_TILE / RS_WAVES;  // 1024
 
 __global__ __launch_bounds__(RS_THREADS) void k_rs_hist(const uint64_t *keys, size_t n, int shift, uint32_t *hist,
                                                         unsigned nblocks) {
-    __shared__ uint32_t h[256];
-    h[threadIdx.x] = 0;
+    // one histogram per wavefront: skewed digits (Zipf keys) contend less
+    __shared__ uint32_t h[RS_WAVES][256];
+    for (int w = 0; w < RS_WAVES; w++) h[w][threadIdx.x] = 0;
     __syncthreads();
+    const int wave = threadIdx.x >> 6;
     size_t t0 = (size_t)blockIdx.x * RS_TILE;
     uint32_t dg[RS_ITEMS];  // all loads in flight before the first atomic
 #pragma unroll
@@ -256,102 +262,104 @@ __global__ __launch_bounds__(RS_THREADS) void k_rs_hist(const uint64_t *keys, si
     }
 #pragma unroll
     for (int j = 0; j < RS_ITEMS; j++)
-        if (dg[j] < 256u) atomicAdd(&h[dg[j]], 1u);
+        if (dg[j] < 256u) atomicAdd(&h[wave][dg[j]], 1u);
     __syncthreads();
-    hist[(size_t)threadIdx.x * nblocks + blockIdx.x] = h[threadIdx.x];
+    uint32_t c = 0;
+#pragma unroll
+    for (int w = 0; w < RS_WAVES; w++) c += h[w][threadIdx.x];
+    hist[(size_t)threadIdx.x * nblocks + blockIdx.x] = c;
 }
 
 __global__ __launch_bounds__(RS_THREADS) void k_rs_scatter(const uint64_t *keys, const uint32_t *vals, uint64_t *okeys,
                                                            uint32_t *ovals, size_t n, int shift,
                                                            const uint32_t *hist_scan, unsigned nblocks) {
-    // The tile is first placed in LDS in stable digit order, then each digit's
-    // run is written to its global position by consecutive threads
-    // (coalesced), instead of scattering element by element to HBM.
     __shared__ uint64_t s_key[RS_TILE];
     __shared__ uint32_t s_val[RS_TILE];
-    __shared__ uint32_t s_gbase[256];   // global start of this tile's run of digit d
-    __shared__ uint32_t s_lbase[256];   // start of digit d's run inside the tile
-    __shared__ uint32_t s_run[256];     // elements of digit d placed so far
-    __shared__ uint32_t wcnt[RS_THREADS / 64][256];
+    __shared__ uint32_t s_wh[RS_WAVES][256];  // per wavefront: digit counts, then its digit bases
+    __shared__ uint32_t s_gbase[256];         // global start of this tile's run of digit d
+    __shared__ uint32_t s_lbase[256];         // start of digit d's run inside the tile
+    __shared__ uint32_t s_tot[RS_WAVES];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const size_t t0 = (size_t)blockIdx.x * RS_TILE;
     const uint32_t tn = (uint32_t)(n - t0 < (size_t)RS_TILE ? n - t0 : (size_t)RS_TILE);
-    // The whole tile is loaded into registers up front (item j of a thread is
-    // tile element j * RS_THREADS + tid): one memory latency per tile instead
-    // of one per ranking round, and no second read for the local histogram.
+    // wavefront w owns tile elements [w * 1024, (w + 1) * 1024); item j of a
+    // lane is element w * 1024 + j * 64 + lane (coalesced loads)
     uint64_t rk[RS_ITEMS];
     uint32_t rv[RS_ITEMS];
 #pragma unroll
     for (int j = 0; j < RS_ITEMS; j++) {
-        const uint32_t li = (uint32_t)j * RS_THREADS + tid;
+        const uint32_t li = (uint32_t)(wave * RS_WAVE_ELEMS + j * 64 + lane);
         rk[j] = li < tn ? keys[t0 + li] : 0ull;
         rv[j] = li < tn ? vals[t0 + li] : 0u;
     }
-    // local histogram -> exclusive scan (one digit per thread)
-    s_run[tid] = 0;
-    __syncthreads();
 #pragma unroll
-    for (int j = 0; j < RS_ITEMS; j++)
-        if ((uint32_t)j * RS_THREADS + tid < tn) atomicAdd(&s_run[(uint32_t)(rk[j] >> shift) & 255u], 1u);
+    for (int w = 0; w < RS_WAVES; w++) s_wh[w][tid] = 0;
     __syncthreads();
-    {
-        uint32_t c = s_run[tid];
-        uint32_t incl = c;
-        // block scan over 256 digits: wave scan, then combine 4 wave totals
-        for (int d = 1; d < 64; d <<= 1) {
-            uint32_t o = __shfl_up(incl, d, 64);
-            if (lane >= d) incl += o;
-        }
-        if (lane == 63) wcnt[0][wave] = incl;
-        __syncthreads();
-        uint32_t off = 0;
-        for (int w = 0; w < wave; w++) off += wcnt[0][w];
-        s_lbase[tid] = off + incl - c;
-        s_gbase[tid] = hist_scan[(size_t)tid * nblocks + blockIdx.x];
-        __syncthreads();
-        s_run[tid] = 0;
-#pragma unroll
-        for (int w = 0; w < RS_THREADS / 64; w++) wcnt[w][tid] = 0;
-    }
-    __syncthreads();
+    // ranks inside the wavefront's sub-tile: count of earlier equal digits
+    uint32_t *wh = s_wh[wave];
     const uint64_t lt_mask = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+    uint32_t rank[RS_ITEMS];
 #pragma unroll
     for (int j = 0; j < RS_ITEMS; j++) {
-        uint32_t li = (uint32_t)j * RS_THREADS + tid;
-        bool valid = li < tn;
-        const uint64_t k = rk[j];
-        const uint32_t v = rv[j];
-        uint32_t d = (uint32_t)(k >> shift) & 255u;
+        const bool valid = (uint32_t)(wave * RS_WAVE_ELEMS + j * 64 + lane) < tn;
+        const uint32_t d = (uint32_t)(rk[j] >> shift) & 255u;
         uint64_t peers = __ballot(valid);
 #pragma unroll
         for (int b = 0; b < 8; b++) {
-            bool bit = (d >> b) & 1u;
-            uint64_t bb = __ballot(bit);
+            const bool bit = (d >> b) & 1u;
+            const uint64_t bb = __ballot(bit);
             peers &= bit ? bb : ~bb;
         }
-        uint32_t rank = (uint32_t)__popcll(peers & lt_mask);
-        if (valid && rank == 0) wcnt[wave][d] = (uint32_t)__popcll(peers);
-        __syncthreads();
-        if (valid) {
-            uint32_t off = s_lbase[d] + s_run[d];
-            for (int w = 0; w < wave; w++) off += wcnt[w][d];
-            s_key[off + rank] = k;
-            s_val[off + rank] = v;
-        }
-        __syncthreads();
-        uint32_t t = 0;
-#pragma unroll
-        for (int w = 0; w < RS_THREADS / 64; w++) {
-            t += wcnt[w][tid];
-            wcnt[w][tid] = 0;
-        }
-        s_run[tid] += t;
-        __syncthreads();
+        const uint32_t before = (uint32_t)__popcll(peers & lt_mask);
+        // every lane reads the running count before the group's first lane
+        // adds the group (one wavefront: LDS operations stay in order)
+        const uint32_t run = valid ? wh[d] : 0u;
+        rank[j] = run + before;
+        if (valid && before == 0) wh[d] = run + (uint32_t)__popcll(peers);
     }
+    __syncthreads();
+    // digit `tid`: tile total, exclusive scan over digits, per-wavefront bases
+    {
+        uint32_t c[RS_WAVES], tot = 0;
+#pragma unroll
+        for (int w = 0; w < RS_WAVES; w++) {
+            c[w] = s_wh[w][tid];
+            tot += c[w];
+        }
+        uint32_t incl = tot;
+        for (int d = 1; d < 64; d <<= 1) {
+            const uint32_t o = __shfl_up(incl, d, 64);
+            if (lane >= d) incl += o;
+        }
+        if (lane == 63) s_tot[wave] = incl;
+        __syncthreads();
+        uint32_t off = 0;
+        for (int w = 0; w < wave; w++) off += s_tot[w];
+        const uint32_t lb = off + incl - tot;
+        s_lbase[tid] = lb;
+        s_gbase[tid] = hist_scan[(size_t)tid * nblocks + blockIdx.x];
+        uint32_t acc = lb;
+#pragma unroll
+        for (int w = 0; w < RS_WAVES; w++) {
+            s_wh[w][tid] = acc;
+            acc += c[w];
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < RS_ITEMS; j++) {
+        if ((uint32_t)(wave * RS_WAVE_ELEMS + j * 64 + lane) < tn) {
+            const uint32_t d = (uint32_t)(rk[j] >> shift) & 255u;
+            const uint32_t pos = wh[d] + rank[j];
+            s_key[pos] = rk[j];
+            s_val[pos] = rv[j];
+        }
+    }
+    __syncthreads();
     for (uint32_t i = tid; i < tn; i += RS_THREADS) {
-        uint64_t k = s_key[i];
-        uint32_t d = (uint32_t)(k >> shift) & 255u;
-        uint32_t pos = s_gbase[d] + (i - s_lbase[d]);
+        const uint64_t k = s_key[i];
+        const uint32_t d = (uint32_t)(k >> shift) & 255u;
+        const uint32_t pos = s_gbase[d] + (i - s_lbase[d]);
         okeys[pos] = k;
         ovals[pos] = s_val[i];
     }
