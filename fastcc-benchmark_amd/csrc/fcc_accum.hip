@@ -68,7 +68,7 @@ __device__ uint32_t wave_lower_bound(const uint32_t *arr, uint32_t lo, uint32_t 
 // the insert gives up (flag set) once the table is 7/8 full; the host then
 // grows the table or re-plans.
 __device__ __forceinline__ bool gtable_full(const AccArgs &A, int probe) {
-    if (probe < 64 || (probe & 63)) return false;
+    if (probe < 16 || (probe & 15)) return false;
     unsigned long long c = __hip_atomic_load(A.gcount, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     return c > ((A.gmask + 1) >> 3) * 7;
 }
@@ -307,6 +307,11 @@ __device__ __forceinline__ void direct_emit(const AccArgs &A, const UnitCtx &U, 
 template <bool NUM, bool BITMAP>
 __device__ __forceinline__ void accum(const AccArgs &A, const UnitCtx &U, const Acc<NUM, BITMAP> &S, uint32_t li,
                                       uint32_t col, double v) {
+    if (!NUM && U.kind == KIND_CBITS) {
+        const uint32_t slot = li * U.Wp + col;
+        atomicOr(&S.keys[slot >> 5], 1u << (slot & 31));
+        return;
+    }
     if (U.kind == KIND_DENSE) {
         uint32_t slot = li * U.Wp + col;
         if (!NUM) S.seen[slot] = 1;
@@ -1028,6 +1033,11 @@ __device__ __forceinline__ void dk_seed_cells(const AccArgs &A, const UnitCtx &U
         if (row >= A.spanA) continue;
         uint32_t w = dk_word(A, row, (uint64_t)U.c0 + cl0);
         if (U.W - cl0 < 32) w &= (1u << (U.W - cl0)) - 1u;
+        if (!NUM && U.kind == KIND_CBITS) {
+            // rows are padded to 32 cells: the word lands whole
+            if (w) atomicOr(&S.keys[(li * U.Wp + cl0) >> 5], w);
+            continue;
+        }
         while (w) {
             const uint32_t cl = cl0 + (uint32_t)__builtin_ctz(w);
             w &= w - 1;
@@ -1384,7 +1394,15 @@ __global__ __launch_bounds__(ACC_THREADS) void k_count(AccArgs A) {
         // through the global set (the same count)
         if (U.kind == KIND_LHASH && A.set_words < SYM_SLOTS) U.kind = KIND_GHASH;
         U.multi = A.unit_multi[U.u];
-        U.Wp = U.kind == KIND_DENSE ? ((U.W + 31) & ~31u) : U.W;
+        // A one-window (all columns) hash unit whose rows x padded width fit
+        // the launch's key-set LDS as one bit per cell counts through that
+        // bit set: the number of distinct cells does not depend on the
+        // accumulator the numeric pass will use, and an LDS bit beats a probe
+        // (for the global set, a compare-and-swap in HBM).
+        const uint64_t cells = ((uint64_t)1 << A.logTL) * (((uint64_t)U.W + 31) & ~31ull);
+        if ((U.kind == KIND_GHASH || U.kind == KIND_LHASH) && !U.multi && cells <= (uint64_t)A.set_words * 32)
+            U.kind = KIND_CBITS;
+        U.Wp = (U.kind == KIND_DENSE || U.kind == KIND_CBITS) ? ((U.W + 31) & ~31u) : U.W;
         U.row0 = (uint64_t)U.tile << A.logTL;
         if (U.kind == KIND_DIRECT) {  // every product is a distinct output
             if (threadIdx.x == 0) {
@@ -1444,6 +1462,8 @@ __global__ __launch_bounds__(ACC_THREADS) void k_count(AccArgs A) {
             } else {
                 for (uint32_t s = threadIdx.x; s < nwords; s += ACC_THREADS) s_set[s] = 0u;
             }
+        } else if (U.kind == KIND_CBITS) {
+            for (uint32_t s = threadIdx.x; s < TL * U.Wp / 32; s += ACC_THREADS) s_set[s] = 0u;
         } else if (U.kind == KIND_LHASH) {
             for (uint32_t s = threadIdx.x; s < SYM_SLOTS; s += ACC_THREADS) s_set[s] = LEMPTY;
         }
@@ -1471,6 +1491,33 @@ __global__ __launch_bounds__(ACC_THREADS) void k_count(AccArgs A) {
 #pragma unroll
                     for (int j = 0; j < 4; j++)
                         if ((wd >> (8 * j)) & 1u) part += mirror_weight(r, c0 + j);
+                }
+            }
+            for (int d = 32; d > 0; d >>= 1) part += __shfl_xor(part, d, 64);
+            if ((threadIdx.x & 63) == 0) s_w[threadIdx.x >> 6] = part;
+            __syncthreads();
+            c = 0;
+            for (int w = 0; w < NWAVES; w++) c += s_w[w];
+        } else if (U.kind == KIND_CBITS) {
+            uint32_t part = 0;
+            const uint32_t nb = TL * U.Wp / 32;
+            for (uint32_t s = threadIdx.x; s < nb; s += ACC_THREADS) {
+                const uint32_t wd = s_set[s];
+                if (!wd) continue;
+                if (!A.mirror) {
+                    part += __popc(wd);
+                    continue;
+                }
+                // a word's 32 cells share a row: twice right of the diagonal,
+                // once on it
+                const uint32_t li = (32 * s) / U.Wp, cl = 32 * s - li * U.Wp;
+                const uint64_t r = U.row0 + li, c0 = (uint64_t)U.c0 + cl;
+                if (c0 > r) {
+                    part += 2 * __popc(wd);
+                } else if (r - c0 < 32) {
+                    const uint32_t d = (uint32_t)(r - c0);
+                    const uint32_t gt = d == 31 ? 0u : (~0u << (d + 1));
+                    part += 2 * __popc(wd & gt) + ((wd >> d) & 1u);
                 }
             }
             for (int d = 32; d > 0; d >>= 1) part += __shfl_xor(part, d, 64);

@@ -458,6 +458,7 @@ struct Contraction {
         Tmp<uint32_t> out_c;
         Tmp<double> out_v;
         Tmp<float> pred;  // per-tile MLP predictions (learned planner)
+        Tmp<unsigned long long> tile_band;  // mirrored plans: exact outputs of each tile's band
         Tmp<uint32_t> fcut;  // window cuts of every fiber (dense column windows)
         DenseKeys dk;        // dense-key block (dk.nh == 0: none)
         AccArgs X;
@@ -950,6 +951,76 @@ int Contraction::run(const fcc_tensor *ta, const int32_t *am, const fcc_tensor *
     auto &out_c = k.out_c;
     auto &out_v = k.out_v;
     uint64_t stride = 0;
+    // Exact output counts of the whole bands of every S-th tile (0 for the
+    // others) into tn2: a count pass without mirroring over unclipped fiber
+    // ranges, on units planned from the exact bounds. Under mirroring a
+    // tile's own units write only part of its band (cells with column >= row,
+    // and their transposes), so per-tile counts need this pass. With grow,
+    // an overflowing global key set is enlarged and the pass repeated;
+    // without, overflow returns false.
+    auto band_counts = [&](uint32_t S, unsigned long long *tn2, bool grow) -> bool {
+        PlanParams P2 = P;
+        P2.mirror = 0;
+        P2.run = 0;
+        P2.dense_ratio = 0.5f;
+        Tmp<uint32_t> n2(pool, T), W2(pool, T), b2(pool, T), w2(pool, T), tot2(pool, 1);
+        Tmp<uint8_t> k2(pool, T);
+        launch_plan_tiles(ix, nullptr, P2, n2, W2, k2, w2, s);
+        k_sample_tiles<<<grid256(T), 256, 0, s>>>(n2, k2, T, S);
+        scan_u32(pool, n2, b2, T, tot2, s);
+        uint32_t nu = 0;
+        pool.read(&nu, tot2.get(), 4, s);
+        pool.sync(s);
+        Units U2;
+        Tmp<uint32_t> u2t(pool, nu), u2c(pool, nu), u2w(pool, nu);
+        Tmp<uint8_t> u2k(pool, nu), u2m(pool, nu);
+        U2.n = nu;
+        U2.tile = u2t;
+        U2.c0 = u2c;
+        U2.W = u2w;
+        U2.kind = u2k;
+        U2.multi = u2m;
+        Tmp<unsigned long long> cnt2(pool, nu);
+        Tmp<unsigned int> work2(pool, 4);
+        if (nu) launch_emit_units(n2, W2, k2, b2, w2, T, g.spanB, nullptr, U2, s);
+        for (int attempt = 0;; attempt++) {
+            AccArgs XS = X;
+            XS.mirror = 0;
+            XS.unclipped = 1;
+            XS.seg_below = 0;  // no heavy segments: they were cut from clipped ranges
+            XS.nunits = nu;
+            XS.unit_tile = U2.tile;
+            XS.unit_c0 = U2.c0;
+            XS.unit_W = U2.W;
+            XS.unit_kind = U2.kind;
+            XS.unit_multi = U2.multi;
+            XS.unit_order = nullptr;
+            XS.unit_count = cnt2;
+            XS.work = work2;
+            XS.set_words = SYM_SLOTS;
+            FillSet f;
+            f.add(XS.gkeys, XS.gmask + 1, ~0ull, 8);
+            f.add(gmeta.get(), 4, 0, 8);
+            f.add(flags.get(), 4, 0, 4);
+            f.add(work2.get(), 4, 0, 4);
+            if (nu) f.add(cnt2.get(), nu, 0, 8);
+            f.add(tn2, T, 0, 8);
+            fill_many(f, s);
+            if (nu) launch_count(XS, std::max(1u, std::min<unsigned>((unsigned)ctx->num_cus * 2, nu)), s);
+            if (nu) k_unit_to_tile<<<grid256(nu), 256, 0, s>>>(cnt2, U2.tile, U2.kind, nu, tn2);
+            // DIRECT tiles: every product is a distinct output of the band
+            k_direct_tile_count<<<grid256(T), 256, 0, s>>>(ix.st_direct, (const uint64_t *)ix.st_products, T,
+                                                           P.force, tn2);
+            uint32_t gf2 = 0;
+            pool.read(&gf2, flags.get(), 4, s);
+            pool.sync(s);
+            fill_u32(flags, 4, 0, s);
+            if (!gf2) return true;
+            if (!grow) return false;
+            if (attempt >= 3) throw LibError{FCC_ERR_OVERFLOW, "band count global set overflow"};
+            alloc_ghash((X.gmask + 1) * 16);
+        }
+    };
     bool use_exact = !learned;
     bool mape_pending = false;
 
@@ -1017,70 +1088,20 @@ int Contraction::run(const fcc_tensor *ta, const int32_t *am, const fcc_tensor *
         if (!use_exact && mirror && o.judge) {
             // A mirrored tile's outputs are not its whole band, the quantity
             // the network predicts: the exact planner judges the predictions
-            // on a sample of about 256 tiles instead, with a count pass that
-            // counts every output of their bands.
+            // on a sample of about 256 tiles instead.
             const uint32_t S = std::max<uint32_t>(1, T / 256);
-            PlanParams P2 = P;
-            P2.mirror = 0;
-            P2.run = 0;
-            P2.dense_ratio = 0.5f;
-            Tmp<uint32_t> n2(pool, T), W2(pool, T), b2(pool, T), w2(pool, T), tot2(pool, 1);
-            Tmp<uint8_t> k2(pool, T);
-            launch_plan_tiles(ix, nullptr, P2, n2, W2, k2, w2, s);
-            k_sample_tiles<<<grid256(T), 256, 0, s>>>(n2, k2, T, S);
-            scan_u32(pool, n2, b2, T, tot2, s);
-            uint32_t nu = 0;
-            pool.read(&nu, tot2.get(), 4, s);
-            pool.sync(s);
-            Units U2;
-            Tmp<uint32_t> u2t(pool, nu), u2c(pool, nu), u2w(pool, nu);
-            Tmp<uint8_t> u2k(pool, nu), u2m(pool, nu);
-            U2.n = nu;
-            U2.tile = u2t;
-            U2.c0 = u2c;
-            U2.W = u2w;
-            U2.kind = u2k;
-            U2.multi = u2m;
-            Tmp<unsigned long long> cnt2(pool, nu), tn2(pool, T);
-            Tmp<unsigned int> work2(pool, 4);
+            Tmp<unsigned long long> tn2(pool, T);
             Tmp<double> acc2(pool, 2);
-            if (nu) launch_emit_units(n2, W2, k2, b2, w2, T, g.spanB, nullptr, U2, s);
-            AccArgs XS = X;
-            XS.mirror = 0;
-            XS.unclipped = 1;
-            XS.seg_below = 0;  // no heavy segments: they were cut from clipped ranges
-            XS.nunits = nu;
-            XS.unit_tile = U2.tile;
-            XS.unit_c0 = U2.c0;
-            XS.unit_W = U2.W;
-            XS.unit_kind = U2.kind;
-            XS.unit_multi = U2.multi;
-            XS.unit_order = nullptr;
-            XS.unit_count = cnt2;
-            XS.work = work2;
-            XS.set_words = SYM_SLOTS;
-            FillSet f;
-            f.add(XS.gkeys, XS.gmask + 1, ~0ull, 8);
-            f.add(gmeta.get(), 4, 0, 8);
-            f.add(flags.get(), 4, 0, 4);
-            f.add(work2.get(), 4, 0, 4);
-            f.add(cnt2.get(), nu, 0, 8);
-            f.add(tn2.get(), T, 0, 8);
-            f.add(acc2.get(), 2, 0, 8);
-            fill_many(f, s);
-            if (nu) launch_count(XS, std::max(1u, std::min<unsigned>((unsigned)ctx->num_cus * 2, nu)), s);
-            if (nu) k_unit_to_tile<<<grid256(nu), 256, 0, s>>>(cnt2, U2.tile, U2.kind, nu, tn2);
-            k_direct_tile_count<<<grid256(T), 256, 0, s>>>(ix.st_direct, (const uint64_t *)ix.st_products, T,
-                                                           P.force, tn2);
-            k_mape_sampled<<<grid256(T), 256, 0, s>>>(pred_t, tn2, T, S, acc2);
-            double m2[2] = {0.0, 0.0};
-            uint32_t gf2 = 0;
-            pool.read(m2, acc2.get(), 16, s);
-            pool.read(&gf2, flags.get(), 4, s);
-            pool.sync(s);
-            if (!gf2 && m2[1] > 0) {
-                st.mlp_mape = m2[0] / m2[1];
-                st.mape_tiles = (uint64_t)m2[1];
+            fill_f64(acc2, 2, 0.0, s);
+            if (band_counts(S, tn2, false)) {
+                k_mape_sampled<<<grid256(T), 256, 0, s>>>(pred_t, tn2, T, S, acc2);
+                double m2[2] = {0.0, 0.0};
+                pool.read(m2, acc2.get(), 16, s);
+                pool.sync(s);
+                if (m2[1] > 0) {
+                    st.mlp_mape = m2[0] / m2[1];
+                    st.mape_tiles = (uint64_t)m2[1];
+                }
             }
         }
     }
@@ -1093,7 +1114,7 @@ int Contraction::run(const fcc_tensor *ta, const int32_t *am, const fcc_tensor *
         alloc_ghash(1u << 16);
         // the key set's LDS (see make_units) and, when it is only the dense
         // byte map, a default grid of four workgroups per CU
-        const bool small_set = tkh[KIND_LHASH] == 0;
+        const bool small_set = tkh[KIND_LHASH] == 0 && tkh[KIND_GHASH] == 0;
         X.set_words = small_set ? DENSE_SLOTS / 4 : SYM_SLOTS;
         unsigned cgrid = grid;
         if (small_set && o.workgroups <= 0 && tuned_wg <= 0)
@@ -1162,6 +1183,13 @@ int Contraction::run(const fcc_tensor *ta, const int32_t *am, const fcc_tensor *
         if (hf[0]) throw LibError{FCC_ERR_OVERFLOW, "global hash overflow in the numeric pass"};
         if (hf[1]) throw LibError{FCC_ERR_OVERFLOW, "numeric pass disagreed with the count pass"};
         st.ghash_entries = hm[0];
+    }
+
+    if (keep && mirror) {
+        // a plan answers per-tile output counts (fcc_plan_tiles): under
+        // mirroring these are counted once here, over every tile's band
+        k.tile_band = Tmp<unsigned long long>(pool, T);
+        band_counts(1, k.tile_band, true);
     }
 
     // ---- results (read back by fetch_tail)
@@ -2212,13 +2240,19 @@ int fcc_plan_tiles(fcc_plan *plan, uint64_t *nnz, uint64_t *products, float *pre
     try {
         if (nnz) {
             // per-unit exact counts: the count pass (exact planner) or the
-            // units' emitted outputs of the last numeric pass
+            // units' emitted outputs of the last numeric pass; under mirroring
+            // the band counts taken when the plan was made
             Tmp<unsigned long long> tn(ctx->pool, T);
-            fill_u64(tn, T, 0, s);
-            const unsigned long long *uc = plan->c->o.planner == FCC_PLAN_LEARNED ? k.unit_cur.get() : k.unit_count.get();
-            k_unit_to_tile<<<grid256(k.U.n), 256, 0, s>>>(uc, k.U.tile, k.U.kind, k.U.n, tn);
-            k_direct_tile_count<<<grid256(T), 256, 0, s>>>(k.ix.st_direct, (const uint64_t *)k.ix.st_outd, T,
-                                                           k.plan_force, tn);
+            if (k.tile_band.get()) {
+                FCC_HIP(hipMemcpyAsync(tn.get(), k.tile_band.get(), 8ull * T, hipMemcpyDeviceToDevice, s));
+            } else {
+                fill_u64(tn, T, 0, s);
+                const unsigned long long *uc =
+                    plan->c->o.planner == FCC_PLAN_LEARNED ? k.unit_cur.get() : k.unit_count.get();
+                k_unit_to_tile<<<grid256(k.U.n), 256, 0, s>>>(uc, k.U.tile, k.U.kind, k.U.n, tn);
+                k_direct_tile_count<<<grid256(T), 256, 0, s>>>(k.ix.st_direct, (const uint64_t *)k.ix.st_outd, T,
+                                                               k.plan_force, tn);
+            }
             FCC_HIP(hipMemcpyAsync(nnz, tn.get(), 8ull * T, hipMemcpyDeviceToHost, s));
             FCC_HIP(hipStreamSynchronize(s));
         }

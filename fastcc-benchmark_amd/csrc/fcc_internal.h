@@ -157,6 +157,9 @@ struct Operand {
 // DIRECT: a tile whose entries all sit on distinct rows; every product is
 // then a distinct output and is written straight out, no accumulator.
 enum : uint8_t { KIND_DENSE = 1, KIND_LHASH = 2, KIND_GHASH = 3, KIND_DIRECT = 4 };
+// CBITS: count pass only -- a hash unit counted through one LDS bit per cell
+// of its rows x (padded) window when that fits the launch's key-set LDS.
+constexpr uint8_t KIND_CBITS = 5;
 constexpr uint32_t DIRECT_MAX_TL = 64;  // rows per tile up to which DIRECT tiles are detected
 constexpr uint64_t DIRECT_RUN = 65536;  // products per unit made of consecutive DIRECT tiles
 

@@ -3,7 +3,7 @@ the exact planner, tile-level error, and the gain from skipping the count pass."
 import numpy as np
 import pytest
 
-from conftest import assert_same_result
+from conftest import assert_same_result, record_parity
 
 from oracle import oracle
 from fastcc_amd import Context, planner
@@ -73,6 +73,19 @@ def test_any_prediction_stays_exact(bias):
     if bias < 0:
         assert r.stats["reruns"] >= 1 or r.stats["ghash_entries"] > 0
     r.free()
+    # what the mispredictions cost: best of three warm calls against the exact planner
+    ms = {}
+    for pl in ("exact", "learned"):
+        ts = []
+        for _ in range(3):
+            r = ctx.contract(a, list(wl.a_modes), b, list(wl.b_modes), planner=pl)
+            ts.append((r.stats["ms_total"], r.stats["reruns"], r.stats["ghash_entries"], r.stats["ms_numeric"]))
+            r.free()
+        ms[pl] = min(ts)
+    print(f"bias {bias}: exact {ms['exact']}, learned {ms['learned']}", flush=True)
+    record_parity(f"c1(0.25) prediction bias {bias}", exact_ms=ms["exact"][0], learned_ms=ms["learned"][0],
+                  learned_reruns=ms["learned"][1], learned_ghash_entries=ms["learned"][2],
+                  learned_numeric_ms=ms["learned"][3])
     ctx.close()
 
 

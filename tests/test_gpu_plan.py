@@ -80,6 +80,35 @@ def test_plan_query_bounds(gpu_ctx):
         plan.close()
 
 
+def test_plan_tiles_learned_sink_and_mirrored(gpu_ctx):
+    """Per-tile output counts do not depend on the planner, the output sink or
+    mirroring: a learned count + checksum plan takes them from its units'
+    outputs; a mirrored self-contraction plan (whose tiles write only part of
+    their bands) from a band count pass. All equal the exact planner's count
+    pass on the unmirrored materialised contraction, tile by tile."""
+    _load_mlp(gpu_ctx)
+    for wl, self_c in ((W.c1(0.25), False), (W.c2(0.1), True), (W.c4(0.02), True)):
+        a, b = wl.tensors()
+        bb = a if self_c else b
+        per = {}
+        for planner, sink, mirror in (("exact", False, False), ("learned", True, False), ("exact", False, True),
+                                      ("learned", True, True), ("learned", False, True)):
+            if mirror and not self_c:
+                continue
+            print(wl.name, planner, sink, mirror, flush=True)
+            plan = gpu_ctx.plan(a, list(wl.a_modes), bb, list(wl.b_modes), planner=planner, sink=sink, mirror=mirror,
+                                tile_hint=16)
+            info = plan.info()
+            nnz, prod, _ = plan.tiles()
+            per[(planner, sink, mirror)] = (nnz, prod, info)
+            plan.close()
+        nnz0, prod0, info0 = per[("exact", False, False)]
+        assert int(nnz0.sum()) == info0["out_nnz"]
+        for key, (nnz, prod, info) in per.items():
+            assert np.array_equal(nnz, nnz0), (wl.name, key)
+            assert np.array_equal(prod, prod0), (wl.name, key)
+
+
 @pytest.mark.parametrize("sink", [False, True])
 def test_plan_graph_replays(gpu_ctx, sink):
     """Captured once, replayed with new values each time; eager executions and
