@@ -1278,6 +1278,121 @@ int Contraction::run(const fcc_tensor *ta, const int32_t *am, const fcc_tensor *
     return FCC_OK;
 }
 
+// ------------------------------------------------------------ fused batches
+
+// Per-pair extents (maximum coordinate + 1, or the caller's, whichever is
+// larger) decide how many consecutive pairs one fused launch can hold: the
+// returned chunk ends (fcc_chunk.h).
+std::vector<uint32_t> chunk_pairs(fcc_ctx *ctx, int32_t n, const fcc_tensor *a, const fcc_tensor *b,
+                                  const int32_t *a_modes, const int32_t *b_modes, int32_t n_contr) {
+    DevPool &pool = ctx->pool;
+    hipStream_t s = ctx->stream;
+    const int dA = a[0].order, dB = b[0].order;
+    std::vector<uint32_t> pext[2];
+    std::vector<uint64_t> pnnz[2];
+    for (int k = 0; k < 2; k++) {
+        const fcc_tensor *src = k ? b : a;
+        const int d = k ? dB : dA;
+        pext[k].assign((size_t)n * d, 0);
+        pnnz[k].assign(n, 0);
+        std::vector<PairSrc> dev;
+        std::vector<int32_t> dev_idx;
+        for (int32_t i = 0; i < n; i++) {
+            const fcc_tensor &t = src[i];
+            pnnz[k][i] = (uint64_t)t.nnz;
+            if (t.nnz && t.on_device) {
+                dev.push_back(PairSrc{t.coords, t.vals, (uint64_t)t.nnz});
+                dev_idx.push_back(i);
+            } else if (t.nnz) {
+                for (int m = 0; m < d; m++) {
+                    uint32_t mx = 0;
+                    for (int64_t j = 0; j < t.nnz; j++) mx = std::max(mx, t.coords[(size_t)m * t.nnz + j]);
+                    pext[k][(size_t)i * d + m] = mx + 1;
+                }
+            }
+        }
+        if (!dev.empty()) {
+            Tmp<unsigned char> dps(pool, sizeof(PairSrc) * dev.size());
+            Tmp<uint32_t> dmx(pool, dev.size() * d);
+            FCC_HIP(hipMemcpyAsync(dps.get(), dev.data(), sizeof(PairSrc) * dev.size(), hipMemcpyHostToDevice, s));
+            pair_max((const PairSrc *)dps.get(), (uint32_t)dev.size(), d, dmx, s);
+            std::vector<uint32_t> hm(dev.size() * d);
+            FCC_HIP(hipMemcpyAsync(hm.data(), dmx.get(), 4 * hm.size(), hipMemcpyDeviceToHost, s));
+            FCC_HIP(hipStreamSynchronize(s));
+            for (size_t q = 0; q < dev.size(); q++)
+                for (int m = 0; m < d; m++) pext[k][(size_t)dev_idx[q] * d + m] = hm[q * d + m] + 1;
+        }
+        for (int32_t i = 0; i < n; i++)
+            if (src[i].extents)
+                for (int m = 0; m < d; m++)
+                    pext[k][(size_t)i * d + m] = std::max(pext[k][(size_t)i * d + m], src[i].extents[m]);
+    }
+    return batch_chunks((uint32_t)n, dA, dB, a_modes, b_modes, n_contr, pext[0].data(), pext[1].data(),
+                        pnnz[0].data(), pnnz[1].data(), kMaxBatchPairs);
+}
+
+// One operand side of a fused batch: the cnt pairs' entries concatenated in
+// pair order on the device, with a trailing pair-index mode; extents the
+// bounding box of the pairs' (explicit extents included), pair mode = cnt.
+struct FusedSide {
+    int d = 0;
+    std::vector<unsigned long long> off;  // pair i's entries: [off[i], off[i + 1])
+    Tmp<uint32_t> c;
+    Tmp<double> v;
+    uint32_t E[kMaxOrder + 1] = {};
+};
+
+void fuse_side(fcc_ctx *ctx, uint32_t cnt, const fcc_tensor *src, int d, FusedSide &S) {
+    DevPool &pool = ctx->pool;
+    hipStream_t s = ctx->stream;
+    S.d = d;
+    S.off.assign(cnt + 1, 0);
+    uint64_t host_n = 0;
+    for (uint32_t i = 0; i < cnt; i++) {
+        S.off[i + 1] = S.off[i] + (uint64_t)src[i].nnz;
+        if (!src[i].on_device) host_n += (uint64_t)src[i].nnz;
+    }
+    const uint64_t total = S.off[cnt];
+    if (total >= (1ull << 32)) throw LibError{FCC_ERR_INVALID, "batch operand exceeds 2^32 nonzeros"};
+    Tmp<uint32_t> stc(pool, (size_t)d * host_n);
+    Tmp<double> stv(pool, (size_t)host_n);
+    std::vector<PairSrc> ps(cnt);
+    uint64_t ho = 0;
+    for (uint32_t i = 0; i < cnt; i++) {
+        const fcc_tensor &t = src[i];
+        ps[i].nnz = (uint64_t)t.nnz;
+        if (t.on_device || !t.nnz) {
+            ps[i].coords = t.coords;
+            ps[i].vals = t.vals;
+        } else {
+            FCC_HIP(hipMemcpyAsync(stc.get() + (size_t)d * ho, t.coords, 4ull * d * t.nnz, hipMemcpyHostToDevice, s));
+            FCC_HIP(hipMemcpyAsync(stv.get() + ho, t.vals, 8ull * t.nnz, hipMemcpyHostToDevice, s));
+            ps[i].coords = stc.get() + (size_t)d * ho;
+            ps[i].vals = stv.get() + ho;
+            ho += (uint64_t)t.nnz;
+        }
+    }
+    Tmp<unsigned char> dps(pool, sizeof(PairSrc) * cnt);
+    Tmp<unsigned long long> doff(pool, cnt + 1);
+    Tmp<uint32_t> dmax(pool, kMaxOrder);
+    FCC_HIP(hipMemcpyAsync(dps.get(), ps.data(), sizeof(PairSrc) * cnt, hipMemcpyHostToDevice, s));
+    FCC_HIP(hipMemcpyAsync(doff.get(), S.off.data(), 8ull * (cnt + 1), hipMemcpyHostToDevice, s));
+    fill_u32(dmax, kMaxOrder, 0, s);
+    S.c = Tmp<uint32_t>(pool, (size_t)(d + 1) * total);
+    S.v = Tmp<double>(pool, (size_t)total);
+    gather_pairs((const PairSrc *)dps.get(), doff, cnt, d, total, S.c, S.v, dmax, s);
+    uint32_t hmax[kMaxOrder];
+    FCC_HIP(hipMemcpyAsync(hmax, dmax.get(), sizeof(hmax), hipMemcpyDeviceToHost, s));
+    FCC_HIP(hipStreamSynchronize(s));
+    for (int m = 0; m < d; m++) {
+        uint32_t e = total ? hmax[m] + 1 : 0;
+        for (uint32_t i = 0; i < cnt; i++)
+            if (src[i].extents) e = std::max(e, src[i].extents[m]);
+        S.E[m] = e;
+    }
+    S.E[d] = cnt;
+}
+
 }  // namespace
 
 // ====================================================================== ABI
@@ -1768,115 +1883,16 @@ int fcc_contract_batch(fcc_ctx *ctx, int32_t n, const fcc_tensor *a, const int32
     hipStream_t s = ctx->stream;
     try {
         FCC_HIP(hipSetDevice(ctx->device));
-        // per-pair extents (maximum coordinate + 1, or the caller's, whichever
-        // is larger) decide how many pairs one fused launch can hold
-        std::vector<uint32_t> pext[2];
-        std::vector<uint64_t> pnnz[2];
-        for (int k = 0; k < 2; k++) {
-            const fcc_tensor *src = k ? b : a;
-            const int d = k ? dB : dA;
-            pext[k].assign((size_t)n * d, 0);
-            pnnz[k].assign(n, 0);
-            std::vector<PairSrc> dev;
-            std::vector<int32_t> dev_idx;
-            for (int32_t i = 0; i < n; i++) {
-                const fcc_tensor &t = src[i];
-                pnnz[k][i] = (uint64_t)t.nnz;
-                if (t.nnz && t.on_device) {
-                    dev.push_back(PairSrc{t.coords, t.vals, (uint64_t)t.nnz});
-                    dev_idx.push_back(i);
-                } else if (t.nnz) {
-                    for (int m = 0; m < d; m++) {
-                        uint32_t mx = 0;
-                        for (int64_t j = 0; j < t.nnz; j++) mx = std::max(mx, t.coords[(size_t)m * t.nnz + j]);
-                        pext[k][(size_t)i * d + m] = mx + 1;
-                    }
-                }
-            }
-            if (!dev.empty()) {
-                Tmp<unsigned char> dps(pool, sizeof(PairSrc) * dev.size());
-                Tmp<uint32_t> dmx(pool, dev.size() * d);
-                FCC_HIP(hipMemcpyAsync(dps.get(), dev.data(), sizeof(PairSrc) * dev.size(), hipMemcpyHostToDevice, s));
-                pair_max((const PairSrc *)dps.get(), (uint32_t)dev.size(), d, dmx, s);
-                std::vector<uint32_t> hm(dev.size() * d);
-                FCC_HIP(hipMemcpyAsync(hm.data(), dmx.get(), 4 * hm.size(), hipMemcpyDeviceToHost, s));
-                FCC_HIP(hipStreamSynchronize(s));
-                for (size_t q = 0; q < dev.size(); q++)
-                    for (int m = 0; m < d; m++) pext[k][(size_t)dev_idx[q] * d + m] = hm[q * d + m] + 1;
-            }
-            for (int32_t i = 0; i < n; i++)
-                if (src[i].extents)
-                    for (int m = 0; m < d; m++)
-                        pext[k][(size_t)i * d + m] = std::max(pext[k][(size_t)i * d + m], src[i].extents[m]);
-        }
-        const std::vector<uint32_t> ends = batch_chunks((uint32_t)n, dA, dB, a_modes, b_modes, n_contr,
-                                                        pext[0].data(), pext[1].data(), pnnz[0].data(),
-                                                        pnnz[1].data(), kMaxBatchPairs);
+        const std::vector<uint32_t> ends = chunk_pairs(ctx, n, a, b, a_modes, b_modes, n_contr);
         for (size_t ci = 0; ci < ends.size(); ci++) {
             const int32_t c0 = ci ? (int32_t)ends[ci - 1] : 0;
             const uint32_t cnt = ends[ci] - (uint32_t)c0;
             const fcc_tensor *pa = a + c0, *pb = b + c0;
             fcc_result *po = outs + c0;
             // ---- concatenate each operand with a trailing pair-index mode
-            struct Side {
-                int d;
-                std::vector<unsigned long long> off;
-                Tmp<uint32_t> c;
-                Tmp<double> v;
-                uint32_t E[kMaxOrder + 1];
-            } side[2];
-            const fcc_tensor *src[2] = {pa, pb};
-            for (int k = 0; k < 2; k++) {
-                Side &S = side[k];
-                S.d = k ? dB : dA;
-                S.off.assign(cnt + 1, 0);
-                uint64_t host_n = 0;
-                for (uint32_t i = 0; i < cnt; i++) {
-                    S.off[i + 1] = S.off[i] + (uint64_t)src[k][i].nnz;
-                    if (!src[k][i].on_device) host_n += (uint64_t)src[k][i].nnz;
-                }
-                const uint64_t total = S.off[cnt];
-                if (total >= (1ull << 32)) throw LibError{FCC_ERR_INVALID, "batch operand exceeds 2^32 nonzeros"};
-                Tmp<uint32_t> stc(pool, (size_t)S.d * host_n);
-                Tmp<double> stv(pool, (size_t)host_n);
-                std::vector<PairSrc> ps(cnt);
-                uint64_t ho = 0;
-                for (uint32_t i = 0; i < cnt; i++) {
-                    const fcc_tensor &t = src[k][i];
-                    ps[i].nnz = (uint64_t)t.nnz;
-                    if (t.on_device || !t.nnz) {
-                        ps[i].coords = t.coords;
-                        ps[i].vals = t.vals;
-                    } else {
-                        FCC_HIP(hipMemcpyAsync(stc.get() + (size_t)S.d * ho, t.coords, 4ull * S.d * t.nnz,
-                                               hipMemcpyHostToDevice, s));
-                        FCC_HIP(hipMemcpyAsync(stv.get() + ho, t.vals, 8ull * t.nnz, hipMemcpyHostToDevice, s));
-                        ps[i].coords = stc.get() + (size_t)S.d * ho;
-                        ps[i].vals = stv.get() + ho;
-                        ho += (uint64_t)t.nnz;
-                    }
-                }
-                Tmp<unsigned char> dps(pool, sizeof(PairSrc) * cnt);
-                Tmp<unsigned long long> doff(pool, cnt + 1);
-                Tmp<uint32_t> dmax(pool, kMaxOrder);
-                FCC_HIP(hipMemcpyAsync(dps.get(), ps.data(), sizeof(PairSrc) * cnt, hipMemcpyHostToDevice, s));
-                FCC_HIP(hipMemcpyAsync(doff.get(), S.off.data(), 8ull * (cnt + 1), hipMemcpyHostToDevice, s));
-                fill_u32(dmax, kMaxOrder, 0, s);
-                S.c = Tmp<uint32_t>(pool, (size_t)(S.d + 1) * total);
-                S.v = Tmp<double>(pool, (size_t)total);
-                gather_pairs((const PairSrc *)dps.get(), doff, cnt, S.d, total, S.c, S.v, dmax, s);
-                uint32_t hmax[kMaxOrder];
-                FCC_HIP(hipMemcpyAsync(hmax, dmax.get(), sizeof(hmax), hipMemcpyDeviceToHost, s));
-                FCC_HIP(hipStreamSynchronize(s));
-                // extents: bounding box of every pair (explicit extents included), pair mode = cnt
-                for (int m = 0; m < S.d; m++) {
-                    uint32_t e = total ? hmax[m] + 1 : 0;
-                    for (uint32_t i = 0; i < cnt; i++)
-                        if (src[k][i].extents) e = std::max(e, src[k][i].extents[m]);
-                    S.E[m] = e;
-                }
-                S.E[S.d] = cnt;
-            }
+            FusedSide side[2];
+            fuse_side(ctx, cnt, pa, dA, side[0]);
+            fuse_side(ctx, cnt, pb, dB, side[1]);
             // ---- one batch-mode contraction over all pairs
             fcc_tensor ta{dA + 1, (int64_t)side[0].off[cnt], side[0].c.get(), side[0].v.get(), side[0].E, 1};
             fcc_tensor tb{dB + 1, (int64_t)side[1].off[cnt], side[1].c.get(), side[1].v.get(), side[1].E, 1};
@@ -2090,6 +2106,15 @@ struct fcc_plan {
     fcc_plan_info info{};
     int32_t order = 0;
     uint32_t extents[FCC_MAX_ORDER] = {};
+    // batch plans (fcc_plan_create_batch): the pairs and the fixed per-pair
+    // split of materialised outputs (counts and offsets do not change with
+    // the values)
+    uint32_t pairs = 0;
+    int32_t pair_order = 0;
+    std::vector<unsigned long long> pair_nnz, pair_off;
+    Tmp<unsigned long long> split_counts, split_offs, split_cursor;
+    Tmp<uint32_t> split_c;
+    Tmp<double> split_v;
     ~fcc_plan() {
         if (exec) (void)hipGraphExecDestroy(exec);
         if (graph) (void)hipGraphDestroy(graph);
@@ -2102,9 +2127,20 @@ namespace {
 // values into index order, clear the counters and the global hash with fill
 // kernels (no memset nodes: the sequence may be captured into a graph), run
 // the persistent accumulate kernel and the global-hash drain.
+__global__ void k_stamp_clock(unsigned long long *dst) { *dst = wall_clock64(); }
+
+// A batch plan also stamps the device clock (the pairs' completion times are
+// taken against it), re-arms the per-pair unit counters, and after the drain
+// splits the fused output into the per-pair blocks.
 void plan_sequence(fcc_plan *P, const double *asrc, const double *bsrc, hipStream_t s) {
     Contraction::Kept &k = P->c->k;
     AccArgs &X = k.X;
+    if (P->pairs) {
+        k_stamp_clock<<<1, 1, 0, s>>>(k.g_done.get() + P->pairs);
+        fill_u32(k.g_left, P->pairs, 0, s);
+        fill_u64(k.g_done, P->pairs, 0, s);
+        group_units(X.unit_tile, k.ix.atile_id, X.nunits, k.ix.logTL, X.group_rows, k.g_left, s);
+    }
     perm_gather(bsrc, k.ix.permB, k.ix.nB, k.ix.Bval, s);
     perm_gather(asrc, k.ix.permA, k.ix.nAv, k.ix.Aval, s);
     dense_keys_values(k.ix, k.dk, s);
@@ -2116,6 +2152,145 @@ void plan_sequence(fcc_plan *P, const double *asrc, const double *bsrc, hipStrea
     fill_u64(k.sinkbuf, k.nsink, 0, s);
     launch_numeric(X, k.grid, k.bitmap, k.sink, 0, s);
     launch_gdrain(X, k.sink, 0, s);
+    if (P->pairs && P->split_v.get()) {
+        fill_u64(P->split_cursor, P->pairs, 0, s);
+        split_scatter(k.out_c.get(), k.stride, k.out_v.get(), k.nnz_out, P->pair_order + 1, P->pairs, P->split_offs,
+                      P->split_counts, P->split_cursor, P->split_c.get(), P->split_v, s);
+    }
+}
+
+// Stages new values (host values, or any values of a graph execution, go
+// through the plan's fixed buffers), runs the sequence -- eagerly, or as the
+// plan's graph, captured on first use -- and reads back the flags, the sink
+// words and (batch plans) the pairs' completion clocks. st: device times.
+void run_plan(fcc_plan *plan, const double *a_vals, const double *b_vals, int32_t on_device, int32_t use_graph,
+              std::vector<unsigned long long> &sinkw, std::vector<unsigned long long> &gdone, fcc_stats &st) {
+    fcc_ctx *ctx = plan->ctx;
+    Contraction::Kept &k = plan->c->k;
+    const bool self = plan->self_values;
+    hipStream_t s = ctx->stream;
+    hipEvent_t *ev = ctx->ev;
+    FCC_HIP(hipSetDevice(ctx->device));
+    FCC_HIP(hipEventRecord(ev[0], s));
+    const double *asrc = a_vals, *bsrc = self ? a_vals : b_vals;
+    if (use_graph || !on_device) {
+        const hipMemcpyKind kind = on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
+        if (plan->nA && a_vals != plan->a_stage.get())
+            FCC_HIP(hipMemcpyAsync(plan->a_stage.get(), a_vals, 8ull * plan->nA, kind, s));
+        asrc = plan->a_stage.get();
+        if (self) {
+            bsrc = asrc;
+        } else {
+            if (plan->nB && b_vals != plan->b_stage.get())
+                FCC_HIP(hipMemcpyAsync(plan->b_stage.get(), b_vals, 8ull * plan->nB, kind, s));
+            bsrc = plan->b_stage.get();
+        }
+    }
+    FCC_HIP(hipEventRecord(ev[4], s));
+    if (use_graph) {
+        if (!plan->exec) {
+            hipGraph_t g = nullptr;
+            FCC_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+            try {
+                plan_sequence(plan, asrc, bsrc, s);
+            } catch (...) {
+                (void)hipStreamEndCapture(s, &g);
+                if (g) (void)hipGraphDestroy(g);
+                throw;
+            }
+            FCC_HIP(hipStreamEndCapture(s, &g));
+            plan->graph = g;
+            FCC_HIP(hipGraphInstantiate(&plan->exec, g, nullptr, nullptr, 0));
+        }
+        FCC_HIP(hipGraphLaunch(plan->exec, s));
+    } else {
+        plan_sequence(plan, asrc, bsrc, s);
+    }
+    FCC_HIP(hipEventRecord(ev[5], s));
+    uint32_t hf[2];
+    unsigned long long hm = 0;
+    FCC_HIP(hipMemcpyAsync(hf, k.flags.get(), 8, hipMemcpyDeviceToHost, s));
+    FCC_HIP(hipMemcpyAsync(&hm, k.gmeta.get(), 8, hipMemcpyDeviceToHost, s));
+    sinkw.assign(k.nsink, 0);
+    FCC_HIP(hipMemcpyAsync(sinkw.data(), k.sinkbuf.get(), 8 * k.nsink, hipMemcpyDeviceToHost, s));
+    if (plan->pairs) {
+        gdone.assign(plan->pairs + 1, 0);
+        FCC_HIP(hipMemcpyAsync(gdone.data(), k.g_done.get(), 8 * gdone.size(), hipMemcpyDeviceToHost, s));
+    }
+    FCC_HIP(hipStreamSynchronize(s));
+    if (hf[0]) throw LibError{FCC_ERR_OVERFLOW, "global hash overflow in the numeric pass"};
+    if (hf[1]) throw LibError{FCC_ERR_OVERFLOW, "numeric pass disagreed with the plan"};
+    st = plan->stats;
+    st.ms_index = 0;
+    st.ms_plan = 0;
+    st.ms_symbolic = 0;
+    st.ms_numeric = ev_ms(ev[4], ev[5]);
+    st.ms_numeric_kernel = st.ms_numeric;
+    st.ms_overflow = 0;
+    st.ms_total = ev_ms(ev[0], ev[5]);
+    st.ms_done = st.ms_total;
+    st.reruns = 0;
+    st.ghash_entries = hm;
+}
+
+// After a plan's first contraction: a learned run's per-unit output counts
+// become exact offsets, the global hash is sized for what the run put in it,
+// the value staging buffers are allocated and the plan summary is filled.
+void finish_plan(fcc_plan *P, const fcc_result &first) {
+    fcc_ctx *ctx = P->ctx;
+    Contraction::Kept &k = P->c->k;
+    DevPool &pool = ctx->pool;
+    hipStream_t s = ctx->stream;
+    if (!k.empty && k.mode == 1) {
+        // learned run: its per-unit output counts become exact offsets
+        Tmp<unsigned long long> tot(pool, 1);
+        scan_u64(pool, (const uint64_t *)k.unit_cur.get(), (uint64_t *)k.unit_off.get(), k.U.n,
+                 (uint64_t *)tot.get(), s);
+        k.X.unit_off = k.unit_off;
+        k.X.capacity = k.sink ? ~0ull : k.stride;
+        k.mode = 0;
+    }
+    if (!k.empty && first.stats.ghash_entries * 2 > k.X.gmask + 1) {
+        uint64_t cap = pow2_ceil(first.stats.ghash_entries * 2);
+        k.gk = Tmp<unsigned long long>(pool, cap);
+        k.gv = Tmp<double>(pool, cap);
+        k.gu = Tmp<uint32_t>(pool, cap);
+        k.X.gkeys = k.gk;
+        k.X.gvals = k.gv;
+        k.X.gunit = k.gu;
+        k.X.gmask = cap - 1;
+    }
+    P->a_stage = Tmp<double>(pool, (size_t)std::max<int64_t>(P->nA, 1));
+    P->b_stage = Tmp<double>(pool, (size_t)std::max<int64_t>(P->nB, 1));
+    // plan summary (exact count, bounds, shape)
+    fcc_plan_info &I = P->info;
+    I.out_nnz = first.nnz;
+    I.products = first.stats.products;
+    I.left_tiles = first.stats.left_tiles;
+    I.units = first.stats.units;
+    I.units_dense = first.stats.units_dense;
+    I.units_lhash = first.stats.units_lhash;
+    I.units_ghash = first.stats.units_ghash;
+    I.row_tile = first.stats.row_tile;
+    I.col_window = first.stats.col_window;
+    I.order = first.order;
+    memcpy(I.extents, first.extents, sizeof(I.extents));
+    I.ms_plan = first.stats.ms_total;
+    const uint32_t T = k.ix.natiles;
+    if (!k.empty && T) {
+        std::vector<unsigned long long> mg(T), pr(T);
+        FCC_HIP(hipMemcpyAsync(mg.data(), k.ix.st_maxgp, 8ull * T, hipMemcpyDeviceToHost, s));
+        FCC_HIP(hipMemcpyAsync(pr.data(), k.ix.st_products, 8ull * T, hipMemcpyDeviceToHost, s));
+        FCC_HIP(hipStreamSynchronize(s));
+        const double area = (double)k.ix.TL * (double)k.g.spanB;
+        for (uint32_t t = 0; t < T; t++) {
+            I.nnz_lower_bound += mg[t];
+            I.nnz_upper_bound += (uint64_t)std::min<double>((double)pr[t], area);
+        }
+    }
+    P->stats = first.stats;
+    P->order = first.order;
+    memcpy(P->extents, first.extents, sizeof(P->extents));
 }
 
 }  // namespace
@@ -2153,58 +2328,7 @@ int fcc_plan_create(fcc_ctx *ctx, const fcc_tensor *a, const int32_t *a_modes, c
     try {
         FCC_HIP(hipSetDevice(ctx->device));
         c.run(a, ka.data(), b, kb.data(), n_contr + n_batch, n_batch);
-        Contraction::Kept &k = c.k;
-        DevPool &pool = ctx->pool;
-        if (!k.empty && k.mode == 1) {
-            // learned run: its per-unit output counts become exact offsets
-            Tmp<unsigned long long> tot(pool, 1);
-            scan_u64(pool, (const uint64_t *)k.unit_cur.get(), (uint64_t *)k.unit_off.get(), k.U.n,
-                     (uint64_t *)tot.get(), s);
-            k.X.unit_off = k.unit_off;
-            k.X.capacity = k.sink ? ~0ull : k.stride;
-            k.mode = 0;
-        }
-        if (!k.empty && first.stats.ghash_entries * 2 > k.X.gmask + 1) {
-            uint64_t cap = pow2_ceil(first.stats.ghash_entries * 2);
-            k.gk = Tmp<unsigned long long>(pool, cap);
-            k.gv = Tmp<double>(pool, cap);
-            k.gu = Tmp<uint32_t>(pool, cap);
-            k.X.gkeys = k.gk;
-            k.X.gvals = k.gv;
-            k.X.gunit = k.gu;
-            k.X.gmask = cap - 1;
-        }
-        P->a_stage = Tmp<double>(pool, (size_t)std::max<int64_t>(P->nA, 1));
-        P->b_stage = Tmp<double>(pool, (size_t)std::max<int64_t>(P->nB, 1));
-        // plan summary (exact count, bounds, shape)
-        fcc_plan_info &I = P->info;
-        I.out_nnz = first.nnz;
-        I.products = first.stats.products;
-        I.left_tiles = first.stats.left_tiles;
-        I.units = first.stats.units;
-        I.units_dense = first.stats.units_dense;
-        I.units_lhash = first.stats.units_lhash;
-        I.units_ghash = first.stats.units_ghash;
-        I.row_tile = first.stats.row_tile;
-        I.col_window = first.stats.col_window;
-        I.order = first.order;
-        memcpy(I.extents, first.extents, sizeof(I.extents));
-        I.ms_plan = first.stats.ms_total;
-        const uint32_t T = k.ix.natiles;
-        if (!k.empty && T) {
-            std::vector<unsigned long long> mg(T), pr(T);
-            FCC_HIP(hipMemcpyAsync(mg.data(), k.ix.st_maxgp, 8ull * T, hipMemcpyDeviceToHost, s));
-            FCC_HIP(hipMemcpyAsync(pr.data(), k.ix.st_products, 8ull * T, hipMemcpyDeviceToHost, s));
-            FCC_HIP(hipStreamSynchronize(s));
-            const double area = (double)k.ix.TL * (double)k.g.spanB;
-            for (uint32_t t = 0; t < T; t++) {
-                I.nnz_lower_bound += mg[t];
-                I.nnz_upper_bound += (uint64_t)std::min<double>((double)pr[t], area);
-            }
-        }
-        P->stats = first.stats;
-        P->order = first.order;
-        memcpy(P->extents, first.extents, sizeof(P->extents));
+        finish_plan(P.get(), first);
         *plan = P.release();
         ctx->err.clear();
         return FCC_OK;
@@ -2285,54 +2409,10 @@ int fcc_plan_execute(fcc_plan *plan, const double *a_vals, const double *b_vals,
     memcpy(out->extents, plan->extents, sizeof(out->extents));
     out->stats = plan->stats;
     if (k.empty) return FCC_OK;
-    hipStream_t s = ctx->stream;
-    hipEvent_t *ev = ctx->ev;
+    if (plan->pairs) return fail(ctx, FCC_ERR_INVALID, "a batch plan runs with fcc_plan_execute_batch");
     try {
-        FCC_HIP(hipSetDevice(ctx->device));
-        FCC_HIP(hipEventRecord(ev[0], s));
-        const double *asrc = a_vals, *bsrc = self ? a_vals : b_vals;
-        if (use_graph || !on_device) {
-            // stage into the plan's fixed buffers
-            const hipMemcpyKind kind = on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
-            if (plan->nA && a_vals != plan->a_stage.get())
-                FCC_HIP(hipMemcpyAsync(plan->a_stage.get(), a_vals, 8ull * plan->nA, kind, s));
-            asrc = plan->a_stage.get();
-            if (self) {
-                bsrc = asrc;
-            } else {
-                if (plan->nB && b_vals != plan->b_stage.get())
-                    FCC_HIP(hipMemcpyAsync(plan->b_stage.get(), b_vals, 8ull * plan->nB, kind, s));
-                bsrc = plan->b_stage.get();
-            }
-        }
-        FCC_HIP(hipEventRecord(ev[4], s));
-        if (use_graph) {
-            if (!plan->exec) {
-                hipGraph_t g = nullptr;
-                FCC_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-                try {
-                    plan_sequence(plan, asrc, bsrc, s);
-                } catch (...) {
-                    (void)hipStreamEndCapture(s, &g);
-                    if (g) (void)hipGraphDestroy(g);
-                    throw;
-                }
-                FCC_HIP(hipStreamEndCapture(s, &g));
-                plan->graph = g;
-                FCC_HIP(hipGraphInstantiate(&plan->exec, g, nullptr, nullptr, 0));
-            }
-            FCC_HIP(hipGraphLaunch(plan->exec, s));
-        } else {
-            plan_sequence(plan, asrc, bsrc, s);
-        }
-        FCC_HIP(hipEventRecord(ev[5], s));
-        uint32_t hf[2];
-        FCC_HIP(hipMemcpyAsync(hf, k.flags.get(), 8, hipMemcpyDeviceToHost, s));
-        unsigned long long sk[4];
-        FCC_HIP(hipMemcpyAsync(sk, k.sinkbuf.get(), 32, hipMemcpyDeviceToHost, s));
-        FCC_HIP(hipStreamSynchronize(s));
-        if (hf[0]) throw LibError{FCC_ERR_OVERFLOW, "global hash overflow in the numeric pass"};
-        if (hf[1]) throw LibError{FCC_ERR_OVERFLOW, "numeric pass disagreed with the plan"};
+        std::vector<unsigned long long> sk, gd;
+        run_plan(plan, a_vals, b_vals, on_device, use_graph, sk, gd, out->stats);
         out->nnz = (int64_t)k.nnz_out;
         out->coord_stride = (int64_t)k.stride;
         if (k.sink) {
@@ -2344,16 +2424,162 @@ int fcc_plan_execute(fcc_plan *plan, const double *a_vals, const double *b_vals,
             out->coords = k.out_c.get();
             out->vals = k.out_v.get();
         }
-        fcc_stats &st = out->stats;
-        st.ms_index = 0;
-        st.ms_plan = 0;
-        st.ms_symbolic = 0;
-        st.ms_numeric = ev_ms(ev[4], ev[5]);
-        st.ms_numeric_kernel = st.ms_numeric;
-        st.ms_overflow = 0;
-        st.ms_total = ev_ms(ev[0], ev[5]);
-        st.ms_done = st.ms_total;
-        st.reruns = 0;
+        ctx->err.clear();
+        return FCC_OK;
+    } catch (HipError &e) {
+        (void)hipGetLastError();
+        return fail(ctx, FCC_ERR_HIP, e.where + ": " + hipGetErrorString(e.code));
+    } catch (LibError &e) {
+        return fail(ctx, e.code, e.msg);
+    }
+}
+
+int fcc_plan_create_batch(fcc_ctx *ctx, int32_t n, const fcc_tensor *a, const int32_t *a_modes,
+                          const fcc_tensor *b, const int32_t *b_modes, int32_t n_contr, const fcc_options *options,
+                          fcc_plan **plan) {
+    if (!ctx) return FCC_ERR_INVALID;
+    if (!plan) return fail(ctx, FCC_ERR_INVALID, "plan is NULL");
+    *plan = nullptr;
+    if (n <= 0 || !a || !b) return fail(ctx, FCC_ERR_INVALID, "bad batch arguments");
+    std::string msg;
+    std::vector<int32_t> ka, kb;
+    int rc = FCC_OK;
+    for (int32_t i = 0; i < n && !rc; i++) {
+        rc = validate(&a[i], "a[i]", msg);
+        if (!rc) rc = validate(&b[i], "b[i]", msg);
+        if (!rc && (a[i].order != a[0].order || b[i].order != b[0].order)) {
+            msg = "batch pairs differ in order";
+            rc = FCC_ERR_INVALID;
+        }
+    }
+    if (!rc) rc = check_modes(&a[0], &b[0], a_modes, b_modes, n_contr, nullptr, nullptr, 0, ka, kb, msg);
+    if (rc) return fail(ctx, rc, msg);
+    const int dA = a[0].order, dB = b[0].order, dC = dA + dB - 2 * n_contr;
+    if (dA + 1 > FCC_MAX_ORDER || dB + 1 > FCC_MAX_ORDER || dC + 1 > FCC_MAX_ORDER)
+        return fail(ctx, FCC_ERR_INVALID, "batch plan: no room for the pair-index mode (orders must stay below "
+                                          "FCC_MAX_ORDER)");
+    std::unique_ptr<fcc_plan> P(new fcc_plan);
+    P->ctx = ctx;
+    P->c.reset(new Contraction);
+    Contraction &c = *P->c;
+    c.ctx = ctx;
+    c.s = ctx->stream;
+    fcc_options_default(&c.o);
+    if (options) c.o = *options;
+    if (c.o.left_tile_stride < 1) c.o.left_tile_stride = 1;
+    c.keep = true;
+    c.groups = (uint32_t)n;
+    fcc_result first;
+    memset(&first, 0, sizeof(first));
+    c.out = &first;
+    hipStream_t s = ctx->stream;
+    DevPool &pool = ctx->pool;
+    try {
+        FCC_HIP(hipSetDevice(ctx->device));
+        if (chunk_pairs(ctx, n, a, b, a_modes, b_modes, n_contr).size() != 1)
+            throw LibError{FCC_ERR_INVALID, "batch plan: the pairs do not fit one fused launch (split the batch)"};
+        FusedSide side[2];
+        fuse_side(ctx, (uint32_t)n, a, dA, side[0]);
+        fuse_side(ctx, (uint32_t)n, b, dB, side[1]);
+        fcc_tensor ta{dA + 1, (int64_t)side[0].off[n], side[0].c.get(), side[0].v.get(), side[0].E, 1};
+        fcc_tensor tb{dB + 1, (int64_t)side[1].off[n], side[1].c.get(), side[1].v.get(), side[1].E, 1};
+        std::vector<int32_t> fa{dA}, fb{dB};
+        fa.insert(fa.end(), a_modes, a_modes + n_contr);
+        fb.insert(fb.end(), b_modes, b_modes + n_contr);
+        P->nA = ta.nnz;
+        P->nB = tb.nnz;
+        P->self_values = false;
+        c.run(&ta, fa.data(), &tb, fb.data(), n_contr + 1, 1);
+        side[0].c.reset();
+        side[0].v.reset();
+        side[1].c.reset();
+        side[1].v.reset();
+        finish_plan(P.get(), first);
+        P->pairs = (uint32_t)n;
+        P->pair_order = dC;
+        Contraction::Kept &k = c.k;
+        // completion clocks: stamped at the start of each execution instead
+        k.X.clock_start = nullptr;
+        P->pair_nnz.assign(n, 0);
+        P->pair_off.assign(n, 0);
+        if (!k.empty && !k.sink && first.nnz > 0) {
+            // the fused output's per-pair counts are fixed by the coordinates
+            const uint64_t fn = (uint64_t)first.nnz;
+            P->split_counts = Tmp<unsigned long long>(pool, n);
+            P->split_offs = Tmp<unsigned long long>(pool, n);
+            P->split_cursor = Tmp<unsigned long long>(pool, n);
+            fill_u64(P->split_counts, n, 0, s);
+            split_count(first.coords, fn, (uint32_t)n, P->split_counts, s);
+            FCC_HIP(hipMemcpyAsync(P->pair_nnz.data(), P->split_counts.get(), 8ull * n, hipMemcpyDeviceToHost, s));
+            FCC_HIP(hipStreamSynchronize(s));
+            unsigned long long acc = 0;
+            for (int32_t i = 0; i < n; i++) {
+                P->pair_off[i] = acc;
+                acc += P->pair_nnz[i];
+            }
+            FCC_HIP(hipMemcpyAsync(P->split_offs.get(), P->pair_off.data(), 8ull * n, hipMemcpyHostToDevice, s));
+            if (dC) P->split_c = Tmp<uint32_t>(pool, (size_t)dC * fn);
+            P->split_v = Tmp<double>(pool, (size_t)fn);
+            FCC_HIP(hipStreamSynchronize(s));
+        }
+        *plan = P.release();
+        ctx->err.clear();
+        return FCC_OK;
+    } catch (HipError &e) {
+        (void)hipGetLastError();
+        return fail(ctx, FCC_ERR_HIP, e.where + ": " + hipGetErrorString(e.code));
+    } catch (LibError &e) {
+        return fail(ctx, e.code, e.msg);
+    } catch (std::bad_alloc &) {
+        return fail(ctx, FCC_ERR_OOM, "host allocation failed");
+    }
+}
+
+int fcc_plan_execute_batch(fcc_plan *plan, const double *a_vals, const double *b_vals, int32_t on_device,
+                           int32_t use_graph, fcc_result *outs) {
+    if (!plan) return FCC_ERR_INVALID;
+    fcc_ctx *ctx = plan->ctx;
+    if (!plan->pairs) return fail(ctx, FCC_ERR_INVALID, "not a batch plan (fcc_plan_create_batch)");
+    if (!outs) return fail(ctx, FCC_ERR_INVALID, "outs is NULL");
+    const uint32_t n = plan->pairs;
+    for (uint32_t i = 0; i < n; i++) memset(&outs[i], 0, sizeof(fcc_result));
+    if ((plan->nA && !a_vals) || (plan->nB && !b_vals)) return fail(ctx, FCC_ERR_INVALID, "NULL values");
+    Contraction::Kept &k = plan->c->k;
+    const int dC = plan->pair_order;
+    for (uint32_t i = 0; i < n; i++) {
+        outs[i].order = dC;
+        for (int m = 0; m < dC; m++) outs[i].extents[m] = plan->extents[m + 1];
+        outs[i].stats = plan->stats;
+    }
+    if (k.empty) return FCC_OK;
+    try {
+        std::vector<unsigned long long> sk, gd;
+        fcc_stats st;
+        run_plan(plan, a_vals, b_vals, on_device, use_graph, sk, gd, st);
+        // a pair's output is complete when its last unit ends (entries that
+        // went through the global hash only after the drain)
+        const double t0 = ev_ms(ctx->ev[0], ctx->ev[4]);
+        const unsigned long long c0 = gd[n];
+        for (uint32_t i = 0; i < n; i++) {
+            fcc_result &r = outs[i];
+            r.stats = st;
+            if (!st.ghash_entries && c0 && gd[i] >= c0)
+                r.stats.ms_done = t0 + (double)(gd[i] - c0) / (double)ctx->clock_khz;
+            if (k.sink) {
+                const unsigned long long *g4 = &sk[4 * (size_t)i];
+                r.nnz = (int64_t)g4[0];
+                r.coord_hash = g4[0] ? g4[1] : 0;
+                memcpy(&r.val_sum, &g4[2], 8);
+                memcpy(&r.val_wsum, &g4[3], 8);
+            } else {
+                r.nnz = (int64_t)plan->pair_nnz[i];
+                r.coord_stride = r.nnz;
+                if (r.nnz) {
+                    r.coords = dC ? plan->split_c.get() + (size_t)dC * plan->pair_off[i] : nullptr;
+                    r.vals = plan->split_v.get() + plan->pair_off[i];
+                }
+            }
+        }
         ctx->err.clear();
         return FCC_OK;
     } catch (HipError &e) {

@@ -193,6 +193,32 @@ class Plan:
             pass
 
 
+class BatchPlan(Plan):
+    """A fused batch planned for its pairs' coordinate patterns
+    (fcc_plan_create_batch): execute() reruns the numeric pass of every pair
+    on new values, optionally as one replayed hipGraph."""
+
+    def __init__(self, ctx: "Context", h, n: int, a_nnz, b_nnz):
+        super().__init__(ctx, h)
+        self.n = n
+        self.a_off = np.concatenate([[0], np.cumsum(a_nnz)]).astype(np.int64)
+        self.b_off = np.concatenate([[0], np.cumsum(b_nnz)]).astype(np.int64)
+
+    def execute(self, a_vals, b_vals, graph=False) -> list:
+        """a_vals / b_vals: every pair's values in pair order (one array each:
+        host float64, or device tensors); one Result view per pair, valid
+        until the next execute."""
+        keep = []
+        pa, da = _vals_arg(a_vals, keep)
+        pb, db = _vals_arg(b_vals, keep)
+        if da != db:
+            raise FastccError(_abi.FCC_ERR_INVALID, "a_vals and b_vals must both be host or both device")
+        raws = (_abi.FccResult * max(self.n, 1))()
+        rc = self._ctx.lib.fcc_plan_execute_batch(self.h, pa, pb, da, 1 if graph else 0, raws)
+        self._ctx._check(rc)
+        return [Result(self._ctx, raws[i]) for i in range(self.n)]
+
+
 class Context:
     """One GPU, one HIP stream, a caching device pool and the MLP planner."""
 
@@ -289,6 +315,24 @@ class Context:
                                       C.byref(h))
         self._check(rc)
         return Plan(self, h)
+
+    def plan_batch(self, pairs, a_modes, b_modes, **opts) -> BatchPlan:
+        """fcc_plan_create_batch: plans (and performs once) the fused batch."""
+        keep = []
+        n = len(pairs)
+        ta = (_abi.FccTensor * max(n, 1))()
+        tb = (_abi.FccTensor * max(n, 1))()
+        for i, (a, b) in enumerate(pairs):
+            ta[i] = _as_fcc_tensor(a, keep)
+            tb[i] = _as_fcc_tensor(b, keep)
+        r = len(a_modes)
+        am = (C.c_int32 * max(r, 1))(*a_modes)
+        bm = (C.c_int32 * max(r, 1))(*b_modes)
+        o = self.options(**opts)
+        h = C.c_void_p()
+        rc = self.lib.fcc_plan_create_batch(self.h, n, ta, am, tb, bm, r, C.byref(o), C.byref(h))
+        self._check(rc)
+        return BatchPlan(self, h, n, [ta[i].nnz for i in range(n)], [tb[i].nnz for i in range(n)])
 
     def contract_batch(self, pairs, a_modes, b_modes, **opts):
         keep = []

@@ -23,7 +23,8 @@ EXPORTS = (
     "fcc_contract_batch", "fcc_result_to_host", "fcc_result_free", "fcc_mlp_param_count", "fcc_mlp_set_params",
     "fcc_mlp_get_params", "fcc_tile_stats_count", "fcc_tile_stats_copy", "fcc_mlp_train",
     "fcc_mlp_forward_backward", "fcc_density_bucket", "fcc_tune_set", "fcc_tune_get",
-    "fcc_set_memory_budget", "fcc_memory_stats", "fcc_contract_noalloc",
+    "fcc_set_memory_budget", "fcc_memory_stats", "fcc_contract_noalloc", "fcc_plan_create_batch",
+    "fcc_plan_execute_batch",
 )
 FCC_NOALLOC_SCRATCH_BYTES = 64 << 20
 
@@ -213,6 +214,12 @@ def load(path: str | None = None):
     lib.fcc_plan_tiles.restype = C.c_int
     lib.fcc_plan_execute.argtypes = [P, P, P, C.c_int32, C.c_int32, C.POINTER(FccResult)]
     lib.fcc_plan_execute.restype = C.c_int
+    lib.fcc_plan_create_batch.argtypes = [P, C.c_int32, C.POINTER(FccTensor), C.POINTER(C.c_int32),
+                                          C.POINTER(FccTensor), C.POINTER(C.c_int32), C.c_int32,
+                                          C.POINTER(FccOptions), C.POINTER(P)]
+    lib.fcc_plan_create_batch.restype = C.c_int
+    lib.fcc_plan_execute_batch.argtypes = [P, P, P, C.c_int32, C.c_int32, C.POINTER(FccResult)]
+    lib.fcc_plan_execute_batch.restype = C.c_int
     lib.fcc_plan_destroy.argtypes = [P]
     lib.fcc_plan_destroy.restype = None
     lib.fcc_contract_noalloc.argtypes = [P, C.POINTER(FccTensor), C.POINTER(C.c_int32), C.POINTER(FccTensor),

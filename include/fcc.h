@@ -267,6 +267,25 @@ int fcc_plan_tiles(fcc_plan *plan, uint64_t *nnz, uint64_t *products, float *pre
  * Executions of one plan must not overlap. */
 int fcc_plan_execute(fcc_plan *plan, const double *a_vals, const double *b_vals, int32_t on_device,
                      int32_t use_graph, fcc_result *out);
+/* Batch plans: fcc_contract_batch's n pairs, planned (and performed once) as
+ * its one fused launch; every pair's coordinates are then fixed.
+ * fcc_plan_execute_batch reruns the numeric pass on new values of all pairs,
+ * eagerly or as ONE replayed hipGraph (use_graph: value staging, the
+ * accumulate launch, the drain and the per-pair split captured together, no
+ * host read-back inside): a_vals holds every pair's A values in pair order
+ * (pair i's a[i].nnz values after those of pairs 0..i-1), b_vals likewise.
+ * outs[i] (n of them) as fcc_contract_batch's; materialised outputs are views
+ * of the plan's buffers, valid until the next execution or fcc_plan_destroy;
+ * stats.ms_done is pair i's completion time. All pairs must fit one fused
+ * launch (FCC_ERR_INVALID otherwise: split the batch). The reference runs one
+ * fastcc_multiply per pair (benchmark.cpp:68-73). fcc_plan_info_get and
+ * fcc_plan_tiles describe the fused contraction (the pair index leads the
+ * output coordinates); fcc_plan_execute rejects a batch plan. */
+int fcc_plan_create_batch(fcc_ctx *ctx, int32_t n, const fcc_tensor *a, const int32_t *a_modes,
+                          const fcc_tensor *b, const int32_t *b_modes, int32_t n_contr,
+                          const fcc_options *options, fcc_plan **plan);
+int fcc_plan_execute_batch(fcc_plan *plan, const double *a_vals, const double *b_vals, int32_t on_device,
+                           int32_t use_graph, fcc_result *outs);
 /* Destroy plans before their context. */
 void fcc_plan_destroy(fcc_plan *plan);
 
