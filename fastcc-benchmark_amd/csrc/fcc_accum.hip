@@ -202,6 +202,7 @@ struct SinkAcc {
     uint64_t count = 0, hash = 0;
     double vsum = 0.0, wsum = 0.0;
     uint64_t group = 0;  // row group this accumulator belongs to
+    uint64_t glo = 0;    // its first row (group * group_rows)
 };
 
 __device__ __forceinline__ double hash_weight(uint64_t z) { return (double)(z >> 11) * 0x1.0p-53; }
@@ -218,9 +219,12 @@ __device__ __forceinline__ void emit_output(const AccArgs &A, uint64_t pos, uint
                                             bool sink, SinkAcc &sk) {
     if (sink) {
         if (A.group_rows) {
-            const uint64_t g = row / A.group_rows;
-            row -= g * A.group_rows;
-            if (g != sk.group) {  // a unit straddling two groups: add this output directly
+            // rows and group sizes are below 2^32 (left spans are)
+            if (row - sk.glo < A.group_rows) {
+                row -= sk.glo;
+            } else {  // a unit straddling two groups: add this output directly
+                const uint32_t g = (uint32_t)row / (uint32_t)A.group_rows;
+                row -= (uint64_t)g * A.group_rows;
                 uint64_t z = mix64(row * A.spanB + col);
                 sink_atomic(A, g, 1ull, z, v, v * hash_weight(z));
                 return;
@@ -1186,6 +1190,7 @@ __global__ __launch_bounds__(ACC_THREADS, 4) void k_numeric(AccArgs A) {
             sink_flush(A, sk, s_su, s_sd);
             sk = SinkAcc();
             sk.group = U.row0 / A.group_rows;
+            sk.glo = sk.group * A.group_rows;
         }
         if (A.clock_start && ui == 0 && threadIdx.x == 0) *A.clock_start = wall_clock64();
         if (U.kind == KIND_DENSE && A.dk_tile && A.dk_tile[U.a]) {

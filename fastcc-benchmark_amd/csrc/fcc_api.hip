@@ -331,15 +331,22 @@ __global__ void k_labels(const unsigned long long *tile_nnz, const unsigned long
 }
 
 // MAPE of predicted vs actual tile nnz: sum |p - a| / max(a, 1)
-__global__ void k_mape(const float *pred, const unsigned long long *actual, uint32_t n, double *acc) {
-    uint32_t a = blockIdx.x * blockDim.x + threadIdx.x;
+// Grid-stride over the tiles (a capped grid): one atomic per workgroup.
+__global__ __launch_bounds__(256) void k_mape(const float *pred, const unsigned long long *actual, uint32_t n,
+                                              double *acc) {
+    __shared__ double s_e[4];
     double e = 0.0;
-    if (a < n) {
-        double act = (double)actual[a];
-        e = fabs((double)pred[a] - act) / (act > 1.0 ? act : 1.0);
+    for (uint32_t a = blockIdx.x * 256 + threadIdx.x; a < n; a += gridDim.x * 256) {
+        const double act = (double)actual[a];
+        e += fabs((double)pred[a] - act) / (act > 1.0 ? act : 1.0);
     }
     for (int d = 32; d > 0; d >>= 1) e += __shfl_xor(e, d, 64);
-    if ((threadIdx.x & 63) == 0 && e != 0.0) unsafeAtomicAdd(acc, e);
+    if ((threadIdx.x & 63) == 0) s_e[threadIdx.x >> 6] = e;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const double t = s_e[0] + s_e[1] + s_e[2] + s_e[3];
+        if (t != 0.0) unsafeAtomicAdd(acc, t);
+    }
 }
 
 // The same over every S-th tile (acc[1] counts them).
@@ -370,22 +377,29 @@ __global__ void k_sample_tiles(uint32_t *t_nwin, const uint8_t *t_kind, uint32_t
 }
 
 // Sum of predicted tile outputs, each capped by its exact bounds (products, area).
-__global__ void k_pred_sum(const float *pred, const uint64_t *st_products, const uint64_t *st_maxgp,
+__global__ __launch_bounds__(256) void k_pred_sum(const float *pred, const uint64_t *st_products, const uint64_t *st_maxgp,
                            const uint8_t *st_direct, const uint64_t *st_outd, double area, uint32_t n,
                            unsigned long long *sum) {
-    uint32_t a = blockIdx.x * blockDim.x + threadIdx.x;
+    __shared__ unsigned long long s_v[4];
     unsigned long long v = 0ull;
-    if (a < n && st_direct[a]) {
-        v = st_outd[a];  // DIRECT tiles: exactly their outputs
-    } else if (a < n) {
-        double p = ceil((double)pred[a]);
-        p = p < (double)st_maxgp[a] ? (double)st_maxgp[a] : p;
-        p = p > (double)st_products[a] ? (double)st_products[a] : p;
-        p = p > area ? area : p;
-        v = (unsigned long long)p;
+    for (uint32_t a = blockIdx.x * 256 + threadIdx.x; a < n; a += gridDim.x * 256) {
+        if (st_direct[a]) {
+            v += st_outd[a];  // DIRECT tiles: exactly their outputs
+        } else {
+            double p = ceil((double)pred[a]);
+            p = p < (double)st_maxgp[a] ? (double)st_maxgp[a] : p;
+            p = p > (double)st_products[a] ? (double)st_products[a] : p;
+            p = p > area ? area : p;
+            v += (unsigned long long)p;
+        }
     }
     for (int d = 32; d > 0; d >>= 1) v += __shfl_xor(v, d, 64);
-    if ((threadIdx.x & 63) == 0 && v) atomicAdd(sum, v);
+    if ((threadIdx.x & 63) == 0) s_v[threadIdx.x >> 6] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const unsigned long long t = s_v[0] + s_v[1] + s_v[2] + s_v[3];
+        if (t) atomicAdd(sum, t);
+    }
 }
 
 __global__ void k_summarize(const uint32_t *coords, const double *vals, uint64_t n, uint64_t stride, int d,
@@ -891,7 +905,7 @@ int Contraction::run(const fcc_tensor *ta, const int32_t *am, const fcc_tensor *
     Tmp<unsigned long long> psum(pool, 1);
     if (learned) {
         fill_u64(psum, 1, 0, s);
-        k_pred_sum<<<grid256(T), 256, 0, s>>>(pred_t, (const uint64_t *)ix.st_products,
+        k_pred_sum<<<std::min(grid256(T), 2048u), 256, 0, s>>>(pred_t, (const uint64_t *)ix.st_products,
                                                (const uint64_t *)ix.st_maxgp, ix.st_direct,
                                                (const uint64_t *)ix.st_outd, (double)ix.TL * (double)g.spanB, T,
                                                psum);
@@ -1057,7 +1071,7 @@ int Contraction::run(const fcc_tensor *ta, const int32_t *am, const fcc_tensor *
                 k_unit_to_tile<<<grid256(U.n), 256, 0, s>>>(unit_cur, U.tile, U.kind, U.n, tn);
                 k_direct_tile_count<<<grid256(T), 256, 0, s>>>(ix.st_direct, (const uint64_t *)ix.st_outd, T,
                                                                P.force, tn);
-                k_mape<<<grid256(T), 256, 0, s>>>(pred_t, tn, T, reinterpret_cast<double *>(tail.get() + 8));
+                k_mape<<<std::min(grid256(T), 2048u), 256, 0, s>>>(pred_t, tn, T, reinterpret_cast<double *>(tail.get() + 8));
             }
             fetch_tail();
             st.ghash_entries = hm[0];
@@ -1315,7 +1329,9 @@ std::vector<uint32_t> chunk_pairs(fcc_ctx *ctx, int32_t n, const fcc_tensor *a, 
             Tmp<unsigned char> dps(pool, sizeof(PairSrc) * dev.size());
             Tmp<uint32_t> dmx(pool, dev.size() * d);
             FCC_HIP(hipMemcpyAsync(dps.get(), dev.data(), sizeof(PairSrc) * dev.size(), hipMemcpyHostToDevice, s));
-            pair_max((const PairSrc *)dps.get(), (uint32_t)dev.size(), d, dmx, s);
+            uint64_t mx_nnz = 0;
+            for (const PairSrc &q : dev) mx_nnz = std::max<uint64_t>(mx_nnz, q.nnz);
+            pair_max((const PairSrc *)dps.get(), (uint32_t)dev.size(), d, mx_nnz, dmx, s);
             std::vector<uint32_t> hm(dev.size() * d);
             FCC_HIP(hipMemcpyAsync(hm.data(), dmx.get(), 4 * hm.size(), hipMemcpyDeviceToHost, s));
             FCC_HIP(hipStreamSynchronize(s));
@@ -2100,8 +2116,11 @@ struct fcc_plan {
     bool self_values = false;  // b's values are a's (the same tensor on both sides)
     int64_t nA = 0, nB = 0;
     Tmp<double> a_stage, b_stage;  // value staging: the graph reads fixed addresses
-    hipGraph_t graph = nullptr;
-    hipGraphExec_t exec = nullptr;
+    // one graph per accumulate variant (dense slots marked empty by -0.0, or
+    // by a bitmap when a value is zero or tiny)
+    hipGraph_t graph[2] = {nullptr, nullptr};
+    hipGraphExec_t exec[2] = {nullptr, nullptr};
+    Tmp<uint32_t> vflag;  // value flags of an execution's inputs
     fcc_stats stats{};
     fcc_plan_info info{};
     int32_t order = 0;
@@ -2116,8 +2135,10 @@ struct fcc_plan {
     Tmp<uint32_t> split_c;
     Tmp<double> split_v;
     ~fcc_plan() {
-        if (exec) (void)hipGraphExecDestroy(exec);
-        if (graph) (void)hipGraphDestroy(graph);
+        for (int i = 0; i < 2; i++) {
+            if (exec[i]) (void)hipGraphExecDestroy(exec[i]);
+            if (graph[i]) (void)hipGraphDestroy(graph[i]);
+        }
     }
 };
 
@@ -2129,10 +2150,23 @@ namespace {
 // the persistent accumulate kernel and the global-hash drain.
 __global__ void k_stamp_clock(unsigned long long *dst) { *dst = wall_clock64(); }
 
+// flag |= 1 when a value is zero or below 2^-511 in magnitude (a product of
+// two could be +-0), |= 2 when a value is not finite
+__global__ __launch_bounds__(256) void k_value_flags(const double *v, uint64_t n, uint32_t *flag) {
+    uint32_t f = 0;
+    for (uint64_t i = blockIdx.x * 256ull + threadIdx.x; i < n; i += (uint64_t)gridDim.x * 256) {
+        const double x = v[i];
+        if (!(fabs(x) >= 0x1.0p-511) && !isnan(x)) f |= 1u;
+        if (!isfinite(x)) f |= 2u;
+    }
+    const uint64_t b1 = __ballot(f & 1u), b2 = __ballot(f & 2u);
+    if ((threadIdx.x & 63) == 0 && (b1 | b2)) atomicOr(flag, (b1 ? 1u : 0u) | (b2 ? 2u : 0u));
+}
+
 // A batch plan also stamps the device clock (the pairs' completion times are
 // taken against it), re-arms the per-pair unit counters, and after the drain
 // splits the fused output into the per-pair blocks.
-void plan_sequence(fcc_plan *P, const double *asrc, const double *bsrc, hipStream_t s) {
+void plan_sequence(fcc_plan *P, const double *asrc, const double *bsrc, bool bitmap, hipStream_t s) {
     Contraction::Kept &k = P->c->k;
     AccArgs &X = k.X;
     if (P->pairs) {
@@ -2150,7 +2184,7 @@ void plan_sequence(fcc_plan *P, const double *asrc, const double *bsrc, hipStrea
     fill_u32(k.flags, 4, 0, s);
     fill_u32(k.work, 4, 0, s);
     fill_u64(k.sinkbuf, k.nsink, 0, s);
-    launch_numeric(X, k.grid, k.bitmap, k.sink, 0, s);
+    launch_numeric(X, k.grid, bitmap, k.sink, 0, s);
     launch_gdrain(X, k.sink, 0, s);
     if (P->pairs && P->split_v.get()) {
         fill_u64(P->split_cursor, P->pairs, 0, s);
@@ -2186,25 +2220,43 @@ void run_plan(fcc_plan *plan, const double *a_vals, const double *b_vals, int32_
             bsrc = plan->b_stage.get();
         }
     }
+    // The values pick the accumulate variant: without zero or tiny values no
+    // sum of products is -0.0, which then marks an empty dense slot (the
+    // bitmap variant costs about 10% on sparse-output shapes).
+    if (!plan->vflag.get()) plan->vflag = Tmp<uint32_t>(ctx->pool, 1);
+    fill_u32(plan->vflag, 1, 0, s);
+    const unsigned vgrid = (unsigned)std::min<uint64_t>(1024, std::max<int64_t>(1, (plan->nA + 4095) / 4096));
+    if (plan->nA) k_value_flags<<<vgrid, 256, 0, s>>>(asrc, (uint64_t)plan->nA, plan->vflag);
+    if (!self && plan->nB) {
+        const unsigned g2 = (unsigned)std::min<uint64_t>(1024, std::max<int64_t>(1, (plan->nB + 4095) / 4096));
+        k_value_flags<<<g2, 256, 0, s>>>(bsrc, (uint64_t)plan->nB, plan->vflag);
+    }
+    uint32_t vf = 0;
+    FCC_HIP(hipMemcpyAsync(&vf, plan->vflag.get(), 4, hipMemcpyDeviceToHost, s));
+    FCC_HIP(hipStreamSynchronize(s));
+    if ((vf & 2u) && k.dk.nh)
+        throw LibError{FCC_ERR_INVALID, "non-finite values in a plan that multiplies dense keys on the matrix cores "
+                                        "(a zero-padded panel would spread them; plan with dense_keys < 0)"};
+    const int variant = (vf & 1u) ? 1 : 0;
     FCC_HIP(hipEventRecord(ev[4], s));
     if (use_graph) {
-        if (!plan->exec) {
+        if (!plan->exec[variant]) {
             hipGraph_t g = nullptr;
             FCC_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
             try {
-                plan_sequence(plan, asrc, bsrc, s);
+                plan_sequence(plan, asrc, bsrc, variant == 1, s);
             } catch (...) {
                 (void)hipStreamEndCapture(s, &g);
                 if (g) (void)hipGraphDestroy(g);
                 throw;
             }
             FCC_HIP(hipStreamEndCapture(s, &g));
-            plan->graph = g;
-            FCC_HIP(hipGraphInstantiate(&plan->exec, g, nullptr, nullptr, 0));
+            plan->graph[variant] = g;
+            FCC_HIP(hipGraphInstantiate(&plan->exec[variant], g, nullptr, nullptr, 0));
         }
-        FCC_HIP(hipGraphLaunch(plan->exec, s));
+        FCC_HIP(hipGraphLaunch(plan->exec[variant], s));
     } else {
-        plan_sequence(plan, asrc, bsrc, s);
+        plan_sequence(plan, asrc, bsrc, variant == 1, s);
     }
     FCC_HIP(hipEventRecord(ev[5], s));
     uint32_t hf[2];

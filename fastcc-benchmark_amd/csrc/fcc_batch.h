@@ -19,8 +19,9 @@ struct PairSrc {
     uint64_t nnz;
 };
 
-// out[i * order + m] = max coordinate of mode m of pair i (0 for an empty pair).
-void pair_max(const PairSrc *src, uint32_t n, int order, uint32_t *out, hipStream_t s);
+// out[i * order + m] = max coordinate of mode m of pair i (0 for an empty pair);
+// max_nnz: the largest pair's entry count (sizes the launch).
+void pair_max(const PairSrc *src, uint32_t n, int order, uint64_t max_nnz, uint32_t *out, hipStream_t s);
 
 // dst_coords: (order + 1) rows of `total`; row `order` receives the pair
 // index. offs: n + 1 exclusive offsets (device). dmax[m] |= max coordinate of

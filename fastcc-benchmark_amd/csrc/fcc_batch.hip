@@ -35,17 +35,29 @@ __device__ __forceinline__ uint32_t wave_add(uint32_t *s_h, bool t, uint32_t key
     return t ? atomicAdd(&s_h[key], 1u) : 0u;
 }
 
-// One workgroup per pair: the maximum coordinate of each mode (0 when empty).
+__global__ void k_zero_u32(uint32_t *p, uint64_t n) {
+    for (uint64_t i = blockIdx.x * 256ull + threadIdx.x; i < n; i += (uint64_t)gridDim.x * 256) p[i] = 0u;
+}
+
+// The maximum coordinate of each mode per pair (0 when empty), atomically
+// into out (zeroed by the caller): blockIdx.y is the pair, blockIdx.x one of
+// gridDim.x equal slices of its entries (slices past its end exit at once).
 __global__ __launch_bounds__(256) void k_pair_max(const PairSrc *src, int order, uint32_t *out) {
     __shared__ uint32_t s_m[4];
-    const PairSrc ps = src[blockIdx.x];
+    const PairSrc ps = src[blockIdx.y];
+    const uint64_t per = (ps.nnz + gridDim.x - 1) / gridDim.x;
+    const uint64_t j0 = (uint64_t)blockIdx.x * per, j1 = min(ps.nnz, j0 + per);
+    if (j0 >= j1) return;
     for (int m = 0; m < order; m++) {
+        const uint32_t *c = ps.coords + (uint64_t)m * ps.nnz;
         uint32_t v = 0;
-        for (uint64_t j = threadIdx.x; j < ps.nnz; j += 256) v = max(v, ps.coords[(uint64_t)m * ps.nnz + j]);
+#pragma unroll 4
+        for (uint64_t j = j0 + threadIdx.x; j < j1; j += 256) v = max(v, c[j]);
         for (int d = 32; d > 0; d >>= 1) v = max(v, (uint32_t)__shfl_xor((int)v, d, 64));
         if ((threadIdx.x & 63) == 0) s_m[threadIdx.x >> 6] = v;
         __syncthreads();
-        if (threadIdx.x == 0) out[(uint64_t)blockIdx.x * order + m] = max(max(s_m[0], s_m[1]), max(s_m[2], s_m[3]));
+        if (threadIdx.x == 0)
+            atomicMax(&out[(uint64_t)blockIdx.y * order + m], max(max(s_m[0], s_m[1]), max(s_m[2], s_m[3])));
         __syncthreads();
     }
 }
@@ -180,8 +192,16 @@ __global__ void k_group_units(const uint32_t *unit_tile, const uint32_t *atile_i
 
 }  // namespace
 
-void pair_max(const PairSrc *src, uint32_t n, int order, uint32_t *out, hipStream_t s) {
-    if (n) k_pair_max<<<n, 256, 0, s>>>(src, order, out);
+void pair_max(const PairSrc *src, uint32_t n, int order, uint64_t max_nnz, uint32_t *out, hipStream_t s) {
+    if (!n) return;
+    k_zero_u32<<<blocks_for((uint64_t)n * order, 1024), 256, 0, s>>>(out, (uint64_t)n * order);
+    // about 64k entries per slice, at most 256 slices of the largest pair;
+    // at most 65535 pairs per launch (grid y)
+    const uint32_t slices = (uint32_t)std::min<uint64_t>(256, std::max<uint64_t>(1, (max_nnz + 65535) / 65536));
+    for (uint32_t p0 = 0; p0 < n; p0 += 65535) {
+        const uint32_t np = std::min<uint32_t>(65535, n - p0);
+        k_pair_max<<<dim3(slices, np), 256, 0, s>>>(src + p0, order, out + (uint64_t)p0 * order);
+    }
 }
 
 void gather_pairs(const PairSrc *src, const unsigned long long *offs, uint32_t n, int order, uint64_t total,

@@ -188,3 +188,38 @@ def test_plan_errors(gpu_ctx):
     plan = gpu_ctx.plan(e, [1], b, [0])
     assert plan.execute(np.zeros(0), np.ones(b.nnz)).nnz == 0
     plan.close()
+
+
+def test_plan_nonfinite_values_with_dense_keys(gpu_ctx):
+    """A plan that multiplies dense keys on the matrix cores refuses values
+    holding inf or NaN (a zero-padded panel would spread them to cells they
+    never reach) and keeps working with finite values; a plan made without
+    dense keys takes them, and matches the oracle."""
+    wl = W.c2(0.1)
+    a, _ = wl.tensors()
+    rng = np.random.default_rng(4700)
+    r = gpu_ctx.contract(a, list(wl.a_modes), a, list(wl.b_modes), dense_keys=1e-3)
+    assert r.stats["dense_keys"] > 0
+    r.free()
+    for dk in (1e-3, -1.0):
+        plan = gpu_ctx.plan(a, list(wl.a_modes), a, list(wl.b_modes), dense_keys=dk)
+        v = rng.uniform(-1, 1, a.nnz)
+        v[7] = np.inf
+        if dk > 0:
+            with pytest.raises(FastccError) as ei:
+                plan.execute(v)
+            assert ei.value.code == -1
+        else:
+            r = plan.execute(v, graph=True)
+            ref = oracle.contract_tiled(_with_vals(a, v), wl.a_modes, _with_vals(a, v), wl.b_modes, tile_size=128)
+            c, vv = r.to_host()
+            assert_same_result(c, vv, ref.coords, ref.vals, rel=REL, abs_floor=ABS, what="inf without dense keys",
+                               absum=ref.absum, nterms=ref.nterms)
+        v[7] = 0.5
+        for graph in (False, True):
+            r = plan.execute(v, graph=graph)
+            ref = oracle.contract_tiled(_with_vals(a, v), wl.a_modes, _with_vals(a, v), wl.b_modes, tile_size=128)
+            c, vv = r.to_host()
+            assert_same_result(c, vv, ref.coords, ref.vals, rel=REL, abs_floor=ABS, what=f"dense_keys {dk}",
+                               absum=ref.absum, nterms=ref.nterms)
+        plan.close()
