@@ -389,8 +389,12 @@ def bench_batch(args):
     ctx.set_mlp_params(np.load(MLP_PARAMS))
     a_modes, b_modes = list(wls[0].a_modes), list(wls[0].b_modes)
 
+    force = {"auto": 0, "dense": 1, "lhash": 2, "ghash": 3}[args.force_acc]
+    bopts = dict(planner=args.planner, sink=True, tile_hint=args.tile_hint, col_window=args.col_window,
+                 workgroups=args.workgroups, force_acc=force)
+
     def step():
-        rs = ctx.contract_batch(pairs, a_modes, b_modes, planner=args.planner, sink=True)
+        rs = ctx.contract_batch(pairs, a_modes, b_modes, **bopts)
         n = sum(r.nnz for r in rs)
         lat = [r.stats["ms_done"] for r in rs]
         st = rs[0].stats
@@ -425,7 +429,7 @@ def bench_batch(args):
         # fixed; made once, untimed), then per step ONE hipGraph replay of
         # value staging + accumulate + drain on new values of every pair.
         t = time.perf_counter()
-        plan = ctx.plan_batch(pairs, a_modes, b_modes, planner=args.planner, sink=True)
+        plan = ctx.plan_batch(pairs, a_modes, b_modes, **bopts)
         plan_s = time.perf_counter() - t
         A1, B1 = torch.cat(vals_a), torch.cat(vals_b)
         A2, B2 = A1 * 0.5, B1 * -2.0
@@ -484,6 +488,8 @@ def bench_batch(args):
                                   "A 10k-2M / B 5k-1M, mode-1 Zipf(1.1), normal values",
                    "planner": args.planner, "output": "count + checksum per pair", "output_nnz": int(nnz),
                    "products": int(stats[-1]["products"]),
+                   "row_tile": int(stats[-1]["row_tile"]), "left_tiles": int(stats[-1]["left_tiles"]),
+                   "units": {k: int(stats[-1][f"units_{k}"]) for k in ("dense", "lhash", "ghash", "direct")},
                    "nnz_a": int(sum(t[0].nnz for t in host)), "nnz_b": int(sum(t[1].nnz for t in host))},
         "latency_ms": {"p50": float(np.percentile(lats, 50)), "p99": float(np.percentile(lats, 99)),
                        "max": float(np.max(lats)),

@@ -696,14 +696,22 @@ __device__ __forceinline__ void heavy_claimed(const AccArgs &A, const UnitCtx &U
         if (lane < nb) d = desc[bg + lane];
         uint32_t s_l = d.x, e_l = d.x + d.y;
         if (A.unclipped && lane < nb && d.y) s_l = A.fiber_ptr[A.group_fiber[bg + lane]];
-        const bool cuts = U.multi && !seg && A.fcut && U.kind == KIND_DENSE;
+        const bool cuts = U.multi && A.fcut && U.kind == KIND_DENSE;
         if (cuts && lane < nb && d.y) {
             // precomputed window cuts of the group's fiber (in mirror mode the
-            // descriptor may start later: no cell left of the tile's first row)
+            // descriptor may start later: no cell left of the tile's first
+            // row); a heavy segment, a piece of its group's fiber, keeps the
+            // part of the piece inside the cuts
             const uint32_t w = U.c0 / A.fcut_W;
-            const uint32_t *fc = A.fcut + (size_t)A.group_fiber[bg + lane] * A.fcut_n + w;
-            e_l = fc[1];
-            s_l = A.mirror ? min(max(fc[0], d.x), e_l) : fc[0];
+            const uint32_t f = seg ? A.Afib[d.z] : A.group_fiber[bg + lane];
+            const uint32_t *fc = A.fcut + (size_t)f * A.fcut_n + w;
+            if (seg) {
+                e_l = min(e_l, fc[1]);
+                s_l = min(max(s_l, fc[0]), e_l);
+            } else {
+                e_l = fc[1];
+                s_l = A.mirror ? min(max(fc[0], d.x), e_l) : fc[0];
+            }
         } else if (U.multi && lane < nb && s_l < e_l) {
             // segments wholly outside the window are dropped on their end columns
             if (A.Bcol[s_l] >= cend || A.Bcol[e_l - 1] < U.c0) {
