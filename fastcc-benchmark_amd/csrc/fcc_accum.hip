@@ -1769,31 +1769,31 @@ __global__ void k_refine_kinds(const unsigned long long *unit_count, const uint3
 
 // fcut[f * n + w] = first position of fiber f whose column is >= w * W
 // (w = 0..n-1; w = n-1 is the fiber's end when (n-1) * W >= spanB)
-__global__ void k_fiber_cuts(const uint32_t *fiber_ptr, const uint32_t *Bcol, uint32_t nfib, uint32_t n,
-                             uint32_t W, uint32_t *fcut) {
-    const uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
-    if (i >= (uint64_t)nfib * n) return;
-    const uint32_t f = (uint32_t)(i / n), w = (uint32_t)(i - (uint64_t)f * n);
-    uint32_t lo = fiber_ptr[f], hi = fiber_ptr[f + 1];
-    const uint64_t x = (uint64_t)w * W;
-    if (x > 0xFFFFFFFFull) {
-        fcut[i] = hi;
-        return;
-    }
-    while (lo < hi) {
-        const uint32_t mid = (lo + hi) >> 1;
-        if (Bcol[mid] < (uint32_t)x) lo = mid + 1;
-        else hi = mid;
-    }
-    fcut[i] = lo;
+// fcut[f * n + w] = the first position of fiber f whose column is >= w * W
+// (the fiber's end when none), w < n. One thread per right position p: the
+// windows from just after its predecessor's window (window 0 at the fiber's
+// first entry) up to its own window all start at p, and the fiber's last
+// entry also closes the windows after its own. Each cut is written once, by
+// one sequential pass over the right operand.
+__global__ void k_fiber_cuts(const uint32_t *fiber_ptr, const uint32_t *Bfib, const uint32_t *Bcol, uint32_t nB,
+                             uint32_t n, uint32_t W, uint32_t *fcut) {
+    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= nB) return;
+    const uint32_t f = Bfib[p];
+    const uint32_t f0 = fiber_ptr[f], f1 = fiber_ptr[f + 1];
+    const uint32_t w = min(Bcol[p] / W, n - 1);
+    const uint32_t wlo = p == f0 ? 0u : Bcol[p - 1] / W + 1;
+    uint32_t *fc = fcut + (size_t)f * n;
+    for (uint32_t x = wlo; x <= w; x++) fc[x] = p;
+    if (p + 1 == f1)
+        for (uint32_t x = w + 1; x < n; x++) fc[x] = f1;
 }
 
 // ------------------------------------------------------------ launchers
 
 void launch_fiber_cuts(const Index &ix, uint32_t n, uint32_t W, uint32_t *fcut, hipStream_t s) {
-    const uint64_t tot = (uint64_t)ix.nfib * n;
-    if (!tot) return;
-    k_fiber_cuts<<<(unsigned)div_up(tot, 256), 256, 0, s>>>(ix.fiber_ptr, ix.Bcol, ix.nfib, n, W, fcut);
+    if (!ix.nfib || !n || !ix.nB) return;
+    k_fiber_cuts<<<(unsigned)div_up(ix.nB, 256), 256, 0, s>>>(ix.fiber_ptr, ix.Bfib, ix.Bcol, ix.nB, n, W, fcut);
 }
 
 void launch_plan_tiles(const Index &ix, const float *pred, const PlanParams &P, uint32_t *t_nwin, uint32_t *t_W,

@@ -578,15 +578,13 @@ int Contraction::run(const fcc_tensor *ta, const int32_t *am, const fcc_tensor *
     bool self = nb == 0 && A.coords == B.coords && A.vals == B.vals && ta->order == tb->order && ta->nnz == tb->nnz;
     for (int i = 0; i < r && self; i++) self = am[i] == bm[i];
     for (int m = 0; m < ta->order && self; m++) self = EA[m] == EB[m];
-    Tmp<uint32_t> bfib, vsB;
+    Tmp<uint32_t> vsB;
     // the per-entry row and fiber of a self-contraction are only read when
     // left rows are sampled (the tiles come from the right operand's order)
     const bool need_join = o.left_tile_stride > 1;
     SelfJoin sj{need_join ? rowA.get() : nullptr, need_join ? fidA.get() : nullptr, nullptr, nullptr, 0};
     if (self) {
-        bfib = Tmp<uint32_t>(pool, (size_t)A.nnz);
         vsB = Tmp<uint32_t>(pool, (size_t)A.nnz);
-        sj.bfib = bfib;
         sj.vsB = vsB;
         build_right(pool, g, B, ix, hold, s, &sj);
         products = sj.products;
@@ -693,7 +691,6 @@ int Contraction::run(const fcc_tensor *ta, const int32_t *am, const fcc_tensor *
     if (self && o.left_tile_stride <= 1) build_left_self(pool, g, A, sj, logTL, ix, hold, s, after_sort);
     else build_left(pool, g, A, rowA, fidA, nvalid, logTL, ix, hold, s, after_sort);
     if (dk.nh && !side_join.st) throw LibError{FCC_ERR_INVALID, "dense-key block was not started"};
-    bfib.reset();
     vsB.reset();
     st.row_tile = (int32_t)ix.TL;
     st.col_window = (int32_t)Wd;

@@ -29,6 +29,8 @@ struct DenseKeys {
     uint64_t products = 0;    // products of the selected keys
     float tau = 0;            // density threshold used
     bool self = false;        // self-contraction: AT is Bd, row bitsets are column bitsets
+    bool full = false;        // a selected key is stored at every left row and every right column:
+                              // every cell of the block is present (P all ones, no bitsets)
     uint32_t splits = 1;      // GEMM k splits: C holds this many ra x rb slabs, summed by the readers
     uint32_t *hkey = nullptr;              // per fiber: selected key index, or ~0
     uint32_t *hfib = nullptr;              // per selected key: its fiber
@@ -47,7 +49,7 @@ struct DenseKeys {
 // require_gain only when their products average four or more per block cell.
 // fidA: the join's fiber per left entry (nA entries), or null for a
 // self-contraction (left fibers = right fibers). Fills nh, products, hkey,
-// hfib and the sizes; synchronises. Leaves dk.nh = 0 when none qualify.
+// hfib, full and the sizes; synchronises. Leaves dk.nh = 0 when none qualify.
 void dense_keys_select(DevPool &pool, const Geom &g, const Index &ix, const uint32_t *fidA, size_t nA, float tau,
                        uint64_t max_cells, bool require_gain, DenseKeys &dk, std::vector<std::shared_ptr<void>> &hold,
                        hipStream_t s);

@@ -33,27 +33,44 @@ __global__ void k_dk_na(const uint32_t *fidA, size_t n, const uint32_t *fiber_pt
 }
 
 // a fiber is selected when nA * nB >= tau * spanA * spanB; tot[0] counts the
-// selected fibers, tot[1] sums their products (wave-aggregated). na == null:
-// a self-contraction, whose left fibers are the right ones (nA = nB).
+// selected fibers, tot[1] sums their products (wave-aggregated), tot[2] is set
+// when a selected fiber is full on both sides (nA = spanA, nB = spanB: its
+// products alone reach every cell of the block). na == null: a
+// self-contraction, whose left fibers are the right ones (nA = nB).
 __global__ void k_dk_flag(const uint32_t *fiber_ptr, const uint32_t *na, uint32_t nfib, double inv_area, double tau,
-                          uint32_t *flag, unsigned long long *tot) {
+                          uint64_t spanA, uint64_t spanB, uint32_t *flag, unsigned long long *tot) {
     const uint32_t f = blockIdx.x * blockDim.x + threadIdx.x;
     unsigned long long c = 0, prod = 0;
+    bool full = false;
     if (f < nfib) {
         const uint64_t b = fiber_ptr[f + 1] - fiber_ptr[f], a = na ? na[f] : b;
         const bool sel = a && (double)a * (double)b * inv_area >= tau;
         flag[f] = sel ? 1u : 0u;
         c = sel ? 1ull : 0ull;
         prod = sel ? a * b : 0ull;
+        full = sel && a == spanA && b == spanB;
     }
     for (int d = 32; d > 0; d >>= 1) {
         c += __shfl_xor(c, d, 64);
         prod += __shfl_xor(prod, d, 64);
     }
+    const bool any_full = __ballot(full) != 0ull;
     if ((threadIdx.x & 63) == 0 && c) {
         atomicAdd(&tot[0], c);
         atomicAdd(&tot[1], prod);
+        if (any_full) atomicOr(&tot[2], 1ull);
     }
+}
+
+// Presence map when a full key is selected: every cell (row < spanA, column
+// < spanB) is present; the padding rows and columns of the block are not.
+__global__ void k_dk_full_presence(uint32_t *P, uint64_t ra, uint64_t pw, uint64_t spanA, uint64_t spanB) {
+    const uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+    if (i >= ra * pw) return;
+    const uint64_t row = i / pw, c0 = (i - row * pw) * 32;
+    uint32_t v = 0u;
+    if (row < spanA && c0 < spanB) v = spanB - c0 >= 32 ? 0xFFFFFFFFu : ((1u << (spanB - c0)) - 1u);
+    P[i] = v;
 }
 
 // hkey[f] = the selected key index of fiber f (or none), hfib[h] = its fiber;
@@ -398,7 +415,7 @@ void dense_keys_select(DevPool &pool, const Geom &g, const Index &ix, const uint
     };
     const uint32_t nfib = ix.nfib;
     Tmp<uint32_t> na(pool, fidA ? nfib : 0), flag(pool, nfib), idx(pool, nfib), dtot(pool, 1);
-    Tmp<unsigned long long> tot(pool, 2);
+    Tmp<unsigned long long> tot(pool, 3);
     if (fidA) {
         // nA <= spanA, so a selected key has nB >= tau * spanB
         const double nbm = std::floor((double)tau * (double)g.spanB);
@@ -415,13 +432,14 @@ void dense_keys_select(DevPool &pool, const Geom &g, const Index &ix, const uint
     uint32_t *hkey = (uint32_t *)keep((size_t)nfib * 4);
     uint32_t *hfib = (uint32_t *)keep((size_t)max_keys * 4);
     double t = tau;
-    unsigned long long h[2] = {0, 0};
+    unsigned long long h[3] = {0, 0, 0};
     for (int it = 0; it < 40; it++) {
-        fill_u64(tot, 2, 0, s);
-        k_dk_flag<<<grid_of(nfib), 256, 0, s>>>(ix.fiber_ptr, fidA ? na.get() : nullptr, nfib, inv_area, t, flag, tot);
+        fill_u64(tot, 3, 0, s);
+        k_dk_flag<<<grid_of(nfib), 256, 0, s>>>(ix.fiber_ptr, fidA ? na.get() : nullptr, nfib, inv_area, t, g.spanA,
+                                                 g.spanB, flag, tot);
         scan_u32(pool, flag, idx, nfib, dtot, s);
         k_dk_keys<<<grid_of(nfib), 256, 0, s>>>(flag, idx, nfib, (uint32_t)max_keys, hkey, hfib);
-        pool.read(h, tot.get(), 16, s);
+        pool.read(h, tot.get(), 24, s);
         pool.sync(s);
         if (h[0] <= max_keys) break;
         t *= 1.25;
@@ -438,6 +456,7 @@ void dense_keys_select(DevPool &pool, const Geom &g, const Index &ix, const uint
     dk.ra = ra;
     dk.rb = rb;
     dk.self = fidA == nullptr;
+    dk.full = h[2] != 0;
     dk.hkey = hkey;
     dk.hfib = hfib;
 }
@@ -465,10 +484,12 @@ void dense_keys_block(const Index &ix, const DenseKeys &dk, const uint32_t *fidA
                       const double *Avals, size_t nA, hipStream_t s, hipEvent_t gemm0, hipEvent_t gemm1) {
     if (!dk.nh) return;
     const uint64_t ra = dk.ra, rb = dk.rb;
-    fill_u64(dk.colbits, (size_t)dk.nw * rb, 0, s);
     fill_u32(dk.P + ra * (rb / 32), 2, 0, s);
     fill_f64(dk.Bd, (size_t)dk.kp * rb, 0.0, s);
-    k_dk_colbits<<<dk.nh, 256, 0, s>>>(dk.hfib, ix.fiber_ptr, ix.Bcol, rb, dk.colbits);
+    if (!dk.full) {
+        fill_u64(dk.colbits, (size_t)dk.nw * rb, 0, s);
+        k_dk_colbits<<<dk.nh, 256, 0, s>>>(dk.hfib, ix.fiber_ptr, ix.Bcol, rb, dk.colbits);
+    }
     k_dk_scatter_B<<<dk.nh, 256, 0, s>>>(dk.hfib, ix.fiber_ptr, ix.Bcol, ix.Bval, rb, dk.Bd);
     if (!dk.self) {
         fill_u64(dk.rowbits, (size_t)dk.spanA * dk.nw, 0, s);
@@ -476,8 +497,11 @@ void dense_keys_block(const Index &ix, const DenseKeys &dk, const uint32_t *fidA
         if (nA)
             k_dk_join_A<<<grid_of(nA), 256, 0, s>>>(fidA, rowA, Avals, nA, dk.hkey, dk.nw, ra, dk.rowbits, dk.AT);
     }
-    k_dk_presence<<<dim3((unsigned)(rb / 64), (unsigned)(ra / 64)), 256, 0, s>>>(dk.rowbits, dk.colbits, dk.spanA, rb,
-                                                                                 dk.nw, dk.P);
+    if (dk.full)
+        k_dk_full_presence<<<grid_of(ra * (rb / 32)), 256, 0, s>>>(dk.P, ra, rb / 32, dk.spanA, dk.spanB);
+    else
+        k_dk_presence<<<dim3((unsigned)(rb / 64), (unsigned)(ra / 64)), 256, 0, s>>>(dk.rowbits, dk.colbits, dk.spanA,
+                                                                                     rb, dk.nw, dk.P);
     if (gemm0) FCC_HIP(hipEventRecord(gemm0, s));
     launch_dgemm_tn(dk.AT, dk.Bd, dk.C, ra, rb, dk.kp, dk.splits, dk.self, s);
     if (gemm1) FCC_HIP(hipEventRecord(gemm1, s));

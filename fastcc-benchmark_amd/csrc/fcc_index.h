@@ -36,7 +36,7 @@ void launch_extents(const Operand &t, uint32_t *dev_out, hipStream_t s);
 struct SelfJoin {
     uint32_t *rowA;  // row and fiber per original left entry; null: not written
     uint32_t *fidA;
-    uint32_t *bfib;  // fiber of each right-sorted position
+    uint32_t *bfib;  // fiber of each right-sorted position (set by build_right: the index's Bfib)
     uint32_t *vsB;   // original entry of each right-sorted position
     uint64_t products;
 };

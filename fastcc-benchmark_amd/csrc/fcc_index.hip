@@ -197,14 +197,16 @@ __global__ void k_gather_B(const uint64_t *comp, const uint32_t *idx, const doub
 }
 
 __global__ void k_fibers(const uint64_t *comp, const uint32_t *flag, const uint32_t *fid, size_t n, int cbits,
-                         uint32_t *fiber_ptr, uint64_t *fiber_key) {
+                         uint32_t *fiber_ptr, uint64_t *fiber_key, uint32_t *Bfib) {
     size_t p = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
     if (p >= n) return;
-    if (flag[p]) {
-        fiber_ptr[fid[p]] = (uint32_t)p;
-        fiber_key[fid[p]] = comp[p] >> cbits;
+    const uint32_t fl = flag[p], f = fid[p];
+    if (fl) {
+        fiber_ptr[f] = (uint32_t)p;
+        fiber_key[f] = comp[p] >> cbits;
     }
-    if (p == n - 1) fiber_ptr[fid[p] + flag[p]] = (uint32_t)n;
+    Bfib[p] = f + fl - 1;
+    if (p == n - 1) fiber_ptr[f + fl] = (uint32_t)n;
 }
 
 __global__ void k_keymap(const uint64_t *fiber_key, uint32_t nfib, uint32_t *keymap) {
@@ -257,9 +259,24 @@ __global__ __launch_bounds__(256) void k_join_A(const uint32_t *coords, size_t n
     }
 }
 
+// What the left build needs of a source entry (an A entry, or for a
+// self-contraction a right position): its value, its fiber's range, whether
+// the fiber's key is a dense key, and the fiber's column sketch. The
+// composite kernels pack it into one 32-byte record while they read it in
+// source order, so the gather after the sort makes a single random access per
+// entry (uint4 halves: {value lo, value hi, fiber start, fiber end},
+// {flags, sketch 0, sketch 1, 0}; flag bit 0: dense key).
+__device__ __forceinline__ void put_lrec(uint4 *rec, size_t i, double v, uint32_t fp0, uint32_t fp1, uint32_t flags,
+                                         uint2 sk) {
+    const unsigned long long b = (unsigned long long)__double_as_longlong(v);
+    rec[2 * i] = make_uint4((uint32_t)b, (uint32_t)(b >> 32), fp0, fp1);
+    rec[2 * i + 1] = make_uint4(flags, sk.x, sk.y, 0u);
+}
+
 // A composite: tile | class | fiber | row-in-tile; unmatched entries sort last.
-__global__ void k_comp_A(const uint32_t *rowA, const uint32_t *fidA, const uint32_t *fiber_ptr, size_t nnz,
-                         int logTL, int fbits, uint32_t heavy_min, uint64_t *comp, uint32_t *idx) {
+__global__ void k_comp_A(const uint32_t *rowA, const uint32_t *fidA, const uint32_t *fiber_ptr, const double *vals,
+                         const uint32_t *dk_hkey, const uint2 *fsk, size_t nnz, int logTL, int fbits,
+                         uint32_t heavy_min, uint64_t *comp, uint32_t *idx, uint4 *rec) {
     size_t n = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
     if (n >= nnz) return;
     uint32_t f = fidA[n];
@@ -268,68 +285,172 @@ __global__ void k_comp_A(const uint32_t *rowA, const uint32_t *fidA, const uint3
         uint32_t row = rowA[n];
         uint64_t tile = row >> logTL;
         uint64_t li = row & ((1u << logTL) - 1);
-        uint64_t cls = (fiber_ptr[f + 1] - fiber_ptr[f]) >= heavy_min ? 0 : 1;
+        const uint32_t fp0 = fiber_ptr[f], fp1 = fiber_ptr[f + 1];
+        uint64_t cls = (fp1 - fp0) >= heavy_min ? 0 : 1;
         c = (tile << (1 + fbits + logTL)) | (cls << (fbits + logTL)) | ((uint64_t)f << logTL) | li;
+        const uint32_t dense = dk_hkey && dk_hkey[f] != 0xFFFFFFFFu ? 1u : 0u;
+        put_lrec(rec, n, vals[n], fp0, fp1, dense, fsk ? fsk[f] : make_uint2(0u, 0u));
     }
     comp[n] = c;
     idx[n] = (uint32_t)n;
 }
 
-// Mirror mode (self-contraction): idx is the entry's own position in the right
+// Left entries after the sort (ks: composite keys, vs: source entries), 4096
+// per block in 16 coalesced rounds: the per-entry arrays, and the number of
+// group starts (a new tile, class or fiber) and tile starts in the block
+// (bsum: tiles << 32 | groups), which one small scan turns into each block's
+// first group and tile. At a group start ex holds what its group descriptor
+// and statistics need: {fiber length, flags, sketch}.
+// Mirror mode: the source entry is the entry's own position in the right
 // operand's (key, column) order, i.e. inside its fiber at the column equal to
-// its row, so its fiber range starts there: the cells left of the diagonal are
-// cut without a search.
-__global__ void k_gather_A(const uint64_t *comp, const uint32_t *idx, const double *vals, const uint32_t *fiber_ptr,
-                           size_t n, int logTL, int fbits, bool mirror, uint32_t *Ali, uint32_t *Arow, double *Aval,
-                           uint32_t *Afib, uint2 *Afp, uint64_t *gtflag) {
-    size_t p = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
-    if (p >= n) return;
-    uint64_t c = comp[p];
-    Ali[p] = (uint32_t)(c & ((1ull << logTL) - 1));
-    Arow[p] = (uint32_t)(((c >> (1 + fbits + logTL)) << logTL) | (c & ((1ull << logTL) - 1)));
-    uint32_t f = (uint32_t)((c >> logTL) & ((1ull << fbits) - 1));
-    Afib[p] = f;
-    Afp[p] = make_uint2(mirror ? idx[p] : fiber_ptr[f], fiber_ptr[f + 1]);
-    Aval[p] = vals[idx[p]];
-    uint64_t prev = p ? comp[p - 1] : ~0ull;
-    // group start (low word) and tile start (high word): one scan of the
-    // packed pair numbers both (each half stays below 2^32)
-    const uint64_t gf = (p == 0 || (c >> logTL) != (prev >> logTL)) ? 1u : 0u;
-    const uint64_t tf = (p == 0 || (c >> (1 + fbits + logTL)) != (prev >> (1 + fbits + logTL))) ? 1u : 0u;
-    gtflag[p] = (tf << 32) | gf;
+// its row, so its fiber range starts there: the cells left of the diagonal
+// are cut without a search.
+// Dense keys: a light entry whose key the dense GEMM takes gets an empty
+// fiber range (heavy groups of such keys get an empty descriptor later).
+constexpr int LB_THREADS = 256;
+constexpr int LB_ROUNDS = 16;
+constexpr uint32_t LB_TILE = LB_THREADS * LB_ROUNDS;
+constexpr int LB_BATCH = 4;  // rounds whose loads are issued together
+
+__global__ __launch_bounds__(LB_THREADS) void k_left_entries(const uint64_t *ks, const uint32_t *vs, const uint4 *rec,
+                                                             uint32_t m, int logTL, int fbits, bool mirror,
+                                                             uint32_t *Ali, uint32_t *Arow, double *Aval,
+                                                             uint32_t *Afib, uint2 *Afp, uint4 *ex,
+                                                             unsigned long long *bsum) {
+    __shared__ unsigned long long s_red[LB_THREADS / 64];
+    const int tsh = 1 + fbits + logTL, csh = fbits + logTL;
+    const uint64_t fmask = (1ull << fbits) - 1, lmask = (1ull << logTL) - 1;
+    const uint32_t base = blockIdx.x * LB_TILE;
+    uint32_t ng = 0, nt = 0;
+    for (int r0 = 0; r0 < LB_ROUNDS; r0 += LB_BATCH) {
+        uint64_t c[LB_BATCH], pv[LB_BATCH];
+        uint32_t p[LB_BATCH];
+        uint4 r1[LB_BATCH], r2[LB_BATCH];
+#pragma unroll
+        for (int j = 0; j < LB_BATCH; j++) {
+            const uint32_t q = base + (r0 + j) * LB_THREADS + threadIdx.x;
+            const uint32_t qc = q < m ? q : m - 1;  // clamped: loads stay in bounds, results unused
+            c[j] = ks[qc];
+            pv[j] = qc ? ks[qc - 1] : ~0ull;
+            p[j] = vs[qc];
+        }
+#pragma unroll
+        for (int j = 0; j < LB_BATCH; j++) {
+            r1[j] = rec[2 * (size_t)p[j]];
+            r2[j] = rec[2 * (size_t)p[j] + 1];
+        }
+#pragma unroll
+        for (int j = 0; j < LB_BATCH; j++) {
+            const uint32_t q = base + (r0 + j) * LB_THREADS + threadIdx.x;
+            if (q >= m) continue;
+            const uint64_t cc = c[j];
+            const uint32_t li = (uint32_t)(cc & lmask);
+            const uint64_t tile = cc >> tsh;
+            const bool light = (cc >> csh) & 1;
+            const bool gf = q == 0 || (cc >> logTL) != (pv[j] >> logTL);
+            const bool tf = q == 0 || tile != (pv[j] >> tsh);
+            Ali[q] = li;
+            Arow[q] = (uint32_t)((tile << logTL) | li);
+            Aval[q] = __longlong_as_double((long long)(((unsigned long long)r1[j].y << 32) | r1[j].x));
+            Afib[q] = (uint32_t)((cc >> logTL) & fmask);
+            const uint32_t fp0 = r1[j].z, fend = r1[j].w, flags = r2[j].x;
+            const uint32_t x0 = mirror ? p[j] : fp0;
+            Afp[q] = make_uint2(x0, ((flags & 1u) && light) ? x0 : fend);
+            if (gf) ex[q] = make_uint4(fend - fp0, flags, r2[j].y, r2[j].z);
+            ng += gf ? 1u : 0u;
+            nt += tf ? 1u : 0u;
+        }
+    }
+    unsigned long long v = ((unsigned long long)nt << 32) | ng;
+    for (int d = 32; d > 0; d >>= 1) v += __shfl_xor(v, d, 64);
+    if ((threadIdx.x & 63) == 0) s_red[threadIdx.x >> 6] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned long long t = 0;
+        for (int w = 0; w < LB_THREADS / 64; w++) t += s_red[w];
+        bsum[blockIdx.x] = t;
+    }
 }
 
-// Also each tile's first light group (atile_lg): light groups follow the heavy
-// ones inside a tile, so it is the group where the class turns light, or the
-// tile's group end when its last group is heavy.
-__global__ void k_groups(const uint64_t *comp, const uint64_t *gtflag, const uint64_t *gtid, size_t n, int logTL,
-                         int fbits, uint32_t *group_ptr, uint32_t *group_fiber, uint32_t *group_atile,
-                         uint8_t *group_light, uint32_t *atile_id, uint32_t *atile_g0, uint32_t *atile_lg) {
-    size_t p = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
-    if (p >= n) return;
-    uint64_t c = comp[p];
-    const uint64_t fl = gtflag[p], id = gtid[p];
-    const uint32_t gflag = (uint32_t)fl, tflag = (uint32_t)(fl >> 32);
-    const uint32_t gid = (uint32_t)id, aid = (uint32_t)(id >> 32);
-    uint32_t a = aid + tflag - 1;
-    if (gflag) {
-        uint32_t g = gid;
-        group_ptr[g] = (uint32_t)p;
-        group_fiber[g] = (uint32_t)((c >> logTL) & ((1ull << fbits) - 1));
-        group_atile[g] = a;
-        group_light[g] = (uint8_t)((c >> (fbits + logTL)) & 1);
-    }
-    if (tflag) {
-        atile_id[a] = (uint32_t)(c >> (1 + fbits + logTL));
-        atile_g0[a] = gid;
-    }
-    const bool light = (c >> (fbits + logTL)) & 1;
-    if (gflag && light && (tflag || !((comp[p - 1] >> (fbits + logTL)) & 1))) atile_lg[a] = gid;
-    const bool tile_end = p == n - 1 || (gtflag[p + 1] >> 32) != 0;
-    if (tile_end && !light) atile_lg[a] = gid + gflag;
-    if (p == n - 1) {
-        group_ptr[gid + gflag] = (uint32_t)n;
-        atile_g0[a + 1] = gid + gflag;
+// Groups and active tiles, over the same 4096-entry blocks: boff holds the
+// exclusive prefix of bsum (tiles << 32 | groups before the block); inside
+// the block each round numbers its starts with ballots. A group start writes
+// the group (entry range start, fiber, tile, descriptor without its entry
+// count, fiber length and flags, sketch); a tile start its id and first
+// group; each tile's first light group is where the class turns light (light
+// groups follow the heavy ones inside a tile), or the tile's group end when
+// its last group is heavy.
+__global__ __launch_bounds__(LB_THREADS) void k_left_groups(const uint64_t *ks, const uint2 *Afp, const uint4 *ex,
+                                                            const unsigned long long *boff, uint32_t m, int logTL,
+                                                            int fbits, bool features, uint32_t *group_ptr,
+                                                            uint32_t *group_fiber, uint32_t *group_atile,
+                                                            uint4 *gdesc, uint2 *gnf, uint2 *gsk, uint32_t *atile_id,
+                                                            uint32_t *atile_g0, uint32_t *atile_lg) {
+    __shared__ uint32_t s_g[LB_THREADS / 64], s_t[LB_THREADS / 64];
+    const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const uint64_t lt = lane ? (~0ull >> (64 - lane)) : 0ull;
+    const int tsh = 1 + fbits + logTL, csh = fbits + logTL;
+    const uint64_t fmask = (1ull << fbits) - 1;
+    const unsigned long long bo = boff[blockIdx.x];
+    uint32_t gb = (uint32_t)bo, tb = (uint32_t)(bo >> 32);  // starts before this round
+    const uint32_t base = blockIdx.x * LB_TILE;
+    for (int r = 0; r < LB_ROUNDS; r++) {
+        const uint32_t q = base + r * LB_THREADS + threadIdx.x;
+        const bool ok = q < m;
+        uint64_t c = 0, prev = 0;
+        bool gf = false, tf = false, tend = false;
+        if (ok) {
+            c = ks[q];
+            prev = q ? ks[q - 1] : 0ull;
+            gf = q == 0 || (c >> logTL) != (prev >> logTL);
+            tf = q == 0 || (c >> tsh) != (prev >> tsh);
+            tend = q + 1 == m || (ks[q + 1] >> tsh) != (c >> tsh);
+        }
+        const uint64_t bg = __ballot(gf), bt = __ballot(tf);
+        if (lane == 0) {
+            s_g[wave] = (uint32_t)__popcll(bg);
+            s_t[wave] = (uint32_t)__popcll(bt);
+        }
+        __syncthreads();
+        uint32_t wg = 0, wt = 0, rg = 0, rt = 0;
+#pragma unroll
+        for (int w = 0; w < LB_THREADS / 64; w++) {
+            if (w < (int)wave) {
+                wg += s_g[w];
+                wt += s_t[w];
+            }
+            rg += s_g[w];
+            rt += s_t[w];
+        }
+        __syncthreads();  // s_g / s_t are rewritten by the next round
+        if (ok) {
+            // group and tile of entry q: starts up to and including q, minus one
+            const uint32_t g = gb + wg + (uint32_t)__popcll(bg & lt) + (gf ? 1u : 0u) - 1u;
+            const uint32_t a = tb + wt + (uint32_t)__popcll(bt & lt) + (tf ? 1u : 0u) - 1u;
+            const bool light = (c >> csh) & 1;
+            if (gf) {
+                const uint2 fp = Afp[q];
+                const uint4 x = ex[q];
+                group_ptr[g] = q;
+                group_fiber[g] = (uint32_t)((c >> logTL) & fmask);
+                group_atile[g] = a;
+                gdesc[g] = make_uint4(fp.x, (x.y & 1u) ? 0u : fp.y - fp.x, q, 0u);
+                gnf[g] = make_uint2(x.x, x.y);
+                if (features) gsk[g] = make_uint2(x.z, x.w);
+                if (light && (tf || !((prev >> csh) & 1))) atile_lg[a] = g;
+            }
+            if (tf) {
+                atile_id[a] = (uint32_t)(c >> tsh);
+                atile_g0[a] = g;
+            }
+            if (tend && !light) atile_lg[a] = g + 1;
+            if (q + 1 == m) {
+                group_ptr[g + 1] = m;
+                atile_g0[a + 1] = g + 1;
+            }
+        }
+        gb += rg;
+        tb += rt;
     }
 }
 
@@ -438,27 +559,21 @@ __global__ void k_fiber_sketch(const uint32_t *fiber_ptr, const uint32_t *Bcol, 
     fsk[f] = make_uint2(h1, h2);
 }
 
-// Per group: its descriptor (fiber start, fiber length, first entry, entries)
-// and, aggregated per active tile (block-wide when the block's groups share a
-// tile, else per group), the tile statistics:
+// Per group: its entry count (the last field of its descriptor) and,
+// aggregated per active tile (block-wide when the block's groups share a
+// tile, else per run of equal tiles in a wavefront), the tile statistics:
 //  nnz      A entries of the tile          (num_nnz_in_tile, index_tensors.hpp:562)
 //  keys     distinct contraction keys      (num_active_columns, index_tensors.hpp:604)
 //  sumfib   summed right-fiber lengths
 //  products sum of m * |fiber| = an upper bound on the tile's output count
 //  sketch   two min-hashes over sampled fiber columns (distinct-column estimate)
-// Mirror mode (Afp_clip != null): a group's descriptor starts at its first
-// entry's clipped fiber start. The group's entries are the fiber's only
-// elements in the tile's columns (a self-contraction: fiber k holds exactly
-// the rows with key k), so that is the fiber's first column at or right of
-// the tile's first row. The statistics keep the whole fibers.
-// Dense keys (dk_hkey != null): a group whose key the dense GEMM takes leaves
-// the tile traversal -- a heavy group gets an empty fiber in its descriptor, a
-// light group's entries get empty fiber ranges -- and its tile is flagged in
-// tile_dk (its outputs meet the dense block). The statistics keep it.
-__global__ __launch_bounds__(256) void k_tile_stats(const uint32_t *group_ptr, const uint32_t *group_fiber,
-                                                   const uint32_t *group_atile, const uint32_t *fiber_ptr,
-                                                   const uint2 *fsk, bool mirror, uint2 *Afp,
-                                                   const uint32_t *dk_hkey, const uint8_t *glight, uint8_t *tile_dk,
+// gnf: per group {fiber length, flags} (flag bit 0: a dense key), gsk its
+// fiber's sketch (planner features only). A dense key's group leaves the tile
+// traversal (its descriptor and its light entries' ranges are already empty)
+// and its tile is flagged in tile_dk (its outputs meet the dense block); the
+// statistics keep it. Every read is in group order.
+__global__ __launch_bounds__(256) void k_tile_stats(const uint32_t *group_ptr, const uint32_t *group_atile,
+                                                   const uint2 *gnf, const uint2 *gsk, uint8_t *tile_dk,
                                                    uint32_t ngroups, uint4 *gdesc, uint32_t *st_nnz,
                                                    uint32_t *st_keys, unsigned long long *st_sumfib,
                                                    unsigned long long *st_products, uint32_t *st_sketch,
@@ -467,6 +582,7 @@ __global__ __launch_bounds__(256) void k_tile_stats(const uint32_t *group_ptr, c
     __shared__ unsigned long long s_red[4][4];
     __shared__ uint32_t s_h[4][2];
     __shared__ unsigned long long s_mx[4];
+    const bool fsk = gsk != nullptr;
     uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
     bool ok = g < ngroups;
     uint32_t a = ok ? group_atile[g] : 0xFFFFFFFFu;
@@ -475,23 +591,12 @@ __global__ __launch_bounds__(256) void k_tile_stats(const uint32_t *group_ptr, c
     if (ok) {
         const uint32_t e0 = group_ptr[g];
         m = group_ptr[g + 1] - e0;
-        uint32_t f = group_fiber[g];
-        const uint32_t fp0 = fiber_ptr[f], fp1 = fiber_ptr[f + 1];
-        nf = fp1 - fp0;
-        const uint32_t c0 = mirror ? Afp[e0].x : fp0;
-        const bool dense = dk_hkey && dk_hkey[f] != 0xFFFFFFFFu;
-        gdesc[g] = make_uint4(c0, dense ? 0u : fp1 - c0, e0, (uint32_t)m);
-        if (dense) {
-            tile_dk[a] = 1;
-            if (glight[g])
-                for (uint32_t e = e0; e < e0 + (uint32_t)m; e++) {
-                    uint2 fp = Afp[e];
-                    fp.y = fp.x;
-                    Afp[e] = fp;
-                }
-        }
+        const uint2 x = gnf[g];
+        nf = x.x;
+        reinterpret_cast<uint32_t *>(gdesc)[4 * (size_t)g + 3] = (uint32_t)m;
+        if ((x.y & 1u) && tile_dk) tile_dk[a] = 1;
         if (fsk) {
-            const uint2 hh = fsk[f];
+            const uint2 hh = gsk[g];
             h1 = hh.x;
             h2 = hh.y;
         }
@@ -589,20 +694,18 @@ static unsigned grid_for(size_t n, unsigned bs = 256) { return (unsigned)div_up(
 // Self-contraction (same tensor, same mode lists): the left join is read off
 // the right operand's sort. Sorted position p holds original entry vs[p], its
 // column (= the left row) and its fiber.
-__global__ __launch_bounds__(256) void k_self_join(const uint32_t *vs, const uint32_t *flag, const uint32_t *fid,
-                                                  const uint32_t *Bcol, const uint32_t *fiber_ptr, size_t n,
-                                                  uint32_t *rowA, uint32_t *fidA, uint32_t *bfib, uint32_t *vsB,
-                                                  unsigned long long *products) {
+__global__ __launch_bounds__(256) void k_self_join(const uint32_t *vs, const uint32_t *Bfib, const uint32_t *Bcol,
+                                                  const uint32_t *fiber_ptr, size_t n, uint32_t *rowA, uint32_t *fidA,
+                                                  uint32_t *vsB, unsigned long long *products) {
     __shared__ unsigned long long s_p[4];
     uint64_t prod = 0;
     for (size_t p = blockIdx.x * (size_t)blockDim.x + threadIdx.x; p < n; p += (size_t)gridDim.x * blockDim.x) {
-        uint32_t f = fid[p] + flag[p] - 1;
+        const uint32_t f = Bfib[p];
         uint32_t o = vs[p];
         if (rowA) {  // per original entry (random writes): only when asked for
             rowA[o] = Bcol[p];
             fidA[o] = f;
         }
-        bfib[p] = f;
         vsB[p] = o;
         prod += fiber_ptr[f + 1] - fiber_ptr[f];
     }
@@ -643,12 +746,14 @@ void build_right(DevPool &pool, const Geom &g, const Operand &B, Index &ix, std:
     scan_u32(pool, flag, fid, n, total, s);
     ix.fiber_ptr = (uint32_t *)keep((n + 1) * 4);
     ix.fiber_key = (uint64_t *)keep(n * 8);
-    k_fibers<<<grid_for(n), 256, 0, s>>>(ks, flag, fid, n, g.cbits, ix.fiber_ptr, ix.fiber_key);
+    ix.Bfib = (uint32_t *)keep(n * 4);
+    k_fibers<<<grid_for(n), 256, 0, s>>>(ks, flag, fid, n, g.cbits, ix.fiber_ptr, ix.fiber_key, ix.Bfib);
     if (self) {
         Tmp<unsigned long long> prod(pool, 1);
         fill_u64(prod, 1, 0, s);
-        k_self_join<<<std::min(grid_for(n), 2048u), 256, 0, s>>>(vs, flag, fid, ix.Bcol, ix.fiber_ptr, n, self->rowA,
-                                                                  self->fidA, self->bfib, self->vsB, prod);
+        self->bfib = ix.Bfib;
+        k_self_join<<<std::min(grid_for(n), 2048u), 256, 0, s>>>(vs, ix.Bfib, ix.Bcol, ix.fiber_ptr, n, self->rowA,
+                                                                  self->fidA, self->vsB, prod);
         pool.read(&self->products, prod.get(), 8, s);
     }
     pool.read(&ix.nfib, total.get(), 4, s);
@@ -781,10 +886,11 @@ void build_hseg(DevPool &pool, Index &ix, std::vector<std::shared_ptr<void>> &ho
                                                      ix.hseg, ix.seg_below, ix.hdesc);
 }
 
-// vals: the values vs indexes (A's, or for a self-contraction the right
-// operand's sorted values, vs then holding right positions); perm_map (when
-// not null) maps vs to A's entry order for plans.
-static void left_from_sorted(DevPool &pool, const double *vals, const uint64_t *ks, const uint32_t *vs,
+// After the left sort: ks the sorted composite keys, vs their source entries
+// (A entries, or for a self-contraction right positions), rec the source
+// records (put_lrec); perm_map (when not null) maps vs to A's entry order for
+// plans.
+static void left_from_sorted(DevPool &pool, const uint4 *rec, const uint64_t *ks, const uint32_t *vs,
                              const uint32_t *perm_map, uint32_t nvalid, int logTL, int fbits, Index &ix,
                              std::vector<std::shared_ptr<void>> &hold, hipStream_t s) {
     auto keep = [&](size_t bytes) -> void * {
@@ -798,17 +904,19 @@ static void left_from_sorted(DevPool &pool, const double *vals, const uint64_t *
     ix.Arow = (uint32_t *)keep(m * 4);
     ix.Aval = (double *)keep(m * 8);
     ix.Afib = (uint32_t *)keep(m * 4);
-    Tmp<uint64_t> gtflag(pool, m), gtid(pool, m), tot(pool, 1);
     ix.Afp = (uint2 *)keep(m * 8);
+    const size_t nb = div_up(m, LB_TILE);
+    Tmp<uint4> ex(pool, m);
+    Tmp<unsigned long long> bsum(pool, nb), boff(pool, nb), tot(pool, 1);
     if (m)
-        k_gather_A<<<grid_for(m), 256, 0, s>>>(ks, vs, vals, ix.fiber_ptr, m, logTL, fbits, ix.mirror, ix.Ali, ix.Arow,
-                                               ix.Aval, ix.Afib, ix.Afp, gtflag);
+        k_left_entries<<<(unsigned)nb, LB_THREADS, 0, s>>>(ks, vs, rec, (uint32_t)m, logTL, fbits, ix.mirror, ix.Ali,
+                                                            ix.Arow, ix.Aval, ix.Afib, ix.Afp, ex, bsum);
     if (ix.keep_perm) {
         ix.permA = (uint32_t *)keep(m * 4);
         if (m && perm_map) k_map_u32<<<grid_for(m), 256, 0, s>>>(perm_map, vs, m, ix.permA);
         else if (m) FCC_HIP(hipMemcpyAsync(ix.permA, vs, m * 4, hipMemcpyDeviceToDevice, s));
     }
-    scan_u64(pool, gtflag, gtid, m, tot.get(), s);
+    scan_u64(pool, (const uint64_t *)bsum.get(), (uint64_t *)boff.get(), nb, (uint64_t *)tot.get(), s);
     uint64_t h = 0;
     pool.read(&h, tot.get(), 8, s);
     pool.sync(s);
@@ -818,21 +926,22 @@ static void left_from_sorted(DevPool &pool, const double *vals, const uint64_t *
     ix.group_fiber = (uint32_t *)keep((size_t)ix.ngroups * 4);
     ix.group_atile = (uint32_t *)keep((size_t)ix.ngroups * 4);
     ix.gdesc = (uint4 *)keep((size_t)ix.ngroups * 16);
-    ix.glight = (uint8_t *)keep((size_t)ix.ngroups + 1);
-    uint8_t *glight = ix.glight;
     ix.atile_id = (uint32_t *)keep((size_t)ix.natiles * 4);
     ix.atile_g0 = (uint32_t *)keep((size_t)(ix.natiles + 1) * 4);
     ix.atile_lg = (uint32_t *)keep((size_t)ix.natiles * 4);
+    Tmp<uint2> gnf(pool, ix.ngroups), gsk(pool, ix.features ? ix.ngroups : 0);
     if (m) {
-        k_groups<<<grid_for(m), 256, 0, s>>>(ks, gtflag, gtid, m, logTL, fbits, ix.group_ptr, ix.group_fiber,
-                                             ix.group_atile, glight, ix.atile_id, ix.atile_g0, ix.atile_lg);
+        k_left_groups<<<(unsigned)nb, LB_THREADS, 0, s>>>(ks, ix.Afp, ex, boff, (uint32_t)m, logTL, fbits,
+                                                           ix.features, ix.group_ptr, ix.group_fiber, ix.group_atile,
+                                                           ix.gdesc, gnf, gsk, ix.atile_id, ix.atile_g0,
+                                                           ix.atile_lg);
     } else {
         // no valid left entry: no groups, no tiles; the one-entry offset
         // arrays still hold their end (read by the segment table below)
         fill_u32(ix.atile_g0, 1, 0, s);
         fill_u32(ix.group_ptr, 1, 0, s);
     }
-    // descriptors + statistics (one pass over the groups)
+    // entry counts + statistics (one pass over the groups)
     size_t T = ix.natiles;
     if (ix.dk_hkey) {
         ix.tile_dk = (uint8_t *)keep(T + 4);
@@ -855,13 +964,9 @@ static void left_from_sorted(DevPool &pool, const double *vals, const uint64_t *
         fill_many(f, s);
         // the planner features (key counts, fiber sums, column sketches) only
         // when a learned planner or statistics collection reads them
-        Tmp<uint2> fsk(pool, ix.features ? ix.nfib : 0);
-        if (ix.features && ix.nfib)
-            k_fiber_sketch<<<grid_for(ix.nfib), 256, 0, s>>>(ix.fiber_ptr, ix.Bcol, ix.nfib, fsk);
-        k_tile_stats<<<grid_for(ix.ngroups), 256, 0, s>>>(ix.group_ptr, ix.group_fiber, ix.group_atile, ix.fiber_ptr,
-                                                          ix.features ? fsk.get() : nullptr, ix.mirror, ix.Afp,
-                                                          ix.dk_hkey, ix.glight, ix.tile_dk, ix.ngroups, ix.gdesc,
-                                                          ix.st_nnz, ix.st_keys,
+        k_tile_stats<<<grid_for(ix.ngroups), 256, 0, s>>>(ix.group_ptr, ix.group_atile, gnf,
+                                                          ix.features ? gsk.get() : nullptr, ix.tile_dk, ix.ngroups,
+                                                          ix.gdesc, ix.st_nnz, ix.st_keys,
                                                           (unsigned long long *)ix.st_sumfib,
                                                           (unsigned long long *)ix.st_products, ix.st_sketch,
                                                           (unsigned long long *)ix.st_maxgp);
@@ -886,6 +991,14 @@ static void left_from_sorted(DevPool &pool, const double *vals, const uint64_t *
     else fill_u64((unsigned long long *)ix.st_pprefix, 1, 0, s);
 }
 
+// per-fiber column sketches for the planner features (null when not wanted)
+static Tmp<uint2> fiber_sketches(DevPool &pool, const Index &ix, hipStream_t s) {
+    Tmp<uint2> fsk(pool, ix.features ? ix.nfib : 0);
+    if (ix.features && ix.nfib)
+        k_fiber_sketch<<<grid_for(ix.nfib), 256, 0, s>>>(ix.fiber_ptr, ix.Bcol, ix.nfib, fsk);
+    return fsk;
+}
+
 void build_left(DevPool &pool, const Geom &g, const Operand &A, const uint32_t *rowA, const uint32_t *fidA,
                 uint32_t nvalid, int logTL, Index &ix, std::vector<std::shared_ptr<void>> &hold, hipStream_t s,
                 const std::function<void()> *after_sort) {
@@ -899,29 +1012,39 @@ void build_left(DevPool &pool, const Geom &g, const Operand &A, const uint32_t *
     if (nbits > 64) throw LibError{ERR_INVALID, "left composite key exceeds 64 bits"};
     Tmp<uint64_t> k1(pool, n), k2(pool, n);
     Tmp<uint32_t> v1(pool, n), v2(pool, n);
-    k_comp_A<<<grid_for(n), 256, 0, s>>>(rowA, fidA, ix.fiber_ptr, n, logTL, fbits, ix.heavy_min, k1, v1);
+    Tmp<uint4> rec(pool, 2 * n);
+    {
+        Tmp<uint2> fsk = fiber_sketches(pool, ix, s);
+        k_comp_A<<<grid_for(n), 256, 0, s>>>(rowA, fidA, ix.fiber_ptr, A.vals, ix.dk_hkey,
+                                             ix.features ? fsk.get() : nullptr, n, logTL, fbits, ix.heavy_min, k1,
+                                             v1, rec);
+    }
     uint64_t *ks;
     uint32_t *vs;
     radix_sort_pairs(pool, k1, v1, k2, v2, n, std::min(nbits, 64), s, &ks, &vs);
     if (after_sort) (*after_sort)();
-    left_from_sorted(pool, A.vals, ks, vs, nullptr, nvalid, logTL, fbits, ix, hold, s);
+    left_from_sorted(pool, rec, ks, vs, nullptr, nvalid, logTL, fbits, ix, hold, s);
 }
 
 // Self-contraction: the right operand is already sorted by (fiber, column)
 // and column == left row, so a stable sort on (tile, class) alone yields the
-// left order (tile, class, fiber, row-in-tile). The whole composite key and
-// the entry's position in the right operand's order are written here in that
-// order (coalesced reads) and the sort touches only the (tile, class) bits;
-// values are then read from the right operand's sorted values.
-__global__ void k_self_composite(const uint32_t *Bcol, const uint32_t *bfib, const uint32_t *fiber_ptr, size_t n,
-                                 int logTL, int fbits, uint32_t heavy_min, uint64_t *ks, uint32_t *vs) {
+// left order (tile, class, fiber, row-in-tile). The whole composite key, the
+// entry's position in the right operand's order and its record are written
+// here in that order (coalesced reads) and the sort touches only the (tile,
+// class) bits.
+__global__ void k_self_composite(const uint32_t *Bcol, const uint32_t *Bfib, const uint32_t *fiber_ptr,
+                                 const double *Bval, const uint32_t *dk_hkey, const uint2 *fsk, size_t n, int logTL,
+                                 int fbits, uint32_t heavy_min, uint64_t *ks, uint32_t *vs, uint4 *rec) {
     size_t p = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
     if (p >= n) return;
-    const uint32_t f = bfib[p], c = Bcol[p];
-    const uint64_t cls = (fiber_ptr[f + 1] - fiber_ptr[f]) >= heavy_min ? 0 : 1;
+    const uint32_t f = Bfib[p], c = Bcol[p];
+    const uint32_t fp0 = fiber_ptr[f], fp1 = fiber_ptr[f + 1];
+    const uint64_t cls = (fp1 - fp0) >= heavy_min ? 0 : 1;
     const uint64_t tile = c >> logTL, li = c & ((1u << logTL) - 1);
     ks[p] = (tile << (1 + fbits + logTL)) | (cls << (fbits + logTL)) | ((uint64_t)f << logTL) | li;
     vs[p] = (uint32_t)p;
+    const uint32_t dense = dk_hkey && dk_hkey[f] != 0xFFFFFFFFu ? 1u : 0u;
+    put_lrec(rec, p, Bval[p], fp0, fp1, dense, fsk ? fsk[f] : make_uint2(0u, 0u));
 }
 
 void build_left_self(DevPool &pool, const Geom &g, const Operand &A, const SelfJoin &sj, int logTL, Index &ix,
@@ -935,13 +1058,18 @@ void build_left_self(DevPool &pool, const Geom &g, const Operand &A, const SelfJ
     if (tbits + 1 + fbits + logTL + 1 > 64) throw LibError{ERR_INVALID, "left composite key exceeds 64 bits"};
     Tmp<uint64_t> k1(pool, n), k2(pool, n);
     Tmp<uint32_t> v1(pool, n), v2(pool, n);
-    k_self_composite<<<grid_for(n), 256, 0, s>>>(ix.Bcol, sj.bfib, ix.fiber_ptr, n, logTL, fbits, ix.heavy_min, k1,
-                                                 v1);
+    Tmp<uint4> rec(pool, 2 * n);
+    {
+        Tmp<uint2> fsk = fiber_sketches(pool, ix, s);
+        k_self_composite<<<grid_for(n), 256, 0, s>>>(ix.Bcol, sj.bfib, ix.fiber_ptr, ix.Bval, ix.dk_hkey,
+                                                     ix.features ? fsk.get() : nullptr, n, logTL, fbits,
+                                                     ix.heavy_min, k1, v1, rec);
+    }
     uint64_t *ks;
     uint32_t *vs;
     radix_sort_pairs(pool, k1, v1, k2, v2, n, tbits + 1, s, &ks, &vs, fbits + logTL);
     if (after_sort) (*after_sort)();
-    left_from_sorted(pool, ix.Bval, ks, vs, sj.vsB, (uint32_t)n, logTL, fbits, ix, hold, s);
+    left_from_sorted(pool, rec, ks, vs, sj.vsB, (uint32_t)n, logTL, fbits, ix, hold, s);
 }
 
 }  // namespace fcc

@@ -182,6 +182,7 @@ struct Index {
     double *Bval = nullptr;
     uint32_t *fiber_ptr = nullptr;
     uint64_t *fiber_key = nullptr;
+    uint32_t *Bfib = nullptr;  // nB: the fiber of each position
     // A: nAv entries with a matching fiber, sorted by (tile, class, fiber, row-in-tile)
     uint32_t nAv = 0, ngroups = 0, natiles = 0;
     uint32_t *Ali = nullptr;   // row within tile
@@ -193,7 +194,6 @@ struct Index {
     uint4 *gdesc = nullptr;           // per group: fiber start, fiber length, first entry, entries
     uint32_t *group_fiber = nullptr;
     uint32_t *group_atile = nullptr;
-    uint8_t *glight = nullptr;        // per group: 1 = light class (short fiber)
     uint32_t *atile_id = nullptr;     // tile index of active tile a
     uint32_t *atile_g0 = nullptr;     // natiles+1: first group of active tile a
     uint32_t *atile_lg = nullptr;     // first light group of active tile a

@@ -333,7 +333,7 @@ int fcc_tune_get(fcc_ctx *ctx, fcc_tune_entry *entries, int32_t cap, int32_t *n)
  * entry of that bucket); *best receives it. Calls that leave tile_hint,
  * col_window and workgroups at 0 then use the entry. No reference
  * counterpart: the reference's scripts sweep tile_size by hand
- * (scripts/*.sh). */
+ * (scripts/chicago01.sh and its siblings). */
 int fcc_autotune(fcc_ctx *ctx, const fcc_tensor *a, const int32_t *a_modes, const fcc_tensor *b,
                  const int32_t *b_modes, int32_t n_contr, const fcc_options *options, int32_t reps,
                  fcc_tune_entry *best);
