@@ -142,6 +142,8 @@ def main():
     ap.add_argument("--tile-hint", type=int, default=0, help="rows per left tile (0 = planner)")
     ap.add_argument("--col-window", type=int, default=0, help="columns per dense unit (0 = planner)")
     ap.add_argument("--workgroups", type=int, default=0, help="persistent workgroups (0 = planner)")
+    ap.add_argument("--dense-keys", type=float, default=0.0,
+                    help="dense-key threshold tau (0 = library default, -1 = off)")
     ap.add_argument("--force-acc", default="auto", choices=["auto", "dense", "lhash", "ghash"],
                     help="force one accumulator kind for every unit (profiling per kind)")
     ap.add_argument("--autotune", default=None, choices=["on", "off"],
@@ -195,7 +197,7 @@ def main():
     sink = args.workload == "c3"  # ~3.3e10 outputs do not fit in HBM: count + checksum
     force = {"auto": 0, "dense": 1, "lhash": 2, "ghash": 3}[args.force_acc]
     opts = dict(tile_hint=args.tile_hint, col_window=args.col_window, workgroups=args.workgroups, sink=sink,
-                force_acc=force)
+                force_acc=force, dense_keys=args.dense_keys)
     tuned = None
     if args.autotune == "on":
         t0 = time.perf_counter()

@@ -241,20 +241,23 @@ __device__ __forceinline__ void emit_output(const AccArgs &A, uint64_t pos, uint
         atomicOr(A.fail_flag, 2u);
         return;
     }
+    // Outputs are written once and not read again by this pass: non-temporal
+    // stores keep them from evicting the right fibers the units re-read from
+    // the L2.
     uint32_t r = (uint32_t)row, c = (uint32_t)col;
     for (int m = A.nxa - 1; m >= 0; m--) {
         uint32_t E = A.xaE[m];
         uint32_t q = r / E;
-        A.ocoords[(uint64_t)m * A.ostride + pos] = r - q * E;
+        __builtin_nontemporal_store(r - q * E, &A.ocoords[(uint64_t)m * A.ostride + pos]);
         r = q;
     }
     for (int m = A.nxb - 1; m >= 0; m--) {
         uint32_t E = A.xbE[m];
         uint32_t q = c / E;
-        A.ocoords[(uint64_t)(A.nxa + m) * A.ostride + pos] = c - q * E;
+        __builtin_nontemporal_store(c - q * E, &A.ocoords[(uint64_t)(A.nxa + m) * A.ostride + pos]);
         c = q;
     }
-    A.ovals[pos] = v;
+    __builtin_nontemporal_store(v, &A.ovals[pos]);
 }
 
 

@@ -672,6 +672,7 @@ int Contraction::run(const fcc_tensor *ta, const int32_t *am, const fcc_tensor *
     // (its labels are whole-tile output counts)
     const bool mirror = self && o.left_tile_stride <= 1 && !o.collect_stats && o.mirror >= 0 && !groups;
     ix.mirror = mirror;
+    dk.upper = mirror && dk.self;  // mirror mode reads the block on and above its diagonal only
     if (const char *hm = getenv("FCC_HEAVY_MIN")) ix.heavy_min = (uint32_t)std::max(1, atoi(hm));
     if (const char *hs = getenv("FCC_HSEG")) ix.hseg = (uint32_t)std::max(64, atoi(hs));
     ix.dk_hkey = dk.nh ? dk.hkey : nullptr;  // the left build takes these keys out of the tiles

@@ -29,6 +29,8 @@ struct DenseKeys {
     uint64_t products = 0;    // products of the selected keys
     float tau = 0;            // density threshold used
     bool self = false;        // self-contraction: AT is Bd, row bitsets are column bitsets
+    bool upper = false;       // C's lower triangle is never read (mirror mode reads the cells on and
+                              // above the diagonal only): the symmetric GEMM stores its tiles once
     bool full = false;        // a selected key is stored at every left row and every right column:
                               // every cell of the block is present (P all ones, no bitsets)
     uint32_t splits = 1;      // GEMM k splits: C holds this many ra x rb slabs, summed by the readers
@@ -78,9 +80,11 @@ void dense_keys_values(const Index &ix, const DenseKeys &dk, hipStream_t s);
 // AT[kp][ra], Bd[kp][rb]; ra, rb multiples of DK_BM / DK_BN, kp a multiple
 // of DK_BK; split z sums its share of the k steps. sym (AT == Bd, ra == rb:
 // C is symmetric) computes the tiles on and above the diagonal and mirrors
-// them, half the matrix-core work.
+// them, half the matrix-core work; with upper the mirror copies are not
+// stored (the 128 x 128 tiles on and above the diagonal hold every cell
+// with column >= row).
 void launch_dgemm_tn(const double *AT, const double *Bd, double *C, uint64_t ra, uint64_t rb, uint32_t kp,
-                     uint32_t splits, bool sym, hipStream_t s);
+                     uint32_t splits, bool sym, bool upper, hipStream_t s);
 // k splits that fill the GPU's workgroup slots for a small grid (1 for large).
 uint32_t dgemm_splits(uint64_t ra, uint64_t rb, uint32_t kp, bool sym, int num_cus);
 

@@ -210,7 +210,8 @@ static_assert(DK_BM == DK_BN, "one padded LDS row layout for both panels");
 
 __global__ __launch_bounds__(256, 2) void k_dgemm_tn(const double *__restrict__ AT, const double *__restrict__ Bd,
                                                      double *__restrict__ C, uint32_t nbm, uint32_t nbn, uint64_t ra,
-                                                     uint64_t rb, uint32_t kp, uint32_t splits, bool sym) {
+                                                     uint64_t rb, uint32_t kp, uint32_t splits, bool sym,
+                                                     bool upper) {
     // two buffers as four distinct LDS objects: the compiler then knows that
     // reads of one buffer do not depend on panel loads still landing in the
     // other, and waits for those loads only at the step's end
@@ -334,7 +335,7 @@ __global__ __launch_bounds__(256, 2) void k_dgemm_tn(const double *__restrict__ 
         for (int j = 0; j < 4; j++)
 #pragma unroll
             for (int r = 0; r < 4; r++) C[(row0 + i * 16 + 4 * r) * rb + col0 + j * 16] = acc[i][j][r];
-    if (sym && tm != tn) {
+    if (sym && !upper && tm != tn) {
         // The mirror tile (ra == rb), transposed through LDS so that every
         // store writes whole 512-byte column runs: each wavefront takes one
         // of the four panel buffers (free once all have passed the barrier)
@@ -368,12 +369,12 @@ __global__ __launch_bounds__(256, 2) void k_dgemm_tn(const double *__restrict__ 
 }  // namespace
 
 void launch_dgemm_tn(const double *AT, const double *Bd, double *C, uint64_t ra, uint64_t rb, uint32_t kp,
-                     uint32_t splits, bool sym, hipStream_t s) {
+                     uint32_t splits, bool sym, bool upper, hipStream_t s) {
     const uint32_t nbm = (uint32_t)(ra / DK_BM), nbn = (uint32_t)(rb / DK_BN);
     if (!nbm || !nbn || !kp) return;
     if (sym && (AT != Bd || ra != rb)) throw LibError{ERR_INVALID, "launch_dgemm_tn: sym needs AT == Bd and ra == rb"};
     const uint32_t nblk = sym ? nbm * (nbm + 1) / 2 : nbm * nbn;
-    k_dgemm_tn<<<nblk * splits, 256, 0, s>>>(AT, Bd, C, nbm, nbn, ra, rb, kp, splits, sym);
+    k_dgemm_tn<<<nblk * splits, 256, 0, s>>>(AT, Bd, C, nbm, nbn, ra, rb, kp, splits, sym, sym && upper);
 }
 
 uint32_t dgemm_splits(uint64_t ra, uint64_t rb, uint32_t kp, bool sym, int num_cus) {
@@ -503,7 +504,7 @@ void dense_keys_block(const Index &ix, const DenseKeys &dk, const uint32_t *fidA
         k_dk_presence<<<dim3((unsigned)(rb / 64), (unsigned)(ra / 64)), 256, 0, s>>>(dk.rowbits, dk.colbits, dk.spanA,
                                                                                      rb, dk.nw, dk.P);
     if (gemm0) FCC_HIP(hipEventRecord(gemm0, s));
-    launch_dgemm_tn(dk.AT, dk.Bd, dk.C, ra, rb, dk.kp, dk.splits, dk.self, s);
+    launch_dgemm_tn(dk.AT, dk.Bd, dk.C, ra, rb, dk.kp, dk.splits, dk.self, dk.upper, s);
     if (gemm1) FCC_HIP(hipEventRecord(gemm1, s));
 }
 
@@ -515,7 +516,7 @@ void dense_keys_values(const Index &ix, const DenseKeys &dk, hipStream_t s) {
         k_dk_scatter_A<<<grid_of(ix.ngroups), 256, 0, s>>>(ix.ngroups, ix.group_ptr, ix.group_fiber, dk.hkey, ix.Arow,
                                                             ix.Aval, dk.ra, dk.AT);
     k_dk_scatter_B<<<dk.nh, 256, 0, s>>>(dk.hfib, ix.fiber_ptr, ix.Bcol, ix.Bval, dk.rb, dk.Bd);
-    launch_dgemm_tn(dk.AT, dk.Bd, dk.C, dk.ra, dk.rb, dk.kp, dk.splits, dk.self, s);
+    launch_dgemm_tn(dk.AT, dk.Bd, dk.C, dk.ra, dk.rb, dk.kp, dk.splits, dk.self, dk.upper, s);
 }
 
 }  // namespace fcc
