@@ -177,22 +177,26 @@ __device__ inline uint32_t lin_ext(const uint32_t *coords, size_t nnz, size_t n,
 }
 
 // B: comp = key << cbits | col  (coordinate.hpp:723-733 and 767-781)
+// (idx null: the sort carries the values, not the entry numbers)
 __global__ void k_lin_B(const uint32_t *coords, size_t nnz, Geom g, uint64_t *comp, uint32_t *idx) {
     size_t n = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
     if (n >= nnz) return;
     uint64_t key = lin_key(coords, nnz, n, g.bm, g.R, g.r);
     uint32_t col = lin_ext(coords, nnz, n, g.xb, g.xbE, g.nxb);
     comp[n] = (key << g.cbits) | col;
-    idx[n] = (uint32_t)n;
+    if (idx) idx[n] = (uint32_t)n;
 }
 
-__global__ void k_gather_B(const uint64_t *comp, const uint32_t *idx, const double *vals, size_t n, int cbits,
-                           uint32_t *Bcol, double *Bval, uint32_t *flag) {
+// Sorted right entries: column, value and fiber-start flag. The value comes
+// with the sort (sval) or, when the sort carried entry numbers, is gathered
+// through them (idx).
+__global__ void k_gather_B(const uint64_t *comp, const uint32_t *idx, const double *vals, const uint64_t *sval,
+                           size_t n, int cbits, uint32_t *Bcol, double *Bval, uint32_t *flag) {
     size_t p = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
     if (p >= n) return;
     uint64_t c = comp[p];
     Bcol[p] = (uint32_t)(c & ((1ull << cbits) - 1));
-    Bval[p] = vals[idx[p]];
+    Bval[p] = sval ? __longlong_as_double((long long)sval[p]) : vals[idx[p]];
     flag[p] = (p == 0 || (c >> cbits) != (comp[p - 1] >> cbits)) ? 1u : 0u;
 }
 
@@ -701,12 +705,14 @@ __global__ __launch_bounds__(256) void k_self_join(const uint32_t *vs, const uin
     uint64_t prod = 0;
     for (size_t p = blockIdx.x * (size_t)blockDim.x + threadIdx.x; p < n; p += (size_t)gridDim.x * blockDim.x) {
         const uint32_t f = Bfib[p];
-        uint32_t o = vs[p];
-        if (rowA) {  // per original entry (random writes): only when asked for
-            rowA[o] = Bcol[p];
-            fidA[o] = f;
+        if (vs) {  // entry numbers carried by the sort
+            const uint32_t o = vs[p];
+            if (rowA) {  // per original entry (random writes): only when asked for
+                rowA[o] = Bcol[p];
+                fidA[o] = f;
+            }
+            vsB[p] = o;
         }
-        vsB[p] = o;
         prod += fiber_ptr[f + 1] - fiber_ptr[f];
     }
     for (int d = 32; d > 0; d >>= 1) prod += __shfl_xor(prod, d, 64);
@@ -728,16 +734,25 @@ void build_right(DevPool &pool, const Geom &g, const Operand &B, Index &ix, std:
         hold.emplace_back(p, [pl](void *q) { pl->put(q); });
         return p;
     };
+    // The sort carries the values themselves (no gather through a
+    // permutation afterwards) unless the entry numbers are wanted: plans keep
+    // the permutation, sampled self-contractions join per original entry.
+    const bool by_value = !ix.keep_perm && !(self && self->rowA);
     Tmp<uint64_t> k1(pool, n), k2(pool, n);
-    Tmp<uint32_t> v1(pool, n), v2(pool, n);
-    k_lin_B<<<grid_for(n), 256, 0, s>>>(B.coords, n, g, k1, v1);
-    uint64_t *ks;
-    uint32_t *vs;
-    radix_sort_pairs(pool, k1, v1, k2, v2, n, g.kbits + g.cbits, s, &ks, &vs);
+    Tmp<uint32_t> v1(pool, by_value ? 0 : n), v2(pool, by_value ? 0 : n);
+    Tmp<uint64_t> w1(pool, by_value ? n : 0), w2(pool, by_value ? n : 0);
+    k_lin_B<<<grid_for(n), 256, 0, s>>>(B.coords, n, g, k1, by_value ? nullptr : v1.get());
+    uint64_t *ks, *ws = nullptr;
+    uint32_t *vs = nullptr;
+    if (by_value)
+        radix_sort_keys_values(pool, k1, reinterpret_cast<const uint64_t *>(B.vals), w1, k2, w2, n,
+                               g.kbits + g.cbits, s, &ks, &ws);
+    else
+        radix_sort_pairs(pool, k1, v1, k2, v2, n, g.kbits + g.cbits, s, &ks, &vs);
     ix.Bcol = (uint32_t *)keep(n * 4);
     ix.Bval = (double *)keep(n * 8);
     Tmp<uint32_t> flag(pool, n), fid(pool, n);
-    k_gather_B<<<grid_for(n), 256, 0, s>>>(ks, vs, B.vals, n, g.cbits, ix.Bcol, ix.Bval, flag);
+    k_gather_B<<<grid_for(n), 256, 0, s>>>(ks, vs, B.vals, ws, n, g.cbits, ix.Bcol, ix.Bval, flag);
     if (ix.keep_perm) {
         ix.permB = (uint32_t *)keep(n * 4);
         if (n) FCC_HIP(hipMemcpyAsync(ix.permB, vs, n * 4, hipMemcpyDeviceToDevice, s));

@@ -144,6 +144,11 @@ void scan_u64(DevPool &pool, const uint64_t *in, uint64_t *out, size_t n, uint64
 void radix_sort_pairs(DevPool &pool, uint64_t *keys, uint32_t *vals, uint64_t *keys_alt,
                       uint32_t *vals_alt, size_t n, int nbits, hipStream_t s,
                       uint64_t **keys_out, uint32_t **vals_out, int lo_bit = 0);
+// The same with a 64-bit payload whose first pass reads vals_in (left
+// unchanged) and whose later passes ping-pong between vals and vals_alt.
+void radix_sort_keys_values(DevPool &pool, uint64_t *keys, const uint64_t *vals_in, uint64_t *vals,
+                            uint64_t *keys_alt, uint64_t *vals_alt, size_t n, int nbits, hipStream_t s,
+                            uint64_t **keys_out, uint64_t **vals_out);
 
 // ------------------------------------------------------------------- index
 struct Operand {

@@ -270,11 +270,12 @@ __global__ __launch_bounds__(RS_THREADS) void k_rs_hist(const uint64_t *keys, si
     hist[(size_t)threadIdx.x * nblocks + blockIdx.x] = c;
 }
 
-__global__ __launch_bounds__(RS_THREADS) void k_rs_scatter(const uint64_t *keys, const uint32_t *vals, uint64_t *okeys,
-                                                           uint32_t *ovals, size_t n, int shift,
-                                                           const uint32_t *hist_scan, unsigned nblocks) {
+template <typename VT>
+__global__ __launch_bounds__(RS_THREADS) void k_rs_scatter(const uint64_t *keys, const VT *vals, uint64_t *okeys,
+                                                           VT *ovals, size_t n, int shift, const uint32_t *hist_scan,
+                                                           unsigned nblocks) {
     __shared__ uint64_t s_key[RS_TILE];
-    __shared__ uint32_t s_val[RS_TILE];
+    __shared__ VT s_val[RS_TILE];
     __shared__ uint32_t s_wh[RS_WAVES][256];  // per wavefront: digit counts, then its digit bases
     __shared__ uint32_t s_gbase[256];         // global start of this tile's run of digit d
     __shared__ uint32_t s_lbase[256];         // start of digit d's run inside the tile
@@ -285,12 +286,12 @@ __global__ __launch_bounds__(RS_THREADS) void k_rs_scatter(const uint64_t *keys,
     // wavefront w owns tile elements [w * 1024, (w + 1) * 1024); item j of a
     // lane is element w * 1024 + j * 64 + lane (coalesced loads)
     uint64_t rk[RS_ITEMS];
-    uint32_t rv[RS_ITEMS];
+    VT rv[RS_ITEMS];
 #pragma unroll
     for (int j = 0; j < RS_ITEMS; j++) {
         const uint32_t li = (uint32_t)(wave * RS_WAVE_ELEMS + j * 64 + lane);
         rk[j] = li < tn ? keys[t0 + li] : 0ull;
-        rv[j] = li < tn ? vals[t0 + li] : 0u;
+        rv[j] = li < tn ? vals[t0 + li] : VT(0);
     }
 #pragma unroll
     for (int w = 0; w < RS_WAVES; w++) s_wh[w][tid] = 0;
@@ -365,11 +366,14 @@ __global__ __launch_bounds__(RS_THREADS) void k_rs_scatter(const uint64_t *keys,
     }
 }
 
-void radix_sort_pairs(DevPool &pool, uint64_t *keys, uint32_t *vals, uint64_t *keys_alt, uint32_t *vals_alt,
-                      size_t n, int nbits, hipStream_t s, uint64_t **keys_out, uint32_t **vals_out, int lo_bit) {
+template <typename VT>
+static void radix_sort_impl(DevPool &pool, uint64_t *keys, const VT *vals_in, VT *vals, uint64_t *keys_alt,
+                            VT *vals_alt, size_t n, int nbits, hipStream_t s, uint64_t **keys_out, VT **vals_out,
+                            int lo_bit) {
     if (n >= (1ull << 32)) throw LibError{ERR_INVALID, "radix sort limited to < 2^32 elements"};
     uint64_t *ka = keys, *kb = keys_alt;
-    uint32_t *va = vals, *vb = vals_alt;
+    VT *va = vals, *vb = vals_alt;
+    const VT *src = vals_in;  // the first pass's payload (then the alternates)
     if (n > 1 && nbits > 0) {
         unsigned nblocks = (unsigned)div_up(n, RS_TILE);
         size_t hn = (size_t)256 * nblocks;
@@ -377,13 +381,27 @@ void radix_sort_pairs(DevPool &pool, uint64_t *keys, uint32_t *vals, uint64_t *k
         for (int shift = lo_bit; shift < lo_bit + nbits; shift += 8) {
             k_rs_hist<<<nblocks, RS_THREADS, 0, s>>>(ka, n, shift, hist, nblocks);
             scan_u32(pool, hist, hist, hn, nullptr, s);
-            k_rs_scatter<<<nblocks, RS_THREADS, 0, s>>>(ka, va, kb, vb, n, shift, hist, nblocks);
+            k_rs_scatter<VT><<<nblocks, RS_THREADS, 0, s>>>(ka, src, kb, vb, n, shift, hist, nblocks);
             std::swap(ka, kb);
             std::swap(va, vb);
+            src = va;
         }
+    } else if (n && src != va) {
+        FCC_HIP(hipMemcpyAsync(va, src, n * sizeof(VT), hipMemcpyDeviceToDevice, s));
     }
     *keys_out = ka;
     *vals_out = va;
+}
+
+void radix_sort_pairs(DevPool &pool, uint64_t *keys, uint32_t *vals, uint64_t *keys_alt, uint32_t *vals_alt,
+                      size_t n, int nbits, hipStream_t s, uint64_t **keys_out, uint32_t **vals_out, int lo_bit) {
+    radix_sort_impl<uint32_t>(pool, keys, vals, vals, keys_alt, vals_alt, n, nbits, s, keys_out, vals_out, lo_bit);
+}
+
+void radix_sort_keys_values(DevPool &pool, uint64_t *keys, const uint64_t *vals_in, uint64_t *vals,
+                            uint64_t *keys_alt, uint64_t *vals_alt, size_t n, int nbits, hipStream_t s,
+                            uint64_t **keys_out, uint64_t **vals_out) {
+    radix_sort_impl<uint64_t>(pool, keys, vals_in, vals, keys_alt, vals_alt, n, nbits, s, keys_out, vals_out, 0);
 }
 
 }  // namespace fcc
