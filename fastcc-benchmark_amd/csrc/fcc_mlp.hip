@@ -50,42 +50,50 @@ __global__ void k_tile_features(const uint32_t *st_nnz, const uint32_t *st_keys,
 }
 
 // ------------------------------------------------------------- inference
-// One thread per tile; weights staged in LDS and read as broadcasts.
+// One thread per tile; weights staged in LDS and read as broadcasts. Layer 2
+// runs with the hidden unit j outside and the 64 output sums inside, so the
+// 64 sums are independent chains (each still adds in j order: the result is
+// the plain row-by-row evaluation's).
 __global__ __launch_bounds__(256) void k_mlp_infer(const float *P, const float *feat, uint32_t n, float *pred_nnz,
                                                    float *pred_cols) {
     __shared__ float w[N_PARAMS];
     for (int i = threadIdx.x; i < N_PARAMS; i += blockDim.x) w[i] = P[i];
     __syncthreads();
-    uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= n) return;
-    float x[NI];
+    for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < n; t += gridDim.x * blockDim.x) {
+        const float *f = feat + (size_t)t * NI;
+        float x[NI];
 #pragma unroll
-    for (int i = 0; i < NI; i++) x[i] = (feat[(size_t)t * NI + i] - w[OFF_IM + i]) / w[OFF_IS + i];
-    float h1[NH];
+        for (int i = 0; i < NI; i++) x[i] = (f[i] - w[OFF_IM + i]) / w[OFF_IS + i];
+        float h1[NH];
 #pragma unroll
-    for (int j = 0; j < NH; j++) {
-        float s = w[OFF_B1 + j];
+        for (int j = 0; j < NH; j++) {
+            float s = w[OFF_B1 + j];
 #pragma unroll
-        for (int i = 0; i < NI; i++) s += x[i] * w[OFF_W1 + i * NH + j];
-        h1[j] = s > 0.f ? s : 0.f;
+            for (int i = 0; i < NI; i++) s += x[i] * w[OFF_W1 + i * NH + j];
+            h1[j] = s > 0.f ? s : 0.f;
+        }
+        float h2[NH];
+#pragma unroll
+        for (int k = 0; k < NH; k++) h2[k] = w[OFF_B2 + k];
+#pragma unroll
+        for (int j = 0; j < NH; j++)
+#pragma unroll
+            for (int k = 0; k < NH; k++) h2[k] += h1[j] * w[OFF_W2 + j * NH + k];
+        float y[NO];
+#pragma unroll
+        for (int o = 0; o < NO; o++) y[o] = w[OFF_B3 + o];
+#pragma unroll
+        for (int k = 0; k < NH; k++) {
+            const float r = h2[k] > 0.f ? h2[k] : 0.f;
+#pragma unroll
+            for (int o = 0; o < NO; o++) y[o] += r * w[OFF_W3 + k * NO + o];
+        }
+        float r0 = fminf(y[0] * w[OFF_OS] + w[OFF_OM], 0.f);
+        float r1 = fminf(y[1] * w[OFF_OS + 1] + w[OFF_OM + 1], 0.f);
+        float ub0 = fminf(f[3], f[5]), ub1 = f[2];
+        pred_nnz[t] = fmaxf(expm1f(fminf(r0 + ub0, 80.f)), 0.f);
+        if (pred_cols) pred_cols[t] = fmaxf(expm1f(fminf(r1 + ub1, 80.f)), 0.f);
     }
-    float y[NO];
-#pragma unroll
-    for (int o = 0; o < NO; o++) y[o] = w[OFF_B3 + o];
-    for (int k = 0; k < NH; k++) {
-        float s = w[OFF_B2 + k];
-#pragma unroll
-        for (int j = 0; j < NH; j++) s += h1[j] * w[OFF_W2 + j * NH + k];
-        s = s > 0.f ? s : 0.f;
-#pragma unroll
-        for (int o = 0; o < NO; o++) y[o] += s * w[OFF_W3 + k * NO + o];
-    }
-    const float *f = feat + (size_t)t * NI;
-    float r0 = fminf(y[0] * w[OFF_OS] + w[OFF_OM], 0.f);
-    float r1 = fminf(y[1] * w[OFF_OS + 1] + w[OFF_OM + 1], 0.f);
-    float ub0 = fminf(f[3], f[5]), ub1 = f[2];
-    pred_nnz[t] = fmaxf(expm1f(fminf(r0 + ub0, 80.f)), 0.f);
-    if (pred_cols) pred_cols[t] = fmaxf(expm1f(fminf(r1 + ub1, 80.f)), 0.f);
 }
 
 // ------------------------------------------------------ forward + backward
@@ -220,7 +228,9 @@ void mlp_tile_features(const Index &ix, uint64_t spanB, float *feat, hipStream_t
 void mlp_infer(const float *dparams, const float *feat, uint32_t n, float *pred_nnz, float *pred_cols,
                hipStream_t s) {
     if (!n) return;
-    k_mlp_infer<<<(unsigned)div_up(n, 256), 256, 0, s>>>(dparams, feat, n, pred_nnz, pred_cols);
+    // at most 2048 workgroups (each stages the weights once), looping over the rest
+    k_mlp_infer<<<(unsigned)std::min<uint64_t>(div_up(n, 256), 2048), 256, 0, s>>>(dparams, feat, n, pred_nnz,
+                                                                                     pred_cols);
 }
 
 void mlp_fwd_bwd(const float *dparams, const float *feat, const float *label, const uint32_t *perm, uint32_t n,
