@@ -50,10 +50,15 @@ __global__ void k_tile_features(const uint32_t *st_nnz, const uint32_t *st_keys,
 }
 
 // ------------------------------------------------------------- inference
-// One thread per tile; weights staged in LDS and read as broadcasts. Layer 2
-// runs with the hidden unit j outside and the 64 output sums inside, so the
-// 64 sums are independent chains (each still adds in j order: the result is
-// the plain row-by-row evaluation's).
+// One thread per tile; weights staged in LDS and read as broadcasts. The
+// second layer's 64 sums are taken 16 at a time: a rolled loop over the first
+// layer's units j computes unit j and adds it to the 16 sums, so they are
+// independent chains (each still adds in j order, then the output layer adds
+// k in order: the plain row-by-row evaluation's result). The 16-sum chunks
+// keep the kernel small in registers and code: it runs beside the dense-key
+// GEMM on the other stream (a 400-register version waited for the GEMM to
+// drain) and from the instruction cache.
+constexpr int KC = 16;  // second-layer sums per chunk
 __global__ __launch_bounds__(256) void k_mlp_infer(const float *P, const float *feat, uint32_t n, float *pred_nnz,
                                                    float *pred_cols) {
     __shared__ float w[N_PARAMS];
@@ -64,32 +69,30 @@ __global__ __launch_bounds__(256) void k_mlp_infer(const float *P, const float *
         float x[NI];
 #pragma unroll
         for (int i = 0; i < NI; i++) x[i] = (f[i] - w[OFF_IM + i]) / w[OFF_IS + i];
-        float h1[NH];
+        float y0 = w[OFF_B3], y1 = w[OFF_B3 + 1];
+#pragma unroll 1
+        for (int k0 = 0; k0 < NH; k0 += KC) {
+            float h2[KC];
 #pragma unroll
-        for (int j = 0; j < NH; j++) {
-            float s = w[OFF_B1 + j];
+            for (int k = 0; k < KC; k++) h2[k] = w[OFF_B2 + k0 + k];
+#pragma unroll 2
+            for (int j = 0; j < NH; j++) {
+                float h1 = w[OFF_B1 + j];
 #pragma unroll
-            for (int i = 0; i < NI; i++) s += x[i] * w[OFF_W1 + i * NH + j];
-            h1[j] = s > 0.f ? s : 0.f;
+                for (int i = 0; i < NI; i++) h1 += x[i] * w[OFF_W1 + i * NH + j];
+                h1 = h1 > 0.f ? h1 : 0.f;
+#pragma unroll
+                for (int k = 0; k < KC; k++) h2[k] += h1 * w[OFF_W2 + j * NH + k0 + k];
+            }
+#pragma unroll
+            for (int k = 0; k < KC; k++) {
+                const float r = h2[k] > 0.f ? h2[k] : 0.f;
+                y0 += r * w[OFF_W3 + (k0 + k) * NO];
+                y1 += r * w[OFF_W3 + (k0 + k) * NO + 1];
+            }
         }
-        float h2[NH];
-#pragma unroll
-        for (int k = 0; k < NH; k++) h2[k] = w[OFF_B2 + k];
-#pragma unroll
-        for (int j = 0; j < NH; j++)
-#pragma unroll
-            for (int k = 0; k < NH; k++) h2[k] += h1[j] * w[OFF_W2 + j * NH + k];
-        float y[NO];
-#pragma unroll
-        for (int o = 0; o < NO; o++) y[o] = w[OFF_B3 + o];
-#pragma unroll
-        for (int k = 0; k < NH; k++) {
-            const float r = h2[k] > 0.f ? h2[k] : 0.f;
-#pragma unroll
-            for (int o = 0; o < NO; o++) y[o] += r * w[OFF_W3 + k * NO + o];
-        }
-        float r0 = fminf(y[0] * w[OFF_OS] + w[OFF_OM], 0.f);
-        float r1 = fminf(y[1] * w[OFF_OS + 1] + w[OFF_OM + 1], 0.f);
+        float r0 = fminf(y0 * w[OFF_OS] + w[OFF_OM], 0.f);
+        float r1 = fminf(y1 * w[OFF_OS + 1] + w[OFF_OM + 1], 0.f);
         float ub0 = fminf(f[3], f[5]), ub1 = f[2];
         pred_nnz[t] = fmaxf(expm1f(fminf(r0 + ub0, 80.f)), 0.f);
         if (pred_cols) pred_cols[t] = fmaxf(expm1f(fminf(r1 + ub1, 80.f)), 0.f);
