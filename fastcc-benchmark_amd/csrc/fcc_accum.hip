@@ -1017,6 +1017,41 @@ __device__ __forceinline__ uint32_t dk_dense_word(const AccArgs &A, const UnitCt
 template <bool BITMAP>
 __device__ __forceinline__ void dk_seed_dense(const AccArgs &A, const UnitCtx &U, double *fv, uint32_t *bits, uint32_t nslots) {
     const double init = BITMAP ? 0.0 : -0.0;
+    const uint32_t TL = 1u << A.logTL;
+    if (U.Wp >= 2 * ACC_THREADS && !(U.c0 & 1u)) {
+        // wide rows: row by row, two cells per thread (one 16-byte load of
+        // each slab, no division); in mirror mode the cells left of a row's
+        // diagonal are never emitted and are not read
+        const uint32_t np = U.Wp / 2;
+        for (uint32_t li = 0; li < TL; li++) {
+            const uint64_t row = U.row0 + li;
+            double2 *out = reinterpret_cast<double2 *>(fv + (size_t)li * U.Wp);
+            const uint32_t *prow = A.dk_P + row * A.dk_pw;
+            const double *crow = A.dk_C + row * A.dk_rb + U.c0;
+            const uint32_t cfirst = (A.mirror && row > U.c0) ? (uint32_t)min<uint64_t>(row - U.c0, U.W) : 0u;
+#pragma unroll 4
+            for (uint32_t q = threadIdx.x; q < np; q += ACC_THREADS) {
+                const uint32_t cl = 2 * q;
+                double2 v = make_double2(init, init);
+                if (row < A.spanA && cl + 1 >= cfirst && cl < U.W) {
+                    const uint64_t col = (uint64_t)U.c0 + cl;
+                    const uint32_t pw = prow[col >> 5];
+                    double2 c = *reinterpret_cast<const double2 *>(crow + cl);
+                    for (uint32_t z = 1; z < A.dk_splits; z++) {
+                        const double2 c2 = *reinterpret_cast<const double2 *>(crow + cl + (uint64_t)z * A.dk_slab);
+                        c.x += c2.x;
+                        c.y += c2.y;
+                    }
+                    if ((pw >> (col & 31)) & 1u) v.x = BITMAP ? c.x : c.x + 0.0;
+                    if (cl + 1 < U.W && ((pw >> ((col + 1) & 31)) & 1u)) v.y = BITMAP ? c.y : c.y + 0.0;
+                }
+                out[q] = v;
+            }
+        }
+        if (BITMAP)
+            for (uint32_t w = threadIdx.x; w < nslots / 32; w += ACC_THREADS) bits[w] = dk_dense_word(A, U, w);
+        return;
+    }
     // the presence word and the value are loaded together (every cell of the
     // unit's rows x window lies inside the padded block), several cells per
     // thread in flight: the loop is bound by load latency, not bandwidth
