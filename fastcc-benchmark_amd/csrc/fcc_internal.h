@@ -168,7 +168,7 @@ constexpr uint8_t KIND_CBITS = 5;
 constexpr uint32_t DIRECT_MAX_TL = 64;  // rows per tile up to which DIRECT tiles are detected
 constexpr uint64_t DIRECT_RUN = 65536;  // products per unit made of consecutive DIRECT tiles
 
-constexpr uint32_t HEAVY_MIN = 32;      // default: fibers at least this long are processed wave-per-group
+constexpr uint32_t HEAVY_MIN = 64;      // default: fibers at least this long are processed wave-per-group
 constexpr uint32_t DENSE_SLOTS = 9984;  // f64 LDS accumulator slots (78 KB: two workgroups per CU)
 constexpr uint32_t LH_SLOTS = 4096;     // LDS hash slots (f64 value + u32 key)
 constexpr uint32_t LH_MAX = 3072;       // LDS hash distinct-key limit before sealing
