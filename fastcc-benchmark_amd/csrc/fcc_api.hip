@@ -658,7 +658,7 @@ int Contraction::run(const fcc_tensor *ta, const int32_t *am, const fcc_tensor *
         FCC_HIP(hipStreamWaitEvent(ctx->side, ev[9], 0));
         side_join.st = ctx->side;
         FCC_HIP(hipEventRecord(ev[7], ctx->side));
-        dense_keys_block(ix, dk, fidA.get(), rowA.get(), A.vals, (size_t)A.nnz, ctx->side, ev[10], ev[11]);
+        dense_keys_block(ix, dk, fidA.get(), rowA.get(), A.vals, (size_t)A.nnz, ctx->side, s, ev[10], ev[11]);
         FCC_HIP(hipEventRecord(ev[8], ctx->side));
         if (k.fcut.get()) launch_fiber_cuts(ix, fcut_n, fcut_W, k.fcut, s);
     };

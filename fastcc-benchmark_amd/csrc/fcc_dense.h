@@ -18,6 +18,7 @@ namespace fcc {
 
 constexpr uint32_t DK_BM = 128;  // GEMM tile rows (left rows)
 constexpr uint32_t DK_BN = 128;  // GEMM tile columns (right columns)
+constexpr uint32_t DK_FIBER_SPLIT = 8;  // workgroups per selected fiber in the panel and bitset scatters
 constexpr uint32_t DK_BK = 16;   // GEMM k step (selected keys)
 
 struct DenseKeys {
@@ -66,9 +67,12 @@ void dense_keys_alloc(DevPool &pool, const Geom &g, DenseKeys &dk, int num_cus,
 // entry, nA entries; unused for a self-contraction): column (and row)
 // bitsets, presence map, value panels, GEMM. Reads nothing the left-tile
 // build writes; fidA/rowA/Avals must stay valid until it completes.
-// gemm0 / gemm1 (optional) are recorded around the GEMM launch.
+// gemm0 / gemm1 (optional) are recorded around the GEMM launch. When a full
+// key is selected the presence map is a plain fill, queued on stream sp (the
+// caller's stream, which reads the map next) so that it does not delay the
+// GEMM on s.
 void dense_keys_block(const Index &ix, const DenseKeys &dk, const uint32_t *fidA, const uint32_t *rowA,
-                      const double *Avals, size_t nA, hipStream_t s, hipEvent_t gemm0 = nullptr,
+                      const double *Avals, size_t nA, hipStream_t s, hipStream_t sp, hipEvent_t gemm0 = nullptr,
                       hipEvent_t gemm1 = nullptr);
 
 // Values again (plans): scatters the selected keys' current values (ix.Aval,

@@ -64,13 +64,19 @@ __global__ void k_dk_flag(const uint32_t *fiber_ptr, const uint32_t *na, uint32_
 
 // Presence map when a full key is selected: every cell (row < spanA, column
 // < spanB) is present; the padding rows and columns of the block are not.
-__global__ void k_dk_full_presence(uint32_t *P, uint64_t ra, uint64_t pw, uint64_t spanA, uint64_t spanB) {
-    const uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
-    if (i >= ra * pw) return;
-    const uint64_t row = i / pw, c0 = (i - row * pw) * 32;
-    uint32_t v = 0u;
-    if (row < spanA && c0 < spanB) v = spanB - c0 >= 32 ? 0xFFFFFFFFu : ((1u << (spanB - c0)) - 1u);
-    P[i] = v;
+// One thread per four words (rows hold pw = rb / 32 words, a multiple of 4;
+// the block holds at most 2^29 cells, so word indices fit 32 bits).
+__global__ void k_dk_full_presence(uint4 *P4, uint32_t ra, uint32_t pw4, uint32_t spanA, uint32_t spanB) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= ra * pw4) return;
+    const uint32_t row = i / pw4, c0 = (i - row * pw4) * 128;
+    uint32_t w[4];
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+        const uint32_t c = c0 + 32u * j;
+        w[j] = (row < spanA && c < spanB) ? (spanB - c >= 32 ? 0xFFFFFFFFu : ((1u << (spanB - c)) - 1u)) : 0u;
+    }
+    P4[i] = make_uint4(w[0], w[1], w[2], w[3]);
 }
 
 // hkey[f] = the selected key index of fiber f (or none), hfib[h] = its fiber;
@@ -85,12 +91,16 @@ __global__ void k_dk_keys(const uint32_t *flag, const uint32_t *idx, uint32_t nf
 }
 
 // one workgroup per selected key: column bitsets
+// A selected key's fiber is spread over gridDim.y workgroups (Zipf keys:
+// the longest fibers hold a large share of the selected entries).
 __global__ __launch_bounds__(256) void k_dk_colbits(const uint32_t *hfib, const uint32_t *fiber_ptr,
                                                    const uint32_t *Bcol, uint64_t rb, unsigned long long *colbits) {
     const uint32_t h = blockIdx.x, f = hfib[h];
     const unsigned long long bit = 1ull << (h & 63);
     unsigned long long *row = colbits + (uint64_t)(h >> 6) * rb;
-    for (uint32_t p = fiber_ptr[f] + threadIdx.x; p < fiber_ptr[f + 1]; p += blockDim.x) atomicOr(&row[Bcol[p]], bit);
+    const uint32_t step = gridDim.y * blockDim.x;
+    for (uint32_t p = fiber_ptr[f] + blockIdx.y * blockDim.x + threadIdx.x; p < fiber_ptr[f + 1]; p += step)
+        atomicOr(&row[Bcol[p]], bit);
 }
 
 // P[row][c >> 5] bit (c & 31) = (rowbits[row] & colbits[c]) != 0.
@@ -189,7 +199,8 @@ __global__ __launch_bounds__(256) void k_dk_scatter_B(const uint32_t *hfib, cons
                                                      double *Bd) {
     const uint32_t h = blockIdx.x, f = hfib[h];
     double *row = Bd + (uint64_t)h * rb;
-    for (uint32_t p = fiber_ptr[f] + threadIdx.x; p < fiber_ptr[f + 1]; p += blockDim.x)
+    const uint32_t step = gridDim.y * blockDim.x;
+    for (uint32_t p = fiber_ptr[f] + blockIdx.y * blockDim.x + threadIdx.x; p < fiber_ptr[f + 1]; p += step)
         atomicAdd(&row[Bcol[p]], Bval[p]);
 }
 
@@ -482,16 +493,17 @@ void dense_keys_alloc(DevPool &pool, const Geom &g, DenseKeys &dk, int num_cus,
 }
 
 void dense_keys_block(const Index &ix, const DenseKeys &dk, const uint32_t *fidA, const uint32_t *rowA,
-                      const double *Avals, size_t nA, hipStream_t s, hipEvent_t gemm0, hipEvent_t gemm1) {
+                      const double *Avals, size_t nA, hipStream_t s, hipStream_t sp, hipEvent_t gemm0,
+                      hipEvent_t gemm1) {
     if (!dk.nh) return;
     const uint64_t ra = dk.ra, rb = dk.rb;
     fill_u32(dk.P + ra * (rb / 32), 2, 0, s);
     fill_f64(dk.Bd, (size_t)dk.kp * rb, 0.0, s);
     if (!dk.full) {
         fill_u64(dk.colbits, (size_t)dk.nw * rb, 0, s);
-        k_dk_colbits<<<dk.nh, 256, 0, s>>>(dk.hfib, ix.fiber_ptr, ix.Bcol, rb, dk.colbits);
+        k_dk_colbits<<<dim3(dk.nh, DK_FIBER_SPLIT), 256, 0, s>>>(dk.hfib, ix.fiber_ptr, ix.Bcol, rb, dk.colbits);
     }
-    k_dk_scatter_B<<<dk.nh, 256, 0, s>>>(dk.hfib, ix.fiber_ptr, ix.Bcol, ix.Bval, rb, dk.Bd);
+    k_dk_scatter_B<<<dim3(dk.nh, DK_FIBER_SPLIT), 256, 0, s>>>(dk.hfib, ix.fiber_ptr, ix.Bcol, ix.Bval, rb, dk.Bd);
     if (!dk.self) {
         fill_u64(dk.rowbits, (size_t)dk.spanA * dk.nw, 0, s);
         fill_f64(dk.AT, (size_t)dk.kp * ra, 0.0, s);
@@ -499,7 +511,8 @@ void dense_keys_block(const Index &ix, const DenseKeys &dk, const uint32_t *fidA
             k_dk_join_A<<<grid_of(nA), 256, 0, s>>>(fidA, rowA, Avals, nA, dk.hkey, dk.nw, ra, dk.rowbits, dk.AT);
     }
     if (dk.full)
-        k_dk_full_presence<<<grid_of(ra * (rb / 32)), 256, 0, s>>>(dk.P, ra, rb / 32, dk.spanA, dk.spanB);
+        k_dk_full_presence<<<grid_of(ra * (rb / 128)), 256, 0, sp>>>((uint4 *)dk.P, (uint32_t)ra, (uint32_t)(rb / 128),
+                                                                     (uint32_t)dk.spanA, (uint32_t)dk.spanB);
     else
         k_dk_presence<<<dim3((unsigned)(rb / 64), (unsigned)(ra / 64)), 256, 0, s>>>(dk.rowbits, dk.colbits, dk.spanA,
                                                                                      rb, dk.nw, dk.P);
@@ -515,7 +528,7 @@ void dense_keys_values(const Index &ix, const DenseKeys &dk, hipStream_t s) {
     if (ix.ngroups && !dk.self)
         k_dk_scatter_A<<<grid_of(ix.ngroups), 256, 0, s>>>(ix.ngroups, ix.group_ptr, ix.group_fiber, dk.hkey, ix.Arow,
                                                             ix.Aval, dk.ra, dk.AT);
-    k_dk_scatter_B<<<dk.nh, 256, 0, s>>>(dk.hfib, ix.fiber_ptr, ix.Bcol, ix.Bval, dk.rb, dk.Bd);
+    k_dk_scatter_B<<<dim3(dk.nh, DK_FIBER_SPLIT), 256, 0, s>>>(dk.hfib, ix.fiber_ptr, ix.Bcol, ix.Bval, dk.rb, dk.Bd);
     launch_dgemm_tn(dk.AT, dk.Bd, dk.C, dk.ra, dk.rb, dk.kp, dk.splits, dk.self, dk.upper, s);
 }
 

@@ -248,11 +248,15 @@ constexpr int RS_WAVE_ELEMS = RS_TILE / RS_WAVES;  // 1024
 
 __global__ __launch_bounds__(RS_THREADS) void k_rs_hist(const uint64_t *keys, size_t n, int shift, uint32_t *hist,
                                                         unsigned nblocks) {
-    // one histogram per wavefront: skewed digits (Zipf keys) contend less
-    __shared__ uint32_t h[RS_WAVES][256];
-    for (int w = 0; w < RS_WAVES; w++) h[w][threadIdx.x] = 0;
+    // four histograms per wavefront, lane l counting into copy l % 4:
+    // skewed digits (Zipf keys, many lanes on one digit) serialise 4x less on
+    // one address; rows are 257 words apart so equal digits of different
+    // copies sit in different banks
+    constexpr int HC = RS_WAVES * 4, HS = 257;
+    __shared__ uint32_t h[HC * HS];
+    for (int c = 0; c < HC; c++) h[c * HS + threadIdx.x] = 0;
     __syncthreads();
-    const int wave = threadIdx.x >> 6;
+    uint32_t *hc = h + ((threadIdx.x >> 6) * 4 + (threadIdx.x & 3)) * HS;
     size_t t0 = (size_t)blockIdx.x * RS_TILE;
     uint32_t dg[RS_ITEMS];  // all loads in flight before the first atomic
 #pragma unroll
@@ -262,11 +266,11 @@ __global__ __launch_bounds__(RS_THREADS) void k_rs_hist(const uint64_t *keys, si
     }
 #pragma unroll
     for (int j = 0; j < RS_ITEMS; j++)
-        if (dg[j] < 256u) atomicAdd(&h[wave][dg[j]], 1u);
+        if (dg[j] < 256u) atomicAdd(&hc[dg[j]], 1u);
     __syncthreads();
     uint32_t c = 0;
 #pragma unroll
-    for (int w = 0; w < RS_WAVES; w++) c += h[w][threadIdx.x];
+    for (int k = 0; k < HC; k++) c += h[k * HS + threadIdx.x];
     hist[(size_t)threadIdx.x * nblocks + blockIdx.x] = c;
 }
 
