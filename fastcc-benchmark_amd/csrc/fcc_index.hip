@@ -187,16 +187,16 @@ __global__ void k_lin_B(const uint32_t *coords, size_t nnz, Geom g, uint64_t *co
     if (idx) idx[n] = (uint32_t)n;
 }
 
-// Sorted right entries: column, value and fiber-start flag. The value comes
-// with the sort (sval) or, when the sort carried entry numbers, is gathered
-// through them (idx).
-__global__ void k_gather_B(const uint64_t *comp, const uint32_t *idx, const double *vals, const uint64_t *sval,
-                           size_t n, int cbits, uint32_t *Bcol, double *Bval, uint32_t *flag) {
+// Sorted right entries: column, value and fiber-start flag. The value came
+// with the sort, whose last pass wrote it straight into Bval (idx null), or,
+// when the sort carried entry numbers, is gathered through them.
+__global__ void k_gather_B(const uint64_t *comp, const uint32_t *idx, const double *vals, size_t n, int cbits,
+                           uint32_t *Bcol, double *Bval, uint32_t *flag) {
     size_t p = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
     if (p >= n) return;
     uint64_t c = comp[p];
     Bcol[p] = (uint32_t)(c & ((1ull << cbits) - 1));
-    Bval[p] = sval ? __longlong_as_double((long long)sval[p]) : vals[idx[p]];
+    if (idx) Bval[p] = vals[idx[p]];
     flag[p] = (p == 0 || (c >> cbits) != (comp[p - 1] >> cbits)) ? 1u : 0u;
 }
 
@@ -740,19 +740,26 @@ void build_right(DevPool &pool, const Geom &g, const Operand &B, Index &ix, std:
     const bool by_value = !ix.keep_perm && !(self && self->rowA);
     Tmp<uint64_t> k1(pool, n), k2(pool, n);
     Tmp<uint32_t> v1(pool, by_value ? 0 : n), v2(pool, by_value ? 0 : n);
-    Tmp<uint64_t> w1(pool, by_value ? n : 0), w2(pool, by_value ? n : 0);
+    ix.Bcol = (uint32_t *)keep(n * 4);
+    ix.Bval = (double *)keep(n * 8);
+    // by value: the passes alternate between Bval and one spare array, in the
+    // order that makes the last pass write Bval
+    Tmp<uint64_t> w2(pool, by_value ? n : 0);
+    uint64_t *bv = reinterpret_cast<uint64_t *>(ix.Bval);
+    const int nbits = g.kbits + g.cbits;
+    const bool odd = n > 1 && ((nbits + 7) / 8) % 2 == 1;
     k_lin_B<<<grid_for(n), 256, 0, s>>>(B.coords, n, g, k1, by_value ? nullptr : v1.get());
     uint64_t *ks, *ws = nullptr;
     uint32_t *vs = nullptr;
-    if (by_value)
-        radix_sort_keys_values(pool, k1, reinterpret_cast<const uint64_t *>(B.vals), w1, k2, w2, n,
-                               g.kbits + g.cbits, s, &ks, &ws);
-    else
-        radix_sort_pairs(pool, k1, v1, k2, v2, n, g.kbits + g.cbits, s, &ks, &vs);
-    ix.Bcol = (uint32_t *)keep(n * 4);
-    ix.Bval = (double *)keep(n * 8);
+    if (by_value) {
+        radix_sort_keys_values(pool, k1, reinterpret_cast<const uint64_t *>(B.vals), odd ? w2.get() : bv, k2,
+                               odd ? bv : w2.get(), n, nbits, s, &ks, &ws);
+        if (ws != bv) throw LibError{ERR_INVALID, "build_right: sorted values not in place"};
+    } else {
+        radix_sort_pairs(pool, k1, v1, k2, v2, n, nbits, s, &ks, &vs);
+    }
     Tmp<uint32_t> flag(pool, n), fid(pool, n);
-    k_gather_B<<<grid_for(n), 256, 0, s>>>(ks, vs, B.vals, ws, n, g.cbits, ix.Bcol, ix.Bval, flag);
+    k_gather_B<<<grid_for(n), 256, 0, s>>>(ks, vs, B.vals, n, g.cbits, ix.Bcol, ix.Bval, flag);
     if (ix.keep_perm) {
         ix.permB = (uint32_t *)keep(n * 4);
         if (n) FCC_HIP(hipMemcpyAsync(ix.permB, vs, n * 4, hipMemcpyDeviceToDevice, s));
