@@ -724,6 +724,27 @@ __global__ __launch_bounds__(256) void k_self_join(const uint32_t *vs, const uin
     }
 }
 
+// A self-contraction's product count when no per-entry join is wanted: the
+// sum over fibers of (fiber length)^2, one thread per fiber (nfib is read on
+// the device, before the host has it).
+__global__ __launch_bounds__(256) void k_fiber_products(const uint32_t *fiber_ptr, const uint32_t *nfib,
+                                                        unsigned long long *products) {
+    __shared__ unsigned long long s_p[4];
+    const uint32_t nf = *nfib;
+    uint64_t prod = 0;
+    for (uint32_t f = blockIdx.x * blockDim.x + threadIdx.x; f < nf; f += gridDim.x * blockDim.x) {
+        const uint64_t l = fiber_ptr[f + 1] - fiber_ptr[f];
+        prod += l * l;
+    }
+    for (int d = 32; d > 0; d >>= 1) prod += __shfl_xor(prod, d, 64);
+    if ((threadIdx.x & 63) == 0) s_p[threadIdx.x >> 6] = prod;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        prod = s_p[0] + s_p[1] + s_p[2] + s_p[3];
+        if (prod) atomicAdd(products, (unsigned long long)prod);
+    }
+}
+
 void build_right(DevPool &pool, const Geom &g, const Operand &B, Index &ix, std::vector<std::shared_ptr<void>> &hold,
                  hipStream_t s, SelfJoin *self) {
     size_t n = (size_t)B.nnz;
@@ -774,8 +795,11 @@ void build_right(DevPool &pool, const Geom &g, const Operand &B, Index &ix, std:
         Tmp<unsigned long long> prod(pool, 1);
         fill_u64(prod, 1, 0, s);
         self->bfib = ix.Bfib;
-        k_self_join<<<std::min(grid_for(n), 2048u), 256, 0, s>>>(vs, ix.Bfib, ix.Bcol, ix.fiber_ptr, n, self->rowA,
-                                                                  self->fidA, self->vsB, prod);
+        if (vs)
+            k_self_join<<<std::min(grid_for(n), 2048u), 256, 0, s>>>(vs, ix.Bfib, ix.Bcol, ix.fiber_ptr, n,
+                                                                      self->rowA, self->fidA, self->vsB, prod);
+        else
+            k_fiber_products<<<std::min(grid_for(n), 1024u), 256, 0, s>>>(ix.fiber_ptr, total, prod);
         pool.read(&self->products, prod.get(), 8, s);
     }
     pool.read(&ix.nfib, total.get(), 4, s);
