@@ -384,12 +384,25 @@ __global__ __launch_bounds__(LB_THREADS) void k_left_entries(const uint64_t *ks,
 // group; each tile's first light group is where the class turns light (light
 // groups follow the heavy ones inside a tile), or the tile's group end when
 // its last group is heavy.
+// SINGLE (every group holds one entry: one-row tiles over distinct
+// coordinates): the group's entry count (1) goes straight into its
+// descriptor, and the per-tile statistics of k_tile_stats are summed here,
+// runs of equal tiles reduced inside each wavefront, one set of atomics per
+// run, so that pass is not needed.
+struct TileStatsOut {
+    uint8_t *tile_dk;
+    uint32_t *nnz, *keys, *sketch;
+    unsigned long long *sumfib, *products, *maxgp;
+};
+
+template <bool SINGLE>
 __global__ __launch_bounds__(LB_THREADS) void k_left_groups(const uint64_t *ks, const uint2 *Afp, const uint4 *ex,
                                                             const unsigned long long *boff, uint32_t m, int logTL,
                                                             int fbits, bool features, uint32_t *group_ptr,
                                                             uint32_t *group_fiber, uint32_t *group_atile,
                                                             uint4 *gdesc, uint2 *gnf, uint2 *gsk, uint32_t *atile_id,
-                                                            uint32_t *atile_g0, uint32_t *atile_lg) {
+                                                            uint32_t *atile_g0, uint32_t *atile_lg,
+                                                            TileStatsOut so) {
     __shared__ uint32_t s_g[LB_THREADS / 64], s_t[LB_THREADS / 64];
     const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const uint64_t lt = lane ? (~0ull >> (64 - lane)) : 0ull;
@@ -427,21 +440,23 @@ __global__ __launch_bounds__(LB_THREADS) void k_left_groups(const uint64_t *ks, 
             rt += s_t[w];
         }
         __syncthreads();  // s_g / s_t are rewritten by the next round
+        // group and tile of entry q: starts up to and including q, minus one
+        const uint32_t g = gb + wg + (uint32_t)__popcll(bg & lt) + (gf ? 1u : 0u) - 1u;
+        const uint32_t a = tb + wt + (uint32_t)__popcll(bt & lt) + (tf ? 1u : 0u) - 1u;
+        uint4 x = make_uint4(0u, 0u, 0xFFFFFFFFu, 0xFFFFFFFFu);
         if (ok) {
-            // group and tile of entry q: starts up to and including q, minus one
-            const uint32_t g = gb + wg + (uint32_t)__popcll(bg & lt) + (gf ? 1u : 0u) - 1u;
-            const uint32_t a = tb + wt + (uint32_t)__popcll(bt & lt) + (tf ? 1u : 0u) - 1u;
             const bool light = (c >> csh) & 1;
             if (gf) {
                 const uint2 fp = Afp[q];
-                const uint4 x = ex[q];
+                x = ex[q];
                 group_ptr[g] = q;
                 group_fiber[g] = (uint32_t)((c >> logTL) & fmask);
                 group_atile[g] = a;
-                gdesc[g] = make_uint4(fp.x, (x.y & 1u) ? 0u : fp.y - fp.x, q, 0u);
+                gdesc[g] = make_uint4(fp.x, (x.y & 1u) ? 0u : fp.y - fp.x, q, SINGLE ? 1u : 0u);
                 gnf[g] = make_uint2(x.x, x.y);
                 if (features) gsk[g] = make_uint2(x.z, x.w);
                 if (light && (tf || !((prev >> csh) & 1))) atile_lg[a] = g;
+                if (SINGLE && (x.y & 1u) && so.tile_dk) so.tile_dk[a] = 1;
             }
             if (tf) {
                 atile_id[a] = (uint32_t)(c >> tsh);
@@ -451,6 +466,38 @@ __global__ __launch_bounds__(LB_THREADS) void k_left_groups(const uint64_t *ks, 
             if (q + 1 == m) {
                 group_ptr[g + 1] = m;
                 atile_g0[a + 1] = g + 1;
+            }
+        }
+        if (SINGLE) {
+            // the statistics k_tile_stats takes from each group: entries m = 1,
+            // products m * nf, distinct keys 1, fiber entries nf, sketch minima,
+            // the largest group's products; lanes of one tile are contiguous
+            const uint32_t ta = ok ? a : 0xFFFFFFFFu;
+            uint32_t sm = ok ? 1u : 0u, h1 = x.z, h2 = x.w;
+            uint64_t snf = ok ? x.x : 0u, mx = snf;
+            for (int d = 1; d < 64; d <<= 1) {
+                const uint32_t oa = __shfl_down(ta, d, 64), om = __shfl_down(sm, d, 64);
+                const uint64_t onf = __shfl_down(snf, d, 64), omx = __shfl_down(mx, d, 64);
+                const uint32_t o1 = __shfl_down(h1, d, 64), o2 = __shfl_down(h2, d, 64);
+                if (lane + d < 64 && oa == ta) {
+                    sm += om;
+                    snf += onf;
+                    mx = omx > mx ? omx : mx;
+                    h1 = o1 < h1 ? o1 : h1;
+                    h2 = o2 < h2 ? o2 : h2;
+                }
+            }
+            const uint32_t pa = __shfl_up(ta, 1, 64);
+            if (ok && (lane == 0 || pa != ta)) {
+                atomicAdd(&so.nnz[ta], sm);
+                atomicAdd(&so.products[ta], (unsigned long long)snf);
+                atomicMax(&so.maxgp[ta], (unsigned long long)mx);
+                if (features) {
+                    atomicAdd(&so.keys[ta], sm);
+                    atomicAdd(&so.sumfib[ta], (unsigned long long)snf);
+                    atomicMin(&so.sketch[2 * ta], h1);
+                    atomicMin(&so.sketch[2 * ta + 1], h2);
+                }
             }
         }
         gb += rg;
@@ -976,18 +1023,11 @@ static void left_from_sorted(DevPool &pool, const uint4 *rec, const uint64_t *ks
     ix.atile_g0 = (uint32_t *)keep((size_t)(ix.natiles + 1) * 4);
     ix.atile_lg = (uint32_t *)keep((size_t)ix.natiles * 4);
     Tmp<uint2> gnf(pool, ix.ngroups), gsk(pool, ix.features ? ix.ngroups : 0);
-    if (m) {
-        k_left_groups<<<(unsigned)nb, LB_THREADS, 0, s>>>(ks, ix.Afp, ex, boff, (uint32_t)m, logTL, fbits,
-                                                           ix.features, ix.group_ptr, ix.group_fiber, ix.group_atile,
-                                                           ix.gdesc, gnf, gsk, ix.atile_id, ix.atile_g0,
-                                                           ix.atile_lg);
-    } else {
-        // no valid left entry: no groups, no tiles; the one-entry offset
-        // arrays still hold their end (read by the segment table below)
-        fill_u32(ix.atile_g0, 1, 0, s);
-        fill_u32(ix.group_ptr, 1, 0, s);
-    }
-    // entry counts + statistics (one pass over the groups)
+    // entry counts + statistics: one pass over the groups, or, when every
+    // group holds a single entry, summed by the group pass itself
+    // (FCC_SINGLE_GROUPS=0 keeps the separate statistics pass: tests compare the two)
+    const char *sg = getenv("FCC_SINGLE_GROUPS");
+    const bool single = m && ix.ngroups == m && !(sg && atoi(sg) == 0);
     size_t T = ix.natiles;
     if (ix.dk_hkey) {
         ix.tile_dk = (uint8_t *)keep(T + 4);
@@ -1008,15 +1048,38 @@ static void left_from_sorted(DevPool &pool, const uint4 *rec, const uint64_t *ks
         f.add(ix.st_sketch, 2 * T, 0xFFFFFFFFull, 4);
         f.add(ix.st_maxgp, T, 0, 8);
         fill_many(f, s);
-        // the planner features (key counts, fiber sums, column sketches) only
-        // when a learned planner or statistics collection reads them
+    }
+    const TileStatsOut so{ix.tile_dk,
+                          ix.st_nnz,
+                          ix.st_keys,
+                          ix.st_sketch,
+                          (unsigned long long *)ix.st_sumfib,
+                          (unsigned long long *)ix.st_products,
+                          (unsigned long long *)ix.st_maxgp};
+    if (m) {
+        if (single)
+            k_left_groups<true><<<(unsigned)nb, LB_THREADS, 0, s>>>(
+                ks, ix.Afp, ex, boff, (uint32_t)m, logTL, fbits, ix.features, ix.group_ptr, ix.group_fiber,
+                ix.group_atile, ix.gdesc, gnf, gsk, ix.atile_id, ix.atile_g0, ix.atile_lg, so);
+        else
+            k_left_groups<false><<<(unsigned)nb, LB_THREADS, 0, s>>>(
+                ks, ix.Afp, ex, boff, (uint32_t)m, logTL, fbits, ix.features, ix.group_ptr, ix.group_fiber,
+                ix.group_atile, ix.gdesc, gnf, gsk, ix.atile_id, ix.atile_g0, ix.atile_lg, so);
+    } else {
+        // no valid left entry: no groups, no tiles; the one-entry offset
+        // arrays still hold their end (read by the segment table below)
+        fill_u32(ix.atile_g0, 1, 0, s);
+        fill_u32(ix.group_ptr, 1, 0, s);
+    }
+    // the planner features (key counts, fiber sums, column sketches) only
+    // when a learned planner or statistics collection reads them
+    if (T && !single)
         k_tile_stats<<<grid_for(ix.ngroups), 256, 0, s>>>(ix.group_ptr, ix.group_atile, gnf,
                                                           ix.features ? gsk.get() : nullptr, ix.tile_dk, ix.ngroups,
                                                           ix.gdesc, ix.st_nnz, ix.st_keys,
                                                           (unsigned long long *)ix.st_sumfib,
                                                           (unsigned long long *)ix.st_products, ix.st_sketch,
                                                           (unsigned long long *)ix.st_maxgp);
-    }
     build_hseg(pool, ix, hold, s);
     ix.st_direct = (uint8_t *)keep(T + 1);
     if (T)
