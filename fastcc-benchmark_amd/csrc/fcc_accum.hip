@@ -1806,34 +1806,37 @@ __global__ void k_refine_kinds(const unsigned long long *unit_count, const uint3
 }
 
 // fcut[f * n + w] = the first position of fiber f whose column is >= w * W
-// (the fiber's end when none), w < n. One thread per right position p: the
-// windows from just after its predecessor's window (window 0 at the fiber's
-// first entry) up to its own window all start at p, and the fiber's last
-// entry also closes the windows after its own. Each cut is written once, by
-// one sequential pass over the right operand. A position finds the ends of
-// its fiber from its neighbours' fiber numbers (coalesced loads, none
-// waiting on another), not from the fiber pointers.
-__global__ void k_fiber_cuts(const uint32_t *Bfib, const uint32_t *Bcol, uint32_t nB, uint32_t n, uint32_t W,
+// (the fiber's end when none), w < n. One thread per fiber: its n cuts are
+// consecutive words, so a wavefront's stores cover one contiguous run; each
+// cut advances from the previous one over the fiber's sorted columns, by a
+// binary search down to 8 positions, then a short walk.
+__global__ void k_fiber_cuts(const uint32_t *fiber_ptr, const uint32_t *Bcol, uint32_t nfib, uint32_t n, uint32_t W,
                              uint32_t *fcut) {
-    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= nB) return;
-    const uint32_t f = Bfib[p];
-    const bool first = p == 0 || Bfib[p - 1] != f;
-    const bool last = p + 1 == nB || Bfib[p + 1] != f;
-    const uint32_t c = Bcol[p], cp = p ? Bcol[p - 1] : 0u;
-    const uint32_t w = min(c / W, n - 1);
-    const uint32_t wlo = first ? 0u : cp / W + 1;
+    const uint32_t f = blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= nfib) return;
+    uint32_t lo = fiber_ptr[f];
+    const uint32_t hi = fiber_ptr[f + 1];
     uint32_t *fc = fcut + (size_t)f * n;
-    for (uint32_t x = wlo; x <= w; x++) fc[x] = p;
-    if (last)
-        for (uint32_t x = w + 1; x < n; x++) fc[x] = p + 1;
+    fc[0] = lo;
+    for (uint32_t w = 1; w < n; w++) {
+        const uint64_t c = (uint64_t)w * W;
+        uint32_t a = lo, b = hi;
+        while (b - a > 8) {
+            const uint32_t mid = a + (b - a) / 2;
+            if (Bcol[mid] < c) a = mid + 1;
+            else b = mid;
+        }
+        while (a < b && Bcol[a] < c) a++;
+        fc[w] = a;
+        lo = a;
+    }
 }
 
 // ------------------------------------------------------------ launchers
 
 void launch_fiber_cuts(const Index &ix, uint32_t n, uint32_t W, uint32_t *fcut, hipStream_t s) {
     if (!ix.nfib || !n || !ix.nB) return;
-    k_fiber_cuts<<<(unsigned)div_up(ix.nB, 256), 256, 0, s>>>(ix.Bfib, ix.Bcol, ix.nB, n, W, fcut);
+    k_fiber_cuts<<<(unsigned)div_up(ix.nfib, 256), 256, 0, s>>>(ix.fiber_ptr, ix.Bcol, ix.nfib, n, W, fcut);
 }
 
 void launch_plan_tiles(const Index &ix, const float *pred, const PlanParams &P, uint32_t *t_nwin, uint32_t *t_W,

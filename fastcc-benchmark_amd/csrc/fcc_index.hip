@@ -388,7 +388,8 @@ __global__ __launch_bounds__(LB_THREADS) void k_left_entries(const uint64_t *ks,
 // coordinates): the group's entry count (1) goes straight into its
 // descriptor, and the per-tile statistics of k_tile_stats are summed here,
 // runs of equal tiles reduced inside each wavefront, one set of atomics per
-// run, so that pass is not needed.
+// run, so that pass is not needed, nor the per-group arrays only it reads
+// (group_atile, gnf, gsk: left unwritten).
 struct TileStatsOut {
     uint8_t *tile_dk;
     uint32_t *nnz, *keys, *sketch;
@@ -451,10 +452,12 @@ __global__ __launch_bounds__(LB_THREADS) void k_left_groups(const uint64_t *ks, 
                 x = ex[q];
                 group_ptr[g] = q;
                 group_fiber[g] = (uint32_t)((c >> logTL) & fmask);
-                group_atile[g] = a;
                 gdesc[g] = make_uint4(fp.x, (x.y & 1u) ? 0u : fp.y - fp.x, q, SINGLE ? 1u : 0u);
-                gnf[g] = make_uint2(x.x, x.y);
-                if (features) gsk[g] = make_uint2(x.z, x.w);
+                if (!SINGLE) {  // read by the statistics pass only
+                    group_atile[g] = a;
+                    gnf[g] = make_uint2(x.x, x.y);
+                    if (features) gsk[g] = make_uint2(x.z, x.w);
+                }
                 if (light && (tf || !((prev >> csh) & 1))) atile_lg[a] = g;
                 if (SINGLE && (x.y & 1u) && so.tile_dk) so.tile_dk[a] = 1;
             }
@@ -1022,12 +1025,12 @@ static void left_from_sorted(DevPool &pool, const uint4 *rec, const uint64_t *ks
     ix.atile_id = (uint32_t *)keep((size_t)ix.natiles * 4);
     ix.atile_g0 = (uint32_t *)keep((size_t)(ix.natiles + 1) * 4);
     ix.atile_lg = (uint32_t *)keep((size_t)ix.natiles * 4);
-    Tmp<uint2> gnf(pool, ix.ngroups), gsk(pool, ix.features ? ix.ngroups : 0);
     // entry counts + statistics: one pass over the groups, or, when every
     // group holds a single entry, summed by the group pass itself
     // (FCC_SINGLE_GROUPS=0 keeps the separate statistics pass: tests compare the two)
     const char *sg = getenv("FCC_SINGLE_GROUPS");
     const bool single = m && ix.ngroups == m && !(sg && atoi(sg) == 0);
+    Tmp<uint2> gnf(pool, single ? 0 : ix.ngroups), gsk(pool, (ix.features && !single) ? ix.ngroups : 0);
     size_t T = ix.natiles;
     if (ix.dk_hkey) {
         ix.tile_dk = (uint8_t *)keep(T + 4);
