@@ -163,28 +163,40 @@ void launch_extents(const Operand &t, uint32_t *dev_out, hipStream_t s) {
 }
 
 // -------------------------------------------------------------- linearise
+// Both load every mode's coordinate first (unrolled to the largest order,
+// the mode lists are uniform kernel arguments), so a thread waits for one
+// memory round trip, not one per mode.
 __device__ inline uint64_t lin_key(const uint32_t *coords, size_t nnz, size_t n, const int *modes, const uint64_t *R,
                                    int r) {
+    uint32_t cv[kMaxOrder];
+#pragma unroll
+    for (int i = 0; i < kMaxOrder; i++) cv[i] = i < r ? coords[(size_t)modes[i] * nnz + n] : 0u;
     uint64_t k = 0;
-    for (int i = 0; i < r; i++) k = k * R[i] + coords[(size_t)modes[i] * nnz + n];
+#pragma unroll
+    for (int i = 0; i < kMaxOrder; i++)
+        if (i < r) k = k * R[i] + cv[i];
     return k;
 }
 __device__ inline uint32_t lin_ext(const uint32_t *coords, size_t nnz, size_t n, const int *xm, const uint32_t *xE,
                                    int nx) {
+    uint32_t cv[kMaxOrder];
+#pragma unroll
+    for (int i = 0; i < kMaxOrder; i++) cv[i] = i < nx ? coords[(size_t)xm[i] * nnz + n] : 0u;
     uint32_t e = 0;
-    for (int i = 0; i < nx; i++) e = e * xE[i] + coords[(size_t)xm[i] * nnz + n];
+#pragma unroll
+    for (int i = 0; i < kMaxOrder; i++)
+        if (i < nx) e = e * xE[i] + cv[i];
     return e;
 }
 
 // B: comp = key << cbits | col  (coordinate.hpp:723-733 and 767-781)
-// (idx null: the sort carries the values, not the entry numbers)
-__global__ void k_lin_B(const uint32_t *coords, size_t nnz, Geom g, uint64_t *comp, uint32_t *idx) {
+// (the entry numbers, when the sort carries them, are made by its first pass)
+__global__ void k_lin_B(const uint32_t *coords, size_t nnz, Geom g, uint64_t *comp) {
     size_t n = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
     if (n >= nnz) return;
     uint64_t key = lin_key(coords, nnz, n, g.bm, g.R, g.r);
     uint32_t col = lin_ext(coords, nnz, n, g.xb, g.xbE, g.nxb);
     comp[n] = (key << g.cbits) | col;
-    if (idx) idx[n] = (uint32_t)n;
 }
 
 // Sorted right entries: column, value and fiber-start flag. The value came
@@ -207,7 +219,7 @@ __global__ void k_fibers(const uint64_t *comp, const uint32_t *flag, const uint3
     const uint32_t fl = flag[p], f = fid[p];
     if (fl) {
         fiber_ptr[f] = (uint32_t)p;
-        fiber_key[f] = comp[p] >> cbits;
+        if (fiber_key) fiber_key[f] = comp[p] >> cbits;  // (only the left join looks keys up)
     }
     Bfib[p] = f + fl - 1;
     if (p == n - 1) fiber_ptr[f + fl] = (uint32_t)n;
@@ -280,7 +292,7 @@ __device__ __forceinline__ void put_lrec(uint4 *rec, size_t i, double v, uint32_
 // A composite: tile | class | fiber | row-in-tile; unmatched entries sort last.
 __global__ void k_comp_A(const uint32_t *rowA, const uint32_t *fidA, const uint32_t *fiber_ptr, const double *vals,
                          const uint32_t *dk_hkey, const uint2 *fsk, size_t nnz, int logTL, int fbits,
-                         uint32_t heavy_min, uint64_t *comp, uint32_t *idx, uint4 *rec) {
+                         uint32_t heavy_min, uint64_t *comp, uint4 *rec) {
     size_t n = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
     if (n >= nnz) return;
     uint32_t f = fidA[n];
@@ -296,7 +308,6 @@ __global__ void k_comp_A(const uint32_t *rowA, const uint32_t *fidA, const uint3
         put_lrec(rec, n, vals[n], fp0, fp1, dense, fsk ? fsk[f] : make_uint2(0u, 0u));
     }
     comp[n] = c;
-    idx[n] = (uint32_t)n;
 }
 
 // Left entries after the sort (ks: composite keys, vs: source entries), 4096
@@ -819,7 +830,7 @@ void build_right(DevPool &pool, const Geom &g, const Operand &B, Index &ix, std:
     uint64_t *bv = reinterpret_cast<uint64_t *>(ix.Bval);
     const int nbits = g.kbits + g.cbits;
     const bool odd = n > 1 && ((nbits + 7) / 8) % 2 == 1;
-    k_lin_B<<<grid_for(n), 256, 0, s>>>(B.coords, n, g, k1, by_value ? nullptr : v1.get());
+    k_lin_B<<<grid_for(n), 256, 0, s>>>(B.coords, n, g, k1);
     uint64_t *ks, *ws = nullptr;
     uint32_t *vs = nullptr;
     if (by_value) {
@@ -827,7 +838,7 @@ void build_right(DevPool &pool, const Geom &g, const Operand &B, Index &ix, std:
                                odd ? bv : w2.get(), n, nbits, s, &ks, &ws);
         if (ws != bv) throw LibError{ERR_INVALID, "build_right: sorted values not in place"};
     } else {
-        radix_sort_pairs(pool, k1, v1, k2, v2, n, nbits, s, &ks, &vs);
+        radix_sort_pairs(pool, k1, v1, k2, v2, n, nbits, s, &ks, &vs, 0, true);
     }
     Tmp<uint32_t> flag(pool, n), fid(pool, n);
     k_gather_B<<<grid_for(n), 256, 0, s>>>(ks, vs, B.vals, n, g.cbits, ix.Bcol, ix.Bval, flag);
@@ -838,7 +849,8 @@ void build_right(DevPool &pool, const Geom &g, const Operand &B, Index &ix, std:
     Tmp<uint32_t> total(pool, 1);
     scan_u32(pool, flag, fid, n, total, s);
     ix.fiber_ptr = (uint32_t *)keep((n + 1) * 4);
-    ix.fiber_key = (uint64_t *)keep(n * 8);
+    // a self-contraction reads its left side off this sort: no key lookups
+    ix.fiber_key = self ? nullptr : (uint64_t *)keep(n * 8);
     ix.Bfib = (uint32_t *)keep(n * 4);
     k_fibers<<<grid_for(n), 256, 0, s>>>(ks, flag, fid, n, g.cbits, ix.fiber_ptr, ix.fiber_key, ix.Bfib);
     if (self) {
@@ -1129,11 +1141,11 @@ void build_left(DevPool &pool, const Geom &g, const Operand &A, const uint32_t *
         Tmp<uint2> fsk = fiber_sketches(pool, ix, s);
         k_comp_A<<<grid_for(n), 256, 0, s>>>(rowA, fidA, ix.fiber_ptr, A.vals, ix.dk_hkey,
                                              ix.features ? fsk.get() : nullptr, n, logTL, fbits, ix.heavy_min, k1,
-                                             v1, rec);
+                                             rec);
     }
     uint64_t *ks;
     uint32_t *vs;
-    radix_sort_pairs(pool, k1, v1, k2, v2, n, std::min(nbits, 64), s, &ks, &vs);
+    radix_sort_pairs(pool, k1, v1, k2, v2, n, std::min(nbits, 64), s, &ks, &vs, 0, true);
     if (after_sort) (*after_sort)();
     left_from_sorted(pool, rec, ks, vs, nullptr, nvalid, logTL, fbits, ix, hold, s);
 }
@@ -1146,7 +1158,7 @@ void build_left(DevPool &pool, const Geom &g, const Operand &A, const uint32_t *
 // class) bits.
 __global__ void k_self_composite(const uint32_t *Bcol, const uint32_t *Bfib, const uint32_t *fiber_ptr,
                                  const double *Bval, const uint32_t *dk_hkey, const uint2 *fsk, size_t n, int logTL,
-                                 int fbits, uint32_t heavy_min, uint64_t *ks, uint32_t *vs, uint4 *rec) {
+                                 int fbits, uint32_t heavy_min, uint64_t *ks, uint4 *rec) {
     size_t p = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
     if (p >= n) return;
     const uint32_t f = Bfib[p], c = Bcol[p];
@@ -1154,7 +1166,6 @@ __global__ void k_self_composite(const uint32_t *Bcol, const uint32_t *Bfib, con
     const uint64_t cls = (fp1 - fp0) >= heavy_min ? 0 : 1;
     const uint64_t tile = c >> logTL, li = c & ((1u << logTL) - 1);
     ks[p] = (tile << (1 + fbits + logTL)) | (cls << (fbits + logTL)) | ((uint64_t)f << logTL) | li;
-    vs[p] = (uint32_t)p;
     const uint32_t dense = dk_hkey && dk_hkey[f] != 0xFFFFFFFFu ? 1u : 0u;
     put_lrec(rec, p, Bval[p], fp0, fp1, dense, fsk ? fsk[f] : make_uint2(0u, 0u));
 }
@@ -1175,11 +1186,11 @@ void build_left_self(DevPool &pool, const Geom &g, const Operand &A, const SelfJ
         Tmp<uint2> fsk = fiber_sketches(pool, ix, s);
         k_self_composite<<<grid_for(n), 256, 0, s>>>(ix.Bcol, sj.bfib, ix.fiber_ptr, ix.Bval, ix.dk_hkey,
                                                      ix.features ? fsk.get() : nullptr, n, logTL, fbits,
-                                                     ix.heavy_min, k1, v1, rec);
+                                                     ix.heavy_min, k1, rec);
     }
     uint64_t *ks;
     uint32_t *vs;
-    radix_sort_pairs(pool, k1, v1, k2, v2, n, tbits + 1, s, &ks, &vs, fbits + logTL);
+    radix_sort_pairs(pool, k1, v1, k2, v2, n, tbits + 1, s, &ks, &vs, fbits + logTL, true);
     if (after_sort) (*after_sort)();
     left_from_sorted(pool, rec, ks, vs, sj.vsB, (uint32_t)n, logTL, fbits, ix, hold, s);
 }

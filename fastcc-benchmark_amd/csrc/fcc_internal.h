@@ -140,10 +140,11 @@ void scan_u64(DevPool &pool, const uint64_t *in, uint64_t *out, size_t n, uint64
 // Stable LSD radix sort of (key, value) pairs on key bits [lo_bit, lo_bit +
 // nbits) (8-bit digits; key bits above lo_bit + nbits must be zero, bits below
 // lo_bit keep their input order). Returns pointers to the sorted arrays
-// (either the inputs or the alternates).
+// (either the inputs or the alternates). positions: the values are the keys'
+// input positions 0..n-1, generated by the sort (vals is only a buffer).
 void radix_sort_pairs(DevPool &pool, uint64_t *keys, uint32_t *vals, uint64_t *keys_alt,
                       uint32_t *vals_alt, size_t n, int nbits, hipStream_t s,
-                      uint64_t **keys_out, uint32_t **vals_out, int lo_bit = 0);
+                      uint64_t **keys_out, uint32_t **vals_out, int lo_bit = 0, bool positions = false);
 // The same with a 64-bit payload whose first pass reads vals_in (left
 // unchanged) and whose later passes ping-pong between vals and vals_alt.
 void radix_sort_keys_values(DevPool &pool, uint64_t *keys, const uint64_t *vals_in, uint64_t *vals,

@@ -295,7 +295,7 @@ __global__ __launch_bounds__(RS_THREADS) void k_rs_scatter(const uint64_t *keys,
     for (int j = 0; j < RS_ITEMS; j++) {
         const uint32_t li = (uint32_t)(wave * RS_WAVE_ELEMS + j * 64 + lane);
         rk[j] = li < tn ? keys[t0 + li] : 0ull;
-        rv[j] = li < tn ? vals[t0 + li] : VT(0);
+        rv[j] = li < tn ? (vals ? vals[t0 + li] : VT(t0 + li)) : VT(0);  // no vals: the entry numbers
     }
 #pragma unroll
     for (int w = 0; w < RS_WAVES; w++) s_wh[w][tid] = 0;
@@ -371,6 +371,14 @@ __global__ __launch_bounds__(RS_THREADS) void k_rs_scatter(const uint64_t *keys,
 }
 
 template <typename VT>
+__global__ void k_iota(VT *v, size_t n) {
+    const size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+    if (i < n) v[i] = VT(i);
+}
+
+// vals_in null: the payload is each key's input position (0, 1, 2, ...),
+// made by the first pass instead of read
+template <typename VT>
 static void radix_sort_impl(DevPool &pool, uint64_t *keys, const VT *vals_in, VT *vals, uint64_t *keys_alt,
                             VT *vals_alt, size_t n, int nbits, hipStream_t s, uint64_t **keys_out, VT **vals_out,
                             int lo_bit) {
@@ -391,15 +399,18 @@ static void radix_sort_impl(DevPool &pool, uint64_t *keys, const VT *vals_in, VT
             src = va;
         }
     } else if (n && src != va) {
-        FCC_HIP(hipMemcpyAsync(va, src, n * sizeof(VT), hipMemcpyDeviceToDevice, s));
+        if (src) FCC_HIP(hipMemcpyAsync(va, src, n * sizeof(VT), hipMemcpyDeviceToDevice, s));
+        else k_iota<VT><<<(unsigned)div_up(n, 256), 256, 0, s>>>(va, n);
     }
     *keys_out = ka;
     *vals_out = va;
 }
 
 void radix_sort_pairs(DevPool &pool, uint64_t *keys, uint32_t *vals, uint64_t *keys_alt, uint32_t *vals_alt,
-                      size_t n, int nbits, hipStream_t s, uint64_t **keys_out, uint32_t **vals_out, int lo_bit) {
-    radix_sort_impl<uint32_t>(pool, keys, vals, vals, keys_alt, vals_alt, n, nbits, s, keys_out, vals_out, lo_bit);
+                      size_t n, int nbits, hipStream_t s, uint64_t **keys_out, uint32_t **vals_out, int lo_bit,
+                      bool positions) {
+    radix_sort_impl<uint32_t>(pool, keys, positions ? nullptr : vals, vals, keys_alt, vals_alt, n, nbits, s,
+                              keys_out, vals_out, lo_bit);
 }
 
 void radix_sort_keys_values(DevPool &pool, uint64_t *keys, const uint64_t *vals_in, uint64_t *vals,
