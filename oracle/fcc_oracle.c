@@ -1,7 +1,8 @@
 /* CPU oracle for the FaSTCC contraction -- TEST INFRASTRUCTURE ONLY.
  *
  * Loaded only by tests/, the smoke check and bench.py's CPU-baseline leg, as
- * the checker and as the timed host-core baseline; never by the product path.
+ * the reference result the tests compare against and as the timed host-core
+ * baseline; never by the product path.
  *
  * oc_contract_tiled restates the reference's hot path,
  * Tensor<double>::fastcc_multiply<TileAccumulator<double>,double>
@@ -335,8 +336,8 @@ static void tls_free(oc_tls *t) {
 typedef struct {
     double *d;        /* double accumulator, or NULL */
     long double *q;   /* long double accumulator, or NULL */
-    double *ab;       /* sum of |terms| (checker build only) */
-    uint32_t *nt;     /* number of terms (checker build only) */
+    double *ab;       /* sum of |terms| (long-double runs only) */
+    uint32_t *nt;     /* number of terms (long-double runs only) */
     uint8_t *bits;
     uint64_t *active;
     uint64_t nact;

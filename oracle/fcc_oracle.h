@@ -1,7 +1,8 @@
 /* CPU oracle for the FaSTCC contraction -- TEST INFRASTRUCTURE ONLY.
  *
  * Only the tests, the smoke check and the benchmark's CPU-baseline leg load
- * this library, and only as the checker / the timed host-core baseline.
+ * this library, and only as the reference result / the timed host-core
+ * baseline.
  * The product path (the HIP library) never calls it.
  */
 #ifndef FCC_ORACLE_H
@@ -41,7 +42,7 @@ typedef struct {
  *   tile_size      reference tile size; rounded up to a power of two like
  *                  contract.hpp:844-846
  *   nthreads       0 = all hardware threads (contract.hpp:847)
- *   high_precision 1 = accumulate in long double (checker); 0 = double (baseline)
+ *   high_precision 1 = accumulate in long double (test reference); 0 = double (baseline)
  *   sample_stride  1 = every left tile; s > 1 = only left tiles i with i % s == 0
  *   sink           0 = materialise SoA output; 1 = count + checksums only
  * Returns 0, or a negative error code. */

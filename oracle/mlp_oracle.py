@@ -1,5 +1,5 @@
 """CPU (float64, numpy) restatement of the planner MLP -- TEST INFRASTRUCTURE
-ONLY: the checker for the GPU forward/backward kernels in csrc/fcc_mlp.hip.
+ONLY: the reference result for the GPU forward/backward kernels in csrc/fcc_mlp.hip.
 
 Network: x_n = (x - in_mean) / in_std;  h1 = relu(x_n W1 + b1);
 h2 = relu(h1 W2 + b2);  y = h2 W3 + b3  (normalised outputs).

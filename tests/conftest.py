@@ -11,7 +11,7 @@ for p in (os.path.join(ROOT, "fastcc-benchmark_amd"), ROOT):
 
 
 def pytest_configure(config):
-    config.addinivalue_line("markers", "gpu: needs an MI355X (runs on the GPU box)")
+    config.addinivalue_line("markers", "gpu: needs an MI355X GPU")
 
 
 # ------------------------------------------------------------------ helpers
