@@ -153,7 +153,7 @@ def main():
                     help="only run a device-to-device copy of MIB MiB (HBM counter calibration) and exit")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "
+        print(json.dumps({"error": "the library runs on one GPU; --gpus must be 1 "
                           "(sharding by left tiles over several GPUs is not implemented)"}))
         sys.exit(2)
 
