@@ -670,6 +670,14 @@ __device__ __forceinline__ void heavy_ring(const AccArgs &A, const UnitCtx &U, c
     }
 }
 
+// Mirror mode, a row's last column window when the window starts at or left
+// of the tile's first row: every range (an entry's, a group's or a segment's)
+// starts at or right of its row's own column, inside the window, and ends
+// with its fiber at the last cut, so the ranges need no window cut.
+__device__ __forceinline__ bool mirror_whole_fiber(const AccArgs &A, const UnitCtx &U) {
+    return A.mirror && (uint64_t)U.c0 <= U.row0 && U.c0 / A.fcut_W + 2 == A.fcut_n;
+}
+
 // DEEP: dense units keep HRING_DENSE items in flight; without it (kernels
 // whose other paths already need many registers) they keep HRING_HASH, so
 // the kernel stays within 128 VGPRs and two workgroups fit per CU.
@@ -700,7 +708,9 @@ __device__ __forceinline__ void heavy_claimed(const AccArgs &A, const UnitCtx &U
         uint32_t s_l = d.x, e_l = d.x + d.y;
         if (A.unclipped && lane < nb && d.y) s_l = A.fiber_ptr[A.group_fiber[bg + lane]];
         const bool cuts = U.multi && A.fcut && U.kind == KIND_DENSE;
-        if (cuts && lane < nb && d.y) {
+        if (cuts && mirror_whole_fiber(A, U)) {
+            // the descriptor's range as it is
+        } else if (cuts && lane < nb && d.y) {
             // precomputed window cuts of the group's fiber (in mirror mode the
             // descriptor may start later: no cell left of the tile's first
             // row); a heavy segment, a piece of its group's fiber, keeps the
@@ -780,9 +790,14 @@ __device__ __forceinline__ void light_entries(const AccArgs &A, const UnitCtx &U
     const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const uint32_t cend = U.c0 + U.W;
     const bool cuts = U.multi && A.fcut && U.kind == KIND_DENSE;
+    const bool whole = cuts && mirror_whole_fiber(A, U);
     auto span_of = [&](uint32_t e, uint32_t &s0, uint32_t &len) {
         uint32_t a, b;
-        if (cuts) {
+        if (whole) {
+            const uint2 fp = A.Afp[e];
+            a = fp.x;
+            b = fp.y;
+        } else if (cuts) {
             const uint32_t *fc = A.fcut + (size_t)A.Afib[e] * A.fcut_n + U.c0 / A.fcut_W;
             a = fc[0];
             b = fc[1];
