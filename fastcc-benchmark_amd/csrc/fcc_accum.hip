@@ -1822,9 +1822,12 @@ __global__ void k_refine_kinds(const unsigned long long *unit_count, const uint3
 
 // fcut[f * n + w] = the first position of fiber f whose column is >= w * W
 // (the fiber's end when none), w < n. One thread per fiber: its n cuts are
-// consecutive words, so a wavefront's stores cover one contiguous run; each
-// cut advances from the previous one over the fiber's sorted columns, by a
-// binary search down to 8 positions, then a short walk.
+// consecutive words, so a wavefront's stores cover one contiguous run. The
+// fiber's first and last columns settle every cut outside them (the last
+// cut, past every column, among them) without a search; a cut inside advances
+// from the previous one over the fiber's sorted columns, by a binary search
+// down to 8 positions, then counts the columns below the cut in those 8 with
+// independent loads.
 __global__ void k_fiber_cuts(const uint32_t *fiber_ptr, const uint32_t *Bcol, uint32_t nfib, uint32_t n, uint32_t W,
                              uint32_t *fcut) {
     const uint32_t f = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1833,15 +1836,25 @@ __global__ void k_fiber_cuts(const uint32_t *fiber_ptr, const uint32_t *Bcol, ui
     const uint32_t hi = fiber_ptr[f + 1];
     uint32_t *fc = fcut + (size_t)f * n;
     fc[0] = lo;
+    const uint32_t first = lo < hi ? Bcol[lo] : 0u, last = lo < hi ? Bcol[hi - 1] : 0u;
     for (uint32_t w = 1; w < n; w++) {
         const uint64_t c = (uint64_t)w * W;
         uint32_t a = lo, b = hi;
-        while (b - a > 8) {
-            const uint32_t mid = a + (b - a) / 2;
-            if (Bcol[mid] < c) a = mid + 1;
-            else b = mid;
+        if (a == b || (uint64_t)last < c) {
+            a = hi;
+        } else if ((uint64_t)first >= c) {
+            a = lo;
+        } else {
+            while (b - a > 8) {
+                const uint32_t mid = a + (b - a) / 2;
+                if (Bcol[mid] < c) a = mid + 1;
+                else b = mid;
+            }
+            uint32_t k = 0;
+#pragma unroll
+            for (uint32_t j = 0; j < 8; j++) k += (a + j < b && Bcol[a + j] < c) ? 1u : 0u;
+            a += k;
         }
-        while (a < b && Bcol[a] < c) a++;
         fc[w] = a;
         lo = a;
     }
