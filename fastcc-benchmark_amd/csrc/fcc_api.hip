@@ -692,7 +692,11 @@ int Contraction::run(const fcc_tensor *ta, const int32_t *am, const fcc_tensor *
     const bool early_fork = dk.nh && 2.0 * dk_tiles * DK_BM * DK_BN * (double)dk.kp > 2.5e11;
     if (early_fork) fork_dense();
     const std::function<void()> *after_sort = (dk.nh && !early_fork) ? &fork_dense : nullptr;
-    if (self && o.left_tile_stride <= 1) build_left_self(pool, g, A, sj, logTL, ix, hold, s, after_sort);
+    // a long block (the order-5 self-contraction's 1.7e11 flops, not the
+    // order-4 Zipf 1.1 one's 1.5e10) takes the left records beside it rather
+    // than waiting for them
+    const bool records_late = after_sort && 2.0 * dk_tiles * DK_BM * DK_BN * (double)dk.kp >= 5e10;
+    if (self && o.left_tile_stride <= 1) build_left_self(pool, g, A, sj, logTL, ix, hold, s, after_sort, records_late);
     else build_left(pool, g, A, rowA, fidA, nvalid, logTL, ix, hold, s, after_sort);
     if (dk.nh && !side_join.st) throw LibError{FCC_ERR_INVALID, "dense-key block was not started"};
     vsB.reset();

@@ -54,10 +54,12 @@ void join_left(DevPool &pool, const Geom &g, const Operand &A, const Index &ix, 
 // Self-contraction: left tiles from the right operand's order (2 radix passes).
 // after_sort (optional) runs once the left entries are sorted, before the
 // rest of the build is queued (a point to start independent work on another
-// stream that should not compete with the sort).
+// stream that should not compete with the sort). records_late: the entries'
+// source records are written after after_sort instead of before the sort
+// (beside a long block rather than ahead of it).
 void build_left_self(DevPool &pool, const Geom &g, const Operand &A, const SelfJoin &sj, int logTL, Index &ix,
                      std::vector<std::shared_ptr<void>> &hold, hipStream_t s,
-                     const std::function<void()> *after_sort = nullptr);
+                     const std::function<void()> *after_sort = nullptr, bool records_late = false);
 
 // Left operand tiles of 2^logTL rows: sort, groups, active tiles, statistics.
 void build_left(DevPool &pool, const Geom &g, const Operand &A, const uint32_t *rowA, const uint32_t *fidA,
